@@ -1,0 +1,133 @@
+"""The reference's cfg format (src/util/SIMPLE_PARSER.cpp:64-236) and the
+scene cfgs used for the unit-cube ball workloads.
+
+Semantics kept from the reference: lines ``key = value``; a ``#`` in column 0
+comments the line; keys and values are lowercased; floats go through
+``atof``; integers are ``(int)atof`` (truncation); vectors split on ``,``.
+A line without ``=`` that is not blank is a parse error.
+"""
+import math
+import re
+
+_FLOAT_PREFIX = re.compile(r"^\s*[+-]?((\d+\.?\d*)|(\.\d+))([eE][+-]?\d+)?")
+
+
+def atof(s):
+    """C atof: the longest numeric prefix, 0.0 when there is none."""
+    m = _FLOAT_PREFIX.match(s)
+    if not m:
+        low = s.strip().lower()
+        for w, v in (("inf", math.inf), ("+inf", math.inf), ("-inf", -math.inf),
+                     ("nan", math.nan)):
+            if low.startswith(w):
+                return v
+        return 0.0
+    return float(m.group(0))
+
+
+class Cfg:
+    def __init__(self, text=None, path=None):
+        self.vals = {}
+        if path is not None:
+            with open(path, "r") as f:
+                text = f.read()
+        if text is not None:
+            self.parse_text(text)
+
+    def parse_text(self, text):
+        for lineno, line in enumerate(text.split("\n"), 1):
+            line = line[:511]
+            if not line:
+                continue
+            if line[0] == "#":
+                continue
+            pos = line.find("=")
+            if pos >= 0:
+                self.vals[line[:pos].strip().lower()] = line[pos + 1:].strip().lower()
+            elif line.strip():
+                raise ValueError(f"unable to parse line {lineno}: {line!r}")
+
+    def has(self, k):
+        return k.lower() in self.vals
+
+    def get_float(self, k, default):
+        v = self.vals.get(k.lower())
+        return default if v is None else atof(v)
+
+    def get_int(self, k, default):
+        v = self.vals.get(k.lower())
+        return default if v is None else int(atof(v))
+
+    def get_string(self, k, default):
+        v = self.vals.get(k.lower())
+        return default if v is None else v
+
+    def get_vec3(self, k, default):
+        v = self.vals.get(k.lower())
+        if v is None:
+            return list(default)
+        toks = v.split(",")
+        return [atof(toks[i]) if i < len(toks) else 0.0 for i in range(3)]
+
+
+def params_from_cfg(cfg, dh):
+    """cfg keys and defaults of the integrator, solver and solid
+    (COUPLED_INTEGRATOR_3D.cpp:43-74, :445-446, :499-501;
+    CONJUGATE_RESIDUAL.inl:26-27; DEFOR_SOLID_3D.cpp:62, :243-244)."""
+    return {
+        "dt": cfg.get_float("dt", 0.01),
+        "vorticity_eps": cfg.get_float("vorticity eps", 2.0),
+        "gravity": cfg.get_float("gravity", 0.0),
+        "buoyancy": cfg.get_float("buoyancy", 0.1),
+        "rayleigh_alpha": cfg.get_float("rayleigh alpha", 0.0),
+        "rayleigh_beta": cfg.get_float("rayleigh beta", 0.0),
+        "fluid_density": cfg.get_float("fluid density", 1.0),
+        "implicit_eps": cfg.get_float("implicit solver eps", 1e-8),
+        "divfree_eps": cfg.get_float("div free solver eps", 1e-8),
+        "cr_max_iterations": cfg.get_int("cr max iterations", 10),
+        "mass_sample_dh": cfg.get_float("mass sample dh", dh / 10),
+        "tight_border": cfg.get_int("tight border", 3),
+        "extrapolation_cells": cfg.get_int("extrapolation cells", 20),
+    }
+
+
+def lame_from_young(E, nu):
+    """(lambda, mu) of Young's modulus E and Poisson ratio nu."""
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
+                       center=(0.5, 0.65, 0.5), E=1e4, nu=0.3, density=1000.0,
+                       fluid_density=1000.0, carpet=0.3, dt=1.0 / 240.0,
+                       vorticity_eps=1.5, gravity=9.8, frames=250, velocity=(0, 0, 0)):
+    """cfg text of the unit-cube ball scene on an n^3-cell grid: (n+1)^3 nodes,
+    dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool."""
+    lam, mu = lame_from_young(E, nu)
+    dh = 1.0 / n
+    lines = [
+        "# unit-cube rubber ball, %d^3 cells" % n,
+        "xRes = %d" % (n + 1), "yRes = %d" % (n + 1), "zRes = %d" % (n + 1),
+        "dh = %r" % dh, "origin = 0,0,0",
+        "number of solids = 1",
+        "solid path 0 = ./", "solid name 0 = analytic_sphere",
+        "solid sphere radius 0 = %r" % radius,
+        "sphere mass dh 0 = %r" % (dh / 4),
+        "solid mass density 0 = %r" % density,
+        "material lambda 0 = %r" % lam, "material mu 0 = %r" % mu,
+        "solid plastic yield 0 = -1",
+        "solid initial velocity 0 = %r, %r, %r" % tuple(velocity),
+        "solid translation 0 = %r, %r, %r" % tuple(center),
+        "solid scaling 0 = 1,1,1", "solid rotation 0 = 0,0,0",
+        "fluid density = %r" % fluid_density, "fluid viscocity = 0",
+        "vorticity eps = %r" % vorticity_eps, "buoyancy = 0",
+        "smoke carpet thickness = %r" % carpet,
+        "dt = %r" % dt, "gravity = %r" % gravity,
+        "extrapolation cells = 20", "tight border = 5",
+        "rayleigh alpha = 0", "rayleigh beta = 0",
+        "cr max iterations = %d" % max_iterations,
+        "implicit solver eps = %r" % eps, "div free solver eps = %r" % eps,
+        "mass sample dh = %r" % (dh / 8),
+        "start frame = 0", "end frame = %d" % frames,
+        "save data = 0",
+    ]
+    return "\n".join(lines) + "\n"

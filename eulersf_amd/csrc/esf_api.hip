@@ -1,0 +1,709 @@
+// esf_api.hip -- the extern "C" entry points declared in include/esf.h.
+// Host code: allocation, transfers in the reference's layouts, and the
+// stage sequence of COUPLED_INTEGRATOR_3D::stepImplicit on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "esf_internal.h"
+
+namespace esf {
+// esf_pressure.hip
+esf_status pressureRhs(esf_ctx* ctx);
+esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
+template <typename T>
+esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D, T* x, T* r,
+                   T* r2, T* p, T* s, T* Ap, double eps, int maxIt, int* iters, bool* converged,
+                   double* relres);
+esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
+                      double* relres, int* refinements, int* innerIters);
+int spmvBlocks(const Dims& d);
+// esf_solid.hip
+esf_status stressStage(esf_ctx* ctx, double* energy);
+esf_status solidAdiag(esf_ctx* ctx);
+esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y);
+esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps, int maxIt,
+                      int* iters, bool* converged, double* relres);
+__global__ void kCorotBatch(int n, double lambda, double mu, const double* F, const double* dF,
+                            double* out, double* energy, double* dP);
+// esf_grid.hip
+esf_status gatherSolidMasses(esf_ctx* ctx);
+esf_status computeSolidMass(esf_ctx* ctx, int solid);
+esf_status updateBoxes(esf_ctx* ctx, int solid);
+esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
+esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
+esf_status solidRhs(esf_ctx* ctx);
+esf_status scatterSolidV0(esf_ctx* ctx);
+esf_status correctVelocity(esf_ctx* ctx);
+esf_status solidVelocity(esf_ctx* ctx);
+esf_status advectVelocity(esf_ctx* ctx);
+esf_status advectIntensity(esf_ctx* ctx);
+esf_status solidCopyVelocity(esf_ctx* ctx);
+esf_status solidFinalize(esf_ctx* ctx, double dt);
+esf_status recoverAndBC(esf_ctx* ctx, int bc);
+esf_status fillInt(esf_ctx* ctx, long n, int* p, int v);
+esf_status zeroD(esf_ctx* ctx, long n, double* p);
+
+template <typename T>
+static esf_status dalloc(esf_ctx* ctx, T** p, size_t count) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+               " bytes: " + hipGetErrorString(e);
+    *p = nullptr;
+    return ESF_ERR_OOM;
+  }
+  e = hipMemsetAsync(*p, 0, count * sizeof(T), ctx->stream);
+  return e == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status growSolidDofs(esf_ctx* ctx, int need) {
+  if (need <= ctx->capSolid && ctx->sForce) return ESF_OK;
+  const size_t cap = std::max<size_t>(need + need / 4 + 64, 256);
+  double** vecs[] = {&ctx->sForce, &ctx->sMv,  &ctx->sDamped, &ctx->sKdiag, &ctx->sAdiag,
+                     &ctx->sRhs,   &ctx->sV0,  &ctx->sDv,     &ctx->sB,     &ctx->sr,
+                     &ctx->sp,     &ctx->ss,   &ctx->sAp,     &ctx->stmp};
+  for (double** v : vecs) {
+    esf_status st = dalloc(ctx, v, 3 * cap);
+    if (st) return st;
+  }
+  esf_status st = dalloc(ctx, &ctx->solidNodes, cap);
+  if (st) return st;
+  ctx->capSolid = (int)cap;
+  return ESF_OK;
+}
+
+esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need) {
+  if (need <= so.capValid && so.cellList) return ESF_OK;
+  const size_t cap = std::max<size_t>(need + need / 4 + 64, 256);
+  esf_status st;
+  if ((st = dalloc(ctx, &so.cellList, cap))) return st;
+  if ((st = dalloc(ctx, &so.cellMat, 28 * cap))) return st;
+  if ((st = dalloc(ctx, &so.cellOut, 24 * cap))) return st;
+  if ((st = dalloc(ctx, &so.cellDiag, 24 * cap))) return st;
+  so.capValid = (int)cap;
+  return ESF_OK;
+}
+
+static void freeAll(esf_ctx* ctx) {
+  void* ptrs[] = {ctx->vel,    ctx->velws,  ctx->vort,    ctx->intensity, ctx->intws,
+                  ctx->ws,     ctx->solidMass, ctx->nodeFrac, ctx->qfracSum, ctx->sidx,
+                  ctx->solidNodes, ctx->scanBlocks, ctx->v0, ctx->minv, ctx->minv32,
+                  ctx->c,      ctx->b,      ctx->D,       ctx->lambda,    ctx->lambda2,
+                  ctx->r,      ctx->r2,     ctx->pdir,    ctx->s,         ctx->Ap,
+                  ctx->D32,    ctx->e32,    ctx->r32,     ctx->r32b,      ctx->p32,
+                  ctx->s32,    ctx->Ap32,   ctx->rhs32,   ctx->sForce,    ctx->sMv,
+                  ctx->sDamped, ctx->sKdiag, ctx->sAdiag, ctx->sRhs,      ctx->sV0,
+                  ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
+                  ctx->sAp,    ctx->stmp,   ctx->partials, ctx->counter,  ctx->sc,
+                  ctx->dmisc};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (auto& so : ctx->solids) {
+    void* sp[] = {so.rest, so.u,       so.unew,     so.X,       so.vel,     so.velws,
+                  so.mass, so.detFInv, so.qfrac,    so.valid,   so.cellList, so.cellSlot,
+                  so.cellMat, so.cellOut, so.cellDiag};
+    for (void* p : sp)
+      if (p) hipFree(p);
+  }
+  if (ctx->hostFlags) hipHostFree(ctx->hostFlags);
+  if (ctx->hostSc) hipHostFree(ctx->hostSc);
+}
+
+struct StageTimer {
+  esf_ctx* ctx;
+  int stage;
+  // stage timing is opt-in: it synchronises the stream at every stage end
+  StageTimer(esf_ctx* c, int s) : ctx(c), stage(s) {
+    if (ctx->stageTiming) hipEventRecord(ctx->ev[0], ctx->stream);
+  }
+  ~StageTimer() {
+    if (!ctx->stageTiming) return;
+    hipEventRecord(ctx->ev[1], ctx->stream);
+    hipEventSynchronize(ctx->ev[1]);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->stageMs[stage] += ms;
+  }
+};
+
+}  // namespace esf
+
+using namespace esf;
+
+static esf_status checkCtx(esf_ctx* ctx) { return ctx ? ESF_OK : ESF_ERR_INVALID; }
+
+static void applyParams(esf_ctx* ctx, const esf_params* p) {
+  ctx->p = *p;
+  if (ctx->p.inner_eps <= 0) ctx->p.inner_eps = 1e-4;
+  const int maxRes = std::max(std::max(ctx->d.X, ctx->d.Y), ctx->d.Z);
+  // COUPLED_INTEGRATOR_3D.cpp:67-71, float arithmetic as written there
+  double scaling = 64.0f / maxRes;
+  scaling = (scaling < 1.0f) ? 1.0f : scaling;
+  ctx->vortEps = p->vorticity_eps / scaling;
+}
+
+extern "C" {
+
+esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p) {
+  if (!out || !g || !p) return ESF_ERR_INVALID;
+  *out = nullptr;
+  if (g->xres < 4 || g->yres < 4 || g->zres < 4 || !(g->dh > 0)) return ESF_ERR_INVALID;
+  if ((long)g->xres * g->yres * g->zres > (1L << 31) / 8) return ESF_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ESF_ERR_DEVICE;
+  esf_ctx* ctx = new esf_ctx();
+  hipGetDevice(&ctx->device);
+  Dims& d = ctx->d;
+  d.X = g->xres;
+  d.Y = g->yres;
+  d.Z = g->zres;
+  d.CX = d.X - 1;
+  d.CY = d.Y - 1;
+  d.CZ = d.Z - 1;
+  d.slab = (long)d.X * d.Y;
+  d.nodes = d.slab * d.Z;
+  d.cslab = (long)d.CX * d.CY;
+  d.cells = d.cslab * d.CZ;
+  d.dh = g->dh;
+  d.dhInv = 1.0 / g->dh;
+  d.o0 = g->origin[0];
+  d.o1 = g->origin[1];
+  d.o2 = g->origin[2];
+  applyParams(ctx, p);
+  ctx->defaultDt = p->dt;
+  ctx->dt = p->dt;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return ESF_ERR_DEVICE;
+  }
+  for (int i = 0; i < 24; i++) hipEventCreate(&ctx->ev[i]);
+  hipEventCreate(&ctx->spmvEv[0]);
+  hipEventCreate(&ctx->spmvEv[1]);
+  esf_status st = ESF_OK;
+  const long N = d.nodes, C = d.cells;
+  const bool mixed = p->precision == ESF_MIXED;
+  struct {
+    double** p;
+    long n;
+  } dd[] = {{&ctx->vel, 3 * N},  {&ctx->velws, 3 * N}, {&ctx->vort, 3 * N},
+            {&ctx->intensity, N}, {&ctx->intws, N},    {&ctx->ws, N},
+            {&ctx->solidMass, N}, {&ctx->nodeFrac, N}, {&ctx->v0, 3 * N},
+            {&ctx->minv, N},     {&ctx->c, C},         {&ctx->b, C},
+            {&ctx->D, C},        {&ctx->lambda, C},    {&ctx->lambda2, C},
+            {&ctx->r, C},        {&ctx->r2, C},        {&ctx->pdir, C},
+            {&ctx->s, C},        {&ctx->Ap, C}};
+  for (auto& e : dd)
+    if (!st) st = dalloc(ctx, e.p, e.n);
+  if (!st && mixed) {
+    float** fs[] = {&ctx->D32, &ctx->e32, &ctx->r32, &ctx->r32b,
+                    &ctx->p32, &ctx->s32, &ctx->Ap32, &ctx->rhs32};
+    for (float** f : fs)
+      if (!st) st = dalloc(ctx, f, C);
+    if (!st) st = dalloc(ctx, &ctx->minv32, N);
+  }
+  if (!st) st = dalloc(ctx, &ctx->sidx, N);
+  const long scanBlocks = std::max<long>(N, C) / 1024 + 2;
+  if (!st) st = dalloc(ctx, &ctx->scanBlocks, scanBlocks);
+  // partials: enough for the largest reducing grid, 2 values per block
+  ctx->nPartials = std::max<int>(4 * std::max(spmvBlocks(d), cdiv(N, 256)) + 64, 1 << 16);
+  if (!st) st = dalloc(ctx, &ctx->partials, ctx->nPartials);
+  if (!st) st = dalloc(ctx, &ctx->counter, 8);
+  if (!st) st = dalloc(ctx, &ctx->sc, 8);
+  if (!st) st = dalloc(ctx, &ctx->dmisc, 16);
+  if (!st) st = growSolidDofs(ctx, 0);
+  if (!st && hipHostMalloc((void**)&ctx->hostFlags, 64 * sizeof(int)) != hipSuccess)
+    st = ESF_ERR_OOM;
+  if (!st && hipHostMalloc((void**)&ctx->hostSc, sizeof(CrScalars)) != hipSuccess)
+    st = ESF_ERR_OOM;
+  if (!st) fillInt(ctx, N, ctx->sidx, -1);
+  if (!st && hipStreamSynchronize(ctx->stream) != hipSuccess) st = ESF_ERR_DEVICE;
+  if (st) {
+    freeAll(ctx);
+    delete ctx;
+    return st;
+  }
+  *out = ctx;
+  return ESF_OK;
+}
+
+void esf_ctx_destroy(esf_ctx* ctx) {
+  if (!ctx) return;
+  hipStreamSynchronize(ctx->stream);
+  freeAll(ctx);
+  for (int i = 0; i < 24; i++) hipEventDestroy(ctx->ev[i]);
+  hipEventDestroy(ctx->spmvEv[0]);
+  hipEventDestroy(ctx->spmvEv[1]);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* esf_last_error(const esf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+esf_status esf_set_params(esf_ctx* ctx, const esf_params* p) {
+  if (!ctx || !p) return ESF_ERR_INVALID;
+  if (p->precision != ctx->p.precision) {
+    ctx->err = "precision is fixed at context creation";
+    return ESF_ERR_INVALID;
+  }
+  applyParams(ctx, p);
+  return ESF_OK;
+}
+esf_status esf_set_max_dt(esf_ctx* ctx, double v) {
+  if (!ctx) return ESF_ERR_INVALID;
+  ctx->defaultDt = v;
+  return ESF_OK;
+}
+esf_status esf_set_dt(esf_ctx* ctx, double v) {
+  if (!ctx) return ESF_ERR_INVALID;
+  ctx->dt = v;
+  return ESF_OK;
+}
+esf_status esf_get_dt(esf_ctx* ctx, double* v) {
+  if (!ctx || !v) return ESF_ERR_INVALID;
+  *v = ctx->dt;
+  return ESF_OK;
+}
+esf_status esf_num_solid_nodes(esf_ctx* ctx, int* n) {
+  if (!ctx || !n) return ESF_ERR_INVALID;
+  *n = ctx->nSolid;
+  return ESF_OK;
+}
+
+esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu, double density, int rx, int ry,
+                         int rz, double rdh, const double ro[3], const double* rest, int* id) {
+  if (!ctx || !rest || !ro || rx < 2 || ry < 2 || rz < 2 || !(rdh > 0)) return ESF_ERR_INVALID;
+  if (ctx->solids.size() >= 4) {
+    ctx->err = "at most 4 solids per context";
+    return ESF_ERR_INVALID;
+  }
+  ctx->solids.emplace_back();
+  DevSolid& so = ctx->solids.back();
+  const Dims& d = ctx->d;
+  so.lambda = lambda;
+  so.mu = mu;
+  so.qcm = density * d.dh * d.dh * d.dh * 0.125;  // setMassDensity
+  so.rX = rx;
+  so.rY = ry;
+  so.rZ = rz;
+  so.rdh = rdh;
+  for (int k = 0; k < 3; k++) so.ro[k] = ro[k];
+  for (int k = 0; k < 3; k++) {
+    so.box.tMin[k] = so.box.bMin[k] = 0;
+  }
+  so.box.tMax[0] = so.box.bMax[0] = d.X;
+  so.box.tMax[1] = so.box.bMax[1] = d.Y;
+  so.box.tMax[2] = so.box.bMax[2] = d.Z;
+  const long N = d.nodes;
+  esf_status st = ESF_OK;
+  double** v3[] = {&so.u, &so.unew, &so.X, &so.vel, &so.velws};
+  for (double** p : v3)
+    if (!st) st = dalloc(ctx, p, 3 * N);
+  if (!st) st = dalloc(ctx, &so.mass, N);
+  if (!st) st = dalloc(ctx, &so.detFInv, N);
+  if (!st) st = dalloc(ctx, &so.qfrac, 8 * N);
+  if (!st) st = dalloc(ctx, &so.valid, d.cells);
+  if (!st) st = dalloc(ctx, &so.cellSlot, d.cells);
+  if (!st) st = dalloc(ctx, &so.rest, (long)rx * ry * rz);
+  if (!st) st = growSolidCells(ctx, so, 0);
+  if (st) {
+    ctx->solids.pop_back();
+    return st;
+  }
+  ESF_HIP(hipMemcpyAsync(so.rest, rest, sizeof(double) * (long)rx * ry * rz,
+                         hipMemcpyHostToDevice, ctx->stream));
+  fillInt(ctx, d.cells, so.cellSlot, -1);
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  if (id) *id = (int)ctx->solids.size() - 1;
+  return ESF_OK;
+}
+
+esf_status esf_solid_set_boxes(esf_ctx* ctx, int solid, const int b[12]) {
+  if (!ctx || !b || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  Box& bx = ctx->solids[solid].box;
+  const int res[3] = {ctx->d.X, ctx->d.Y, ctx->d.Z};
+  for (int k = 0; k < 3; k++) {
+    if (b[k] < 0 || b[3 + k] > res[k] || b[6 + k] < 0 || b[9 + k] > res[k]) {
+      ctx->err = "box outside the grid";
+      return ESF_ERR_INVALID;
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    bx.tMin[k] = b[k];
+    bx.tMax[k] = b[3 + k];
+    bx.bMin[k] = b[6 + k];
+    bx.bMax[k] = b[9 + k];
+  }
+  ctx->solids[solid].listDirty = true;
+  return ESF_OK;
+}
+
+esf_status esf_solid_get_boxes(esf_ctx* ctx, int solid, int b[12]) {
+  if (!ctx || !b || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  const Box& bx = ctx->solids[solid].box;
+  for (int k = 0; k < 3; k++) {
+    b[k] = bx.tMin[k];
+    b[3 + k] = bx.tMax[k];
+    b[6 + k] = bx.bMin[k];
+    b[9 + k] = bx.bMax[k];
+  }
+  return ESF_OK;
+}
+
+static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_t* bytes,
+                           bool forWrite) {
+  const Dims& d = ctx->d;
+  const size_t N = d.nodes, C = d.cells, NS = 3 * (size_t)ctx->nSolid;
+  *ptr = nullptr;
+  switch (field) {
+    case ESF_F_VELOCITY: *ptr = ctx->vel; *bytes = 24 * N; return ESF_OK;
+    case ESF_F_INTENSITY: *ptr = ctx->intensity; *bytes = 8 * N; return ESF_OK;
+    case ESF_F_SOLID_MASS: *ptr = ctx->solidMass; *bytes = 8 * N; break;
+    case ESF_F_NODE_FRACTION: *ptr = ctx->nodeFrac; *bytes = 8 * N; break;
+    case ESF_F_LAMBDA:
+      *ptr = ctx->lambda;
+      *bytes = 8 * C;
+      if (forWrite) ctx->lambdaInit = true;
+      return ESF_OK;
+    case ESF_F_SOLID_INDEX: *ptr = ctx->sidx; *bytes = 4 * N; break;
+    case ESF_F_MINV: *ptr = ctx->minv; *bytes = 8 * N; break;
+    case ESF_F_PRESSURE_RHS: *ptr = ctx->b; *bytes = 8 * C; break;
+    case ESF_F_DIV_RHS: *ptr = ctx->c; *bytes = 8 * C; break;
+    case ESF_F_V0: *ptr = ctx->v0; *bytes = 24 * N; break;
+    case ESF_F_SOLID_FORCE: *ptr = ctx->sForce; *bytes = 8 * NS; break;
+    case ESF_F_SOLID_MV: *ptr = ctx->sMv; *bytes = 8 * NS; break;
+    case ESF_F_DAMPED_MASS: *ptr = ctx->sDamped; *bytes = 8 * NS; break;
+    case ESF_F_SYS_DIAG: *ptr = ctx->sAdiag; *bytes = 8 * NS; break;
+    case ESF_F_PRESSURE_DIAG: *ptr = ctx->D; *bytes = 8 * C; break;
+    case ESF_F_SOLID_V0: *ptr = ctx->sV0; *bytes = 8 * NS; break;
+    case ESF_F_SOLID_DV: *ptr = ctx->sDv; *bytes = 8 * NS; break;
+    case ESF_F_WORKSPACE: *ptr = ctx->ws; *bytes = 8 * N; break;
+    default: break;
+  }
+  if (*ptr) {
+    if (forWrite) {
+      ctx->err = "field is derived and read-only";
+      return ESF_ERR_INVALID;
+    }
+    return ESF_OK;
+  }
+  if (solid < 0 || solid >= (int)ctx->solids.size()) {
+    ctx->err = "bad field id or solid id";
+    return ESF_ERR_INVALID;
+  }
+  DevSolid& so = ctx->solids[solid];
+  switch (field) {
+    case ESF_S_U: *ptr = so.u; *bytes = 24 * N; break;
+    case ESF_S_X: *ptr = so.X; *bytes = 24 * N; break;
+    case ESF_S_VELOCITY: *ptr = so.vel; *bytes = 24 * N; break;
+    case ESF_S_MASS: *ptr = so.mass; *bytes = 8 * N; if (forWrite) so.listDirty = true; break;
+    case ESF_S_VALID: *ptr = so.valid; *bytes = C; if (forWrite) so.listDirty = true; break;
+    case ESF_S_QFRAC: *ptr = so.qfrac; *bytes = 64 * N; break;
+    case ESF_S_VEL_WORKSPACE: *ptr = so.velws; *bytes = 24 * N; break;
+    default:
+      ctx->err = "bad field id";
+      return ESF_ERR_INVALID;
+  }
+  return ESF_OK;
+}
+
+esf_status esf_upload(esf_ctx* ctx, int field, int solid, const void* src, size_t bytes) {
+  if (!ctx || !src) return ESF_ERR_INVALID;
+  void* p;
+  size_t want;
+  esf_status st = fieldPtr(ctx, field, solid, &p, &want, true);
+  if (st) return st;
+  if (bytes != want) {
+    ctx->err = "upload size " + std::to_string(bytes) + " != " + std::to_string(want);
+    return ESF_ERR_INVALID;
+  }
+  ESF_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_download(esf_ctx* ctx, int field, int solid, void* dst, size_t bytes) {
+  if (!ctx || !dst) return ESF_ERR_INVALID;
+  void* p;
+  size_t want;
+  esf_status st = fieldPtr(ctx, field, solid, &p, &want, false);
+  if (st) return st;
+  if (bytes != want) {
+    ctx->err = "download size " + std::to_string(bytes) + " != " + std::to_string(want);
+    return ESF_ERR_INVALID;
+  }
+  ESF_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  if (!(ctx->p.mass_sample_dh > 0) || (int)(ctx->d.dh / ctx->p.mass_sample_dh) < 1) {
+    ctx->err = "mass sample dh must be positive and at most dh";
+    return ESF_ERR_INVALID;
+  }
+  esf_status st = computeSolidMass(ctx, solid);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_update_boxes(esf_ctx* ctx, int solid) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  return updateBoxes(ctx, solid);
+}
+
+esf_status esf_gather_solid_masses(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 5);
+  return gatherSolidMasses(ctx);
+}
+
+esf_status esf_recover_velocity(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  esf_status st = recoverAndBC(ctx, 0);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_cfl(esf_ctx* ctx, double alpha, double* dt) {
+  if (!ctx) return ESF_ERR_INVALID;
+  double v;
+  StageTimer t(ctx, 0);
+  esf_status st = cfl(ctx, alpha, &v);
+  if (dt) *dt = v;
+  return st;
+}
+
+esf_status esf_forces(esf_ctx* ctx, double* energy) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 1);
+  for (auto& so : ctx->solids)
+    if (so.listDirty) {
+      ctx->err = "solid masses changed: call esf_gather_solid_masses first";
+      return ESF_ERR_STATE;
+    }
+  double e = 0;
+  esf_status st = stressStage(ctx, energy ? &e : nullptr);
+  if (st) return st;
+  st = vorticityAndBodyForces(ctx, energy ? &e : nullptr);
+  if (st) return st;
+  if (energy) *energy = e;
+  return ESF_OK;
+}
+
+esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats) {
+  if (!ctx) return ESF_ERR_INVALID;
+  esf_solve_stats s{};
+  esf_status st;
+  bool conv[3] = {true, true, true};
+  const int maxIt = ctx->p.cr_max_iterations;
+  {
+    StageTimer t(ctx, 2);
+    if ((st = solidAdiag(ctx))) return st;
+    if ((st = solidRhs(ctx))) return st;
+    if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, ctx->p.implicit_eps, maxIt,
+                         &s.iterations[0], &conv[0], &s.rel_residual[0])))
+      return st;
+    if ((st = scatterSolidV0(ctx))) return st;
+  }
+  {
+    StageTimer t(ctx, 3);
+    if ((st = pressureRhs(ctx))) return st;
+    if (!ctx->lambdaInit) {
+      if ((st = zeroD(ctx, ctx->d.cells, ctx->lambda))) return st;
+      ctx->lambdaInit = true;
+    }
+    if (ctx->p.precision == ESF_MIXED) {
+      st = mixedSolve(ctx, ctx->p.divfree_eps, maxIt, &s.iterations[1], &conv[1],
+                      &s.rel_residual[1], &s.refinements, &s.inner_iterations);
+    } else {
+      st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
+                           ctx->pdir, ctx->s, ctx->Ap, ctx->p.divfree_eps, maxIt,
+                           &s.iterations[1], &conv[1], &s.rel_residual[1]);
+    }
+    if (st) return st;
+    if ((st = correctVelocity(ctx))) return st;
+  }
+  {
+    StageTimer t(ctx, 2);
+    if ((st = solidSolve(ctx, 2, ctx->sB, ctx->sDv, ctx->p.implicit_eps, maxIt,
+                         &s.iterations[2], &conv[2], &s.rel_residual[2])))
+      return st;
+    if ((st = solidVelocity(ctx))) return st;
+  }
+  for (int k = 0; k < 3; k++) s.converged[k] = conv[k];
+  s.dt = ctx->dt;
+  s.n_solid_nodes = ctx->nSolid;
+  if (stats) *stats = s;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ctx->err = std::string("coupled solve: ") + hipGetErrorString(e);
+    return ESF_ERR_DEVICE;
+  }
+  for (double r : s.rel_residual)
+    if (std::isnan(r)) {
+      ctx->err = "NaN residual in the coupled solve";
+      return ESF_ERR_NAN;
+    }
+  return (conv[0] && conv[1] && conv[2]) ? ESF_OK : ESF_NOT_CONVERGED;
+}
+
+esf_status esf_solid_copy_velocity(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 4);
+  return solidCopyVelocity(ctx);
+}
+esf_status esf_solid_finalize(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 4);
+  return solidFinalize(ctx, ctx->dt);
+}
+esf_status esf_advect_velocity(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 6);
+  return advectVelocity(ctx);
+}
+esf_status esf_advect_intensity(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 7);
+  return advectIntensity(ctx);
+}
+esf_status esf_recover_and_bc(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  StageTimer t(ctx, 8);
+  return recoverAndBC(ctx, 1);
+}
+
+esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
+  if (!ctx) return ESF_ERR_INVALID;
+  double dt;
+  esf_status st, solveSt;
+  if ((st = esf_cfl(ctx, alpha < 0 ? 1.8 : alpha, &dt))) return st;
+  if ((st = esf_forces(ctx, nullptr))) return st;
+  solveSt = esf_coupled_solve(ctx, stats);
+  if (solveSt < 0) return solveSt;
+  if ((st = esf_solid_copy_velocity(ctx))) return st;
+  if ((st = esf_solid_finalize(ctx))) return st;
+  if ((st = esf_gather_solid_masses(ctx))) return st;
+  if ((st = esf_advect_velocity(ctx))) return st;
+  if ((st = esf_advect_intensity(ctx))) return st;
+  if ((st = esf_recover_and_bc(ctx))) return st;
+  return solveSt;
+}
+
+esf_status esf_pressure_apply(esf_ctx* ctx, const double* x, double* y) {
+  if (!ctx || !x || !y) return ESF_ERR_INVALID;
+  const size_t bytes = 8 * ctx->d.cells;
+  ESF_HIP(hipMemcpyAsync(ctx->r, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+  esf_status st = pressureApplyDevice(ctx, ctx->r, ctx->s);
+  if (st) return st;
+  ESF_HIP(hipMemcpyAsync(y, ctx->s, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_pressure_diag(esf_ctx* ctx, double* dOut) {
+  if (!ctx || !dOut) return ESF_ERR_INVALID;
+  esf_status st = pressureRhs(ctx);
+  if (st) return st;
+  ESF_HIP(hipMemcpyAsync(dOut, ctx->D, 8 * ctx->d.cells, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_solid_apply(esf_ctx* ctx, const double* x, double* y) {
+  if (!ctx || !x || !y) return ESF_ERR_INVALID;
+  const size_t bytes = 24 * (size_t)ctx->nSolid;
+  if (!bytes) return ESF_OK;
+  ESF_HIP(hipMemcpyAsync(ctx->stmp, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+  esf_status st = solidApplyDevice(ctx, ctx->stmp, ctx->sr);
+  if (st) return st;
+  ESF_HIP(hipMemcpyAsync(y, ctx->sr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double eps, int maxIt,
+                              int* iters, double* relres) {
+  if (!ctx || !b || !x) return ESF_ERR_INVALID;
+  const size_t bytes = 8 * ctx->d.cells;
+  ESF_HIP(hipMemcpyAsync(ctx->b, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(ctx->lambda, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+  int it = 0;
+  bool conv = false;
+  double rr = 0;
+  esf_status st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r,
+                                  ctx->r2, ctx->pdir, ctx->s, ctx->Ap, eps, maxIt, &it, &conv, &rr);
+  if (st) return st;
+  ESF_HIP(hipMemcpyAsync(x, ctx->lambda, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return conv ? ESF_OK : ESF_NOT_CONVERGED;
+}
+
+esf_status esf_corotation_batch(int n, double lambda, double mu, const double* F,
+                                const double* dF, double* out, double* energy, double* dP) {
+  if (n <= 0 || !F || !dF || !out || !energy || !dP) return ESF_ERR_INVALID;
+  double *dFm, *ddF, *dout, *den, *ddP;
+  if (hipMalloc(&dFm, 72L * n) != hipSuccess) return ESF_ERR_OOM;
+  hipMalloc(&ddF, 72L * n);
+  hipMalloc(&dout, 288L * n);
+  hipMalloc(&den, 8L * n);
+  hipMalloc(&ddP, 72L * n);
+  hipMemcpy(dFm, F, 72L * n, hipMemcpyHostToDevice);
+  hipMemcpy(ddF, dF, 72L * n, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(kCorotBatch, dim3(cdiv(n, 128)), dim3(128), 0, 0, n, lambda, mu, dFm, ddF,
+                     dout, den, ddP);
+  hipError_t e = hipDeviceSynchronize();
+  hipMemcpy(out, dout, 288L * n, hipMemcpyDeviceToHost);
+  hipMemcpy(energy, den, 8L * n, hipMemcpyDeviceToHost);
+  hipMemcpy(dP, ddP, 72L * n, hipMemcpyDeviceToHost);
+  hipFree(dFm);
+  hipFree(ddF);
+  hipFree(dout);
+  hipFree(den);
+  hipFree(ddP);
+  return e == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status esf_stage_times(esf_ctx* ctx, double* ms, int n, int reset) {
+  if (!ctx || !ms) return ESF_ERR_INVALID;
+  for (int i = 0; i < n && i < 16; i++) ms[i] = ctx->stageMs[i];
+  if (reset)
+    for (double& v : ctx->stageMs) v = 0;
+  return ESF_OK;
+}
+
+esf_status esf_spmv_timing(esf_ctx* ctx, double* avg, long* launches, int reset) {
+  if (!ctx) return ESF_ERR_INVALID;
+  if (avg) *avg = ctx->spmvLaunches ? ctx->spmvMs / ctx->spmvLaunches : 0;
+  if (launches) *launches = ctx->spmvLaunches;
+  if (reset) {
+    ctx->spmvMs = 0;
+    ctx->spmvLaunches = 0;
+  }
+  return ESF_OK;
+}
+
+esf_status esf_set_stage_timing(esf_ctx* ctx, int on) {
+  if (!ctx) return ESF_ERR_INVALID;
+  ctx->stageTiming = on != 0;
+  return ESF_OK;
+}
+
+esf_status esf_set_spmv_timing(esf_ctx* ctx, int on) {
+  if (!ctx) return ESF_ERR_INVALID;
+  ctx->spmvTiming = on != 0;
+  return ESF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1059 @@
+// esf_grid.hip -- node-grid stages of the coupled step: solid mass gathering
+// and solid DOF numbering, the solid mass resampling used at scene setup,
+// CFL, vorticity and body forces, the scatter/correct steps of the Schur
+// solve, semi-Lagrangian advection and the boundary conditions.
+//
+// Every node-parallel kernel is a gather: each thread writes only its own
+// node, and sums that the reference builds by scattering (mass gathering,
+// COUPLED_INTEGRATOR_3D.cpp:346-359; solid mass, DEFOR_SOLID_3D.cpp:655-667)
+// are re-ordered into per-node sums in the reference's addition order.
+#include <algorithm>
+
+#include "esf_internal.h"
+
+namespace esf {
+
+// ---------------------------------------------------------------------------
+// trilinear interpolation with the reference's clamp rule
+// (FIELD_3D.h:198-216, :368-382; MATH_UTILS.h:326-342)
+// ---------------------------------------------------------------------------
+struct FieldGeom {
+  int X, Y, Z;
+  double dh, o0, o1, o2;
+};
+
+__device__ __forceinline__ void bary(double x, double dh, int hi, int& i, double& f) {
+  x /= dh;
+  const double s = floor(x);
+  i = (int)s;
+  if (i < 0) {
+    i = 0;
+    f = 0.5;
+  } else if (i > hi) {
+    i = hi;
+    f = 0.5;
+  } else {
+    f = x - s;
+  }
+}
+__device__ __forceinline__ double lerp(double a, double b, double f) { return (1 - f) * a + f * b; }
+
+__device__ __forceinline__ long locate(const FieldGeom& g, const double* p, double& fx, double& fy,
+                                       double& fz) {
+  int i, j, k;
+  bary(p[0] - g.o0, g.dh, g.X - 2, i, fx);
+  bary(p[1] - g.o1, g.dh, g.Y - 2, j, fy);
+  bary(p[2] - g.o2, g.dh, g.Z - 2, k, fz);
+  return i + (long)j * g.X + (long)k * g.X * g.Y;
+}
+
+__device__ __forceinline__ double interpS(const double* d, const FieldGeom& g, const double* p) {
+  double fx, fy, fz;
+  const long b = locate(g, p, fx, fy, fz);
+  const long X = g.X, S = (long)g.X * g.Y;
+  const double z0 = lerp(lerp(d[b], d[b + 1], fx), lerp(d[b + X], d[b + X + 1], fx), fy);
+  const double z1 =
+      lerp(lerp(d[b + S], d[b + S + 1], fx), lerp(d[b + S + X], d[b + S + X + 1], fx), fy);
+  return lerp(z0, z1, fz);
+}
+
+__device__ __forceinline__ void interpV(const double* d, const FieldGeom& g, const double* p,
+                                        double* out) {
+  double fx, fy, fz;
+  const long b = locate(g, p, fx, fy, fz);
+  const long X = g.X, S = (long)g.X * g.Y;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double* q = d + c;
+    const double z0 = lerp(lerp(q[3 * b], q[3 * (b + 1)], fx),
+                           lerp(q[3 * (b + X)], q[3 * (b + X + 1)], fx), fy);
+    const double z1 = lerp(lerp(q[3 * (b + S)], q[3 * (b + S + 1)], fx),
+                           lerp(q[3 * (b + S + X)], q[3 * (b + S + X + 1)], fx), fy);
+    out[c] = lerp(z0, z1, fz);
+  }
+}
+
+__device__ __forceinline__ FieldGeom gridGeom(const Dims& d) {
+  return FieldGeom{d.X, d.Y, d.Z, d.dh, d.o0, d.o1, d.o2};
+}
+
+__device__ __forceinline__ bool isInterior(const Dims& d, int x, int y, int z) {
+  return x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2;
+}
+
+// ---------------------------------------------------------------------------
+// stream compaction (exclusive scan of flags) in three launches
+// ---------------------------------------------------------------------------
+constexpr int kScanTile = 1024;  // 256 threads x 4
+
+template <class Flag>
+__global__ void kScanCount(long n, Flag flag, int* blockCounts) {
+  __shared__ int red[256];
+  const long base = (long)blockIdx.x * kScanTile;
+  int c = 0;
+  for (int k = 0; k < 4; k++) {
+    const long i = base + threadIdx.x * 4 + k;
+    if (i < n && flag(i)) c++;
+  }
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockCounts[blockIdx.x] = red[0];
+}
+
+// exclusive scan of nb block counts by one block; total into blockCounts[nb]
+__global__ void kScanBlocks(int nb, int* blockCounts) {
+  __shared__ int part[1024];
+  const int per = (nb + 1023) / 1024;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int i = 0; i < per; i++)
+    if (b0 + i < nb) s += blockCounts[b0 + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < 1024; t++) {
+      const int v = part[t];
+      part[t] = acc;
+      acc += v;
+    }
+    blockCounts[nb] = acc;
+  }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int i = 0; i < per; i++)
+    if (b0 + i < nb) {
+      const int v = blockCounts[b0 + i];
+      blockCounts[b0 + i] = acc;
+      acc += v;
+    }
+}
+
+template <class Flag, class Emit>
+__global__ void kScanEmit(long n, Flag flag, const int* blockCounts, Emit emit) {
+  __shared__ int pre[256];
+  const long base = (long)blockIdx.x * kScanTile;
+  int f[4], c = 0;
+  for (int k = 0; k < 4; k++) {
+    const long i = base + threadIdx.x * 4 + k;
+    f[k] = (i < n && flag(i)) ? 1 : 0;
+    c += f[k];
+  }
+  pre[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < 256; t++) {
+      const int v = pre[t];
+      pre[t] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int idx = blockCounts[blockIdx.x] + pre[threadIdx.x];
+  for (int k = 0; k < 4; k++) {
+    const long i = base + threadIdx.x * 4 + k;
+    if (i >= n) break;
+    emit(i, f[k] ? idx : -1);
+    idx += f[k];
+  }
+}
+
+template <class Flag, class Emit>
+esf_status compact(esf_ctx* ctx, long n, Flag flag, Emit emit, int* total) {
+  const int nb = std::max(cdiv(n, kScanTile), 1);
+  hipLaunchKernelGGL((kScanCount<Flag>), dim3(nb), dim3(256), 0, ctx->stream, n, flag,
+                     ctx->scanBlocks);
+  hipLaunchKernelGGL(kScanBlocks, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->scanBlocks);
+  hipLaunchKernelGGL((kScanEmit<Flag, Emit>), dim3(nb), dim3(256), 0, ctx->stream, n, flag,
+                     (const int*)ctx->scanBlocks, emit);
+  ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  *total = ctx->hostFlags[0];
+  return ESF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gatherSolidMasses  COUPLED_INTEGRATOR_3D.cpp:327-375
+// ---------------------------------------------------------------------------
+struct SolidPtrs {
+  int n;
+  const double* mass[4];
+  const double* qfrac[4];
+};
+
+__global__ void kGatherMass(Dims d, SolidPtrs sp, double* solidMass, double* nodeFrac) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  double m = 0;
+  for (int b = 0; b < sp.n; b++) m += sp.mass[b][n];
+  solidMass[n] = m;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  double f = 0;
+  // corner i of the cell whose min corner is n - off[i], for cells with
+  // min corner in [1, X-2]^3 (the reference's loop range, :350-356)
+  for (int i = 0; i < 8; i++) {
+    const int ox = i & 1, oy = (i >> 1) & 1, oz = i >> 2;
+    const int cx = x - ox, cy = y - oy, cz = z - oz;
+    if (!isInterior(d, cx, cy, cz)) continue;
+    const long c = cx + (long)cy * d.X + (long)cz * d.slab;
+    double q = 0;
+    for (int b = 0; b < sp.n; b++) q += sp.qfrac[b][8 * c + i];
+    f += q;
+  }
+  nodeFrac[n] = f * 0.125;
+}
+
+struct SolidFlag {
+  Dims d;
+  const double* solidMass;
+  __device__ bool operator()(long n) const {
+    const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+    return isInterior(d, x, y, z) && solidMass[n] > 0;
+  }
+};
+struct SolidEmit {
+  int* sidx;
+  int* solidNodes;
+  __device__ void operator()(long n, int idx) const {
+    sidx[n] = idx;
+    if (idx >= 0) solidNodes[idx] = (int)n;
+  }
+};
+
+esf_status gatherSolidMasses(esf_ctx* ctx);
+
+// valid-cell list of one solid: cells of the tight box with _valid == 1
+struct ValidFlag {
+  Dims d;
+  Box box;
+  const int8_t* valid;
+  __device__ bool operator()(long c) const {
+    const int x = c % d.CX, y = (c / d.CX) % d.CY, z = c / d.cslab;
+    return x >= box.tMin[0] && x < box.tMax[0] - 1 && y >= box.tMin[1] &&
+           y < box.tMax[1] - 1 && z >= box.tMin[2] && z < box.tMax[2] - 1 && valid[c] == 1;
+  }
+};
+struct ValidEmit {
+  Dims d;
+  int* cellSlot;
+  int* cellList;
+  __device__ void operator()(long c, int idx) const {
+    cellSlot[c] = idx;
+    if (idx >= 0) {
+      const int x = c % d.CX, y = (c / d.CX) % d.CY, z = c / d.cslab;
+      cellList[idx] = x + y * d.X + z * (int)d.slab;
+    }
+  }
+};
+
+esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need);
+esf_status growSolidDofs(esf_ctx* ctx, int need);
+
+esf_status buildValidList(esf_ctx* ctx, DevSolid& so) {
+  int total = 0;
+  // count first, grow, then emit
+  ValidFlag fl{ctx->d, so.box, so.valid};
+  const long n = ctx->d.cells;
+  const int nb = std::max(cdiv(n, kScanTile), 1);
+  hipLaunchKernelGGL((kScanCount<ValidFlag>), dim3(nb), dim3(256), 0, ctx->stream, n, fl,
+                     ctx->scanBlocks);
+  hipLaunchKernelGGL(kScanBlocks, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->scanBlocks);
+  ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  total = ctx->hostFlags[0];
+  esf_status st = growSolidCells(ctx, so, total);
+  if (st) return st;
+  ValidEmit em{ctx->d, so.cellSlot, so.cellList};
+  hipLaunchKernelGGL((kScanEmit<ValidFlag, ValidEmit>), dim3(nb), dim3(256), 0, ctx->stream, n,
+                     fl, (const int*)ctx->scanBlocks, em);
+  so.nValid = total;
+  so.listDirty = false;
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status gatherSolidMasses(esf_ctx* ctx) {
+  const Dims& d = ctx->d;
+  if (ctx->solids.size() > 4) {
+    ctx->err = "at most 4 solids per context";
+    return ESF_ERR_INVALID;
+  }
+  SolidPtrs sp{};
+  sp.n = (int)ctx->solids.size();
+  for (int b = 0; b < sp.n; b++) {
+    sp.mass[b] = ctx->solids[b].mass;
+    sp.qfrac[b] = ctx->solids[b].qfrac;
+  }
+  hipLaunchKernelGGL(kGatherMass, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, sp,
+                     ctx->solidMass, ctx->nodeFrac);
+  // count solid DOF nodes, grow, then number them
+  SolidFlag fl{d, ctx->solidMass};
+  const long n = d.nodes;
+  const int nb = std::max(cdiv(n, kScanTile), 1);
+  hipLaunchKernelGGL((kScanCount<SolidFlag>), dim3(nb), dim3(256), 0, ctx->stream, n, fl,
+                     ctx->scanBlocks);
+  hipLaunchKernelGGL(kScanBlocks, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->scanBlocks);
+  ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  const int total = ctx->hostFlags[0];
+  esf_status st = growSolidDofs(ctx, total);
+  if (st) return st;
+  SolidEmit em{ctx->sidx, ctx->solidNodes};
+  hipLaunchKernelGGL((kScanEmit<SolidFlag, SolidEmit>), dim3(nb), dim3(256), 0, ctx->stream, n,
+                     fl, (const int*)ctx->scanBlocks, em);
+  ctx->nSolid = total;
+  for (auto& so : ctx->solids)
+    if (so.listDirty) {
+      if ((st = buildValidList(ctx, so))) return st;
+    }
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+// ---------------------------------------------------------------------------
+// computeSolidMass(usesdf = false)  DEFOR_SOLID_3D.cpp:576-669
+// ---------------------------------------------------------------------------
+__global__ void kZero(long n, double* a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+__global__ void kZero8(long n, int8_t* a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    a[i] = 0;
+}
+__global__ void kFillInt(long n, int* a, int v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__device__ inline double det3(const double* m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[3] * (m[1] * m[8] - m[2] * m[7]) +
+         m[6] * (m[1] * m[5] - m[2] * m[4]);
+}
+
+// computeF's determinant: det F = 1 / det(F^-1) with the |det| <= 1e-4 rule
+__device__ inline double detF(const Dims& d, const double* u, long index) {
+  const long off[8] = {0, 1, d.X, d.X + 1, d.slab, d.slab + 1, d.slab + d.X, d.slab + d.X + 1};
+  const double g = 0.25 * d.dhInv;
+  double Finv[9];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      double acc = 0;
+      for (int i = 0; i < 8; i++) {
+        const int bit = c == 0 ? (i & 1) : (c == 1 ? ((i >> 1) & 1) : (i >> 2));
+        const double t = u[3 * (index + off[i]) + r] * ((bit ? 1 : -1) * g);
+        acc = (i == 0) ? t : acc + t;
+      }
+      Finv[3 * r + c] = ((r == c) ? 1.0 : 0.0) - acc;
+    }
+  double J = det3(Finv);
+  if (!(fabs(J) > 1e-4)) J = 1;
+  return 1.0 / J;
+}
+
+struct MassArgs {
+  Box box;
+  const double* u;
+  const double* X;
+  const double* rest;
+  FieldGeom rg;
+  double sdh, subV, fraction, qcm;
+  int msr, half;
+  double* detFInv;
+  double* qfrac;
+  int8_t* valid;
+  double* mass;
+};
+
+__device__ inline bool inTight(const Box& b, int x, int y, int z) {
+  return x >= b.tMin[0] && x < b.tMax[0] - 1 && y >= b.tMin[1] && y < b.tMax[1] - 1 &&
+         z >= b.tMin[2] && z < b.tMax[2] - 1;
+}
+
+// one thread per tight-box cell: |det F^-1| and the subsampled quarter
+// volume fractions (512 samples with dh / mass sample dh = 8)
+__global__ void kMassSamples(Dims d, MassArgs a) {
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= d.cells) return;
+  const int x = c % d.CX, y = (c / d.CX) % d.CY, z = c / d.cslab;
+  if (!inTight(a.box, x, y, z)) return;
+  const long index = x + (long)y * d.X + (long)z * d.slab;
+  a.detFInv[index] = fabs(1.0 / detF(d, a.u, index));
+  const FieldGeom g = gridGeom(d);
+  const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
+  double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int p = 0; p < a.msr; p++)
+    for (int qq = 0; qq < a.msr; qq++)
+      for (int r = 0; r < a.msr; r++) {
+        const double pos[3] = {node[0] + (p + 0.5) * a.sdh, node[1] + (qq + 0.5) * a.sdh,
+                               node[2] + (r + 0.5) * a.sdh};
+        double rPos[3];
+        interpV(a.X, g, pos, rPos);
+        const double dd = interpS(a.rest, a.rg, rPos) * a.subV;
+        if (dd < 1e-6) continue;
+        int o = p < a.half ? 0 : 1;
+        o += qq < a.half ? 0 : 2;
+        o += r < a.half ? 0 : 4;
+        q[o] += a.fraction;
+      }
+  double sum = 0;
+  for (int i = 0; i < 8; i++) sum += q[i];
+  int8_t v = 0;
+  double qv = 0;
+  if (sum > 0.5) {
+    v = 1;
+    qv = 1;
+  } else if (sum > 0) {
+    v = 2;
+  }
+  for (int i = 0; i < 8; i++) a.qfrac[8 * index + i] = qv;
+  a.valid[c] = v;
+}
+
+// mass per node: corners i = 0..7 in the reference's outer-loop order
+__global__ void kMassGather(Dims d, MassArgs a) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  double m = 0;
+  for (int i = 0; i < 8; i++) {
+    const int cx = x - (i & 1), cy = y - ((i >> 1) & 1), cz = z - (i >> 2);
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= d.CX || cy >= d.CY || cz >= d.CZ) continue;
+    if (!inTight(a.box, cx, cy, cz)) continue;
+    const long cell = cx + (long)cy * d.CX + (long)cz * d.cslab;
+    if (!a.valid[cell]) continue;
+    const long index = cx + (long)cy * d.X + (long)cz * d.slab;
+    m += a.qfrac[8 * index + i] * a.detFInv[index] * a.qcm;
+  }
+  a.mass[n] = m;
+}
+
+esf_status computeSolidMass(esf_ctx* ctx, int solid) {
+  const Dims& d = ctx->d;
+  DevSolid& so = ctx->solids[solid];
+  hipLaunchKernelGGL(kZero, dim3(1024), dim3(256), 0, ctx->stream, d.nodes, so.detFInv);
+  hipLaunchKernelGGL(kZero, dim3(1024), dim3(256), 0, ctx->stream, 8 * d.nodes, so.qfrac);
+  hipLaunchKernelGGL(kZero8, dim3(1024), dim3(256), 0, ctx->stream, d.cells, so.valid);
+  MassArgs a{};
+  a.box = so.box;
+  a.u = so.u;
+  a.X = so.X;
+  a.rest = so.rest;
+  a.rg = FieldGeom{so.rX, so.rY, so.rZ, so.rdh, so.ro[0], so.ro[1], so.ro[2]};
+  a.sdh = ctx->p.mass_sample_dh;
+  a.subV = a.sdh * a.sdh * a.sdh;
+  a.msr = (int)(d.dh / a.sdh);
+  a.half = a.msr >> 1;
+  a.fraction = 1.0 / (a.msr * a.msr * a.msr);
+  a.qcm = so.qcm;
+  a.detFInv = so.detFInv;
+  a.qfrac = so.qfrac;
+  a.valid = so.valid;
+  a.mass = so.mass;
+  hipLaunchKernelGGL(kMassSamples, dim3(cdiv(d.cells, 128)), dim3(128), 0, ctx->stream, d, a);
+  hipLaunchKernelGGL(kMassGather, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, a);
+  so.listDirty = true;
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+// bounding box of nodes with mass > 0 (computeSDF :227-241): per-block
+// min/max of x, y, z into partials, finished on the host
+__global__ void kMassBox(Dims d, const double* mass, int* part) {
+  __shared__ int s[6][256];
+  int mn[3] = {1 << 30, 1 << 30, 1 << 30}, mx[3] = {-1, -1, -1};
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < d.nodes;
+       n += (long)gridDim.x * blockDim.x) {
+    if (mass[n] > 0) {
+      const int c[3] = {(int)(n % d.X), (int)((n / d.X) % d.Y), (int)(n / d.slab)};
+      for (int k = 0; k < 3; k++) {
+        mn[k] = min(mn[k], c[k]);
+        mx[k] = max(mx[k], c[k]);
+      }
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    s[k][threadIdx.x] = mn[k];
+    s[3 + k][threadIdx.x] = mx[k];
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = min(s[k][threadIdx.x], s[k][threadIdx.x + st]);
+        s[3 + k][threadIdx.x] = max(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = s[k][0];
+}
+
+esf_status updateBoxes(esf_ctx* ctx, int solid) {
+  DevSolid& so = ctx->solids[solid];
+  const int nb = 256;
+  hipLaunchKernelGGL(kMassBox, dim3(nb), dim3(256), 0, ctx->stream, ctx->d, so.mass,
+                     (int*)ctx->partials);
+  std::vector<int> h(6 * nb);
+  ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, h.size() * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  const Dims& d = ctx->d;
+  int mn[3] = {d.X, d.Y, d.Z}, mx[3] = {0, 0, 0};
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 3; k++) {
+      if (h[6 * b + 3 + k] < 0) continue;
+      mn[k] = std::min(mn[k], h[6 * b + k]);
+      mx[k] = std::max(mx[k], h[6 * b + 3 + k]);
+    }
+  const int res[3] = {d.X, d.Y, d.Z};
+  const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
+  for (int k = 0; k < 3; k++) {
+    so.box.tMin[k] = std::max(mn[k] - tb, 0);
+    so.box.tMax[k] = std::min(mx[k] + tb, res[k]);
+    so.box.bMin[k] = std::max(mn[k] - ex, 0);
+    so.box.bMax[k] = std::min(mx[k] + ex, res[k]);
+  }
+  so.listDirty = true;
+  return ESF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CFL  COUPLED_INTEGRATOR_3D.cpp:88-102 (solid DOF nodes are exactly the
+// interior nodes with solid mass)
+// ---------------------------------------------------------------------------
+__global__ void kCfl(int nSolid, const int* solidNodes, const double* vel, double dhInv,
+                     double alpha, double* part) {
+  __shared__ double s[256];
+  double dt = 1e9;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nSolid; i += gridDim.x * blockDim.x) {
+    const long n = solidNodes[i];
+    const double m = fmax(fmax(fabs(vel[3 * n]), fabs(vel[3 * n + 1])), fabs(vel[3 * n + 2]));
+    const double vmax = m * dhInv;
+    dt = fmin(dt, alpha / vmax);
+  }
+  s[threadIdx.x] = dt;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut) {
+  double dt = 1e9;
+  if (ctx->nSolid > 0) {
+    const int nb = std::min(cdiv(ctx->nSolid, 256), 256);
+    hipLaunchKernelGGL(kCfl, dim3(nb), dim3(256), 0, ctx->stream, ctx->nSolid, ctx->solidNodes,
+                       ctx->vel, ctx->d.dhInv, alpha, ctx->partials);
+    std::vector<double> h(nb);
+    ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, nb * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ESF_HIP(hipStreamSynchronize(ctx->stream));
+    for (double v : h) dt = std::min(dt, v);
+  }
+  dt = std::min(dt, ctx->defaultDt);
+  ctx->dt = dt;
+  *dtOut = dt;
+  return ESF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// vorticity  COUPLED_INTEGRATOR_3D.cpp:450-497; _workspace cleared first
+// (:443) and written with |omega| where the node is not fully solid
+// ---------------------------------------------------------------------------
+__global__ void kVorticity(Dims d, const double* __restrict__ v, const double* __restrict__ nf,
+                           double* vort, double* ws, int enabled) {
+  const long index = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (index >= d.nodes) return;
+  const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
+  double out = 0.0;
+  if (enabled && isInterior(d, x, y, z) && !(nf[index] >= 1)) {
+    const long X = d.X, S = d.slab;
+    const double gridSize = 0.5 / d.dh, dhInv = d.dhInv;
+    const long up = nf[index + X] >= 1 || y == d.Y - 2 ? index : index + X;
+    const long down = nf[index - X] >= 1 || y == 1 ? index : index - X;
+    const double dy = (up == index || down == index) ? dhInv : gridSize;
+    const long out_ = nf[index + S] >= 1 || z == d.Z - 2 ? index : index + S;
+    const long in = nf[index - S] >= 1 || z == 1 ? index : index - S;
+    const double dz = (out_ == index || in == index) ? dhInv : gridSize;
+    const long right = nf[index + 1] >= 1 || x == d.X - 2 ? index : index + 1;
+    const long left = nf[index - 1] >= 1 || x == 1 ? index : index - 1;
+    const double dx = (right == index || left == index) ? dhInv : gridSize;
+    const double w0 = ((v[3 * up + 2] - v[3 * down + 2]) + (-v[3 * out_ + 1] + v[3 * in + 1])) * dz;
+    const double w1 = ((v[3 * out_] - v[3 * in]) + (-v[3 * right + 2] + v[3 * left + 2])) * dx;
+    const double w2 = ((v[3 * right + 1] - v[3 * left + 1]) + (-v[3 * up] + v[3 * down])) * dy;
+    vort[3 * index] = w0;
+    vort[3 * index + 1] = w1;
+    vort[3 * index + 2] = w2;
+    out = sqrt(w0 * w0 + (w1 * w1 + w2 * w2));
+  }
+  ws[index] = out;
+}
+
+// body forces and mass vectors  COUPLED_INTEGRATOR_3D.cpp:505-606, with the
+// fluid part of pcrImplicitSolve folded in: v0 = (dt f + m_f v) / m
+// (:669-670, :686-688).  Solid nodes add to the solid force / momentum /
+// damped mass (after the stress gather).
+struct BodyArgs {
+  const double* vel;
+  const double* nf;
+  const double* solidMass;
+  const double* intensity;
+  const double* ws;
+  const double* vort;
+  const int* sidx;
+  double* v0;
+  double* minv;
+  double* sForce;
+  double* sMv;
+  double* sDamped;
+  double fluidDensity, gravity, buoyancy, dampCoeff, vortEps, dt;
+  double* energyPart;
+};
+
+__global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
+  __shared__ double red[8];
+  const long index = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double en = 0;
+  if (index < d.nodes) {
+    const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
+    if (isInterior(d, x, y, z)) {
+      const double* nf = a.nf;
+      const double dh = d.dh;
+      const double fluidMass = fmax(1.0 - nf[index], 0.0) * a.fluidDensity * dh * dh * dh;
+      const double mass = fluidMass + a.solidMass[index];
+      const double h = dh * y + d.o1;
+      en += mass * a.gravity * h;
+      const int solidIdx = a.sidx[index];
+      const double* v = &a.vel[3 * index];
+      if (solidIdx < 0) {
+        const double mg = mass * a.gravity;
+        double f0 = 0.0 + mg * 0, f1 = 0.0 + mg * -1, f2 = 0.0 + mg * 0;
+        f1 += a.buoyancy * fluidMass * a.intensity[index];
+        const double mv0 = fluidMass * v[0], mv1 = fluidMass * v[1], mv2 = fluidMass * v[2];
+        if (a.vortEps > 0) {
+          const long X = d.X, S = d.slab;
+          if (nf[index + X] == 0 && nf[index - X] == 0 && nf[index + 1] == 0 &&
+              nf[index - 1] == 0 && nf[index + S] == 0 && nf[index - S] == 0) {
+            const double gridSize = 0.5 / dh, dhInv = d.dhInv;
+            const long up = y == d.Y - 2 ? index : index + X;
+            const long down = nf[index - X] >= 1 || y == 1 ? index : index - X;
+            const double dy = (up == index || down == index) ? dhInv : gridSize;
+            const long out = nf[index + S] >= 1 || z == d.Z - 2 ? index : index + S;
+            const long in = nf[index - S] >= 1 || z == 1 ? index : index - S;
+            const double dz = (out == index || in == index) ? dhInv : gridSize;
+            const long right = nf[index + 1] >= 1 || x == d.X - 2 ? index : index + 1;
+            const long left = nf[index - 1] >= 1 || x == 1 ? index : index - 1;
+            const double dx = (right == index || left == index) ? dhInv : gridSize;
+            double N0 = (a.ws[right] - a.ws[left]) * dx;
+            double N1 = (a.ws[up] - a.ws[down]) * dy;
+            double N2 = (a.ws[out] - a.ws[in]) * dz;
+            double mag = sqrt(N0 * N0 + N1 * N1 + N2 * N2);
+            if (mag > 0.0) {
+              mag = 1.0 / mag;
+              N0 *= mag;
+              N1 *= mag;
+              N2 *= mag;
+              const double* w = &a.vort[3 * index];
+              f0 += (N1 * w[2] - N2 * w[1]) * dh * a.vortEps * fluidMass;
+              f1 -= (N0 * w[2] - N2 * w[0]) * dh * a.vortEps * fluidMass;
+              f2 += (N0 * w[1] - N1 * w[0]) * dh * a.vortEps * fluidMass;
+            }
+          }
+        }
+        const double mi = 1.0 / mass;
+        a.v0[3 * index] = (f0 * a.dt + mv0) * mi;
+        a.v0[3 * index + 1] = (f1 * a.dt + mv1) * mi;
+        a.v0[3 * index + 2] = (f2 * a.dt + mv2) * mi;
+      } else {
+        const long k = 3L * solidIdx;
+        const double dm = a.solidMass[index] * a.dampCoeff + fluidMass;
+        a.sDamped[k] = dm;
+        a.sDamped[k + 1] = dm;
+        a.sDamped[k + 2] = dm;
+        for (int c = 0; c < 3; c++) a.sMv[k + c] += fluidMass * v[c];
+        const double mg = mass * a.gravity;
+        a.sForce[k] += mg * 0;
+        a.sForce[k + 1] += mg * -1;
+        a.sForce[k + 2] += mg * 0;
+        a.sForce[k + 1] += a.buoyancy * fluidMass * a.intensity[index];
+      }
+      a.minv[index] = 1.0 / mass;
+      en += 0.5 * fluidMass * (v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]));
+    }
+  }
+  const double t = blockSum(en, red);
+  if (threadIdx.x == 0) a.energyPart[blockIdx.x] = t;
+}
+
+// solid right-hand side dt F + M v  (COUPLED_INTEGRATOR_3D.cpp:664-665)
+__global__ void kSolidRhs(long n, const double* f, const double* mv, double dt, double* rhs) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) rhs[i] = f[i] * dt + mv[i];
+}
+
+// v0 at solid nodes from the first solid solve (:690-691)
+__global__ void kScatterSolid(int nSolid, const int* solidNodes, const double* sv, double* v0) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nSolid) return;
+  const long n = solidNodes[s];
+  for (int c = 0; c < 3; c++) v0[3 * n + c] = sv[3 * (long)s + c];
+}
+
+// g = -J^T lambda per interior node (cells in ascending order, the column
+// order of Eigen's transposed product, :719); fluid nodes get
+// v1 = v0 + g / m (:745-747, :753-763); solid nodes hand g to the second
+// solid solve (:724-729)
+__global__ void kCorrect(Dims d, const double* __restrict__ lam, const double* __restrict__ v0,
+                         const double* __restrict__ minv, const int* __restrict__ sidx,
+                         double* vel, double* sB) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int i = n % d.X, j = (n / d.X) % d.Y, k = n / d.slab;
+  if (!isInterior(d, i, j, k)) return;
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int dz = 0; dz < 2; dz++)
+    for (int dy = 0; dy < 2; dy++)
+      for (int dx = 0; dx < 2; dx++) {
+        const double l = lam[(i - 1 + dx) + (long)(j - 1 + dy) * d.CX + (long)(k - 1 + dz) * d.cslab];
+        a0 += dx == 0 ? -l : l;
+        a1 += dy == 0 ? -l : l;
+        a2 += dz == 0 ? -l : l;
+      }
+  const int s = sidx[n];
+  if (s >= 0) {
+    sB[3L * s] = a0;
+    sB[3L * s + 1] = a1;
+    sB[3L * s + 2] = a2;
+  } else {
+    const double mi = minv[n];
+    vel[3 * n] = v0[3 * n] + a0 * mi;
+    vel[3 * n + 1] = v0[3 * n + 1] + a1 * mi;
+    vel[3 * n + 2] = v0[3 * n + 2] + a2 * mi;
+  }
+}
+
+// v1 at solid nodes = v0 + delta_s  (:748-753)
+__global__ void kSolidVelocity(int nSolid, const int* solidNodes, const double* v0,
+                               const double* dv, double* vel) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nSolid) return;
+  const long n = solidNodes[s];
+  for (int c = 0; c < 3; c++) vel[3 * n + c] = v0[3 * n + c] + dv[3 * (long)s + c];
+}
+
+// ---------------------------------------------------------------------------
+// advection
+// ---------------------------------------------------------------------------
+// advectFluidVelocityField  COUPLED_INTEGRATOR_3D.cpp:797-826
+__global__ void __launch_bounds__(256) kAdvectVelocity(Dims d, const double* __restrict__ vel,
+                                                       const double* __restrict__ solidMass,
+                                                       double dt, double* velws) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  if (!isInterior(d, x, y, z) || solidMass[n] > 0) return;
+  const FieldGeom g = gridGeom(d);
+  const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
+  double mp[3], v[3], np[3];
+  for (int c = 0; c < 3; c++) mp[c] = node[c] - 0.5 * vel[3 * n + c] * dt;
+  interpV(vel, g, mp, v);
+  for (int c = 0; c < 3; c++) np[c] = node[c] - v[c] * dt;
+  double mass = interpS(solidMass, g, np);
+  double newdt = dt / 2;
+  while (mass > 0 && newdt > dt / 16) {
+    for (int c = 0; c < 3; c++) np[c] += v[c] * newdt;
+    newdt /= 2;
+    mass = interpS(solidMass, g, np);
+  }
+  interpV(vel, g, np, &velws[3 * n]);
+}
+
+// advectField(_intensity)  COUPLED_INTEGRATOR_3D.cpp:766-796 (workspace
+// cleared first, so skipped nodes receive 0)
+__global__ void __launch_bounds__(256) kAdvectScalar(Dims d, const double* __restrict__ vel,
+                                                     const double* __restrict__ solidMass,
+                                                     const double* __restrict__ field, double dt,
+                                                     double* out) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  if (solidMass[n] > 0) {
+    out[n] = 0.0;
+    return;
+  }
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  const FieldGeom g = gridGeom(d);
+  const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
+  const double v[3] = {vel[3 * n], vel[3 * n + 1], vel[3 * n + 2]};
+  double np[3];
+  for (int c = 0; c < 3; c++) np[c] = node[c] - v[c] * dt;
+  double mass = interpS(solidMass, g, np);
+  double newdt = dt / 2;
+  while (mass > 0 && newdt > dt / 16) {
+    for (int c = 0; c < 3; c++) np[c] += v[c] * newdt;
+    newdt /= 2;
+    mass = interpS(solidMass, g, np);
+  }
+  out[n] = interpS(field, g, np);
+}
+
+// DEFOR_SOLID_3D::finalizeSolution()  DEFOR_SOLID_3D.cpp:1198-1233.
+// u += v dt over all nodes and the advection of u (:1108-1123) fuse: each
+// thread forms u' = u + v dt at the 8 corners it interpolates, rounding as
+// the separate pass would.
+__global__ void __launch_bounds__(256) kSolidAdvectU(Dims d, const double* __restrict__ u,
+                                                     const double* __restrict__ sv, double dt,
+                                                     double* unew) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  const FieldGeom g = gridGeom(d);
+  const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
+  double np[3];
+  for (int c = 0; c < 3; c++) np[c] = node[c] - sv[3 * n + c] * dt;
+  double fx, fy, fz;
+  const long b = locate(g, np, fx, fy, fz);
+  const long X = d.X, S = d.slab;
+  const long cn[8] = {b, b + 1, b + X, b + X + 1, b + S, b + S + 1, b + S + X, b + S + X + 1};
+  for (int c = 0; c < 3; c++) {
+    double q[8];
+    for (int k = 0; k < 8; k++) q[k] = u[3 * cn[k] + c] + sv[3 * cn[k] + c] * dt;
+    const double z0 = lerp(lerp(q[0], q[1], fx), lerp(q[2], q[3], fx), fy);
+    const double z1 = lerp(lerp(q[4], q[5], fx), lerp(q[6], q[7], fx), fy);
+    unew[3 * n + c] = lerp(z0, z1, fz);
+  }
+}
+
+// face extrapolation of u on the extended box (:1205-1220); axis 2 (z
+// faces) first, then y, then x, each face pair per thread in the
+// reference's statement order
+__global__ void kSolidExtrap(Dims d, Box bx, int axis, double* u) {
+  const int a0 = axis == 2 ? 0 : 0;  // first free axis
+  int fa = 0, fb = 1;
+  if (axis == 2) {
+    fa = 0;
+    fb = 1;
+  } else if (axis == 1) {
+    fa = 0;
+    fb = 2;
+  } else {
+    fa = 1;
+    fb = 2;
+  }
+  (void)a0;
+  const int na = bx.bMax[fa] - bx.bMin[fa], nbb = bx.bMax[fb] - bx.bMin[fb];
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)na * nbb) return;
+  int c[3];
+  c[fa] = bx.bMin[fa] + (int)(t % na);
+  c[fb] = bx.bMin[fb] + (int)(t / na);
+  auto at = [&](int v) -> double* {
+    c[axis] = v;
+    return &u[3 * (c[0] + (long)c[1] * d.X + (long)c[2] * d.slab)];
+  };
+  const int lo = bx.bMin[axis], hi = bx.bMax[axis];
+  for (int k = 0; k < 3; k++) {
+    const double a1 = at(lo + 1)[k], a2 = at(lo + 2)[k];
+    at(lo)[k] = 2 * a1 - a2;
+  }
+  for (int k = 0; k < 3; k++) {
+    const double a1 = at(hi - 2)[k], a2 = at(hi - 3)[k];
+    at(hi - 1)[k] = 2 * a1 - a2;
+  }
+}
+
+// X = x - u (:1222-1230) and _vecWorkspace = _velocity (:1232)
+__global__ void kSolidRecoverX(Dims d, const double* u, const double* sv, double* X,
+                               double* svws) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  const double pos[3] = {x * d.dh, y * d.dh, z * d.dh};
+  const double o[3] = {d.o0, d.o1, d.o2};
+  for (int c = 0; c < 3; c++) {
+    X[3 * n + c] = pos[c] + (o[c] - u[3 * n + c]);
+    svws[3 * n + c] = sv[3 * n + c];
+  }
+}
+
+// DEFOR_SOLID_3D::finalizeSolution(velocity, dt)  (:1182-1197): copy the
+// grid velocity into the solid's extended box (nodes outside keep their
+// old value: VEC3F_FIELD_3D::clear() is a no-op, VEC3F_FIELD_3D.h:177)
+__global__ void kSolidCopy(Dims d, Box bx, const double* vel, double* sv) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  if (x < bx.bMin[0] || x >= bx.bMax[0] || y < bx.bMin[1] || y >= bx.bMax[1] ||
+      z < bx.bMin[2] || z >= bx.bMax[2])
+    return;
+  for (int c = 0; c < 3; c++) sv[3 * n + c] = vel[3 * n + c];
+}
+
+// recoverVelocityField for every solid (DEFOR_SOLID_3D.cpp:1235-1241) then
+// the Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:900-916)
+struct RecoverArgs {
+  int n;
+  Box box[4];
+  const double* mass[4];
+  const double* vel[4];
+};
+__global__ void kRecoverBC(Dims d, RecoverArgs a, double* vel, double* intensity, int bc) {
+  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n >= d.nodes) return;
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  for (int b = 0; b < a.n; b++) {
+    const Box& bx = a.box[b];
+    if (x >= bx.bMin[0] && x < bx.bMax[0] - 1 && y >= bx.bMin[1] && y < bx.bMax[1] - 1 &&
+        z >= bx.bMin[2] && z < bx.bMax[2] - 1 && a.mass[b][n] > 0)
+      for (int c = 0; c < 3; c++) vel[3 * n + c] = a.vel[b][3 * n + c];
+  }
+  if (!bc) return;
+  if (x == 0 || x == d.X - 1 || y == 0 || y == d.Y - 1 || z == 0 || z == d.Z - 1)
+    for (int c = 0; c < 3; c++) vel[3 * n + c] = 0;
+  if (y == 0) intensity[n] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host wrappers
+// ---------------------------------------------------------------------------
+#define LAUNCH_NODES(kernel, ...)                                                            \
+  hipLaunchKernelGGL(kernel, dim3(cdiv(ctx->d.nodes, 256)), dim3(256), 0, ctx->stream, \
+                     __VA_ARGS__)
+
+esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy) {
+  const Dims& d = ctx->d;
+  LAUNCH_NODES(kVorticity, d, ctx->vel, ctx->nodeFrac, ctx->vort, ctx->ws,
+               ctx->vortEps > 0 ? 1 : 0);
+  BodyArgs a{};
+  a.vel = ctx->vel;
+  a.nf = ctx->nodeFrac;
+  a.solidMass = ctx->solidMass;
+  a.intensity = ctx->intensity;
+  a.ws = ctx->ws;
+  a.vort = ctx->vort;
+  a.sidx = ctx->sidx;
+  a.v0 = ctx->v0;
+  a.minv = ctx->minv;
+  a.sForce = ctx->sForce;
+  a.sMv = ctx->sMv;
+  a.sDamped = ctx->sDamped;
+  a.fluidDensity = ctx->p.fluid_density;
+  a.gravity = ctx->p.gravity;
+  a.buoyancy = ctx->p.buoyancy;
+  a.dampCoeff = 1 + ctx->dt * ctx->p.rayleigh_alpha;
+  a.vortEps = ctx->vortEps;
+  a.dt = ctx->dt;
+  a.energyPart = ctx->partials;
+  const int nb = cdiv(d.nodes, 256);
+  LAUNCH_NODES(kBodyForces, d, a);
+  if (energy) {
+    std::vector<double> h(nb);
+    ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, nb * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ESF_HIP(hipStreamSynchronize(ctx->stream));
+    double e = 0;
+    for (double v : h) e += v;
+    *energy += e;
+  }
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status solidRhs(esf_ctx* ctx) {
+  const long n = 3L * ctx->nSolid;
+  if (n)
+    hipLaunchKernelGGL(kSolidRhs, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, n, ctx->sForce,
+                       ctx->sMv, ctx->dt, ctx->sRhs);
+  return ESF_OK;
+}
+
+esf_status scatterSolidV0(esf_ctx* ctx) {
+  if (ctx->nSolid)
+    hipLaunchKernelGGL(kScatterSolid, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
+                       ctx->nSolid, ctx->solidNodes, ctx->sV0, ctx->v0);
+  return ESF_OK;
+}
+
+esf_status correctVelocity(esf_ctx* ctx) {
+  LAUNCH_NODES(kCorrect, ctx->d, ctx->lambda, ctx->v0, ctx->minv, ctx->sidx, ctx->vel, ctx->sB);
+  return ESF_OK;
+}
+
+esf_status solidVelocity(esf_ctx* ctx) {
+  if (ctx->nSolid)
+    hipLaunchKernelGGL(kSolidVelocity, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
+                       ctx->nSolid, ctx->solidNodes, ctx->v0, ctx->sDv, ctx->vel);
+  return ESF_OK;
+}
+
+esf_status advectVelocity(esf_ctx* ctx) {
+  LAUNCH_NODES(kAdvectVelocity, ctx->d, ctx->vel, ctx->solidMass, ctx->dt, ctx->velws);
+  std::swap(ctx->vel, ctx->velws);
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status advectIntensity(esf_ctx* ctx) {
+  LAUNCH_NODES(kAdvectScalar, ctx->d, ctx->vel, ctx->solidMass, ctx->intensity, ctx->dt,
+               ctx->intws);
+  std::swap(ctx->intensity, ctx->intws);
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status solidCopyVelocity(esf_ctx* ctx) {
+  for (auto& so : ctx->solids) LAUNCH_NODES(kSolidCopy, ctx->d, so.box, ctx->vel, so.vel);
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status solidFinalize(esf_ctx* ctx, double dt) {
+  for (auto& so : ctx->solids) {
+    LAUNCH_NODES(kSolidAdvectU, ctx->d, so.u, so.vel, dt, so.unew);
+    std::swap(so.u, so.unew);
+    const Box& b = so.box;
+    const int axes[3] = {2, 1, 0};
+    for (int a : axes) {
+      int fa = a == 0 ? 1 : 0, fb = a == 2 ? 1 : 2;
+      const long cnt = (long)(b.bMax[fa] - b.bMin[fa]) * (b.bMax[fb] - b.bMin[fb]);
+      if (cnt > 0 && b.bMax[a] - b.bMin[a] >= 3)
+        hipLaunchKernelGGL(kSolidExtrap, dim3(cdiv(cnt, 256)), dim3(256), 0, ctx->stream, ctx->d,
+                           b, a, so.u);
+    }
+    LAUNCH_NODES(kSolidRecoverX, ctx->d, so.u, so.vel, so.X, so.velws);
+  }
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status recoverAndBC(esf_ctx* ctx, int bc) {
+  if (ctx->solids.size() > 4) {
+    ctx->err = "at most 4 solids per context";
+    return ESF_ERR_INVALID;
+  }
+  RecoverArgs a{};
+  a.n = (int)ctx->solids.size();
+  for (int b = 0; b < a.n; b++) {
+    a.box[b] = ctx->solids[b].box;
+    a.mass[b] = ctx->solids[b].mass;
+    a.vel[b] = ctx->solids[b].vel;
+  }
+  LAUNCH_NODES(kRecoverBC, ctx->d, a, ctx->vel, ctx->intensity, bc);
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status fillInt(esf_ctx* ctx, long n, int* p, int v) {
+  hipLaunchKernelGGL(kFillInt, dim3(1024), dim3(256), 0, ctx->stream, n, p, v);
+  return ESF_OK;
+}
+esf_status zeroD(esf_ctx* ctx, long n, double* p) {
+  hipLaunchKernelGGL(kZero, dim3(1024), dim3(256), 0, ctx->stream, n, p);
+  return ESF_OK;
+}
+
+}  // namespace esf

@@ -1,0 +1,198 @@
+// esf_internal.h -- device context and helpers shared by the libesf.so
+// translation units.  gfx950 only: wave64, FP64 arithmetic compiled with
+// -ffp-contract=off so every stage that mirrors a reference expression
+// rounds exactly as the reference's left-to-right evaluation does.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/esf.h"
+
+namespace esf {
+
+// ---------------------------------------------------------------------------
+// Grid dimensions passed by value to every kernel
+// ---------------------------------------------------------------------------
+struct Dims {
+  int X, Y, Z;       // nodes
+  int CX, CY, CZ;    // cells = nodes - 1
+  long slab, nodes;  // X*Y, X*Y*Z
+  long cslab, cells; // CX*CY, CX*CY*CZ
+  double dh, dhInv;
+  double o0, o1, o2;
+};
+
+// Per-solid box, in node coordinates (DEFOR_SOLID_3D.h:138-140)
+struct Box {
+  int tMin[3], tMax[3];  // tight box  (_tightMin/_tightMax)
+  int bMin[3], bMax[3];  // extended box (_xmin.._xmax)
+};
+
+// Scalars of one Krylov solve, resident in device memory.  The last block
+// of each reducing kernel writes them; the next kernel reads them.
+struct CrScalars {
+  double delta;    // r . s
+  double beta;     // delta_new / delta_old
+  double alpha;    // delta / (Ap . D^-1 Ap)
+  double bnorm;    // |b|
+  double resnorm;  // last true residual |Ax - b|
+  double eps;      // relative tolerance
+  double aux;      // mixed: |r| of the outer residual
+  int done;        // 1 = converged
+  int iter;        // iteration count at convergence
+  int pad[2];
+};
+
+// per-solid device state: DEFOR_SOLID_3D members kept on the GPU
+struct DevSolid {
+  double lambda = 0, mu = 0, qcm = 0;  // qcm = _quarterCellSolidMass
+  int rX = 0, rY = 0, rZ = 0;
+  double rdh = 0, ro[3] = {0, 0, 0};
+  double* rest = nullptr;              // rest mass FIELD_3Df
+  double *u = nullptr, *unew = nullptr, *X = nullptr;
+  double *vel = nullptr, *velws = nullptr;  // 3*nodes
+  double *mass = nullptr, *detFInv = nullptr;
+  double* qfrac = nullptr;                  // 8*nodes
+  int8_t* valid = nullptr;                  // cells
+  Box box;
+  // compact list of cells with _valid == 1 inside the tight box
+  int nValid = 0, capValid = 0;
+  int* cellList = nullptr;   // node index of each valid cell's min corner
+  int* cellSlot = nullptr;   // cells -> slot or -1
+  double* cellMat = nullptr; // per slot: R(9) L(9) F(9) Jinv(1)
+  double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
+  double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
+  bool listDirty = true;
+};
+
+}  // namespace esf
+
+struct esf_ctx {
+  esf::Dims d;
+  esf_params p;
+  double vortEps = 0, defaultDt = 0, dt = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  // grid fields
+  double *vel = nullptr, *velws = nullptr, *vort = nullptr;  // 3*nodes
+  double *intensity = nullptr, *intws = nullptr, *ws = nullptr;
+  double *solidMass = nullptr, *nodeFrac = nullptr, *qfracSum = nullptr;
+  int* sidx = nullptr;        // nodes -> solid DOF node index or -1
+  int* solidNodes = nullptr;  // solid DOF node -> node index
+  int nSolid = 0, capSolid = 0;
+  int* scanBlocks = nullptr;  // scan scratch
+  // solve: node-based
+  double *v0 = nullptr, *minv = nullptr;  // 3*nodes, nodes
+  float* minv32 = nullptr;
+  // solve: cell-based (pressure)
+  double *c = nullptr, *b = nullptr, *D = nullptr, *lambda = nullptr,
+         *lambda2 = nullptr;
+  double *r = nullptr, *r2 = nullptr, *pdir = nullptr, *s = nullptr, *Ap = nullptr;
+  float *D32 = nullptr, *e32 = nullptr, *r32 = nullptr, *r32b = nullptr,
+        *p32 = nullptr, *s32 = nullptr, *Ap32 = nullptr, *rhs32 = nullptr;
+  bool lambdaInit = false;
+  // solid DOF vectors (3*capSolid)
+  double *sForce = nullptr, *sMv = nullptr, *sDamped = nullptr,
+         *sKdiag = nullptr, *sAdiag = nullptr, *sRhs = nullptr,
+         *sV0 = nullptr, *sDv = nullptr, *sB = nullptr;
+  double *sr = nullptr, *sp = nullptr, *ss = nullptr, *sAp = nullptr,
+         *stmp = nullptr;
+  // reductions
+  double* partials = nullptr;  // per-block partial sums
+  unsigned* counter = nullptr; // last-block counters
+  esf::CrScalars* sc = nullptr;   // device scalars (several solves)
+  double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
+  int* hostFlags = nullptr;       // pinned ring of done flags
+  esf::CrScalars* hostSc = nullptr;  // pinned copy
+  int nPartials = 0;
+  std::vector<esf::DevSolid> solids;
+  // timing
+  hipEvent_t ev[24];
+  double stageMs[16] = {0};
+  bool spmvTiming = false;
+  bool stageTiming = false;
+  hipEvent_t spmvEv[2];
+  double spmvMs = 0;
+  long spmvLaunches = 0;
+};
+
+namespace esf {
+
+// error helpers
+#define ESF_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) {                                             \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);      \
+      return ESF_ERR_DEVICE;                                            \
+    }                                                                   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+
+// wave64 butterfly then fixed-order sum across the block's waves; the
+// result is valid in thread 0.  Deterministic for a fixed launch shape.
+__device__ inline double blockSum(double v, double* lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += lds[w];
+  return r;
+}
+
+__device__ inline double loadAcq(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Each block writes K partials; the block that finishes last sums them in
+// block order and returns true in every thread of that block with the K
+// totals in `tot` (valid in all threads).  The counter is reset for reuse.
+template <int K>
+__device__ inline bool lastBlockReduce(const double (&mine)[K], double* partials,
+                                       unsigned* counter, double* lds,
+                                       double (&tot)[K]) {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  for (int k = 0; k < K; k++) {
+    double s = blockSum(mine[k], lds);
+    if (threadIdx.x == 0) partials[(size_t)k * nb + bid] = s;
+  }
+  __shared__ unsigned isLast;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    isLast = (prev == nb - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!isLast) return false;
+  __threadfence();
+  for (int k = 0; k < K; k++) {
+    double acc = 0;
+    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
+      acc += loadAcq(&partials[(size_t)k * nb + i]);
+    double t = blockSum(acc, lds);
+    __shared__ double bc;
+    if (threadIdx.x == 0) bc = t;
+    __syncthreads();
+    tot[k] = bc;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace esf

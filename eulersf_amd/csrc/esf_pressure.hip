@@ -1,0 +1,475 @@
+// esf_pressure.hip -- the Schur pressure system S = J M^-1 J^T on the
+// collocated Q1-P0 grid of the reference, and its Krylov solvers.
+//
+// S x (COUPLED_INTEGRATOR_3D.cpp:638-660) is never assembled.  One kernel
+// streams x through LDS in 64x4-cell tiles marching along z: for each node
+// layer it forms w = minv * (J^T x) (8 cells, signs from the corner bits,
+// COUPLED_INTEGRATOR_3D.cpp:239-249), for each cell layer it forms
+// y = J w (8 corners x 3 components).  Sums run in the reference's sparse
+// row/column order, so the FP64 result is bitwise the reference's CSR
+// product.  Bytes per apply: x once, y once, minv once (8 B each in FP64).
+//
+// The Krylov loop is CONJUGATE_RESIDUAL::solvePCR (CONJUGATE_RESIDUAL.inl:
+// 109-192) split into two launches per iteration:
+//   update+apply : x += a p, r -= a D^-1 Ap (recomputed on the halo),
+//                  s = S r, partial r.s          -> beta
+//   direction    : p = beta p + r, Ap = beta Ap + s, partial Ap.D^-1 Ap -> alpha
+// plus a true-residual check every 5 iterations, as the reference does.
+// Reductions finish in the last block (fixed order, deterministic).
+#include "esf_internal.h"
+#include "esf_krylov.h"
+
+namespace esf {
+
+enum SpmvMode {
+  M_APPLY = 0,
+  M_SETUP_R = 1,
+  M_SETUP_S = 2,
+  M_UPDATE = 3,
+  M_CHECK = 4,
+  M_IR_RESID = 5,
+};
+
+template <typename T>
+struct SpmvArgs {
+  const T* in;       // input vector (x, r, or r_old / x_old)
+  const T* minv;     // per node
+  T* out;            // M_APPLY output
+  const T* b;        // right-hand side
+  const T* D;        // Jacobi diagonal
+  T* x;              // solution (M_UPDATE, M_IR_RESID)
+  T* r;              // residual
+  T* p;              // direction
+  T* s;              // S r
+  T* Ap;             // A p
+  const float* e32;  // M_IR_RESID: correction added to x (may be null)
+  float* r32;        // M_IR_RESID: residual handed to the FP32 solve
+  CrScalars* sc;
+  double* partials;
+  unsigned* counter;
+  int iter;
+  int first;         // M_IR_RESID: also compute |b|
+};
+
+constexpr int TX = 64, TY = 4, TZ = 8;
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
+                                             int tilesY, int tilesZ) {
+  if (MODE != M_APPLY && a.sc && a.sc->done) return;
+  __shared__ T xs[2][(TY + 2) * (TX + 2)];
+  __shared__ T ws[2][(TY + 1) * (TX + 1) * 3];
+  __shared__ double red[8];
+
+  // XCD-aware tile order: blocks are dealt round-robin to the 8 XCDs, so
+  // give each XCD a contiguous run of tiles (shared halos stay in one L2).
+  const int nb = tilesX * tilesY * tilesZ;
+  int bid = blockIdx.x;
+  int tile = (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+  const int tx = tile % tilesX, ty = (tile / tilesX) % tilesY,
+            tz = tile / (tilesX * tilesY);
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  const int zEnd = min(z0 + TZ, d.CZ);
+
+  T alpha = 0;
+  if (MODE == M_UPDATE) alpha = (T)a.sc->alpha;
+
+  double part0 = 0, part1 = 0;
+
+  auto loadLayer = [&](int cz) {
+    T* L = xs[cz & 1];
+    for (int t = threadIdx.x; t < (TY + 2) * (TX + 2); t += 256) {
+      const int lx = t % (TX + 2), ly = t / (TX + 2);
+      const int cx = x0 - 1 + lx, cy = y0 - 1 + ly;
+      T v = 0;
+      if (cx >= 0 && cx < d.CX && cy >= 0 && cy < d.CY && cz >= 0 && cz < d.CZ) {
+        const long c = cx + (long)cy * d.CX + (long)cz * d.cslab;
+        const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY &&
+                         cz >= z0 && cz < zEnd;
+        if (MODE == M_UPDATE) {
+          // r_old is read on the halo, r_new written to the other buffer
+          const T q = a.Ap[c] / a.D[c];
+          v = a.in[c] - alpha * q;
+          if (own) {
+            a.r[c] = v;
+            a.x[c] = a.x[c] + alpha * a.p[c];
+          }
+        } else if (MODE == M_IR_RESID) {
+          // x_old (+ e) read on the halo, x_new written to the other buffer
+          v = a.in[c];
+          if (a.e32) {
+            v = v + (T)a.e32[c];
+            if (own) a.x[c] = v;
+          }
+        } else {
+          v = a.in[c];
+        }
+      }
+      L[t] = v;
+    }
+  };
+
+  auto nodeLayer = [&](int k) {
+    T* W = ws[k & 1];
+    const T* L0 = xs[(k - 1) & 1];
+    const T* L1 = xs[k & 1];
+    for (int t = threadIdx.x; t < (TY + 1) * (TX + 1); t += 256) {
+      const int lx = t % (TX + 1), ly = t / (TX + 1);
+      const int i = x0 + lx, j = y0 + ly;
+      T w0 = 0, w1 = 0, w2 = 0;
+      if (i >= 1 && i <= d.X - 2 && j >= 1 && j <= d.Y - 2 && k >= 1 && k <= d.Z - 2) {
+        T m0 = 0, m1 = 0, m2 = 0;
+        // adjacent cells in ascending cell index: dz, dy, dx
+#pragma unroll
+        for (int dz = 0; dz < 2; dz++) {
+          const T* L = dz ? L1 : L0;
+#pragma unroll
+          for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++) {
+              const T v = L[(ly + dy) * (TX + 2) + lx + dx];
+              m0 += dx == 0 ? v : -v;
+              m1 += dy == 0 ? v : -v;
+              m2 += dz == 0 ? v : -v;
+            }
+        }
+        const T mi = a.minv[i + (long)j * d.X + (long)k * d.slab];
+        w0 = m0 * mi;
+        w1 = m1 * mi;
+        w2 = m2 * mi;
+      }
+      W[3 * t] = w0;
+      W[3 * t + 1] = w1;
+      W[3 * t + 2] = w2;
+    }
+  };
+
+  loadLayer(z0 - 1);
+  loadLayer(z0);
+  __syncthreads();
+  nodeLayer(z0);
+  __syncthreads();
+  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+  const int cx = x0 + lx, cy = y0 + ly;
+  for (int cz = z0; cz < zEnd; cz++) {
+    loadLayer(cz + 1);
+    __syncthreads();
+    nodeLayer(cz + 1);
+    __syncthreads();
+    if (cx < d.CX && cy < d.CY) {
+      T acc = 0;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+        const int ni = cx + ox, nj = cy + oy, nk = cz + oz;
+        if (ni < 1 || ni > d.X - 2 || nj < 1 || nj > d.Y - 2 || nk < 1 || nk > d.Z - 2)
+          continue;
+        const T* W = &ws[(cz + oz) & 1][3 * ((ly + oy) * (TX + 1) + lx + ox)];
+        acc += ox ? W[0] : -W[0];
+        acc += oy ? W[1] : -W[1];
+        acc += oz ? W[2] : -W[2];
+      }
+      const long c = cx + (long)cy * d.CX + (long)cz * d.cslab;
+      const T ownIn = xs[cz & 1][(ly + 1) * (TX + 2) + lx + 1];
+      if (MODE == M_APPLY) {
+        a.out[c] = acc;
+      } else if (MODE == M_SETUP_R) {
+        const T bv = a.b[c];
+        const T rv = (bv - acc) / a.D[c];
+        a.r[c] = rv;
+        a.p[c] = rv;
+        part0 += (double)bv * (double)bv;
+      } else if (MODE == M_SETUP_S) {
+        a.s[c] = acc;
+        a.Ap[c] = acc;
+        part0 += (double)ownIn * (double)acc;
+        const T q = acc / a.D[c];
+        part1 += (double)acc * (double)q;
+      } else if (MODE == M_UPDATE) {
+        a.s[c] = acc;
+        part0 += (double)ownIn * (double)acc;
+      } else if (MODE == M_CHECK) {
+        const T t = acc - a.b[c];
+        part0 += (double)t * (double)t;
+      } else if (MODE == M_IR_RESID) {
+        const T bv = a.b[c];
+        const T rr = bv - acc;
+        a.r32[c] = (float)rr;
+        part0 += (double)rr * (double)rr;
+        part1 += (double)bv * (double)bv;
+      }
+    }
+  }
+  if (MODE == M_APPLY) return;
+  double mine[2] = {part0, part1};
+  double tot[2];
+  if (lastBlockReduce<2>(mine, a.partials, a.counter, red, tot) && threadIdx.x == 0) {
+    CrScalars* sc = a.sc;
+    if (MODE == M_SETUP_R) {
+      sc->bnorm = sqrt(tot[0]);
+    } else if (MODE == M_SETUP_S) {
+      sc->delta = tot[0];
+      double apq = tot[1];
+      sc->alpha = (fabs(apq) > 0.0) ? tot[0] / apq : apq;
+    } else if (MODE == M_UPDATE) {
+      const double dOld = sc->delta;
+      sc->delta = tot[0];
+      sc->beta = tot[0] / dOld;
+    } else if (MODE == M_CHECK) {
+      const double rn = sqrt(tot[0]);
+      sc->resnorm = rn;
+      if (rn < sc->eps * sc->bnorm) {
+        sc->done = 1;
+        sc->iter = a.iter;
+      }
+    } else if (MODE == M_IR_RESID) {
+      sc->resnorm = sqrt(tot[0]);
+      if (a.first) sc->bnorm = sqrt(tot[1]);
+      if (sc->resnorm < sc->eps * sc->bnorm) sc->done = 1;
+    }
+  }
+}
+
+// b = J v0 - c, c (boundary velocities), D = diag(S); also the FP32 copies
+// used by the mixed solve.  COUPLED_INTEGRATOR_3D.cpp:286-325 (c),
+// :694-709 (D, b).
+__global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
+                             const double* __restrict__ vel,
+                             const double* __restrict__ minv, double* c,
+                             double* b, double* D, float* D32) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= d.cells) return;
+  const int cx = r % d.CX, cy = (r / d.CX) % d.CY, cz = r / d.cslab;
+  const long base = cx + (long)cy * d.X + (long)cz * d.slab;
+  double tmp = 0, cc = 0, dd = 0;
+  for (int k = 0; k < 8; k++) {
+    const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+    const long n = base + ox + (long)oy * d.X + (long)oz * d.slab;
+    const int ni = cx + ox, nj = cy + oy, nk = cz + oz;
+    const bool interior =
+        ni >= 1 && ni <= d.X - 2 && nj >= 1 && nj <= d.Y - 2 && nk >= 1 && nk <= d.Z - 2;
+    const double sx = ox ? 1.0 : -1.0, sy = oy ? 1.0 : -1.0, sz = oz ? 1.0 : -1.0;
+    if (interior) {
+      tmp += sx * v0[3 * n];
+      tmp += sy * v0[3 * n + 1];
+      tmp += sz * v0[3 * n + 2];
+      const double mi = minv[n];
+      dd += mi * 1.0;
+      dd += mi * 1.0;
+      dd += mi * 1.0;
+    } else {
+      cc -= sx * vel[3 * n];
+      cc -= sy * vel[3 * n + 1];
+      cc -= sz * vel[3 * n + 2];
+    }
+  }
+  c[r] = cc;
+  b[r] = tmp - cc;
+  D[r] = dd;
+  if (D32) D32[r] = (float)dd;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static void tileShape(const Dims& d, int& tx, int& ty, int& tz) {
+  tx = cdiv(d.CX, TX);
+  ty = cdiv(d.CY, TY);
+  tz = cdiv(d.CZ, TZ);
+}
+
+template <typename T, int MODE>
+esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
+  int tx, ty, tz;
+  tileShape(ctx->d, tx, ty, tz);
+  const int nb = tx * ty * tz;
+  const bool timed = (MODE == M_APPLY || MODE == M_UPDATE) && ctx->spmvTiming;
+  if (timed) hipEventRecord(ctx->spmvEv[0], ctx->stream);
+  hipLaunchKernelGGL((kSpmv<T, MODE>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d, a,
+                     tx, ty, tz);
+  if (timed) {
+    hipEventRecord(ctx->spmvEv[1], ctx->stream);
+    hipEventSynchronize(ctx->spmvEv[1]);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->spmvEv[0], ctx->spmvEv[1]);
+    ctx->spmvMs += ms;
+    ctx->spmvLaunches++;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    ctx->err = std::string("kSpmv launch: ") + hipGetErrorString(e);
+    return ESF_ERR_DEVICE;
+  }
+  return ESF_OK;
+}
+
+int spmvBlocks(const Dims& d) {
+  int tx, ty, tz;
+  tileShape(d, tx, ty, tz);
+  return tx * ty * tz;
+}
+
+esf_status pressureRhs(esf_ctx* ctx) {
+  const Dims& d = ctx->d;
+  hipLaunchKernelGGL(kPressureRhs, dim3(cdiv(d.cells, 256)), dim3(256), 0, ctx->stream, d,
+                     ctx->v0, ctx->vel, ctx->minv, ctx->c, ctx->b, ctx->D,
+                     ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr);
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y) {
+  SpmvArgs<double> a{};
+  a.in = x;
+  a.minv = ctx->minv;
+  a.out = y;
+  return launchSpmv<double, M_APPLY>(ctx, a);
+}
+
+static esf_status initScalars(esf_ctx* ctx, int which, double eps) {
+  CrScalars h{};
+  h.eps = eps;
+  h.iter = -1;
+  ESF_HIP(hipMemcpyAsync(&ctx->sc[which], &h, sizeof(h), hipMemcpyHostToDevice,
+                         ctx->stream));
+  return ESF_OK;
+}
+
+static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
+  ESF_HIP(hipMemcpyAsync(ctx->hostSc, &ctx->sc[which], sizeof(CrScalars),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  *out = *ctx->hostSc;
+  return ESF_OK;
+}
+
+// CONJUGATE_RESIDUAL::solvePCR (matrix-free) on S, FP64 or FP32 vectors.
+// x is the start vector.  Scalar block `which`.  *iters receives the
+// reference's iteration count; *relres the last checked |Sx-b|/|b|.
+template <typename T>
+esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
+                   T* x, T* r, T* r2, T* p, T* s, T* Ap, double eps, int maxIt,
+                   int* iters, bool* converged, double* relres) {
+  esf_status st = initScalars(ctx, which, eps);
+  if (st) return st;
+  const Dims& d = ctx->d;
+  SpmvArgs<T> a{};
+  a.minv = minv;
+  a.b = b;
+  a.D = D;
+  a.x = x;
+  a.r = r;
+  a.p = p;
+  a.s = s;
+  a.Ap = Ap;
+  a.sc = &ctx->sc[which];
+  a.partials = ctx->partials;
+  a.counter = &ctx->counter[which];
+  T* rA = r;
+  T* rB = r2;
+  a.in = x;
+  a.r = rA;
+  if ((st = launchSpmv<T, M_SETUP_R>(ctx, a))) return st;
+  a.in = rA;
+  if ((st = launchSpmv<T, M_SETUP_S>(ctx, a))) return st;
+  const int dirBlocks = 1024;
+  DonePoller poll(ctx, which);
+  int i = 0;
+  for (; i < maxIt; i++) {
+    a.iter = i;
+    a.in = rA;
+    a.r = rB;
+    if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
+    hipLaunchKernelGGL(kCrDirection<T>, dim3(dirBlocks), dim3(256), 0, ctx->stream,
+                       (long)d.cells, p, Ap, (const T*)rB, (const T*)s, D, a.sc,
+                       ctx->partials, &ctx->counter[which]);
+    std::swap(rA, rB);
+    if (i > 0 && i % 5 == 0) {
+      a.in = x;
+      if ((st = launchSpmv<T, M_CHECK>(ctx, a))) return st;
+      if (poll.mark()) break;
+    }
+  }
+  CrScalars h;
+  if ((st = readScalars(ctx, which, &h))) return st;
+  *converged = h.done == 1;
+  *iters = h.done ? h.iter : maxIt;
+  *relres = h.bnorm > 0 ? h.resnorm / h.bnorm : h.resnorm;
+  return ESF_OK;
+}
+
+template esf_status crSolve<double>(esf_ctx*, int, const double*, const double*,
+                                    const double*, double*, double*, double*, double*,
+                                    double*, double*, double, int, int*, bool*, double*);
+template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*,
+                                   const float*, float*, float*, float*, float*, float*,
+                                   float*, double, int, int*, bool*, double*);
+
+__global__ void kToFloat(long n, const double* a, float* b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+__global__ void kZeroF(long n, float* a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    a[i] = 0.0f;
+}
+
+// Mixed-precision pressure solve: FP32 CR on S (FP32 minv, FP32 vectors,
+// FP64 inner products) inside FP64 iterative refinement.  The outer
+// residual b - S x, the correction x += e and |r| stay in FP64.
+esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
+                      double* relres, int* refinements, int* innerIters) {
+  const Dims& d = ctx->d;
+  esf_status st = initScalars(ctx, 0, eps);
+  if (st) return st;
+  hipLaunchKernelGGL(kToFloat, dim3(1024), dim3(256), 0, ctx->stream, d.nodes,
+                     (const double*)ctx->minv, ctx->minv32);
+  SpmvArgs<double> o{};
+  o.minv = ctx->minv;
+  o.b = ctx->b;
+  o.in = ctx->lambda;
+  o.x = ctx->lambda2;
+  o.r32 = ctx->rhs32;
+  o.e32 = nullptr;
+  o.sc = &ctx->sc[0];
+  o.partials = ctx->partials;
+  o.counter = &ctx->counter[0];
+  o.first = 1;
+  if ((st = launchSpmv<double, M_IR_RESID>(ctx, o))) return st;
+  CrScalars h;
+  if ((st = readScalars(ctx, 0, &h))) return st;
+  int outer = 0, inner = 0;
+  const int maxOuter = 30;
+  while (!h.done && outer < maxOuter && inner < maxIt) {
+    // inner tolerance: ask for the reduction the outer test still needs,
+    // bounded by what FP32 can deliver
+    double need = h.resnorm > 0 ? 0.3 * eps * h.bnorm / h.resnorm : 1.0;
+    double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
+    hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
+    int it = 0;
+    bool conv = false;
+    double rr = 0;
+    if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->e32, ctx->r32,
+                             ctx->r32b, ctx->p32, ctx->s32, ctx->Ap32, innerEps,
+                             maxIt - inner, &it, &conv, &rr)))
+      return st;
+    inner += it;
+    outer++;
+    o.e32 = ctx->e32;
+    o.first = 0;
+    o.in = ctx->lambda;
+    o.x = ctx->lambda2;
+    if ((st = launchSpmv<double, M_IR_RESID>(ctx, o))) return st;
+    std::swap(ctx->lambda, ctx->lambda2);
+    if ((st = readScalars(ctx, 0, &h))) return st;
+  }
+  *converged = h.done == 1;
+  *iters = inner;
+  *relres = h.bnorm > 0 ? h.resnorm / h.bnorm : h.resnorm;
+  *refinements = outer;
+  *innerIters = inner;
+  return ESF_OK;
+}
+
+}  // namespace esf

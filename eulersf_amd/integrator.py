@@ -1,0 +1,259 @@
+"""Python mirror of COUPLED_INTEGRATOR_3D (include/integrator/
+COUPLED_INTEGRATOR_3D.h:30-76) over the C ABI of libesf.so.
+
+Method names follow the reference's member functions; every call goes to
+the HIP library and raises EsfError when it reports an error.
+"""
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, dptr
+
+
+class CoupledIntegrator3D:
+    """One scene resident on the GPU.  Create it from a reference cfg file
+    (``from_cfg``) or from explicit grid/params (``__init__``)."""
+
+    def __init__(self, grid, params, precision=_abi.ESF_PLAIN_FP64, _ctx=None):
+        self.lib = _abi.load()
+        self.grid = grid
+        if _ctx is None:
+            g = _abi.Grid(grid["xres"], grid["yres"], grid["zres"], grid["dh"],
+                          (ctypes.c_double * 3)(*grid["origin"]))
+            p = _abi.Params(**{k: v for k, v in params.items() if k in dict(_abi.Params._fields_)})
+            p.precision = precision
+            ctx = ctypes.c_void_p()
+            check(None, self.lib.esf_ctx_create(ctypes.byref(ctx), ctypes.byref(g), ctypes.byref(p)),
+                  "esf_ctx_create")
+            _ctx = ctx
+        self.ctx = _ctx
+        X, Y, Z = grid["xres"], grid["yres"], grid["zres"]
+        self.nodes = X * Y * Z
+        self.cells = (X - 1) * (Y - 1) * (Z - 1)
+        self.info = None
+
+    @classmethod
+    def from_cfg(cls, path, precision=_abi.ESF_PLAIN_FP64):
+        """APPLICATION::init (projects/rubberBall/main.cpp:63-97) from a cfg."""
+        from .cfg import Cfg
+        lib = _abi.load()
+        ctx = ctypes.c_void_p()
+        info = _abi.SceneInfo()
+        st = lib.esf_scene_load_cfg(path.encode(), precision, ctypes.byref(ctx), ctypes.byref(info))
+        if st < 0:
+            raise _abi.EsfError(st, f"esf_scene_load_cfg({path})")
+        c = Cfg(path=path)
+        grid = {"xres": c.get_int("xRes", 100), "yres": c.get_int("yRes", 100),
+                "zres": c.get_int("zRes", 100), "dh": c.get_float("dh", 0.01),
+                "origin": c.get_vec3("origin", (0, 0, 0))}
+        obj = cls(grid, None, precision, _ctx=ctx)
+        obj.info = info
+        return obj
+
+    @classmethod
+    def from_cfg_text(cls, text, precision=_abi.ESF_PLAIN_FP64):
+        fd, path = tempfile.mkstemp(suffix=".cfg")
+        with os.fdopen(fd, "w") as f:
+            f.write(text)
+        try:
+            return cls.from_cfg(path, precision)
+        finally:
+            os.unlink(path)
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if self.ctx:
+            self.lib.esf_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- fields -------------------------------------------------------------
+    def _shape(self, field):
+        n, c = self.nodes, self.cells
+        ns = self.num_solid_nodes()
+        table = {
+            _abi.F_VELOCITY: (3 * n, np.float64), _abi.F_INTENSITY: (n, np.float64),
+            _abi.F_SOLID_MASS: (n, np.float64), _abi.F_NODE_FRACTION: (n, np.float64),
+            _abi.F_LAMBDA: (c, np.float64), _abi.F_SOLID_INDEX: (n, np.int32),
+            _abi.F_V0: (3 * n, np.float64), _abi.F_MINV: (n, np.float64),
+            _abi.F_PRESSURE_RHS: (c, np.float64), _abi.F_DIV_RHS: (c, np.float64),
+            _abi.F_SOLID_FORCE: (3 * ns, np.float64), _abi.F_SOLID_MV: (3 * ns, np.float64),
+            _abi.F_DAMPED_MASS: (3 * ns, np.float64), _abi.F_SYS_DIAG: (3 * ns, np.float64),
+            _abi.F_PRESSURE_DIAG: (c, np.float64), _abi.F_SOLID_V0: (3 * ns, np.float64),
+            _abi.F_SOLID_DV: (3 * ns, np.float64), _abi.F_WORKSPACE: (n, np.float64),
+            _abi.S_U: (3 * n, np.float64), _abi.S_X: (3 * n, np.float64),
+            _abi.S_VELOCITY: (3 * n, np.float64), _abi.S_MASS: (n, np.float64),
+            _abi.S_VALID: (c, np.int8), _abi.S_QFRAC: (8 * n, np.float64),
+            _abi.S_VEL_WORKSPACE: (3 * n, np.float64),
+        }
+        return table[field]
+
+    def download(self, field, solid=-1):
+        count, dtype = self._shape(field)
+        out = np.empty(count, dtype=dtype)
+        check(self.ctx, self.lib.esf_download(self.ctx, field, solid, out.ctypes.data_as(ctypes.c_void_p),
+                                              out.nbytes), f"download {field}")
+        return out
+
+    def upload(self, field, arr, solid=-1):
+        count, dtype = self._shape(field)
+        a = np.ascontiguousarray(arr, dtype=dtype).reshape(-1)
+        if a.size != count:
+            raise ValueError(f"field {field}: expected {count} values, got {a.size}")
+        check(self.ctx, self.lib.esf_upload(self.ctx, field, solid, a.ctypes.data_as(ctypes.c_void_p),
+                                            a.nbytes), f"upload {field}")
+
+    def num_solid_nodes(self):
+        n = ctypes.c_int()
+        check(self.ctx, self.lib.esf_num_solid_nodes(self.ctx, ctypes.byref(n)))
+        return n.value
+
+    def boxes(self, solid=0):
+        b = (ctypes.c_int * 12)()
+        check(self.ctx, self.lib.esf_solid_get_boxes(self.ctx, solid, b))
+        return list(b)
+
+    def set_boxes(self, solid, boxes):
+        b = (ctypes.c_int * 12)(*boxes)
+        check(self.ctx, self.lib.esf_solid_set_boxes(self.ctx, solid, b))
+
+    # -- stages (COUPLED_INTEGRATOR_3D / DEFOR_SOLID_3D members) -------------
+    def setMaxDt(self, dt):
+        check(self.ctx, self.lib.esf_set_max_dt(self.ctx, dt))
+
+    def setDt(self, dt):
+        check(self.ctx, self.lib.esf_set_dt(self.ctx, dt))
+
+    def dt(self):
+        v = ctypes.c_double()
+        check(self.ctx, self.lib.esf_get_dt(self.ctx, ctypes.byref(v)))
+        return v.value
+
+    def CFL(self, alpha):
+        v = ctypes.c_double()
+        check(self.ctx, self.lib.esf_cfl(self.ctx, alpha, ctypes.byref(v)), "CFL")
+        return v.value
+
+    def computeSolidMass(self, solid=0):
+        check(self.ctx, self.lib.esf_compute_solid_mass(self.ctx, solid), "computeSolidMass")
+
+    def updateBoxes(self, solid=0):
+        check(self.ctx, self.lib.esf_update_boxes(self.ctx, solid), "updateBoxes")
+
+    def gatherSolidMasses(self):
+        check(self.ctx, self.lib.esf_gather_solid_masses(self.ctx), "gatherSolidMasses")
+
+    def recoverVelocityField(self):
+        check(self.ctx, self.lib.esf_recover_velocity(self.ctx), "recoverVelocityField")
+
+    def forces(self):
+        e = ctypes.c_double()
+        check(self.ctx, self.lib.esf_forces(self.ctx, ctypes.byref(e)), "forces")
+        return e.value
+
+    def pcrImplicitSolve(self):
+        s = _abi.SolveStats()
+        st = check(self.ctx, self.lib.esf_coupled_solve(self.ctx, ctypes.byref(s)), "coupled solve")
+        d = s.as_dict()
+        d["status"] = st
+        return d
+
+    def solidCopyVelocity(self):
+        check(self.ctx, self.lib.esf_solid_copy_velocity(self.ctx))
+
+    def solidFinalize(self):
+        check(self.ctx, self.lib.esf_solid_finalize(self.ctx))
+
+    def advectFluidVelocityField(self):
+        check(self.ctx, self.lib.esf_advect_velocity(self.ctx))
+
+    def advectIntensity(self):
+        check(self.ctx, self.lib.esf_advect_intensity(self.ctx))
+
+    def recoverAndBoundary(self):
+        check(self.ctx, self.lib.esf_recover_and_bc(self.ctx))
+
+    def stepImplicit(self, alpha=-1.0):
+        """One drop-in substep (esf_step); returns the solve report."""
+        s = _abi.SolveStats()
+        st = check(self.ctx, self.lib.esf_step(self.ctx, alpha, ctypes.byref(s)), "stepImplicit")
+        d = s.as_dict()
+        d["status"] = st
+        return d
+
+    def frame(self, frame_dt, alpha=-1.0):
+        n = ctypes.c_int()
+        check(self.ctx, self.lib.esf_frame(self.ctx, frame_dt, alpha, ctypes.byref(n)), "frame")
+        return n.value
+
+    # -- operators ----------------------------------------------------------
+    def getMatVecMult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        check(self.ctx, self.lib.esf_pressure_apply(self.ctx, dptr(x), dptr(y)), "pressure apply")
+        return y
+
+    def pressureDiag(self):
+        d = np.empty(self.cells)
+        check(self.ctx, self.lib.esf_pressure_diag(self.ctx, dptr(d)))
+        return d
+
+    def solidApply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        check(self.ctx, self.lib.esf_solid_apply(self.ctx, dptr(x), dptr(y)))
+        return y
+
+    def pressureSolve(self, b, x0, eps, max_it):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        st = check(self.ctx, self.lib.esf_pressure_solve(self.ctx, dptr(b), dptr(x), eps, max_it,
+                                                         ctypes.byref(it), ctypes.byref(rr)))
+        return x, it.value, rr.value, st
+
+    # -- timing -------------------------------------------------------------
+    def set_stage_timing(self, on=True):
+        check(self.ctx, self.lib.esf_set_stage_timing(self.ctx, 1 if on else 0))
+
+    def stage_times(self, reset=False):
+        a = np.zeros(16)
+        check(self.ctx, self.lib.esf_stage_times(self.ctx, dptr(a), 16, 1 if reset else 0))
+        return dict(zip(_abi.STAGES, a[:len(_abi.STAGES)]))
+
+    def set_spmv_timing(self, on=True):
+        check(self.ctx, self.lib.esf_set_spmv_timing(self.ctx, 1 if on else 0))
+
+    def spmv_timing(self, reset=False):
+        avg = ctypes.c_double()
+        n = ctypes.c_long()
+        check(self.ctx, self.lib.esf_spmv_timing(self.ctx, ctypes.byref(avg), ctypes.byref(n),
+                                                 1 if reset else 0))
+        return avg.value, n.value
+
+
+def corotation_batch(F, dF, lam, mu):
+    """COROTATION on a batch of F (n,3,3) on the GPU: returns R, S, L, P,
+    energy, dP(dF)."""
+    lib = _abi.load()
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1, 9)
+    dF = np.ascontiguousarray(dF, dtype=np.float64).reshape(-1, 9)
+    n = F.shape[0]
+    out = np.empty((n, 36))
+    en = np.empty(n)
+    dP = np.empty((n, 9))
+    st = lib.esf_corotation_batch(n, lam, mu, dptr(F), dptr(dF), dptr(out), dptr(en), dptr(dP))
+    if st < 0:
+        raise _abi.EsfError(st, "esf_corotation_batch")
+    o = out.reshape(n, 4, 3, 3)
+    return o[:, 0], o[:, 1], o[:, 2], o[:, 3], en, dP.reshape(n, 3, 3)

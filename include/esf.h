@@ -1,0 +1,256 @@
+/*
+ * esf.h -- C ABI of libesf.so, the MI355X (gfx950) drop-in for the
+ * stress, coupled-solve and advection stages of the reference's
+ * COUPLED_INTEGRATOR_3D::stepImplicit().
+ *
+ * The reference has no C ABI; these entry points replace C++ member calls
+ * (cited per entry point, paths relative to the reference root).  Its only
+ * C-style extension point is the matrix-free callback of the CR solver
+ *   bool solvePCR(const VECTOR& b, VECTOR& x,
+ *                 void (*getMatVecMult)(void*, const VECTOR&, VECTOR&), void*)
+ *   (include/linearalgebra/CONJUGATE_RESIDUAL.h:38-40), served by
+ *   COUPLED_INTEGRATOR_3D::staticGetMatVecMult
+ *   (include/integrator/COUPLED_INTEGRATOR_3D.h:69-73);
+ * esf_pressure_apply() has exactly that callback's semantics.
+ *
+ * Conventions
+ *  - Every call returns esf_status: 0 = ok, 1 = a Krylov solve did not reach
+ *    its tolerance (reported, not fatal, as in CONJUGATE_RESIDUAL.inl:186-189),
+ *    negative = error; esf_last_error() describes the last error.
+ *  - Host buffers are owned by the caller and never retained.  Device state
+ *    (grids, solids, Krylov vectors) is owned by the context and stays
+ *    resident in HBM between calls.
+ *  - One host thread per context; each context owns one HIP stream.
+ *    Contexts are independent (no global state), unlike the reference's
+ *    function-local statics (COUPLED_INTEGRATOR_3D.cpp:639, :712).
+ *  - Layouts are the reference's: node n = x + y*X + z*X*Y (FIELD_3D.h:72);
+ *    vector fields are AoS, 3 doubles per node (VEC3F_FIELD_3D.h:64-72);
+ *    pressure cell r = x + y*(X-1) + z*(X-1)*(Y-1)
+ *    (COUPLED_INTEGRATOR_3D.cpp:302-306); quarter fractions [8*n + corner];
+ *    solid DOFs 3*s + c with s numbered in z,y,x order over interior nodes
+ *    carrying solid mass (COUPLED_INTEGRATOR_3D.cpp:361-372).
+ *  - The pressure solve warm-starts from the previous call's lambda, as the
+ *    reference's static lambda1 does (COUPLED_INTEGRATOR_3D.cpp:712).
+ */
+#ifndef ESF_H
+#define ESF_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct esf_ctx esf_ctx;
+typedef int esf_status;
+
+#define ESF_OK 0
+#define ESF_NOT_CONVERGED 1
+#define ESF_ERR_INVALID (-1)
+#define ESF_ERR_OOM (-2)
+#define ESF_ERR_DEVICE (-3)
+#define ESF_ERR_NAN (-4)
+#define ESF_ERR_STATE (-5)
+#define ESF_ERR_IO (-6)
+
+/* precision of the pressure solve */
+#define ESF_PLAIN_FP64 0  /* CR in FP64, the reference's algorithm */
+#define ESF_MIXED 1       /* FP32 CR inside FP64 iterative refinement */
+
+typedef struct {
+  int xres, yres, zres; /* node counts ("xRes","yRes","zRes") */
+  double dh;            /* "dh" */
+  double origin[3];     /* "origin" */
+} esf_grid;
+
+/* cfg-derived parameters; keys and defaults are the reference's
+ * (COUPLED_INTEGRATOR_3D.cpp:43-74, :445-446, :499-501;
+ *  CONJUGATE_RESIDUAL.inl:26-27; DEFOR_SOLID_3D.cpp:62, :243-244). */
+typedef struct {
+  double dt;                 /* "dt" */
+  double vorticity_eps;      /* "vorticity eps" as written (scaled inside) */
+  double gravity;            /* "gravity" */
+  double buoyancy;           /* "buoyancy" */
+  double rayleigh_alpha;     /* "rayleigh alpha" */
+  double rayleigh_beta;      /* "rayleigh beta" */
+  double fluid_density;      /* "fluid density" */
+  double implicit_eps;       /* "implicit solver eps" (solid solves) */
+  double divfree_eps;        /* "div free solver eps" (pressure solve) */
+  int cr_max_iterations;     /* "cr max iterations" */
+  double mass_sample_dh;     /* "mass sample dh" */
+  int tight_border;          /* "tight border" */
+  int extrapolation_cells;   /* "extrapolation cells" */
+  int precision;             /* ESF_PLAIN_FP64 or ESF_MIXED */
+  double inner_eps;          /* mixed: FP32 inner solve relative tolerance */
+} esf_params;
+
+/* per-step solve report */
+typedef struct {
+  int iterations[3];      /* solid solve 1, pressure solve, solid solve 2 */
+  int converged[3];
+  double rel_residual[3]; /* true |Ax-b|/|b| at exit, FP64 */
+  int refinements;        /* mixed: outer FP64 refinement steps */
+  int inner_iterations;   /* mixed: FP32 iterations summed over refinements */
+  double dt;              /* the substep's dt */
+  int n_solid_nodes;      /* solid DOF nodes this step */
+} esf_solve_stats;
+
+/* field ids for esf_upload / esf_download */
+enum {
+  ESF_F_VELOCITY = 0,      /* double[3*nodes]  _velocity                 */
+  ESF_F_INTENSITY = 1,     /* double[nodes]    _intensity                */
+  ESF_F_SOLID_MASS = 2,    /* double[nodes]    _solidMass (download)     */
+  ESF_F_NODE_FRACTION = 3, /* double[nodes]    _nodeSolidVolumeFraction  */
+  ESF_F_LAMBDA = 4,        /* double[cells]    pressure (warm start)     */
+  ESF_F_SOLID_INDEX = 6,   /* int32[nodes]     _gridIdxToSolidIdx        */
+  ESF_F_MINV = 9,          /* double[nodes]    1/m at interior nodes     */
+  ESF_F_PRESSURE_RHS = 10, /* double[cells]    b = J v0 - c              */
+  ESF_F_DIV_RHS = 11,      /* double[cells]    c                         */
+  ESF_F_V0 = 7,            /* double[3*nodes]  v0 on interior nodes      */
+  ESF_F_SOLID_FORCE = 13,  /* double[3*nSolid] elastic+body force (pre-dt) */
+  ESF_F_SOLID_MV = 14,     /* double[3*nSolid] */
+  ESF_F_DAMPED_MASS = 15,  /* double[3*nSolid] */
+  ESF_F_SYS_DIAG = 20,     /* double[3*nSolid] diag(A) */
+  ESF_F_PRESSURE_DIAG = 21,/* double[cells]    diag(J M^-1 J^T) */
+  ESF_F_SOLID_V0 = 18,     /* double[3*nSolid] */
+  ESF_F_SOLID_DV = 19,     /* double[3*nSolid] */
+  ESF_F_WORKSPACE = 23,    /* double[nodes]    |omega| */
+  /* per solid (pass the solid id) -- DEFOR_SOLID_3D members */
+  ESF_S_U = 100,           /* double[3*nodes]  _u            */
+  ESF_S_X = 101,           /* double[3*nodes]  _X            */
+  ESF_S_VELOCITY = 102,    /* double[3*nodes]  _velocity     */
+  ESF_S_MASS = 103,        /* double[nodes]    _mass         */
+  ESF_S_VALID = 104,       /* int8[cells]      _valid        */
+  ESF_S_QFRAC = 105,       /* double[8*nodes]  _quarterVolumeFractions */
+  ESF_S_VEL_WORKSPACE = 106/* double[3*nodes]  _vecWorkspace */
+};
+
+/* --- context --------------------------------------------------------- */
+/* COUPLED_INTEGRATOR_3D ctor (COUPLED_INTEGRATOR_3D.cpp:36-78) */
+esf_status esf_ctx_create(esf_ctx** out, const esf_grid* grid,
+                          const esf_params* params);
+void esf_ctx_destroy(esf_ctx* ctx);
+const char* esf_last_error(const esf_ctx* ctx);
+esf_status esf_set_params(esf_ctx* ctx, const esf_params* params);
+/* COUPLED_INTEGRATOR_3D::setMaxDt / dt (COUPLED_INTEGRATOR_3D.h:45-46) */
+esf_status esf_set_max_dt(esf_ctx* ctx, double max_dt);
+esf_status esf_set_dt(esf_ctx* ctx, double dt);
+esf_status esf_get_dt(esf_ctx* ctx, double* dt);
+esf_status esf_num_solid_nodes(esf_ctx* ctx, int* n);
+
+/* --- solids -------------------------------------------------------------*/
+/* DEFOR_SOLID_3D ctor + setMaterial + setMassDensity
+ * (DEFOR_SOLID_3D.cpp:43-84, :113-124; DEFOR_SOLID_3D.h:64-66); rest mass
+ * is the FIELD_3Df the ctor takes (read from "<name>.mass" by
+ * rubberBall/main.cpp:108-111). */
+esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu,
+                         double density, int rest_x, int rest_y, int rest_z,
+                         double rest_dh, const double rest_origin[3],
+                         const double* rest_mass, int* solid_id);
+/* tight box and extended box, as tightMin,tightMax,min,max (12 ints) */
+esf_status esf_solid_set_boxes(esf_ctx* ctx, int solid, const int boxes[12]);
+esf_status esf_solid_get_boxes(esf_ctx* ctx, int solid, int boxes[12]);
+
+/* --- host <-> device ----------------------------------------------------*/
+esf_status esf_upload(esf_ctx* ctx, int field, int solid, const void* src,
+                      size_t bytes);
+esf_status esf_download(esf_ctx* ctx, int field, int solid, void* dst,
+                        size_t bytes);
+
+/* --- stages -------------------------------------------------------------*/
+/* DEFOR_SOLID_3D::computeSolidMass(false) (DEFOR_SOLID_3D.cpp:576-669) */
+esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid);
+/* box part of DEFOR_SOLID_3D::computeSDF (DEFOR_SOLID_3D.cpp:216-266) */
+esf_status esf_update_boxes(esf_ctx* ctx, int solid);
+/* COUPLED_INTEGRATOR_3D::gatherSolidMasses (COUPLED_INTEGRATOR_3D.cpp:327-375) */
+esf_status esf_gather_solid_masses(esf_ctx* ctx);
+/* DEFOR_SOLID_3D::recoverVelocityField for every solid (:1235-1241) */
+esf_status esf_recover_velocity(esf_ctx* ctx);
+/* COUPLED_INTEGRATOR_3D::CFL (COUPLED_INTEGRATOR_3D.cpp:88-102); sets dt */
+esf_status esf_cfl(esf_ctx* ctx, double alpha, double* dt);
+/* stepImplicit :413-607: corotational stress and element stiffness
+ * (DEFOR_SOLID_3D.cpp:697-871), computeMV (:873-889), vorticity and body
+ * forces.  energy may be NULL. */
+esf_status esf_forces(esf_ctx* ctx, double* energy);
+/* stepImplicit :609-632 + pcrImplicitSolve (:662-764): system diagonal,
+ * divergence RHS, solid solve, pressure solve, solid correction solve,
+ * velocity write-back.  Returns ESF_NOT_CONVERGED if a solve stopped at
+ * "cr max iterations". */
+esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats);
+/* DEFOR_SOLID_3D::finalizeSolution(velocity, dt) (:1182-1197) */
+esf_status esf_solid_copy_velocity(esf_ctx* ctx);
+/* DEFOR_SOLID_3D::finalizeSolution() (:1198-1233) */
+esf_status esf_solid_finalize(esf_ctx* ctx);
+/* advectFluidVelocityField (COUPLED_INTEGRATOR_3D.cpp:797-826) */
+esf_status esf_advect_velocity(esf_ctx* ctx);
+/* advectField(_intensity) (COUPLED_INTEGRATOR_3D.cpp:766-796) */
+esf_status esf_advect_intensity(esf_ctx* ctx);
+/* recoverVelocityField + Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:894-916) */
+esf_status esf_recover_and_bc(esf_ctx* ctx);
+/* One substep of the drop-in: CFL(alpha) -> forces -> coupled solve ->
+ * solid velocity copy -> solid finalize -> gather solid masses -> advect
+ * velocity -> advect intensity -> recover + Dirichlet.  The stages of
+ * COUPLED_INTEGRATOR_3D::finalizeSolution that stay on the host (FLIP
+ * particles, mass resampling, SDF, surface mesh; :833-879) are not run. */
+esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
+
+/* --- operators (unit level) ---------------------------------------------*/
+/* getMatVecMult (COUPLED_INTEGRATOR_3D.cpp:638-660): y = J M^-1 J^T x on
+ * the current masses.  x, y are host arrays of (X-1)(Y-1)(Z-1) doubles. */
+esf_status esf_pressure_apply(esf_ctx* ctx, const double* x, double* y);
+/* _JMInvJTDiag (COUPLED_INTEGRATOR_3D.cpp:694-705) */
+esf_status esf_pressure_diag(esf_ctx* ctx, double* d);
+/* A x with A = (dt^2 + beta dt) K + diag(M~) (COUPLED_INTEGRATOR_3D.cpp:
+ * 609-619), matrix-free, after esf_forces.  x, y: 3*nSolid doubles. */
+esf_status esf_solid_apply(esf_ctx* ctx, const double* x, double* y);
+/* solvePCR(b, x, getMatVecMult, app) on the pressure system
+ * (CONJUGATE_RESIDUAL.inl:109-192); x in/out (warm start). */
+esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x,
+                              double eps, int max_it, int* iterations,
+                              double* rel_residual);
+/* COROTATION::init + firstPiolaKirchhoff + strainEnergy +
+ * firstPKDifferential (COROTATION.cpp:37-107) on n deformation gradients,
+ * one GPU thread each.  F, dF: n*9 row-major; out: n*36 (R,S,L,P);
+ * energy: n; dP: n*9 (dF applied). */
+esf_status esf_corotation_batch(int n, double lambda, double mu,
+                                const double* F, const double* dF,
+                                double* out, double* energy, double* dP);
+
+/* --- timing ---------------------------------------------------------------*/
+/* enable per-stage HIP-event timing (synchronises at every stage end) */
+esf_status esf_set_stage_timing(esf_ctx* ctx, int on);
+/* milliseconds spent per stage since the last reset, measured with HIP
+ * events on the context's stream (when enabled): 0 cfl, 1 forces, 2 solid solves,
+ * 3 pressure solve, 4 solid finalize, 5 gather, 6 advect velocity,
+ * 7 advect intensity, 8 recover+bc */
+esf_status esf_stage_times(esf_ctx* ctx, double* ms, int n, int reset);
+/* average duration (ms) of the pressure SpMV kernel over the launches timed
+ * since the last reset, and the launch count */
+esf_status esf_spmv_timing(esf_ctx* ctx, double* avg_ms, long* launches,
+                           int reset);
+/* enable HIP-event timing around every pressure SpMV launch (costly) */
+esf_status esf_set_spmv_timing(esf_ctx* ctx, int on);
+
+/* --- host scene setup (cfg-driven, rubberBall/main.cpp:63-160) ------------*/
+typedef struct {
+  int n_solids;
+  int frame_start, frame_end;
+  double frame_dt;
+  double carpet_thickness;
+} esf_scene_info;
+/* Parse a reference cfg (SIMPLE_PARSER semantics), create the context,
+ * add the solids, run COUPLED_INTEGRATOR_3D::init and lay the smoke carpet.
+ * A solid's rest mass is read from "<solid path i><solid name i>.mass"
+ * (FIELD_3D::readGz format) when that file exists, otherwise rasterized
+ * from the analytic sphere "solid sphere radius i" in rest space. */
+esf_status esf_scene_load_cfg(const char* cfg_path, int precision,
+                              esf_ctx** out, esf_scene_info* info);
+/* APPLICATION::step (rubberBall/main.cpp:174-205): substeps until the
+ * frame's dt is covered.  alpha < 0 chooses the reference's broad-phase
+ * rule from the solid mass bounding box. */
+esf_status esf_frame(esf_ctx* ctx, double frame_dt, double alpha,
+                     int* substeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ESF_H */

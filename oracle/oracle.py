@@ -1,0 +1,262 @@
+"""ctypes bindings of oracle/liboracle.so (see esf_oracle.h) and of the
+reference-built checker oracle/_ref/libesfref.so.
+
+TEST INFRASTRUCTURE ONLY.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libesfref.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+# field ids (esf_oracle.h)
+VELOCITY, INTENSITY, SOLID_MASS, NODE_FRACTION, LAMBDA, VEL_WORKSPACE, SOLID_INDEX = 0, 1, 2, 3, 4, 5, 6
+V0, V1, MINV, PRESSURE_RHS, DIV_RHS, JT_LAMBDA = 7, 8, 9, 10, 11, 12
+SOLID_FORCE, SOLID_MV, DAMPED_MASS, UNIT_FORCE, MV, SOLID_V0, SOLID_DV = 13, 14, 15, 16, 17, 18, 19
+SYS_DIAG, PRESSURE_DIAG, VORTICITY, WORKSPACE = 20, 21, 22, 23
+S_U, S_X, S_VELOCITY, S_MASS, S_VALID, S_QFRAC, S_VEL_WORKSPACE, S_DETFINV = range(100, 108)
+
+_INT_FIELDS = {SOLID_INDEX: np.int32, S_VALID: np.int8}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("vorticity_eps", ctypes.c_double),
+                ("gravity", ctypes.c_double), ("buoyancy", ctypes.c_double),
+                ("rayleigh_alpha", ctypes.c_double), ("rayleigh_beta", ctypes.c_double),
+                ("fluid_density", ctypes.c_double), ("implicit_eps", ctypes.c_double),
+                ("divfree_eps", ctypes.c_double), ("cr_max_iterations", ctypes.c_int),
+                ("mass_sample_dh", ctypes.c_double), ("tight_border", ctypes.c_int),
+                ("extrapolation_cells", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int * 3), ("converged", ctypes.c_int * 3),
+                ("rel_residual", ctypes.c_double * 3), ("dt", ctypes.c_double),
+                ("energy", ctypes.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise ImportError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, _D, ctypes.POINTER(Params)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_add_solid.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D,
+                                    _D, _D, _D, _D]
+        L.orc_init.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_field.restype = ctypes.c_void_p
+        L.orc_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+        L.orc_num_solid_dofs.argtypes = [ctypes.c_void_p]
+        L.orc_get_boxes.argtypes = [ctypes.c_void_p, ctypes.c_int, _I]
+        L.orc_set_boxes.argtypes = [ctypes.c_void_p, ctypes.c_int, _I]
+        L.orc_cfl.restype = ctypes.c_double
+        L.orc_cfl.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_max_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_forces.restype = ctypes.c_double
+        L.orc_forces.argtypes = [ctypes.c_void_p]
+        for n in ("orc_assemble", "orc_solid_copy_velocity", "orc_solid_finalize", "orc_gather",
+                  "orc_advect_velocity", "orc_advect_intensity", "orc_recover_and_bc"):
+            getattr(L, n).argtypes = [ctypes.c_void_p]
+        L.orc_compute_solid_mass.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_update_boxes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Stats)]
+        L.orc_pressure_apply.argtypes = [ctypes.c_void_p, _D, _D]
+        L.orc_pressure_diag.argtypes = [ctypes.c_void_p, _D]
+        L.orc_solid_apply.argtypes = [ctypes.c_void_p, _D, _D]
+        L.orc_pressure_cr.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_double, ctypes.c_int, _I]
+        L.orc_pcr_dense.argtypes = [ctypes.c_int, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int, _I]
+        L.orc_corotation.argtypes = [ctypes.c_double, ctypes.c_double, _D, _D, _D, _D, _D]
+        L.orc_interp_scalar.restype = ctypes.c_double
+        L.orc_interp_scalar.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D]
+        L.orc_interp_vec3.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D]
+        L.orc_last_stage_seconds.restype = ctypes.c_double
+        L.orc_last_stage_seconds.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_sphere_rest_dims.argtypes = [ctypes.c_double, ctypes.c_double, _I, _D]
+        L.orc_sphere_rest_fill.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own COROTATION / CR / FIELD_3D / SIMPLE_PARSER code."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise ImportError(f"{REF_SO} missing (built only where the reference tree exists)")
+        R = ctypes.CDLL(REF_SO)
+        R.ref_corotation.argtypes = [ctypes.c_double, ctypes.c_double, _D, _D, _D, _D, _D]
+        R.ref_interp_scalar.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        _D, ctypes.c_int, _D, _D]
+        R.ref_interp_vec3.argtypes = R.ref_interp_scalar.argtypes
+        R.ref_pcr_csr.argtypes = [ctypes.c_int, ctypes.c_int, _I, _I, _D, _D, _D, ctypes.c_double,
+                                  ctypes.c_int, _I]
+        R.ref_pcr_matfree_dense.argtypes = [ctypes.c_int, _D, _D, _D, _D, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_int, _I]
+        R.ref_parse.argtypes = [ctypes.c_char_p]
+        R.ref_get_float.restype = ctypes.c_double
+        R.ref_get_float.argtypes = [ctypes.c_char_p, ctypes.c_double]
+        R.ref_get_int.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        R.ref_get_string.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        R.ref_get_vec3.argtypes = [ctypes.c_char_p, _D, _D]
+        _ref = R
+    return _ref
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def corotation(lam, mu, F, dF, which="oracle"):
+    """R, S, L, P, energy, dP(dF) of one F by the oracle or by the reference."""
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    dF = np.ascontiguousarray(dF, dtype=np.float64).reshape(9)
+    out = np.zeros(36)
+    e = np.zeros(1)
+    dP = np.zeros(9)
+    if which == "oracle":
+        lib().orc_corotation(lam, mu, dp(F), dp(dF), dp(out), dp(e), dp(dP))
+    else:
+        ref().ref_corotation(lam, mu, dp(F), dp(dF), dp(out), dp(e), dp(dP))
+    o = out.reshape(4, 3, 3)
+    return o[0], o[1], o[2], o[3], e[0], dP.reshape(3, 3)
+
+
+def sphere_rest_mass(radius, mass_dh, density):
+    dims = (ctypes.c_int * 3)()
+    org = np.zeros(3)
+    lib().orc_sphere_rest_dims(radius, mass_dh, dims, dp(org))
+    data = np.zeros(dims[0] * dims[1] * dims[2])
+    lib().orc_sphere_rest_fill(radius, mass_dh, density, dp(data))
+    return tuple(dims), org, data
+
+
+class OracleScene:
+    """A scene of the CPU oracle, built from the same cfg keys as libesf's
+    esf_scene_load_cfg (analytic sphere solids)."""
+
+    def __init__(self, grid, params):
+        self.L = lib()
+        p = Params(**params)
+        self.grid = grid
+        o = np.asarray(grid["origin"], dtype=np.float64)
+        self.h = self.L.orc_create(grid["xres"], grid["yres"], grid["zres"], grid["dh"], dp(o),
+                                   ctypes.byref(p))
+        X, Y, Z = grid["xres"], grid["yres"], grid["zres"]
+        self.nodes = X * Y * Z
+        self.cells = (X - 1) * (Y - 1) * (Z - 1)
+
+    @classmethod
+    def from_cfg_text(cls, text):
+        import sys
+        sys.path.insert(0, os.path.dirname(HERE))
+        from eulersf_amd.cfg import Cfg, params_from_cfg
+        c = Cfg(text=text)
+        grid = {"xres": c.get_int("xRes", 100), "yres": c.get_int("yRes", 100),
+                "zres": c.get_int("zRes", 100), "dh": c.get_float("dh", 0.01),
+                "origin": c.get_vec3("origin", (0, 0, 0))}
+        params = params_from_cfg(c, grid["dh"])
+        s = cls(grid, params)
+        for i in range(c.get_int("number of solids", 1)):
+            k = str(i)
+            density = c.get_float("solid mass density " + k, 1)
+            dims, org, data = sphere_rest_mass(c.get_float("solid sphere radius " + k, 0.1),
+                                               c.get_float("sphere mass dh " + k, params["mass_sample_dh"]),
+                                               density)
+            rot = np.eye(3).reshape(-1)
+            r = c.get_vec3("solid rotation " + k, (0, 0, 0))
+            if any(v != 0 for v in r):
+                raise ValueError("oracle scenes from cfg support zero rotation only")
+            s.add_solid(c.get_float("material lambda " + k, 100), c.get_float("material mu " + k, 100),
+                        density, dims, c.get_float("sphere mass dh " + k, params["mass_sample_dh"]),
+                        org, data, rot, c.get_vec3("solid translation " + k, (0, 0, 0)),
+                        c.get_vec3("solid scaling " + k, (1, 1, 1)),
+                        c.get_vec3("solid initial velocity " + k, (0, 0, 0)))
+        s.L.orc_init(s.h, c.get_float("smoke carpet thickness", 0.06))
+        s.frame_dt = c.get_float("dt", 0.005)
+        return s
+
+    def add_solid(self, lam, mu, density, dims, rdh, rorigin, rest, rot, trans, scale, v0):
+        a = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+        self._keep = [a(rorigin), a(rest), a(rot), a(trans), a(scale), a(v0)]
+        k = self._keep
+        return self.L.orc_add_solid(self.h, lam, mu, density, dims[0], dims[1], dims[2], rdh,
+                                    dp(k[0]), dp(k[1]), dp(k[2]), dp(k[3]), dp(k[4]), dp(k[5]))
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field(self, fid, solid=-1):
+        n = ctypes.c_size_t()
+        p = self.L.orc_field(self.h, fid, solid, ctypes.byref(n))
+        if not p or n.value == 0:
+            return np.zeros(0)
+        dt = _INT_FIELDS.get(fid, np.float64)
+        ct = {np.int32: ctypes.c_int32, np.int8: ctypes.c_int8}.get(dt, ctypes.c_double)
+        arr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(n.value,))
+        return arr.copy()
+
+    def set_field(self, fid, values, solid=-1):
+        n = ctypes.c_size_t()
+        p = self.L.orc_field(self.h, fid, solid, ctypes.byref(n))
+        dt = _INT_FIELDS.get(fid, np.float64)
+        ct = {np.int32: ctypes.c_int32, np.int8: ctypes.c_int8}.get(dt, ctypes.c_double)
+        arr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(n.value,))
+        arr[:] = np.asarray(values, dtype=dt).reshape(-1)
+
+    def boxes(self, solid=0):
+        b = (ctypes.c_int * 12)()
+        self.L.orc_get_boxes(self.h, solid, b)
+        return list(b)
+
+    def num_solid_nodes(self):
+        return self.L.orc_num_solid_dofs(self.h)
+
+    def step(self, alpha=-1.0):
+        st = Stats()
+        rc = self.L.orc_step(self.h, alpha, ctypes.byref(st))
+        return rc, st
+
+    def pressure_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.L.orc_pressure_apply(self.h, dp(x), dp(y))
+        return y
+
+    def pressure_diag(self):
+        d = np.zeros(self.cells)
+        self.L.orc_pressure_diag(self.h, dp(d))
+        return d
+
+    def solid_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.L.orc_solid_apply(self.h, dp(x), dp(y))
+        return y

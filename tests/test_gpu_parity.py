@@ -1,0 +1,118 @@
+"""GPU (libesf.so) against the CPU oracle on the same scenes: per stage and
+per step.  Integer/byte work and reordered-but-identical arithmetic must be
+bit-exact; Krylov outputs must agree to 1e-10 relative (north-star parity
+bar), both sides solving to 1e-13."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests._scenes import ball_cfg, rel_err, solution_order
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(eps=1e-13, max_iterations=20000)
+
+
+@pytest.fixture(scope="module")
+def esf():
+    import eulersf_amd
+    eulersf_amd.load()
+    return eulersf_amd
+
+
+def make_pair(esf, n=16, **kw):
+    text = ball_cfg(n, **kw)
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    return g, o
+
+
+def test_corotation_batch_matches_oracle(esf):
+    rng = np.random.default_rng(1)
+    F = np.concatenate([
+        np.eye(3) + 0.2 * rng.standard_normal((200, 3, 3)),
+        rng.standard_normal((200, 3, 3)),
+        np.eye(3)[None].repeat(4, 0),
+    ])
+    dF = rng.standard_normal(F.shape)
+    lam, mu = 5769.2307692307695, 3846.153846153846
+    R, S, L, P, e, dP = esf.corotation_batch(F, dF, lam, mu)
+    scale = lam + mu
+    for i in range(F.shape[0]):
+        r = O.corotation(lam, mu, F[i], dF[i])
+        assert np.max(np.abs(R[i] - r[0])) < 1e-12
+        assert np.max(np.abs(S[i] - r[1])) < 1e-12 * max(1, np.abs(r[1]).max())
+        assert np.max(np.abs(L[i] - r[2])) < 1e-12 * scale
+        assert np.max(np.abs(P[i] - r[3])) < 1e-12 * scale * max(1, np.abs(F[i]).max())
+        assert abs(e[i] - r[4]) <= 1e-12 * scale * max(1, np.abs(F[i]).max() ** 2)
+        assert np.max(np.abs(dP[i] - r[5])) < 1e-11 * scale * max(1, np.abs(dF[i]).max())
+
+
+def test_scene_init_bit_exact(esf):
+    g, o = make_pair(esf, 16)
+    assert g.num_solid_nodes() == o.num_solid_nodes()
+    assert g.boxes(0) == o.boxes(0)
+    for gf, of in [(esf._abi.S_MASS, O.S_MASS), (esf._abi.S_QFRAC, O.S_QFRAC),
+                   (esf._abi.S_X, O.S_X), (esf._abi.S_U, O.S_U)]:
+        np.testing.assert_array_equal(g.download(gf, 0), o.field(of, 0))
+    np.testing.assert_array_equal(g.download(esf._abi.S_VALID, 0), o.field(O.S_VALID, 0))
+    np.testing.assert_array_equal(g.download(esf._abi.F_SOLID_MASS), o.field(O.SOLID_MASS))
+    np.testing.assert_array_equal(g.download(esf._abi.F_NODE_FRACTION), o.field(O.NODE_FRACTION))
+    np.testing.assert_array_equal(g.download(esf._abi.F_SOLID_INDEX), o.field(O.SOLID_INDEX))
+    np.testing.assert_array_equal(g.download(esf._abi.F_VELOCITY), o.field(O.VELOCITY))
+    np.testing.assert_array_equal(g.download(esf._abi.F_INTENSITY), o.field(O.INTENSITY))
+
+
+def test_forces_and_pressure_operator(esf):
+    g, o = make_pair(esf, 16, velocity=(0.3, -1.0, 0.2))
+    X = Y = Z = 17
+    dt_g = g.CFL(1.8)
+    dt_o = o.L.orc_cfl(o.h, 1.8)
+    assert dt_g == dt_o
+    g.forces()
+    o.L.orc_forces(o.h)
+    o.L.orc_assemble(o.h)
+    # solid force, momentum, damped mass: same sums in the same order
+    np.testing.assert_array_equal(g.download(esf._abi.F_SOLID_FORCE), o.field(O.SOLID_FORCE))
+    np.testing.assert_array_equal(g.download(esf._abi.F_SOLID_MV), o.field(O.SOLID_MV))
+    np.testing.assert_array_equal(g.download(esf._abi.F_DAMPED_MASS), o.field(O.DAMPED_MASS))
+    minv_g = solution_order(np.repeat(g.download(esf._abi.F_MINV), 3), X, Y, Z)
+    np.testing.assert_array_equal(minv_g, o.field(O.MINV))
+    np.testing.assert_array_equal(g.download(esf._abi.F_WORKSPACE), o.field(O.WORKSPACE))
+    # the pressure operator: bitwise the reference's sparse order
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(g.cells)
+    np.testing.assert_array_equal(g.getMatVecMult(x), o.pressure_apply(x))
+    np.testing.assert_array_equal(g.pressureDiag(), o.pressure_diag())
+
+
+def test_coupled_solve_matches_oracle(esf):
+    g, o = make_pair(esf, 16, velocity=(0.3, -1.0, 0.2), **TIGHT)
+    g.CFL(1.8)
+    o.L.orc_cfl(o.h, 1.8)
+    g.forces()
+    o.L.orc_forces(o.h)
+    o.L.orc_assemble(o.h)
+    gs = g.pcrImplicitSolve()
+    st = O.Stats()
+    o.L.orc_solve(o.h, st)
+    assert gs["status"] == 0
+    v_g = g.download(esf._abi.F_VELOCITY)
+    v_o = o.field(O.VELOCITY)
+    assert rel_err(v_g, v_o) < 1e-10
+    np.testing.assert_allclose(g.download(esf._abi.F_SYS_DIAG), o.field(O.SYS_DIAG), rtol=1e-12)
+    np.testing.assert_allclose(g.download(esf._abi.F_DIV_RHS), o.field(O.DIV_RHS), rtol=0, atol=0)
+
+
+def test_steps_match_oracle(esf):
+    g, o = make_pair(esf, 16, velocity=(0.3, -1.0, 0.2), **TIGHT)
+    for k in range(3):
+        gs = g.stepImplicit(1.8)
+        rc, st = o.step(1.8)
+        assert gs["dt"] == st.dt
+        v_g, v_o = g.download(esf._abi.F_VELOCITY), o.field(O.VELOCITY)
+        assert rel_err(v_g, v_o) < 1e-10, k
+        u_g, u_o = g.download(esf._abi.S_U, 0), o.field(O.S_U, 0)
+        assert rel_err(u_g, u_o) < 1e-10, k
+        i_g, i_o = g.download(esf._abi.F_INTENSITY), o.field(O.INTENSITY)
+        assert rel_err(i_g, i_o) < 1e-10, k
