@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Coupled steps/s of the MI355X drop-in on the unit-cube rubber-ball scene.
+
+Workload (the 128^3 configuration): 129^3 nodes, dh = 1/128, one
+corotational sphere (r = 0.15 at (0.5, 0.65, 0.5), E = 1e4, nu = 0.3,
+solid:fluid density 1000:1000), smoke carpet (fluid pool) over the bottom
+30%, gravity 9.8, dt = 1/240, every Krylov solve to 1e-8 relative residual.
+
+A step is one esf_step: CFL, corotational stress, body forces, the three
+Schur-complement solves, solid finalize, mass gathering, advection of
+velocity and intensity, boundary conditions -- with all fields resident in
+HBM.  The scene runs twice: FP64 (the reference's CR, "plain") and mixed
+(FP32 CR inside FP64 iterative refinement).  `value` is the mixed form's
+steps/s; the FP64 rate, the speedup and the per-field difference between
+the two forms are reported beside it.
+
+roofline: the pressure SpMV y = J M^-1 J^T x, timed with HIP events over
+back-to-back launches on the context stream, algorithmic bytes
+w * (2 N_cells + N_interior_nodes).
+
+cpu_baseline: the CPU oracle (C++/OpenMP restatement of the reference step)
+on the same scene, teacher-forced from the GPU's state after warm-up, timed
+for a bounded number of steps on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "coupled steps/s @128^3 mixed vs FP64 at 1e-8 resid; p50/p99 served step latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(esf, text, precision, warmup, steps):
+    g = esf.CoupledIntegrator3D.from_cfg_text(text, precision)
+    g.u0 = g.download(esf._abi.S_U, 0)
+    reports = []
+    for _ in range(warmup):
+        reports.append(g.stepImplicit(-1.0))
+    g.synchronize()
+    g.kernel_timing(reset=True)
+    g.set_kernel_timing(True)
+    lat = []
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        r = g.stepImplicit(-1.0)
+        g.synchronize()
+        lat.append(time.perf_counter() - s0)
+        iters.append(r)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    g.set_kernel_timing(False)
+    return g, wall, np.array(lat), iters, g.kernel_timing(reset=True)
+
+
+def kernel_bytes(cls, n_cells, n_int):
+    """algorithmic HBM bytes per launch of each timed Krylov kernel class:
+    every cell/node stream the kernel must read or write once"""
+    w = 4 if "32" in cls else 8
+    c, i = n_cells, n_int
+    table = {"apply": w * (2 * c + i), "setup_r": w * (5 * c + i), "setup_s": w * (4 * c + i),
+             "update": w * (8 * c + i), "check": w * (2 * c + i),
+             "ir_resid": 8 * c + 4 * c + 8 * c + 8 * i + 8 * c + 4 * c, "dir": w * 7 * c}
+    for k, v in table.items():
+        if cls.endswith(k) or (k == "dir" and cls.startswith("cr_dir")):
+            return v
+    raise KeyError(cls)
+
+
+def dominant_roofline(kt, n_cells, n_int):
+    """the Krylov kernel class with the most device time in the timed steps"""
+    cls = max(kt, key=lambda k: kt[k][0])
+    total, count = kt[cls]
+    if count == 0:
+        return None
+    ms = total / count
+    b = kernel_bytes(cls, n_cells, n_int)
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel_class": cls,
+            "algorithmic_bytes": b, "avg_launch_ms": round(ms, 5), "launches": count,
+            "share_of_timed_kernel_ms": round(total / sum(v[0] for v in kt.values()), 3)}
+
+
+def spmv_roofline(g, n_cells, n_int, fp32, reps):
+    w = 4 if fp32 else 8
+    ms = g.bench_pressure_apply(fp32=fp32, reps=reps)
+    bytes_ = w * (2 * n_cells + n_int)
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "kSpmv<%s,M_APPLY>" % ("float" if fp32 else "double"),
+            "algorithmic_bytes": bytes_, "avg_launch_ms": round(ms, 5)}
+
+
+def cpu_baseline(text, g_fp64, esf, seconds, max_steps):
+    from oracle import oracle as O
+    o = O.OracleScene.from_cfg_text(text)
+    A = esf._abi
+    # teacher forcing: start from the GPU FP64 state after its timed steps
+    o.set_field(O.VELOCITY, g_fp64.download(A.F_VELOCITY))
+    o.set_field(O.INTENSITY, g_fp64.download(A.F_INTENSITY))
+    o.set_field(O.LAMBDA, g_fp64.download(A.F_LAMBDA))
+    for f_o, f_g in ((O.S_U, A.S_U), (O.S_X, A.S_X), (O.S_VELOCITY, A.S_VELOCITY),
+                     (O.S_VEL_WORKSPACE, A.S_VEL_WORKSPACE)):
+        o.set_field(f_o, g_fp64.download(f_g, 0), 0)
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps:
+        o.step(-1.0)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    el = time.perf_counter() - t0
+    cores = O.lib().orc_num_threads()
+    o.close()
+    return {"value": round(n / el, 4), "unit": "coupled steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} oracle step(s) of the same 128^3 scene from the GPU's warm state, "
+                      f"{el:.1f} s wall, OpenMP threads={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--spmv-reps", type=int, default=100)
+    ap.add_argument("--roofline-n", type=int, default=256,
+                    help="grid of the HBM-bound SpMV measurement (0 to skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the coupled step has no multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import eulersf_amd as esf
+    esf.load()
+    from eulersf_amd.cfg import unit_cube_ball_cfg
+    n = args.n
+    text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000)
+    n_cells, n_int = n ** 3, (n - 1) ** 3
+
+    log(f"[bench] FP64 form: {args.warmup} warm-up + {args.steps} timed steps at {n}^3")
+    g64, wall64, lat64, it64, kt64 = run_form(esf, text, esf.ESF_PLAIN_FP64, args.warmup,
+                                              args.steps)
+    log(f"[bench] FP64: {args.steps / wall64:.2f} steps/s")
+    log(f"[bench] mixed form: {args.warmup} warm-up + {args.steps} timed steps")
+    gmx, wallmx, latmx, itmx, ktmx = run_form(esf, text, esf.ESF_MIXED, args.warmup, args.steps)
+    log(f"[bench] mixed: {args.steps / wallmx:.2f} steps/s")
+
+    A = esf._abi
+    g64_u0 = g64.u0
+    errs = {}
+    # max |mixed - fp64| over the field, relative to the field's scale; the
+    # velocity scale is at least g*dt (the velocity one step of gravity adds
+    # before the projection removes it), the displacement scale at least dh
+    dt = 1.0 / 240.0
+    floors = {"velocity": 9.8 * dt, "intensity": 0.0, "displacement": 1.0 / n, "pressure": 0.0}
+    for name, fid, solid in (("velocity", A.F_VELOCITY, -1), ("intensity", A.F_INTENSITY, -1),
+                             ("displacement", A.S_U, 0), ("pressure", A.F_LAMBDA, -1)):
+        a = g64.download(fid, solid)
+        b = gmx.download(fid, solid)
+        if name == "displacement":  # compare the motion, not the placement
+            u0 = g64_u0
+            a, b = a - u0, b - u0
+        scale = max(float(np.max(np.abs(a))), floors[name]) or 1.0
+        errs[name] = {"max_abs": float(np.max(np.abs(a - b))), "scale": scale,
+                      "rel": float(np.max(np.abs(a - b)) / scale)}
+
+    roof64 = spmv_roofline(g64, n_cells, n_int, False, args.spmv_reps)
+    roof32 = spmv_roofline(gmx, n_cells, n_int, True, args.spmv_reps)
+    roof_big = None
+    if args.roofline_n:
+        nb = args.roofline_n
+        log(f"[bench] SpMV roofline at {nb}^3")
+        big = esf.CoupledIntegrator3D.from_cfg_text(unit_cube_ball_cfg(nb, eps=args.eps),
+                                                    esf.ESF_MIXED)
+        big.forces()  # masses -> 1/m
+        roof_big = {"fp64": spmv_roofline(big, nb ** 3, (nb - 1) ** 3, False, args.spmv_reps),
+                    "fp32": spmv_roofline(big, nb ** 3, (nb - 1) ** 3, True, args.spmv_reps)}
+        big.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle baseline (<= {args.cpu_steps} steps / {args.cpu_seconds} s)")
+        cpu = cpu_baseline(text, g64, esf, args.cpu_seconds, args.cpu_steps)
+        log(f"[bench] CPU: {cpu['value']} steps/s on {cpu['cores']} threads")
+
+    def summary(lat, iters):
+        pr = [r["iterations"][1] for r in iters]
+        return {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                "pressure_iterations_mean": float(np.mean(pr)),
+                "solid_iterations_mean": float(np.mean([r["iterations"][0] + r["iterations"][2]
+                                                        for r in iters])),
+                "max_rel_residual": float(max(max(r["rel_residual"]) for r in iters))}
+
+    value = args.steps / wallmx
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "coupled steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wallmx / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Krylov in f64 refinement (f64 elsewhere)",
+        "data": "synthetic scene (analytic sphere), no external data",
+        "config": {"workload": f"unit-cube rubber ball, {n}^3 cells ({n + 1}^3 nodes), "
+                               "sphere r=0.15 E=1e4 nu=0.3, rho 1000:1000, dt=1/240, "
+                               f"solves to {args.eps:g}",
+                   "grid_cells": n_cells, "solid_nodes": g64.num_solid_nodes(),
+                   "parallelism": "single GPU"},
+        "fp64": {"steps_per_s": round(args.steps / wall64, 3),
+                 "ms_per_step": round(wall64 / args.steps * 1e3, 3), **summary(lat64, it64)},
+        "mixed": {"steps_per_s": round(value, 3), **summary(latmx, itmx),
+                  "refinements_mean": float(np.mean([r["refinements"] for r in itmx]))},
+        "mixed_over_fp64": round(value / (args.steps / wall64), 3),
+        "mixed_vs_fp64_max_rel_err": errs,
+        "step_latency_ms": {"p50": round(float(np.percentile(latmx, 50)) * 1e3, 3),
+                            "p99": round(float(np.percentile(latmx, 99)) * 1e3, 3)},
+        "roofline": dominant_roofline(ktmx, n_cells, n_int),
+        "roofline_fp64_form": dominant_roofline(kt64, n_cells, n_int),
+        "kernel_ms_per_step": {"mixed": {k: round(v[0] / args.steps, 4) for k, v in ktmx.items() if v[1]},
+                               "fp64": {k: round(v[0] / args.steps, 4) for k, v in kt64.items() if v[1]}},
+        "spmv_apply_at_workload": {"fp64": roof64, "fp32": roof32},
+        "roofline_hbm_sized": roof_big,
+        "cpu_baseline": cpu,
+    }
+    g64.close()
+    gmx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

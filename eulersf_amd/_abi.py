@@ -48,6 +48,11 @@ S_VALID = 104
 S_QFRAC = 105
 S_VEL_WORKSPACE = 106
 
+KERNEL_CLASSES = ["spmv64_apply", "spmv64_setup_r", "spmv64_setup_s", "spmv64_update",
+                  "spmv64_check", "spmv64_ir_resid", "spmv32_apply", "spmv32_setup_r",
+                  "spmv32_setup_s", "spmv32_update", "spmv32_check", "spmv32_ir_resid",
+                  "cr_dir64", "cr_dir32"]
+
 STAGES = ["cfl", "forces", "solid_solves", "pressure_solve", "solid_finalize", "gather",
           "advect_velocity", "advect_intensity", "recover_bc"]
 
@@ -108,6 +113,7 @@ _PROTOS = {
     "esf_set_dt": (_S, [_P, ctypes.c_double]),
     "esf_get_dt": (_S, [_P, _D]),
     "esf_num_solid_nodes": (_S, [_P, _I]),
+    "esf_synchronize": (_S, [_P]),
     "esf_solid_add": (_S, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                           ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _I]),
     "esf_solid_set_boxes": (_S, [_P, ctypes.c_int, _I]),
@@ -133,13 +139,16 @@ _PROTOS = {
     "esf_pressure_solve": (_S, [_P, _D, _D, ctypes.c_double, ctypes.c_int, _I, _D]),
     "esf_corotation_batch": (_S, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                                  _D, _D]),
+    "esf_bench_pressure_apply": (_S, [_P, ctypes.c_int, ctypes.c_int, _D]),
     "esf_set_stage_timing": (_S, [_P, ctypes.c_int]),
     "esf_stage_times": (_S, [_P, _D, ctypes.c_int, ctypes.c_int]),
-    "esf_spmv_timing": (_S, [_P, _D, ctypes.POINTER(ctypes.c_long), ctypes.c_int]),
-    "esf_set_spmv_timing": (_S, [_P, ctypes.c_int]),
+    "esf_set_kernel_timing": (_S, [_P, ctypes.c_int]),
+    "esf_kernel_timing": (_S, [_P, _D, ctypes.POINTER(ctypes.c_long), ctypes.c_int]),
     "esf_scene_load_cfg": (_S, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P),
                                ctypes.POINTER(SceneInfo)]),
     "esf_frame": (_S, [_P, ctypes.c_double, ctypes.c_double, _I]),
+    "esf_cfg_read": (_S, [ctypes.c_char_p, ctypes.POINTER(Grid), ctypes.POINTER(Params),
+                         ctypes.POINTER(SceneInfo)]),
 }
 
 _lib = None

@@ -104,6 +104,11 @@ def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
     dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool."""
     lam, mu = lame_from_young(E, nu)
     dh = 1.0 / n
+    # The reference drops mass samples lighter than 1e-6 (absolute,
+    # DEFOR_SOLID_3D.cpp:629-630): density * sdh^3 must stay above it, so
+    # the sample spacing is dh/8 but never finer than 1/512 (the shipped
+    # cfgs use 0.00125).  Both are exact binary fractions for n = 2^k.
+    sdh = max(dh / 8, 1.0 / 512)
     lines = [
         "# unit-cube rubber ball, %d^3 cells" % n,
         "xRes = %d" % (n + 1), "yRes = %d" % (n + 1), "zRes = %d" % (n + 1),
@@ -111,7 +116,7 @@ def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
         "number of solids = 1",
         "solid path 0 = ./", "solid name 0 = analytic_sphere",
         "solid sphere radius 0 = %r" % radius,
-        "sphere mass dh 0 = %r" % (dh / 4),
+        "sphere mass dh 0 = %r" % sdh,
         "solid mass density 0 = %r" % density,
         "material lambda 0 = %r" % lam, "material mu 0 = %r" % mu,
         "solid plastic yield 0 = -1",
@@ -126,7 +131,7 @@ def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
         "rayleigh alpha = 0", "rayleigh beta = 0",
         "cr max iterations = %d" % max_iterations,
         "implicit solver eps = %r" % eps, "div free solver eps = %r" % eps,
-        "mass sample dh = %r" % (dh / 8),
+        "mass sample dh = %r" % sdh,
         "start frame = 0", "end frame = %d" % frames,
         "save data = 0",
     ]

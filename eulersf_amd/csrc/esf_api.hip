@@ -18,6 +18,7 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
                       double* relres, int* refinements, int* innerIters);
 int spmvBlocks(const Dims& d);
+esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs);
 // esf_solid.hip
 esf_status stressStage(esf_ctx* ctx, double* energy);
 esf_status solidAdiag(esf_ctx* ctx);
@@ -183,8 +184,6 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
     return ESF_ERR_DEVICE;
   }
   for (int i = 0; i < 24; i++) hipEventCreate(&ctx->ev[i]);
-  hipEventCreate(&ctx->spmvEv[0]);
-  hipEventCreate(&ctx->spmvEv[1]);
   esf_status st = ESF_OK;
   const long N = d.nodes, C = d.cells;
   const bool mixed = p->precision == ESF_MIXED;
@@ -237,8 +236,10 @@ void esf_ctx_destroy(esf_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   freeAll(ctx);
   for (int i = 0; i < 24; i++) hipEventDestroy(ctx->ev[i]);
-  hipEventDestroy(ctx->spmvEv[0]);
-  hipEventDestroy(ctx->spmvEv[1]);
+  for (auto& t : ctx->timed) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -683,15 +684,15 @@ esf_status esf_stage_times(esf_ctx* ctx, double* ms, int n, int reset) {
   return ESF_OK;
 }
 
-esf_status esf_spmv_timing(esf_ctx* ctx, double* avg, long* launches, int reset) {
+esf_status esf_synchronize(esf_ctx* ctx) {
   if (!ctx) return ESF_ERR_INVALID;
-  if (avg) *avg = ctx->spmvLaunches ? ctx->spmvMs / ctx->spmvLaunches : 0;
-  if (launches) *launches = ctx->spmvLaunches;
-  if (reset) {
-    ctx->spmvMs = 0;
-    ctx->spmvLaunches = 0;
-  }
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
   return ESF_OK;
+}
+
+esf_status esf_bench_pressure_apply(esf_ctx* ctx, int fp32, int reps, double* avg_ms) {
+  if (!ctx || !avg_ms || reps < 1) return ESF_ERR_INVALID;
+  return benchPressureApply(ctx, fp32, reps, avg_ms);
 }
 
 esf_status esf_set_stage_timing(esf_ctx* ctx, int on) {
@@ -700,9 +701,32 @@ esf_status esf_set_stage_timing(esf_ctx* ctx, int on) {
   return ESF_OK;
 }
 
-esf_status esf_set_spmv_timing(esf_ctx* ctx, int on) {
+esf_status esf_set_kernel_timing(esf_ctx* ctx, int on) {
   if (!ctx) return ESF_ERR_INVALID;
-  ctx->spmvTiming = on != 0;
+  ctx->kernelTiming = on != 0;
+  return ESF_OK;
+}
+
+esf_status esf_kernel_timing(esf_ctx* ctx, double* total_ms, long* counts, int reset) {
+  if (!ctx) return ESF_ERR_INVALID;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < ctx->timedUsed; i++) {
+    float ms = 0;
+    const auto& t = ctx->timed[i];
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      ctx->kernelMs[t.cls] += ms;
+      ctx->kernelCount[t.cls]++;
+    }
+  }
+  ctx->timedUsed = 0;
+  for (int c = 0; c < 16; c++) {
+    if (total_ms) total_ms[c] = ctx->kernelMs[c];
+    if (counts) counts[c] = ctx->kernelCount[c];
+    if (reset) {
+      ctx->kernelMs[c] = 0;
+      ctx->kernelCount[c] = 0;
+    }
+  }
   return ESF_OK;
 }
 

@@ -113,11 +113,18 @@ struct esf_ctx {
   // timing
   hipEvent_t ev[24];
   double stageMs[16] = {0};
-  bool spmvTiming = false;
   bool stageTiming = false;
-  hipEvent_t spmvEv[2];
-  double spmvMs = 0;
-  long spmvLaunches = 0;
+  // kernel timing: event pairs recorded around Krylov kernel launches
+  // without synchronising, read back by esf_kernel_timing()
+  bool kernelTiming = false;
+  struct TimedLaunch {
+    hipEvent_t a, b;
+    int cls;
+  };
+  std::vector<TimedLaunch> timed;
+  size_t timedUsed = 0;
+  double kernelMs[16] = {0};
+  long kernelCount[16] = {0};
 };
 
 namespace esf {
@@ -194,5 +201,24 @@ __device__ inline bool lastBlockReduce(const double (&mine)[K], double* partials
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// kernel classes for esf_kernel_timing: SpMV modes 0-5 (FP64), 6-11 (FP32),
+// CR direction 12 (FP64) / 13 (FP32)
+inline esf_ctx::TimedLaunch* timedBegin(esf_ctx* ctx, int cls) {
+  if (!ctx->kernelTiming) return nullptr;
+  if (ctx->timedUsed == ctx->timed.size()) {
+    esf_ctx::TimedLaunch t{};
+    hipEventCreate(&t.a);
+    hipEventCreate(&t.b);
+    ctx->timed.push_back(t);
+  }
+  esf_ctx::TimedLaunch* t = &ctx->timed[ctx->timedUsed++];
+  t->cls = cls;
+  hipEventRecord(t->a, ctx->stream);
+  return t;
+}
+inline void timedEnd(esf_ctx* ctx, esf_ctx::TimedLaunch* t) {
+  if (t) hipEventRecord(t->b, ctx->stream);
+}
 
 }  // namespace esf

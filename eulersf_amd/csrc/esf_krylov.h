@@ -37,14 +37,15 @@ __global__ void __launch_bounds__(256) kCrDirection(long n, T* p, T* Ap, const T
 }  // namespace
 
 // Poll the device "done" flag of scalar block `which` without stalling the
-// queue: every `chunk` iterations an async copy of the flag lands in a
-// pinned ring slot behind an event; the host waits only on the slot
-// `lookahead` chunks back.
+// queue: after each convergence check an async copy of the flag lands in a
+// pinned ring slot behind an event; the host keeps one check interval
+// queued ahead and waits on the previous one, so the device never idles
+// and at most one interval of early-exiting launches follows convergence.
 struct DonePoller {
   esf_ctx* ctx;
   int which;
   static constexpr int kRing = 8;
-  static constexpr int kLook = 3;
+  static constexpr int kLook = 1;
   int issued = 0, waited = 0;
   hipEvent_t ev[kRing];
   explicit DonePoller(esf_ctx* c, int w) : ctx(c), which(w) {

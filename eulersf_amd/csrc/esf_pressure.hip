@@ -283,18 +283,10 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   int tx, ty, tz;
   tileShape(ctx->d, tx, ty, tz);
   const int nb = tx * ty * tz;
-  const bool timed = (MODE == M_APPLY || MODE == M_UPDATE) && ctx->spmvTiming;
-  if (timed) hipEventRecord(ctx->spmvEv[0], ctx->stream);
+  auto* tl = timedBegin(ctx, MODE + (sizeof(T) == 4 ? 6 : 0));
   hipLaunchKernelGGL((kSpmv<T, MODE>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d, a,
                      tx, ty, tz);
-  if (timed) {
-    hipEventRecord(ctx->spmvEv[1], ctx->stream);
-    hipEventSynchronize(ctx->spmvEv[1]);
-    float ms = 0;
-    hipEventElapsedTime(&ms, ctx->spmvEv[0], ctx->spmvEv[1]);
-    ctx->spmvMs += ms;
-    ctx->spmvLaunches++;
-  }
+  timedEnd(ctx, tl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("kSpmv launch: ") + hipGetErrorString(e);
@@ -315,6 +307,47 @@ esf_status pressureRhs(esf_ctx* ctx) {
                      ctx->v0, ctx->vel, ctx->minv, ctx->c, ctx->b, ctx->D,
                      ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr);
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+__global__ void kToFloat(long n, const double* a, float* b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+// back-to-back applies between two events on the context stream: the
+// average launch duration of the pressure SpMV on the current masses
+esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs) {
+  const Dims& d = ctx->d;
+  if (fp32 && !ctx->minv32) {
+    ctx->err = "FP32 apply needs a mixed-precision context";
+    return ESF_ERR_INVALID;
+  }
+  esf_status st = ESF_OK;
+  if (fp32)
+    hipLaunchKernelGGL(kToFloat, dim3(1024), dim3(256), 0, ctx->stream, d.nodes,
+                       (const double*)ctx->minv, ctx->minv32);
+  hipEventRecord(ctx->ev[2], ctx->stream);
+  for (int i = 0; i < reps && !st; i++) {
+    if (fp32) {
+      SpmvArgs<float> a{};
+      a.in = ctx->e32;
+      a.minv = ctx->minv32;
+      a.out = ctx->s32;
+      st = launchSpmv<float, M_APPLY>(ctx, a);
+    } else {
+      SpmvArgs<double> a{};
+      a.in = ctx->lambda;
+      a.minv = ctx->minv;
+      a.out = ctx->s;
+      st = launchSpmv<double, M_APPLY>(ctx, a);
+    }
+  }
+  hipEventRecord(ctx->ev[3], ctx->stream);
+  if (hipEventSynchronize(ctx->ev[3]) != hipSuccess) return ESF_ERR_DEVICE;
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+  *avgMs = reps > 0 ? ms / reps : 0;
+  return st;
 }
 
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y) {
@@ -379,9 +412,11 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     a.in = rA;
     a.r = rB;
     if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
+    auto* tl = timedBegin(ctx, sizeof(T) == 4 ? 13 : 12);
     hipLaunchKernelGGL(kCrDirection<T>, dim3(dirBlocks), dim3(256), 0, ctx->stream,
                        (long)d.cells, p, Ap, (const T*)rB, (const T*)s, D, a.sc,
                        ctx->partials, &ctx->counter[which]);
+    timedEnd(ctx, tl);
     std::swap(rA, rB);
     if (i > 0 && i % 5 == 0) {
       a.in = x;
@@ -404,11 +439,6 @@ template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*,
                                    const float*, float*, float*, float*, float*, float*,
                                    float*, double, int, int*, bool*, double*);
 
-__global__ void kToFloat(long n, const double* a, float* b) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
-}
 __global__ void kZeroF(long n, float* a) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)

@@ -220,6 +220,30 @@ void initialState(int X, int Y, int Z, double dh, const double o[3], const doubl
 
 using namespace esf;
 
+extern "C" esf_status esf_cfg_read(const char* path, esf_grid* g, esf_params* p,
+                                   esf_scene_info* info) {
+  if (!path || !g || !p) return ESF_ERR_INVALID;
+  Cfg cfg;
+  std::string err;
+  if (!cfg.parse(path, &err)) return ESF_ERR_IO;
+  g->xres = cfg.getInt("xRes", 100);
+  g->yres = cfg.getInt("yRes", 100);
+  g->zres = cfg.getInt("zRes", 100);
+  g->dh = cfg.getFloat("dh", 0.01);
+  const double zero[3] = {0, 0, 0};
+  cfg.getVec3("origin", zero, g->origin);
+  paramsFromCfg(cfg, g->dh, p);
+  p->precision = ESF_PLAIN_FP64;
+  if (info) {
+    info->n_solids = cfg.getInt("number of solids", 1);
+    info->frame_start = cfg.getInt("start frame", 0);
+    info->frame_end = cfg.getInt("end frame", 100);
+    info->frame_dt = cfg.getFloat("dt", 0.005);
+    info->carpet_thickness = cfg.getFloat("smoke carpet thickness", 0.06);
+  }
+  return ESF_OK;
+}
+
 extern "C" esf_status esf_scene_load_cfg(const char* path, int precision, esf_ctx** out,
                                          esf_scene_info* info) {
   if (!path || !out) return ESF_ERR_INVALID;

@@ -112,6 +112,9 @@ class CoupledIntegrator3D:
         check(self.ctx, self.lib.esf_upload(self.ctx, field, solid, a.ctypes.data_as(ctypes.c_void_p),
                                             a.nbytes), f"upload {field}")
 
+    def synchronize(self):
+        check(self.ctx, self.lib.esf_synchronize(self.ctx))
+
     def num_solid_nodes(self):
         n = ctypes.c_int()
         check(self.ctx, self.lib.esf_num_solid_nodes(self.ctx, ctypes.byref(n)))
@@ -223,6 +226,12 @@ class CoupledIntegrator3D:
         return x, it.value, rr.value, st
 
     # -- timing -------------------------------------------------------------
+    def bench_pressure_apply(self, fp32=False, reps=50):
+        v = ctypes.c_double()
+        check(self.ctx, self.lib.esf_bench_pressure_apply(self.ctx, 1 if fp32 else 0, reps,
+                                                          ctypes.byref(v)), "bench apply")
+        return v.value
+
     def set_stage_timing(self, on=True):
         check(self.ctx, self.lib.esf_set_stage_timing(self.ctx, 1 if on else 0))
 
@@ -231,15 +240,15 @@ class CoupledIntegrator3D:
         check(self.ctx, self.lib.esf_stage_times(self.ctx, dptr(a), 16, 1 if reset else 0))
         return dict(zip(_abi.STAGES, a[:len(_abi.STAGES)]))
 
-    def set_spmv_timing(self, on=True):
-        check(self.ctx, self.lib.esf_set_spmv_timing(self.ctx, 1 if on else 0))
+    def set_kernel_timing(self, on=True):
+        check(self.ctx, self.lib.esf_set_kernel_timing(self.ctx, 1 if on else 0))
 
-    def spmv_timing(self, reset=False):
-        avg = ctypes.c_double()
-        n = ctypes.c_long()
-        check(self.ctx, self.lib.esf_spmv_timing(self.ctx, ctypes.byref(avg), ctypes.byref(n),
-                                                 1 if reset else 0))
-        return avg.value, n.value
+    def kernel_timing(self, reset=False):
+        """{class: (total_ms, launches)} for the Krylov kernels timed so far."""
+        ms = np.zeros(16)
+        n = (ctypes.c_long * 16)()
+        check(self.ctx, self.lib.esf_kernel_timing(self.ctx, dptr(ms), n, 1 if reset else 0))
+        return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(_abi.KERNEL_CLASSES)}
 
 
 def corotation_batch(F, dF, lam, mu):

@@ -136,6 +136,8 @@ esf_status esf_set_max_dt(esf_ctx* ctx, double max_dt);
 esf_status esf_set_dt(esf_ctx* ctx, double dt);
 esf_status esf_get_dt(esf_ctx* ctx, double* dt);
 esf_status esf_num_solid_nodes(esf_ctx* ctx, int* n);
+/* wait for all work queued on the context's stream */
+esf_status esf_synchronize(esf_ctx* ctx);
 
 /* --- solids -------------------------------------------------------------*/
 /* DEFOR_SOLID_3D ctor + setMaterial + setMassDensity
@@ -223,12 +225,21 @@ esf_status esf_set_stage_timing(esf_ctx* ctx, int on);
  * 3 pressure solve, 4 solid finalize, 5 gather, 6 advect velocity,
  * 7 advect intensity, 8 recover+bc */
 esf_status esf_stage_times(esf_ctx* ctx, double* ms, int n, int reset);
-/* average duration (ms) of the pressure SpMV kernel over the launches timed
- * since the last reset, and the launch count */
-esf_status esf_spmv_timing(esf_ctx* ctx, double* avg_ms, long* launches,
-                           int reset);
-/* enable HIP-event timing around every pressure SpMV launch (costly) */
-esf_status esf_set_spmv_timing(esf_ctx* ctx, int on);
+/* measurement entry point, no reference counterpart: `reps` back-to-back
+ * launches of the pressure SpMV (FP64, or FP32 on a mixed context) on the
+ * current masses, bracketed by HIP events on the context stream; *avg_ms is
+ * the average launch duration */
+esf_status esf_bench_pressure_apply(esf_ctx* ctx, int fp32, int reps,
+                                    double* avg_ms);
+/* Record HIP events around every Krylov kernel launch (no synchronisation;
+ * events are read back by esf_kernel_timing). */
+esf_status esf_set_kernel_timing(esf_ctx* ctx, int on);
+/* Accumulated device time (ms) and launch count per kernel class since the
+ * last reset: 0-5 pressure SpMV FP64 (apply, setup-r, setup-s, CR update,
+ * residual check, refinement residual), 6-11 the same in FP32, 12/13 the CR
+ * direction update FP64/FP32.  Arrays of 16. */
+esf_status esf_kernel_timing(esf_ctx* ctx, double* total_ms, long* counts,
+                             int reset);
 
 /* --- host scene setup (cfg-driven, rubberBall/main.cpp:63-160) ------------*/
 typedef struct {
@@ -237,6 +248,11 @@ typedef struct {
   double frame_dt;
   double carpet_thickness;
 } esf_scene_info;
+/* Read a reference cfg (SIMPLE_PARSER semantics, src/util/SIMPLE_PARSER.cpp:
+ * 64-236) into the grid and parameters the context would get, with the
+ * reference's defaults for absent keys.  Host only: needs no GPU. */
+esf_status esf_cfg_read(const char* cfg_path, esf_grid* grid,
+                        esf_params* params, esf_scene_info* info);
 /* Parse a reference cfg (SIMPLE_PARSER semantics), create the context,
  * add the solids, run COUPLED_INTEGRATOR_3D::init and lay the smoke carpet.
  * A solid's rest mass is read from "<solid path i><solid name i>.mass"

@@ -1702,6 +1702,19 @@ int orc_pressure_cr(void* h, const double* b, double* x, double eps, int max_it,
   return c ? 0 : 1;
 }
 
+void orc_apply_J(void* h, const double* v, double* out) {
+  Scene& s = *static_cast<Scene*>(h);
+  vector<double> vv(v, v + 3 * s.totalActive), o;
+  applyJ(s, vv, o);
+  std::memcpy(out, o.data(), sizeof(double) * o.size());
+}
+void orc_apply_negJT(void* h, const double* lam, double* out) {
+  Scene& s = *static_cast<Scene*>(h);
+  vector<double> l(lam, lam + s.nCells), o;
+  applyNegJT(s, l, o);
+  std::memcpy(out, o.data(), sizeof(double) * o.size());
+}
+
 int orc_pcr_dense(int n, const double* A, const double* diag, const double* b,
                   double* x, double eps, int max_it, int* iters) {
   vector<double> d(diag, diag + n), bb(b, b + n), xx(x, x + n);
@@ -1746,6 +1759,14 @@ double orc_last_stage_seconds(void* h, int stage) {
   Scene& s = *static_cast<Scene*>(h);
   if (stage < 0 || stage >= 16) return 0;
   return s.stageSec[stage];
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 void orc_sphere_rest_dims(double radius, double mdh, int dims[3], double origin[3]) {

@@ -128,6 +128,9 @@ void orc_solid_apply(void* h, const double* x, double* y);      /* A x (CSR) */
 int orc_pressure_cr(void* h, const double* b, double* x, double eps,
                     int max_it, int* iters);
 
+/* J v (v in solution order, 3*interior) -> cells, and -J^T lam -> 3*interior */
+void orc_apply_J(void* h, const double* v, double* out);
+void orc_apply_negJT(void* h, const double* lam, double* out);
 /* the oracle's CR on a dense operator with an explicit Jacobi diagonal;
  * x is the start vector (warm start) */
 int orc_pcr_dense(int n, const double* A, const double* diag, const double* b,
@@ -144,6 +147,7 @@ double orc_interp_scalar(const double* f, int X, int Y, int Z, double dh,
 void orc_interp_vec3(const double* f, int X, int Y, int Z, double dh,
                      const double o[3], const double p[3], double out[3]);
 double orc_last_stage_seconds(void* h, int stage);
+int orc_num_threads(void);
 /* rest-space mass field of a sphere of `radius` centred at the origin,
  * rasterized as projects/rasterizeSolid/main.cpp:42-72 does with the SDF
  * |p| - radius: dims and origin first, then the data (X*Y*Z doubles) */

@@ -80,6 +80,8 @@ def lib():
         L.orc_pressure_diag.argtypes = [ctypes.c_void_p, _D]
         L.orc_solid_apply.argtypes = [ctypes.c_void_p, _D, _D]
         L.orc_pressure_cr.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_double, ctypes.c_int, _I]
+        L.orc_apply_J.argtypes = [ctypes.c_void_p, _D, _D]
+        L.orc_apply_negJT.argtypes = [ctypes.c_void_p, _D, _D]
         L.orc_pcr_dense.argtypes = [ctypes.c_int, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int, _I]
         L.orc_corotation.argtypes = [ctypes.c_double, ctypes.c_double, _D, _D, _D, _D, _D]
         L.orc_interp_scalar.restype = ctypes.c_double
@@ -87,6 +89,7 @@ def lib():
         L.orc_interp_vec3.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D]
         L.orc_last_stage_seconds.restype = ctypes.c_double
         L.orc_last_stage_seconds.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_num_threads.restype = ctypes.c_int
         L.orc_sphere_rest_dims.argtypes = [ctypes.c_double, ctypes.c_double, _I, _D]
         L.orc_sphere_rest_fill.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]
         _lib = L
@@ -254,6 +257,19 @@ class OracleScene:
         d = np.zeros(self.cells)
         self.L.orc_pressure_diag(self.h, dp(d))
         return d
+
+    def apply_J(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.cells)
+        self.L.orc_apply_J(self.h, dp(v), dp(out))
+        return out
+
+    def apply_negJT(self, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        X, Y, Z = self.grid["xres"], self.grid["yres"], self.grid["zres"]
+        out = np.zeros(3 * (X - 2) * (Y - 2) * (Z - 2))
+        self.L.orc_apply_negJT(self.h, dp(lam), dp(out))
+        return out
 
     def solid_apply(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
