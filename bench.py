@@ -131,6 +131,20 @@ def cpu_baseline(text, g_fp64, esf, seconds, max_steps):
                       f"{el:.1f} s wall, OpenMP threads={cores}"}
 
 
+def spmv_only(esf, cfg, n, reps):
+    g = esf.CoupledIntegrator3D.from_cfg_text(cfg(n), esf.ESF_MIXED)
+    g.forces()
+    r64 = spmv_roofline(g, n ** 3, (n - 1) ** 3, False, reps)
+    r32 = spmv_roofline(g, n ** 3, (n - 1) ** 3, True, reps)
+    nbytes = r64["algorithmic_bytes"] // 2
+    ms = g.bench_copy(nbytes, reps)
+    copy = {"bytes_read_plus_written": 2 * nbytes, "avg_launch_ms": round(ms, 5),
+            "achieved": round(2 * nbytes / (ms * 1e-3) / 1e9, 1), "unit": "GB/s"}
+    print(json.dumps({"workload": f"pressure SpMV, {n}^3 cells", "fp64": r64, "fp32": r32,
+                      "copy": copy}), flush=True)
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +158,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-only", type=int, default=0, metavar="N",
+                    help="only time the pressure SpMV (FP64, FP32) on the N^3 scene and a "
+                         "streaming copy of the same byte count; prints one JSON line")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -154,6 +171,9 @@ def main():
     import eulersf_amd as esf
     esf.load()
     from eulersf_amd.cfg import unit_cube_ball_cfg
+    if args.spmv_only:
+        spmv_only(esf, unit_cube_ball_cfg, args.spmv_only, args.spmv_reps)
+        return
     n = args.n
     text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000)
     n_cells, n_int = n ** 3, (n - 1) ** 3

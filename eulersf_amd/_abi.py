@@ -140,6 +140,7 @@ _PROTOS = {
     "esf_corotation_batch": (_S, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                                  _D, _D]),
     "esf_bench_pressure_apply": (_S, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    "esf_bench_copy": (_S, [_P, ctypes.c_long, ctypes.c_int, _D]),
     "esf_set_stage_timing": (_S, [_P, ctypes.c_int]),
     "esf_stage_times": (_S, [_P, _D, ctypes.c_int, ctypes.c_int]),
     "esf_set_kernel_timing": (_S, [_P, ctypes.c_int]),

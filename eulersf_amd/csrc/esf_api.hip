@@ -19,6 +19,7 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
                       double* relres, int* refinements, int* innerIters);
 int spmvBlocks(const Dims& d);
 esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs);
+esf_status benchCopy(esf_ctx* ctx, long bytes, int reps, double* avgMs);
 // esf_solid.hip
 esf_status stressStage(esf_ctx* ctx, double* energy);
 esf_status solidAdiag(esf_ctx* ctx);
@@ -693,6 +694,11 @@ esf_status esf_synchronize(esf_ctx* ctx) {
 esf_status esf_bench_pressure_apply(esf_ctx* ctx, int fp32, int reps, double* avg_ms) {
   if (!ctx || !avg_ms || reps < 1) return ESF_ERR_INVALID;
   return benchPressureApply(ctx, fp32, reps, avg_ms);
+}
+
+esf_status esf_bench_copy(esf_ctx* ctx, long bytes, int reps, double* avg_ms) {
+  if (!ctx || !avg_ms || reps < 1 || bytes < 16) return ESF_ERR_INVALID;
+  return benchCopy(ctx, bytes, reps, avg_ms);
 }
 
 esf_status esf_set_stage_timing(esf_ctx* ctx, int on) {

@@ -184,11 +184,12 @@ __device__ inline bool lastBlockReduce(const double (&mine)[K], double* partials
   }
   __syncthreads();
   if (!isLast) return false;
+  // acquire: the agent-scope fence invalidates this CU's L1, so plain
+  // (independent, in-flight together) loads see every block's partial
   __threadfence();
   for (int k = 0; k < K; k++) {
     double acc = 0;
-    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
-      acc += loadAcq(&partials[(size_t)k * nb + i]);
+    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[(size_t)k * nb + i];
     double t = blockSum(acc, lds);
     __shared__ double bc;
     if (threadIdx.x == 0) bc = t;

@@ -51,14 +51,29 @@ struct SpmvArgs {
   int first;         // M_IR_RESID: also compute |b|
 };
 
-constexpr int TX = 64, TY = 4, TZ = 8;
+constexpr int TX = 32, TY = 8, TZ = 8;
+constexpr int HALO = (TX + 2) * (TY + 2);  // cells of one layer incl. halo
+constexpr int NODES = (TX + 1) * (TY + 1); // nodes of one layer
+constexpr int HPT = (HALO + 255) / 256;    // halo cells per thread
+constexpr int NPT = (NODES + 255) / 256;   // nodes per thread
+
+// raw global values of one halo cell, fetched a layer ahead and turned
+// into the LDS value when stored (so the loads overlap the previous
+// layer's arithmetic)
+template <typename T>
+struct Fetched {
+  T in, ap, d, x, p;
+  float e;
+  bool valid, own;
+  long c;
+};
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
                                              int tilesY, int tilesZ) {
   if (MODE != M_APPLY && a.sc && a.sc->done) return;
-  __shared__ T xs[2][(TY + 2) * (TX + 2)];
-  __shared__ T ws[2][(TY + 1) * (TX + 1) * 3];
+  __shared__ T xs[2][HALO];
+  __shared__ T ws[2][NODES * 3];
   __shared__ double red[8];
 
   // XCD-aware tile order: blocks are dealt round-robin to the 8 XCDs, so
@@ -76,44 +91,87 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
 
   double part0 = 0, part1 = 0;
 
-  auto loadLayer = [&](int cz) {
-    T* L = xs[cz & 1];
-    for (int t = threadIdx.x; t < (TY + 2) * (TX + 2); t += 256) {
+  Fetched<T> f[HPT];
+  auto fetch = [&](int cz) {
+#pragma unroll
+    for (int h = 0; h < HPT; h++) {
+      const int t = threadIdx.x + 256 * h;
+      Fetched<T>& q = f[h];
+      q.valid = false;
+      if (t >= HALO) continue;
       const int lx = t % (TX + 2), ly = t / (TX + 2);
       const int cx = x0 - 1 + lx, cy = y0 - 1 + ly;
+      if (cx < 0 || cx >= d.CX || cy < 0 || cy >= d.CY || cz < 0 || cz >= d.CZ) continue;
+      q.valid = true;
+      q.c = cx + (long)cy * d.CX + (long)cz * d.cslab;
+      q.own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && cz >= z0 && cz < zEnd;
+      q.in = a.in[q.c];
+      if (MODE == M_UPDATE) {
+        q.ap = a.Ap[q.c];
+        q.d = a.D[q.c];
+        if (q.own) {
+          q.x = a.x[q.c];
+          q.p = a.p[q.c];
+        }
+      } else if (MODE == M_IR_RESID) {
+        if (a.e32) q.e = a.e32[q.c];
+      }
+    }
+  };
+  auto store = [&](int cz) {
+    T* L = xs[cz & 1];
+#pragma unroll
+    for (int h = 0; h < HPT; h++) {
+      const int t = threadIdx.x + 256 * h;
+      if (t >= HALO) continue;
+      const Fetched<T>& q = f[h];
       T v = 0;
-      if (cx >= 0 && cx < d.CX && cy >= 0 && cy < d.CY && cz >= 0 && cz < d.CZ) {
-        const long c = cx + (long)cy * d.CX + (long)cz * d.cslab;
-        const bool own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY &&
-                         cz >= z0 && cz < zEnd;
+      if (q.valid) {
         if (MODE == M_UPDATE) {
-          // r_old is read on the halo, r_new written to the other buffer
-          const T q = a.Ap[c] / a.D[c];
-          v = a.in[c] - alpha * q;
-          if (own) {
-            a.r[c] = v;
-            a.x[c] = a.x[c] + alpha * a.p[c];
+          // r_old read on the halo, r_new written to the other buffer
+          const T qq = q.ap / q.d;
+          v = q.in - alpha * qq;
+          if (q.own) {
+            a.r[q.c] = v;
+            a.x[q.c] = q.x + alpha * q.p;
           }
         } else if (MODE == M_IR_RESID) {
           // x_old (+ e) read on the halo, x_new written to the other buffer
-          v = a.in[c];
+          v = q.in;
           if (a.e32) {
-            v = v + (T)a.e32[c];
-            if (own) a.x[c] = v;
+            v = v + (T)q.e;
+            if (q.own) a.x[q.c] = v;
           }
         } else {
-          v = a.in[c];
+          v = q.in;
         }
       }
       L[t] = v;
     }
   };
 
-  auto nodeLayer = [&](int k) {
+  // minv of the next node layer is loaded into one of two register sets
+  // while the other feeds the current layer (selected by layer parity)
+  T mvA[NPT], mvB[NPT];
+  auto fetchMinv = [&](int k, T* mv) {
+#pragma unroll
+    for (int h = 0; h < NPT; h++) {
+      const int t = threadIdx.x + 256 * h;
+      mv[h] = 0;
+      if (t >= NODES) continue;
+      const int i = x0 + t % (TX + 1), j = y0 + t / (TX + 1);
+      if (i >= 1 && i <= d.X - 2 && j >= 1 && j <= d.Y - 2 && k >= 1 && k <= d.Z - 2)
+        mv[h] = a.minv[i + (long)j * d.X + (long)k * d.slab];
+    }
+  };
+  auto nodeLayer = [&](int k, const T* mv) {
     T* W = ws[k & 1];
     const T* L0 = xs[(k - 1) & 1];
     const T* L1 = xs[k & 1];
-    for (int t = threadIdx.x; t < (TY + 1) * (TX + 1); t += 256) {
+#pragma unroll
+    for (int h = 0; h < NPT; h++) {
+      const int t = threadIdx.x + 256 * h;
+      if (t >= NODES) continue;
       const int lx = t % (TX + 1), ly = t / (TX + 1);
       const int i = x0 + lx, j = y0 + ly;
       T w0 = 0, w1 = 0, w2 = 0;
@@ -133,7 +191,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
               m2 += dz == 0 ? v : -v;
             }
         }
-        const T mi = a.minv[i + (long)j * d.X + (long)k * d.slab];
+        const T mi = mv[h];
         w0 = m0 * mi;
         w1 = m1 * mi;
         w2 = m2 * mi;
@@ -144,17 +202,30 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
     }
   };
 
-  loadLayer(z0 - 1);
-  loadLayer(z0);
+  fetch(z0 - 1);
+  store(z0 - 1);
+  fetch(z0);
+  fetchMinv(z0, mvA);
+  store(z0);
   __syncthreads();
-  nodeLayer(z0);
+  nodeLayer(z0, mvA);
+  fetch(z0 + 1);
+  fetchMinv(z0 + 1, mvB);  // layer z0 + 1 + 2m lives in mvB, z0 + 2 + 2m in mvA
   __syncthreads();
   const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
   const int cx = x0 + lx, cy = y0 + ly;
   for (int cz = z0; cz < zEnd; cz++) {
-    loadLayer(cz + 1);
+    // this cell's own input, read before the barrier after which faster
+    // threads may already overwrite layer cz's buffer with layer cz + 2
+    const T ownIn = xs[cz & 1][(ly + 1) * (TX + 2) + lx + 1];
+    store(cz + 1);
+    const bool odd = ((cz - z0) & 1) != 0;
+    if (cz + 2 <= zEnd) {  // next layer's loads in flight during this layer
+      fetch(cz + 2);
+      fetchMinv(cz + 2, odd ? mvB : mvA);
+    }
     __syncthreads();
-    nodeLayer(cz + 1);
+    nodeLayer(cz + 1, odd ? mvA : mvB);
     __syncthreads();
     if (cx < d.CX && cy < d.CY) {
       T acc = 0;
@@ -170,7 +241,6 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
         acc += oz ? W[2] : -W[2];
       }
       const long c = cx + (long)cy * d.CX + (long)cz * d.cslab;
-      const T ownIn = xs[cz & 1][(ly + 1) * (TX + 2) + lx + 1];
       if (MODE == M_APPLY) {
         a.out[c] = acc;
       } else if (MODE == M_SETUP_R) {
@@ -314,6 +384,39 @@ __global__ void kToFloat(long n, const double* a, float* b) {
        i += (long)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
 }
+// streaming copy of n doubles (calibration of the HBM byte counters and
+// the box's copy bandwidth): 16-byte vector loads and stores
+__global__ void __launch_bounds__(256) kCopy(long n2, const double2* __restrict__ a,
+                                             double2* __restrict__ b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2;
+       i += (long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+esf_status benchCopy(esf_ctx* ctx, long bytes, int reps, double* avgMs) {
+  const long n2 = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n2 * 16) != hipSuccess) return ESF_ERR_OOM;
+  if (hipMalloc(&b, n2 * 16) != hipSuccess) {
+    hipFree(a);
+    return ESF_ERR_OOM;
+  }
+  hipMemsetAsync(a, 0, n2 * 16, ctx->stream);
+  const int blocks = 256 * 32;
+  hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, ctx->stream, n2, (const double2*)a, b);
+  hipEventRecord(ctx->ev[2], ctx->stream);
+  for (int i = 0; i < reps; i++)
+    hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, ctx->stream, n2, (const double2*)a, b);
+  hipEventRecord(ctx->ev[3], ctx->stream);
+  hipError_t e = hipEventSynchronize(ctx->ev[3]);
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+  *avgMs = ms / reps;
+  hipFree(a);
+  hipFree(b);
+  return e == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
 // back-to-back applies between two events on the context stream: the
 // average launch duration of the pressure SpMV on the current masses
 esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs) {

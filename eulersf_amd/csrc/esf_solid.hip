@@ -26,7 +26,11 @@ __device__ __forceinline__ int sgn(int bit) { return bit ? 1 : -1; }
 
 // kStress: per valid cell slot.  out 24 forces, 24 diag entries, 28 doubles
 // of (R, L, F, Jinv).  energy partial per block.
-__global__ void __launch_bounds__(256) kStress(Dims d, int nValid, const int* __restrict__ cellList,
+// Per-cell arrays are structure-of-arrays with stride `cap` (the slot
+// capacity): component k of slot e lives at [k * cap + e], so a wave reads
+// 64 consecutive slots of one component in one coalesced access.
+__global__ void __launch_bounds__(256) kStress(Dims d, int nValid, int cap,
+                                               const int* __restrict__ cellList,
                                                const double* __restrict__ u,
                                                const double* __restrict__ mass,
                                                const int* __restrict__ sidx, double lambda,
@@ -79,7 +83,7 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, const int* __
       for (int c = 0; c < 3; c++)
         GF[3 * i + c] = (G0 * F[c] + G1 * F[3 + c]) + G2 * F[6 + c];
     }
-    double* out = cellOut + 24 * (long)e;
+    double* out = cellOut + e;
     const double dh = d.dh;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -87,12 +91,12 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, const int* __
 #pragma unroll
       for (int r = 0; r < 3; r++) {
         const double t = (cs[3 * r] * G0 + cs[3 * r + 1] * G1) + cs[3 * r + 2] * G2;
-        out[3 * i + r] = 0.0 - ((t * dh) * dh) * dh;  // nodalForcesFEM (:801)
+        out[(long)(3 * i + r) * cap] = 0.0 - ((t * dh) * dh) * dh;  // nodalForcesFEM (:801)
       }
     }
     en = ((corotEnergy(S, lambda, mu) * dh) * dh) * dh;
     // diagonal of K_e over unmasked DOFs (:805-827, :834-846)
-    double* dg = cellDiag + 24 * (long)e;
+    double* dg = cellDiag + e;
     for (int i = 0; i < 8; i++) {
       const long n = index + off[i];
       const bool keep = mass[n] != 0 && sidx[n] >= 0;
@@ -107,16 +111,16 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, const int* __
                      dP[3 * j + 2] * GF[3 * i + 2];
           val = (((t * Jinv) * dh) * dh) * dh;
         }
-        dg[3 * i + j] = val;
+        dg[(long)(3 * i + j) * cap] = val;
       }
     }
-    double* m = cellMat + 28 * (long)e;
+    double* m = cellMat + e;
     for (int k = 0; k < 9; k++) {
-      m[k] = R[k];
-      m[9 + k] = L[k];
-      m[18 + k] = F[k];
+      m[(long)k * cap] = R[k];
+      m[(long)(9 + k) * cap] = L[k];
+      m[(long)(18 + k) * cap] = F[k];
     }
-    m[27] = Jinv;
+    m[27L * cap] = Jinv;
   }
   const double tot = blockSum(en, red);
   if (threadIdx.x == 0) energyPart[blockIdx.x] = tot;
@@ -126,7 +130,7 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, const int* __
 // from adjacent valid cells (ascending cell order), and computeMV
 // (DEFOR_SOLID_3D.cpp:873-889).  `first` initialises (solid 0), later
 // solids accumulate, matching `force += localForce` per solid (:865).
-__global__ void kSolidGather(Dims d, int nSolid, const int* __restrict__ solidNodes,
+__global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict__ solidNodes,
                              const int* __restrict__ cellSlot, const double* __restrict__ cellOut,
                              const double* __restrict__ cellDiag,
                              const double* __restrict__ smass, const double* __restrict__ svel,
@@ -145,10 +149,10 @@ __global__ void kSolidGather(Dims d, int nSolid, const int* __restrict__ solidNo
         const int slot = cellSlot[cell];
         if (slot < 0) continue;
         const int corner = (1 - dx) + 2 * (1 - dy) + 4 * (1 - dz);
-        for (int r = 0; r < 3; r++) f[r] += cellOut[24 * (long)slot + 3 * corner + r];
+        for (int r = 0; r < 3; r++) f[r] += cellOut[(long)(3 * corner + r) * cap + slot];
         if (rowKeep) {
           for (int r = 0; r < 3; r++) {
-            const double v = cellDiag[24 * (long)slot + 3 * corner + r];
+            const double v = cellDiag[(long)(3 * corner + r) * cap + slot];
             kd[r] = nk == 0 ? v : kd[r] + v;
           }
           nk++;
@@ -172,7 +176,8 @@ __global__ void kSolidGather(Dims d, int nSolid, const int* __restrict__ solidNo
 }
 
 // K_e x_e per valid cell (see file comment)
-__global__ void __launch_bounds__(256) kSolidCellApply(Dims d, int nValid, const int* __restrict__ cellList,
+__global__ void __launch_bounds__(256) kSolidCellApply(Dims d, int nValid, int cap,
+                                                       const int* __restrict__ cellList,
                                                        const double* __restrict__ cellMat,
                                                        const double* __restrict__ smass,
                                                        const int* __restrict__ sidx,
@@ -182,14 +187,14 @@ __global__ void __launch_bounds__(256) kSolidCellApply(Dims d, int nValid, const
   if (e >= nValid) return;
   const long index = cellList[e];
   const long off[8] = {0, 1, d.X, d.X + 1, d.slab, d.slab + 1, d.slab + d.X, d.slab + d.X + 1};
-  const double* m = cellMat + 28 * (long)e;
+  const double* m = cellMat + e;
   double R[9], L[9], F[9];
   for (int k = 0; k < 9; k++) {
-    R[k] = m[k];
-    L[k] = m[9 + k];
-    F[k] = m[18 + k];
+    R[k] = m[(long)k * cap];
+    L[k] = m[(long)(9 + k) * cap];
+    F[k] = m[(long)(18 + k) * cap];
   }
-  const double Jinv = m[27];
+  const double Jinv = m[27L * cap];
   const double g = 0.25 * d.dhInv;
   double GF[24];
   for (int i = 0; i < 8; i++) {
@@ -211,7 +216,7 @@ __global__ void __launch_bounds__(256) kSolidCellApply(Dims d, int nValid, const
   double dP[9];
   corotDP(R, L, lambda, mu, dF, dP);
   const double dh = d.dh;
-  double* out = cellOut + 24 * (long)e;
+  double* out = cellOut + e;
   for (int i = 0; i < 8; i++)
     for (int r = 0; r < 3; r++) {
       double t = 0;
@@ -219,11 +224,11 @@ __global__ void __launch_bounds__(256) kSolidCellApply(Dims d, int nValid, const
         t = (dP[3 * r] * GF[3 * i] + dP[3 * r + 1] * GF[3 * i + 1]) + dP[3 * r + 2] * GF[3 * i + 2];
         t = (((t * Jinv) * dh) * dh) * dh;
       }
-      out[3 * i + r] = t;
+      out[(long)(3 * i + r) * cap] = t;
     }
 }
 
-enum SolidMode { S_APPLY = 0, S_SETUP_S = 1, S_UPDATE = 2, S_CHECK = 3 };
+enum SolidMode { S_APPLY = 0, S_SETUP_S = 1, S_CHECK = 3 };
 
 struct SolidNodeArgs {
   int nSolid;
@@ -232,6 +237,7 @@ struct SolidNodeArgs {
   // up to 4 solids gathered in one launch
   const int* cellSlot[4];
   const double* cellOut[4];
+  int cap[4];
   const double* smass[4];
   double scale;          // dt^2 + beta dt
   const double* damped;  // diag mass
@@ -268,7 +274,8 @@ __global__ void __launch_bounds__(256) kSolidNodeApply(Dims d, SolidNodeArgs a) 
             const int slot = a.cellSlot[b][cell];
             if (slot < 0) continue;
             const int corner = (1 - dx) + 2 * (1 - dy) + 4 * (1 - dz);
-            for (int r = 0; r < 3; r++) kx[r] += a.cellOut[b][24 * (long)slot + 3 * corner + r];
+            for (int r = 0; r < 3; r++)
+              kx[r] += a.cellOut[b][(long)(3 * corner + r) * a.cap[b] + slot];
           }
     }
     for (int r = 0; r < 3; r++) {
@@ -283,9 +290,6 @@ __global__ void __launch_bounds__(256) kSolidNodeApply(Dims d, SolidNodeArgs a) 
         p0 += xv * yv;
         const double q = yv / a.diag[k];
         p1 += yv * q;
-      } else if (MODE == S_UPDATE) {
-        a.s[k] = yv;
-        p0 += xv * yv;
       } else if (MODE == S_CHECK) {
         const double t = yv - a.b[k];
         p0 += t * t;
@@ -299,10 +303,6 @@ __global__ void __launch_bounds__(256) kSolidNodeApply(Dims d, SolidNodeArgs a) 
     if (MODE == S_SETUP_S) {
       sc->delta = tot[0];
       sc->alpha = (fabs(tot[1]) > 0.0) ? tot[0] / tot[1] : tot[1];
-    } else if (MODE == S_UPDATE) {
-      const double dOld = sc->delta;
-      sc->delta = tot[0];
-      sc->beta = tot[0] / dOld;
     } else if (MODE == S_CHECK) {
       const double rn = sqrt(tot[0]);
       sc->resnorm = rn;
@@ -314,36 +314,7 @@ __global__ void __launch_bounds__(256) kSolidNodeApply(Dims d, SolidNodeArgs a) 
   }
 }
 
-// x = 0, r = b / diag, p = r, |b|  (CONJUGATE_RESIDUAL.inl:197-205, :218)
-__global__ void kSolidSetup(long n, const double* b, const double* diag, double* x, double* r,
-                            double* p, CrScalars* sc, double* partials, unsigned* counter) {
-  __shared__ double red[8];
-  double part = 0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const double rv = b[i] / diag[i];
-    x[i] = 0.0;
-    r[i] = rv;
-    p[i] = rv;
-    part += b[i] * b[i];
-  }
-  double mine[1] = {part}, tot[1];
-  if (lastBlockReduce<1>(mine, partials, counter, red, tot) && threadIdx.x == 0)
-    sc->bnorm = sqrt(tot[0]);
-}
 
-// x += alpha p ; r -= alpha Ap / diag   (CONJUGATE_RESIDUAL.inl:223-230)
-__global__ void kSolidUpdate(long n, double* x, double* r, const double* p, const double* Ap,
-                             const double* diag, const CrScalars* sc) {
-  if (sc->done) return;
-  const double alpha = sc->alpha;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const double q = Ap[i] / diag[i];
-    x[i] = x[i] + alpha * p[i];
-    r[i] = r[i] - alpha * q;
-  }
-}
 
 // diag(A) = scale * diag(K) + M~   (COUPLED_INTEGRATOR_3D.cpp:613-617)
 __global__ void kSolidDiag(long n, const double* kdiag, const double* damped, double scale,
@@ -363,7 +334,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
     DevSolid& so = ctx->solids[b];
     const int nb = cdiv(std::max(so.nValid, 1), 256);
     if (so.nValid > 0)
-      hipLaunchKernelGGL(kStress, dim3(nb), dim3(256), 0, ctx->stream, d, so.nValid,
+      hipLaunchKernelGGL(kStress, dim3(nb), dim3(256), 0, ctx->stream, d, so.nValid, so.capValid,
                          so.cellList, so.u, so.mass, ctx->sidx, so.lambda, so.mu, so.cellMat,
                          so.cellOut, so.cellDiag, ctx->partials);
     if (energy && so.nValid > 0) {
@@ -375,7 +346,8 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
     }
     if (ctx->nSolid > 0)
       hipLaunchKernelGGL(kSolidGather, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0,
-                         ctx->stream, d, ctx->nSolid, ctx->solidNodes, so.cellSlot, so.cellOut,
+                         ctx->stream, d, ctx->nSolid, so.capValid, ctx->solidNodes, so.cellSlot,
+                         so.cellOut,
                          so.cellDiag, so.mass, so.vel, so.box, b == 0 ? 1 : 0, ctx->sForce,
                          ctx->sKdiag, ctx->sMv);
   }
@@ -395,6 +367,7 @@ static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
   for (int b = 0; b < a.nSolidBodies; b++) {
     a.cellSlot[b] = ctx->solids[b].cellSlot;
     a.cellOut[b] = ctx->solids[b].cellOut;
+    a.cap[b] = ctx->solids[b].capValid;
     a.smass[b] = ctx->solids[b].mass;
   }
   a.scale = ctx->dt * ctx->dt + ctx->p.rayleigh_beta * ctx->dt;
@@ -406,7 +379,8 @@ static void cellApply(esf_ctx* ctx, const double* x) {
   for (auto& so : ctx->solids)
     if (so.nValid > 0)
       hipLaunchKernelGGL(kSolidCellApply, dim3(cdiv(so.nValid, 256)), dim3(256), 0,
-                         ctx->stream, ctx->d, so.nValid, so.cellList, so.cellMat, so.mass,
+                         ctx->stream, ctx->d, so.nValid, so.capValid, so.cellList, so.cellMat,
+                         so.mass,
                          ctx->sidx, x, so.lambda, so.mu, so.cellOut);
 }
 
@@ -440,7 +414,143 @@ esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
 }
 
 // CONJUGATE_RESIDUAL::solvePCR(SpMat&, b, x)  (CONJUGATE_RESIDUAL.inl:194-276)
-// with the matrix-free A; x starts at zero.  Scalar block `which`.
+// with the matrix-free A; x starts at zero.  Three launches per iteration:
+//   cells : K_e r_new per valid cell, r_new = r - alpha D^-1 Ap formed at the
+//           corners (the update of .inl:223-230 folded into the apply)
+//   nodes : x += alpha p, r = r_new, s = A r_new, partial r.s -> beta
+//   dofs  : p = beta p + r, Ap = beta Ap + s, partial Ap.D^-1 Ap -> alpha
+// plus the true-residual check every 5 iterations (.inl:246-255).
+__global__ void kSolidSetup(long n, const double* b, const double* diag, double* x, double* r,
+                            double* p, CrScalars* sc, double* partials, unsigned* counter) {
+  __shared__ double red[8];
+  double part = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double rv = b[i] / diag[i];
+    x[i] = 0.0;
+    r[i] = rv;
+    p[i] = rv;
+    part += b[i] * b[i];
+  }
+  double mine[1] = {part}, tot[1];
+  if (lastBlockReduce<1>(mine, partials, counter, red, tot) && threadIdx.x == 0)
+    sc->bnorm = sqrt(tot[0]);
+}
+
+// cell phase of the CR update: v = r - alpha Ap / diag at unmasked corners
+__global__ void __launch_bounds__(256) kSolidCellUpdate(Dims d, int nValid, int cap,
+                                                        const int* __restrict__ cellList,
+                                                        const double* __restrict__ cellMat,
+                                                        const double* __restrict__ smass,
+                                                        const int* __restrict__ sidx,
+                                                        const double* __restrict__ r,
+                                                        const double* __restrict__ Ap,
+                                                        const double* __restrict__ diag,
+                                                        const CrScalars* sc, double lambda,
+                                                        double mu, double* cellOut) {
+  if (sc->done) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nValid) return;
+  const double alpha = sc->alpha;
+  const long index = cellList[e];
+  const long off[8] = {0, 1, d.X, d.X + 1, d.slab, d.slab + 1, d.slab + d.X, d.slab + d.X + 1};
+  const double* m = cellMat + e;
+  double R[9], L[9], F[9];
+  for (int k = 0; k < 9; k++) {
+    R[k] = m[(long)k * cap];
+    L[k] = m[(long)(9 + k) * cap];
+    F[k] = m[(long)(18 + k) * cap];
+  }
+  const double Jinv = m[27L * cap];
+  const double g = 0.25 * d.dhInv, dh = d.dh;
+  double GF[24];
+  for (int i = 0; i < 8; i++) {
+    const double G0 = sgn(i & 1) * g, G1 = sgn((i >> 1) & 1) * g, G2 = sgn(i >> 2) * g;
+    for (int c = 0; c < 3; c++) GF[3 * i + c] = (G0 * F[c] + G1 * F[3 + c]) + G2 * F[6 + c];
+  }
+  double dF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned keep = 0;
+  for (int i = 0; i < 8; i++) {
+    const long n = index + off[i];
+    const int si = sidx[n];
+    if (!(smass[n] != 0 && si >= 0)) continue;
+    keep |= 1u << i;
+    for (int j = 0; j < 3; j++) {
+      const long k = 3L * si + j;
+      const double q = Ap[k] / diag[k];
+      const double xv = r[k] - alpha * q;
+      for (int c = 0; c < 3; c++) dF[3 * j + c] += xv * GF[3 * i + c];
+    }
+  }
+  double dP[9];
+  corotDP(R, L, lambda, mu, dF, dP);
+  double* out = cellOut + e;
+  for (int i = 0; i < 8; i++)
+    for (int rr = 0; rr < 3; rr++) {
+      double t = 0;
+      if (keep & (1u << i)) {
+        t = (dP[3 * rr] * GF[3 * i] + dP[3 * rr + 1] * GF[3 * i + 1]) + dP[3 * rr + 2] * GF[3 * i + 2];
+        t = (((t * Jinv) * dh) * dh) * dh;
+      }
+      out[(long)(3 * i + rr) * cap] = t;
+    }
+}
+
+// node phase of the CR update: x += alpha p, r = r_new, s = A r_new, r.s
+__global__ void __launch_bounds__(256) kSolidNodeUpdate(Dims d, SolidNodeArgs a, double* x,
+                                                        double* r, const double* p) {
+  if (a.sc->done) return;
+  __shared__ double red[8];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const double alpha = a.sc->alpha;
+  double prs = 0;
+  if (s < a.nSolid) {
+    const long n = a.solidNodes[s];
+    const int x0 = n % d.X, y0 = (n / d.X) % d.Y, z0 = n / d.slab;
+    double kx[3] = {0, 0, 0};
+    for (int b = 0; b < a.nSolidBodies; b++) {
+      if (a.smass[b][n] == 0) continue;
+      for (int dz = 0; dz < 2; dz++)
+        for (int dy = 0; dy < 2; dy++)
+          for (int dx = 0; dx < 2; dx++) {
+            const long cell =
+                (x0 - 1 + dx) + (long)(y0 - 1 + dy) * d.CX + (long)(z0 - 1 + dz) * d.cslab;
+            const int slot = a.cellSlot[b][cell];
+            if (slot < 0) continue;
+            const int corner = (1 - dx) + 2 * (1 - dy) + 4 * (1 - dz);
+            for (int c = 0; c < 3; c++)
+              kx[c] += a.cellOut[b][(long)(3 * corner + c) * a.cap[b] + slot];
+          }
+    }
+    for (int c = 0; c < 3; c++) {
+      const long k = 3L * s + c;
+      const double q = a.Ap[k] / a.diag[k];
+      const double rn = r[k] - alpha * q;
+      x[k] = x[k] + alpha * p[k];
+      const double y = kx[c] * a.scale + a.damped[k] * rn;
+      r[k] = rn;
+      a.s[k] = y;
+      prs += rn * y;
+    }
+  }
+  double mine[1] = {prs}, tot[1];
+  if (lastBlockReduce<1>(mine, a.partials, a.counter, red, tot) && threadIdx.x == 0) {
+    CrScalars* sc = a.sc;
+    const double dOld = sc->delta;
+    sc->delta = tot[0];
+    sc->beta = tot[0] / dOld;
+  }
+}
+
+static void cellUpdate(esf_ctx* ctx, const CrScalars* sc) {
+  for (auto& so : ctx->solids)
+    if (so.nValid > 0)
+      hipLaunchKernelGGL(kSolidCellUpdate, dim3(cdiv(so.nValid, 256)), dim3(256), 0, ctx->stream,
+                         ctx->d, so.nValid, so.capValid, so.cellList, so.cellMat, so.mass,
+                         ctx->sidx, (const double*)ctx->sr, (const double*)ctx->sAp,
+                         (const double*)ctx->sAdiag, sc, so.lambda, so.mu, so.cellOut);
+}
+
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps,
                       int maxIt, int* iters, bool* converged, double* relres) {
   if (ctx->solids.size() > 4) {
@@ -460,9 +570,11 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   ESF_HIP(hipMemcpyAsync(&ctx->sc[which], &h0, sizeof(h0), hipMemcpyHostToDevice, ctx->stream));
   CrScalars* sc = &ctx->sc[which];
   unsigned* cnt = &ctx->counter[which];
-  const int eb = std::min(cdiv(n, 256), 1024);
-  hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b, ctx->sAdiag, x,
-                     ctx->sr, ctx->sp, sc, ctx->partials, cnt);
+  // few blocks for the DOF-parallel reductions: ~4 elements per thread
+  const int eb = std::max(1, std::min(cdiv(n, 1024), 256));
+  const int nb = cdiv(ctx->nSolid, 256);
+  hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
+                     (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, sc, ctx->partials, cnt);
   SolidNodeArgs a{};
   fillNodeArgs(ctx, a);
   a.b = b;
@@ -474,22 +586,18 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   cellApply(ctx, ctx->sr);
   a.in = ctx->sr;
   nodeApply<S_SETUP_S>(ctx, a);
-  int i = 0;
   DonePoller poll(ctx, which);
-  for (; i < maxIt; i++) {
-    hipLaunchKernelGGL(kSolidUpdate, dim3(eb), dim3(256), 0, ctx->stream, n, x, ctx->sr,
-                       (const double*)ctx->sp, (const double*)ctx->sAp,
-                       (const double*)ctx->sAdiag, (const CrScalars*)sc);
-    cellApply(ctx, ctx->sr);
-    a.in = ctx->sr;
-    a.iter = i;
-    nodeApply<S_UPDATE>(ctx, a);
+  for (int i = 0; i < maxIt; i++) {
+    cellUpdate(ctx, sc);
+    hipLaunchKernelGGL(kSolidNodeUpdate, dim3(nb), dim3(256), 0, ctx->stream, ctx->d, a, x,
+                       ctx->sr, (const double*)ctx->sp);
     hipLaunchKernelGGL(kCrDirection<double>, dim3(eb), dim3(256), 0, ctx->stream, n, ctx->sp,
                        ctx->sAp, (const double*)ctx->sr, (const double*)ctx->ss,
                        (const double*)ctx->sAdiag, sc, ctx->partials, cnt);
     if (i > 0 && i % 5 == 0) {
       cellApply(ctx, x);
       a.in = x;
+      a.iter = i;
       nodeApply<S_CHECK>(ctx, a);
       if (poll.mark()) break;
     }

@@ -240,6 +240,11 @@ class CoupledIntegrator3D:
         check(self.ctx, self.lib.esf_stage_times(self.ctx, dptr(a), 16, 1 if reset else 0))
         return dict(zip(_abi.STAGES, a[:len(_abi.STAGES)]))
 
+    def bench_copy(self, nbytes, reps=20):
+        v = ctypes.c_double()
+        check(self.ctx, self.lib.esf_bench_copy(self.ctx, nbytes, reps, ctypes.byref(v)), "bench copy")
+        return v.value
+
     def set_kernel_timing(self, on=True):
         check(self.ctx, self.lib.esf_set_kernel_timing(self.ctx, 1 if on else 0))
 

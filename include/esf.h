@@ -231,6 +231,10 @@ esf_status esf_stage_times(esf_ctx* ctx, double* ms, int n, int reset);
  * the average launch duration */
 esf_status esf_bench_pressure_apply(esf_ctx* ctx, int fp32, int reps,
                                     double* avg_ms);
+/* measurement entry point, no reference counterpart: a streaming copy of
+ * `bytes` between two device buffers, `reps` launches timed with HIP events
+ * (reads + writes = 2 * bytes per launch) */
+esf_status esf_bench_copy(esf_ctx* ctx, long bytes, int reps, double* avg_ms);
 /* Record HIP events around every Krylov kernel launch (no synchronisation;
  * events are read back by esf_kernel_timing). */
 esf_status esf_set_kernel_timing(esf_ctx* ctx, int on);
