@@ -51,7 +51,7 @@ S_VEL_WORKSPACE = 106
 KERNEL_CLASSES = ["spmv64_apply", "spmv64_setup_r", "spmv64_setup_s", "spmv64_update",
                   "spmv64_check", "spmv64_ir_resid", "spmv32_apply", "spmv32_setup_r",
                   "spmv32_setup_s", "spmv32_update", "spmv32_check", "spmv32_ir_resid",
-                  "cr_dir64", "cr_dir32"]
+                  "cr_dir64", "cr_dir32", "solid_cell_cr", "solid_node_cr"]
 
 STAGES = ["cfl", "forces", "solid_solves", "pressure_solve", "solid_finalize", "gather",
           "advect_velocity", "advect_intensity", "recover_bc"]
@@ -77,13 +77,13 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int * 3), ("converged", ctypes.c_int * 3),
                 ("rel_residual", ctypes.c_double * 3), ("refinements", ctypes.c_int),
                 ("inner_iterations", ctypes.c_int), ("dt", ctypes.c_double),
-                ("n_solid_nodes", ctypes.c_int)]
+                ("n_solid_nodes", ctypes.c_int), ("n_solid_cells", ctypes.c_int)]
 
     def as_dict(self):
         return {"iterations": list(self.iterations), "converged": list(self.converged),
                 "rel_residual": list(self.rel_residual), "refinements": self.refinements,
                 "inner_iterations": self.inner_iterations, "dt": self.dt,
-                "n_solid_nodes": self.n_solid_nodes}
+                "n_solid_nodes": self.n_solid_nodes, "n_solid_cells": self.n_solid_cells}
 
 
 class SceneInfo(ctypes.Structure):

@@ -76,6 +76,7 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   }
   esf_status st = dalloc(ctx, &ctx->solidNodes, cap);
   if (st) return st;
+  if ((st = dalloc(ctx, &ctx->nodeSlot, 4 * 8 * cap))) return st;
   ctx->capSolid = (int)cap;
   return ESF_OK;
 }
@@ -88,6 +89,8 @@ esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need) {
   if ((st = dalloc(ctx, &so.cellMat, 28 * cap))) return st;
   if ((st = dalloc(ctx, &so.cellOut, 24 * cap))) return st;
   if ((st = dalloc(ctx, &so.cellDiag, 24 * cap))) return st;
+  if ((st = dalloc(ctx, &so.cornerDof, 8 * cap))) return st;
+  if ((st = dalloc(ctx, &so.cellKx, 24 * cap))) return st;
   so.capValid = (int)cap;
   return ESF_OK;
 }
@@ -102,14 +105,14 @@ static void freeAll(esf_ctx* ctx) {
                   ctx->s32,    ctx->Ap32,   ctx->rhs32,   ctx->sForce,    ctx->sMv,
                   ctx->sDamped, ctx->sKdiag, ctx->sAdiag, ctx->sRhs,      ctx->sV0,
                   ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
-                  ctx->sAp,    ctx->stmp,   ctx->partials, ctx->counter,  ctx->sc,
-                  ctx->dmisc};
+                  ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
+                  ctx->dmisc,  ctx->nodeSlot};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& so : ctx->solids) {
     void* sp[] = {so.rest, so.u,       so.unew,     so.X,       so.vel,     so.velws,
                   so.mass, so.detFInv, so.qfrac,    so.valid,   so.cellList, so.cellSlot,
-                  so.cellMat, so.cellOut, so.cellDiag};
+                  so.cellMat, so.cellOut, so.cellDiag, so.cornerDof, so.cellKx};
     for (void* p : sp)
       if (p) hipFree(p);
   }
@@ -210,10 +213,10 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = dalloc(ctx, &ctx->sidx, N);
   const long scanBlocks = std::max<long>(N, C) / 1024 + 2;
   if (!st) st = dalloc(ctx, &ctx->scanBlocks, scanBlocks);
-  // partials: enough for the largest reducing grid, 2 values per block
-  ctx->nPartials = std::max<int>(4 * std::max(spmvBlocks(d), cdiv(N, 256)) + 64, 1 << 16);
+  // 4 partials per block of the widest reducing launch (solid node pass:
+  // 128-thread blocks over at most every node)
+  ctx->nPartials = std::max<int>(4 * std::max(spmvBlocks(d), cdiv(N, 128)) + 64, 1 << 16);
   if (!st) st = dalloc(ctx, &ctx->partials, ctx->nPartials);
-  if (!st) st = dalloc(ctx, &ctx->counter, 8);
   if (!st) st = dalloc(ctx, &ctx->sc, 8);
   if (!st) st = dalloc(ctx, &ctx->dmisc, 16);
   if (!st) st = growSolidDofs(ctx, 0);
@@ -544,6 +547,7 @@ esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats) {
   for (int k = 0; k < 3; k++) s.converged[k] = conv[k];
   s.dt = ctx->dt;
   s.n_solid_nodes = ctx->nSolid;
+  for (auto& so : ctx->solids) s.n_solid_cells += so.nValid;
   if (stats) *stats = s;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

@@ -65,6 +65,8 @@ struct DevSolid {
   double* cellMat = nullptr; // per slot: R(9) L(9) F(9) Jinv(1)
   double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
+  int* cornerDof = nullptr;  // per slot and corner: 3 * solid index, or -1 if masked
+  double* cellKx = nullptr;  // per slot: K_e v, 24 contiguous doubles
   bool listDirty = true;
 };
 
@@ -84,6 +86,7 @@ struct esf_ctx {
   int* sidx = nullptr;        // nodes -> solid DOF node index or -1
   int* solidNodes = nullptr;  // solid DOF node -> node index
   int nSolid = 0, capSolid = 0;
+  int* nodeSlot = nullptr;    // per solid (<= 4) x 8 cells x capSolid: K gather table
   int* scanBlocks = nullptr;  // scan scratch
   // solve: node-based
   double *v0 = nullptr, *minv = nullptr;  // 3*nodes, nodes
@@ -103,7 +106,6 @@ struct esf_ctx {
          *stmp = nullptr;
   // reductions
   double* partials = nullptr;  // per-block partial sums
-  unsigned* counter = nullptr; // last-block counters
   esf::CrScalars* sc = nullptr;   // device scalars (several solves)
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
   int* hostFlags = nullptr;       // pinned ring of done flags
@@ -158,53 +160,39 @@ __device__ inline double blockSum(double v, double* lds) {
   return r;
 }
 
-__device__ inline double loadAcq(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Each block writes K partials; the block that finishes last sums them in
-// block order and returns true in every thread of that block with the K
-// totals in `tot` (valid in all threads).  The counter is reset for reuse.
+// Grid-wide sums.  Each block of a reducing kernel writes its K block sums
+// with plain stores, block-major ([k * nb + block]); a one-block finisher
+// kernel launched next on the same stream adds them in block order (fixed
+// order: deterministic for a fixed launch shape).  The kernel boundary
+// orders the partials for the finisher.  A last-block reduction inside the
+// producer would need an agent-scope release fence per block, which MI355X
+// serialises at ~20 ns per block (1024 blocks: ~22 us per reduction).
 template <int K>
-__device__ inline bool lastBlockReduce(const double (&mine)[K], double* partials,
-                                       unsigned* counter, double* lds,
-                                       double (&tot)[K]) {
+__device__ inline void blockPartials(const double (&mine)[K], double* partials, double* lds) {
   const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   for (int k = 0; k < K; k++) {
-    double s = blockSum(mine[k], lds);
+    const double s = blockSum(mine[k], lds);
     if (threadIdx.x == 0) partials[(size_t)k * nb + bid] = s;
   }
-  __shared__ unsigned isLast;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    isLast = (prev == nb - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!isLast) return false;
-  // acquire: the agent-scope fence invalidates this CU's L1, so plain
-  // (independent, in-flight together) loads see every block's partial
-  __threadfence();
+}
+
+// in the finisher (one block, up to 1024 threads): the K totals of the
+// producer's nb blocks, valid in thread 0
+template <int K>
+__device__ inline void finishSums(int nb, const double* partials, double* lds,
+                                  double (&tot)[K]) {
   for (int k = 0; k < K; k++) {
     double acc = 0;
-    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[(size_t)k * nb + i];
-    double t = blockSum(acc, lds);
-    __shared__ double bc;
-    if (threadIdx.x == 0) bc = t;
-    __syncthreads();
-    tot[k] = bc;
-    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[(size_t)k * nb + i];
+    tot[k] = blockSum(acc, lds);
   }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // kernel classes for esf_kernel_timing: SpMV modes 0-5 (FP64), 6-11 (FP32),
-// CR direction 12 (FP64) / 13 (FP32)
+// CR direction 12 (FP64) / 13 (FP32), solid cell pass 14, solid node pass 15
 inline esf_ctx::TimedLaunch* timedBegin(esf_ctx* ctx, int cls) {
   if (!ctx->kernelTiming) return nullptr;
   if (ctx->timedUsed == ctx->timed.size()) {

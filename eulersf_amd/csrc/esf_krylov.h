@@ -12,8 +12,7 @@ namespace {
 template <typename T>
 __global__ void __launch_bounds__(256) kCrDirection(long n, T* p, T* Ap, const T* r,
                                                     const T* s, const T* D,
-                                                    CrScalars* sc, double* partials,
-                                                    unsigned* counter) {
+                                                    const CrScalars* sc, double* partials) {
   if (sc->done) return;
   __shared__ double red[8];
   const T beta = (T)sc->beta;
@@ -27,11 +26,70 @@ __global__ void __launch_bounds__(256) kCrDirection(long n, T* p, T* Ap, const T
     const T q = an / D[i];
     part += (double)an * (double)q;
   }
-  double mine[1] = {part}, tot[1];
-  if (lastBlockReduce<1>(mine, partials, counter, red, tot) && threadIdx.x == 0) {
-    const double apq = tot[0];
-    sc->alpha = (fabs(apq) > 0.0) ? sc->delta / apq : apq;
+  double mine[1] = {part};
+  blockPartials<1>(mine, partials, red);
+}
+
+// What the finisher does with the totals (t0, t1, ...) of the producer
+// that ran just before it.  The scalar updates follow CONJUGATE_RESIDUAL.inl
+// (:118-131 setup, :141-160 loop, :166-176 convergence test); alpha stays
+// the denominator when that is zero, as there.
+enum FinishOp {
+  F_BNORM = 0,    // |b| = sqrt(t0)
+  F_SETUP_S = 1,  // delta = t0 (r.s); alpha = delta / t1 (Ap.D^-1 Ap)
+  F_UPDATE = 2,   // beta = t0 / delta; delta = t0
+  F_CHECK = 3,    // |Ax - b| = sqrt(t0); done when below eps |b|
+  F_IR = 4,       // outer residual sqrt(t0); |b| = sqrt(t1) on the first call
+  F_DIR = 5,      // alpha = delta / t0
+  F_SOLID_CR = 6, // delta, beta from t0; alpha from t1 + 2 beta t2 + beta^2 t3
+};
+
+template <int OP, int K>
+__global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, CrScalars* sc,
+                                                int iter, int first) {
+  if (sc->done) return;
+  __shared__ double red[16];
+  double t[K];
+  finishSums<K>(nb, partials, red, t);
+  if (threadIdx.x != 0) return;
+  if constexpr (OP == F_BNORM) {
+    sc->bnorm = sqrt(t[0]);
+  } else if constexpr (OP == F_SETUP_S) {
+    sc->delta = t[0];
+    sc->beta = 0.0;
+    sc->alpha = (fabs(t[K - 1]) > 0.0) ? t[0] / t[K - 1] : t[K - 1];
+  } else if constexpr (OP == F_UPDATE) {
+    const double dOld = sc->delta;
+    sc->delta = t[0];
+    sc->beta = t[0] / dOld;
+  } else if constexpr (OP == F_CHECK) {
+    const double rn = sqrt(t[0]);
+    sc->resnorm = rn;
+    if (rn < sc->eps * sc->bnorm) {
+      sc->done = 1;
+      sc->iter = iter;
+    }
+  } else if constexpr (OP == F_IR) {
+    sc->resnorm = sqrt(t[0]);
+    if (first) sc->bnorm = sqrt(t[K - 1]);
+    if (sc->resnorm < sc->eps * sc->bnorm) sc->done = 1;
+  } else if constexpr (OP == F_DIR) {
+    sc->alpha = (fabs(t[0]) > 0.0) ? sc->delta / t[0] : t[0];
+  } else if constexpr (OP == F_SOLID_CR) {
+    static_assert(K == 4, "solid CR finisher takes four sums");
+    const double dNew = t[0];
+    const double beta = dNew / sc->delta;
+    const double apq = (t[1] + (2.0 * beta) * t[2]) + (beta * beta) * t[3];
+    sc->delta = dNew;
+    sc->beta = beta;
+    sc->alpha = (fabs(apq) > 0.0) ? dNew / apq : apq;
   }
+}
+
+template <int OP, int K>
+inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int first = 0) {
+  hipLaunchKernelGGL((kFinish<OP, K>), dim3(1), dim3(1024), 0, ctx->stream, nb,
+                     (const double*)ctx->partials, sc, iter, first);
 }
 
 }  // namespace

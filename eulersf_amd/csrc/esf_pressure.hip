@@ -46,7 +46,6 @@ struct SpmvArgs {
   float* r32;        // M_IR_RESID: residual handed to the FP32 solve
   CrScalars* sc;
   double* partials;
-  unsigned* counter;
   int iter;
   int first;         // M_IR_RESID: also compute |b|
 };
@@ -272,32 +271,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
   }
   if (MODE == M_APPLY) return;
   double mine[2] = {part0, part1};
-  double tot[2];
-  if (lastBlockReduce<2>(mine, a.partials, a.counter, red, tot) && threadIdx.x == 0) {
-    CrScalars* sc = a.sc;
-    if (MODE == M_SETUP_R) {
-      sc->bnorm = sqrt(tot[0]);
-    } else if (MODE == M_SETUP_S) {
-      sc->delta = tot[0];
-      double apq = tot[1];
-      sc->alpha = (fabs(apq) > 0.0) ? tot[0] / apq : apq;
-    } else if (MODE == M_UPDATE) {
-      const double dOld = sc->delta;
-      sc->delta = tot[0];
-      sc->beta = tot[0] / dOld;
-    } else if (MODE == M_CHECK) {
-      const double rn = sqrt(tot[0]);
-      sc->resnorm = rn;
-      if (rn < sc->eps * sc->bnorm) {
-        sc->done = 1;
-        sc->iter = a.iter;
-      }
-    } else if (MODE == M_IR_RESID) {
-      sc->resnorm = sqrt(tot[0]);
-      if (a.first) sc->bnorm = sqrt(tot[1]);
-      if (sc->resnorm < sc->eps * sc->bnorm) sc->done = 1;
-    }
-  }
+  blockPartials<2>(mine, a.partials, red);
 }
 
 // b = J v0 - c, c (boundary velocities), D = diag(S); also the FP32 copies
@@ -357,6 +331,12 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   hipLaunchKernelGGL((kSpmv<T, MODE>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d, a,
                      tx, ty, tz);
   timedEnd(ctx, tl);
+  // the grid-wide sums of the mode (kSpmv's epilogue partials)
+  if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
+  if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
+  if constexpr (MODE == M_UPDATE) launchFinish<F_UPDATE, 1>(ctx, nb, a.sc);
+  if constexpr (MODE == M_CHECK) launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter);
+  if constexpr (MODE == M_IR_RESID) launchFinish<F_IR, 2>(ctx, nb, a.sc, 0, a.first);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("kSpmv launch: ") + hipGetErrorString(e);
@@ -499,7 +479,6 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   a.Ap = Ap;
   a.sc = &ctx->sc[which];
   a.partials = ctx->partials;
-  a.counter = &ctx->counter[which];
   T* rA = r;
   T* rB = r2;
   a.in = x;
@@ -517,9 +496,10 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
     auto* tl = timedBegin(ctx, sizeof(T) == 4 ? 13 : 12);
     hipLaunchKernelGGL(kCrDirection<T>, dim3(dirBlocks), dim3(256), 0, ctx->stream,
-                       (long)d.cells, p, Ap, (const T*)rB, (const T*)s, D, a.sc,
-                       ctx->partials, &ctx->counter[which]);
+                       (long)d.cells, p, Ap, (const T*)rB, (const T*)s, D,
+                       (const CrScalars*)a.sc, ctx->partials);
     timedEnd(ctx, tl);
+    launchFinish<F_DIR, 1>(ctx, dirBlocks, a.sc);
     std::swap(rA, rB);
     if (i > 0 && i % 5 == 0) {
       a.in = x;
@@ -567,7 +547,6 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
   o.e32 = nullptr;
   o.sc = &ctx->sc[0];
   o.partials = ctx->partials;
-  o.counter = &ctx->counter[0];
   o.first = 1;
   if ((st = launchSpmv<double, M_IR_RESID>(ctx, o))) return st;
   CrScalars h;

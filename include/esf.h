@@ -92,6 +92,7 @@ typedef struct {
   int inner_iterations;   /* mixed: FP32 iterations summed over refinements */
   double dt;              /* the substep's dt */
   int n_solid_nodes;      /* solid DOF nodes this step */
+  int n_solid_cells;      /* valid solid cells (all solids) this step */
 } esf_solve_stats;
 
 /* field ids for esf_upload / esf_download */
@@ -241,7 +242,8 @@ esf_status esf_set_kernel_timing(esf_ctx* ctx, int on);
 /* Accumulated device time (ms) and launch count per kernel class since the
  * last reset: 0-5 pressure SpMV FP64 (apply, setup-r, setup-s, CR update,
  * residual check, refinement residual), 6-11 the same in FP32, 12/13 the CR
- * direction update FP64/FP32.  Arrays of 16. */
+ * direction update FP64/FP32, 14 the solid CR cell pass (K_e r per cell), 15
+ * the solid CR node pass (gather + update).  Arrays of 16. */
 esf_status esf_kernel_timing(esf_ctx* ctx, double* total_ms, long* counts,
                              int reset);
 
