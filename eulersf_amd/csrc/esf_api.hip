@@ -76,7 +76,11 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   }
   esf_status st = dalloc(ctx, &ctx->solidNodes, cap);
   if (st) return st;
-  if ((st = dalloc(ctx, &ctx->nodeSlot, 4 * 8 * cap))) return st;
+  if ((st = dalloc(ctx, &ctx->nodeMask, 4 * cap))) return st;
+  for (int b = 0; b < 4; b++)
+    if ((st = dalloc(ctx, &ctx->inbox[b], 24 * cap))) return st;
+  // 4 sets x 4 sums x node-pass blocks (3 cap DOFs / 256)
+  if ((st = dalloc(ctx, &ctx->sPartials, 16 * (3 * cap / 256 + 1)))) return st;
   ctx->capSolid = (int)cap;
   return ESF_OK;
 }
@@ -90,7 +94,6 @@ esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need) {
   if ((st = dalloc(ctx, &so.cellOut, 24 * cap))) return st;
   if ((st = dalloc(ctx, &so.cellDiag, 24 * cap))) return st;
   if ((st = dalloc(ctx, &so.cornerDof, 8 * cap))) return st;
-  if ((st = dalloc(ctx, &so.cellKx, 24 * cap))) return st;
   so.capValid = (int)cap;
   return ESF_OK;
 }
@@ -106,13 +109,14 @@ static void freeAll(esf_ctx* ctx) {
                   ctx->sDamped, ctx->sKdiag, ctx->sAdiag, ctx->sRhs,      ctx->sV0,
                   ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
                   ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
-                  ctx->dmisc,  ctx->nodeSlot};
+                  ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox[0], ctx->inbox[1],
+                  ctx->inbox[2], ctx->inbox[3]};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& so : ctx->solids) {
     void* sp[] = {so.rest, so.u,       so.unew,     so.X,       so.vel,     so.velws,
                   so.mass, so.detFInv, so.qfrac,    so.valid,   so.cellList, so.cellSlot,
-                  so.cellMat, so.cellOut, so.cellDiag, so.cornerDof, so.cellKx};
+                  so.cellMat, so.cellOut, so.cellDiag, so.cornerDof};
     for (void* p : sp)
       if (p) hipFree(p);
   }

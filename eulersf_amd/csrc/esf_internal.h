@@ -66,7 +66,6 @@ struct DevSolid {
   double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
   int* cornerDof = nullptr;  // per slot and corner: 3 * solid index, or -1 if masked
-  double* cellKx = nullptr;  // per slot: K_e v, 24 contiguous doubles
   bool listDirty = true;
 };
 
@@ -86,7 +85,8 @@ struct esf_ctx {
   int* sidx = nullptr;        // nodes -> solid DOF node index or -1
   int* solidNodes = nullptr;  // solid DOF node -> node index
   int nSolid = 0, capSolid = 0;
-  int* nodeSlot = nullptr;    // per solid (<= 4) x 8 cells x capSolid: K gather table
+  uint8_t* nodeMask = nullptr; // per solid (<= 4) x capSolid: cells with K rows at the node
+  double* inbox[4] = {nullptr, nullptr, nullptr, nullptr};  // per solid: 24 x capSolid
   int* scanBlocks = nullptr;  // scan scratch
   // solve: node-based
   double *v0 = nullptr, *minv = nullptr;  // 3*nodes, nodes
@@ -106,6 +106,7 @@ struct esf_ctx {
          *stmp = nullptr;
   // reductions
   double* partials = nullptr;  // per-block partial sums
+  double* sPartials = nullptr; // solid CR: 4 sets of 4 x node-pass blocks
   esf::CrScalars* sc = nullptr;   // device scalars (several solves)
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
   int* hostFlags = nullptr;       // pinned ring of done flags
@@ -146,10 +147,49 @@ namespace esf {
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
-// wave64 butterfly then fixed-order sum across the block's waves; the
-// result is valid in thread 0.  Deterministic for a fixed launch shape.
+// Cross-lane moves of doubles with DPP (two 32-bit v_mov_dpp, VALU speed,
+// no LDS traffic).  Controls used (gfx9 DPP encoding): quad_perm [1,0,3,2]
+// = 0xB1 (lane ^ 1), quad_perm [2,3,0,1] = 0x4E (lane ^ 2), row_half_mirror
+// = 0x141 (lane l <-> 7 - l within each 8), row_mirror = 0x140 (l <-> 15 - l
+// within each 16).  Every lane of the pattern has a source, so no lane
+// reads a disabled one.
+template <int CTRL>
+__device__ inline double dppMov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double readlaneD(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over each aligned group of 8 lanes, bitwise the same in all 8 (each
+// step adds a pair and its partner adds the same pair commuted).  The
+// pairing is that of an xor butterfly over lane bits 0, 1, 2.
+__device__ inline double sum8(double v) {
+  v = v + dppMov<0xB1>(v);
+  v = v + dppMov<0x4E>(v);
+  v = v + dppMov<0x141>(v);
+  return v;
+}
+
+// Sum over the 64 lanes of the wave, uniform (bitwise the same in every
+// lane): 16-lane sums by DPP, then the four rows added in a fixed order.
+// All 64 lanes must be active.
+__device__ inline double waveSum(double v) {
+  v = sum8(v);
+  v = v + dppMov<0x140>(v);
+  return (readlaneD(v, 0) + readlaneD(v, 16)) + (readlaneD(v, 32) + readlaneD(v, 48));
+}
+
+// wave sums, then a fixed-order sum across the block's waves; the result
+// is valid in thread 0.  Deterministic for a fixed launch shape.  All
+// threads of the block must call it.
 __device__ inline double blockSum(double v, double* lds) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  v = waveSum(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) lds[wid] = v;
@@ -184,7 +224,17 @@ __device__ inline void finishSums(int nb, const double* partials, double* lds,
                                   double (&tot)[K]) {
   for (int k = 0; k < K; k++) {
     double acc = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[(size_t)k * nb + i];
+    // 8 strides per round, loads first
+    for (int base = threadIdx.x; base < nb; base += 8 * blockDim.x) {
+      double c[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int i = base + u * blockDim.x;
+        c[u] = i < nb ? partials[(size_t)k * nb + i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) acc += c[u];
+    }
     tot[k] = blockSum(acc, lds);
   }
 }

@@ -86,10 +86,12 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
   }
 }
 
+// partials: the producer's block partials (default: the context's buffer)
 template <int OP, int K>
-inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int first = 0) {
+inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int first = 0,
+                         const double* partials = nullptr) {
   hipLaunchKernelGGL((kFinish<OP, K>), dim3(1), dim3(1024), 0, ctx->stream, nb,
-                     (const double*)ctx->partials, sc, iter, first);
+                     partials ? partials : (const double*)ctx->partials, sc, iter, first);
 }
 
 }  // namespace

@@ -138,7 +138,7 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
                              const double* __restrict__ cellDiag,
                              const double* __restrict__ smass, const double* __restrict__ svel,
                              Box box, int first, double* sForce, double* sKdiag, double* sMv,
-                             int* nodeSlot, int capS) {
+                             uint8_t* nodeMask) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
@@ -146,14 +146,16 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
   double f[3] = {0, 0, 0}, kd[3] = {0, 0, 0};
   const bool rowKeep = smass[n] != 0;
   int nk = 0;
+  unsigned mask = 0;
   for (int dz = 0; dz < 2; dz++)
     for (int dy = 0; dy < 2; dy++)
       for (int dx = 0; dx < 2; dx++) {
         const long cell = (x - 1 + dx) + (long)(y - 1 + dy) * d.CX + (long)(z - 1 + dz) * d.cslab;
         const int slot = cellSlot[cell];
-        // stiffness gather table of the Krylov loop: rows whose node has
-        // no mass in this solid take no K contribution (DEFOR_SOLID_3D.cpp:836-842)
-        nodeSlot[(long)(dx + 2 * dy + 4 * dz) * capS + s] = rowKeep ? slot : -1;
+        // cells whose stiffness rows reach this node in the Krylov loop:
+        // valid cells, and none at all where the node has no mass in this
+        // solid (DEFOR_SOLID_3D.cpp:836-842)
+        if (slot >= 0 && rowKeep) mask |= 1u << (dx + 2 * dy + 4 * dz);
         if (slot < 0) continue;
         const int corner = (1 - dx) + 2 * (1 - dy) + 4 * (1 - dz);
         for (int r = 0; r < 3; r++) f[r] += cellOut[(long)(3 * corner + r) * cap + slot];
@@ -165,6 +167,7 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
           nk++;
         }
       }
+  nodeMask[s] = (uint8_t)mask;
   double mv[3] = {0, 0, 0};
   const bool inBox = x >= box.bMin[0] && x < box.bMax[0] - 1 && y >= box.bMin[1] &&
                      y < box.bMax[1] - 1 && z >= box.bMin[2] && z < box.bMax[2] - 1;
@@ -182,12 +185,96 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
   }
 }
 
+// alpha and beta of solid CR iteration i, from the partial sums the node
+// pass of iteration i - 1 left in `cur` (delta_i, s.D^-1 s, Ap.D^-1 s,
+// Ap.D^-1 Ap; the setup pass for i = 0) and delta_{i-1} from the pass
+// before (`prev`).  beta = delta_i / delta_{i-1} (0 for i = 0), alpha =
+// delta_i / ((s + beta Ap).D^-1 (s + beta Ap)) with the square expanded
+// (CONJUGATE_RESIDUAL.inl:141-160; alpha stays the denominator when that
+// is zero, as there).
+//
+// Canonical order, whatever the block size: the nb block partials form
+// groups of 64; a group's sum is a wave sum (waveSum: the same in every
+// lane), and the groups are added in ascending order.  Every block of both
+// loop kernels therefore holds bitwise the same alpha and beta.
+// Split in two so the loads go out with the kernel's first round of loads:
+//   scalarLoads  : each wave loads its groups (wave w: groups w, w + nw, ...)
+//   scalarFinish : butterflies, group sums through LDS, alpha and beta
+constexpr int kScalGroupsPerWave = 2;  // groups a wave loads in one round
+constexpr int kScalMaxGroups = 128;    // nb <= 8192 node-pass blocks
+
+struct ScalarLoads {
+  double v[kScalGroupsPerWave][5];
+};
+
+__device__ inline void scalarLoads(const double* cur, const double* prev, int nb, int g0,
+                                   ScalarLoads& L) {
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int u = 0; u < kScalGroupsPerWave; u++) {
+    const int i = 64 * (g0 + u * nw) + lane;
+    const int ic = i < nb ? i : nb - 1;  // in bounds; masked in scalarFinish
+    L.v[u][0] = cur[ic];
+    L.v[u][1] = cur[nb + ic];
+    L.v[u][2] = cur[2 * nb + ic];
+    L.v[u][3] = cur[3 * nb + ic];
+    L.v[u][4] = prev[ic];
+  }
+}
+
+// returns sc->done in every thread (sc->done is read with the partials)
+__device__ inline int scalarFinish(const double* cur, const double* prev, int nb, int first,
+                                   const CrScalars* sc, ScalarLoads& L, double* grp,
+                                   double& alpha, double& beta) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int groups = (nb + 63) / 64;
+  const int done = sc->done;
+  for (int g0 = w; g0 < groups; g0 += kScalGroupsPerWave * nw) {
+    if (g0 != w) scalarLoads(cur, prev, nb, g0, L);  // more groups than one round covers
+#pragma unroll
+    for (int u = 0; u < kScalGroupsPerWave; u++) {
+      const int g = g0 + u * nw;
+      const bool ok = 64 * g + lane < nb;
+      double t[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) t[q] = waveSum(ok ? L.v[u][q] : 0.0);
+      if (lane == 0 && g < groups)
+#pragma unroll
+        for (int q = 0; q < 5; q++) grp[5 * g + q] = t[q];
+    }
+  }
+  __syncthreads();
+  double t0 = 0, t1 = 0, t2 = 0, t3 = 0, d0 = 0;
+  for (int g = 0; g < groups; g++) {
+    t0 += grp[5 * g];
+    t1 += grp[5 * g + 1];
+    t2 += grp[5 * g + 2];
+    t3 += grp[5 * g + 3];
+    d0 += grp[5 * g + 4];
+  }
+  beta = first ? 0.0 : t0 / d0;
+  const double apq = (t1 + (2.0 * beta) * t2) + (beta * beta) * t3;
+  alpha = (fabs(apq) > 0.0) ? t0 / apq : apq;
+  return done;
+}
+
+// where the partial sums of the solid CR loop live: three rotating sets of
+// 4 x nb (node pass i writes set i mod 3, the setup pass set 2), and the
+// residual-check set
+struct SolidPartials {
+  const double* cur;  // sums of the previous node pass
+  const double* prev; // sums of the pass before it
+  int nb;             // node-pass blocks
+  int first;          // iteration 0: no previous direction, beta = 0
+};
+
 // K_e v_e per valid cell, eight lanes per cell (lane c = corner c of cell
 // e = thread / 8).  Each lane forms its corner's share v_c (GF)_c^T of dF,
-// a 3-step xor butterfly sums the shares (commutative at every step, so
-// all eight lanes hold the bitwise-same dF), every lane evaluates dP(dF)
-// and writes its corner's three rows.  Output is per cell, 24 contiguous
-// doubles [24 e + 3 c + r], so a wave's stores are one contiguous run.
+// a 3-step DPP sum adds the shares (sum8: all eight lanes hold the
+// bitwise-same dF), every lane evaluates dP(dF) and writes its corner's
+// three rows of K_e v straight into the inbox of the corner's node (24
+// doubles per solid node: 3 rows for each of its 8 cells), so the node
+// pass reads them without an index table.  Masked corners write nothing.
 //   CELL_IN : v = in                      (apply, residual check)
 //   CELL_CR : v = r - alpha (Ap beta + s) / D, the CR residual update of
 //             CONJUGATE_RESIDUAL.inl:223-230 with the direction update of
@@ -202,32 +289,57 @@ __global__ void __launch_bounds__(256) kSolidCell8(Dims d, int nValid, int cap,
                                                    const double* __restrict__ s,
                                                    const double* __restrict__ Ap,
                                                    const double* __restrict__ diag,
-                                                   const CrScalars* sc, double lambda, double mu,
-                                                   double* __restrict__ cellKx) {
-  if (MODE == CELL_CR && sc->done) return;
+                                                   const CrScalars* sc, SolidPartials sp,
+                                                   double lambda, double mu,
+                                                   double* __restrict__ inbox) {
+  __shared__ double grp[5 * kScalMaxGroups];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = t >> 3, c = t & 7;
-  // whole groups of eight leave together, so the butterfly below only
-  // ever reads lanes of a live group
-  if (e >= nValid) return;
-  const int k = cornerDof[(long)c * cap + e];
-  double v0 = 0, v1 = 0, v2 = 0;
-  if (k >= 0) {
-    if (MODE == CELL_IN) {
-      v0 = in[k];
-      v1 = in[k + 1];
-      v2 = in[k + 2];
-    } else {
-      const double alpha = sc->alpha, beta = sc->beta;
-      v0 = in[k] - alpha * ((Ap[k] * beta + s[k]) / diag[k]);
-      v1 = in[k + 1] - alpha * ((Ap[k + 1] * beta + s[k + 1]) / diag[k + 1]);
-      v2 = in[k + 2] - alpha * ((Ap[k + 2] * beta + s[k + 2]) / diag[k + 2]);
+  const bool live = e < nValid;
+  // first round of loads, all unconditional (indices clamped in bounds,
+  // results masked afterwards) so nothing waits for a branch: the CR
+  // scalars' partials, the corner's DOF index and the cell's (R, L, F, 1/J)
+  ScalarLoads SL;
+  if (MODE == CELL_CR) scalarLoads(sp.cur, sp.prev, sp.nb, threadIdx.x >> 6, SL);
+  const int ec = live ? e : nValid - 1;
+  const int kraw = cornerDof[(long)c * cap + ec];
+  const double* m = cellMat + ec;
+  double F[9], R[9], L[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    F[q] = m[(long)(18 + q) * cap];
+    R[q] = m[(long)q * cap];
+    L[q] = m[(long)(9 + q) * cap];
+  }
+  const double Jinv = m[27L * cap];
+  // second round: the corner's r (or v), s, Ap, D
+  const int k = kraw >= 0 ? kraw : 0;
+  double rr[3], ss[3] = {0, 0, 0}, aa[3] = {0, 0, 0}, dd[3] = {1, 1, 1};
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    rr[j] = in[k + j];
+    if (MODE == CELL_CR) {
+      ss[j] = s[k + j];
+      aa[j] = Ap[k + j];
+      dd[j] = diag[k + j];
     }
   }
-  const double* m = cellMat + e;
-  double F[9];
+  double v[3];
+  if (MODE == CELL_CR) {
+    double alpha, beta;
+    if (scalarFinish(sp.cur, sp.prev, sp.nb, sp.first, sc, SL, grp, alpha, beta)) return;
 #pragma unroll
-  for (int q = 0; q < 9; q++) F[q] = m[(long)(18 + q) * cap];
+    for (int j = 0; j < 3; j++) v[j] = rr[j] - alpha * ((aa[j] * beta + ss[j]) / dd[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 3; j++) v[j] = rr[j];
+  }
+  if (kraw < 0)
+#pragma unroll
+    for (int j = 0; j < 3; j++) v[j] = 0.0;
+  // whole groups of eight leave together, so the butterfly below only
+  // ever reads lanes of a live group
+  if (!live) return;
   const double g = 0.25 * d.dhInv;
   const double G0 = sgn(c & 1) * g, G1 = sgn((c >> 1) & 1) * g, G2 = sgn(c >> 2) * g;
   double gf[3];
@@ -236,33 +348,23 @@ __global__ void __launch_bounds__(256) kSolidCell8(Dims d, int nValid, int cap,
   double dF[9];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    dF[q] = v0 * gf[q];
-    dF[3 + q] = v1 * gf[q];
-    dF[6 + q] = v2 * gf[q];
+    dF[q] = v[0] * gf[q];
+    dF[3 + q] = v[1] * gf[q];
+    dF[6 + q] = v[2] * gf[q];
   }
 #pragma unroll
-  for (int off = 1; off < 8; off <<= 1)
-#pragma unroll
-    for (int q = 0; q < 9; q++) dF[q] += __shfl_xor(dF[q], off, 8);
-  double R[9], L[9];
-#pragma unroll
-  for (int q = 0; q < 9; q++) {
-    R[q] = m[(long)q * cap];
-    L[q] = m[(long)(9 + q) * cap];
-  }
-  const double Jinv = m[27L * cap];
+  for (int q = 0; q < 9; q++) dF[q] = sum8(dF[q]);
   double dP[9];
   corotDP(R, L, lambda, mu, dF, dP);
+  if (kraw < 0) return;
   const double dh = d.dh;
-  double* out = cellKx + 24L * e + 3 * c;
+  // the corner's rows go to its node's inbox: node s = kraw / 3 keeps 8
+  // rows of 3, row 7 - c for the cell at offset 7 - c from it
+  double* out = inbox + 8L * kraw + 3 * (7 - c);
 #pragma unroll
   for (int r = 0; r < 3; r++) {
-    double y = 0;
-    if (k >= 0) {
-      y = (dP[3 * r] * gf[0] + dP[3 * r + 1] * gf[1]) + dP[3 * r + 2] * gf[2];
-      y = (((y * Jinv) * dh) * dh) * dh;
-    }
-    out[r] = y;
+    double y = (dP[3 * r] * gf[0] + dP[3 * r + 1] * gf[1]) + dP[3 * r + 2] * gf[2];
+    out[r] = (((y * Jinv) * dh) * dh) * dh;
   }
 }
 
@@ -271,8 +373,8 @@ enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3 };
 struct SolidNodeArgs {
   int nSolid;
   int nSolidBodies;       // up to 4 solids gathered in one launch
-  const double* cellKx[4];
-  const int* nodeSlot;    // [(8 b + c) capS + s]: slot of the cell at offset c, or -1
+  const double* inbox[4]; // per solid: 24 doubles per solid node (see kSolidCell8)
+  const uint8_t* nodeMask;// [b capS + s]: bit c = the cell at offset c has rows here
   int capS;
   double scale;           // dt^2 + beta dt
   const double* damped;   // diag mass
@@ -285,85 +387,106 @@ struct SolidNodeArgs {
   double* p;              // S_CR
   double* s;              // S_SETUP, S_CR
   double* Ap;             // S_CR
-  CrScalars* sc;
-  double* partials;
-  int iter;
+  const CrScalars* sc;
+  SolidPartials sp;       // S_CR: where alpha and beta come from
+  double* partials;       // this pass's block partials
 };
 
-// y = scale * (K v) + M~ v per solid DOF node, K v gathered from the
-// adjacent cells' outputs in ascending cell order, with CR epilogues:
-//   S_SETUP : s = A r;  r.s and s.D^-1 s  -> delta, alpha (beta = 0)
+// y = scale * (K v) + M~ v per solid DOF (one thread per DOF k = 3 s + q),
+// K v summed from the node's inbox rows in ascending cell order, with CR
+// epilogues (block partials, summed by the consumers):
+//   S_SETUP : s = A r;  r.s, s.D^-1 s
 //   S_CR    : Ap = Ap beta + s, p = p beta + r, x += alpha p,
 //             r -= alpha Ap / D, s = A r;  r.s, s.D^-1 s, Ap.D^-1 s,
-//             Ap.D^-1 Ap -> delta, beta and the next alpha from
-//             (s + beta Ap).D^-1 (s + beta Ap) expanded
-//   S_CHECK : |A x - b|  (every 5 iterations, CONJUGATE_RESIDUAL.inl:246-255)
+//             Ap.D^-1 Ap
+//   S_CHECK : |A x - b|^2  (every 5 iterations, CONJUGATE_RESIDUAL.inl:246-255)
+// All loads are in the first round (indices clamped in bounds, masked
+// afterwards), so a pass costs one memory round trip.
 template <int MODE>
-__global__ void __launch_bounds__(128) kSolidNode(Dims d, SolidNodeArgs a) {
-  if (MODE != S_APPLY && a.sc->done) return;
+__global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   __shared__ double red[8];
-  const int sn = blockIdx.x * blockDim.x + threadIdx.x;
-  double p0 = 0, p1 = 0, p2 = 0, p3 = 0;
-  if (sn < a.nSolid) {
-    double kx[3] = {0, 0, 0};
-    for (int b = 0; b < a.nSolidBodies; b++) {
-      int slot[8];
+  __shared__ double grp[5 * kScalMaxGroups];
+  const long nDof = 3L * a.nSolid;
+  if (nDof == 0) return;
+  const long k0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool live = k0 < nDof;
+  const long k = live ? k0 : nDof - 1;
+  const long sn = k / 3;
+  const int q = (int)(k - 3 * sn);
+  ScalarLoads SL;
+  if (MODE == S_CR) scalarLoads(a.sp.cur, a.sp.prev, a.sp.nb, threadIdx.x >> 6, SL);
+  unsigned mask[4];
+  double rows[4][8];
 #pragma unroll
-      for (int c = 0; c < 8; c++) slot[c] = a.nodeSlot[(long)(8 * b + c) * a.capS + sn];
-      const double* out = a.cellKx[b];
+  for (int b = 0; b < 4; b++) {
+    mask[b] = 0;
+    if (b < a.nSolidBodies) {
+      mask[b] = a.nodeMask[(long)b * a.capS + sn];
+      const double* in = a.inbox[b] + 24 * sn + q;
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        if (slot[c] < 0) continue;
-        // the node is corner 7 - c of the cell at offset c
-        const double* o = out + 24L * slot[c] + 3 * (7 - c);
-        kx[0] += o[0];
-        kx[1] += o[1];
-        kx[2] += o[2];
-      }
+      for (int c = 0; c < 8; c++) rows[b][c] = in[3 * c];
     }
-    double alpha = 0, beta = 0;
-    if (MODE == S_CR) {
-      alpha = a.sc->alpha;
-      beta = a.sc->beta;
-    }
+  }
+  const double dm = a.damped[k];
+  double vin = 0, dg = 1, ap0 = 0, s0 = 0, p0 = 0, r0 = 0, x0 = 0, bb = 0;
+  if (MODE == S_CR) {
+    dg = a.diag[k];
+    ap0 = a.Ap[k];
+    s0 = a.s[k];
+    p0 = a.p[k];
+    r0 = a.r[k];
+    x0 = a.x[k];
+  } else {
+    vin = a.in[k];
+    if (MODE == S_SETUP) dg = a.diag[k];
+    if (MODE == S_CHECK) bb = a.b[k];
+  }
+  double kx = 0;
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-      const long k = 3L * sn + q;
-      if (MODE == S_CR) {
-        const double dg = a.diag[k];
-        const double ap = a.Ap[k] * beta + a.s[k];
-        const double pp = a.p[k] * beta + a.r[k];
-        a.x[k] = a.x[k] + alpha * pp;
-        const double rn = a.r[k] - alpha * (ap / dg);
-        const double y = kx[q] * a.scale + a.damped[k] * rn;
-        a.Ap[k] = ap;
-        a.p[k] = pp;
-        a.r[k] = rn;
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (b < a.nSolidBodies && (mask[b] >> c & 1u)) kx += rows[b][c];
+  double alpha = 0, beta = 0;
+  if (MODE == S_CR) {
+    if (scalarFinish(a.sp.cur, a.sp.prev, a.sp.nb, a.sp.first, a.sc, SL, grp, alpha, beta))
+      return;
+  }
+  double part[4] = {0, 0, 0, 0};
+  if (MODE == S_CR) {
+    const double ap = ap0 * beta + s0;
+    const double pp = p0 * beta + r0;
+    const double rn = r0 - alpha * (ap / dg);
+    const double y = kx * a.scale + dm * rn;
+    if (live) {
+      a.x[k] = x0 + alpha * pp;
+      a.Ap[k] = ap;
+      a.p[k] = pp;
+      a.r[k] = rn;
+      a.s[k] = y;
+      const double yq = y / dg;
+      part[0] = rn * y;
+      part[1] = y * yq;
+      part[2] = ap * yq;
+      part[3] = ap * (ap / dg);
+    }
+  } else {
+    const double y = kx * a.scale + dm * vin;
+    if (live) {
+      if (MODE == S_APPLY) {
+        a.out[k] = y;
+      } else if (MODE == S_SETUP) {
         a.s[k] = y;
-        const double yq = y / dg;
-        p0 += rn * y;
-        p1 += y * yq;
-        p2 += ap * yq;
-        p3 += ap * (ap / dg);
+        part[0] = vin * y;
+        part[1] = y * (y / dg);
       } else {
-        const double v = a.in[k];
-        const double y = kx[q] * a.scale + a.damped[k] * v;
-        if (MODE == S_APPLY) {
-          a.out[k] = y;
-        } else if (MODE == S_SETUP) {
-          a.s[k] = y;
-          p0 += v * y;
-          p1 += y * (y / a.diag[k]);
-        } else {
-          const double t = y - a.b[k];
-          p0 += t * t;
-        }
+        const double t = y - bb;
+        part[0] = t * t;
       }
     }
   }
   if (MODE == S_APPLY) return;
-  double mine[4] = {p0, p1, p2, p3};
-  blockPartials<4>(mine, a.partials, red);
+  blockPartials<4>(part, a.partials, red);
 }
 
 // diag(A) = scale * diag(K) + M~   (COUPLED_INTEGRATOR_3D.cpp:613-617)
@@ -403,8 +526,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
                          ctx->stream, d, ctx->nSolid, so.capValid, ctx->solidNodes, so.cellSlot,
                          so.cellOut,
                          so.cellDiag, so.mass, so.vel, so.box, b == 0 ? 1 : 0, ctx->sForce,
-                         ctx->sKdiag, ctx->sMv, ctx->nodeSlot + 8L * b * ctx->capSolid,
-                         ctx->capSolid);
+                         ctx->sKdiag, ctx->sMv, ctx->nodeMask + (long)b * ctx->capSolid);
   }
   if (energy) *energy = en;
   hipError_t e = hipGetLastError();
@@ -418,8 +540,8 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
 static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
   a.nSolid = ctx->nSolid;
   a.nSolidBodies = (int)std::min<size_t>(ctx->solids.size(), 4);
-  for (int b = 0; b < a.nSolidBodies; b++) a.cellKx[b] = ctx->solids[b].cellKx;
-  a.nodeSlot = ctx->nodeSlot;
+  for (int b = 0; b < a.nSolidBodies; b++) a.inbox[b] = ctx->inbox[b];
+  a.nodeMask = ctx->nodeMask;
   a.capS = ctx->capSolid;
   a.scale = ctx->dt * ctx->dt + ctx->p.rayleigh_beta * ctx->dt;
   a.damped = ctx->sDamped;
@@ -429,28 +551,30 @@ static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
 // the CR-update launches are kernel classes 14 (cells) and 15 (nodes) of
 // esf_kernel_timing
 template <int MODE>
-static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc) {
-  for (auto& so : ctx->solids)
+static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
+                     const SolidPartials& sp) {
+  for (size_t b = 0; b < ctx->solids.size(); b++) {
+    DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
       auto* tl = MODE == CELL_CR ? timedBegin(ctx, 14) : nullptr;
       hipLaunchKernelGGL((kSolidCell8<MODE>), dim3(cdiv(8L * so.nValid, 256)), dim3(256), 0,
                          ctx->stream, ctx->d, so.nValid, so.capValid, (const int*)so.cornerDof,
                          (const double*)so.cellMat, in, (const double*)ctx->ss,
-                         (const double*)ctx->sAp, (const double*)ctx->sAdiag, sc, so.lambda,
-                         so.mu, so.cellKx);
+                         (const double*)ctx->sAp, (const double*)ctx->sAdiag, sc, sp,
+                         so.lambda, so.mu, ctx->inbox[b]);
       timedEnd(ctx, tl);
     }
+  }
 }
+
+static int nodeBlocks(const esf_ctx* ctx) { return cdiv(std::max(3L * ctx->nSolid, 1L), 256); }
 
 template <int MODE>
 static void nodePass(esf_ctx* ctx, SolidNodeArgs& a) {
   auto* tl = MODE == S_CR ? timedBegin(ctx, 15) : nullptr;
-  const int nb = cdiv(std::max(ctx->nSolid, 1), 128);
-  hipLaunchKernelGGL((kSolidNode<MODE>), dim3(nb), dim3(128), 0, ctx->stream, ctx->d, a);
+  hipLaunchKernelGGL((kSolidNode<MODE>), dim3(nodeBlocks(ctx)), dim3(256), 0, ctx->stream,
+                     ctx->d, a);
   timedEnd(ctx, tl);
-  if constexpr (MODE == S_SETUP) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
-  if constexpr (MODE == S_CR) launchFinish<F_SOLID_CR, 4>(ctx, nb, a.sc);
-  if constexpr (MODE == S_CHECK) launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter);
 }
 
 esf_status solidAdiag(esf_ctx* ctx) {
@@ -467,7 +591,7 @@ esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
     ctx->err = "at most 4 solids per context";
     return ESF_ERR_INVALID;
   }
-  cellPass<CELL_IN>(ctx, x, nullptr);
+  cellPass<CELL_IN>(ctx, x, nullptr, SolidPartials{});
   SolidNodeArgs a{};
   fillNodeArgs(ctx, a);
   a.in = x;
@@ -518,6 +642,11 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
     *relres = 0;
     return ESF_OK;
   }
+  if (nodeBlocks(ctx) > 64 * kScalMaxGroups) {
+    ctx->err = "too many solid nodes for one context (limit " +
+               std::to_string(256L * 64 * kScalMaxGroups / 3) + ")";
+    return ESF_ERR_INVALID;
+  }
   CrScalars h0{};
   h0.eps = eps;
   h0.iter = -1;
@@ -527,6 +656,11 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
                      (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->partials);
   launchFinish<F_BNORM, 1>(ctx, eb, sc);
+  // partial sets: node pass i writes set i mod 3 (the setup pass set 2),
+  // the residual check its own set
+  const int nb = nodeBlocks(ctx);
+  double* set[3] = {ctx->sPartials, ctx->sPartials + 4L * nb, ctx->sPartials + 8L * nb};
+  double* checkSet = ctx->sPartials + 12L * nb;
   SolidNodeArgs a{};
   fillNodeArgs(ctx, a);
   a.b = b;
@@ -536,19 +670,23 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   a.s = ctx->ss;
   a.Ap = ctx->sAp;
   a.sc = sc;
-  a.partials = ctx->partials;
-  cellPass<CELL_IN>(ctx, ctx->sr, sc);
+  cellPass<CELL_IN>(ctx, ctx->sr, sc, SolidPartials{});
   a.in = ctx->sr;
+  a.partials = set[2];
   nodePass<S_SETUP>(ctx, a);
   DonePoller poll(ctx, which);
   for (int i = 0; i < maxIt; i++) {
-    cellPass<CELL_CR>(ctx, ctx->sr, sc);
+    SolidPartials sp{set[(i + 2) % 3], set[(i + 1) % 3], nb, i == 0 ? 1 : 0};
+    cellPass<CELL_CR>(ctx, ctx->sr, sc, sp);
+    a.sp = sp;
+    a.partials = set[i % 3];
     nodePass<S_CR>(ctx, a);
     if (i > 0 && i % 5 == 0) {
-      cellPass<CELL_IN>(ctx, x, sc);
+      cellPass<CELL_IN>(ctx, x, sc, SolidPartials{});
       a.in = x;
-      a.iter = i;
+      a.partials = checkSet;
       nodePass<S_CHECK>(ctx, a);
+      launchFinish<F_CHECK, 1>(ctx, nb, sc, i, 0, checkSet);
       if (poll.mark()) break;
     }
   }
