@@ -111,7 +111,7 @@ def dominant_roofline(kt, n_cells, n_int, n_sc, n_sn):
     b = kernel_bytes(cls, n_cells, n_int, n_sc, n_sn)
     gbs = b / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel_class": cls,
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": load_traffic(cls), "kernel_class": cls,
             "algorithmic_bytes": b, "avg_launch_ms": round(ms, 5), "launches": count,
             "share_of_timed_kernel_ms": round(total / sum(v[0] for v in kt.values()), 3)}
 
@@ -167,6 +167,42 @@ def spmv_only(esf, cfg, n, reps):
     g.close()
 
 
+def traffic_run(esf, text):
+    """The command the HBM byte counters are collected over (rocprofv3 --pmc
+    FETCH_SIZE, then WRITE_SIZE): a few steps of each form, so every Krylov
+    kernel class runs, then a streaming copy of known size (256 MiB read +
+    256 MiB written per launch) that calibrates the counters."""
+    for prec in (esf.ESF_PLAIN_FP64, esf.ESF_MIXED):
+        g = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
+        for _ in range(3):
+            g.stepImplicit(-1.0)
+        g.synchronize()
+        g.close()
+    g = esf.CoupledIntegrator3D.from_cfg_text(text, esf.ESF_PLAIN_FP64)
+    nbytes = 256 << 20
+    ms = g.bench_copy(nbytes, 3)
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "copy_avg_ms": round(ms, 5)}), flush=True)
+    g.close()
+
+
+def load_traffic(cls):
+    """HBM bytes per launch of kernel class `cls` from the committed PMC
+    summary (profiles/traffic_unit_ball_128.md, produced by two rocprofv3
+    --pmc runs over `bench.py --traffic-run`), or None"""
+    path = os.path.join(ROOT, "profiles", "traffic_unit_ball_128.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        table = json.loads(block)["bytes_per_launch"]
+    except (IndexError, ValueError, KeyError):
+        return None
+    v = table.get(cls)
+    return None if v is None else int(v)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +216,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-run", action="store_true",
+                    help="a few steps of both forms and a calibrating copy, for the "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="N",
                     help="only time the pressure SpMV (FP64, FP32) on the N^3 scene and a "
                          "streaming copy of the same byte count; prints one JSON line")
@@ -199,6 +238,9 @@ def main():
     n = args.n
     text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000)
     n_cells, n_int = n ** 3, (n - 1) ** 3
+    if args.traffic_run:
+        traffic_run(esf, text)
+        return
 
     log(f"[bench] FP64 form: {args.warmup} warm-up + {args.steps} timed steps at {n}^3")
     prof = max(1, min(args.steps, 10))

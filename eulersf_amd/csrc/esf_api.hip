@@ -76,9 +76,8 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   }
   esf_status st = dalloc(ctx, &ctx->solidNodes, cap);
   if (st) return st;
-  if ((st = dalloc(ctx, &ctx->nodeMask, 4 * cap))) return st;
-  for (int b = 0; b < 4; b++)
-    if ((st = dalloc(ctx, &ctx->inbox[b], 24 * cap))) return st;
+  if ((st = dalloc(ctx, &ctx->nodeMask, cap))) return st;
+  if ((st = dalloc(ctx, &ctx->inbox, 4 * 24 * cap))) return st;
   // 4 sets x 4 sums x node-pass blocks (3 cap DOFs / 256)
   if ((st = dalloc(ctx, &ctx->sPartials, 16 * (3 * cap / 256 + 1)))) return st;
   ctx->capSolid = (int)cap;
@@ -109,8 +108,7 @@ static void freeAll(esf_ctx* ctx) {
                   ctx->sDamped, ctx->sKdiag, ctx->sAdiag, ctx->sRhs,      ctx->sV0,
                   ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
                   ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
-                  ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox[0], ctx->inbox[1],
-                  ctx->inbox[2], ctx->inbox[3]};
+                  ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& so : ctx->solids) {

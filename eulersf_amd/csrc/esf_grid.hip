@@ -622,7 +622,7 @@ struct BodyArgs {
 };
 
 __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
-  __shared__ double red[8];
+  __shared__ double red[16];
   const long index = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double en = 0;
   if (index < d.nodes) {

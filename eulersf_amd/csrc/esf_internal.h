@@ -3,6 +3,7 @@
 // -ffp-contract=off so every stage that mirrors a reference expression
 // rounds exactly as the reference's left-to-right evaluation does.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -85,8 +86,8 @@ struct esf_ctx {
   int* sidx = nullptr;        // nodes -> solid DOF node index or -1
   int* solidNodes = nullptr;  // solid DOF node -> node index
   int nSolid = 0, capSolid = 0;
-  uint8_t* nodeMask = nullptr; // per solid (<= 4) x capSolid: cells with K rows at the node
-  double* inbox[4] = {nullptr, nullptr, nullptr, nullptr};  // per solid: 24 x capSolid
+  uint32_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
+  double* inbox = nullptr;      // solid b (<= 4) at 24 capSolid b: 24 doubles per node
   int* scanBlocks = nullptr;  // scan scratch
   // solve: node-based
   double *v0 = nullptr, *minv = nullptr;  // 3*nodes, nodes
@@ -207,13 +208,26 @@ __device__ inline double blockSum(double v, double* lds) {
 // orders the partials for the finisher.  A last-block reduction inside the
 // producer would need an agent-scope release fence per block, which MI355X
 // serialises at ~20 ns per block (1024 blocks: ~22 us per reduction).
+// All K values in one pass: wave sums, one LDS exchange, thread k adds
+// quantity k over the waves in ascending order.  lds: K * (waves per
+// block) doubles.  All threads of the block must call it.
 template <int K>
 __device__ inline void blockPartials(const double (&mine)[K], double* partials, double* lds) {
   const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  for (int k = 0; k < K; k++) {
-    const double s = blockSum(mine[k], lds);
-    if (threadIdx.x == 0) partials[(size_t)k * nb + bid] = s;
+  double w[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) w[k] = waveSum(mine[k]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[K * wid + k] = w[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double r = 0;
+    for (int v = 0; v < nw; v++) r += lds[K * v + threadIdx.x];
+    partials[(size_t)threadIdx.x * nb + bid] = r;
   }
 }
 
@@ -243,8 +257,17 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // kernel classes for esf_kernel_timing: SpMV modes 0-5 (FP64), 6-11 (FP32),
 // CR direction 12 (FP64) / 13 (FP32), solid cell pass 14, solid node pass 15
-inline esf_ctx::TimedLaunch* timedBegin(esf_ctx* ctx, int cls) {
-  if (!ctx->kernelTiming) return nullptr;
+//
+// launchTimed: when kernel timing is on and cls >= 0, the launch carries a
+// start/stop event pair in its own dispatch (hipExtLaunchKernelGGL), so the
+// pair brackets the kernel's execution alone -- no separate event packets
+// around it that would add their own latency to the measurement.
+template <typename F, typename... Args>
+inline void launchTimed(esf_ctx* ctx, int cls, F kernel, dim3 grid, dim3 block, Args... args) {
+  if (cls < 0 || !ctx->kernelTiming) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+    return;
+  }
   if (ctx->timedUsed == ctx->timed.size()) {
     esf_ctx::TimedLaunch t{};
     hipEventCreate(&t.a);
@@ -253,11 +276,7 @@ inline esf_ctx::TimedLaunch* timedBegin(esf_ctx* ctx, int cls) {
   }
   esf_ctx::TimedLaunch* t = &ctx->timed[ctx->timedUsed++];
   t->cls = cls;
-  hipEventRecord(t->a, ctx->stream);
-  return t;
-}
-inline void timedEnd(esf_ctx* ctx, esf_ctx::TimedLaunch* t) {
-  if (t) hipEventRecord(t->b, ctx->stream);
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t->a, t->b, 0, args...);
 }
 
 }  // namespace esf

@@ -14,7 +14,7 @@ __global__ void __launch_bounds__(256) kCrDirection(long n, T* p, T* Ap, const T
                                                     const T* s, const T* D,
                                                     const CrScalars* sc, double* partials) {
   if (sc->done) return;
-  __shared__ double red[8];
+  __shared__ double red[16];
   const T beta = (T)sc->beta;
   double part = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;

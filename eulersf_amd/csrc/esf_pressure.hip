@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
   if (MODE != M_APPLY && a.sc && a.sc->done) return;
   __shared__ T xs[2][HALO];
   __shared__ T ws[2][NODES * 3];
-  __shared__ double red[8];
+  __shared__ double red[16];
 
   // XCD-aware tile order: blocks are dealt round-robin to the 8 XCDs, so
   // give each XCD a contiguous run of tiles (shared halos stay in one L2).
@@ -327,10 +327,8 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   int tx, ty, tz;
   tileShape(ctx->d, tx, ty, tz);
   const int nb = tx * ty * tz;
-  auto* tl = timedBegin(ctx, MODE + (sizeof(T) == 4 ? 6 : 0));
-  hipLaunchKernelGGL((kSpmv<T, MODE>), dim3(nb), dim3(256), 0, ctx->stream, ctx->d, a,
-                     tx, ty, tz);
-  timedEnd(ctx, tl);
+  launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE>, dim3(nb), dim3(256),
+              ctx->d, a, tx, ty, tz);
   // the grid-wide sums of the mode (kSpmv's epilogue partials)
   if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
   if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
@@ -494,11 +492,9 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     a.in = rA;
     a.r = rB;
     if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
-    auto* tl = timedBegin(ctx, sizeof(T) == 4 ? 13 : 12);
-    hipLaunchKernelGGL(kCrDirection<T>, dim3(dirBlocks), dim3(256), 0, ctx->stream,
-                       (long)d.cells, p, Ap, (const T*)rB, (const T*)s, D,
-                       (const CrScalars*)a.sc, ctx->partials);
-    timedEnd(ctx, tl);
+    launchTimed(ctx, sizeof(T) == 4 ? 13 : 12, kCrDirection<T>, dim3(dirBlocks), dim3(256),
+                (long)d.cells, p, Ap, (const T*)rB, (const T*)s, (const T*)D,
+                (const CrScalars*)a.sc, ctx->partials);
     launchFinish<F_DIR, 1>(ctx, dirBlocks, a.sc);
     std::swap(rA, rB);
     if (i > 0 && i % 5 == 0) {

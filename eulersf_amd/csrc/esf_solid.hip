@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, int cap,
                                                double mu, double* cellMat, double* cellOut,
                                                double* cellDiag, int* cornerDof,
                                                double* energyPart) {
-  __shared__ double red[8];
+  __shared__ double red[16];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double en = 0;
   if (e < nValid) {
@@ -138,7 +138,7 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
                              const double* __restrict__ cellDiag,
                              const double* __restrict__ smass, const double* __restrict__ svel,
                              Box box, int first, double* sForce, double* sKdiag, double* sMv,
-                             uint8_t* nodeMask) {
+                             uint8_t* nodeMaskByte) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
@@ -167,7 +167,7 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
           nk++;
         }
       }
-  nodeMask[s] = (uint8_t)mask;
+  nodeMaskByte[4L * s] = (uint8_t)mask;
   double mv[3] = {0, 0, 0};
   const bool inBox = x >= box.bMin[0] && x < box.bMax[0] - 1 && y >= box.bMin[1] &&
                      y < box.bMax[1] - 1 && z >= box.bMin[2] && z < box.bMax[2] - 1;
@@ -324,6 +324,9 @@ __global__ void __launch_bounds__(256) kSolidCell8(Dims d, int nValid, int cap,
       dd[j] = diag[k + j];
     }
   }
+  // every load of the pass is issued above; keep the compiler from sinking
+  // any of them below the reductions (emits no instruction)
+  asm volatile("" ::: "memory");
   double v[3];
   if (MODE == CELL_CR) {
     double alpha, beta;
@@ -373,8 +376,8 @@ enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3 };
 struct SolidNodeArgs {
   int nSolid;
   int nSolidBodies;       // up to 4 solids gathered in one launch
-  const double* inbox[4]; // per solid: 24 doubles per solid node (see kSolidCell8)
-  const uint8_t* nodeMask;// [b capS + s]: bit c = the cell at offset c has rows here
+  const double* inbox;    // solid b: inbox + 24 capS b, 24 doubles per node (kSolidCell8)
+  const uint32_t* nodeMask; // per node, byte b: bit c = solid b's cell at offset c has rows here
   int capS;
   double scale;           // dt^2 + beta dt
   const double* damped;   // diag mass
@@ -404,7 +407,7 @@ struct SolidNodeArgs {
 // afterwards), so a pass costs one memory round trip.
 template <int MODE>
 __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
-  __shared__ double red[8];
+  __shared__ double red[16];
   __shared__ double grp[5 * kScalMaxGroups];
   const long nDof = 3L * a.nSolid;
   if (nDof == 0) return;
@@ -415,18 +418,16 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   const int q = (int)(k - 3 * sn);
   ScalarLoads SL;
   if (MODE == S_CR) scalarLoads(a.sp.cur, a.sp.prev, a.sp.nb, threadIdx.x >> 6, SL);
-  unsigned mask[4];
+  const uint32_t mask = a.nodeMask[sn];
   double rows[4][8];
+  const double* in0 = a.inbox + 24 * sn + q;
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    mask[b] = 0;
-    if (b < a.nSolidBodies) {
-      mask[b] = a.nodeMask[(long)b * a.capS + sn];
-      const double* in = a.inbox[b] + 24 * sn + q;
+  for (int c = 0; c < 8; c++) rows[0][c] = in0[3 * c];
 #pragma unroll
-      for (int c = 0; c < 8; c++) rows[b][c] = in[3 * c];
-    }
-  }
+  for (int b = 1; b < 4; b++)
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      rows[b][c] = b < a.nSolidBodies ? in0[24L * a.capS * b + 3 * c] : 0.0;
   const double dm = a.damped[k];
   double vin = 0, dg = 1, ap0 = 0, s0 = 0, p0 = 0, r0 = 0, x0 = 0, bb = 0;
   if (MODE == S_CR) {
@@ -441,12 +442,15 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
     if (MODE == S_SETUP) dg = a.diag[k];
     if (MODE == S_CHECK) bb = a.b[k];
   }
+  // every load of the pass is issued above; keep the compiler from sinking
+  // any of them below the reductions (emits no instruction)
+  asm volatile("" ::: "memory");
   double kx = 0;
 #pragma unroll
   for (int b = 0; b < 4; b++)
 #pragma unroll
     for (int c = 0; c < 8; c++)
-      if (b < a.nSolidBodies && (mask[b] >> c & 1u)) kx += rows[b][c];
+      if (mask >> (8 * b + c) & 1u) kx += rows[b][c];
   double alpha = 0, beta = 0;
   if (MODE == S_CR) {
     if (scalarFinish(a.sp.cur, a.sp.prev, a.sp.nb, a.sp.first, a.sc, SL, grp, alpha, beta))
@@ -526,7 +530,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
                          ctx->stream, d, ctx->nSolid, so.capValid, ctx->solidNodes, so.cellSlot,
                          so.cellOut,
                          so.cellDiag, so.mass, so.vel, so.box, b == 0 ? 1 : 0, ctx->sForce,
-                         ctx->sKdiag, ctx->sMv, ctx->nodeMask + (long)b * ctx->capSolid);
+                         ctx->sKdiag, ctx->sMv, (uint8_t*)ctx->nodeMask + b);
   }
   if (energy) *energy = en;
   hipError_t e = hipGetLastError();
@@ -540,7 +544,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
 static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
   a.nSolid = ctx->nSolid;
   a.nSolidBodies = (int)std::min<size_t>(ctx->solids.size(), 4);
-  for (int b = 0; b < a.nSolidBodies; b++) a.inbox[b] = ctx->inbox[b];
+  a.inbox = ctx->inbox;
   a.nodeMask = ctx->nodeMask;
   a.capS = ctx->capSolid;
   a.scale = ctx->dt * ctx->dt + ctx->p.rayleigh_beta * ctx->dt;
@@ -556,13 +560,11 @@ static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
   for (size_t b = 0; b < ctx->solids.size(); b++) {
     DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
-      auto* tl = MODE == CELL_CR ? timedBegin(ctx, 14) : nullptr;
-      hipLaunchKernelGGL((kSolidCell8<MODE>), dim3(cdiv(8L * so.nValid, 256)), dim3(256), 0,
-                         ctx->stream, ctx->d, so.nValid, so.capValid, (const int*)so.cornerDof,
-                         (const double*)so.cellMat, in, (const double*)ctx->ss,
-                         (const double*)ctx->sAp, (const double*)ctx->sAdiag, sc, sp,
-                         so.lambda, so.mu, ctx->inbox[b]);
-      timedEnd(ctx, tl);
+      launchTimed(ctx, MODE == CELL_CR ? 14 : -1, kSolidCell8<MODE>,
+                  dim3(cdiv(8L * so.nValid, 256)), dim3(256), ctx->d, so.nValid, so.capValid,
+                  (const int*)so.cornerDof, (const double*)so.cellMat, in,
+                  (const double*)ctx->ss, (const double*)ctx->sAp, (const double*)ctx->sAdiag,
+                  sc, sp, so.lambda, so.mu, ctx->inbox + 24L * ctx->capSolid * b);
     }
   }
 }
@@ -571,10 +573,8 @@ static int nodeBlocks(const esf_ctx* ctx) { return cdiv(std::max(3L * ctx->nSoli
 
 template <int MODE>
 static void nodePass(esf_ctx* ctx, SolidNodeArgs& a) {
-  auto* tl = MODE == S_CR ? timedBegin(ctx, 15) : nullptr;
-  hipLaunchKernelGGL((kSolidNode<MODE>), dim3(nodeBlocks(ctx)), dim3(256), 0, ctx->stream,
-                     ctx->d, a);
-  timedEnd(ctx, tl);
+  launchTimed(ctx, MODE == S_CR ? 15 : -1, kSolidNode<MODE>, dim3(nodeBlocks(ctx)), dim3(256),
+              ctx->d, a);
 }
 
 esf_status solidAdiag(esf_ctx* ctx) {
@@ -615,7 +615,7 @@ esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
 // Only the rounding of alpha differs from the reference's evaluation.
 __global__ void kSolidSetup(long n, const double* b, const double* diag, double* x, double* r,
                             double* p, double* Ap, double* partials) {
-  __shared__ double red[8];
+  __shared__ double red[16];
   double part = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
