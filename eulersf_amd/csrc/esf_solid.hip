@@ -268,41 +268,55 @@ struct SolidPartials {
   int first;          // iteration 0: no previous direction, beta = 0
 };
 
-// K_e v_e per valid cell, eight lanes per cell (lane c = corner c of cell
-// e = thread / 8).  Each lane forms its corner's share v_c (GF)_c^T of dF,
-// a 3-step DPP sum adds the shares (sum8: all eight lanes hold the
-// bitwise-same dF), every lane evaluates dP(dF) and writes its corner's
-// three rows of K_e v straight into the inbox of the corner's node (24
-// doubles per solid node: 3 rows for each of its 8 cells), so the node
-// pass reads them without an index table.  Masked corners write nothing.
+// K_e v_e per valid cell, LPC lanes per cell (cell e = thread / LPC); lane l
+// of the group takes corners l, l + LPC, ... (8 / LPC of them).  Each lane
+// forms its corners' shares v_c (GF)_c^T of dF, a DPP sum over the group
+// adds the shares (sumGroup: every lane of the group holds the bitwise-same
+// dF), every lane evaluates dP(dF) and writes its corners' three rows of
+// K_e v straight into the inbox of the corner's node (24 doubles per solid
+// node: 3 rows for each of its 8 cells), so the node pass reads them
+// without an index table.  Masked corners write nothing.
 //   CELL_IN : v = in                      (apply, residual check)
 //   CELL_CR : v = r - alpha (Ap beta + s) / D, the CR residual update of
 //             CONJUGATE_RESIDUAL.inl:223-230 with the direction update of
 //             :257-262 folded in; the node kernel forms the same values
 enum CellMode { CELL_IN = 0, CELL_CR = 1 };
+constexpr int kCellLanes = 2;  // lanes per cell of the solid cell pass
 
-template <int MODE>
-__global__ void __launch_bounds__(256) kSolidCell8(Dims d, int nValid, int cap,
-                                                   const int* __restrict__ cornerDof,
-                                                   const double* __restrict__ cellMat,
-                                                   const double* __restrict__ in,
-                                                   const double* __restrict__ s,
-                                                   const double* __restrict__ Ap,
-                                                   const double* __restrict__ diag,
-                                                   const CrScalars* sc, SolidPartials sp,
-                                                   double lambda, double mu,
-                                                   double* __restrict__ inbox) {
+template <int LPC>
+__device__ inline double sumGroup(double v) {
+  if constexpr (LPC == 8) return sum8(v);
+  v = v + dppMov<0xB1>(v);
+  if constexpr (LPC == 4) v = v + dppMov<0x4E>(v);
+  return v;
+}
+
+template <int MODE, int LPC>
+__global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
+                                                  const int* __restrict__ cornerDof,
+                                                  const double* __restrict__ cellMat,
+                                                  const double* __restrict__ in,
+                                                  const double* __restrict__ s,
+                                                  const double* __restrict__ Ap,
+                                                  const double* __restrict__ diag,
+                                                  const CrScalars* sc, SolidPartials sp,
+                                                  double lambda, double mu,
+                                                  double* __restrict__ inbox) {
+  static_assert(LPC == 2 || LPC == 4 || LPC == 8, "lanes per cell");
+  constexpr int CPL = 8 / LPC;  // corners per lane
   __shared__ double grp[5 * kScalMaxGroups];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e = t >> 3, c = t & 7;
+  const int e = t / LPC, l = t % LPC;
   const bool live = e < nValid;
   // first round of loads, all unconditional (indices clamped in bounds,
   // results masked afterwards) so nothing waits for a branch: the CR
-  // scalars' partials, the corner's DOF index and the cell's (R, L, F, 1/J)
+  // scalars' partials, the corners' DOF indices and the cell's (R, L, F, 1/J)
   ScalarLoads SL;
   if (MODE == CELL_CR) scalarLoads(sp.cur, sp.prev, sp.nb, threadIdx.x >> 6, SL);
   const int ec = live ? e : nValid - 1;
-  const int kraw = cornerDof[(long)c * cap + ec];
+  int kraw[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; j++) kraw[j] = cornerDof[(long)(l + LPC * j) * cap + ec];
   const double* m = cellMat + ec;
   double F[9], R[9], L[9];
 #pragma unroll
@@ -312,62 +326,79 @@ __global__ void __launch_bounds__(256) kSolidCell8(Dims d, int nValid, int cap,
     L[q] = m[(long)(9 + q) * cap];
   }
   const double Jinv = m[27L * cap];
-  // second round: the corner's r (or v), s, Ap, D
-  const int k = kraw >= 0 ? kraw : 0;
-  double rr[3], ss[3] = {0, 0, 0}, aa[3] = {0, 0, 0}, dd[3] = {1, 1, 1};
+  // second round: the corners' r (or v), s, Ap, D
+  double rr[CPL][3], ss[CPL][3], aa[CPL][3], dd[CPL][3];
 #pragma unroll
-  for (int j = 0; j < 3; j++) {
-    rr[j] = in[k + j];
-    if (MODE == CELL_CR) {
-      ss[j] = s[k + j];
-      aa[j] = Ap[k + j];
-      dd[j] = diag[k + j];
+  for (int j = 0; j < CPL; j++) {
+    const int k = kraw[j] >= 0 ? kraw[j] : 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      rr[j][q] = in[k + q];
+      ss[j][q] = 0;
+      aa[j][q] = 0;
+      dd[j][q] = 1;
+      if (MODE == CELL_CR) {
+        ss[j][q] = s[k + q];
+        aa[j][q] = Ap[k + q];
+        dd[j][q] = diag[k + q];
+      }
     }
   }
   // every load of the pass is issued above; keep the compiler from sinking
   // any of them below the reductions (emits no instruction)
   asm volatile("" ::: "memory");
-  double v[3];
+  double v[CPL][3];
   if (MODE == CELL_CR) {
     double alpha, beta;
     if (scalarFinish(sp.cur, sp.prev, sp.nb, sp.first, sc, SL, grp, alpha, beta)) return;
 #pragma unroll
-    for (int j = 0; j < 3; j++) v[j] = rr[j] - alpha * ((aa[j] * beta + ss[j]) / dd[j]);
+    for (int j = 0; j < CPL; j++)
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+        v[j][q] = rr[j][q] - alpha * ((aa[j][q] * beta + ss[j][q]) / dd[j][q]);
   } else {
 #pragma unroll
-    for (int j = 0; j < 3; j++) v[j] = rr[j];
-  }
-  if (kraw < 0)
+    for (int j = 0; j < CPL; j++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) v[j] = 0.0;
-  // whole groups of eight leave together, so the butterfly below only
-  // ever reads lanes of a live group
+      for (int q = 0; q < 3; q++) v[j][q] = rr[j][q];
+  }
+  // whole groups leave together, so the group sum below only ever reads
+  // lanes of a live group
   if (!live) return;
   const double g = 0.25 * d.dhInv;
-  const double G0 = sgn(c & 1) * g, G1 = sgn((c >> 1) & 1) * g, G2 = sgn(c >> 2) * g;
-  double gf[3];
+  double gf[CPL][3];
+  double dF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int q = 0; q < 3; q++) gf[q] = (G0 * F[q] + G1 * F[3 + q]) + G2 * F[6 + q];
-  double dF[9];
+  for (int j = 0; j < CPL; j++) {
+    const int c = l + LPC * j;
+    const double G0 = sgn(c & 1) * g, G1 = sgn((c >> 1) & 1) * g, G2 = sgn(c >> 2) * g;
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
-    dF[q] = v[0] * gf[q];
-    dF[3 + q] = v[1] * gf[q];
-    dF[6 + q] = v[2] * gf[q];
+    for (int q = 0; q < 3; q++) gf[j][q] = (G0 * F[q] + G1 * F[3 + q]) + G2 * F[6 + q];
+    if (kraw[j] >= 0)
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        dF[q] += v[j][0] * gf[j][q];
+        dF[3 + q] += v[j][1] * gf[j][q];
+        dF[6 + q] += v[j][2] * gf[j][q];
+      }
   }
 #pragma unroll
-  for (int q = 0; q < 9; q++) dF[q] = sum8(dF[q]);
+  for (int q = 0; q < 9; q++) dF[q] = sumGroup<LPC>(dF[q]);
   double dP[9];
   corotDP(R, L, lambda, mu, dF, dP);
-  if (kraw < 0) return;
   const double dh = d.dh;
-  // the corner's rows go to its node's inbox: node s = kraw / 3 keeps 8
-  // rows of 3, row 7 - c for the cell at offset 7 - c from it
-  double* out = inbox + 8L * kraw + 3 * (7 - c);
 #pragma unroll
-  for (int r = 0; r < 3; r++) {
-    double y = (dP[3 * r] * gf[0] + dP[3 * r + 1] * gf[1]) + dP[3 * r + 2] * gf[2];
-    out[r] = (((y * Jinv) * dh) * dh) * dh;
+  for (int j = 0; j < CPL; j++) {
+    if (kraw[j] < 0) continue;
+    const int c = l + LPC * j;
+    // the corner's rows go to its node's inbox: node s = kraw / 3 keeps 8
+    // rows of 3, row 7 - c for the cell at offset 7 - c from it
+    double* out = inbox + 8L * kraw[j] + 3 * (7 - c);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      double y = (dP[3 * r] * gf[j][0] + dP[3 * r + 1] * gf[j][1]) + dP[3 * r + 2] * gf[j][2];
+      out[r] = (((y * Jinv) * dh) * dh) * dh;
+    }
   }
 }
 
@@ -376,7 +407,7 @@ enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3 };
 struct SolidNodeArgs {
   int nSolid;
   int nSolidBodies;       // up to 4 solids gathered in one launch
-  const double* inbox;    // solid b: inbox + 24 capS b, 24 doubles per node (kSolidCell8)
+  const double* inbox;    // solid b: inbox + 24 capS b, 24 doubles per node (kSolidCell)
   const uint32_t* nodeMask; // per node, byte b: bit c = solid b's cell at offset c has rows here
   int capS;
   double scale;           // dt^2 + beta dt
@@ -560,8 +591,9 @@ static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
   for (size_t b = 0; b < ctx->solids.size(); b++) {
     DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
-      launchTimed(ctx, MODE == CELL_CR ? 14 : -1, kSolidCell8<MODE>,
-                  dim3(cdiv(8L * so.nValid, 256)), dim3(256), ctx->d, so.nValid, so.capValid,
+      launchTimed(ctx, MODE == CELL_CR ? 14 : -1, kSolidCell<MODE, kCellLanes>,
+                  dim3(cdiv((long)kCellLanes * so.nValid, 256)), dim3(256), ctx->d, so.nValid,
+                  so.capValid,
                   (const int*)so.cornerDof, (const double*)so.cellMat, in,
                   (const double*)ctx->ss, (const double*)ctx->sAp, (const double*)ctx->sAdiag,
                   sc, sp, so.lambda, so.mu, ctx->inbox + 24L * ctx->capSolid * b);
