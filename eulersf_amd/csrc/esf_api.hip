@@ -13,8 +13,7 @@ esf_status pressureRhs(esf_ctx* ctx);
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D, T* x, T* r,
-                   T* r2, T* p, T* s, T* Ap, double eps, int maxIt, int* iters, bool* converged,
-                   double* relres);
+                   T* r2, T* p, T* s, T* Ap, double eps, int maxIt, int slot);
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
                       double* relres, int* refinements, int* innerIters);
 int spmvBlocks(const Dims& d);
@@ -25,7 +24,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy);
 esf_status solidAdiag(esf_ctx* ctx);
 esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y);
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps, int maxIt,
-                      int* iters, bool* converged, double* relres);
+                      int slot);
 __global__ void kCorotBatch(int n, double lambda, double mu, const double* F, const double* dF,
                             double* out, double* energy, double* dP);
 // esf_grid.hip
@@ -224,7 +223,8 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = growSolidDofs(ctx, 0);
   if (!st && hipHostMalloc((void**)&ctx->hostFlags, 64 * sizeof(int)) != hipSuccess)
     st = ESF_ERR_OOM;
-  if (!st && hipHostMalloc((void**)&ctx->hostSc, sizeof(CrScalars)) != hipSuccess)
+  if (!st && hipHostMalloc((void**)&ctx->hostSc, esf_ctx::kReportSlots * sizeof(CrScalars)) !=
+                 hipSuccess)
     st = ESF_ERR_OOM;
   if (!st) fillInt(ctx, N, ctx->sidx, -1);
   if (!st && hipStreamSynchronize(ctx->stream) != hipSuccess) st = ESF_ERR_DEVICE;
@@ -506,19 +506,19 @@ esf_status esf_forces(esf_ctx* ctx, double* energy) {
   return ESF_OK;
 }
 
-esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats) {
-  if (!ctx) return ESF_ERR_INVALID;
-  esf_solve_stats s{};
+// stepImplicit :609-632 + pcrImplicitSolve (:662-764), queued on the
+// stream.  The FP64 solves leave their reports in the pinned slots 0-2
+// (read by coupledSolveReport after the next synchronisation); the mixed
+// pressure solve synchronises per refinement and fills *s itself.
+static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
   esf_status st;
-  bool conv[3] = {true, true, true};
   const int maxIt = ctx->p.cr_max_iterations;
+  *s = esf_solve_stats{};
   {
     StageTimer t(ctx, 2);
     if ((st = solidAdiag(ctx))) return st;
     if ((st = solidRhs(ctx))) return st;
-    if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, ctx->p.implicit_eps, maxIt,
-                         &s.iterations[0], &conv[0], &s.rel_residual[0])))
-      return st;
+    if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, ctx->p.implicit_eps, maxIt, 0))) return st;
     if ((st = scatterSolidV0(ctx))) return st;
   }
   {
@@ -529,39 +529,60 @@ esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats) {
       ctx->lambdaInit = true;
     }
     if (ctx->p.precision == ESF_MIXED) {
-      st = mixedSolve(ctx, ctx->p.divfree_eps, maxIt, &s.iterations[1], &conv[1],
-                      &s.rel_residual[1], &s.refinements, &s.inner_iterations);
+      bool conv = false;
+      st = mixedSolve(ctx, ctx->p.divfree_eps, maxIt, &s->iterations[1], &conv,
+                      &s->rel_residual[1], &s->refinements, &s->inner_iterations);
+      s->converged[1] = conv;
     } else {
       st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
-                           ctx->pdir, ctx->s, ctx->Ap, ctx->p.divfree_eps, maxIt,
-                           &s.iterations[1], &conv[1], &s.rel_residual[1]);
+                           ctx->pdir, ctx->s, ctx->Ap, ctx->p.divfree_eps, maxIt, 1);
     }
     if (st) return st;
     if ((st = correctVelocity(ctx))) return st;
   }
   {
     StageTimer t(ctx, 2);
-    if ((st = solidSolve(ctx, 2, ctx->sB, ctx->sDv, ctx->p.implicit_eps, maxIt,
-                         &s.iterations[2], &conv[2], &s.rel_residual[2])))
-      return st;
+    if ((st = solidSolve(ctx, 2, ctx->sB, ctx->sDv, ctx->p.implicit_eps, maxIt, 2))) return st;
     if ((st = solidVelocity(ctx))) return st;
   }
-  for (int k = 0; k < 3; k++) s.converged[k] = conv[k];
-  s.dt = ctx->dt;
-  s.n_solid_nodes = ctx->nSolid;
-  for (auto& so : ctx->solids) s.n_solid_cells += so.nValid;
-  if (stats) *stats = s;
+  return ESF_OK;
+}
+
+// after a synchronisation of the stream: the reports of the queued solves,
+// and the status (NaN residual, not converged)
+static esf_status coupledSolveReport(esf_ctx* ctx, esf_solve_stats* s) {
+  const int maxIt = ctx->p.cr_max_iterations;
+  for (int k = 0; k < 3; k++) {
+    if (k == 1 && ctx->p.precision == ESF_MIXED) continue;
+    bool conv = false;
+    solveReport(ctx->hostSc[k], maxIt, &s->iterations[k], &conv, &s->rel_residual[k]);
+    s->converged[k] = conv;
+  }
+  s->dt = ctx->dt;
+  s->n_solid_nodes = ctx->nSolid;
+  for (auto& so : ctx->solids) s->n_solid_cells += so.nValid;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("coupled solve: ") + hipGetErrorString(e);
     return ESF_ERR_DEVICE;
   }
-  for (double r : s.rel_residual)
+  for (double r : s->rel_residual)
     if (std::isnan(r)) {
       ctx->err = "NaN residual in the coupled solve";
       return ESF_ERR_NAN;
     }
-  return (conv[0] && conv[1] && conv[2]) ? ESF_OK : ESF_NOT_CONVERGED;
+  return (s->converged[0] && s->converged[1] && s->converged[2]) ? ESF_OK : ESF_NOT_CONVERGED;
+}
+
+esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats) {
+  if (!ctx) return ESF_ERR_INVALID;
+  esf_solve_stats s{};
+  esf_status st = coupledSolveEnqueue(ctx, &s);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  st = coupledSolveReport(ctx, &s);
+  if (stats) *stats = s;
+  return st;
 }
 
 esf_status esf_solid_copy_velocity(esf_ctx* ctx) {
@@ -596,14 +617,19 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   esf_status st, solveSt;
   if ((st = esf_cfl(ctx, alpha < 0 ? 1.8 : alpha, &dt))) return st;
   if ((st = esf_forces(ctx, nullptr))) return st;
-  solveSt = esf_coupled_solve(ctx, stats);
-  if (solveSt < 0) return solveSt;
+  // the solve's reports are read at the end of the step: no stall between
+  // the solve and the stages after it
+  esf_solve_stats s{};
+  if ((st = coupledSolveEnqueue(ctx, &s))) return st;
   if ((st = esf_solid_copy_velocity(ctx))) return st;
   if ((st = esf_solid_finalize(ctx))) return st;
   if ((st = esf_gather_solid_masses(ctx))) return st;
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  solveSt = coupledSolveReport(ctx, &s);
+  if (stats) *stats = s;
   return solveSt;
 }
 
@@ -649,10 +675,11 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double e
   bool conv = false;
   double rr = 0;
   esf_status st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r,
-                                  ctx->r2, ctx->pdir, ctx->s, ctx->Ap, eps, maxIt, &it, &conv, &rr);
+                                  ctx->r2, ctx->pdir, ctx->s, ctx->Ap, eps, maxIt, 3);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(x, ctx->lambda, bytes, hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipStreamSynchronize(ctx->stream));
+  solveReport(ctx->hostSc[3], maxIt, &it, &conv, &rr);
   if (iters) *iters = it;
   if (relres) *relres = rr;
   return conv ? ESF_OK : ESF_NOT_CONVERGED;

@@ -111,7 +111,11 @@ struct esf_ctx {
   esf::CrScalars* sc = nullptr;   // device scalars (several solves)
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
   int* hostFlags = nullptr;       // pinned ring of done flags
-  esf::CrScalars* hostSc = nullptr;  // pinned copy
+  // pinned copies of the Krylov scalars, one slot per solve of a step (0:
+  // solid 1, 1: pressure, 2: solid 2, 3: scratch, 4: inner FP32); a solve
+  // queues its copy and the host reads it after the step's next sync
+  static constexpr int kReportSlots = 8;
+  esf::CrScalars* hostSc = nullptr;
   int nPartials = 0;
   std::vector<esf::DevSolid> solids;
   // timing
@@ -254,6 +258,14 @@ __device__ inline void finishSums(int nb, const double* partials, double* lds,
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// what a finished Krylov solve reports, from its scalar block
+inline void solveReport(const CrScalars& h, int maxIt, int* iters, bool* converged,
+                        double* relres) {
+  *converged = h.done == 1;
+  *iters = h.done ? h.iter : maxIt;
+  *relres = h.bnorm > 0 ? h.resnorm / h.bnorm : h.resnorm;
+}
 
 // kernel classes for esf_kernel_timing: SpMV modes 0-5 (FP64), 6-11 (FP32),
 // CR direction 12 (FP64) / 13 (FP32), solid cell pass 14, solid node pass 15

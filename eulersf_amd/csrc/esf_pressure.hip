@@ -448,11 +448,12 @@ static esf_status initScalars(esf_ctx* ctx, int which, double eps) {
   return ESF_OK;
 }
 
+// synchronous read of scalar block `which` (through report slot 3)
 static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
-  ESF_HIP(hipMemcpyAsync(ctx->hostSc, &ctx->sc[which], sizeof(CrScalars),
+  ESF_HIP(hipMemcpyAsync(&ctx->hostSc[3], &ctx->sc[which], sizeof(CrScalars),
                          hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipStreamSynchronize(ctx->stream));
-  *out = *ctx->hostSc;
+  *out = ctx->hostSc[3];
   return ESF_OK;
 }
 
@@ -461,8 +462,7 @@ static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
 // reference's iteration count; *relres the last checked |Sx-b|/|b|.
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
-                   T* x, T* r, T* r2, T* p, T* s, T* Ap, double eps, int maxIt,
-                   int* iters, bool* converged, double* relres) {
+                   T* x, T* r, T* r2, T* p, T* s, T* Ap, double eps, int maxIt, int slot) {
   esf_status st = initScalars(ctx, which, eps);
   if (st) return st;
   const Dims& d = ctx->d;
@@ -503,20 +503,19 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
       if (poll.mark()) break;
     }
   }
-  CrScalars h;
-  if ((st = readScalars(ctx, which, &h))) return st;
-  *converged = h.done == 1;
-  *iters = h.done ? h.iter : maxIt;
-  *relres = h.bnorm > 0 ? h.resnorm / h.bnorm : h.resnorm;
+  // the report is copied without waiting; the caller reads report slot
+  // `slot` after its next synchronisation
+  ESF_HIP(hipMemcpyAsync(&ctx->hostSc[slot], &ctx->sc[which], sizeof(CrScalars),
+                         hipMemcpyDeviceToHost, ctx->stream));
   return ESF_OK;
 }
 
 template esf_status crSolve<double>(esf_ctx*, int, const double*, const double*,
                                     const double*, double*, double*, double*, double*,
-                                    double*, double*, double, int, int*, bool*, double*);
+                                    double*, double*, double, int, int);
 template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*,
                                    const float*, float*, float*, float*, float*, float*,
-                                   float*, double, int, int*, bool*, double*);
+                                   float*, double, int, int);
 
 __global__ void kZeroF(long n, float* a) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
@@ -555,14 +554,10 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     double need = h.resnorm > 0 ? 0.3 * eps * h.bnorm / h.resnorm : 1.0;
     double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
     hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
-    int it = 0;
-    bool conv = false;
-    double rr = 0;
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->e32, ctx->r32,
                              ctx->r32b, ctx->p32, ctx->s32, ctx->Ap32, innerEps,
-                             maxIt - inner, &it, &conv, &rr)))
+                             maxIt - inner, 4)))
       return st;
-    inner += it;
     outer++;
     o.e32 = ctx->e32;
     o.first = 0;
@@ -570,7 +565,12 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     o.x = ctx->lambda2;
     if ((st = launchSpmv<double, M_IR_RESID>(ctx, o))) return st;
     std::swap(ctx->lambda, ctx->lambda2);
-    if ((st = readScalars(ctx, 0, &h))) return st;
+    if ((st = readScalars(ctx, 0, &h))) return st;  // also completes report slot 4
+    int it = 0;
+    bool conv = false;
+    double rr = 0;
+    solveReport(ctx->hostSc[4], maxIt - inner, &it, &conv, &rr);
+    inner += it;
   }
   *converged = h.done == 1;
   *iters = inner;

@@ -662,16 +662,18 @@ __global__ void kSolidSetup(long n, const double* b, const double* diag, double*
 }
 
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps,
-                      int maxIt, int* iters, bool* converged, double* relres) {
+                      int maxIt, int slot) {
   if (ctx->solids.size() > 4) {
     ctx->err = "at most 4 solids per context";
     return ESF_ERR_INVALID;
   }
   const long n = 3L * ctx->nSolid;
   if (n == 0) {
-    *iters = 0;
-    *converged = true;
-    *relres = 0;
+    // nothing queued for this slot: report "converged after 0 iterations"
+    ESF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->hostSc[slot] = CrScalars{};
+    ctx->hostSc[slot].done = 1;
+    ctx->hostSc[slot].iter = 0;
     return ESF_OK;
   }
   if (nodeBlocks(ctx) > 64 * kScalMaxGroups) {
@@ -722,12 +724,10 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
       if (poll.mark()) break;
     }
   }
-  ESF_HIP(hipMemcpyAsync(ctx->hostSc, sc, sizeof(CrScalars), hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
-  const CrScalars& h = *ctx->hostSc;
-  *converged = h.done == 1;
-  *iters = h.done ? h.iter : maxIt;
-  *relres = h.bnorm > 0 ? h.resnorm / h.bnorm : h.resnorm;
+  // the report is copied without waiting; the caller reads report slot
+  // `slot` after its next synchronisation
+  ESF_HIP(hipMemcpyAsync(&ctx->hostSc[slot], sc, sizeof(CrScalars), hipMemcpyDeviceToHost,
+                         ctx->stream));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("solid solve: ") + hipGetErrorString(e);
