@@ -9,6 +9,8 @@ A line without ``=`` that is not blank is a parse error.
 import math
 import re
 
+import numpy as np
+
 _FLOAT_PREFIX = re.compile(r"^\s*[+-]?((\d+\.?\d*)|(\.\d+))([eE][+-]?\d+)?")
 
 
@@ -96,36 +98,48 @@ def lame_from_young(E, nu):
     return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
 
 
-def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
-                       center=(0.5, 0.65, 0.5), E=1e4, nu=0.3, density=1000.0,
-                       fluid_density=1000.0, carpet=0.3, dt=1.0 / 240.0,
-                       vorticity_eps=1.5, gravity=9.8, frames=250, velocity=(0, 0, 0)):
-    """cfg text of the unit-cube ball scene on an n^3-cell grid: (n+1)^3 nodes,
-    dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool."""
-    lam, mu = lame_from_young(E, nu)
+def mass_sample_dh(n):
+    """sample spacing of the solid mass rasterisation on an n^3-cell unit
+    cube: dh / k with k = min(8, floor(512 dh)) samples per cell edge, so
+    dh / sdh is an exact integer (the reference truncates it) and
+    density * sdh^3 stays above the reference's absolute 1e-6 sample cut
+    (DEFOR_SOLID_3D.cpp:629-630) for density 1000"""
     dh = 1.0 / n
-    # The reference drops mass samples lighter than 1e-6 (absolute,
-    # DEFOR_SOLID_3D.cpp:629-630): density * sdh^3 must stay above it, so
-    # the sample spacing is dh/8 but never finer than 1/512 (the shipped
-    # cfgs use 0.00125).  Both are exact binary fractions for n = 2^k.
-    sdh = max(dh / 8, 1.0 / 512)
+    k = max(1, min(8, int(512 * dh)))
+    return dh / k
+
+
+def scene_cfg(n, spheres, fill=0.3, eps=1e-10, max_iterations=20000, fluid_density=1000.0,
+              dt=1.0 / 240.0, vorticity_eps=1.5, gravity=9.8, frames=250):
+    """cfg text of a unit-cube scene on an n^3-cell grid ((n+1)^3 nodes,
+    dh = 1/n): the smoke carpet (fluid pool) fills the bottom `fill` of the
+    domain; `spheres` is a list of dicts with keys center, radius, E, nu,
+    density, velocity (corotational spheres rasterised analytically)."""
+    dh = 1.0 / n
+    sdh = mass_sample_dh(n)
     lines = [
-        "# unit-cube rubber ball, %d^3 cells" % n,
+        "# unit-cube scene, %d^3 cells, %d sphere(s)" % (n, len(spheres)),
         "xRes = %d" % (n + 1), "yRes = %d" % (n + 1), "zRes = %d" % (n + 1),
         "dh = %r" % dh, "origin = 0,0,0",
-        "number of solids = 1",
-        "solid path 0 = ./", "solid name 0 = analytic_sphere",
-        "solid sphere radius 0 = %r" % radius,
-        "sphere mass dh 0 = %r" % sdh,
-        "solid mass density 0 = %r" % density,
-        "material lambda 0 = %r" % lam, "material mu 0 = %r" % mu,
-        "solid plastic yield 0 = -1",
-        "solid initial velocity 0 = %r, %r, %r" % tuple(velocity),
-        "solid translation 0 = %r, %r, %r" % tuple(center),
-        "solid scaling 0 = 1,1,1", "solid rotation 0 = 0,0,0",
+        "number of solids = %d" % len(spheres),
+    ]
+    for i, sp in enumerate(spheres):
+        lam, mu = lame_from_young(sp.get("E", 1e4), sp.get("nu", 0.3))
+        lines += [
+            "solid path %d = ./" % i, "solid name %d = analytic_sphere" % i,
+            "solid sphere radius %d = %r" % (i, float(sp["radius"])),
+            "sphere mass dh %d = %r" % (i, sdh),
+            "solid mass density %d = %r" % (i, float(sp.get("density", 1000.0))),
+            "material lambda %d = %r" % (i, lam), "material mu %d = %r" % (i, mu),
+            "solid plastic yield %d = -1" % i,
+            "solid initial velocity %d = %r, %r, %r" % ((i,) + tuple(float(v) for v in sp.get("velocity", (0, 0, 0)))),
+            "solid translation %d = %r, %r, %r" % ((i,) + tuple(float(v) for v in sp["center"])),
+            "solid scaling %d = 1,1,1" % i, "solid rotation %d = 0,0,0" % i,
+        ]
+    lines += [
         "fluid density = %r" % fluid_density, "fluid viscocity = 0",
         "vorticity eps = %r" % vorticity_eps, "buoyancy = 0",
-        "smoke carpet thickness = %r" % carpet,
+        "smoke carpet thickness = %r" % float(fill),
         "dt = %r" % dt, "gravity = %r" % gravity,
         "extrapolation cells = 20", "tight border = 5",
         "rayleigh alpha = 0", "rayleigh beta = 0",
@@ -136,3 +150,40 @@ def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
         "save data = 0",
     ]
     return "\n".join(lines) + "\n"
+
+
+def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
+                       center=(0.5, 0.65, 0.5), E=1e4, nu=0.3, density=1000.0,
+                       fluid_density=1000.0, carpet=0.3, dt=1.0 / 240.0,
+                       vorticity_eps=1.5, gravity=9.8, frames=250, velocity=(0, 0, 0)):
+    """cfg text of the unit-cube ball scene on an n^3-cell grid: (n+1)^3 nodes,
+    dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool."""
+    sphere = {"center": center, "radius": radius, "E": E, "nu": nu, "density": density,
+              "velocity": velocity}
+    return scene_cfg(n, [sphere], fill=carpet, eps=eps, max_iterations=max_iterations,
+                     fluid_density=fluid_density, dt=dt, vorticity_eps=vorticity_eps,
+                     gravity=gravity, frames=frames)
+
+
+def random_spheres(rng, k, radius_range, log10_E_range, center_range=None, margin=0.02,
+                   tries=10000):
+    """k non-overlapping spheres (rejection sampling): radius uniform in
+    radius_range, E log-uniform over 10^log10_E_range, nu = 0.3, centres
+    uniform in center_range^3 (default: inside the unit cube with the
+    sphere clear of the walls by `margin`)"""
+    out = []
+    for _ in range(tries):
+        if len(out) == k:
+            break
+        r = float(rng.uniform(*radius_range))
+        lo, hi = center_range if center_range else (r + margin, 1.0 - r - margin)
+        c = rng.uniform(lo, hi, 3)
+        if any(((c - np.asarray(o["center"])) ** 2).sum() < (r + o["radius"] + margin) ** 2
+               for o in out):
+            continue
+        out.append({"center": tuple(float(v) for v in c), "radius": r,
+                    "E": float(10 ** rng.uniform(*log10_E_range)), "nu": 0.3,
+                    "density": 1000.0})
+    if len(out) < k:
+        raise ValueError("could not place %d non-overlapping spheres" % k)
+    return out

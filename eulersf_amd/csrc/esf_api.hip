@@ -64,7 +64,9 @@ static esf_status dalloc(esf_ctx* ctx, T** p, size_t count) {
 }
 
 esf_status growSolidDofs(esf_ctx* ctx, int need) {
-  if (need <= ctx->capSolid && ctx->sForce) return ESF_OK;
+  if (need <= ctx->capSolid && ctx->sForce && ctx->inboxSolids >= (int)ctx->solids.size())
+    return ESF_OK;
+  need = std::max(need, ctx->capSolid);
   const size_t cap = std::max<size_t>(need + need / 4 + 64, 256);
   double** vecs[] = {&ctx->sForce, &ctx->sMv,  &ctx->sDamped, &ctx->sKdiag, &ctx->sAdiag,
                      &ctx->sRhs,   &ctx->sV0,  &ctx->sDv,     &ctx->sB,     &ctx->sr,
@@ -76,7 +78,8 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   esf_status st = dalloc(ctx, &ctx->solidNodes, cap);
   if (st) return st;
   if ((st = dalloc(ctx, &ctx->nodeMask, cap))) return st;
-  if ((st = dalloc(ctx, &ctx->inbox, 4 * 24 * cap))) return st;
+  ctx->inboxSolids = std::max<int>(1, (int)ctx->solids.size());
+  if ((st = dalloc(ctx, &ctx->inbox, 24 * cap * ctx->inboxSolids))) return st;
   // 4 sets x 4 sums x node-pass blocks (3 cap DOFs / 256)
   if ((st = dalloc(ctx, &ctx->sPartials, 16 * (3 * cap / 256 + 1)))) return st;
   ctx->capSolid = (int)cap;
@@ -285,8 +288,8 @@ esf_status esf_num_solid_nodes(esf_ctx* ctx, int* n) {
 esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu, double density, int rx, int ry,
                          int rz, double rdh, const double ro[3], const double* rest, int* id) {
   if (!ctx || !rest || !ro || rx < 2 || ry < 2 || rz < 2 || !(rdh > 0)) return ESF_ERR_INVALID;
-  if (ctx->solids.size() >= 4) {
-    ctx->err = "at most 4 solids per context";
+  if (ctx->solids.size() >= (size_t)kMaxSolids) {
+    ctx->err = "at most " + std::to_string(kMaxSolids) + " solids per context";
     return ESF_ERR_INVALID;
   }
   ctx->solids.emplace_back();

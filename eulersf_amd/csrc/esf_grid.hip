@@ -183,8 +183,8 @@ esf_status compact(esf_ctx* ctx, long n, Flag flag, Emit emit, int* total) {
 // ---------------------------------------------------------------------------
 struct SolidPtrs {
   int n;
-  const double* mass[4];
-  const double* qfrac[4];
+  const double* mass[kMaxSolids];
+  const double* qfrac[kMaxSolids];
 };
 
 __global__ void kGatherMass(Dims d, SolidPtrs sp, double* solidMass, double* nodeFrac) {
@@ -280,8 +280,8 @@ esf_status buildValidList(esf_ctx* ctx, DevSolid& so) {
 
 esf_status gatherSolidMasses(esf_ctx* ctx) {
   const Dims& d = ctx->d;
-  if (ctx->solids.size() > 4) {
-    ctx->err = "at most 4 solids per context";
+  if (ctx->solids.size() > (size_t)kMaxSolids) {
+    ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
   SolidPtrs sp{};
@@ -903,9 +903,9 @@ __global__ void kSolidCopy(Dims d, Box bx, const double* vel, double* sv) {
 // the Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:900-916)
 struct RecoverArgs {
   int n;
-  Box box[4];
-  const double* mass[4];
-  const double* vel[4];
+  Box box[kMaxSolids];
+  const double* mass[kMaxSolids];
+  const double* vel[kMaxSolids];
 };
 __global__ void kRecoverBC(Dims d, RecoverArgs a, double* vel, double* intensity, int bc) {
   const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1032,8 +1032,8 @@ esf_status solidFinalize(esf_ctx* ctx, double dt) {
 }
 
 esf_status recoverAndBC(esf_ctx* ctx, int bc) {
-  if (ctx->solids.size() > 4) {
-    ctx->err = "at most 4 solids per context";
+  if (ctx->solids.size() > (size_t)kMaxSolids) {
+    ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
   RecoverArgs a{};

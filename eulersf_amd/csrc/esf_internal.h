@@ -26,6 +26,10 @@ struct Dims {
   double o0, o1, o2;
 };
 
+// solids per context (scene files of the reference use one; the isolated
+// 256^3 system uses eight)
+constexpr int kMaxSolids = 8;
+
 // Per-solid box, in node coordinates (DEFOR_SOLID_3D.h:138-140)
 struct Box {
   int tMin[3], tMax[3];  // tight box  (_tightMin/_tightMax)
@@ -86,8 +90,9 @@ struct esf_ctx {
   int* sidx = nullptr;        // nodes -> solid DOF node index or -1
   int* solidNodes = nullptr;  // solid DOF node -> node index
   int nSolid = 0, capSolid = 0;
-  uint32_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
-  double* inbox = nullptr;      // solid b (<= 4) at 24 capSolid b: 24 doubles per node
+  uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
+  double* inbox = nullptr;      // solid b at 24 capSolid b: 24 doubles per node
+  int inboxSolids = 0;          // solids the inbox was sized for
   int* scanBlocks = nullptr;  // scan scratch
   // solve: node-based
   double *v0 = nullptr, *minv = nullptr;  // 3*nodes, nodes

@@ -167,7 +167,7 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
           nk++;
         }
       }
-  nodeMaskByte[4L * s] = (uint8_t)mask;
+  nodeMaskByte[8L * s] = (uint8_t)mask;
   double mv[3] = {0, 0, 0};
   const bool inBox = x >= box.bMin[0] && x < box.bMax[0] - 1 && y >= box.bMin[1] &&
                      y < box.bMax[1] - 1 && z >= box.bMin[2] && z < box.bMax[2] - 1;
@@ -406,9 +406,9 @@ enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3 };
 
 struct SolidNodeArgs {
   int nSolid;
-  int nSolidBodies;       // up to 4 solids gathered in one launch
+  int nSolidBodies;       // solids gathered in one launch (<= kMaxSolids)
   const double* inbox;    // solid b: inbox + 24 capS b, 24 doubles per node (kSolidCell)
-  const uint32_t* nodeMask; // per node, byte b: bit c = solid b's cell at offset c has rows here
+  const uint64_t* nodeMask; // per node, byte b: bit c = solid b's cell at offset c has rows here
   int capS;
   double scale;           // dt^2 + beta dt
   const double* damped;   // diag mass
@@ -449,16 +449,11 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   const int q = (int)(k - 3 * sn);
   ScalarLoads SL;
   if (MODE == S_CR) scalarLoads(a.sp.cur, a.sp.prev, a.sp.nb, threadIdx.x >> 6, SL);
-  const uint32_t mask = a.nodeMask[sn];
-  double rows[4][8];
+  const uint64_t mask = a.nodeMask[sn];
+  double rows[8];
   const double* in0 = a.inbox + 24 * sn + q;
 #pragma unroll
-  for (int c = 0; c < 8; c++) rows[0][c] = in0[3 * c];
-#pragma unroll
-  for (int b = 1; b < 4; b++)
-#pragma unroll
-    for (int c = 0; c < 8; c++)
-      rows[b][c] = b < a.nSolidBodies ? in0[24L * a.capS * b + 3 * c] : 0.0;
+  for (int c = 0; c < 8; c++) rows[c] = in0[3 * c];
   const double dm = a.damped[k];
   double vin = 0, dg = 1, ap0 = 0, s0 = 0, p0 = 0, r0 = 0, x0 = 0, bb = 0;
   if (MODE == S_CR) {
@@ -478,10 +473,17 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   asm volatile("" ::: "memory");
   double kx = 0;
 #pragma unroll
-  for (int b = 0; b < 4; b++)
+  for (int c = 0; c < 8; c++)
+    if (mask >> c & 1u) kx += rows[c];
+  // further solids (scenes with several bodies): their rows, in solid order
+  for (int b = 1; b < a.nSolidBodies; b++) {
+    const double* in = in0 + 24L * a.capS * b;
+#pragma unroll
+    for (int c = 0; c < 8; c++) rows[c] = in[3 * c];
 #pragma unroll
     for (int c = 0; c < 8; c++)
-      if (mask >> (8 * b + c) & 1u) kx += rows[b][c];
+      if (mask >> (8 * b + c) & 1u) kx += rows[c];
+  }
   double alpha = 0, beta = 0;
   if (MODE == S_CR) {
     if (scalarFinish(a.sp.cur, a.sp.prev, a.sp.nb, a.sp.first, a.sc, SL, grp, alpha, beta))
@@ -537,8 +539,8 @@ __global__ void kSolidDiag(long n, const double* kdiag, const double* damped, do
 // ---------------------------------------------------------------------------
 esf_status stressStage(esf_ctx* ctx, double* energy) {
   const Dims& d = ctx->d;
-  if (ctx->solids.size() > 4) {
-    ctx->err = "at most 4 solids per context";
+  if (ctx->solids.size() > (size_t)kMaxSolids) {
+    ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
   double en = 0;
@@ -574,7 +576,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
 
 static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
   a.nSolid = ctx->nSolid;
-  a.nSolidBodies = (int)std::min<size_t>(ctx->solids.size(), 4);
+  a.nSolidBodies = (int)ctx->solids.size();
   a.inbox = ctx->inbox;
   a.nodeMask = ctx->nodeMask;
   a.capS = ctx->capSolid;
@@ -619,8 +621,8 @@ esf_status solidAdiag(esf_ctx* ctx) {
 }
 
 esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
-  if (ctx->solids.size() > 4) {
-    ctx->err = "at most 4 solids per context";
+  if (ctx->solids.size() > (size_t)kMaxSolids) {
+    ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
   cellPass<CELL_IN>(ctx, x, nullptr, SolidPartials{});
@@ -663,8 +665,8 @@ __global__ void kSolidSetup(long n, const double* b, const double* diag, double*
 
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps,
                       int maxIt, int slot) {
-  if (ctx->solids.size() > 4) {
-    ctx->err = "at most 4 solids per context";
+  if (ctx->solids.size() > (size_t)kMaxSolids) {
+    ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
   const long n = 3L * ctx->nSolid;
