@@ -167,6 +167,74 @@ def spmv_only(esf, cfg, n, reps):
     g.close()
 
 
+def divergence_field(u, n):
+    """b = J u on an n^3-cell grid for an AoS node field u ((n+1)^3 x 3,
+    zero on the boundary nodes): b_c = sum over the cell's 8 corners of
+    +-u_x +-u_y +-u_z, sign + on the corner's high side
+    (COUPLED_INTEGRATOR_3D.cpp:239-249)"""
+    U = u.reshape(n + 1, n + 1, n + 1, 3)  # z, y, x
+    b = np.zeros((n, n, n))
+    for oz in (0, 1):
+        for oy in (0, 1):
+            for ox in (0, 1):
+                C = U[oz:oz + n, oy:oy + n, ox:ox + n]
+                b += (1 if ox else -1) * C[..., 0]
+                b += (1 if oy else -1) * C[..., 1]
+                b += (1 if oz else -1) * C[..., 2]
+    return b.reshape(-1)
+
+
+def isolated_system(esf, n, seed, eps, reps):
+    """The isolated coupled linear system at n^3 cells: k = 8 spheres from
+    the seed (centres uniform in [0.15, 0.85]^3, non-overlapping, radii
+    uniform in [0.05, 0.12], E log-uniform in [1e3, 1e6]), the domain 60 %
+    fluid-filled; the Schur pressure system S x = b with S = J M^-1 J^T on
+    those masses and b a random divergence field (b = J u, u ~ N(0,1) at
+    interior nodes, so b lies in the range of the singular S), solved from
+    x = 0 to `eps` in FP64 and in the mixed form."""
+    from eulersf_amd.cfg import random_spheres, scene_cfg
+    rng = np.random.default_rng(seed)
+    spheres = random_spheres(rng, 8, (0.05, 0.12), (3.0, 6.0), center_range=(0.15, 0.85))
+    text = scene_cfg(n, spheres, fill=0.6, eps=eps)
+    u = np.zeros(((n + 1) ** 3, 3))
+    inner = np.zeros((n + 1, n + 1, n + 1), dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    u[inner.reshape(-1)] = rng.standard_normal((int(inner.sum()), 3))
+    b = divergence_field(u, n)
+    del u
+    out = {"workload": f"isolated pressure system, {n}^3 cells, 8 spheres (seed {seed}), "
+                       f"60% fluid, b = J u with u ~ N(0,1), x0 = 0, eps {eps:g}",
+           "cells": n ** 3, "seed": seed}
+    sols = {}
+    for name, prec in (("fp64", esf.ESF_PLAIN_FP64), ("mixed", esf.ESF_MIXED)):
+        g = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
+        g.forces()
+        g.pressureDiag()
+        times = []
+        for r in range(reps):
+            g.upload(esf._abi.F_PRESSURE_RHS, b)
+            g.upload(esf._abi.F_LAMBDA, np.zeros(n ** 3))
+            it, ref, rr, ms, st = g.pressureSolveResident(eps, 100000)
+            times.append(ms)
+        sols[name] = g.download(esf._abi.F_LAMBDA)
+        spmv = spmv_roofline(g, n ** 3, (n - 1) ** 3, prec == esf.ESF_MIXED, 20)
+        out[name] = {"iterations": it, "refinements": ref, "rel_residual": rr, "status": st,
+                     "time_to_solution_ms": round(float(np.median(times)), 3),
+                     "time_to_solution_ms_all": [round(t, 3) for t in times],
+                     "spmv": spmv}
+        g.close()
+    # the pressure is fixed up to null(S); compare residuals of the two
+    # solutions through S itself on the FP64 context
+    g = esf.CoupledIntegrator3D.from_cfg_text(text, esf.ESF_PLAIN_FP64)
+    g.forces()
+    d = sols["fp64"] - sols["mixed"]
+    Sd = g.getMatVecMult(d)
+    Sx = g.getMatVecMult(sols["fp64"])
+    out["mixed_vs_fp64"] = {"rel_diff_S_x": float(np.linalg.norm(Sd) / np.linalg.norm(Sx))}
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
 def traffic_run(esf, text):
     """The command the HBM byte counters are collected over (rocprofv3 --pmc
     FETCH_SIZE, then WRITE_SIZE): a few steps of each form, so every Krylov
@@ -219,6 +287,11 @@ def main():
     ap.add_argument("--traffic-run", action="store_true",
                     help="a few steps of both forms and a calibrating copy, for the "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--system", type=int, default=0, metavar="N",
+                    help="the isolated coupled linear system on N^3 cells (8 seeded spheres): "
+                         "FP64 and mixed solves to --eps from x = 0; prints one JSON line")
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--system-reps", type=int, default=3)
     ap.add_argument("--spmv-only", type=int, default=0, metavar="N",
                     help="only time the pressure SpMV (FP64, FP32) on the N^3 scene and a "
                          "streaming copy of the same byte count; prints one JSON line")
@@ -234,6 +307,9 @@ def main():
     from eulersf_amd.cfg import unit_cube_ball_cfg
     if args.spmv_only:
         spmv_only(esf, unit_cube_ball_cfg, args.spmv_only, args.spmv_reps)
+        return
+    if args.system:
+        isolated_system(esf, args.system, args.seed, args.eps, args.system_reps)
         return
     n = args.n
     text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000)

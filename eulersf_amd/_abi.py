@@ -137,6 +137,7 @@ _PROTOS = {
     "esf_pressure_diag": (_S, [_P, _D]),
     "esf_solid_apply": (_S, [_P, _D, _D]),
     "esf_pressure_solve": (_S, [_P, _D, _D, ctypes.c_double, ctypes.c_int, _I, _D]),
+    "esf_pressure_solve_resident": (_S, [_P, ctypes.c_double, ctypes.c_int, _I, _I, _D, _D]),
     "esf_corotation_batch": (_S, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                                  _D, _D]),
     "esf_bench_pressure_apply": (_S, [_P, ctypes.c_int, ctypes.c_int, _D]),

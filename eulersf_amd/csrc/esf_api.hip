@@ -121,6 +121,7 @@ static void freeAll(esf_ctx* ctx) {
       if (p) hipFree(p);
   }
   if (ctx->hostFlags) hipHostFree(ctx->hostFlags);
+  if (ctx->hostRing) hipHostFree(ctx->hostRing);
   if (ctx->hostSc) hipHostFree(ctx->hostSc);
 }
 
@@ -224,6 +225,8 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = dalloc(ctx, &ctx->sc, 8);
   if (!st) st = dalloc(ctx, &ctx->dmisc, 16);
   if (!st) st = growSolidDofs(ctx, 0);
+  if (!st && hipHostMalloc((void**)&ctx->hostRing, 8 * sizeof(CrScalars)) != hipSuccess)
+    st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostFlags, 64 * sizeof(int)) != hipSuccess)
     st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostSc, esf_ctx::kReportSlots * sizeof(CrScalars)) !=
@@ -382,7 +385,10 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
       return ESF_OK;
     case ESF_F_SOLID_INDEX: *ptr = ctx->sidx; *bytes = 4 * N; break;
     case ESF_F_MINV: *ptr = ctx->minv; *bytes = 8 * N; break;
-    case ESF_F_PRESSURE_RHS: *ptr = ctx->b; *bytes = 8 * C; break;
+    case ESF_F_PRESSURE_RHS:  // writable: the right-hand side of esf_pressure_solve_resident
+      *ptr = ctx->b;
+      *bytes = 8 * C;
+      return ESF_OK;
     case ESF_F_DIV_RHS: *ptr = ctx->c; *bytes = 8 * C; break;
     case ESF_F_V0: *ptr = ctx->v0; *bytes = 24 * N; break;
     case ESF_F_SOLID_FORCE: *ptr = ctx->sForce; *bytes = 8 * NS; break;
@@ -685,6 +691,34 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double e
   solveReport(ctx->hostSc[3], maxIt, &it, &conv, &rr);
   if (iters) *iters = it;
   if (relres) *relres = rr;
+  return conv ? ESF_OK : ESF_NOT_CONVERGED;
+}
+
+esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int maxIt, int* iters,
+                                       int* refinements, double* relres, double* deviceMs) {
+  if (!ctx || !(eps > 0) || maxIt < 1) return ESF_ERR_INVALID;
+  int it = 0, outer = 0, inner = 0;
+  bool conv = false;
+  double rr = 0;
+  ESF_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+  esf_status st;
+  if (ctx->p.precision == ESF_MIXED) {
+    st = mixedSolve(ctx, eps, maxIt, &it, &conv, &rr, &outer, &inner);
+  } else {
+    st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
+                         ctx->pdir, ctx->s, ctx->Ap, eps, maxIt, 3);
+  }
+  if (st) return st;
+  ESF_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+  ESF_HIP(hipEventSynchronize(ctx->ev[5]));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
+  if (ctx->p.precision != ESF_MIXED) solveReport(ctx->hostSc[3], maxIt, &it, &conv, &rr);
+  ctx->lambdaInit = true;
+  if (iters) *iters = it;
+  if (refinements) *refinements = outer;
+  if (relres) *relres = rr;
+  if (deviceMs) *deviceMs = ms;
   return conv ? ESF_OK : ESF_NOT_CONVERGED;
 }
 

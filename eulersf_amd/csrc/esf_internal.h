@@ -115,7 +115,9 @@ struct esf_ctx {
   double* sPartials = nullptr; // solid CR: 4 sets of 4 x node-pass blocks
   esf::CrScalars* sc = nullptr;   // device scalars (several solves)
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
-  int* hostFlags = nullptr;       // pinned ring of done flags
+  int* hostFlags = nullptr;       // pinned: counts read back by the host
+  esf::CrScalars* hostRing = nullptr;  // pinned ring of polled scalar blocks
+  int crQueued = 0;  // iterations the last pressure CR solve queued
   // pinned copies of the Krylov scalars, one slot per solve of a step (0:
   // solid 1, 1: pressure, 2: solid 2, 3: scratch, 4: inner FP32); a solve
   // queues its copy and the host reads it after the step's next sync

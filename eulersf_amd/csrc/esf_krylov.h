@@ -1,6 +1,8 @@
 // esf_krylov.h -- pieces shared by the FP64/FP32 pressure solve and the
 // solid solve: the CR direction update and the host-side done-flag poller.
 #pragma once
+#include <algorithm>
+
 #include "esf_internal.h"
 
 namespace esf {
@@ -96,25 +98,46 @@ inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int 
 
 }  // namespace
 
-// Poll the device "done" flag of scalar block `which` without stalling the
-// queue: after each convergence check an async copy of the flag lands in a
-// pinned ring slot behind an event; the host keeps one check interval
+// Poll the scalar block `which` of a Krylov solve without stalling the
+// queue: after each convergence check an async copy of the block lands in
+// a pinned ring slot behind an event; the host keeps one check interval
 // queued ahead and waits on the previous one, so the device never idles
 // and at most one interval of early-exiting launches follows convergence.
+// With `stagnation` set (the FP32 inner solve of the mixed form), the poll
+// also ends the solve when four consecutive checks set no new minimum of
+// the checked residual: FP32 rounding puts a noise floor under the inner
+// residual (a slowly converging solve still sets new minima), and the FP64
+// refinement around it takes over from there.
 struct DonePoller {
   esf_ctx* ctx;
   int which;
+  bool stagnation;
   static constexpr int kRing = 8;
   static constexpr int kLook = 1;
-  int issued = 0, waited = 0;
+  int issued = 0, waited = 0, flat = 0;
+  double best = 0;
+  bool stagnated = false;
   hipEvent_t ev[kRing];
-  explicit DonePoller(esf_ctx* c, int w) : ctx(c), which(w) {
+  explicit DonePoller(esf_ctx* c, int w, bool stag = false) : ctx(c), which(w), stagnation(stag) {
     for (int i = 0; i < kRing; i++) ev[i] = ctx->ev[8 + i];
   }
-  // returns true when the solve is known to be done
+  bool seen(int ws) {
+    const CrScalars& h = ctx->hostRing[ws];
+    if (h.done) return true;
+    if (!stagnation) return false;
+    if (best == 0 || h.resnorm < best) {
+      best = h.resnorm;
+      flat = 0;
+    } else if (++flat >= 4) {
+      stagnated = true;
+      return true;
+    }
+    return false;
+  }
+  // returns true when the solve is known to be done (or stagnated)
   bool mark() {
     const int slot = issued % kRing;
-    hipMemcpyAsync(&ctx->hostFlags[slot], &ctx->sc[which].done, sizeof(int),
+    hipMemcpyAsync(&ctx->hostRing[slot], &ctx->sc[which], sizeof(CrScalars),
                    hipMemcpyDeviceToHost, ctx->stream);
     hipEventRecord(ev[slot], ctx->stream);
     issued++;
@@ -122,14 +145,14 @@ struct DonePoller {
       const int ws = waited % kRing;
       hipEventSynchronize(ev[ws]);
       waited++;
-      if (ctx->hostFlags[ws]) return true;
+      if (seen(ws)) return true;
     }
     // opportunistic non-blocking checks
     while (waited < issued) {
       const int ws = waited % kRing;
       if (hipEventQuery(ev[ws]) != hipSuccess) break;
       waited++;
-      if (ctx->hostFlags[ws]) return true;
+      if (seen(ws)) return true;
     }
     return false;
   }

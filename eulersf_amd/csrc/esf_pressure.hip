@@ -485,7 +485,8 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   a.in = rA;
   if ((st = launchSpmv<T, M_SETUP_S>(ctx, a))) return st;
   const int dirBlocks = 1024;
-  DonePoller poll(ctx, which);
+  // the FP32 inner solve of the mixed form stops when its residual stagnates
+  DonePoller poll(ctx, which, sizeof(T) == 4);
   int i = 0;
   for (; i < maxIt; i++) {
     a.iter = i;
@@ -500,9 +501,13 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     if (i > 0 && i % 5 == 0) {
       a.in = x;
       if ((st = launchSpmv<T, M_CHECK>(ctx, a))) return st;
-      if (poll.mark()) break;
+      if (poll.mark()) {
+        i++;
+        break;
+      }
     }
   }
+  ctx->crQueued = i;
   // the report is copied without waiting; the caller reads report slot
   // `slot` after its next synchronisation
   ESF_HIP(hipMemcpyAsync(&ctx->hostSc[slot], &ctx->sc[which], sizeof(CrScalars),
@@ -570,6 +575,8 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     bool conv = false;
     double rr = 0;
     solveReport(ctx->hostSc[4], maxIt - inner, &it, &conv, &rr);
+    // an inner solve that stopped on stagnation ran the iterations it queued
+    if (!conv) it = ctx->crQueued;
     inner += it;
   }
   *converged = h.done == 1;

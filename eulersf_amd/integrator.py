@@ -225,6 +225,17 @@ class CoupledIntegrator3D:
                                                          ctypes.byref(it), ctypes.byref(rr)))
         return x, it.value, rr.value, st
 
+    def pressureSolveResident(self, eps, max_it):
+        """S x = b with b (F_PRESSURE_RHS) and the start x (F_LAMBDA) already
+        uploaded, in the context's precision; returns (iterations,
+        refinements, relative residual, device ms, status)"""
+        it, ref = ctypes.c_int(), ctypes.c_int()
+        rr, ms = ctypes.c_double(), ctypes.c_double()
+        st = check(self.ctx, self.lib.esf_pressure_solve_resident(
+            self.ctx, eps, max_it, ctypes.byref(it), ctypes.byref(ref), ctypes.byref(rr),
+            ctypes.byref(ms)), "pressure solve")
+        return it.value, ref.value, rr.value, ms.value, st
+
     # -- timing -------------------------------------------------------------
     def bench_pressure_apply(self, fp32=False, reps=50):
         v = ctypes.c_double()

@@ -104,7 +104,7 @@ enum {
   ESF_F_LAMBDA = 4,        /* double[cells]    pressure (warm start)     */
   ESF_F_SOLID_INDEX = 6,   /* int32[nodes]     _gridIdxToSolidIdx        */
   ESF_F_MINV = 9,          /* double[nodes]    1/m at interior nodes     */
-  ESF_F_PRESSURE_RHS = 10, /* double[cells]    b = J v0 - c              */
+  ESF_F_PRESSURE_RHS = 10, /* double[cells]    b = J v0 - c (writable)   */
   ESF_F_DIV_RHS = 11,      /* double[cells]    c                         */
   ESF_F_V0 = 7,            /* double[3*nodes]  v0 on interior nodes      */
   ESF_F_SOLID_FORCE = 13,  /* double[3*nSolid] elastic+body force (pre-dt) */
@@ -210,6 +210,17 @@ esf_status esf_solid_apply(esf_ctx* ctx, const double* x, double* y);
 esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x,
                               double eps, int max_it, int* iterations,
                               double* rel_residual);
+/* Measurement entry point, no reference counterpart (the isolated-system
+ * benchmark): solvePCR on S x = b with b (ESF_F_PRESSURE_RHS) and the start
+ * x (ESF_F_LAMBDA) already resident, D from the last esf_pressure_diag /
+ * esf_forces, in the context's precision (ESF_MIXED: FP32 CR inside FP64
+ * refinement).  iterations: CR iterations (mixed: FP32 ones summed);
+ * refinements: outer FP64 rounds (0 in FP64); device_ms: the solve's device
+ * time between HIP events on the context stream.  The solution stays in
+ * ESF_F_LAMBDA. */
+esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int max_it,
+                                       int* iterations, int* refinements,
+                                       double* rel_residual, double* device_ms);
 /* COROTATION::init + firstPiolaKirchhoff + strainEnergy +
  * firstPKDifferential (COROTATION.cpp:37-107) on n deformation gradients,
  * one GPU thread each.  F, dF: n*9 row-major; out: n*36 (R,S,L,P);

@@ -145,3 +145,46 @@ def test_three_solids_match_oracle(esf):
         assert rel_err(g.download(esf._abi.F_VELOCITY), o.field(O.VELOCITY)) < 1e-10, k
         for b in range(3):
             assert rel_err(g.download(esf._abi.S_U, b), o.field(O.S_U, b)) < 1e-10, (k, b)
+
+
+def test_isolated_pressure_system_matches_oracle(esf):
+    """the isolated system of the 256^3 benchmark, at 24^3 cells: 8 seeded
+    spheres, b = J u.  FP64: the reference's CR from x = 0 to 1e-13, same
+    iteration count (one check interval) and S x as the oracle's CR.  Mixed:
+    FP32 CR in FP64 refinement reaches 1e-8 with S x within 1e-6."""
+    import ctypes
+    import bench
+    from eulersf_amd.cfg import random_spheres, scene_cfg
+    n = 24
+    rng = np.random.default_rng(2026)
+    spheres = random_spheres(rng, 8, (0.05, 0.12), (3.0, 6.0), center_range=(0.15, 0.85))
+    text = scene_cfg(n, spheres, fill=0.6, **TIGHT)
+    X = n + 1
+    u = np.zeros((X ** 3, 3))
+    inner = np.zeros((X, X, X), dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    u[inner.reshape(-1)] = rng.standard_normal((int(inner.sum()), 3))
+    b = bench.divergence_field(u, n)
+    o = O.OracleScene.from_cfg_text(text)
+    o.L.orc_forces(o.h)
+    xo = np.zeros(n ** 3)
+    it_o = ctypes.c_int()
+    assert o.L.orc_pressure_cr(o.h, O.dp(b), O.dp(xo), 1e-13, 20000, ctypes.byref(it_o)) == 0
+    Sxo = o.pressure_apply(xo)
+    for prec, eps, tol in ((esf.ESF_PLAIN_FP64, 1e-13, 1e-10), (esf.ESF_MIXED, 1e-8, 1e-6)):
+        g = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
+        g.forces()
+        g.pressureDiag()
+        g.upload(esf._abi.F_PRESSURE_RHS, b)
+        g.upload(esf._abi.F_LAMBDA, np.zeros(n ** 3))
+        it, ref, rr, ms, st = g.pressureSolveResident(eps, 20000)
+        assert st == 0 and rr < eps, (prec, rr)
+        x = g.download(esf._abi.F_LAMBDA)
+        assert rel_err(g.getMatVecMult(x), Sxo) < tol, prec
+        if prec == esf.ESF_PLAIN_FP64:
+            assert abs(it - it_o.value) <= 5, (it, it_o.value)
+            assert ref == 0
+        else:
+            assert ref >= 1
+        g.close()
+    o.close()
