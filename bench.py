@@ -235,6 +235,59 @@ def isolated_system(esf, n, seed, eps, reps):
     print(json.dumps(out), flush=True)
 
 
+def batched(esf, count, steps, workers, seed):
+    """The batched uneven scenes: `count` scenes with N drawn from
+    {32, 48, 64, 80, 96}, built like the served scenes; K timed steps of
+    every scene, one scene at a time, then all concurrently on `workers`
+    threads (one HIP stream per scene)."""
+    from eulersf_amd.serve import BatchRunner, random_scene
+    rng = np.random.default_rng(seed)
+    ns = [int(rng.choice([32, 48, 64, 80, 96])) for _ in range(count)]
+    texts = [random_scene(rng, n) for n in ns]
+    log(f"[bench] batched: {count} scenes, {steps} steps each, {workers} workers")
+    br = BatchRunner(texts, ns, esf.ESF_MIXED)
+    cells = sum(br.cells())
+    wc, lc = br.run_concurrent(steps, workers)
+    ws, ls = br.run_sequential(steps)
+    mem = sum(g.device_bytes() for g in br.scenes)
+    br.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
+    out = {"workload": f"{count} scenes, N in {{32,48,64,80,96}} (seed {seed}), 1-3 spheres, "
+                       f"mixed precision at 1e-8, {steps} steps per scene after 2 warm-up steps",
+           "cells_total": cells, "device_bytes_total": mem,
+           "sequential": {"wall_s": round(ws, 3), "cells_steps_per_s": round(cells * steps / ws, 1),
+                          "step_latency_ms": {"p50": pct(ls, 50), "p99": pct(ls, 99)}},
+           "concurrent": {"workers": workers, "wall_s": round(wc, 3),
+                          "cells_steps_per_s": round(cells * steps / wc, 1),
+                          "step_latency_ms": {"p50": pct(lc, 50), "p99": pct(lc, 99)}},
+           "speedup": round(ws / wc, 3)}
+    return out
+
+
+def served(esf, seconds, rate, workers, seed, one_at_a_time=True):
+    """The serving stream: Poisson arrivals at `rate`/s for `seconds`, cubic
+    scenes with N uniform in [32, 160] asking for K uniform in [10, 100]
+    steps, admitted against a 200 GB HBM budget; run with `workers` slots
+    and, for comparison, one scene at a time."""
+    from eulersf_amd.serve import Server, poisson_requests, summarize
+    rng = np.random.default_rng(seed)
+    reqs = poisson_requests(rng, rate, seconds)
+    log(f"[bench] served: {len(reqs)} requests over {seconds} s, {workers} slots")
+    out = {"workload": f"Poisson arrivals at {rate}/s for {seconds} s (seed {seed}): N ~ U[32,160], "
+                       "K ~ U[10,100] steps, 1-3 spheres, mixed precision at 1e-8, "
+                       "200 GB HBM budget", "requests": len(reqs)}
+    srv = Server(200e9, workers, esf.ESF_MIXED)
+    recs, span, info = srv.run(reqs)
+    out["concurrent"] = {"slots": workers, **summarize(recs, span), **info}
+    if one_at_a_time:
+        log("[bench] served, one scene at a time")
+        recs1, span1, info1 = Server(200e9, 1, esf.ESF_MIXED).run(reqs)
+        out["one_at_a_time"] = {**summarize(recs1, span1), **info1}
+        out["throughput_ratio"] = round(out["concurrent"]["steps_per_s"] /
+                                        out["one_at_a_time"]["steps_per_s"], 3)
+    return out
+
+
 def traffic_run(esf, text):
     """The command the HBM byte counters are collected over (rocprofv3 --pmc
     FETCH_SIZE, then WRITE_SIZE): a few steps of each form, so every Krylov
@@ -292,6 +345,19 @@ def main():
                          "FP64 and mixed solves to --eps from x = 0; prints one JSON line")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--system-reps", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=0, metavar="S",
+                    help="the batched uneven scenes: S scenes stepped one at a time and "
+                         "concurrently; prints one JSON line")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--workers", type=int, default=16,
+                    help="concurrent scenes (host threads / HIP streams) of --batch and --serve")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+                    help="the serving stream over SECONDS of Poisson arrivals, with the "
+                         "concurrent server and one scene at a time; prints one JSON line")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--served-seconds", type=float, default=3.0,
+                    help="arrival window of the short served-stream block of the default "
+                         "run (0 to skip)")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="N",
                     help="only time the pressure SpMV (FP64, FP32) on the N^3 scene and a "
                          "streaming copy of the same byte count; prints one JSON line")
@@ -302,6 +368,10 @@ def main():
               "the coupled step has no multi-GPU path in this build", file=sys.stderr)
         sys.exit(2)
 
+    # concurrent scenes each own a HIP stream; give the runtime as many
+    # hardware queues as there are concurrent scenes (at most 16)
+    if args.batch or args.serve or args.served_seconds:
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(args.workers, 16))))
     import eulersf_amd as esf
     esf.load()
     from eulersf_amd.cfg import unit_cube_ball_cfg
@@ -310,6 +380,14 @@ def main():
         return
     if args.system:
         isolated_system(esf, args.system, args.seed, args.eps, args.system_reps)
+        return
+    if args.batch:
+        print(json.dumps(batched(esf, args.batch, args.batch_steps, args.workers, args.seed)),
+              flush=True)
+        return
+    if args.serve:
+        print(json.dumps(served(esf, args.serve, args.serve_rate, args.workers, args.seed)),
+              flush=True)
         return
     n = args.n
     text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000)
@@ -359,6 +437,11 @@ def main():
         roof_big = {"fp64": spmv_roofline(big, nb ** 3, (nb - 1) ** 3, False, args.spmv_reps),
                     "fp32": spmv_roofline(big, nb ** 3, (nb - 1) ** 3, True, args.spmv_reps)}
         big.close()
+
+    serve_block = None
+    if args.served_seconds > 0:
+        serve_block = served(esf, args.served_seconds, args.serve_rate, args.workers, args.seed,
+                             one_at_a_time=False)
 
     cpu = None
     if not args.no_cpu:
@@ -410,6 +493,7 @@ def main():
                                "profiled_steps": prof},
         "spmv_apply_at_workload": {"fp64": roof64, "fp32": roof32},
         "roofline_hbm_sized": roof_big,
+        "served_stream": serve_block,
         "cpu_baseline": cpu,
     }
     g64.close()

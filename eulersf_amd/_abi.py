@@ -138,6 +138,7 @@ _PROTOS = {
     "esf_solid_apply": (_S, [_P, _D, _D]),
     "esf_pressure_solve": (_S, [_P, _D, _D, ctypes.c_double, ctypes.c_int, _I, _D]),
     "esf_pressure_solve_resident": (_S, [_P, ctypes.c_double, ctypes.c_int, _I, _I, _D, _D]),
+    "esf_ctx_device_bytes": (_S, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "esf_corotation_batch": (_S, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                                  _D, _D]),
     "esf_bench_pressure_apply": (_S, [_P, ctypes.c_int, ctypes.c_int, _D]),

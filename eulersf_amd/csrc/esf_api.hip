@@ -48,6 +48,11 @@ esf_status zeroD(esf_ctx* ctx, long n, double* p);
 template <typename T>
 static esf_status dalloc(esf_ctx* ctx, T** p, size_t count) {
   if (*p) {
+    auto it = ctx->allocs.find((void*)*p);
+    if (it != ctx->allocs.end()) {
+      ctx->deviceBytes -= it->second;
+      ctx->allocs.erase(it);
+    }
     hipFree(*p);
     *p = nullptr;
   }
@@ -59,6 +64,8 @@ static esf_status dalloc(esf_ctx* ctx, T** p, size_t count) {
     *p = nullptr;
     return ESF_ERR_OOM;
   }
+  ctx->allocs[(void*)*p] = count * sizeof(T);
+  ctx->deviceBytes += count * sizeof(T);
   e = hipMemsetAsync(*p, 0, count * sizeof(T), ctx->stream);
   return e == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -752,6 +759,12 @@ esf_status esf_stage_times(esf_ctx* ctx, double* ms, int n, int reset) {
   for (int i = 0; i < n && i < 16; i++) ms[i] = ctx->stageMs[i];
   if (reset)
     for (double& v : ctx->stageMs) v = 0;
+  return ESF_OK;
+}
+
+esf_status esf_ctx_device_bytes(const esf_ctx* ctx, size_t* bytes) {
+  if (!ctx || !bytes) return ESF_ERR_INVALID;
+  *bytes = ctx->deviceBytes;
   return ESF_OK;
 }
 

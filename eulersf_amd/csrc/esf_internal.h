@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/esf.h"
@@ -118,6 +119,9 @@ struct esf_ctx {
   int* hostFlags = nullptr;       // pinned: counts read back by the host
   esf::CrScalars* hostRing = nullptr;  // pinned ring of polled scalar blocks
   int crQueued = 0;  // iterations the last pressure CR solve queued
+  // device allocations held by the context (admission control of served scenes)
+  std::unordered_map<void*, size_t> allocs;
+  size_t deviceBytes = 0;
   // pinned copies of the Krylov scalars, one slot per solve of a step (0:
   // solid 1, 1: pressure, 2: solid 2, 3: scratch, 4: inner FP32); a solve
   // queues its copy and the host reads it after the step's next sync

@@ -76,6 +76,12 @@ class CoupledIntegrator3D:
         except Exception:
             pass
 
+    def device_bytes(self):
+        """device memory this context holds"""
+        v = ctypes.c_size_t()
+        check(self.ctx, self.lib.esf_ctx_device_bytes(self.ctx, ctypes.byref(v)))
+        return v.value
+
     # -- fields -------------------------------------------------------------
     def _shape(self, field):
         n, c = self.nodes, self.cells

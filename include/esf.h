@@ -139,6 +139,9 @@ esf_status esf_get_dt(esf_ctx* ctx, double* dt);
 esf_status esf_num_solid_nodes(esf_ctx* ctx, int* n);
 /* wait for all work queued on the context's stream */
 esf_status esf_synchronize(esf_ctx* ctx);
+/* device memory the context holds (bytes); no reference counterpart: the
+ * serving loop admits scenes against an HBM budget with it */
+esf_status esf_ctx_device_bytes(const esf_ctx* ctx, size_t* bytes);
 
 /* --- solids -------------------------------------------------------------*/
 /* DEFOR_SOLID_3D ctor + setMaterial + setMassDensity
