@@ -552,7 +552,10 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
   CrScalars h;
   if ((st = readScalars(ctx, 0, &h))) return st;
   int outer = 0, inner = 0;
-  const int maxOuter = 30;
+  // refinement rounds: each gains roughly the FP32 solve's attainable
+  // reduction (about eps32 * cond on the smooth modes), so an ill-conditioned
+  // cold start needs many; the total FP32 iterations stay capped by maxIt
+  const int maxOuter = 1000;
   while (!h.done && outer < maxOuter && inner < maxIt) {
     // inner tolerance: ask for the reduction the outer test still needs,
     // bounded by what FP32 can deliver
