@@ -2,12 +2,13 @@
 // collocated Q1-P0 grid of the reference, and its Krylov solvers.
 //
 // S x (COUPLED_INTEGRATOR_3D.cpp:638-660) is never assembled.  One kernel
-// streams x through LDS in 64x4-cell tiles marching along z: for each node
-// layer it forms w = minv * (J^T x) (8 cells, signs from the corner bits,
-// COUPLED_INTEGRATOR_3D.cpp:239-249), for each cell layer it forms
-// y = J w (8 corners x 3 components).  Sums run in the reference's sparse
-// row/column order, so the FP64 result is bitwise the reference's CSR
-// product.  Bytes per apply: x once, y once, minv once (8 B each in FP64).
+// streams x through LDS in 32x8-cell tiles marching along z, the next layer
+// prefetched into registers: for each node layer it forms w = minv * (J^T x)
+// (8 cells, signs from the corner bits, COUPLED_INTEGRATOR_3D.cpp:239-249),
+// for each cell layer it forms y = J w (8 corners x 3 components).  Sums run
+// in the reference's sparse row/column order, so the FP64 result is bitwise
+// the reference's CSR product.  Bytes per apply: x once, y once, minv once
+// (8 B each in FP64).
 //
 // The Krylov loop is CONJUGATE_RESIDUAL::solvePCR (CONJUGATE_RESIDUAL.inl:
 // 109-192) split into two launches per iteration:
@@ -15,7 +16,8 @@
 //                  s = S r, partial r.s          -> beta
 //   direction    : p = beta p + r, Ap = beta Ap + s, partial Ap.D^-1 Ap -> alpha
 // plus a true-residual check every 5 iterations, as the reference does.
-// Reductions finish in the last block (fixed order, deterministic).
+// Block partials are summed by a one-block finisher launched after each
+// reducing kernel (fixed order, deterministic).
 #include "esf_internal.h"
 #include "esf_krylov.h"
 
