@@ -37,8 +37,8 @@ struct Box {
   int bMin[3], bMax[3];  // extended box (_xmin.._xmax)
 };
 
-// Scalars of one Krylov solve, resident in device memory.  The last block
-// of each reducing kernel writes them; the next kernel reads them.
+// Scalars of one Krylov solve, resident in device memory.  The finisher
+// after each reducing kernel writes them; the next kernel reads them.
 struct CrScalars {
   double delta;    // r . s
   double beta;     // delta_new / delta_old
@@ -46,7 +46,7 @@ struct CrScalars {
   double bnorm;    // |b|
   double resnorm;  // last true residual |Ax - b|
   double eps;      // relative tolerance
-  double aux;      // mixed: |r| of the outer residual
+  double aux;      // unused
   int done;        // 1 = converged
   int iter;        // iteration count at convergence
   int pad[2];
