@@ -424,9 +424,20 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
     case ESF_S_U: *ptr = so.u; *bytes = 24 * N; break;
     case ESF_S_X: *ptr = so.X; *bytes = 24 * N; break;
     case ESF_S_VELOCITY: *ptr = so.vel; *bytes = 24 * N; break;
-    case ESF_S_MASS: *ptr = so.mass; *bytes = 8 * N; if (forWrite) so.listDirty = true; break;
+    case ESF_S_MASS:
+      *ptr = so.mass;
+      *bytes = 8 * N;
+      if (forWrite) {
+        so.listDirty = true;
+        ctx->massGen++;
+      }
+      break;
     case ESF_S_VALID: *ptr = so.valid; *bytes = C; if (forWrite) so.listDirty = true; break;
-    case ESF_S_QFRAC: *ptr = so.qfrac; *bytes = 64 * N; break;
+    case ESF_S_QFRAC:
+      *ptr = so.qfrac;
+      *bytes = 64 * N;
+      if (forWrite) ctx->massGen++;
+      break;
     case ESF_S_VEL_WORKSPACE: *ptr = so.velws; *bytes = 24 * N; break;
     default:
       ctx->err = "bad field id";

@@ -284,6 +284,15 @@ esf_status gatherSolidMasses(esf_ctx* ctx) {
     ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
+  // the gathered fields depend on the solids' masses and quarter fractions
+  // only: when neither moved since the last gather (the usual step: they
+  // are computed once per scene), the fields and the DOF numbering stand
+  if (ctx->gatheredGen == ctx->massGen && ctx->gatheredSolids == (int)ctx->solids.size()) {
+    esf_status st = ESF_OK;
+    for (auto& so : ctx->solids)
+      if (so.listDirty && (st = buildValidList(ctx, so))) return st;
+    return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+  }
   SolidPtrs sp{};
   sp.n = (int)ctx->solids.size();
   for (int b = 0; b < sp.n; b++) {
@@ -309,6 +318,8 @@ esf_status gatherSolidMasses(esf_ctx* ctx) {
   hipLaunchKernelGGL((kScanEmit<SolidFlag, SolidEmit>), dim3(nb), dim3(256), 0, ctx->stream, n,
                      fl, (const int*)ctx->scanBlocks, em);
   ctx->nSolid = total;
+  ctx->gatheredGen = ctx->massGen;
+  ctx->gatheredSolids = (int)ctx->solids.size();
   for (auto& so : ctx->solids)
     if (so.listDirty) {
       if ((st = buildValidList(ctx, so))) return st;
@@ -462,6 +473,7 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid) {
   hipLaunchKernelGGL(kMassSamples, dim3(cdiv(d.cells, 128)), dim3(128), 0, ctx->stream, d, a);
   hipLaunchKernelGGL(kMassGather, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, a);
   so.listDirty = true;
+  ctx->massGen++;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 

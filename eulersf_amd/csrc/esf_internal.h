@@ -91,6 +91,10 @@ struct esf_ctx {
   int* sidx = nullptr;        // nodes -> solid DOF node index or -1
   int* solidNodes = nullptr;  // solid DOF node -> node index
   int nSolid = 0, capSolid = 0;
+  // generation of the solids' masses and quarter fractions (bumped by every
+  // write of them); the gathered fields above are rebuilt only when it moved
+  long massGen = 1, gatheredGen = 0;
+  int gatheredSolids = -1;
   uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
   double* inbox = nullptr;      // solid b at 24 capSolid b: 24 doubles per node
   int inboxSolids = 0;          // solids the inbox was sized for

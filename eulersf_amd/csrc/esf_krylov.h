@@ -46,10 +46,15 @@ enum FinishOp {
   F_SOLID_CR = 6, // delta, beta from t0; alpha from t1 + 2 beta t2 + beta^2 t3
 };
 
+// hostOut (pinned host memory, may be null): the scalar block after the
+// update, written by the kernel itself, so a host poll needs no copy
 template <int OP, int K>
 __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, CrScalars* sc,
-                                                int iter, int first) {
-  if (sc->done) return;
+                                                int iter, int first, CrScalars* hostOut) {
+  if (sc->done) {
+    if (hostOut && threadIdx.x == 0) *hostOut = *sc;
+    return;
+  }
   __shared__ double red[16];
   double t[K];
   finishSums<K>(nb, partials, red, t);
@@ -86,23 +91,49 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
     sc->beta = beta;
     sc->alpha = (fabs(apq) > 0.0) ? dNew / apq : apq;
   }
+  if (hostOut) *hostOut = *sc;
 }
 
-// partials: the producer's block partials (default: the context's buffer)
+// partials: the producer's block partials (default: the context's buffer);
+// hostOut: pinned slot that receives the updated scalar block
 template <int OP, int K>
 inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int first = 0,
-                         const double* partials = nullptr) {
+                         const double* partials = nullptr, CrScalars* hostOut = nullptr) {
   hipLaunchKernelGGL((kFinish<OP, K>), dim3(1), dim3(1024), 0, ctx->stream, nb,
-                     partials ? partials : (const double*)ctx->partials, sc, iter, first);
+                     partials ? partials : (const double*)ctx->partials, sc, iter, first,
+                     hostOut);
+}
+
+// a solve's scalar block at its start: tolerance eps, nothing done yet
+__global__ void kScalarsInit(CrScalars* sc, double eps) {
+  if (threadIdx.x != 0) return;
+  CrScalars h{};
+  h.eps = eps;
+  h.iter = -1;
+  *sc = h;
+}
+
+// a scalar block into pinned host memory (the report of a queued solve)
+__global__ void kScalarsCopy(const CrScalars* sc, CrScalars* out) {
+  if (threadIdx.x == 0) *out = *sc;
+}
+
+inline void initScalarsAsync(esf_ctx* ctx, CrScalars* sc, double eps) {
+  hipLaunchKernelGGL(kScalarsInit, dim3(1), dim3(64), 0, ctx->stream, sc, eps);
+}
+
+inline void copyScalarsAsync(esf_ctx* ctx, const CrScalars* sc, CrScalars* out) {
+  hipLaunchKernelGGL(kScalarsCopy, dim3(1), dim3(64), 0, ctx->stream, sc, out);
 }
 
 }  // namespace
 
 // Poll the scalar block `which` of a Krylov solve without stalling the
-// queue: after each convergence check an async copy of the block lands in
-// a pinned ring slot behind an event; the host keeps one check interval
-// queued ahead and waits on the previous one, so the device never idles
-// and at most one interval of early-exiting launches follows convergence.
+// queue: the finisher of each convergence check writes the block into the
+// pinned ring slot slot() and an event follows it; the host keeps one check
+// interval queued ahead and waits on the previous one, so the device never
+// idles and at most one interval of early-exiting launches follows
+// convergence.
 // With `stagnation` set (the FP32 inner solve of the mixed form), the poll
 // also ends the solve when four consecutive checks set no new minimum of
 // the checked residual: FP32 rounding puts a noise floor under the inner
@@ -134,11 +165,12 @@ struct DonePoller {
     }
     return false;
   }
-  // returns true when the solve is known to be done (or stagnated)
+  // the pinned slot the next check's finisher writes
+  CrScalars* slot() { return &ctx->hostRing[issued % kRing]; }
+  // after that finisher is queued: returns true when the solve is known to
+  // be done (or stagnated)
   bool mark() {
     const int slot = issued % kRing;
-    hipMemcpyAsync(&ctx->hostRing[slot], &ctx->sc[which], sizeof(CrScalars),
-                   hipMemcpyDeviceToHost, ctx->stream);
     hipEventRecord(ev[slot], ctx->stream);
     issued++;
     while (issued - waited > kLook) {

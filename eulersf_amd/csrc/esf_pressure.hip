@@ -50,6 +50,7 @@ struct SpmvArgs {
   double* partials;
   int iter;
   int first;         // M_IR_RESID: also compute |b|
+  CrScalars* hostOut;  // M_CHECK: pinned slot for the checked scalar block
 };
 
 constexpr int TX = 32, TY = 8, TZ = 8;
@@ -335,7 +336,8 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
   if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
   if constexpr (MODE == M_UPDATE) launchFinish<F_UPDATE, 1>(ctx, nb, a.sc);
-  if constexpr (MODE == M_CHECK) launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter);
+  if constexpr (MODE == M_CHECK)
+    launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter, 0, nullptr, a.hostOut);
   if constexpr (MODE == M_IR_RESID) launchFinish<F_IR, 2>(ctx, nb, a.sc, 0, a.first);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -442,18 +444,13 @@ esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y) {
 }
 
 static esf_status initScalars(esf_ctx* ctx, int which, double eps) {
-  CrScalars h{};
-  h.eps = eps;
-  h.iter = -1;
-  ESF_HIP(hipMemcpyAsync(&ctx->sc[which], &h, sizeof(h), hipMemcpyHostToDevice,
-                         ctx->stream));
+  initScalarsAsync(ctx, &ctx->sc[which], eps);
   return ESF_OK;
 }
 
 // synchronous read of scalar block `which` (through report slot 3)
 static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
-  ESF_HIP(hipMemcpyAsync(&ctx->hostSc[3], &ctx->sc[which], sizeof(CrScalars),
-                         hipMemcpyDeviceToHost, ctx->stream));
+  copyScalarsAsync(ctx, &ctx->sc[which], &ctx->hostSc[3]);
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   *out = ctx->hostSc[3];
   return ESF_OK;
@@ -502,6 +499,7 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     std::swap(rA, rB);
     if (i > 0 && i % 5 == 0) {
       a.in = x;
+      a.hostOut = poll.slot();
       if ((st = launchSpmv<T, M_CHECK>(ctx, a))) return st;
       if (poll.mark()) {
         i++;
@@ -510,10 +508,9 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     }
   }
   ctx->crQueued = i;
-  // the report is copied without waiting; the caller reads report slot
+  // the report is written without waiting; the caller reads report slot
   // `slot` after its next synchronisation
-  ESF_HIP(hipMemcpyAsync(&ctx->hostSc[slot], &ctx->sc[which], sizeof(CrScalars),
-                         hipMemcpyDeviceToHost, ctx->stream));
+  copyScalarsAsync(ctx, &ctx->sc[which], &ctx->hostSc[slot]);
   return ESF_OK;
 }
 

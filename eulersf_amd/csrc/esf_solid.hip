@@ -683,11 +683,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
                std::to_string(256L * 64 * kScalMaxGroups / 3) + ")";
     return ESF_ERR_INVALID;
   }
-  CrScalars h0{};
-  h0.eps = eps;
-  h0.iter = -1;
-  ESF_HIP(hipMemcpyAsync(&ctx->sc[which], &h0, sizeof(h0), hipMemcpyHostToDevice, ctx->stream));
   CrScalars* sc = &ctx->sc[which];
+  initScalarsAsync(ctx, sc, eps);
   const int eb = std::max(1, std::min(cdiv(n, 1024), 256));
   hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
                      (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->partials);
@@ -722,14 +719,13 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
       a.in = x;
       a.partials = checkSet;
       nodePass<S_CHECK>(ctx, a);
-      launchFinish<F_CHECK, 1>(ctx, nb, sc, i, 0, checkSet);
+      launchFinish<F_CHECK, 1>(ctx, nb, sc, i, 0, checkSet, poll.slot());
       if (poll.mark()) break;
     }
   }
-  // the report is copied without waiting; the caller reads report slot
+  // the report is written without waiting; the caller reads report slot
   // `slot` after its next synchronisation
-  ESF_HIP(hipMemcpyAsync(&ctx->hostSc[slot], sc, sizeof(CrScalars), hipMemcpyDeviceToHost,
-                         ctx->stream));
+  copyScalarsAsync(ctx, sc, &ctx->hostSc[slot]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("solid solve: ") + hipGetErrorString(e);
