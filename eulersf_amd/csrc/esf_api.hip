@@ -77,7 +77,8 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   const size_t cap = std::max<size_t>(need + need / 4 + 64, 256);
   double** vecs[] = {&ctx->sForce, &ctx->sMv,  &ctx->sDamped, &ctx->sKdiag, &ctx->sAdiag,
                      &ctx->sRhs,   &ctx->sV0,  &ctx->sDv,     &ctx->sB,     &ctx->sr,
-                     &ctx->sp,     &ctx->ss,   &ctx->sAp,     &ctx->stmp};
+                     &ctx->sp,     &ctx->ss,   &ctx->sAp,     &ctx->stmp,   &ctx->sSd,
+                     &ctx->sApd};
   for (double** v : vecs) {
     esf_status st = dalloc(ctx, v, 3 * cap);
     if (st) return st;
@@ -117,7 +118,8 @@ static void freeAll(esf_ctx* ctx) {
                   ctx->sDamped, ctx->sKdiag, ctx->sAdiag, ctx->sRhs,      ctx->sV0,
                   ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
                   ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
-                  ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox};
+                  ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox,
+                  ctx->sSd,    ctx->sApd};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& so : ctx->solids) {

@@ -884,17 +884,36 @@ __global__ void kSolidExtrap(Dims d, Box bx, int axis, double* u) {
   }
 }
 
-// X = x - u (:1222-1230) and _vecWorkspace = _velocity (:1232)
-__global__ void kSolidRecoverX(Dims d, const double* u, const double* sv, double* X,
-                               double* svws) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= d.nodes) return;
+// X = x - u (:1222-1230) and _vecWorkspace = _velocity (:1232), over the
+// 3 N doubles of the AoS fields as flat pairs (16-byte accesses, every
+// lane of a load on consecutive bytes); element i is component i mod 3 of
+// node i / 3
+__device__ inline double recoverX(const Dims& d, long i, double u) {
+  const long n = i / 3;
+  const int c = (int)(i - 3 * n);
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
-  const double pos[3] = {x * d.dh, y * d.dh, z * d.dh};
-  const double o[3] = {d.o0, d.o1, d.o2};
-  for (int c = 0; c < 3; c++) {
-    X[3 * n + c] = pos[c] + (o[c] - u[3 * n + c]);
-    svws[3 * n + c] = sv[3 * n + c];
+  const double pos = (c == 0 ? x : (c == 1 ? y : z)) * d.dh;
+  const double o = c == 0 ? d.o0 : (c == 1 ? d.o1 : d.o2);
+  return pos + (o - u);
+}
+
+__global__ void __launch_bounds__(256) kSolidRecoverX(Dims d, const double* __restrict__ u,
+                                                      const double* __restrict__ sv,
+                                                      double* __restrict__ X,
+                                                      double* __restrict__ svws) {
+  const long n3 = 3 * d.nodes;
+  const long i = 2 * (blockIdx.x * (long)blockDim.x + threadIdx.x);
+  if (i + 1 < n3) {
+    const double2 uu = *reinterpret_cast<const double2*>(u + i);
+    const double2 vv = *reinterpret_cast<const double2*>(sv + i);
+    double2 xx;
+    xx.x = recoverX(d, i, uu.x);
+    xx.y = recoverX(d, i + 1, uu.y);
+    *reinterpret_cast<double2*>(X + i) = xx;
+    *reinterpret_cast<double2*>(svws + i) = vv;
+  } else if (i < n3) {
+    X[i] = recoverX(d, i, u[i]);
+    svws[i] = sv[i];
   }
 }
 
@@ -1038,7 +1057,8 @@ esf_status solidFinalize(esf_ctx* ctx, double dt) {
         hipLaunchKernelGGL(kSolidExtrap, dim3(cdiv(cnt, 256)), dim3(256), 0, ctx->stream, ctx->d,
                            b, a, so.u);
     }
-    LAUNCH_NODES(kSolidRecoverX, ctx->d, so.u, so.vel, so.X, so.velws);
+    hipLaunchKernelGGL(kSolidRecoverX, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)), dim3(256), 0,
+                       ctx->stream, ctx->d, so.u, so.vel, so.X, so.velws);
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }

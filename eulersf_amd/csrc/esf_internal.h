@@ -114,7 +114,7 @@ struct esf_ctx {
          *sKdiag = nullptr, *sAdiag = nullptr, *sRhs = nullptr,
          *sV0 = nullptr, *sDv = nullptr, *sB = nullptr;
   double *sr = nullptr, *sp = nullptr, *ss = nullptr, *sAp = nullptr,
-         *stmp = nullptr;
+         *stmp = nullptr, *sSd = nullptr, *sApd = nullptr;  // sSd = D^-1 s, sApd = D^-1 Ap
   // reductions
   double* partials = nullptr;  // per-block partial sums
   double* sPartials = nullptr; // solid CR: 4 sets of 4 x node-pass blocks

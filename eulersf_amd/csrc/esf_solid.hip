@@ -277,9 +277,13 @@ struct SolidPartials {
 // node: 3 rows for each of its 8 cells), so the node pass reads them
 // without an index table.  Masked corners write nothing.
 //   CELL_IN : v = in                      (apply, residual check)
-//   CELL_CR : v = r - alpha (Ap beta + s) / D, the CR residual update of
-//             CONJUGATE_RESIDUAL.inl:223-230 with the direction update of
-//             :257-262 folded in; the node kernel forms the same values
+//   CELL_CR : v = r - alpha (D^-1 Ap beta + D^-1 s), the CR residual update
+//             of CONJUGATE_RESIDUAL.inl:223-230 (r -= alpha D^-1 Ap) with the
+//             direction update of :257-262 (Ap = Ap beta + s) folded in; the
+//             node pass keeps D^-1 Ap and D^-1 s beside Ap and s, so no lane
+//             divides, and forms the same values with the same expression
+// FMA contraction is on inside the two loop kernels (the products here are
+// matrix-free and need no bitwise match with the reference's CSR order).
 enum CellMode { CELL_IN = 0, CELL_CR = 1 };
 constexpr int kCellLanes = 2;  // lanes per cell of the solid cell pass
 
@@ -296,12 +300,12 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
                                                   const int* __restrict__ cornerDof,
                                                   const double* __restrict__ cellMat,
                                                   const double* __restrict__ in,
-                                                  const double* __restrict__ s,
-                                                  const double* __restrict__ Ap,
-                                                  const double* __restrict__ diag,
+                                                  const double* __restrict__ sd,
+                                                  const double* __restrict__ apd,
                                                   const CrScalars* sc, SolidPartials sp,
                                                   double lambda, double mu,
                                                   double* __restrict__ inbox) {
+#pragma clang fp contract(on)
   static_assert(LPC == 2 || LPC == 4 || LPC == 8, "lanes per cell");
   constexpr int CPL = 8 / LPC;  // corners per lane
   __shared__ double grp[5 * kScalMaxGroups];
@@ -326,8 +330,8 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
     L[q] = m[(long)(9 + q) * cap];
   }
   const double Jinv = m[27L * cap];
-  // second round: the corners' r (or v), s, Ap, D
-  double rr[CPL][3], ss[CPL][3], aa[CPL][3], dd[CPL][3];
+  // second round: the corners' r (or v), D^-1 s, D^-1 Ap
+  double rr[CPL][3], ss[CPL][3], aa[CPL][3];
 #pragma unroll
   for (int j = 0; j < CPL; j++) {
     const int k = kraw[j] >= 0 ? kraw[j] : 0;
@@ -336,11 +340,9 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
       rr[j][q] = in[k + q];
       ss[j][q] = 0;
       aa[j][q] = 0;
-      dd[j][q] = 1;
       if (MODE == CELL_CR) {
-        ss[j][q] = s[k + q];
-        aa[j][q] = Ap[k + q];
-        dd[j][q] = diag[k + q];
+        ss[j][q] = sd[k + q];
+        aa[j][q] = apd[k + q];
       }
     }
   }
@@ -355,7 +357,7 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
     for (int j = 0; j < CPL; j++)
 #pragma unroll
       for (int q = 0; q < 3; q++)
-        v[j][q] = rr[j][q] - alpha * ((aa[j][q] * beta + ss[j][q]) / dd[j][q]);
+        v[j][q] = rr[j][q] - alpha * (aa[j][q] * beta + ss[j][q]);
   } else {
 #pragma unroll
     for (int j = 0; j < CPL; j++)
@@ -421,6 +423,8 @@ struct SolidNodeArgs {
   double* p;              // S_CR
   double* s;              // S_SETUP, S_CR
   double* Ap;             // S_CR
+  double* sd;             // S_SETUP, S_CR: D^-1 s, read by the cell pass
+  double* apd;            // S_CR: D^-1 Ap
   const CrScalars* sc;
   SolidPartials sp;       // S_CR: where alpha and beta come from
   double* partials;       // this pass's block partials
@@ -431,13 +435,15 @@ struct SolidNodeArgs {
 // epilogues (block partials, summed by the consumers):
 //   S_SETUP : s = A r;  r.s, s.D^-1 s
 //   S_CR    : Ap = Ap beta + s, p = p beta + r, x += alpha p,
-//             r -= alpha Ap / D, s = A r;  r.s, s.D^-1 s, Ap.D^-1 s,
+//             r -= alpha (D^-1 Ap beta + D^-1 s) (as the cell pass), s = A r;
+//             D^-1 Ap and D^-1 s kept;  r.s, s.D^-1 s, Ap.D^-1 s,
 //             Ap.D^-1 Ap
 //   S_CHECK : |A x - b|^2  (every 5 iterations, CONJUGATE_RESIDUAL.inl:246-255)
 // All loads are in the first round (indices clamped in bounds, masked
 // afterwards), so a pass costs one memory round trip.
 template <int MODE>
 __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
+#pragma clang fp contract(on)
   __shared__ double red[16];
   __shared__ double grp[5 * kScalMaxGroups];
   const long nDof = 3L * a.nSolid;
@@ -455,7 +461,7 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
 #pragma unroll
   for (int c = 0; c < 8; c++) rows[c] = in0[3 * c];
   const double dm = a.damped[k];
-  double vin = 0, dg = 1, ap0 = 0, s0 = 0, p0 = 0, r0 = 0, x0 = 0, bb = 0;
+  double vin = 0, dg = 1, ap0 = 0, s0 = 0, p0 = 0, r0 = 0, x0 = 0, bb = 0, apd0 = 0, sd0 = 0;
   if (MODE == S_CR) {
     dg = a.diag[k];
     ap0 = a.Ap[k];
@@ -463,6 +469,8 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
     p0 = a.p[k];
     r0 = a.r[k];
     x0 = a.x[k];
+    apd0 = a.apd[k];
+    sd0 = a.sd[k];
   } else {
     vin = a.in[k];
     if (MODE == S_SETUP) dg = a.diag[k];
@@ -493,19 +501,21 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   if (MODE == S_CR) {
     const double ap = ap0 * beta + s0;
     const double pp = p0 * beta + r0;
-    const double rn = r0 - alpha * (ap / dg);
+    const double rn = r0 - alpha * (apd0 * beta + sd0);
     const double y = kx * a.scale + dm * rn;
     if (live) {
+      const double yq = y / dg, aq = ap / dg;
       a.x[k] = x0 + alpha * pp;
       a.Ap[k] = ap;
       a.p[k] = pp;
       a.r[k] = rn;
       a.s[k] = y;
-      const double yq = y / dg;
+      a.sd[k] = yq;
+      a.apd[k] = aq;
       part[0] = rn * y;
       part[1] = y * yq;
       part[2] = ap * yq;
-      part[3] = ap * (ap / dg);
+      part[3] = ap * aq;
     }
   } else {
     const double y = kx * a.scale + dm * vin;
@@ -513,9 +523,11 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
       if (MODE == S_APPLY) {
         a.out[k] = y;
       } else if (MODE == S_SETUP) {
+        const double yq = y / dg;
         a.s[k] = y;
+        a.sd[k] = yq;
         part[0] = vin * y;
-        part[1] = y * (y / dg);
+        part[1] = y * yq;
       } else {
         const double t = y - bb;
         part[0] = t * t;
@@ -597,7 +609,7 @@ static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
                   dim3(cdiv((long)kCellLanes * so.nValid, 256)), dim3(256), ctx->d, so.nValid,
                   so.capValid,
                   (const int*)so.cornerDof, (const double*)so.cellMat, in,
-                  (const double*)ctx->ss, (const double*)ctx->sAp, (const double*)ctx->sAdiag,
+                  (const double*)ctx->sSd, (const double*)ctx->sApd,
                   sc, sp, so.lambda, so.mu, ctx->inbox + 24L * ctx->capSolid * b);
     }
   }
@@ -648,7 +660,7 @@ esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
 // no separate direction pass (and no third grid-wide reduction) is needed.
 // Only the rounding of alpha differs from the reference's evaluation.
 __global__ void kSolidSetup(long n, const double* b, const double* diag, double* x, double* r,
-                            double* p, double* Ap, double* partials) {
+                            double* p, double* Ap, double* apd, double* partials) {
   __shared__ double red[16];
   double part = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
@@ -657,6 +669,7 @@ __global__ void kSolidSetup(long n, const double* b, const double* diag, double*
     r[i] = b[i] / diag[i];
     p[i] = 0.0;   // with beta = 0 the first direction is p = r, Ap = s
     Ap[i] = 0.0;
+    apd[i] = 0.0;
     part += b[i] * b[i];
   }
   double mine[1] = {part};
@@ -687,7 +700,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   initScalarsAsync(ctx, sc, eps);
   const int eb = std::max(1, std::min(cdiv(n, 1024), 256));
   hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
-                     (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->partials);
+                     (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->sApd,
+                     ctx->partials);
   launchFinish<F_BNORM, 1>(ctx, eb, sc);
   // partial sets: node pass i writes set i mod 3 (the setup pass set 2),
   // the residual check its own set
@@ -702,6 +716,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   a.p = ctx->sp;
   a.s = ctx->ss;
   a.Ap = ctx->sAp;
+  a.sd = ctx->sSd;
+  a.apd = ctx->sApd;
   a.sc = sc;
   cellPass<CELL_IN>(ctx, ctx->sr, sc, SolidPartials{});
   a.in = ctx->sr;
