@@ -83,17 +83,18 @@ def kernel_bytes(cls, n_cells, n_int, n_sc=0, n_sn=0):
     every cell/node stream the kernel must read or write once.
     Solid classes (n_sc valid solid cells, n_sn solid DOF nodes):
       cell pass: per cell R, L, F, 1/J (28 f64), 8 corner DOF indices, 24 f64
-                 out; per node r, s, Ap, D (4 x 3 f64)
-      node pass: per node 8 gather slots, x, r, p, s, Ap, D, M~ read and
-                 x, r, p, s, Ap written (12 x 3 f64); per cell 24 f64 gathered"""
+                 out; per node r, D^-1 s, D^-1 Ap (3 x 3 f64)
+      node pass: per node 8 gather slots, x, r, p, s, Ap, D^-1 s, D^-1 Ap, D,
+                 M~ read and x, r, p, s, Ap, D^-1 s, D^-1 Ap written
+                 (16 x 3 f64); per cell 24 f64 gathered"""
     if cls == "solid_cell_cr":
-        return n_sc * (28 * 8 + 8 * 4 + 24 * 8) + n_sn * 4 * 24
+        return n_sc * (28 * 8 + 8 * 4 + 24 * 8) + n_sn * 3 * 24
     if cls == "solid_node_cr":
-        return n_sn * (8 * 4 + 12 * 24) + n_sc * 24 * 8
+        return n_sn * (8 * 4 + 16 * 24) + n_sc * 24 * 8
     w = 4 if "32" in cls else 8
     c, i = n_cells, n_int
     table = {"apply": w * (2 * c + i), "setup_r": w * (5 * c + i), "setup_s": w * (4 * c + i),
-             "update": w * (8 * c + i), "check": w * (2 * c + i),
+             "update": w * (11 * c + i), "check": w * (2 * c + i),
              "ir_resid": 8 * c + 4 * c + 8 * c + 8 * i + 8 * c + 4 * c, "dir": w * 7 * c}
     for k, v in table.items():
         if cls.endswith(k) or (k == "dir" and cls.startswith("cr_dir")):

@@ -13,7 +13,7 @@ esf_status pressureRhs(esf_ctx* ctx);
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D, T* x, T* r,
-                   T* r2, T* p, T* s, T* Ap, double eps, int maxIt, int slot);
+                   T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt, int slot);
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
                       double* relres, int* refinements, int* innerIters);
 int spmvBlocks(const Dims& d);
@@ -119,7 +119,8 @@ static void freeAll(esf_ctx* ctx) {
                   ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
                   ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
                   ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox,
-                  ctx->sSd,    ctx->sApd};
+                  ctx->sSd,    ctx->sApd,   ctx->s2,      ctx->Ap2,       ctx->s32b,
+                  ctx->Ap32b};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& so : ctx->solids) {
@@ -214,12 +215,13 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
             {&ctx->minv, N},     {&ctx->c, C},         {&ctx->b, C},
             {&ctx->D, C},        {&ctx->lambda, C},    {&ctx->lambda2, C},
             {&ctx->r, C},        {&ctx->r2, C},        {&ctx->pdir, C},
-            {&ctx->s, C},        {&ctx->Ap, C}};
+            {&ctx->s, C},        {&ctx->Ap, C},        {&ctx->s2, C},
+            {&ctx->Ap2, C}};
   for (auto& e : dd)
     if (!st) st = dalloc(ctx, e.p, e.n);
   if (!st && mixed) {
-    float** fs[] = {&ctx->D32, &ctx->e32, &ctx->r32, &ctx->r32b,
-                    &ctx->p32, &ctx->s32, &ctx->Ap32, &ctx->rhs32};
+    float** fs[] = {&ctx->D32, &ctx->e32, &ctx->r32,  &ctx->r32b, &ctx->p32,
+                    &ctx->s32, &ctx->s32b, &ctx->Ap32, &ctx->Ap32b, &ctx->rhs32};
     for (float** f : fs)
       if (!st) st = dalloc(ctx, f, C);
     if (!st) st = dalloc(ctx, &ctx->minv32, N);
@@ -564,7 +566,7 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
       s->converged[1] = conv;
     } else {
       st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
-                           ctx->pdir, ctx->s, ctx->Ap, ctx->p.divfree_eps, maxIt, 1);
+                           ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, ctx->p.divfree_eps, maxIt, 1);
     }
     if (st) return st;
     if ((st = correctVelocity(ctx))) return st;
@@ -704,7 +706,7 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double e
   bool conv = false;
   double rr = 0;
   esf_status st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r,
-                                  ctx->r2, ctx->pdir, ctx->s, ctx->Ap, eps, maxIt, 3);
+                                  ctx->r2, ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 3);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(x, ctx->lambda, bytes, hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipStreamSynchronize(ctx->stream));
@@ -726,7 +728,7 @@ esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int maxIt, int*
     st = mixedSolve(ctx, eps, maxIt, &it, &conv, &rr, &outer, &inner);
   } else {
     st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
-                         ctx->pdir, ctx->s, ctx->Ap, eps, maxIt, 3);
+                         ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 3);
   }
   if (st) return st;
   ESF_HIP(hipEventRecord(ctx->ev[5], ctx->stream));

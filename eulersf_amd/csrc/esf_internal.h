@@ -106,8 +106,10 @@ struct esf_ctx {
   double *c = nullptr, *b = nullptr, *D = nullptr, *lambda = nullptr,
          *lambda2 = nullptr;
   double *r = nullptr, *r2 = nullptr, *pdir = nullptr, *s = nullptr, *Ap = nullptr;
+  double *s2 = nullptr, *Ap2 = nullptr;  // second buffers of s and Ap (CR loop)
   float *D32 = nullptr, *e32 = nullptr, *r32 = nullptr, *r32b = nullptr,
         *p32 = nullptr, *s32 = nullptr, *Ap32 = nullptr, *rhs32 = nullptr;
+  float *s32b = nullptr, *Ap32b = nullptr;
   bool lambdaInit = false;
   // solid DOF vectors (3*capSolid)
   double *sForce = nullptr, *sMv = nullptr, *sDamped = nullptr,

@@ -1,5 +1,6 @@
 // esf_krylov.h -- pieces shared by the FP64/FP32 pressure solve and the
-// solid solve: the CR direction update and the host-side done-flag poller.
+// solid solve: the reduction finishers, the scalar-block kernels and the
+// host-side done-flag poller.
 #pragma once
 #include <algorithm>
 
@@ -8,30 +9,6 @@
 namespace esf {
 namespace {
 
-// p = beta p + r ; Ap = beta Ap + s ; partial Ap . (Ap / D) -> alpha
-// (CONJUGATE_RESIDUAL.inl:139-160: _direction *= beta; += _residual; same
-// for _Ap; the next iteration's q = Ap / D and alpha = Ap . q)
-template <typename T>
-__global__ void __launch_bounds__(256) kCrDirection(long n, T* p, T* Ap, const T* r,
-                                                    const T* s, const T* D,
-                                                    const CrScalars* sc, double* partials) {
-  if (sc->done) return;
-  __shared__ double red[16];
-  const T beta = (T)sc->beta;
-  double part = 0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const T pn = p[i] * beta + r[i];
-    const T an = Ap[i] * beta + s[i];
-    p[i] = pn;
-    Ap[i] = an;
-    const T q = an / D[i];
-    part += (double)an * (double)q;
-  }
-  double mine[1] = {part};
-  blockPartials<1>(mine, partials, red);
-}
-
 // What the finisher does with the totals (t0, t1, ...) of the producer
 // that ran just before it.  The scalar updates follow CONJUGATE_RESIDUAL.inl
 // (:118-131 setup, :141-160 loop, :166-176 convergence test); alpha stays
@@ -39,11 +16,9 @@ __global__ void __launch_bounds__(256) kCrDirection(long n, T* p, T* Ap, const T
 enum FinishOp {
   F_BNORM = 0,    // |b| = sqrt(t0)
   F_SETUP_S = 1,  // delta = t0 (r.s); alpha = delta / t1 (Ap.D^-1 Ap)
-  F_UPDATE = 2,   // beta = t0 / delta; delta = t0
   F_CHECK = 3,    // |Ax - b| = sqrt(t0); done when below eps |b|
   F_IR = 4,       // outer residual sqrt(t0); |b| = sqrt(t1) on the first call
-  F_DIR = 5,      // alpha = delta / t0
-  F_SOLID_CR = 6, // delta, beta from t0; alpha from t1 + 2 beta t2 + beta^2 t3
+  F_CR = 6,       // delta, beta from t0; alpha from t1 + 2 beta t2 + beta^2 t3
 };
 
 // hostOut (pinned host memory, may be null): the scalar block after the
@@ -65,10 +40,6 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
     sc->delta = t[0];
     sc->beta = 0.0;
     sc->alpha = (fabs(t[K - 1]) > 0.0) ? t[0] / t[K - 1] : t[K - 1];
-  } else if constexpr (OP == F_UPDATE) {
-    const double dOld = sc->delta;
-    sc->delta = t[0];
-    sc->beta = t[0] / dOld;
   } else if constexpr (OP == F_CHECK) {
     const double rn = sqrt(t[0]);
     sc->resnorm = rn;
@@ -80,10 +51,12 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
     sc->resnorm = sqrt(t[0]);
     if (first) sc->bnorm = sqrt(t[K - 1]);
     if (sc->resnorm < sc->eps * sc->bnorm) sc->done = 1;
-  } else if constexpr (OP == F_DIR) {
-    sc->alpha = (fabs(t[0]) > 0.0) ? sc->delta / t[0] : t[0];
-  } else if constexpr (OP == F_SOLID_CR) {
-    static_assert(K == 4, "solid CR finisher takes four sums");
+  } else if constexpr (OP == F_CR) {
+    // r.s, s.D^-1 s, Ap.D^-1 s, Ap.D^-1 Ap of the iteration just done; alpha
+    // for the next direction Ap' = s + beta Ap (CONJUGATE_RESIDUAL.inl:
+    // 141-160) with (Ap'.D^-1 Ap') expanded; alpha stays the denominator
+    // when that is zero, as there
+    static_assert(K == 4, "CR finisher takes four sums");
     const double dNew = t[0];
     const double beta = dNew / sc->delta;
     const double apq = (t[1] + (2.0 * beta) * t[2]) + (beta * beta) * t[3];

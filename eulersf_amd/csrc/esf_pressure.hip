@@ -11,13 +11,19 @@
 // (8 B each in FP64).
 //
 // The Krylov loop is CONJUGATE_RESIDUAL::solvePCR (CONJUGATE_RESIDUAL.inl:
-// 109-192) split into two launches per iteration:
-//   update+apply : x += a p, r -= a D^-1 Ap (recomputed on the halo),
-//                  s = S r, partial r.s          -> beta
-//   direction    : p = beta p + r, Ap = beta Ap + s, partial Ap.D^-1 Ap -> alpha
-// plus a true-residual check every 5 iterations, as the reference does.
-// Block partials are summed by a one-block finisher launched after each
-// reducing kernel (fixed order, deterministic).
+// 109-192), one SpMV launch per iteration with the direction update of the
+// previous iteration folded in:
+//   Ap = beta Ap + s, p = beta p + r (:157-160), x += alpha p,
+//   r -= alpha D^-1 Ap (recomputed on the halo), s = S r; partial sums r.s,
+//   s.D^-1 s, Ap.D^-1 s, Ap.D^-1 Ap
+// and one finisher: delta and beta from r.s, alpha = delta / (Ap.D^-1 Ap)
+// for the next direction Ap = s + beta Ap with the square expanded, so a
+// single reduction per iteration provides both (only alpha's rounding
+// differs from the reference's evaluation).  s and Ap alternate between two
+// buffers (halo cells read the previous values while owners write the new
+// ones), as r does.  A true-residual check every 5 iterations, as the
+// reference does.  Block partials are summed by a one-block finisher (fixed
+// order, deterministic).
 #include "esf_internal.h"
 #include "esf_krylov.h"
 
@@ -42,8 +48,10 @@ struct SpmvArgs {
   T* x;              // solution (M_UPDATE, M_IR_RESID)
   T* r;              // residual
   T* p;              // direction
-  T* s;              // S r
-  T* Ap;             // A p
+  T* s;              // S r (M_UPDATE: the previous s, read)
+  T* Ap;             // A p (M_UPDATE: the previous Ap, read)
+  T* sOut;           // M_UPDATE: the new s
+  T* ApOut;          // M_UPDATE: the new Ap
   const float* e32;  // M_IR_RESID: correction added to x (may be null)
   float* r32;        // M_IR_RESID: residual handed to the FP32 solve
   CrScalars* sc;
@@ -64,7 +72,7 @@ constexpr int NPT = (NODES + 255) / 256;   // nodes per thread
 // layer's arithmetic)
 template <typename T>
 struct Fetched {
-  T in, ap, d, x, p;
+  T in, ap, s, d, x, p;
   float e;
   bool valid, own;
   long c;
@@ -88,10 +96,13 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int zEnd = min(z0 + TZ, d.CZ);
 
-  T alpha = 0;
-  if (MODE == M_UPDATE) alpha = (T)a.sc->alpha;
+  T alpha = 0, beta = 0;
+  if (MODE == M_UPDATE) {
+    alpha = (T)a.sc->alpha;
+    beta = (T)a.sc->beta;
+  }
 
-  double part0 = 0, part1 = 0;
+  double part0 = 0, part1 = 0, part2 = 0, part3 = 0;
 
   Fetched<T> f[HPT];
   auto fetch = [&](int cz) {
@@ -110,6 +121,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
       q.in = a.in[q.c];
       if (MODE == M_UPDATE) {
         q.ap = a.Ap[q.c];
+        q.s = a.s[q.c];
         q.d = a.D[q.c];
         if (q.own) {
           q.x = a.x[q.c];
@@ -130,12 +142,17 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
       T v = 0;
       if (q.valid) {
         if (MODE == M_UPDATE) {
-          // r_old read on the halo, r_new written to the other buffer
-          const T qq = q.ap / q.d;
+          // r_old, Ap_old, s_old read on the halo; r_new (and at own cells
+          // Ap, p, x) written to the other buffers
+          const T apn = q.ap * beta + q.s;
+          const T qq = apn / q.d;
           v = q.in - alpha * qq;
           if (q.own) {
+            const T pn = q.p * beta + q.in;
             a.r[q.c] = v;
-            a.x[q.c] = q.x + alpha * q.p;
+            a.ApOut[q.c] = apn;
+            a.p[q.c] = pn;
+            a.x[q.c] = q.x + alpha * pn;
           }
         } else if (MODE == M_IR_RESID) {
           // x_old (+ e) read on the halo, x_new written to the other buffer
@@ -258,8 +275,15 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
         const T q = acc / a.D[c];
         part1 += (double)acc * (double)q;
       } else if (MODE == M_UPDATE) {
-        a.s[c] = acc;
+        a.sOut[c] = acc;
+        // this cell's new direction, formed as on the halo (cache hits)
+        const T apn = a.Ap[c] * beta + a.s[c];
+        const T dc = a.D[c];
+        const T sq = acc / dc;
         part0 += (double)ownIn * (double)acc;
+        part1 += (double)acc * (double)sq;
+        part2 += (double)apn * (double)sq;
+        part3 += (double)apn * (double)(apn / dc);
       } else if (MODE == M_CHECK) {
         const T t = acc - a.b[c];
         part0 += (double)t * (double)t;
@@ -273,8 +297,13 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
     }
   }
   if (MODE == M_APPLY) return;
-  double mine[2] = {part0, part1};
-  blockPartials<2>(mine, a.partials, red);
+  if (MODE == M_UPDATE) {
+    double mine[4] = {part0, part1, part2, part3};
+    blockPartials<4>(mine, a.partials, red);
+  } else {
+    double mine[2] = {part0, part1};
+    blockPartials<2>(mine, a.partials, red);
+  }
 }
 
 // b = J v0 - c, c (boundary velocities), D = diag(S); also the FP32 copies
@@ -335,7 +364,7 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   // the grid-wide sums of the mode (kSpmv's epilogue partials)
   if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
   if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
-  if constexpr (MODE == M_UPDATE) launchFinish<F_UPDATE, 1>(ctx, nb, a.sc);
+  if constexpr (MODE == M_UPDATE) launchFinish<F_CR, 4>(ctx, nb, a.sc);
   if constexpr (MODE == M_CHECK)
     launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter, 0, nullptr, a.hostOut);
   if constexpr (MODE == M_IR_RESID) launchFinish<F_IR, 2>(ctx, nb, a.sc, 0, a.first);
@@ -461,7 +490,8 @@ static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
 // reference's iteration count; *relres the last checked |Sx-b|/|b|.
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
-                   T* x, T* r, T* r2, T* p, T* s, T* Ap, double eps, int maxIt, int slot) {
+                   T* x, T* r, T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt,
+                   int slot) {
   esf_status st = initScalars(ctx, which, eps);
   if (st) return st;
   const Dims& d = ctx->d;
@@ -483,20 +513,22 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   if ((st = launchSpmv<T, M_SETUP_R>(ctx, a))) return st;
   a.in = rA;
   if ((st = launchSpmv<T, M_SETUP_S>(ctx, a))) return st;
-  const int dirBlocks = 1024;
   // the FP32 inner solve of the mixed form stops when its residual stagnates
   DonePoller poll(ctx, which, sizeof(T) == 4);
+  T *sA = s, *sB = s2, *apA = Ap, *apB = Ap2;
   int i = 0;
   for (; i < maxIt; i++) {
     a.iter = i;
     a.in = rA;
     a.r = rB;
+    a.s = sA;
+    a.sOut = sB;
+    a.Ap = apA;
+    a.ApOut = apB;
     if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
-    launchTimed(ctx, sizeof(T) == 4 ? 13 : 12, kCrDirection<T>, dim3(dirBlocks), dim3(256),
-                (long)d.cells, p, Ap, (const T*)rB, (const T*)s, (const T*)D,
-                (const CrScalars*)a.sc, ctx->partials);
-    launchFinish<F_DIR, 1>(ctx, dirBlocks, a.sc);
     std::swap(rA, rB);
+    std::swap(sA, sB);
+    std::swap(apA, apB);
     if (i > 0 && i % 5 == 0) {
       a.in = x;
       a.hostOut = poll.slot();
@@ -516,10 +548,10 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
 
 template esf_status crSolve<double>(esf_ctx*, int, const double*, const double*,
                                     const double*, double*, double*, double*, double*,
-                                    double*, double*, double, int, int);
+                                    double*, double*, double*, double*, double, int, int);
 template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*,
                                    const float*, float*, float*, float*, float*, float*,
-                                   float*, double, int, int);
+                                   float*, float*, float*, double, int, int);
 
 __global__ void kZeroF(long n, float* a) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
@@ -562,8 +594,8 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
     hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->e32, ctx->r32,
-                             ctx->r32b, ctx->p32, ctx->s32, ctx->Ap32, innerEps,
-                             maxIt - inner, 4)))
+                             ctx->r32b, ctx->p32, ctx->s32, ctx->s32b, ctx->Ap32, ctx->Ap32b,
+                             innerEps, maxIt - inner, 4)))
       return st;
     outer++;
     o.e32 = ctx->e32;
