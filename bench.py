@@ -91,6 +91,13 @@ def kernel_bytes(cls, n_cells, n_int, n_sc=0, n_sn=0):
         return n_sc * (28 * 8 + 8 * 4 + 24 * 8) + n_sn * 3 * 24
     if cls == "solid_node_cr":
         return n_sn * (8 * 4 + 16 * 24) + n_sc * 24 * 8
+    # the same passes carrying the true-residual check: x read at the
+    # corners and its 24 rows per cell written; the node pass reads those
+    # rows and b and writes the kept x
+    if cls == "solid_cell_crx":
+        return n_sc * (28 * 8 + 8 * 4 + 2 * 24 * 8) + n_sn * 4 * 24
+    if cls == "solid_node_crx":
+        return n_sn * (8 * 4 + 18 * 24) + 2 * n_sc * 24 * 8
     w = 4 if "32" in cls else 8
     c, i = n_cells, n_int
     table = {"apply": w * (2 * c + i), "setup_r": w * (5 * c + i), "setup_s": w * (4 * c + i),

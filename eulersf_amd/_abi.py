@@ -51,7 +51,8 @@ S_VEL_WORKSPACE = 106
 KERNEL_CLASSES = ["spmv64_apply", "spmv64_setup_r", "spmv64_setup_s", "spmv64_update",
                   "spmv64_check", "spmv64_ir_resid", "spmv32_apply", "spmv32_setup_r",
                   "spmv32_setup_s", "spmv32_update", "spmv32_check", "spmv32_ir_resid",
-                  "cr_dir64", "cr_dir32", "solid_cell_cr", "solid_node_cr"]
+                  "unused12", "unused13", "solid_cell_cr", "solid_node_cr",
+                  "solid_cell_crx", "solid_node_crx"]
 
 STAGES = ["cfl", "forces", "solid_solves", "pressure_solve", "solid_finalize", "gather",
           "advect_velocity", "advect_intensity", "recover_bc"]

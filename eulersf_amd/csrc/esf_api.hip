@@ -78,7 +78,7 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   double** vecs[] = {&ctx->sForce, &ctx->sMv,  &ctx->sDamped, &ctx->sKdiag, &ctx->sAdiag,
                      &ctx->sRhs,   &ctx->sV0,  &ctx->sDv,     &ctx->sB,     &ctx->sr,
                      &ctx->sp,     &ctx->ss,   &ctx->sAp,     &ctx->stmp,   &ctx->sSd,
-                     &ctx->sApd};
+                     &ctx->sApd,   &ctx->sXb};
   for (double** v : vecs) {
     esf_status st = dalloc(ctx, v, 3 * cap);
     if (st) return st;
@@ -88,8 +88,9 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
   if ((st = dalloc(ctx, &ctx->nodeMask, cap))) return st;
   ctx->inboxSolids = std::max<int>(1, (int)ctx->solids.size());
   if ((st = dalloc(ctx, &ctx->inbox, 24 * cap * ctx->inboxSolids))) return st;
-  // 4 sets x 4 sums x node-pass blocks (3 cap DOFs / 256)
-  if ((st = dalloc(ctx, &ctx->sPartials, 16 * (3 * cap / 256 + 1)))) return st;
+  if ((st = dalloc(ctx, &ctx->inboxX, 24 * cap * ctx->inboxSolids))) return st;
+  // 3 sets x 5 sums and a set of 4 x node-pass blocks (3 cap DOFs / 256)
+  if ((st = dalloc(ctx, &ctx->sPartials, 20 * (3 * cap / 256 + 1)))) return st;
   ctx->capSolid = (int)cap;
   return ESF_OK;
 }
@@ -120,7 +121,7 @@ static void freeAll(esf_ctx* ctx) {
                   ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
                   ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox,
                   ctx->sSd,    ctx->sApd,   ctx->s2,      ctx->Ap2,       ctx->s32b,
-                  ctx->Ap32b};
+                  ctx->Ap32b,  ctx->sXb,    ctx->inboxX};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& so : ctx->solids) {
@@ -823,7 +824,7 @@ esf_status esf_kernel_timing(esf_ctx* ctx, double* total_ms, long* counts, int r
     }
   }
   ctx->timedUsed = 0;
-  for (int c = 0; c < 16; c++) {
+  for (int c = 0; c < ESF_KERNEL_CLASSES; c++) {
     if (total_ms) total_ms[c] = ctx->kernelMs[c];
     if (counts) counts[c] = ctx->kernelCount[c];
     if (reset) {

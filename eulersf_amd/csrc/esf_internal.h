@@ -46,7 +46,8 @@ struct CrScalars {
   double bnorm;    // |b|
   double resnorm;  // last true residual |Ax - b|
   double eps;      // relative tolerance
-  double aux;      // unused
+  double aux;      // solid CR: 1 when a check riding along with the next
+                   // iteration converged (x is then in the backup)
   int done;        // 1 = converged
   int iter;        // iteration count at convergence
   int pad[2];
@@ -117,6 +118,8 @@ struct esf_ctx {
          *sV0 = nullptr, *sDv = nullptr, *sB = nullptr;
   double *sr = nullptr, *sp = nullptr, *ss = nullptr, *sAp = nullptr,
          *stmp = nullptr, *sSd = nullptr, *sApd = nullptr;  // sSd = D^-1 s, sApd = D^-1 Ap
+  double* sXb = nullptr;        // solid CR: x before the iteration a check rides along with
+  double* inboxX = nullptr;     // rows of K x for that check, laid out as inbox
   // reductions
   double* partials = nullptr;  // per-block partial sums
   double* sPartials = nullptr; // solid CR: 4 sets of 4 x node-pass blocks
@@ -148,8 +151,8 @@ struct esf_ctx {
   };
   std::vector<TimedLaunch> timed;
   size_t timedUsed = 0;
-  double kernelMs[16] = {0};
-  long kernelCount[16] = {0};
+  double kernelMs[ESF_KERNEL_CLASSES] = {0};
+  long kernelCount[ESF_KERNEL_CLASSES] = {0};
 };
 
 namespace esf {
@@ -285,7 +288,8 @@ inline void solveReport(const CrScalars& h, int maxIt, int* iters, bool* converg
 }
 
 // kernel classes for esf_kernel_timing: SpMV modes 0-5 (FP64), 6-11 (FP32),
-// CR direction 12 (FP64) / 13 (FP32), solid cell pass 14, solid node pass 15
+// 12/13 unused, solid cell pass 14, solid node pass 15, the same two
+// carrying the true-residual check 16 / 17
 //
 // launchTimed: when kernel timing is on and cls >= 0, the launch carries a
 // start/stop event pair in its own dispatch (hipExtLaunchKernelGGL), so the

@@ -41,11 +41,14 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
     sc->beta = 0.0;
     sc->alpha = (fabs(t[K - 1]) > 0.0) ? t[0] / t[K - 1] : t[K - 1];
   } else if constexpr (OP == F_CHECK) {
+    // first = 1: the check rode along with the next iteration, whose x
+    // update the solve undoes at its end (aux)
     const double rn = sqrt(t[0]);
     sc->resnorm = rn;
     if (rn < sc->eps * sc->bnorm) {
       sc->done = 1;
       sc->iter = iter;
+      sc->aux = first;
     }
   } else if constexpr (OP == F_IR) {
     sc->resnorm = sqrt(t[0]);

@@ -284,7 +284,10 @@ struct SolidPartials {
 //             divides, and forms the same values with the same expression
 // FMA contraction is on inside the two loop kernels (the products here are
 // matrix-free and need no bitwise match with the reference's CSR order).
-enum CellMode { CELL_IN = 0, CELL_CR = 1 };
+//   CELL_CRX: CELL_CR and, into a second inbox, K_e x: the true-residual
+//             check of the previous iteration's x rides along (the node pass
+//             S_CRX finishes it)
+enum CellMode { CELL_IN = 0, CELL_CR = 1, CELL_CRX = 2 };
 constexpr int kCellLanes = 2;  // lanes per cell of the solid cell pass
 
 template <int LPC>
@@ -302,11 +305,15 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
                                                   const double* __restrict__ in,
                                                   const double* __restrict__ sd,
                                                   const double* __restrict__ apd,
+                                                  const double* __restrict__ xin,
                                                   const CrScalars* sc, SolidPartials sp,
                                                   double lambda, double mu,
-                                                  double* __restrict__ inbox) {
+                                                  double* __restrict__ inbox,
+                                                  double* __restrict__ inboxX) {
 #pragma clang fp contract(on)
   static_assert(LPC == 2 || LPC == 4 || LPC == 8, "lanes per cell");
+  constexpr bool kCR = MODE != CELL_IN;   // the CR residual update at the corners
+  constexpr bool kX = MODE == CELL_CRX;   // also K_e x
   constexpr int CPL = 8 / LPC;  // corners per lane
   __shared__ double grp[5 * kScalMaxGroups];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -316,7 +323,7 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
   // results masked afterwards) so nothing waits for a branch: the CR
   // scalars' partials, the corners' DOF indices and the cell's (R, L, F, 1/J)
   ScalarLoads SL;
-  if (MODE == CELL_CR) scalarLoads(sp.cur, sp.prev, sp.nb, threadIdx.x >> 6, SL);
+  if (kCR) scalarLoads(sp.cur, sp.prev, sp.nb, threadIdx.x >> 6, SL);
   const int ec = live ? e : nValid - 1;
   int kraw[CPL];
 #pragma unroll
@@ -330,8 +337,8 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
     L[q] = m[(long)(9 + q) * cap];
   }
   const double Jinv = m[27L * cap];
-  // second round: the corners' r (or v), D^-1 s, D^-1 Ap
-  double rr[CPL][3], ss[CPL][3], aa[CPL][3];
+  // second round: the corners' r (or v), D^-1 s, D^-1 Ap (and x)
+  double rr[CPL][3], ss[CPL][3], aa[CPL][3], xx[CPL][3];
 #pragma unroll
   for (int j = 0; j < CPL; j++) {
     const int k = kraw[j] >= 0 ? kraw[j] : 0;
@@ -340,17 +347,19 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
       rr[j][q] = in[k + q];
       ss[j][q] = 0;
       aa[j][q] = 0;
-      if (MODE == CELL_CR) {
+      xx[j][q] = 0;
+      if (kCR) {
         ss[j][q] = sd[k + q];
         aa[j][q] = apd[k + q];
       }
+      if (kX) xx[j][q] = xin[k + q];
     }
   }
   // every load of the pass is issued above; keep the compiler from sinking
   // any of them below the reductions (emits no instruction)
   asm volatile("" ::: "memory");
   double v[CPL][3];
-  if (MODE == CELL_CR) {
+  if (kCR) {
     double alpha, beta;
     if (scalarFinish(sp.cur, sp.prev, sp.nb, sp.first, sc, SL, grp, alpha, beta)) return;
 #pragma unroll
@@ -368,43 +377,50 @@ __global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
   // lanes of a live group
   if (!live) return;
   const double g = 0.25 * d.dhInv;
+  const double dh = d.dh;
   double gf[CPL][3];
-  double dF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int j = 0; j < CPL; j++) {
     const int c = l + LPC * j;
     const double G0 = sgn(c & 1) * g, G1 = sgn((c >> 1) & 1) * g, G2 = sgn(c >> 2) * g;
 #pragma unroll
     for (int q = 0; q < 3; q++) gf[j][q] = (G0 * F[q] + G1 * F[3 + q]) + G2 * F[6 + q];
-    if (kraw[j] >= 0)
+  }
+  // K_e w for the corner values w: dF summed over the group, one dP, the
+  // corners' rows into their nodes' inbox (node s = kraw / 3 keeps 8 rows of
+  // 3, row 7 - c for the cell at offset 7 - c from it)
+  auto product = [&](const double (&w)[CPL][3], double* box) {
+    double dF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        dF[q] += v[j][0] * gf[j][q];
-        dF[3 + q] += v[j][1] * gf[j][q];
-        dF[6 + q] += v[j][2] * gf[j][q];
+    for (int j = 0; j < CPL; j++)
+      if (kraw[j] >= 0)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          dF[q] += w[j][0] * gf[j][q];
+          dF[3 + q] += w[j][1] * gf[j][q];
+          dF[6 + q] += w[j][2] * gf[j][q];
+        }
+#pragma unroll
+    for (int q = 0; q < 9; q++) dF[q] = sumGroup<LPC>(dF[q]);
+    double dP[9];
+    corotDP(R, L, lambda, mu, dF, dP);
+#pragma unroll
+    for (int j = 0; j < CPL; j++) {
+      if (kraw[j] < 0) continue;
+      const int c = l + LPC * j;
+      double* out = box + 8L * kraw[j] + 3 * (7 - c);
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        double y = (dP[3 * r] * gf[j][0] + dP[3 * r + 1] * gf[j][1]) + dP[3 * r + 2] * gf[j][2];
+        out[r] = (((y * Jinv) * dh) * dh) * dh;
       }
-  }
-#pragma unroll
-  for (int q = 0; q < 9; q++) dF[q] = sumGroup<LPC>(dF[q]);
-  double dP[9];
-  corotDP(R, L, lambda, mu, dF, dP);
-  const double dh = d.dh;
-#pragma unroll
-  for (int j = 0; j < CPL; j++) {
-    if (kraw[j] < 0) continue;
-    const int c = l + LPC * j;
-    // the corner's rows go to its node's inbox: node s = kraw / 3 keeps 8
-    // rows of 3, row 7 - c for the cell at offset 7 - c from it
-    double* out = inbox + 8L * kraw[j] + 3 * (7 - c);
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-      double y = (dP[3 * r] * gf[j][0] + dP[3 * r + 1] * gf[j][1]) + dP[3 * r + 2] * gf[j][2];
-      out[r] = (((y * Jinv) * dh) * dh) * dh;
     }
-  }
+  };
+  product(v, inbox);
+  if (kX) product(xx, inboxX);
 }
 
-enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3 };
+enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3, S_CRX = 4 };
 
 struct SolidNodeArgs {
   int nSolid;
@@ -425,6 +441,8 @@ struct SolidNodeArgs {
   double* Ap;             // S_CR
   double* sd;             // S_SETUP, S_CR: D^-1 s, read by the cell pass
   double* apd;            // S_CR: D^-1 Ap
+  const double* inboxX;   // S_CRX: rows of K x (laid out as inbox)
+  double* xb;             // S_CRX: x before this iteration's update
   const CrScalars* sc;
   SolidPartials sp;       // S_CR: where alpha and beta come from
   double* partials;       // this pass's block partials
@@ -439,12 +457,16 @@ struct SolidNodeArgs {
 //             D^-1 Ap and D^-1 s kept;  r.s, s.D^-1 s, Ap.D^-1 s,
 //             Ap.D^-1 Ap
 //   S_CHECK : |A x - b|^2  (every 5 iterations, CONJUGATE_RESIDUAL.inl:246-255)
+//   S_CRX   : S_CR, and the check of the x it starts from (rows of K x from
+//             CELL_CRX): |A x - b|^2 as a fifth sum, x kept in xb
 // All loads are in the first round (indices clamped in bounds, masked
 // afterwards), so a pass costs one memory round trip.
 template <int MODE>
 __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
 #pragma clang fp contract(on)
-  __shared__ double red[16];
+  constexpr bool kCR = MODE == S_CR || MODE == S_CRX;
+  constexpr bool kX = MODE == S_CRX;
+  __shared__ double red[20];
   __shared__ double grp[5 * kScalMaxGroups];
   const long nDof = 3L * a.nSolid;
   if (nDof == 0) return;
@@ -454,15 +476,20 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   const long sn = k / 3;
   const int q = (int)(k - 3 * sn);
   ScalarLoads SL;
-  if (MODE == S_CR) scalarLoads(a.sp.cur, a.sp.prev, a.sp.nb, threadIdx.x >> 6, SL);
+  if (kCR) scalarLoads(a.sp.cur, a.sp.prev, a.sp.nb, threadIdx.x >> 6, SL);
   const uint64_t mask = a.nodeMask[sn];
-  double rows[8];
+  double rows[8], rowsX[8];
   const double* in0 = a.inbox + 24 * sn + q;
+  const double* inX = a.inboxX + 24 * sn + q;
 #pragma unroll
-  for (int c = 0; c < 8; c++) rows[c] = in0[3 * c];
+  for (int c = 0; c < 8; c++) {
+    rows[c] = in0[3 * c];
+    rowsX[c] = kX ? inX[3 * c] : 0.0;
+  }
   const double dm = a.damped[k];
   double vin = 0, dg = 1, ap0 = 0, s0 = 0, p0 = 0, r0 = 0, x0 = 0, bb = 0, apd0 = 0, sd0 = 0;
-  if (MODE == S_CR) {
+  if (kX) bb = a.b[k];
+  if (kCR) {
     dg = a.diag[k];
     ap0 = a.Ap[k];
     s0 = a.s[k];
@@ -479,26 +506,43 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
   // every load of the pass is issued above; keep the compiler from sinking
   // any of them below the reductions (emits no instruction)
   asm volatile("" ::: "memory");
-  double kx = 0;
+  double kx = 0, kxx = 0;
 #pragma unroll
   for (int c = 0; c < 8; c++)
-    if (mask >> c & 1u) kx += rows[c];
+    if (mask >> c & 1u) {
+      kx += rows[c];
+      if (kX) kxx += rowsX[c];
+    }
   // further solids (scenes with several bodies): their rows, in solid order
   for (int b = 1; b < a.nSolidBodies; b++) {
     const double* in = in0 + 24L * a.capS * b;
+    const double* ix = inX + 24L * a.capS * b;
 #pragma unroll
-    for (int c = 0; c < 8; c++) rows[c] = in[3 * c];
+    for (int c = 0; c < 8; c++) {
+      rows[c] = in[3 * c];
+      if (kX) rowsX[c] = ix[3 * c];
+    }
 #pragma unroll
     for (int c = 0; c < 8; c++)
-      if (mask >> (8 * b + c) & 1u) kx += rows[c];
+      if (mask >> (8 * b + c) & 1u) {
+        kx += rows[c];
+        if (kX) kxx += rowsX[c];
+      }
   }
   double alpha = 0, beta = 0;
-  if (MODE == S_CR) {
+  if (kCR) {
     if (scalarFinish(a.sp.cur, a.sp.prev, a.sp.nb, a.sp.first, a.sc, SL, grp, alpha, beta))
       return;
   }
-  double part[4] = {0, 0, 0, 0};
-  if (MODE == S_CR) {
+  double part[5] = {0, 0, 0, 0, 0};
+  if (kX && live) {
+    // the check of x as this iteration found it (as S_CHECK forms it)
+    const double y = kxx * a.scale + dm * x0;
+    const double t = y - bb;
+    part[4] = t * t;
+    a.xb[k] = x0;
+  }
+  if (kCR) {
     const double ap = ap0 * beta + s0;
     const double pp = p0 * beta + r0;
     const double rn = r0 - alpha * (apd0 * beta + sd0);
@@ -535,7 +579,21 @@ __global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
     }
   }
   if (MODE == S_APPLY) return;
-  blockPartials<4>(part, a.partials, red);
+  if (kX) {
+    blockPartials<5>(part, a.partials, red);
+  } else {
+    double p4[4] = {part[0], part[1], part[2], part[3]};
+    blockPartials<4>(p4, a.partials, red);
+  }
+}
+
+// the end of a solid solve whose convergence a riding check found: x back
+// to the iterate that check accepted
+__global__ void kSolidFixX(long n, const CrScalars* sc, const double* xb, double* x) {
+  if (!(sc->done && sc->aux != 0.0)) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    x[i] = xb[i];
 }
 
 // diag(A) = scale * diag(K) + M~   (COUPLED_INTEGRATOR_3D.cpp:613-617)
@@ -601,16 +659,18 @@ static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
 // esf_kernel_timing
 template <int MODE>
 static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
-                     const SolidPartials& sp) {
+                     const SolidPartials& sp, const double* x = nullptr) {
+  const int cls = MODE == CELL_CR ? 14 : (MODE == CELL_CRX ? 16 : -1);
   for (size_t b = 0; b < ctx->solids.size(); b++) {
     DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
-      launchTimed(ctx, MODE == CELL_CR ? 14 : -1, kSolidCell<MODE, kCellLanes>,
+      launchTimed(ctx, cls, kSolidCell<MODE, kCellLanes>,
                   dim3(cdiv((long)kCellLanes * so.nValid, 256)), dim3(256), ctx->d, so.nValid,
                   so.capValid,
                   (const int*)so.cornerDof, (const double*)so.cellMat, in,
-                  (const double*)ctx->sSd, (const double*)ctx->sApd,
-                  sc, sp, so.lambda, so.mu, ctx->inbox + 24L * ctx->capSolid * b);
+                  (const double*)ctx->sSd, (const double*)ctx->sApd, x,
+                  sc, sp, so.lambda, so.mu, ctx->inbox + 24L * ctx->capSolid * b,
+                  ctx->inboxX + 24L * ctx->capSolid * b);
     }
   }
 }
@@ -619,8 +679,8 @@ static int nodeBlocks(const esf_ctx* ctx) { return cdiv(std::max(3L * ctx->nSoli
 
 template <int MODE>
 static void nodePass(esf_ctx* ctx, SolidNodeArgs& a) {
-  launchTimed(ctx, MODE == S_CR ? 15 : -1, kSolidNode<MODE>, dim3(nodeBlocks(ctx)), dim3(256),
-              ctx->d, a);
+  launchTimed(ctx, MODE == S_CR ? 15 : (MODE == S_CRX ? 17 : -1), kSolidNode<MODE>,
+              dim3(nodeBlocks(ctx)), dim3(256), ctx->d, a);
 }
 
 esf_status solidAdiag(esf_ctx* ctx) {
@@ -703,11 +763,11 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
                      (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->sApd,
                      ctx->partials);
   launchFinish<F_BNORM, 1>(ctx, eb, sc);
-  // partial sets: node pass i writes set i mod 3 (the setup pass set 2),
-  // the residual check its own set
+  // partial sets of 5 x nb: node pass i writes set i mod 3 (the setup pass
+  // set 2; row 4 is the check that rides along), a lone check its own set
   const int nb = nodeBlocks(ctx);
-  double* set[3] = {ctx->sPartials, ctx->sPartials + 4L * nb, ctx->sPartials + 8L * nb};
-  double* checkSet = ctx->sPartials + 12L * nb;
+  double* set[3] = {ctx->sPartials, ctx->sPartials + 5L * nb, ctx->sPartials + 10L * nb};
+  double* checkSet = ctx->sPartials + 15L * nb;
   SolidNodeArgs a{};
   fillNodeArgs(ctx, a);
   a.b = b;
@@ -723,14 +783,30 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   a.in = ctx->sr;
   a.partials = set[2];
   nodePass<S_SETUP>(ctx, a);
+  a.inboxX = ctx->inboxX;
+  a.xb = ctx->sXb;
   DonePoller poll(ctx, which);
+  // The true residual of the x that iteration i leaves (every 5 iterations,
+  // CONJUGATE_RESIDUAL.inl:246-255) is checked by the passes of iteration
+  // i + 1: its cell pass forms K_e x beside K_e r_new, its node pass the
+  // sum |A x - b|^2 and a copy of x before its own update, and the finisher
+  // decides for iteration i.  If that check converges, kSolidFixX restores
+  // the accepted x at the end, so the solve returns the reference's iterate
+  // and iteration count.  A check after the last iteration runs alone.
   for (int i = 0; i < maxIt; i++) {
     SolidPartials sp{set[(i + 2) % 3], set[(i + 1) % 3], nb, i == 0 ? 1 : 0};
-    cellPass<CELL_CR>(ctx, ctx->sr, sc, sp);
     a.sp = sp;
     a.partials = set[i % 3];
-    nodePass<S_CR>(ctx, a);
-    if (i > 0 && i % 5 == 0) {
+    if (i > 1 && (i - 1) % 5 == 0) {
+      cellPass<CELL_CRX>(ctx, ctx->sr, sc, sp, x);
+      nodePass<S_CRX>(ctx, a);
+      launchFinish<F_CHECK, 1>(ctx, nb, sc, i - 1, 1, set[i % 3] + 4L * nb, poll.slot());
+      if (poll.mark()) break;
+    } else {
+      cellPass<CELL_CR>(ctx, ctx->sr, sc, sp);
+      nodePass<S_CR>(ctx, a);
+    }
+    if (i > 0 && i % 5 == 0 && i + 1 == maxIt) {
       cellPass<CELL_IN>(ctx, x, sc, SolidPartials{});
       a.in = x;
       a.partials = checkSet;
@@ -739,6 +815,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
       if (poll.mark()) break;
     }
   }
+  hipLaunchKernelGGL(kSolidFixX, dim3(std::max(1, std::min(cdiv(n, 256), 1024))), dim3(256), 0,
+                     ctx->stream, n, (const CrScalars*)sc, (const double*)ctx->sXb, x);
   // the report is written without waiting; the caller reads report slot
   // `slot` after its next synchronisation
   copyScalarsAsync(ctx, sc, &ctx->hostSc[slot]);

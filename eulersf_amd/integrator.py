@@ -267,8 +267,9 @@ class CoupledIntegrator3D:
 
     def kernel_timing(self, reset=False):
         """{class: (total_ms, launches)} for the Krylov kernels timed so far."""
-        ms = np.zeros(16)
-        n = (ctypes.c_long * 16)()
+        nc = len(_abi.KERNEL_CLASSES)
+        ms = np.zeros(nc)
+        n = (ctypes.c_long * nc)()
         check(self.ctx, self.lib.esf_kernel_timing(self.ctx, dptr(ms), n, 1 if reset else 0))
         return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(_abi.KERNEL_CLASSES)}
 

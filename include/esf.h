@@ -255,9 +255,12 @@ esf_status esf_bench_copy(esf_ctx* ctx, long bytes, int reps, double* avg_ms);
 esf_status esf_set_kernel_timing(esf_ctx* ctx, int on);
 /* Accumulated device time (ms) and launch count per kernel class since the
  * last reset: 0-5 pressure SpMV FP64 (apply, setup-r, setup-s, CR update,
- * residual check, refinement residual), 6-11 the same in FP32, 12/13 the CR
- * direction update FP64/FP32, 14 the solid CR cell pass (K_e r per cell), 15
- * the solid CR node pass (gather + update).  Arrays of 16. */
+ * residual check, refinement residual), 6-11 the same in FP32, 12/13 unused,
+ * 14 the solid CR cell pass (K_e r per cell), 15 the solid CR node pass
+ * (gather + update), 16/17 the same two passes when they also carry the
+ * true-residual check of the previous iteration.  Arrays of
+ * ESF_KERNEL_CLASSES. */
+#define ESF_KERNEL_CLASSES 18
 esf_status esf_kernel_timing(esf_ctx* ctx, double* total_ms, long* counts,
                              int reset);
 

@@ -188,3 +188,28 @@ def test_isolated_pressure_system_matches_oracle(esf):
             assert ref >= 1
         g.close()
     o.close()
+
+
+@pytest.mark.parametrize("eps,max_it", [(1e-4, 20000), (1e-13, 6)])
+def test_solid_check_returns_checked_iterate(esf, eps, max_it):
+    """the true-residual check of iteration i runs inside iteration i + 1's
+    passes; the solve must still return iteration i's x and count.  The
+    first solid solve of the coupled solve is compared with the oracle's
+    (same right-hand side): at 1e-4 one extra iteration would move x by
+    about 1e-4 relative; with 6 iterations at most, the check after the
+    last iteration runs alone and the solve stops unconverged, as the
+    oracle's does"""
+    g, o = make_pair(esf, 16, velocity=(0.3, -1.0, 0.2), eps=eps, max_iterations=max_it)
+    g.CFL(1.8)
+    o.L.orc_cfl(o.h, 1.8)
+    g.forces()
+    o.L.orc_forces(o.h)
+    o.L.orc_assemble(o.h)
+    gs = g.pcrImplicitSolve()
+    st = O.Stats()
+    o.L.orc_solve(o.h, st)
+    assert gs["iterations"][0] == st.iterations[0], (gs, list(st.iterations))
+    assert gs["converged"][0] == st.converged[0] == (1 if max_it > 6 else 0)
+    assert rel_err(g.download(esf._abi.F_SOLID_V0), o.field(O.SOLID_V0)) < 1e-12
+    g.close()
+    o.close()
