@@ -298,8 +298,10 @@ __device__ inline double sumGroup(double v) {
   return v;
 }
 
+constexpr int kCellThreads = 256;  // cell-pass block (512 measured slower)
+
 template <int MODE, int LPC>
-__global__ void __launch_bounds__(256) kSolidCell(Dims d, int nValid, int cap,
+__global__ void __launch_bounds__(kCellThreads) kSolidCell(Dims d, int nValid, int cap,
                                                   const int* __restrict__ cornerDof,
                                                   const double* __restrict__ cellMat,
                                                   const double* __restrict__ in,
@@ -461,12 +463,14 @@ struct SolidNodeArgs {
 //             CELL_CRX): |A x - b|^2 as a fifth sum, x kept in xb
 // All loads are in the first round (indices clamped in bounds, masked
 // afterwards), so a pass costs one memory round trip.
+constexpr int kNodeThreads = 512;  // node-pass block: half the partials of 256 (1024 measured slower)
+
 template <int MODE>
-__global__ void __launch_bounds__(256) kSolidNode(Dims d, SolidNodeArgs a) {
+__global__ void __launch_bounds__(kNodeThreads) kSolidNode(Dims d, SolidNodeArgs a) {
 #pragma clang fp contract(on)
   constexpr bool kCR = MODE == S_CR || MODE == S_CRX;
   constexpr bool kX = MODE == S_CRX;
-  __shared__ double red[20];
+  __shared__ double red[5 * (kNodeThreads / 64)];
   __shared__ double grp[5 * kScalMaxGroups];
   const long nDof = 3L * a.nSolid;
   if (nDof == 0) return;
@@ -665,7 +669,8 @@ static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
     DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
       launchTimed(ctx, cls, kSolidCell<MODE, kCellLanes>,
-                  dim3(cdiv((long)kCellLanes * so.nValid, 256)), dim3(256), ctx->d, so.nValid,
+                  dim3(cdiv((long)kCellLanes * so.nValid, kCellThreads)), dim3(kCellThreads),
+                  ctx->d, so.nValid,
                   so.capValid,
                   (const int*)so.cornerDof, (const double*)so.cellMat, in,
                   (const double*)ctx->sSd, (const double*)ctx->sApd, x,
@@ -675,12 +680,14 @@ static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
   }
 }
 
-static int nodeBlocks(const esf_ctx* ctx) { return cdiv(std::max(3L * ctx->nSolid, 1L), 256); }
+static int nodeBlocks(const esf_ctx* ctx) {
+  return cdiv(std::max(3L * ctx->nSolid, 1L), kNodeThreads);
+}
 
 template <int MODE>
 static void nodePass(esf_ctx* ctx, SolidNodeArgs& a) {
   launchTimed(ctx, MODE == S_CR ? 15 : (MODE == S_CRX ? 17 : -1), kSolidNode<MODE>,
-              dim3(nodeBlocks(ctx)), dim3(256), ctx->d, a);
+              dim3(nodeBlocks(ctx)), dim3(kNodeThreads), ctx->d, a);
 }
 
 esf_status solidAdiag(esf_ctx* ctx) {
@@ -753,7 +760,7 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   }
   if (nodeBlocks(ctx) > 64 * kScalMaxGroups) {
     ctx->err = "too many solid nodes for one context (limit " +
-               std::to_string(256L * 64 * kScalMaxGroups / 3) + ")";
+               std::to_string((long)kNodeThreads * 64 * kScalMaxGroups / 3) + ")";
     return ESF_ERR_INVALID;
   }
   CrScalars* sc = &ctx->sc[which];
