@@ -13,7 +13,8 @@ esf_status pressureRhs(esf_ctx* ctx);
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D, T* x, T* r,
-                   T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt, int slot);
+                   T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt, int slot,
+                   bool recursive = false);
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
                       double* relres, int* refinements, int* innerIters);
 int spmvBlocks(const Dims& d);

@@ -58,8 +58,19 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
     // r.s, s.D^-1 s, Ap.D^-1 s, Ap.D^-1 Ap of the iteration just done; alpha
     // for the next direction Ap' = s + beta Ap (CONJUGATE_RESIDUAL.inl:
     // 141-160) with (Ap'.D^-1 Ap') expanded; alpha stays the denominator
-    // when that is zero, as there
-    static_assert(K == 4, "CR finisher takes four sums");
+    // when that is zero, as there.  first = 1: done when the fifth sum, the
+    // recursively carried |b - A x|^2, is below (eps |b|)^2
+    static_assert(K >= 4, "CR finisher takes four sums");
+    if constexpr (K >= 5) {
+      if (first) {
+        const double rn = sqrt(t[4]);
+        sc->resnorm = rn;
+        if (rn < sc->eps * sc->bnorm) {
+          sc->done = 1;
+          sc->iter = iter + 1;
+        }
+      }
+    }
     const double dNew = t[0];
     const double beta = dNew / sc->delta;
     const double apq = (t[1] + (2.0 * beta) * t[2]) + (beta * beta) * t[3];
@@ -106,26 +117,32 @@ inline void copyScalarsAsync(esf_ctx* ctx, const CrScalars* sc, CrScalars* out) 
 
 // Poll the scalar block `which` of a Krylov solve without stalling the
 // queue: the finisher of each convergence check writes the block into the
-// pinned ring slot slot() and an event follows it; the host keeps one check
-// interval queued ahead and waits on the previous one, so the device never
-// idles and at most one interval of early-exiting launches follows
-// convergence.
+// pinned ring slot slot() and mark(it) records an event behind it (it: the
+// iteration just queued).  After queueing each iteration the loop calls
+// drain(it): pending checks are queried without blocking, and a check is
+// waited for once `lag` further iterations are queued behind it, so the
+// device never idles on the host and at most `lag` iterations of
+// early-exiting launches follow convergence.
 // With `stagnation` set (the FP32 inner solve of the mixed form), the poll
-// also ends the solve when four consecutive checks set no new minimum of
-// the checked residual: FP32 rounding puts a noise floor under the inner
-// residual (a slowly converging solve still sets new minima), and the FP64
-// refinement around it takes over from there.
+// also ends the solve when `flatMax` consecutive polls set no new minimum
+// of the polled residual (4 checks of 5 iterations, or 12 iterations when
+// every iteration is polled): FP32 rounding puts a noise floor under the
+// inner residual (a slowly converging solve still sets new minima), and the
+// FP64 refinement around it takes over from there.
 struct DonePoller {
   esf_ctx* ctx;
   int which;
   bool stagnation;
   static constexpr int kRing = 8;
-  static constexpr int kLook = 1;
+  int lag;      // iterations queued behind a check before the host waits for it
+  int flatMax;  // polls without a new minimum that end a stagnating solve
   int issued = 0, waited = 0, flat = 0;
+  int markIt[kRing];
   double best = 0;
   bool stagnated = false;
   hipEvent_t ev[kRing];
-  explicit DonePoller(esf_ctx* c, int w, bool stag = false) : ctx(c), which(w), stagnation(stag) {
+  explicit DonePoller(esf_ctx* c, int w, bool stag = false, int lagIters = 2, int flatPolls = 4)
+      : ctx(c), which(w), stagnation(stag), lag(lagIters), flatMax(flatPolls) {
     for (int i = 0; i < kRing; i++) ev[i] = ctx->ev[8 + i];
   }
   bool seen(int ws) {
@@ -135,7 +152,7 @@ struct DonePoller {
     if (best == 0 || h.resnorm < best) {
       best = h.resnorm;
       flat = 0;
-    } else if (++flat >= 4) {
+    } else if (++flat >= flatMax) {
       stagnated = true;
       return true;
     }
@@ -143,22 +160,23 @@ struct DonePoller {
   }
   // the pinned slot the next check's finisher writes
   CrScalars* slot() { return &ctx->hostRing[issued % kRing]; }
-  // after that finisher is queued: returns true when the solve is known to
-  // be done (or stagnated)
-  bool mark() {
+  // after that finisher is queued with iteration `it`
+  void mark(int it) {
     const int slot = issued % kRing;
+    markIt[slot] = it;
     hipEventRecord(ev[slot], ctx->stream);
     issued++;
-    while (issued - waited > kLook) {
-      const int ws = waited % kRing;
-      hipEventSynchronize(ev[ws]);
-      waited++;
-      if (seen(ws)) return true;
-    }
-    // opportunistic non-blocking checks
+  }
+  // after iteration `it` is queued: true when the solve is known to be done
+  // (or stagnated)
+  bool drain(int it) {
     while (waited < issued) {
       const int ws = waited % kRing;
-      if (hipEventQuery(ev[ws]) != hipSuccess) break;
+      if (markIt[ws] + lag <= it || issued - waited >= kRing) {
+        hipEventSynchronize(ev[ws]);
+      } else if (hipEventQuery(ev[ws]) != hipSuccess) {
+        break;
+      }
       waited++;
       if (seen(ws)) return true;
     }

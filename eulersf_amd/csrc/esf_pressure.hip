@@ -58,7 +58,8 @@ struct SpmvArgs {
   double* partials;
   int iter;
   int first;         // M_IR_RESID: also compute |b|
-  CrScalars* hostOut;  // M_CHECK: pinned slot for the checked scalar block
+  int recursive;     // M_UPDATE: the finisher tests the recursive residual
+  CrScalars* hostOut;  // M_CHECK (M_UPDATE when recursive): pinned slot for the scalar block
 };
 
 constexpr int TX = 32, TY = 8, TZ = 8;
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
     beta = (T)a.sc->beta;
   }
 
-  double part0 = 0, part1 = 0, part2 = 0, part3 = 0;
+  double part0 = 0, part1 = 0, part2 = 0, part3 = 0, part4 = 0;
 
   Fetched<T> f[HPT];
   auto fetch = [&](int cz) {
@@ -284,6 +285,9 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
         part1 += (double)acc * (double)sq;
         part2 += (double)apn * (double)sq;
         part3 += (double)apn * (double)(apn / dc);
+        // |b - S x| as the recursion carries it: r is D^-1 (b - S x)
+        const double rb = (double)dc * (double)ownIn;
+        part4 += rb * rb;
       } else if (MODE == M_CHECK) {
         const T t = acc - a.b[c];
         part0 += (double)t * (double)t;
@@ -298,8 +302,8 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
   }
   if (MODE == M_APPLY) return;
   if (MODE == M_UPDATE) {
-    double mine[4] = {part0, part1, part2, part3};
-    blockPartials<4>(mine, a.partials, red);
+    double mine[5] = {part0, part1, part2, part3, part4};
+    blockPartials<5>(mine, a.partials, red);
   } else {
     double mine[2] = {part0, part1};
     blockPartials<2>(mine, a.partials, red);
@@ -364,7 +368,8 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   // the grid-wide sums of the mode (kSpmv's epilogue partials)
   if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
   if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
-  if constexpr (MODE == M_UPDATE) launchFinish<F_CR, 4>(ctx, nb, a.sc);
+  if constexpr (MODE == M_UPDATE)
+    launchFinish<F_CR, 5>(ctx, nb, a.sc, a.iter, a.recursive, nullptr, a.hostOut);
   if constexpr (MODE == M_CHECK)
     launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter, 0, nullptr, a.hostOut);
   if constexpr (MODE == M_IR_RESID) launchFinish<F_IR, 2>(ctx, nb, a.sc, 0, a.first);
@@ -488,10 +493,15 @@ static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
 // CONJUGATE_RESIDUAL::solvePCR (matrix-free) on S, FP64 or FP32 vectors.
 // x is the start vector.  Scalar block `which`.  *iters receives the
 // reference's iteration count; *relres the last checked |Sx-b|/|b|.
+// recursive = false: the reference's test, the true residual every 5
+// iterations (CONJUGATE_RESIDUAL.inl:166-176).  recursive = true (the FP32
+// inner solve of the mixed form, whose result the FP64 outer residual
+// checks): the finisher of every iteration tests the residual the
+// recursion carries, and no check SpMV runs.
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
                    T* x, T* r, T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt,
-                   int slot) {
+                   int slot, bool recursive) {
   esf_status st = initScalars(ctx, which, eps);
   if (st) return st;
   const Dims& d = ctx->d;
@@ -513,9 +523,11 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   if ((st = launchSpmv<T, M_SETUP_R>(ctx, a))) return st;
   a.in = rA;
   if ((st = launchSpmv<T, M_SETUP_S>(ctx, a))) return st;
-  // the FP32 inner solve of the mixed form stops when its residual stagnates
-  DonePoller poll(ctx, which, sizeof(T) == 4);
+  // the FP32 inner solve of the mixed form stops when its residual
+  // stagnates; a check is waited for once two iterations are queued behind it
+  DonePoller poll(ctx, which, sizeof(T) == 4, 2, recursive ? 12 : 4);
   T *sA = s, *sB = s2, *apA = Ap, *apB = Ap2;
+  a.recursive = recursive ? 1 : 0;
   int i = 0;
   for (; i < maxIt; i++) {
     a.iter = i;
@@ -525,18 +537,22 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
     a.sOut = sB;
     a.Ap = apA;
     a.ApOut = apB;
+    a.hostOut = recursive ? poll.slot() : nullptr;
     if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
     std::swap(rA, rB);
     std::swap(sA, sB);
     std::swap(apA, apB);
-    if (i > 0 && i % 5 == 0) {
+    if (recursive) {
+      poll.mark(i);
+    } else if (i > 0 && i % 5 == 0) {
       a.in = x;
       a.hostOut = poll.slot();
       if ((st = launchSpmv<T, M_CHECK>(ctx, a))) return st;
-      if (poll.mark()) {
-        i++;
-        break;
-      }
+      poll.mark(i);
+    }
+    if (poll.drain(i)) {
+      i++;
+      break;
     }
   }
   ctx->crQueued = i;
@@ -548,10 +564,10 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
 
 template esf_status crSolve<double>(esf_ctx*, int, const double*, const double*,
                                     const double*, double*, double*, double*, double*,
-                                    double*, double*, double*, double*, double, int, int);
+                                    double*, double*, double*, double*, double, int, int, bool);
 template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*,
                                    const float*, float*, float*, float*, float*, float*,
-                                   float*, float*, float*, double, int, int);
+                                   float*, float*, float*, double, int, int, bool);
 
 __global__ void kZeroF(long n, float* a) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
@@ -595,7 +611,7 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->e32, ctx->r32,
                              ctx->r32b, ctx->p32, ctx->s32, ctx->s32b, ctx->Ap32, ctx->Ap32b,
-                             innerEps, maxIt - inner, 4)))
+                             innerEps, maxIt - inner, 4, true)))
       return st;
     outer++;
     o.e32 = ctx->e32;

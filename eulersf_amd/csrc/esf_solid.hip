@@ -792,7 +792,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   nodePass<S_SETUP>(ctx, a);
   a.inboxX = ctx->inboxX;
   a.xb = ctx->sXb;
-  DonePoller poll(ctx, which);
+  // a check is waited for once three iterations are queued behind it
+  DonePoller poll(ctx, which, false, 3);
   // The true residual of the x that iteration i leaves (every 5 iterations,
   // CONJUGATE_RESIDUAL.inl:246-255) is checked by the passes of iteration
   // i + 1: its cell pass forms K_e x beside K_e r_new, its node pass the
@@ -808,7 +809,7 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
       cellPass<CELL_CRX>(ctx, ctx->sr, sc, sp, x);
       nodePass<S_CRX>(ctx, a);
       launchFinish<F_CHECK, 1>(ctx, nb, sc, i - 1, 1, set[i % 3] + 4L * nb, poll.slot());
-      if (poll.mark()) break;
+      poll.mark(i);
     } else {
       cellPass<CELL_CR>(ctx, ctx->sr, sc, sp);
       nodePass<S_CR>(ctx, a);
@@ -819,8 +820,9 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
       a.partials = checkSet;
       nodePass<S_CHECK>(ctx, a);
       launchFinish<F_CHECK, 1>(ctx, nb, sc, i, 0, checkSet, poll.slot());
-      if (poll.mark()) break;
+      poll.mark(i);
     }
+    if (poll.drain(i)) break;
   }
   hipLaunchKernelGGL(kSolidFixX, dim3(std::max(1, std::min(cdiv(n, 256), 1024))), dim3(256), 0,
                      ctx->stream, n, (const CrScalars*)sc, (const double*)ctx->sXb, x);
