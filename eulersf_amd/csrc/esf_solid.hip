@@ -792,8 +792,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   nodePass<S_SETUP>(ctx, a);
   a.inboxX = ctx->inboxX;
   a.xb = ctx->sXb;
-  // a check is waited for once three iterations are queued behind it
-  DonePoller poll(ctx, which, false, 3);
+  // a check is waited for once two iterations are queued behind it
+  DonePoller poll(ctx, which, false, 2);
   // The true residual of the x that iteration i leaves (every 5 iterations,
   // CONJUGATE_RESIDUAL.inl:246-255) is checked by the passes of iteration
   // i + 1: its cell pass forms K_e x beside K_e r_new, its node pass the
