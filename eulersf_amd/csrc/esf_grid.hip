@@ -767,6 +767,28 @@ __global__ void kSolidVelocity(int nSolid, const int* solidNodes, const double* 
 // ---------------------------------------------------------------------------
 // advection
 // ---------------------------------------------------------------------------
+// Tiling of the solid-displacement advection: a block owns 32 x 8 nodes of
+// each layer and marches kSLZ layers in z; the interpolated field on the
+// tile plus a one-node halo sits in LDS for three consecutive layers (loaded
+// row by row, each row of the halo box one contiguous run), so the 8-corner
+// interpolations read LDS.  A point whose corners leave that window reads
+// device memory instead, with the same arithmetic.  (The fluid advection
+// kernels stay one thread per node: tiled the same way they measured
+// slower, 137 vs 86 us and 66 vs 32 us at 128^3, their departure points
+// leaving the window more often.)
+constexpr int kSLX = 32, kSLY = 8, kSLZ = 4;
+constexpr int kSLHX = kSLX + 2, kSLHY = kSLY + 2;
+constexpr int kSLLayer = kSLHX * kSLHY;  // halo nodes per layer
+static_assert(kSLX * kSLY == 256, "one node per thread and layer");
+
+__device__ inline void tileOf(int tilesX, int tilesY, int& x0, int& y0, int& z0) {
+  const int tx = blockIdx.x % tilesX, ty = (blockIdx.x / tilesX) % tilesY,
+            tz = blockIdx.x / (tilesX * tilesY);
+  x0 = tx * kSLX;
+  y0 = ty * kSLY;
+  z0 = tz * kSLZ;
+}
+
 // advectFluidVelocityField  COUPLED_INTEGRATOR_3D.cpp:797-826
 __global__ void __launch_bounds__(256) kAdvectVelocity(Dims d, const double* __restrict__ vel,
                                                        const double* __restrict__ solidMass,
@@ -820,29 +842,72 @@ __global__ void __launch_bounds__(256) kAdvectScalar(Dims d, const double* __res
 }
 
 // DEFOR_SOLID_3D::finalizeSolution()  DEFOR_SOLID_3D.cpp:1198-1233.
-// u += v dt over all nodes and the advection of u (:1108-1123) fuse: each
-// thread forms u' = u + v dt at the 8 corners it interpolates, rounding as
-// the separate pass would.
+// u += v dt over all nodes and the advection of u (:1108-1123) fuse: w =
+// u + v dt is formed where the interpolation reads it, rounding as the
+// separate pass would.
+// Tiled as described above; the window holds w.
 __global__ void __launch_bounds__(256) kSolidAdvectU(Dims d, const double* __restrict__ u,
                                                      const double* __restrict__ sv, double dt,
-                                                     double* unew) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= d.nodes) return;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+                                                     double* __restrict__ unew, int tilesX,
+                                                     int tilesY) {
+  __shared__ double w[3][3 * kSLLayer];
+  int x0, y0, z0;
+  tileOf(tilesX, tilesY, x0, y0, z0);
+  const int zEnd = min(z0 + kSLZ, d.Z);
+  // layer z of w into ring slot (z + 3) mod 3 (zeros outside the grid:
+  // never read, the interpolation corners stay inside)
+  auto loadLayer = [&](int z) {
+    double* L = w[(z + 3) % 3];
+    for (int e = threadIdx.x; e < 3 * kSLLayer; e += 256) {
+      const int hn = e / 3, c = e - 3 * hn;
+      const int gx = x0 - 1 + hn % kSLHX, gy = y0 - 1 + hn / kSLHX;
+      double val = 0;
+      if (z >= 0 && z < d.Z && gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y) {
+        const long i = 3 * (gx + (long)gy * d.X + (long)z * d.slab) + c;
+        val = u[i] + sv[i] * dt;
+      }
+      L[e] = val;
+    }
+  };
+  loadLayer(z0 - 1);
+  loadLayer(z0);
+  const int lx = threadIdx.x % kSLX, ly = threadIdx.x / kSLX;
+  const int x = x0 + lx, y = y0 + ly;
   const FieldGeom g = gridGeom(d);
-  const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
-  double np[3];
-  for (int c = 0; c < 3; c++) np[c] = node[c] - sv[3 * n + c] * dt;
-  double fx, fy, fz;
-  const long b = locate(g, np, fx, fy, fz);
   const long X = d.X, S = d.slab;
-  const long cn[8] = {b, b + 1, b + X, b + X + 1, b + S, b + S + 1, b + S + X, b + S + X + 1};
-  for (int c = 0; c < 3; c++) {
-    double q[8];
-    for (int k = 0; k < 8; k++) q[k] = u[3 * cn[k] + c] + sv[3 * cn[k] + c] * dt;
-    const double z0 = lerp(lerp(q[0], q[1], fx), lerp(q[2], q[3], fx), fy);
-    const double z1 = lerp(lerp(q[4], q[5], fx), lerp(q[6], q[7], fx), fy);
-    unew[3 * n + c] = lerp(z0, z1, fz);
+  for (int z = z0; z < zEnd; z++) {
+    loadLayer(z + 1);
+    __syncthreads();
+    if (x < d.X && y < d.Y) {
+      const long n = x + (long)y * X + (long)z * S;
+      const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
+      double np[3];
+      for (int c = 0; c < 3; c++) np[c] = node[c] - sv[3 * n + c] * dt;
+      double fx, fy, fz;
+      const long b = locate(g, np, fx, fy, fz);
+      const int bi = (int)(b % X), bj = (int)((b / X) % d.Y), bk = (int)(b / S);
+      const int hx = bi - (x0 - 1), hy = bj - (y0 - 1);
+      const bool inWin = hx >= 0 && hx + 1 < kSLHX && hy >= 0 && hy + 1 < kSLHY &&
+                         bk >= z - 1 && bk <= z;
+      const long cn[8] = {b, b + 1, b + X, b + X + 1, b + S, b + S + 1, b + S + X, b + S + X + 1};
+      for (int c = 0; c < 3; c++) {
+        double q[8];
+        if (inWin) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+            q[k] = w[(bk + oz + 3) % 3][3 * ((hx + ox) + kSLHX * (hy + oy)) + c];
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; k++) q[k] = u[3 * cn[k] + c] + sv[3 * cn[k] + c] * dt;
+        }
+        const double z0v = lerp(lerp(q[0], q[1], fx), lerp(q[2], q[3], fx), fy);
+        const double z1v = lerp(lerp(q[4], q[5], fx), lerp(q[6], q[7], fx), fy);
+        unew[3 * n + c] = lerp(z0v, z1v, fz);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1046,7 +1111,9 @@ esf_status solidCopyVelocity(esf_ctx* ctx) {
 
 esf_status solidFinalize(esf_ctx* ctx, double dt) {
   for (auto& so : ctx->solids) {
-    LAUNCH_NODES(kSolidAdvectU, ctx->d, so.u, so.vel, dt, so.unew);
+    const int tX = cdiv(ctx->d.X, kSLX), tY = cdiv(ctx->d.Y, kSLY), tZ = cdiv(ctx->d.Z, kSLZ);
+    hipLaunchKernelGGL(kSolidAdvectU, dim3(tX * tY * tZ), dim3(256), 0, ctx->stream, ctx->d,
+                       (const double*)so.u, (const double*)so.vel, dt, so.unew, tX, tY);
     std::swap(so.u, so.unew);
     const Box& b = so.box;
     const int axes[3] = {2, 1, 0};
