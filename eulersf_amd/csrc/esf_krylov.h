@@ -125,7 +125,7 @@ inline void copyScalarsAsync(esf_ctx* ctx, const CrScalars* sc, CrScalars* out) 
 // early-exiting launches follow convergence.
 // With `stagnation` set (the FP32 inner solve of the mixed form), the poll
 // also ends the solve when `flatMax` consecutive polls set no new minimum
-// of the polled residual (4 checks of 5 iterations, or 12 iterations when
+// of the polled residual (4 checks of 5 iterations, or 4 iterations when
 // every iteration is polled): FP32 rounding puts a noise floor under the
 // inner residual (a slowly converging solve still sets new minima), and the
 // FP64 refinement around it takes over from there.

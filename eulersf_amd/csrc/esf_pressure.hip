@@ -525,7 +525,7 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   if ((st = launchSpmv<T, M_SETUP_S>(ctx, a))) return st;
   // the FP32 inner solve of the mixed form stops when its residual
   // stagnates; a check is waited for once two iterations are queued behind it
-  DonePoller poll(ctx, which, sizeof(T) == 4, 2, recursive ? 12 : 4);
+  DonePoller poll(ctx, which, sizeof(T) == 4, 2, 4);
   T *sA = s, *sB = s2, *apA = Ap, *apB = Ap2;
   a.recursive = recursive ? 1 : 0;
   int i = 0;
