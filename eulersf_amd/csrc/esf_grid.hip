@@ -585,7 +585,7 @@ esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut) {
 // ---------------------------------------------------------------------------
 __global__ void kVorticity(Dims d, const double* __restrict__ v, const double* __restrict__ nf,
                            double* vort, double* ws, int enabled) {
-  const long index = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long index = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (index >= d.nodes) return;
   const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
   double out = 0.0;
@@ -635,7 +635,7 @@ struct BodyArgs {
 
 __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
   __shared__ double red[16];
-  const long index = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long index = xcdBlock() * (long)blockDim.x + threadIdx.x;
   double en = 0;
   if (index < d.nodes) {
     const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
@@ -705,7 +705,7 @@ __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
     }
   }
   const double t = blockSum(en, red);
-  if (threadIdx.x == 0) a.energyPart[blockIdx.x] = t;
+  if (threadIdx.x == 0) a.energyPart[xcdBlock()] = t;
 }
 
 // solid right-hand side dt F + M v  (COUPLED_INTEGRATOR_3D.cpp:664-665)
@@ -729,7 +729,7 @@ __global__ void kScatterSolid(int nSolid, const int* solidNodes, const double* s
 __global__ void kCorrect(Dims d, const double* __restrict__ lam, const double* __restrict__ v0,
                          const double* __restrict__ minv, const int* __restrict__ sidx,
                          double* vel, double* sB) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int i = n % d.X, j = (n / d.X) % d.Y, k = n / d.slab;
   if (!isInterior(d, i, j, k)) return;
@@ -782,8 +782,8 @@ constexpr int kSLLayer = kSLHX * kSLHY;  // halo nodes per layer
 static_assert(kSLX * kSLY == 256, "one node per thread and layer");
 
 __device__ inline void tileOf(int tilesX, int tilesY, int& x0, int& y0, int& z0) {
-  const int tx = blockIdx.x % tilesX, ty = (blockIdx.x / tilesX) % tilesY,
-            tz = blockIdx.x / (tilesX * tilesY);
+  const int b = xcdBlock();
+  const int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
   x0 = tx * kSLX;
   y0 = ty * kSLY;
   z0 = tz * kSLZ;
@@ -793,7 +793,7 @@ __device__ inline void tileOf(int tilesX, int tilesY, int& x0, int& y0, int& z0)
 __global__ void __launch_bounds__(256) kAdvectVelocity(Dims d, const double* __restrict__ vel,
                                                        const double* __restrict__ solidMass,
                                                        double dt, double* velws) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   if (!isInterior(d, x, y, z) || solidMass[n] > 0) return;
@@ -819,7 +819,7 @@ __global__ void __launch_bounds__(256) kAdvectScalar(Dims d, const double* __res
                                                      const double* __restrict__ solidMass,
                                                      const double* __restrict__ field, double dt,
                                                      double* out) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   if (solidMass[n] > 0) {
     out[n] = 0.0;
@@ -1004,7 +1004,7 @@ struct RecoverArgs {
   const double* vel[kMaxSolids];
 };
 __global__ void kRecoverBC(Dims d, RecoverArgs a, double* vel, double* intensity, int bc) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   for (int b = 0; b < a.n; b++) {

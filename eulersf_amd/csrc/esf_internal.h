@@ -235,10 +235,23 @@ __device__ inline double blockSum(double v, double* lds) {
 // All K values in one pass: wave sums, one LDS exchange, thread k adds
 // quantity k over the waves in ascending order.  lds: K * (waves per
 // block) doubles.  All threads of the block must call it.
+// Work-group renumbering for the 8 XCDs: the dispatcher deals the
+// work-groups of a launch to the XCDs in turn (group b to XCD b mod 8), and
+// each XCD has its own L2.  xcdBlock() gives each XCD one contiguous run of
+// logical indices instead, so blocks that read overlapping data (stencil
+// neighbours, shared corner nodes) find it in the same L2.  A bijection on
+// [0, gridDim.x) for every grid size; 1-D grids only.
+__device__ __forceinline__ int xcdBlock() {
+  const int b = blockIdx.x, nb = gridDim.x, x = b & 7;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
+// the same with the block's slot given (a kernel numbering its blocks by
+// xcdBlock() keeps its partials in that order)
 template <int K>
-__device__ inline void blockPartials(const double (&mine)[K], double* partials, double* lds) {
-  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+__device__ inline void blockPartialsAt(const double (&mine)[K], double* partials, double* lds,
+                                       unsigned nb, unsigned bid) {
   double w[K];
 #pragma unroll
   for (int k = 0; k < K; k++) w[k] = waveSum(mine[k]);
@@ -253,6 +266,11 @@ __device__ inline void blockPartials(const double (&mine)[K], double* partials, 
     for (int v = 0; v < nw; v++) r += lds[K * v + threadIdx.x];
     partials[(size_t)threadIdx.x * nb + bid] = r;
   }
+}
+template <int K>
+__device__ inline void blockPartials(const double (&mine)[K], double* partials, double* lds) {
+  blockPartialsAt<K>(mine, partials, lds, gridDim.x * gridDim.y * gridDim.z,
+                     blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 // in the finisher (one block, up to 1024 threads): the K totals of the

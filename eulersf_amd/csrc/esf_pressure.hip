@@ -317,7 +317,7 @@ __global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
                              const double* __restrict__ vel,
                              const double* __restrict__ minv, double* c,
                              double* b, double* D, float* D32) {
-  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long r = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (r >= d.cells) return;
   const int cx = r % d.CX, cy = (r / d.CX) % d.CY, cz = r / d.cslab;
   const long base = cx + (long)cy * d.X + (long)cz * d.slab;

@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(kCellThreads) kSolidCell(Dims d, int nValid, i
   constexpr bool kX = MODE == CELL_CRX;   // also K_e x
   constexpr int CPL = 8 / LPC;  // corners per lane
   __shared__ double grp[5 * kScalMaxGroups];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcdBlock() * blockDim.x + threadIdx.x;
   const int e = t / LPC, l = t % LPC;
   const bool live = e < nValid;
   // first round of loads, all unconditional (indices clamped in bounds,
@@ -474,7 +474,8 @@ __global__ void __launch_bounds__(kNodeThreads) kSolidNode(Dims d, SolidNodeArgs
   __shared__ double grp[5 * kScalMaxGroups];
   const long nDof = 3L * a.nSolid;
   if (nDof == 0) return;
-  const long k0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int blk = xcdBlock();
+  const long k0 = blk * (long)blockDim.x + threadIdx.x;
   const bool live = k0 < nDof;
   const long k = live ? k0 : nDof - 1;
   const long sn = k / 3;
@@ -584,10 +585,10 @@ __global__ void __launch_bounds__(kNodeThreads) kSolidNode(Dims d, SolidNodeArgs
   }
   if (MODE == S_APPLY) return;
   if (kX) {
-    blockPartials<5>(part, a.partials, red);
+    blockPartialsAt<5>(part, a.partials, red, gridDim.x, blk);
   } else {
     double p4[4] = {part[0], part[1], part[2], part[3]};
-    blockPartials<4>(p4, a.partials, red);
+    blockPartialsAt<4>(p4, a.partials, red, gridDim.x, blk);
   }
 }
 
