@@ -274,25 +274,45 @@ __device__ inline void blockPartials(const double (&mine)[K], double* partials, 
 }
 
 // in the finisher (one block, up to 1024 threads): the K totals of the
-// producer's nb blocks, valid in thread 0
+// producer's nb blocks, valid in thread 0.  Every load of a round of 8
+// strides is issued for all K sums before any is added, then one LDS
+// exchange serves all K; per sum the order of additions is that of
+// blockSum over per-thread sums taken in stride order.  lds: 16 * K doubles.
 template <int K>
 __device__ inline void finishSums(int nb, const double* partials, double* lds,
                                   double (&tot)[K]) {
-  for (int k = 0; k < K; k++) {
-    double acc = 0;
-    // 8 strides per round, loads first
-    for (int base = threadIdx.x; base < nb; base += 8 * blockDim.x) {
-      double c[8];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0;
+  for (int base = threadIdx.x; base < nb; base += 8 * blockDim.x) {
+    double c[K][8];
+#pragma unroll
+    for (int k = 0; k < K; k++)
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int i = base + u * blockDim.x;
-        c[u] = i < nb ? partials[(size_t)k * nb + i] : 0.0;
+        c[k][u] = i < nb ? partials[(size_t)k * nb + i] : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 8; u++) acc += c[u];
-    }
-    tot[k] = blockSum(acc, lds);
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) acc[k] += c[k][u];
   }
+  double w[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) w[k] = waveSum(acc[k]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[16 * k + wid] = w[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double r = 0;
+      for (int v = 0; v < nw; v++) r += lds[16 * k + v];
+      tot[k] = r;
+    }
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }

@@ -26,14 +26,17 @@ enum FinishOp {
 template <int OP, int K>
 __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, CrScalars* sc,
                                                 int iter, int first, CrScalars* hostOut) {
-  if (sc->done) {
-    if (hostOut && threadIdx.x == 0) *hostOut = *sc;
-    return;
-  }
-  __shared__ double red[16];
+  // the sums' loads go out together with the done flag's (the partials
+  // of a finished solve's early-exit launches are never used)
+  const bool done = sc->done != 0;
+  __shared__ double red[16 * K];
   double t[K];
   finishSums<K>(nb, partials, red, t);
   if (threadIdx.x != 0) return;
+  if (done) {
+    if (hostOut) *hostOut = *sc;
+    return;
+  }
   if constexpr (OP == F_BNORM) {
     sc->bnorm = sqrt(t[0]);
   } else if constexpr (OP == F_SETUP_S) {
