@@ -499,6 +499,10 @@ def main():
         "kernel_ms_per_step": {"mixed": {k: round(v[0] / prof, 4) for k, v in ktmx.items() if v[1]},
                                "fp64": {k: round(v[0] / prof, 4) for k, v in kt64.items() if v[1]},
                                "profiled_steps": prof},
+        "kernel_us_per_launch": {"mixed": {k: round(v[0] / v[1] * 1e3, 2)
+                                           for k, v in ktmx.items() if v[1]},
+                                 "fp64": {k: round(v[0] / v[1] * 1e3, 2)
+                                          for k, v in kt64.items() if v[1]}},
         "spmv_apply_at_workload": {"fp64": roof64, "fp32": roof32},
         "roofline_hbm_sized": roof_big,
         "served_stream": serve_block,

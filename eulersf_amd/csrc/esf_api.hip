@@ -231,9 +231,9 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = dalloc(ctx, &ctx->sidx, N);
   const long scanBlocks = std::max<long>(N, C) / 1024 + 2;
   if (!st) st = dalloc(ctx, &ctx->scanBlocks, scanBlocks);
-  // 4 partials per block of the widest reducing launch (solid node pass:
-  // 128-thread blocks over at most every node)
-  ctx->nPartials = std::max<int>(4 * std::max(spmvBlocks(d), cdiv(N, 128)) + 64, 1 << 16);
+  // up to 5 partials per block of the widest reducing launch (the pressure
+  // CR update over every SpMV tile, or 128-thread blocks over every node)
+  ctx->nPartials = std::max<int>(5 * std::max(spmvBlocks(d), cdiv(N, 128)) + 64, 1 << 16);
   if (!st) st = dalloc(ctx, &ctx->partials, ctx->nPartials);
   if (!st) st = dalloc(ctx, &ctx->sc, 8);
   if (!st) st = dalloc(ctx, &ctx->dmisc, 16);

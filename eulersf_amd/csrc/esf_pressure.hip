@@ -62,243 +62,351 @@ struct SpmvArgs {
   CrScalars* hostOut;  // M_CHECK (M_UPDATE when recursive): pinned slot for the scalar block
 };
 
-constexpr int TX = 32, TY = 8, TZ = 8;
+constexpr int TX = 32, TY = 8;
 constexpr int HALO = (TX + 2) * (TY + 2);  // cells of one layer incl. halo
 constexpr int NODES = (TX + 1) * (TY + 1); // nodes of one layer
 constexpr int HPT = (HALO + 255) / 256;    // halo cells per thread
-constexpr int NPT = (NODES + 255) / 256;   // nodes per thread
+constexpr int NEDGE = NODES - TX * TY;     // nodes on the tile's low x and low y edges
 
-// raw global values of one halo cell, fetched a layer ahead and turned
-// into the LDS value when stored (so the loads overlap the previous
-// layer's arithmetic)
-template <typename T>
-struct Fetched {
-  T in, ap, s, d, x, p;
-  float e;
-  bool valid, own;
-  long c;
-};
-
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX,
-                                             int tilesY, int tilesZ) {
+// One block owns a TX x TY tile of cells and `layers` cell layers of it,
+// marching up in z.  Iteration k of the march forms node layer k and cell
+// layer k - 2, so one barrier per layer separates producers from
+// consumers:
+//   - node layer k: w = minv * (J^T x) from cell layers k - 1 and k.  Each
+//     thread owns the node at the upper corner of its cell and carries that
+//     node's sums over its four cells of the lower layer from the previous
+//     iteration, so it reads only the four of the upper layer; the NEDGE
+//     nodes on the tile's low x and low y edges are formed by the first
+//     threads from LDS alone (three x buffers: layers k - 1, k, k + 1);
+//   - cell layer k - 2: y = J w, its sum over the corner nodes of layer
+//     k - 2 carried from the previous iteration, those of layer k - 1 read
+//     from LDS (two w buffers);
+//   - x of layer k + 1 goes to LDS; layer k + 2 and the pointwise operands
+//     of the next output layer are fetched, minv of layer k + 1 at the top
+//     of the iteration.
+// The additions keep the reference's sparse row/column order exactly.  A
+// corner node off the interior carries w = 0 and is added like the others:
+// the running sum starts at +0 and adding a zero never changes it.
+// Out-of-grid halo cells are read at a clamped address and replaced by 0,
+// so the loop has no per-cell branches.
+template <typename T, int MODE, int DEPTH>
+__global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, int tilesY,
+                                             int tilesZ, int layers) {
   if (MODE != M_APPLY && a.sc && a.sc->done) return;
-  __shared__ T xs[2][HALO];
-  __shared__ T ws[2][NODES * 3];
-  __shared__ double red[16];
+  __shared__ T xs[3 * HALO];
+  __shared__ T ws[2 * NODES * 3];
+  __shared__ double red[5 * 4];
 
-  // XCD-aware tile order: blocks are dealt round-robin to the 8 XCDs, so
-  // give each XCD a contiguous run of tiles (shared halos stay in one L2).
-  const int nb = tilesX * tilesY * tilesZ;
-  int bid = blockIdx.x;
-  int tile = (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+  // XCD-aware tile order: each XCD gets a contiguous run of tiles, so
+  // neighbouring tiles' shared halos stay in one L2
+  const int tile = xcdBlock();
   const int tx = tile % tilesX, ty = (tile / tilesX) % tilesY,
             tz = tile / (tilesX * tilesY);
-  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-  const int zEnd = min(z0 + TZ, d.CZ);
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * layers;
+  const int zEnd = min(z0 + layers, d.CZ);
 
   T alpha = 0, beta = 0;
   if (MODE == M_UPDATE) {
     alpha = (T)a.sc->alpha;
     beta = (T)a.sc->beta;
   }
-
   double part0 = 0, part1 = 0, part2 = 0, part3 = 0, part4 = 0;
 
-  Fetched<T> f[HPT];
-  auto fetch = [&](int cz) {
-#pragma unroll
-    for (int h = 0; h < HPT; h++) {
-      const int t = threadIdx.x + 256 * h;
-      Fetched<T>& q = f[h];
-      q.valid = false;
-      if (t >= HALO) continue;
-      const int lx = t % (TX + 2), ly = t / (TX + 2);
-      const int cx = x0 - 1 + lx, cy = y0 - 1 + ly;
-      if (cx < 0 || cx >= d.CX || cy < 0 || cy >= d.CY || cz < 0 || cz >= d.CZ) continue;
-      q.valid = true;
-      q.c = cx + (long)cy * d.CX + (long)cz * d.cslab;
-      q.own = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && cz >= z0 && cz < zEnd;
-      q.in = a.in[q.c];
-      if (MODE == M_UPDATE) {
-        q.ap = a.Ap[q.c];
-        q.s = a.s[q.c];
-        q.d = a.D[q.c];
-        if (q.own) {
-          q.x = a.x[q.c];
-          q.p = a.p[q.c];
-        }
-      } else if (MODE == M_IR_RESID) {
-        if (a.e32) q.e = a.e32[q.c];
-      }
-    }
-  };
-  auto store = [&](int cz) {
-    T* L = xs[cz & 1];
-#pragma unroll
-    for (int h = 0; h < HPT; h++) {
-      const int t = threadIdx.x + 256 * h;
-      if (t >= HALO) continue;
-      const Fetched<T>& q = f[h];
-      T v = 0;
-      if (q.valid) {
-        if (MODE == M_UPDATE) {
-          // r_old, Ap_old, s_old read on the halo; r_new (and at own cells
-          // Ap, p, x) written to the other buffers
-          const T apn = q.ap * beta + q.s;
-          const T qq = apn / q.d;
-          v = q.in - alpha * qq;
-          if (q.own) {
-            const T pn = q.p * beta + q.in;
-            a.r[q.c] = v;
-            a.ApOut[q.c] = apn;
-            a.p[q.c] = pn;
-            a.x[q.c] = q.x + alpha * pn;
-          }
-        } else if (MODE == M_IR_RESID) {
-          // x_old (+ e) read on the halo, x_new written to the other buffer
-          v = q.in;
-          if (a.e32) {
-            v = v + (T)q.e;
-            if (q.own) a.x[q.c] = v;
-          }
-        } else {
-          v = q.in;
-        }
-      }
-      L[t] = v;
-    }
-  };
-
-  // minv of the next node layer is loaded into one of two register sets
-  // while the other feeds the current layer (selected by layer parity)
-  T mvA[NPT], mvB[NPT];
-  auto fetchMinv = [&](int k, T* mv) {
-#pragma unroll
-    for (int h = 0; h < NPT; h++) {
-      const int t = threadIdx.x + 256 * h;
-      mv[h] = 0;
-      if (t >= NODES) continue;
-      const int i = x0 + t % (TX + 1), j = y0 + t / (TX + 1);
-      if (i >= 1 && i <= d.X - 2 && j >= 1 && j <= d.Y - 2 && k >= 1 && k <= d.Z - 2)
-        mv[h] = a.minv[i + (long)j * d.X + (long)k * d.slab];
-    }
-  };
-  auto nodeLayer = [&](int k, const T* mv) {
-    T* W = ws[k & 1];
-    const T* L0 = xs[(k - 1) & 1];
-    const T* L1 = xs[k & 1];
-#pragma unroll
-    for (int h = 0; h < NPT; h++) {
-      const int t = threadIdx.x + 256 * h;
-      if (t >= NODES) continue;
-      const int lx = t % (TX + 1), ly = t / (TX + 1);
-      const int i = x0 + lx, j = y0 + ly;
-      T w0 = 0, w1 = 0, w2 = 0;
-      if (i >= 1 && i <= d.X - 2 && j >= 1 && j <= d.Y - 2 && k >= 1 && k <= d.Z - 2) {
-        T m0 = 0, m1 = 0, m2 = 0;
-        // adjacent cells in ascending cell index: dz, dy, dx
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++) {
-          const T* L = dz ? L1 : L0;
-#pragma unroll
-          for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-            for (int dx = 0; dx < 2; dx++) {
-              const T v = L[(ly + dy) * (TX + 2) + lx + dx];
-              m0 += dx == 0 ? v : -v;
-              m1 += dy == 0 ? v : -v;
-              m2 += dz == 0 ? v : -v;
-            }
-        }
-        const T mi = mv[h];
-        w0 = m0 * mi;
-        w1 = m1 * mi;
-        w2 = m2 * mi;
-      }
-      W[3 * t] = w0;
-      W[3 * t + 1] = w1;
-      W[3 * t + 2] = w2;
-    }
-  };
-
-  fetch(z0 - 1);
-  store(z0 - 1);
-  fetch(z0);
-  fetchMinv(z0, mvA);
-  store(z0);
-  __syncthreads();
-  nodeLayer(z0, mvA);
-  fetch(z0 + 1);
-  fetchMinv(z0 + 1, mvB);  // layer z0 + 1 + 2m lives in mvB, z0 + 2 + 2m in mvA
-  __syncthreads();
   const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
   const int cx = x0 + lx, cy = y0 + ly;
-  for (int cz = z0; cz < zEnd; cz++) {
-    // this cell's own input, read before the barrier after which faster
-    // threads may already overwrite layer cz's buffer with layer cz + 2
-    const T ownIn = xs[cz & 1][(ly + 1) * (TX + 2) + lx + 1];
-    store(cz + 1);
-    const bool odd = ((cz - z0) & 1) != 0;
-    if (cz + 2 <= zEnd) {  // next layer's loads in flight during this layer
-      fetch(cz + 2);
-      fetchMinv(cz + 2, odd ? mvB : mvA);
-    }
-    __syncthreads();
-    nodeLayer(cz + 1, odd ? mvA : mvB);
-    __syncthreads();
-    if (cx < d.CX && cy < d.CY) {
-      T acc = 0;
+  const bool cellXY = cx < d.CX && cy < d.CY;
+  // in-layer offset of this thread's cell (clamped into the grid)
+  const int cOff = min(cx, d.CX - 1) + min(cy, d.CY - 1) * d.CX;
+
+  // halo slots of this thread: in-layer offset (clamped) and validity
+  int hOff[HPT];
+  bool hVal[HPT];
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
-        const int ni = cx + ox, nj = cy + oy, nk = cz + oz;
-        if (ni < 1 || ni > d.X - 2 || nj < 1 || nj > d.Y - 2 || nk < 1 || nk > d.Z - 2)
-          continue;
-        const T* W = &ws[(cz + oz) & 1][3 * ((ly + oy) * (TX + 1) + lx + ox)];
-        acc += ox ? W[0] : -W[0];
-        acc += oy ? W[1] : -W[1];
-        acc += oz ? W[2] : -W[2];
-      }
-      const long c = cx + (long)cy * d.CX + (long)cz * d.cslab;
-      if (MODE == M_APPLY) {
-        a.out[c] = acc;
-      } else if (MODE == M_SETUP_R) {
-        const T bv = a.b[c];
-        const T rv = (bv - acc) / a.D[c];
-        a.r[c] = rv;
-        a.p[c] = rv;
-        part0 += (double)bv * (double)bv;
-      } else if (MODE == M_SETUP_S) {
-        a.s[c] = acc;
-        a.Ap[c] = acc;
-        part0 += (double)ownIn * (double)acc;
-        const T q = acc / a.D[c];
-        part1 += (double)acc * (double)q;
-      } else if (MODE == M_UPDATE) {
-        a.sOut[c] = acc;
-        // this cell's new direction, formed as on the halo (cache hits)
-        const T apn = a.Ap[c] * beta + a.s[c];
-        const T dc = a.D[c];
-        const T sq = acc / dc;
-        part0 += (double)ownIn * (double)acc;
-        part1 += (double)acc * (double)sq;
-        part2 += (double)apn * (double)sq;
-        part3 += (double)apn * (double)(apn / dc);
-        // |b - S x| as the recursion carries it: r is D^-1 (b - S x)
-        const double rb = (double)dc * (double)ownIn;
-        part4 += rb * rb;
-      } else if (MODE == M_CHECK) {
-        const T t = acc - a.b[c];
-        part0 += (double)t * (double)t;
+  for (int h = 0; h < HPT; h++) {
+    const int t = threadIdx.x + 256 * h;
+    const int gx = x0 - 1 + t % (TX + 2), gy = y0 - 1 + t / (TX + 2);
+    hVal[h] = t < HALO && gx >= 0 && gx < d.CX && gy >= 0 && gy < d.CY;
+    // a slot past the halo re-reads the thread's first slot (no extra lines)
+    hOff[h] = t < HALO ? min(max(gx, 0), d.CX - 1) + min(max(gy, 0), d.CY - 1) * d.CX
+                       : hOff[0];
+  }
+  // this thread's node: tile node (lx + 1, ly + 1), the upper corner of its
+  // cell; its edge node (threads below NEDGE): (0, t) for t <= TY, else
+  // (t - TY, 0).  In-layer offsets clamped to the interior.
+  const bool hasEdge = threadIdx.x < NEDGE;
+  const int enx = threadIdx.x <= TY ? 0 : threadIdx.x - TY;
+  const int eny = threadIdx.x <= TY ? threadIdx.x : 0;
+  const int mI = x0 + lx + 1, mJ = y0 + ly + 1, eI = x0 + enx, eJ = y0 + eny;
+  const bool mainIJ = mI >= 1 && mI <= d.X - 2 && mJ >= 1 && mJ <= d.Y - 2;
+  const bool edgeIJ = hasEdge && eI >= 1 && eI <= d.X - 2 && eJ >= 1 && eJ <= d.Y - 2;
+  const int mOff = min(max(mI, 1), d.X - 2) + min(max(mJ, 1), d.Y - 2) * d.X;
+  // a thread without an edge node re-reads its main node's minv
+  const int eOff = hasEdge ? min(max(eI, 1), d.X - 2) + min(max(eJ, 1), d.Y - 2) * d.X : mOff;
+  // LDS positions
+  const int mHalo = (ly + 1) * (TX + 2) + lx + 1;  // lowest of the main node's cells
+  const int eHalo = eny * (TX + 2) + enx;          // lowest of the edge node's cells
+  const int mW = 3 * ((ly + 1) * (TX + 1) + lx + 1);
+  const int eW = 3 * (eny * (TX + 1) + enx);
+  const int cW = 3 * (ly * (TX + 1) + lx);         // this cell's lowest corner node
+
+  // Every global load of the loop is issued unconditionally at the end of
+  // an iteration (clamped addresses; values of out-of-range layers are not
+  // used) for the iteration DEPTH later, in the order that iteration
+  // consumes them -- minv (node stage), the output cell's operands (cell
+  // stage), x (LDS store) -- so each consumer waits only for its own
+  // loads.  With DEPTH 2 two such sets are in flight.
+  struct Pre {
+    T mvMain, mvEdge;                 // minv of the main and the edge node
+    T in[HPT], ap[HPT], s[HPT], dd[HPT];  // raw values of the halo cells
+    float e[HPT];
+    T eB, eD, eAp, eS, eIn, eX, eP;   // operands of the output cell
+  };
+  auto fetchX = [&](Pre& P, int cz) {
+    const long zb = (long)min(max(cz, 0), d.CZ - 1) * d.cslab;
+#pragma unroll
+    for (int h = 0; h < HPT; h++) {
+      const long c = zb + hOff[h];
+      P.in[h] = a.in[c];
+      if (MODE == M_UPDATE) {
+        P.ap[h] = a.Ap[c];
+        P.s[h] = a.s[c];
+        P.dd[h] = a.D[c];
       } else if (MODE == M_IR_RESID) {
-        const T bv = a.b[c];
-        const T rr = bv - acc;
-        a.r32[c] = (float)rr;
-        part0 += (double)rr * (double)rr;
-        part1 += (double)bv * (double)bv;
+        P.e[h] = a.e32 ? a.e32[c] : 0.0f;
       }
     }
+  };
+  // x of layer cz into LDS: the halo values (out-of-grid cells 0)
+  auto store = [&](const Pre& P, T* L, int cz) {
+    const bool zok = cz >= 0 && cz < d.CZ;
+#pragma unroll
+    for (int h = 0; h < HPT; h++) {
+      const int t = threadIdx.x + 256 * h;
+      if (h == 0 || t < HALO) {
+        T v = P.in[h];
+        if (MODE == M_UPDATE) {
+          // r_new from r_old, Ap_old and s_old, on the halo as well (the
+          // owner writes it with the cell's other updates)
+          const T apn = P.ap[h] * beta + P.s[h];
+          const T qq = apn / P.dd[h];
+          v = P.in[h] - alpha * qq;
+        } else if (MODE == M_IR_RESID) {
+          // x_old (+ e); the owner writes x_new to the other buffer
+          if (a.e32) v = v + (T)P.e[h];
+        }
+        L[t] = zok && hVal[h] ? v : (T)0;
+      }
+    }
+  };
+  auto fetchMinv = [&](Pre& P, int k) {
+    const long kb = (long)min(max(k, 1), d.Z - 2) * d.slab;
+    P.mvMain = a.minv[kb + mOff];
+    P.mvEdge = a.minv[kb + eOff];
+  };
+  auto fetchOut = [&](Pre& P, int cz) {
+    const long c = (long)min(max(cz, 0), d.CZ - 1) * d.cslab + cOff;
+    if (MODE == M_SETUP_R || MODE == M_CHECK || MODE == M_IR_RESID) P.eB = a.b[c];
+    if (MODE == M_SETUP_R || MODE == M_SETUP_S || MODE == M_UPDATE) P.eD = a.D[c];
+    if (MODE == M_UPDATE) {
+      P.eAp = a.Ap[c];
+      P.eS = a.s[c];
+      P.eIn = a.in[c];
+      P.eX = a.x[c];
+      P.eP = a.p[c];
+    }
+  };
+  // what iteration k consumes: minv of node layer k, the operands of output
+  // cell layer k - 2, x of layer k + 1
+  auto issue = [&](Pre& P, int k) {
+    fetchMinv(P, k);
+    if (MODE != M_APPLY) fetchOut(P, k - 2);
+    fetchX(P, k + 1);
+  };
+
+  // a node's additions over its adjacent cells, in ascending cell index
+  // (dz, dy, dx): lowerSums adds the four of the lower layer to zero,
+  // nodeW the four of the upper layer, then scales by minv
+  auto lowerSums = [&](const T (&lo)[4], T& m0, T& m1, T& m2) {
+    m0 = 0;
+    m1 = 0;
+    m2 = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const T v = lo[q];
+      m0 += (q & 1) == 0 ? v : -v;
+      m1 += (q >> 1) == 0 ? v : -v;
+      m2 += v;
+    }
+  };
+  auto nodeW = [&](T m0, T m1, T m2, const T (&hi)[4], T mi, bool interior, T* W) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const T v = hi[q];
+      m0 += (q & 1) == 0 ? v : -v;
+      m1 += (q >> 1) == 0 ? v : -v;
+      m2 += -v;
+    }
+    W[0] = interior ? m0 * mi : (T)0;
+    W[1] = interior ? m1 * mi : (T)0;
+    W[2] = interior ? m2 * mi : (T)0;
+  };
+  // a cell's additions over four corner nodes of one layer (oz), in
+  // ascending node index (oy, ox)
+  auto cornerSums = [&](T acc, const T (&wc)[12], int oz) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const T* W = &wc[3 * q];
+      acc += (q & 1) ? W[0] : -W[0];
+      acc += (q >> 1) ? W[1] : -W[1];
+      acc += oz ? W[2] : -W[2];
+    }
+    return acc;
+  };
+
+  T ml0, ml1, ml2;   // main node: sums over its lower-layer cells
+  T accLo = 0;       // output cell: sum over its lower-layer corners
+  T own0 = 0, own1 = 0, own2 = 0;  // own input of cell layers k, k - 1, k - 2
+  // LDS buffer offsets: x of layers k - 1, k, k + 1; w of layers k, k - 1
+  int bLo = 0, bHi = HALO, bNx = 2 * HALO;
+  int wCur = 0, wPrev = 3 * NODES;
+
+  // one iteration of the march (k: node layer), consuming the loads in P
+  // and refilling it for iteration k + DEPTH
+  auto step = [&](int k, Pre& P) {
+    T up0 = 0;
+    if (k <= zEnd) {
+      const bool kin = k >= 1 && k <= d.Z - 2;
+      const T* Lhi = xs + bHi;
+      T hi[4];
+      hi[0] = Lhi[mHalo];
+      hi[1] = Lhi[mHalo + 1];
+      hi[2] = Lhi[mHalo + TX + 2];
+      hi[3] = Lhi[mHalo + TX + 3];
+      nodeW(ml0, ml1, ml2, hi, P.mvMain, mainIJ && kin, ws + wCur + mW);
+      lowerSums(hi, ml0, ml1, ml2);
+      up0 = hi[0];
+      if (hasEdge) {
+        const T* Llo = xs + bLo;
+        T elo[4], ehi[4];
+        elo[0] = Llo[eHalo];
+        elo[1] = Llo[eHalo + 1];
+        elo[2] = Llo[eHalo + TX + 2];
+        elo[3] = Llo[eHalo + TX + 3];
+        ehi[0] = Lhi[eHalo];
+        ehi[1] = Lhi[eHalo + 1];
+        ehi[2] = Lhi[eHalo + TX + 2];
+        ehi[3] = Lhi[eHalo + TX + 3];
+        T e0, e1, e2;
+        lowerSums(elo, e0, e1, e2);
+        nodeW(e0, e1, e2, ehi, P.mvEdge, edgeIJ && kin, ws + wCur + eW);
+      }
+    }
+    own2 = own1;
+    own1 = own0;
+    own0 = up0;
+    if (k >= z0 + 1) {
+      // node layer k - 1: the upper corners of cell layer k - 2, the lower
+      // corners of cell layer k - 1
+      const T* W = ws + wPrev + cW;
+      T wu[12];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        wu[c] = W[c];
+        wu[3 + c] = W[3 + c];
+        wu[6 + c] = W[3 * (TX + 1) + c];
+        wu[9 + c] = W[3 * (TX + 1) + 3 + c];
+      }
+      const int cz = k - 2;
+      if (cz >= z0 && cz < zEnd) {
+        const T acc = cornerSums(accLo, wu, 1);
+        const T ownIn = own2;
+        const long c = (long)cz * d.cslab + cOff;
+        if (cellXY) {
+          if (MODE == M_APPLY) {
+            a.out[c] = acc;
+          } else if (MODE == M_SETUP_R) {
+            const T rv = (P.eB - acc) / P.eD;
+            a.r[c] = rv;
+            a.p[c] = rv;
+            part0 += (double)P.eB * (double)P.eB;
+          } else if (MODE == M_SETUP_S) {
+            a.s[c] = acc;
+            a.Ap[c] = acc;
+            part0 += (double)ownIn * (double)acc;
+            const T q = acc / P.eD;
+            part1 += (double)acc * (double)q;
+          } else if (MODE == M_UPDATE) {
+            // the cell's deferred direction update (Ap, p, x) and its
+            // r_new, written to the other buffers, and s = S r_new
+            const T apn = P.eAp * beta + P.eS;
+            const T pn = P.eP * beta + P.eIn;
+            a.r[c] = ownIn;
+            a.ApOut[c] = apn;
+            a.p[c] = pn;
+            a.x[c] = P.eX + alpha * pn;
+            a.sOut[c] = acc;
+            const T sq = acc / P.eD;
+            part0 += (double)ownIn * (double)acc;
+            part1 += (double)acc * (double)sq;
+            part2 += (double)apn * (double)sq;
+            part3 += (double)apn * (double)(apn / P.eD);
+            // |b - S x| as the recursion carries it: r is D^-1 (b - S x)
+            const double rb = (double)P.eD * (double)ownIn;
+            part4 += rb * rb;
+          } else if (MODE == M_CHECK) {
+            const T t = acc - P.eB;
+            part0 += (double)t * (double)t;
+          } else if (MODE == M_IR_RESID) {
+            if (a.e32) a.x[c] = ownIn;
+            const T rr = P.eB - acc;
+            a.r32[c] = (float)rr;
+            part0 += (double)rr * (double)rr;
+            part1 += (double)P.eB * (double)P.eB;
+          }
+        }
+      }
+      accLo = cornerSums((T)0, wu, 0);
+    }
+    if (k + 1 <= zEnd) store(P, xs + bNx, k + 1);
+    issue(P, k + DEPTH);
+    // rotate the LDS buffers
+    const int b = bLo;
+    bLo = bHi;
+    bHi = bNx;
+    bNx = b;
+    const int w = wCur;
+    wCur = wPrev;
+    wPrev = w;
+    __syncthreads();
+  };
+
+  Pre A, B;
+  fetchX(A, z0 - 1);
+  store(A, xs + bLo, z0 - 1);
+  fetchX(A, z0);
+  store(A, xs + bHi, z0);
+  issue(A, z0);
+  if (DEPTH == 2) issue(B, z0 + 1);
+  __syncthreads();
+  {
+    T lo[4];
+    lo[0] = xs[bLo + mHalo];
+    lo[1] = xs[bLo + mHalo + 1];
+    lo[2] = xs[bLo + mHalo + TX + 2];
+    lo[3] = xs[bLo + mHalo + TX + 3];
+    lowerSums(lo, ml0, ml1, ml2);
+  }
+  // layers z0 - 1 .. zEnd pass through: node layers z0 .. zEnd, cell
+  // layers z0 .. zEnd - 1 (two iterations behind); with DEPTH 2 the march
+  // runs in pairs of iterations (a trailing one past zEnd + 1 does nothing
+  // but load)
+  if (DEPTH == 2) {
+    for (int k = z0; k <= zEnd + 1; k += 2) {
+      step(k, A);
+      step(k + 1, B);
+    }
+  } else {
+    for (int k = z0; k <= zEnd + 1; k++) step(k, A);
   }
   if (MODE == M_APPLY) return;
   if (MODE == M_UPDATE) {
@@ -352,19 +460,54 @@ __global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static void tileShape(const Dims& d, int& tx, int& ty, int& tz) {
+// How many iterations ahead the z-march loads (1 or 2): two sets of loads
+// in flight hide more latency but hold more registers.  ESF_SPMV_DEPTH
+// overrides the choice, for tuning.
+static int spmvDepth(int mode) {
+  static const int forced = [] {
+    const char* e = getenv("ESF_SPMV_DEPTH");
+    const int v = e ? atoi(e) : 0;
+    return v == 1 || v == 2 ? v : 0;
+  }();
+  if (forced) return forced;
+  return mode == M_UPDATE ? 1 : 2;
+}
+
+// Cell layers per block of the z-march.  Each block loads two layers of x
+// and forms one node layer beyond the ones it outputs, so deeper blocks
+// waste less; 16 measured best at 128^3 and 256^3 (MI355X).  Small grids
+// take shallower blocks so that at least 512 blocks run where the grid has
+// them.  ESF_SPMV_LAYERS (4..64) overrides the choice, for tuning.
+static int spmvLayers(const Dims& d, int tilesXY) {
+  static const int forced = [] {
+    const char* e = getenv("ESF_SPMV_LAYERS");
+    const int v = e ? atoi(e) : 0;
+    return v >= 4 && v <= 64 ? v : 0;
+  }();
+  if (forced) return forced;
+  int layers = 16;
+  while (layers > 4 && (long)tilesXY * cdiv(d.CZ, layers) < 512) layers /= 2;
+  return layers;
+}
+
+static void tileShape(const Dims& d, int& tx, int& ty, int& tz, int& layers) {
   tx = cdiv(d.CX, TX);
   ty = cdiv(d.CY, TY);
-  tz = cdiv(d.CZ, TZ);
+  layers = spmvLayers(d, tx * ty);
+  tz = cdiv(d.CZ, layers);
 }
 
 template <typename T, int MODE>
 esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
-  int tx, ty, tz;
-  tileShape(ctx->d, tx, ty, tz);
+  int tx, ty, tz, layers;
+  tileShape(ctx->d, tx, ty, tz, layers);
   const int nb = tx * ty * tz;
-  launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE>, dim3(nb), dim3(256),
-              ctx->d, a, tx, ty, tz);
+  if (spmvDepth(MODE) == 2)
+    launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 2>, dim3(nb), dim3(256),
+                ctx->d, a, tx, ty, tz, layers);
+  else
+    launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 1>, dim3(nb), dim3(256),
+                ctx->d, a, tx, ty, tz, layers);
   // the grid-wide sums of the mode (kSpmv's epilogue partials)
   if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
   if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
@@ -382,8 +525,8 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
 }
 
 int spmvBlocks(const Dims& d) {
-  int tx, ty, tz;
-  tileShape(d, tx, ty, tz);
+  int tx, ty, tz, layers;
+  tileShape(d, tx, ty, tz, layers);
   return tx * ty * tz;
 }
 
