@@ -380,13 +380,13 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     __syncthreads();
   };
 
-  Pre A, B;
-  fetchX(A, z0 - 1);
-  store(A, xs + bLo, z0 - 1);
-  fetchX(A, z0);
-  store(A, xs + bHi, z0);
-  issue(A, z0);
-  if (DEPTH == 2) issue(B, z0 + 1);
+  Pre P[DEPTH];
+  fetchX(P[0], z0 - 1);
+  store(P[0], xs + bLo, z0 - 1);
+  fetchX(P[0], z0);
+  store(P[0], xs + bHi, z0);
+#pragma unroll
+  for (int j = 0; j < DEPTH; j++) issue(P[j], z0 + j);
   __syncthreads();
   {
     T lo[4];
@@ -397,16 +397,16 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     lowerSums(lo, ml0, ml1, ml2);
   }
   // layers z0 - 1 .. zEnd pass through: node layers z0 .. zEnd, cell
-  // layers z0 .. zEnd - 1 (two iterations behind); with DEPTH 2 the march
-  // runs in pairs of iterations (a trailing one past zEnd + 1 does nothing
-  // but load)
-  if (DEPTH == 2) {
-    for (int k = z0; k <= zEnd + 1; k += 2) {
-      step(k, A);
-      step(k + 1, B);
-    }
+  // layers z0 .. zEnd - 1 (two iterations behind).  The march runs in
+  // groups of DEPTH iterations, one load set each (trailing iterations past
+  // zEnd + 1 do nothing but load).
+  if (DEPTH == 1) {
+    for (int k = z0; k <= zEnd + 1; k++) step(k, P[0]);
   } else {
-    for (int k = z0; k <= zEnd + 1; k++) step(k, A);
+    for (int k = z0; k <= zEnd + 1; k += DEPTH) {
+#pragma unroll
+      for (int j = 0; j < DEPTH; j++) step(k + j, P[j]);
+    }
   }
   if (MODE == M_APPLY) return;
   if (MODE == M_UPDATE) {
@@ -467,7 +467,7 @@ static int spmvDepth(int mode) {
   static const int forced = [] {
     const char* e = getenv("ESF_SPMV_DEPTH");
     const int v = e ? atoi(e) : 0;
-    return v == 1 || v == 2 ? v : 0;
+    return v >= 1 && v <= 3 ? v : 0;
   }();
   if (forced) return forced;
   return mode == M_UPDATE ? 1 : 2;
@@ -502,7 +502,11 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   int tx, ty, tz, layers;
   tileShape(ctx->d, tx, ty, tz, layers);
   const int nb = tx * ty * tz;
-  if (spmvDepth(MODE) == 2)
+  const int depth = spmvDepth(MODE);
+  if (depth == 3)
+    launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 3>, dim3(nb), dim3(256),
+                ctx->d, a, tx, ty, tz, layers);
+  else if (depth == 2)
     launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 2>, dim3(nb), dim3(256),
                 ctx->d, a, tx, ty, tz, layers);
   else
