@@ -153,6 +153,11 @@ _PROTOS = {
     "esf_frame": (_S, [_P, ctypes.c_double, ctypes.c_double, _I]),
     "esf_cfg_read": (_S, [ctypes.c_char_p, ctypes.POINTER(Grid), ctypes.POINTER(Params),
                          ctypes.POINTER(SceneInfo)]),
+    "esf_field_write_gz": (_S, [ctypes.c_char_p, ctypes.POINTER(Grid), ctypes.c_int, _D]),
+    "esf_field_read_gz": (_S, [ctypes.c_char_p, ctypes.POINTER(Grid), ctypes.c_int, _D,
+                              ctypes.c_size_t]),
+    "esf_write_frame": (_S, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    "esf_read_frame": (_S, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
 }
 
 _lib = None
@@ -189,3 +194,35 @@ def check(ctx, status, what=""):
 
 def dptr(a):
     return a.ctypes.data_as(_D)
+
+
+def field_write_gz(path, grid, data):
+    """FIELD_3D / VEC3F_FIELD_3D::writeGz: `grid` (xres, yres, zres, dh,
+    origin), `data` of 1 or 3 doubles per node (host only)"""
+    import numpy as np
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    xres, yres, zres, dh, origin = grid
+    n = xres * yres * zres
+    comps = a.size // n if n else 0
+    if comps not in (1, 3) or comps * n != a.size:
+        raise ValueError("data must hold 1 or 3 doubles per node")
+    g = Grid(xres, yres, zres, dh, (ctypes.c_double * 3)(*origin))
+    st = load().esf_field_write_gz(path.encode(), ctypes.byref(g), comps, dptr(a))
+    if st:
+        raise EsfError(st, f"cannot write {path}")
+
+
+def field_read_gz(path, components):
+    """FIELD_3D / VEC3F_FIELD_3D::readGz: returns ((xres, yres, zres, dh,
+    origin), data) (host only)"""
+    import numpy as np
+    g = Grid()
+    st = load().esf_field_read_gz(path.encode(), ctypes.byref(g), components, None, 0)
+    if st:
+        raise EsfError(st, f"cannot read {path}")
+    n = g.xres * g.yres * g.zres * components
+    a = np.empty(n, dtype=np.float64)
+    st = load().esf_field_read_gz(path.encode(), ctypes.byref(g), components, dptr(a), n)
+    if st:
+        raise EsfError(st, f"cannot read {path}")
+    return (g.xres, g.yres, g.zres, g.dh, tuple(g.origin)), a

@@ -205,6 +205,16 @@ class CoupledIntegrator3D:
         return n.value
 
     # -- operators ----------------------------------------------------------
+    def writeFrame(self, data_path, frame, debug=False):
+        """COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011)"""
+        check(self.ctx, self.lib.esf_write_frame(self.ctx, data_path.encode(), frame,
+                                                 1 if debug else 0), "writeFrame")
+
+    def readFrame(self, data_path, frame, debug=False):
+        """COUPLED_INTEGRATOR_3D::readFrame (COUPLED_INTEGRATOR_3D.cpp:976-997)"""
+        check(self.ctx, self.lib.esf_read_frame(self.ctx, data_path.encode(), frame,
+                                                1 if debug else 0), "readFrame")
+
     def getMatVecMult(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)

@@ -289,6 +289,37 @@ esf_status esf_scene_load_cfg(const char* cfg_path, int precision,
 esf_status esf_frame(esf_ctx* ctx, double frame_dt, double alpha,
                      int* substeps);
 
+/* --- frame files (host; the reference's gz field format) -------------------*/
+/* FIELD_3D::writeGz / VEC3F_FIELD_3D::writeGz (FIELD_3D.h:286-321,
+ * VEC3F_FIELD_3D.h:199-229): gzip level 1, header int32 X, Y, Z; double dh,
+ * origin[3]; then components * X * Y * Z doubles, x fastest, vector
+ * components interleaved.  components: 1 or 3.  Host only: needs no GPU. */
+esf_status esf_field_write_gz(const char* path, const esf_grid* grid,
+                              int components, const double* data);
+/* FIELD_3D::readGz / VEC3F_FIELD_3D::readGz (FIELD_3D.h:323-365,
+ * VEC3F_FIELD_3D.h:231-267): the header into *grid, then, when data is not
+ * NULL, the payload (ESF_ERR_INVALID if it exceeds `capacity` doubles).
+ * ESF_ERR_IO for a missing or short file (the reference exits there).
+ * Host only. */
+esf_status esf_field_read_gz(const char* path, esf_grid* grid, int components,
+                             double* data, size_t capacity);
+/* COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011) with
+ * DEFOR_SOLID_3D::writeFrame (DEFOR_SOLID_3D.cpp:1361-1371): under
+ * "<data_path>frame.NNNN" (at least four digits, IO.h:27-37) the intensity
+ * every frame; the grid velocity and, per solid i, ".solid.i.displacement"
+ * and ".solid.i.velocity" when debug or frame % 10 == 0.  The solid SDF
+ * (".phi") and surface mesh (".obj", ".pbrt") are not kept by this library
+ * and are not written. */
+esf_status esf_write_frame(esf_ctx* ctx, const char* data_path, int frame,
+                           int debug);
+/* COUPLED_INTEGRATOR_3D::readFrame (COUPLED_INTEGRATOR_3D.cpp:976-997) with
+ * DEFOR_SOLID_3D::readFrame (DEFOR_SOLID_3D.cpp:1299-1314): the same files
+ * back (a grid that differs from the context's is ESF_ERR_INVALID), each
+ * solid's velocity workspace set from its velocity and X = x - u, then
+ * gatherSolidMasses and recoverVelocityField. */
+esf_status esf_read_frame(esf_ctx* ctx, const char* data_path, int frame,
+                          int debug);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,10 @@ def ref():
         R.ref_get_int.argtypes = [ctypes.c_char_p, ctypes.c_int]
         R.ref_get_string.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         R.ref_get_vec3.argtypes = [ctypes.c_char_p, _D, _D]
+        R.ref_field_write_gz.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, _D, ctypes.c_int, _D]
+        R.ref_field_read_gz.restype = ctypes.c_long
+        R.ref_field_read_gz.argtypes = [ctypes.c_char_p, ctypes.c_int, _I, _D, _D, ctypes.c_long]
         _ref = R
     return _ref
 

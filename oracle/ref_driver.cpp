@@ -8,7 +8,8 @@
  *   COROTATION            src/material/COROTATION.cpp
  *   CONJUGATE_RESIDUAL    include/linearalgebra/CONJUGATE_RESIDUAL.inl
  *   JACOBI_PRECONDITIONER include/linearalgebra/JACOBI_PRECONDITIONER.h
- *   FIELD_3D, VEC3F_FIELD_3D include/geometry/*.h (interpolate_value)
+ *   FIELD_3D, VEC3F_FIELD_3D include/geometry/*.h (interpolate_value,
+ *                         writeGz / readGz)
  *   SIMPLE_PARSER         src/util/SIMPLE_PARSER.cpp
  */
 #include <SETTINGS.h>
@@ -166,6 +167,51 @@ int ref_get_string(const char* key, const char* def, char* out, int cap) {
 void ref_get_vec3(const char* key, const double def[3], double out[3]) {
   VEC3F v = SIMPLE_PARSER::getVEC3F(key, VEC3F(def[0], def[1], def[2]));
   for (int i = 0; i < 3; i++) out[i] = v[i];
+}
+
+// FIELD_3D<double>::writeGz (components 1) or VEC3F_FIELD_3D::writeGz
+// (components 3) of an x-fastest field
+void ref_field_write_gz(const char* path, int X, int Y, int Z, double dh, const double o[3],
+                        int components, const double* data) {
+  const long n = (long)X * Y * Z;
+  if (components == 1) {
+    FIELD_3Df field(X, Y, Z, dh, VEC3F(o[0], o[1], o[2]));
+    std::memcpy(field.data().data(), data, sizeof(double) * n);
+    field.writeGz(std::string(path));
+  } else {
+    VEC3F_FIELD_3D field(X, Y, Z, dh, VEC3F(o[0], o[1], o[2]));
+    for (long i = 0; i < n; i++) field[i] = VEC3F(data[3 * i], data[3 * i + 1], data[3 * i + 2]);
+    field.writeGz(std::string(path));
+  }
+}
+
+// FIELD_3D<double>::readGz / VEC3F_FIELD_3D::readGz: dims (3 ints), header
+// (dh, origin), and up to cap doubles of the payload; returns the node count
+long ref_field_read_gz(const char* path, int components, int dims[3], double hdr[4],
+                       double* data, long cap) {
+  if (components == 1) {
+    FIELD_3Df field;
+    field.readGz(std::string(path));
+    dims[0] = field.xRes();
+    dims[1] = field.yRes();
+    dims[2] = field.zRes();
+    hdr[0] = field.dh();
+    for (int c = 0; c < 3; c++) hdr[1 + c] = field.origin()[c];
+    const long n = (long)dims[0] * dims[1] * dims[2];
+    for (long i = 0; i < n && i < cap; i++) data[i] = field[i];
+    return n;
+  }
+  VEC3F_FIELD_3D field;
+  field.readGz(std::string(path));
+  dims[0] = field.xRes();
+  dims[1] = field.yRes();
+  dims[2] = field.zRes();
+  hdr[0] = field.dh();
+  for (int c = 0; c < 3; c++) hdr[1 + c] = field.origin()[c];
+  const long n = (long)dims[0] * dims[1] * dims[2];
+  for (long i = 0; i < n && 3 * i + 2 < cap; i++)
+    for (int c = 0; c < 3; c++) data[3 * i + c] = field[i][c];
+  return n;
 }
 
 }  // extern "C"
