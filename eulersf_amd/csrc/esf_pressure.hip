@@ -150,8 +150,10 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
   const int cW = 3 * (ly * (TX + 1) + lx);         // this cell's lowest corner node
 
   // Every global load of the loop is issued unconditionally at the end of
-  // an iteration (clamped addresses; values of out-of-range layers are not
-  // used) for the iteration DEPTH later, in the order that iteration
+  // an iteration for the iteration DEPTH later (addresses clamped into the
+  // grid and to the block's own last layers, so the loads issued past the
+  // end of the march re-read lines just fetched; their values are not
+  // used), in the order that iteration
   // consumes them -- minv (node stage), the output cell's operands (cell
   // stage), x (LDS store) -- so each consumer waits only for its own
   // loads.  With DEPTH 2 two such sets are in flight.
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     T eB, eD, eAp, eS, eIn, eX, eP;   // operands of the output cell
   };
   auto fetchX = [&](Pre& P, int cz) {
-    const long zb = (long)min(max(cz, 0), d.CZ - 1) * d.cslab;
+    const long zb = (long)min(max(cz, 0), min(zEnd, d.CZ - 1)) * d.cslab;
 #pragma unroll
     for (int h = 0; h < HPT; h++) {
       const long c = zb + hOff[h];
@@ -199,12 +201,12 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     }
   };
   auto fetchMinv = [&](Pre& P, int k) {
-    const long kb = (long)min(max(k, 1), d.Z - 2) * d.slab;
+    const long kb = (long)max(min(max(k, 1), min(zEnd, d.Z - 2)), 0) * d.slab;
     P.mvMain = a.minv[kb + mOff];
     P.mvEdge = a.minv[kb + eOff];
   };
   auto fetchOut = [&](Pre& P, int cz) {
-    const long c = (long)min(max(cz, 0), d.CZ - 1) * d.cslab + cOff;
+    const long c = (long)min(max(cz, 0), zEnd - 1) * d.cslab + cOff;
     if (MODE == M_SETUP_R || MODE == M_CHECK || MODE == M_IR_RESID) P.eB = a.b[c];
     if (MODE == M_SETUP_R || MODE == M_SETUP_S || MODE == M_UPDATE) P.eD = a.D[c];
     if (MODE == M_UPDATE) {
