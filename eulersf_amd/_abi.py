@@ -157,6 +157,13 @@ _PROTOS = {
     "esf_field_read_gz": (_S, [ctypes.c_char_p, ctypes.POINTER(Grid), ctypes.c_int, _D,
                               ctypes.c_size_t]),
     "esf_write_frame": (_S, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    "esf_solid_set_mesh": (_S, [_P, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _I]),
+    "esf_solid_mesh_size": (_S, [_P, ctypes.c_int, _I, _I]),
+    "esf_solid_get_mesh": (_S, [_P, ctypes.c_int, _D, _D, _I]),
+    "esf_advect_surface_mesh": (_S, [_P]),
+    "esf_broad_phase": (_S, [_P, _D]),
+    "esf_obj_read": (_S, [ctypes.c_char_p, _I, _I, _D, ctypes.c_int, _I, ctypes.c_int]),
+    "esf_obj_write": (_S, [ctypes.c_char_p, ctypes.c_int, _D, ctypes.c_int, _I]),
     "esf_read_frame": (_S, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
 }
 
@@ -226,3 +233,29 @@ def field_read_gz(path, components):
     if st:
         raise EsfError(st, f"cannot read {path}")
     return (g.xres, g.yres, g.zres, g.dh, tuple(g.origin)), a
+
+
+def obj_read(path):
+    """OBJ::Load as the solver uses it: (vertices (n, 3), triangles (m, 3))"""
+    import numpy as np
+    nv, nt = ctypes.c_int(), ctypes.c_int()
+    st = load().esf_obj_read(path.encode(), ctypes.byref(nv), ctypes.byref(nt), None, 0, None, 0)
+    if st:
+        raise EsfError(st, f"cannot read {path}")
+    v = np.empty(3 * nv.value)
+    t = np.empty(3 * nt.value, dtype=np.int32)
+    st = load().esf_obj_read(path.encode(), ctypes.byref(nv), ctypes.byref(nt), dptr(v), nv.value,
+                             t.ctypes.data_as(_I), nt.value)
+    if st:
+        raise EsfError(st, f"cannot read {path}")
+    return v.reshape(-1, 3), t.reshape(-1, 3)
+
+
+def obj_write(path, vertices, triangles):
+    import numpy as np
+    v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1)
+    st = load().esf_obj_write(path.encode(), v.size // 3, dptr(v), t.size // 3,
+                              t.ctypes.data_as(_I))
+    if st:
+        raise EsfError(st, f"cannot write {path}")

@@ -237,6 +237,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = dalloc(ctx, &ctx->partials, ctx->nPartials);
   if (!st) st = dalloc(ctx, &ctx->sc, 8);
   if (!st) st = dalloc(ctx, &ctx->dmisc, 16);
+  if (!st) st = dalloc(ctx, &ctx->meshOut, 6 * kMaxSolids + 32);
   if (!st) st = growSolidDofs(ctx, 0);
   if (!st && hipHostMalloc((void**)&ctx->hostRing, 8 * sizeof(CrScalars)) != hipSuccess)
     st = ESF_ERR_OOM;
@@ -346,6 +347,73 @@ esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu, double density,
   fillInt(ctx, d.cells, so.cellSlot, -1);
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   if (id) *id = (int)ctx->solids.size() - 1;
+  return ESF_OK;
+}
+
+esf_status esf_solid_set_mesh(esf_ctx* ctx, int solid, int nv, const double* rest,
+                             const double* vertices, int nt, const int* tris) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size() || nv < 0 || nt < 0)
+    return ESF_ERR_INVALID;
+  if (nv > 0 && (!rest || !vertices || nt <= 0 || !tris)) return ESF_ERR_INVALID;
+  for (long k = 0; k < 3L * nt; k++)
+    if (tris[k] < 0 || tris[k] >= nv) {
+      ctx->err = "triangle vertex index out of range";
+      return ESF_ERR_INVALID;
+    }
+  DevSolid& so = ctx->solids[solid];
+  esf_status st = ESF_OK;
+  if (nv == 0) {
+    so.meshV = so.meshT = 0;
+    so.meshTriHost.clear();
+    return ESF_OK;
+  }
+  if (!st) st = dalloc(ctx, &so.meshRest, 3L * nv);
+  if (!st) st = dalloc(ctx, &so.meshVert, 3L * nv);
+  if (!st) st = dalloc(ctx, &so.meshTri, 3L * nt);
+  if (!st) st = dalloc(ctx, &so.meshBox, 6L * nt);
+  if (st) return st;
+  ESF_HIP(hipMemcpyAsync(so.meshRest, rest, 24L * nv, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(so.meshVert, vertices, 24L * nv, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(so.meshTri, tris, 12L * nt, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  so.meshTriHost.assign(tris, tris + 3L * nt);
+  so.meshV = nv;
+  so.meshT = nt;
+  return ESF_OK;
+}
+
+esf_status esf_solid_mesh_size(esf_ctx* ctx, int solid, int* nv, int* nt) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size() || !nv || !nt)
+    return ESF_ERR_INVALID;
+  *nv = ctx->solids[solid].meshV;
+  *nt = ctx->solids[solid].meshT;
+  return ESF_OK;
+}
+
+esf_status esf_solid_get_mesh(esf_ctx* ctx, int solid, double* vertices, double* rest,
+                              int* tris) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  const DevSolid& so = ctx->solids[solid];
+  if (so.meshV == 0) {
+    ctx->err = "solid has no surface mesh";
+    return ESF_ERR_STATE;
+  }
+  if (vertices)
+    ESF_HIP(hipMemcpyAsync(vertices, so.meshVert, 24L * so.meshV, hipMemcpyDeviceToHost,
+                           ctx->stream));
+  if (rest)
+    ESF_HIP(hipMemcpyAsync(rest, so.meshRest, 24L * so.meshV, hipMemcpyDeviceToHost,
+                           ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  if (tris) std::copy(so.meshTriHost.begin(), so.meshTriHost.end(), tris);
+  return ESF_OK;
+}
+
+esf_status esf_advect_surface_mesh(esf_ctx* ctx) {
+  if (!ctx) return ESF_ERR_INVALID;
+  esf_status st = advectMeshes(ctx);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
   return ESF_OK;
 }
 
@@ -656,6 +724,8 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if ((st = coupledSolveEnqueue(ctx, &s))) return st;
   if ((st = esf_solid_copy_velocity(ctx))) return st;
   if ((st = esf_solid_finalize(ctx))) return st;
+  // DEFOR_SOLID_3D::advectSurfaceMesh (COUPLED_INTEGRATOR_3D.cpp:863-866)
+  if ((st = advectMeshes(ctx))) return st;
   if ((st = esf_gather_solid_masses(ctx))) return st;
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;

@@ -10,72 +10,9 @@
 #include <algorithm>
 
 #include "esf_internal.h"
+#include "esf_interp.h"
 
 namespace esf {
-
-// ---------------------------------------------------------------------------
-// trilinear interpolation with the reference's clamp rule
-// (FIELD_3D.h:198-216, :368-382; MATH_UTILS.h:326-342)
-// ---------------------------------------------------------------------------
-struct FieldGeom {
-  int X, Y, Z;
-  double dh, o0, o1, o2;
-};
-
-__device__ __forceinline__ void bary(double x, double dh, int hi, int& i, double& f) {
-  x /= dh;
-  const double s = floor(x);
-  i = (int)s;
-  if (i < 0) {
-    i = 0;
-    f = 0.5;
-  } else if (i > hi) {
-    i = hi;
-    f = 0.5;
-  } else {
-    f = x - s;
-  }
-}
-__device__ __forceinline__ double lerp(double a, double b, double f) { return (1 - f) * a + f * b; }
-
-__device__ __forceinline__ long locate(const FieldGeom& g, const double* p, double& fx, double& fy,
-                                       double& fz) {
-  int i, j, k;
-  bary(p[0] - g.o0, g.dh, g.X - 2, i, fx);
-  bary(p[1] - g.o1, g.dh, g.Y - 2, j, fy);
-  bary(p[2] - g.o2, g.dh, g.Z - 2, k, fz);
-  return i + (long)j * g.X + (long)k * g.X * g.Y;
-}
-
-__device__ __forceinline__ double interpS(const double* d, const FieldGeom& g, const double* p) {
-  double fx, fy, fz;
-  const long b = locate(g, p, fx, fy, fz);
-  const long X = g.X, S = (long)g.X * g.Y;
-  const double z0 = lerp(lerp(d[b], d[b + 1], fx), lerp(d[b + X], d[b + X + 1], fx), fy);
-  const double z1 =
-      lerp(lerp(d[b + S], d[b + S + 1], fx), lerp(d[b + S + X], d[b + S + X + 1], fx), fy);
-  return lerp(z0, z1, fz);
-}
-
-__device__ __forceinline__ void interpV(const double* d, const FieldGeom& g, const double* p,
-                                        double* out) {
-  double fx, fy, fz;
-  const long b = locate(g, p, fx, fy, fz);
-  const long X = g.X, S = (long)g.X * g.Y;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const double* q = d + c;
-    const double z0 = lerp(lerp(q[3 * b], q[3 * (b + 1)], fx),
-                           lerp(q[3 * (b + X)], q[3 * (b + X + 1)], fx), fy);
-    const double z1 = lerp(lerp(q[3 * (b + S)], q[3 * (b + S + 1)], fx),
-                           lerp(q[3 * (b + S + X)], q[3 * (b + S + X + 1)], fx), fy);
-    out[c] = lerp(z0, z1, fz);
-  }
-}
-
-__device__ __forceinline__ FieldGeom gridGeom(const Dims& d) {
-  return FieldGeom{d.X, d.Y, d.Z, d.dh, d.o0, d.o1, d.o2};
-}
 
 __device__ __forceinline__ bool isInterior(const Dims& d, int x, int y, int z) {
   return x >= 1 && x <= d.X - 2 && y >= 1 && y <= d.Y - 2 && z >= 1 && z <= d.Z - 2;

@@ -74,6 +74,14 @@ struct DevSolid {
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
   int* cornerDof = nullptr;  // per slot and corner: 3 * solid index, or -1 if masked
   bool listDirty = true;
+  // surface mesh (DEFOR_SOLID_3D::_surfaceMesh): rest pose and current
+  // vertices (3 doubles each), triangles (3 vertex indices), triangle boxes
+  int meshV = 0, meshT = 0;
+  double* meshRest = nullptr;
+  double* meshVert = nullptr;
+  int* meshTri = nullptr;
+  double* meshBox = nullptr;
+  std::vector<int> meshTriHost;
 };
 
 }  // namespace esf
@@ -125,6 +133,7 @@ struct esf_ctx {
   double* sPartials = nullptr; // solid CR: 4 sets of 4 x node-pass blocks
   esf::CrScalars* sc = nullptr;   // device scalars (several solves)
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
+  double* meshOut = nullptr;      // mesh broad phase: root boxes, then pair flags
   int* hostFlags = nullptr;       // pinned: counts read back by the host
   esf::CrScalars* hostRing = nullptr;  // pinned ring of polled scalar blocks
   int crQueued = 0;  // iterations the last pressure CR solve queued
@@ -349,5 +358,11 @@ inline void launchTimed(esf_ctx* ctx, int cls, F kernel, dim3 grid, dim3 block, 
   t->cls = cls;
   hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t->a, t->b, 0, args...);
 }
+
+// esf_mesh.hip: surface meshes (advectSurfaceMesh for every solid with a
+// mesh; the mesh broad phase: root boxes, 6 doubles per solid, and one
+// collide flag per solid pair i < j)
+esf_status advectMeshes(esf_ctx* ctx);
+esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<int>& pairs);
 
 }  // namespace esf

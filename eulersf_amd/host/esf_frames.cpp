@@ -4,11 +4,16 @@
 // files the integrator writes and resumes from
 // (COUPLED_INTEGRATOR_3D::writeFrame / readFrame,
 // COUPLED_INTEGRATOR_3D.cpp:976-1011; DEFOR_SOLID_3D::writeFrame /
-// readFrame, DEFOR_SOLID_3D.cpp:1299-1371).  Device fields cross PCIe
+// readFrame, DEFOR_SOLID_3D.cpp:1299-1371), and the OBJ surface meshes
+// (OBJ::Load / Save, src/geometry/OBJ.cpp:97-375).  Device fields cross PCIe
 // through esf_download / esf_upload; nothing here runs on the GPU.
 #include <zlib.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -101,6 +106,62 @@ static esf_grid gridOf(const esf_ctx* ctx) {
   return g;
 }
 
+// OBJ::Load (src/geometry/OBJ.cpp:97-295) for what the solver uses: "v"
+// lines give the vertices, each "f" line (indices v, v/t, v/t/n or v//n,
+// 1-based, negative = relative to the end) a triangle from its first three
+// vertices (:256-277).  Other lines are skipped.
+static bool readObj(const std::string& path, std::vector<double>& v, std::vector<int>& tri) {
+  std::ifstream in(path);
+  if (in.fail()) return false;
+  v.clear();
+  tri.clear();
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream is(line);
+    std::string type;
+    is >> type;
+    if (is.fail()) continue;
+    if (type == "v") {
+      double x = 0, y = 0, z = 0;
+      is >> x >> y >> z;
+      if (is.fail()) return false;
+      v.push_back(x);
+      v.push_back(y);
+      v.push_back(z);
+    } else if (type == "f") {
+      std::vector<int> idx;
+      std::string tok;
+      while (is >> tok) {
+        const long nvert = (long)(v.size() / 3);
+        const size_t slash = tok.find('/');
+        const std::string first = tok.substr(0, slash);
+        char* end = nullptr;
+        long k = std::strtol(first.c_str(), &end, 10);
+        if (end == first.c_str() || *end != '\0') return false;
+        if (k < 0) k = nvert + 1 + k;
+        if (k < 1 || k > nvert) return false;
+        idx.push_back((int)(k - 1));
+      }
+      if (idx.size() < 3) return false;
+      for (int y = 0; y < 3; y++) tri.push_back(idx[y]);
+    }
+  }
+  return true;
+}
+
+static bool writeObj(const std::string& path, int nv, const double* v, int nt, const int* tri) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return false;
+  // full precision (OBJ::Save, OBJ.cpp:336-375, streams 6 significant
+  // digits; the extra digits keep a resumed mesh identical)
+  for (int i = 0; i < nv; i++)
+    std::fprintf(f, "v %.17g %.17g %.17g\n", v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+  for (int t = 0; t < nt; t++)
+    std::fprintf(f, "f %d %d %d\n", tri[3 * t] + 1, tri[3 * t + 1] + 1, tri[3 * t + 2] + 1);
+  const bool ok = std::ferror(f) == 0;
+  return std::fclose(f) == 0 && ok;
+}
+
 static bool sameGrid(const esf_grid& a, const esf_grid& b) {
   return a.xres == b.xres && a.yres == b.yres && a.zres == b.zres && a.dh == b.dh &&
          a.origin[0] == b.origin[0] && a.origin[1] == b.origin[1] && a.origin[2] == b.origin[2];
@@ -124,6 +185,33 @@ extern "C" esf_status esf_field_read_gz(const char* path, esf_grid* grid, int co
   return readGzField(path, grid, components, data, capacity);
 }
 
+extern "C" esf_status esf_obj_read(const char* path, int* nv, int* nt, double* vertices,
+                                   int cap_v, int* tris, int cap_t) {
+  if (!path || !nv || !nt) return ESF_ERR_INVALID;
+  std::vector<double> v;
+  std::vector<int> t;
+  if (!readObj(path, v, t)) return ESF_ERR_IO;
+  *nv = (int)(v.size() / 3);
+  *nt = (int)(t.size() / 3);
+  if (vertices) {
+    if (cap_v < *nv) return ESF_ERR_INVALID;
+    std::copy(v.begin(), v.end(), vertices);
+  }
+  if (tris) {
+    if (cap_t < *nt) return ESF_ERR_INVALID;
+    std::copy(t.begin(), t.end(), tris);
+  }
+  return ESF_OK;
+}
+
+extern "C" esf_status esf_obj_write(const char* path, int nv, const double* vertices, int nt,
+                                    const int* tris) {
+  if (!path || nv < 0 || nt < 0 || (nv && !vertices) || (nt && !tris)) return ESF_ERR_INVALID;
+  for (long k = 0; k < 3L * nt; k++)
+    if (tris[k] < 0 || tris[k] >= nv) return ESF_ERR_INVALID;
+  return writeObj(path, nv, vertices, nt, tris) ? ESF_OK : ESF_ERR_IO;
+}
+
 extern "C" esf_status esf_write_frame(esf_ctx* ctx, const char* dataPath, int frame, int debug) {
   if (!ctx || !dataPath || frame < 0) return ESF_ERR_INVALID;
   const esf_grid g = gridOf(ctx);
@@ -140,6 +228,16 @@ extern "C" esf_status esf_write_frame(esf_ctx* ctx, const char* dataPath, int fr
     if (!st) st = writeGzField(sb + ".displacement", g, 3, buf.data());
     if (!st) st = esf_download(ctx, ESF_S_VELOCITY, (int)i, buf.data(), 24 * N);
     if (!st) st = writeGzField(sb + ".velocity", g, 3, buf.data());
+  }
+  // the surface mesh every frame (DEFOR_SOLID_3D.cpp:1368)
+  for (size_t i = 0; !st && i < ctx->solids.size(); i++) {
+    const DevSolid& so = ctx->solids[i];
+    if (so.meshV == 0) continue;
+    std::vector<double> mv(3L * so.meshV);
+    st = esf_solid_get_mesh(ctx, (int)i, mv.data(), nullptr, nullptr);
+    if (!st && !writeObj(base + ".solid." + std::to_string(i) + ".obj", so.meshV, mv.data(),
+                         so.meshT, so.meshTriHost.data()))
+      st = ESF_ERR_IO;
   }
   if (full && !st) {
     st = esf_download(ctx, ESF_F_VELOCITY, -1, buf.data(), 24 * N);
@@ -196,6 +294,24 @@ extern "C" esf_status esf_read_frame(esf_ctx* ctx, const char* dataPath, int fra
           for (int c = 0; c < 3; c++) buf[3 * n + c] = pos[c] - u[3 * n + c];
         }
     st = esf_upload(ctx, ESF_S_X, (int)i, buf.data(), 24 * N);
+    // the surface mesh's vertices (:1315-1317); the topology must match
+    DevSolid& so = ctx->solids[i];
+    if (!st && so.meshV > 0) {
+      const std::string path = sb + ".obj";
+      std::vector<double> mv;
+      std::vector<int> mt;
+      if (!readObj(path, mv, mt)) {
+        ctx->err = "cannot read " + path;
+        st = ESF_ERR_IO;
+      } else if ((int)(mv.size() / 3) != so.meshV || mt != so.meshTriHost) {
+        ctx->err = path + ": mesh differs from the solid's";
+        st = ESF_ERR_INVALID;
+      } else {
+        ESF_HIP(hipMemcpyAsync(so.meshVert, mv.data(), mv.size() * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+        ESF_HIP(hipStreamSynchronize(ctx->stream));
+      }
+    }
   }
   // grid (COUPLED_INTEGRATOR_3D.cpp:983-990)
   if (!st && full) {

@@ -190,6 +190,56 @@ void rotationFromCfg(const double r[3], double R[9]) {
   R[8] = 1 - (txx + tyy);
 }
 
+// A triangulated sphere of radius r about the rest-space origin, the
+// surface of an analytic solid sphere (the reference loads its sphere's
+// surface from an OBJ file): the icosahedron with each triangle split in
+// four `level` times, the new vertices pushed out onto the sphere.
+static void icosphere(double r, int level, std::vector<double>& v, std::vector<int>& t) {
+  const double a = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double base[12][3] = {{-1, a, 0}, {1, a, 0}, {-1, -a, 0}, {1, -a, 0},
+                              {0, -1, a}, {0, 1, a}, {0, -1, -a}, {0, 1, -a},
+                              {a, 0, -1}, {a, 0, 1}, {-a, 0, -1}, {-a, 0, 1}};
+  const int faces[20][3] = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11},
+                            {1, 5, 9},  {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+                            {3, 9, 4},  {3, 4, 2},  {3, 2, 6},   {3, 6, 8},  {3, 8, 9},
+                            {4, 9, 5},  {2, 4, 11}, {6, 2, 10},  {8, 6, 7},  {9, 8, 1}};
+  std::vector<std::array<double, 3>> p;
+  auto push = [&](double x, double y, double z) {
+    const double n = std::sqrt(x * x + y * y + z * z);
+    p.push_back({x / n, y / n, z / n});
+    return (int)p.size() - 1;
+  };
+  for (const auto& q : base) push(q[0], q[1], q[2]);
+  std::vector<std::array<int, 3>> tri;
+  for (const auto& f : faces) tri.push_back({f[0], f[1], f[2]});
+  for (int l = 0; l < level; l++) {
+    std::map<std::pair<int, int>, int> mid;
+    auto midpoint = [&](int i, int j) {
+      const std::pair<int, int> key(std::min(i, j), std::max(i, j));
+      auto it = mid.find(key);
+      if (it != mid.end()) return it->second;
+      const int k = push(p[i][0] + p[j][0], p[i][1] + p[j][1], p[i][2] + p[j][2]);
+      mid[key] = k;
+      return k;
+    };
+    std::vector<std::array<int, 3>> next;
+    for (const auto& f : tri) {
+      const int ab = midpoint(f[0], f[1]), bc = midpoint(f[1], f[2]), ca = midpoint(f[2], f[0]);
+      next.push_back({f[0], ab, ca});
+      next.push_back({f[1], bc, ab});
+      next.push_back({f[2], ca, bc});
+      next.push_back({ab, bc, ca});
+    }
+    tri.swap(next);
+  }
+  v.clear();
+  for (const auto& q : p)
+    for (int c = 0; c < 3; c++) v.push_back(r * q[c]);
+  t.clear();
+  for (const auto& f : tri)
+    for (int c = 0; c < 3; c++) t.push_back(f[c]);
+}
+
 // DEFOR_SOLID_3D::setInitialState  DEFOR_SOLID_3D.cpp:126-152
 void initialState(int X, int Y, int Z, double dh, const double o[3], const double R[9],
                   const double t[3], const double s[3], std::vector<double>& Xm,
@@ -315,6 +365,39 @@ extern "C" esf_status esf_scene_load_cfg(const char* path, int precision, esf_ct
       esf_ctx_destroy(ctx);
       return st;
     }
+    // surface mesh (main.cpp:140-152): the OBJ's vertices are the rest pose
+    // (OBJ.cpp:255); the current vertices are scaled, rotated, translated.
+    // Without "<name>.obj" an analytic sphere gets a triangulated sphere
+    // ("solid sphere mesh level i" subdivisions, default 2; negative: none).
+    std::vector<double> mrest;
+    std::vector<int> mtri;
+    int nv = 0, nt = 0;
+    if (esf_obj_read((base + ".obj").c_str(), &nv, &nt, nullptr, 0, nullptr, 0) == ESF_OK &&
+        nv > 0 && nt > 0) {
+      mrest.resize(3L * nv);
+      mtri.resize(3L * nt);
+      esf_obj_read((base + ".obj").c_str(), &nv, &nt, mrest.data(), nv, mtri.data(), nt);
+    } else if (cfg.has("solid sphere radius " + idx)) {
+      const int level = cfg.getInt("solid sphere mesh level " + idx, 2);
+      if (level >= 0)
+        icosphere(cfg.getFloat("solid sphere radius " + idx, 0.1), std::min(level, 6), mrest,
+                  mtri);
+    }
+    if (!mrest.empty()) {
+      std::vector<double> mv(mrest.size());
+      for (size_t k = 0; k < mrest.size() / 3; k++) {
+        double q[3];
+        for (int c = 0; c < 3; c++) q[c] = mrest[3 * k + c] * s[c];  // OBJ::scale
+        for (int r2 = 0; r2 < 3; r2++)                                // transformMesh
+          mv[3 * k + r2] = ((R[3 * r2] * q[0] + R[3 * r2 + 1] * q[1]) + R[3 * r2 + 2] * q[2]) +
+                           t[r2];                                     // translateMesh
+      }
+      if ((st = esf_solid_set_mesh(ctx, id, (int)(mrest.size() / 3), mrest.data(), mv.data(),
+                                   (int)(mtri.size() / 3), mtri.data()))) {
+        esf_ctx_destroy(ctx);
+        return st;
+      }
+    }
   }
   // COUPLED_INTEGRATOR_3D::init  COUPLED_INTEGRATOR_3D.cpp:104-135 with
   // DEFOR_SOLID_3D::init (:269-331): full-grid boxes, mass, boxes from mass
@@ -352,12 +435,15 @@ extern "C" esf_status esf_scene_load_cfg(const char* path, int precision, esf_ct
   return ESF_OK;
 }
 
-// broad phase stand-in: the reference tests surface-mesh vertices against
-// [origin + 2dh, origin + (res-3)dh] (DEFOR_SOLID_3D.cpp:64-66, .h:85) and
-// solid pairs for BVH overlap (.h:84), then takes CFL(0.6) or CFL(1.8)
-// (COUPLED_INTEGRATOR_3D.cpp:386-410).  The meshes stay outside the drop-in,
-// so the solid-mass bounding box (tight box less its border) stands in.
-static double broadPhaseAlpha(esf_ctx* ctx) {
+// The broad phase of stepImplicit (COUPLED_INTEGRATOR_3D.cpp:384-411): a
+// solid within 2 cells of a wall (its surface mesh's BVH root box outside
+// [origin + 2dh, origin + (res - 3)dh], DEFOR_SOLID_3D.h:85 and
+// DEFOR_SOLID_3D.cpp:64-66) or two solids whose meshes collide
+// (DEFOR_SOLID_3D.h:84) give CFL(0.6), otherwise CFL(1.8).  When every
+// solid has a surface mesh the test runs on the meshes (on the GPU,
+// esf_mesh.hip); otherwise each solid's mass bounding box (tight box less
+// its border) stands in for its mesh.
+static double boxBroadPhase(esf_ctx* ctx) {
   const int res[3] = {ctx->d.X, ctx->d.Y, ctx->d.Z};
   const int border = ctx->p.tight_border;
   std::vector<std::array<int, 6>> bb;
@@ -366,7 +452,6 @@ static double broadPhaseAlpha(esf_ctx* ctx) {
     for (int k = 0; k < 3; k++) {
       m[k] = so.box.tMin[k] + border;
       m[3 + k] = so.box.tMax[k] - border;
-      // the solid is within 2 cells of a wall (safe box :64-66)
       if (so.box.tMin[k] == 0 || m[k] < 2 || so.box.tMax[k] == res[k] || m[3 + k] > res[k] - 3)
         return 0.6;
     }
@@ -381,6 +466,35 @@ static double broadPhaseAlpha(esf_ctx* ctx) {
   return 1.8;
 }
 
+extern "C" esf_status esf_broad_phase(esf_ctx* ctx, double* alpha) {
+  if (!ctx || !alpha) return ESF_ERR_INVALID;
+  bool meshes = !ctx->solids.empty();
+  for (const DevSolid& so : ctx->solids) meshes = meshes && so.meshV > 0;
+  if (!meshes) {
+    *alpha = boxBroadPhase(ctx);
+    return ESF_OK;
+  }
+  std::vector<double> roots;
+  std::vector<int> pairs;
+  esf_status st = meshBroadPhase(ctx, roots, pairs);
+  if (st) return st;
+  const double o[3] = {ctx->d.o0, ctx->d.o1, ctx->d.o2};
+  const int res[3] = {ctx->d.X, ctx->d.Y, ctx->d.Z};
+  bool contact = false;
+  for (size_t s = 0; s < ctx->solids.size() && !contact; s++) {
+    // _safeFromWallBox.inside(root min) && inside(root max) (aabb.cpp:135-142)
+    for (int k = 0; k < 3; k++) {
+      const double lo = o[k] + 2.0 * ctx->d.dh;
+      const double hi = o[k] + (res[k] - 3) * ctx->d.dh;
+      const double mn = roots[6 * s + k], mx = roots[6 * s + 3 + k];
+      if (mn < lo || mn > hi || mx < lo || mx > hi) contact = true;
+    }
+  }
+  for (int f : pairs) contact = contact || f != 0;
+  *alpha = contact ? 0.6 : 1.8;
+  return ESF_OK;
+}
+
 extern "C" esf_status esf_frame(esf_ctx* ctx, double frameDt, double alpha, int* substeps) {
   if (!ctx || !(frameDt > 0)) return ESF_ERR_INVALID;
   // APPLICATION::step  rubberBall/main.cpp:180-198
@@ -390,7 +504,12 @@ extern "C" esf_status esf_frame(esf_ctx* ctx, double frameDt, double alpha, int*
   while (sim < frameDt) {
     esf_set_max_dt(ctx, frameDt - sim);
     esf_solve_stats s;
-    esf_status st = esf_step(ctx, alpha < 0 ? broadPhaseAlpha(ctx) : alpha, &s);
+    double a = alpha;
+    if (a < 0) {
+      esf_status bs = esf_broad_phase(ctx, &a);
+      if (bs) return bs;
+    }
+    esf_status st = esf_step(ctx, a, &s);
     if (st < 0) return st;
     if (st) worst = st;
     double dt = 0;

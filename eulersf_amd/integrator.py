@@ -205,6 +205,39 @@ class CoupledIntegrator3D:
         return n.value
 
     # -- operators ----------------------------------------------------------
+    def setSurfaceMesh(self, solid, rest, vertices, triangles):
+        """DEFOR_SOLID_3D::loadSurfaceMesh with the placed vertices"""
+        r = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1)
+        t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1)
+        check(self.ctx, self.lib.esf_solid_set_mesh(self.ctx, solid, r.size // 3, _abi.dptr(r),
+                                                    _abi.dptr(v), t.size // 3,
+                                                    t.ctypes.data_as(_abi._I)), "setSurfaceMesh")
+
+    def surfaceMesh(self, solid=0):
+        """(vertices (n, 3), rest pose (n, 3), triangles (m, 3)) or None"""
+        nv, nt = ctypes.c_int(), ctypes.c_int()
+        check(self.ctx, self.lib.esf_solid_mesh_size(self.ctx, solid, ctypes.byref(nv),
+                                                     ctypes.byref(nt)), "surfaceMesh")
+        if nv.value == 0:
+            return None
+        v = np.empty(3 * nv.value)
+        r = np.empty(3 * nv.value)
+        t = np.empty(3 * nt.value, dtype=np.int32)
+        check(self.ctx, self.lib.esf_solid_get_mesh(self.ctx, solid, _abi.dptr(v), _abi.dptr(r),
+                                                    t.ctypes.data_as(_abi._I)), "surfaceMesh")
+        return v.reshape(-1, 3), r.reshape(-1, 3), t.reshape(-1, 3)
+
+    def advectSurfaceMesh(self):
+        """DEFOR_SOLID_3D::advectSurfaceMesh for every solid with a mesh"""
+        check(self.ctx, self.lib.esf_advect_surface_mesh(self.ctx), "advectSurfaceMesh")
+
+    def broadPhase(self):
+        """CFL alpha of the stepImplicit broad phase: 0.6 or 1.8"""
+        a = ctypes.c_double()
+        check(self.ctx, self.lib.esf_broad_phase(self.ctx, ctypes.byref(a)), "broadPhase")
+        return a.value
+
     def writeFrame(self, data_path, frame, debug=False):
         """COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011)"""
         check(self.ctx, self.lib.esf_write_frame(self.ctx, data_path.encode(), frame,

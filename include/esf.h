@@ -156,6 +156,21 @@ esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu,
 esf_status esf_solid_set_boxes(esf_ctx* ctx, int solid, const int boxes[12]);
 esf_status esf_solid_get_boxes(esf_ctx* ctx, int solid, int boxes[12]);
 
+/* Surface mesh of a solid (DEFOR_SOLID_3D::loadSurfaceMesh,
+ * DEFOR_SOLID_3D.cpp:157-164): nv vertices as the rest pose (the OBJ's own
+ * coordinates, OBJ.cpp:255) and as placed in the scene (scaled, rotated,
+ * translated, rubberBall/main.cpp:148-152), 3 doubles each; nt triangles,
+ * 3 zero-based vertex indices each (OBJ.cpp:256-277).  nv = 0 removes the
+ * mesh.  esf_step advects the mesh of every solid that has one. */
+esf_status esf_solid_set_mesh(esf_ctx* ctx, int solid, int nv, const double* rest,
+                              const double* vertices, int nt, const int* tris);
+/* vertex and triangle counts of a solid's mesh (0, 0 without one) */
+esf_status esf_solid_mesh_size(esf_ctx* ctx, int solid, int* nv, int* nt);
+/* current vertices, rest pose and triangles of a solid's mesh; any output
+ * may be NULL.  ESF_ERR_STATE when the solid has no mesh. */
+esf_status esf_solid_get_mesh(esf_ctx* ctx, int solid, double* vertices,
+                              double* rest, int* tris);
+
 /* --- host <-> device ----------------------------------------------------*/
 esf_status esf_upload(esf_ctx* ctx, int field, int solid, const void* src,
                       size_t bytes);
@@ -190,6 +205,18 @@ esf_status esf_solid_finalize(esf_ctx* ctx);
 esf_status esf_advect_velocity(esf_ctx* ctx);
 /* advectField(_intensity) (COUPLED_INTEGRATOR_3D.cpp:766-796) */
 esf_status esf_advect_intensity(esf_ctx* ctx);
+/* DEFOR_SOLID_3D::advectSurfaceMesh (DEFOR_SOLID_3D.cpp:1125-1180) for every
+ * solid with a mesh, one GPU thread per vertex: move with the solid
+ * velocity for dt, then up to 10 Newton steps towards x - u(x) = rest,
+ * clamped into the domain */
+esf_status esf_advect_surface_mesh(esf_ctx* ctx);
+/* The broad phase of stepImplicit (COUPLED_INTEGRATOR_3D.cpp:384-411): *alpha
+ * = 0.6 when a solid is within 2 cells of a wall (DEFOR_SOLID_3D.h:85) or
+ * two solids collide (DEFOR_SOLID_3D.h:84, TRIANGLE_BVH.cpp:179-192),
+ * else 1.8.  With a mesh on every solid the test runs on the meshes' triangle
+ * boxes (on the GPU; a BVH collision is exactly a pair of overlapping
+ * triangle boxes); otherwise each solid's mass bounding box stands in. */
+esf_status esf_broad_phase(esf_ctx* ctx, double* alpha);
 /* recoverVelocityField + Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:894-916) */
 esf_status esf_recover_and_bc(esf_ctx* ctx);
 /* One substep of the drop-in: CFL(alpha) -> forces -> coupled solve ->
@@ -303,19 +330,32 @@ esf_status esf_field_write_gz(const char* path, const esf_grid* grid,
  * Host only. */
 esf_status esf_field_read_gz(const char* path, esf_grid* grid, int components,
                              double* data, size_t capacity);
+/* OBJ::Load (src/geometry/OBJ.cpp:97-295) as the solver uses it: "v" lines,
+ * and one triangle per "f" line from its first three vertex indices.  Counts
+ * into *nv, *nt; vertices (3 per vertex) and tris (3 zero-based indices per
+ * triangle) are filled when not NULL and cap_v >= nv, cap_t >= nt.
+ * ESF_ERR_IO for a missing or malformed file.  Host only. */
+esf_status esf_obj_read(const char* path, int* nv, int* nt, double* vertices,
+                        int cap_v, int* tris, int cap_t);
+/* OBJ::Save (OBJ.cpp:336-375): "v x y z" lines (17 significant digits
+ * where the reference streams 6) and "f a b c" lines.  Host only. */
+esf_status esf_obj_write(const char* path, int nv, const double* vertices,
+                         int nt, const int* tris);
 /* COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011) with
  * DEFOR_SOLID_3D::writeFrame (DEFOR_SOLID_3D.cpp:1361-1371): under
  * "<data_path>frame.NNNN" (at least four digits, IO.h:27-37) the intensity
  * every frame; the grid velocity and, per solid i, ".solid.i.displacement"
- * and ".solid.i.velocity" when debug or frame % 10 == 0.  The solid SDF
- * (".phi") and surface mesh (".obj", ".pbrt") are not kept by this library
- * and are not written. */
+ * and ".solid.i.velocity" when debug or frame % 10 == 0, and every frame
+ * ".solid.i.obj" for a solid with a surface mesh.  The solid SDF (".phi")
+ * and the ".pbrt" render file are not kept by this library and are not
+ * written. */
 esf_status esf_write_frame(esf_ctx* ctx, const char* data_path, int frame,
                            int debug);
 /* COUPLED_INTEGRATOR_3D::readFrame (COUPLED_INTEGRATOR_3D.cpp:976-997) with
- * DEFOR_SOLID_3D::readFrame (DEFOR_SOLID_3D.cpp:1299-1314): the same files
- * back (a grid that differs from the context's is ESF_ERR_INVALID), each
- * solid's velocity workspace set from its velocity and X = x - u, then
+ * DEFOR_SOLID_3D::readFrame (DEFOR_SOLID_3D.cpp:1299-1317): the same files
+ * back (a grid that differs from the context's, or a mesh of other
+ * topology, is ESF_ERR_INVALID), each solid's velocity workspace set from
+ * its velocity and X = x - u, mesh vertices from ".solid.i.obj", then
  * gatherSolidMasses and recoverVelocityField. */
 esf_status esf_read_frame(esf_ctx* ctx, const char* data_path, int frame,
                           int debug);

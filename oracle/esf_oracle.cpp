@@ -19,6 +19,7 @@
 #include "esf_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -1755,6 +1756,168 @@ void orc_interp_vec3(const double* f, int X, int Y, int Z, double dh,
   Geom g{X, Y, Z, dh, {o[0], o[1], o[2]}};
   interpV(f, g, p, out);
 }
+// DEFOR_SOLID_3D::advectSurfaceMesh (DEFOR_SOLID_3D.cpp:1125-1180) on one
+// solid's fields (velocity and displacement, AoS over the X*Y*Z grid):
+// every vertex moves with the interpolated velocity for dt, then takes up to
+// 10 Newton steps vertex -= F (vertex - u(vertex) - rest) with F from the
+// computeF of the cell holding it (:671-682), then is clamped into the box.
+void orc_advect_surface_mesh(int X, int Y, int Z, double dh, const double o[3],
+                             const double* vel, const double* u, double dt, int nv,
+                             const double* rest, double* vert) {
+  Geom g{X, Y, Z, dh, {o[0], o[1], o[2]}};
+  const long S = (long)X * Y;
+  const long off[8] = {0, 1, X, X + 1, S, S + 1, S + X, S + X + 1};
+  const double gg = 0.25 * (1.0 / dh);
+  for (int x = 0; x < nv; x++) {
+    double p[3] = {vert[3 * x], vert[3 * x + 1], vert[3 * x + 2]};
+    double v[3];
+    interpV(vel, g, p, v);
+    for (int c = 0; c < 3; c++) p[c] += v[c] * dt;
+    double residual = 1;
+    int count = 0;
+    while (residual > 1e-4 && count++ < 10) {
+      double uu[3], prest[3], diff[3];
+      interpV(u, g, p, uu);
+      for (int c = 0; c < 3; c++) prest[c] = p[c] - uu[c];
+      for (int c = 0; c < 3; c++) diff[c] = prest[c] - rest[3 * x + c];
+      residual = std::sqrt(diff[0] * diff[0] + (diff[1] * diff[1] + diff[2] * diff[2]));
+      int i = (int)((p[0] - o[0]) / dh);
+      if (i < 0) i = 0;
+      if (i > X - 2) i = X - 2;
+      int j = (int)((p[1] - o[1]) / dh);
+      if (j < 0) j = 0;
+      if (j > Y - 2) j = Y - 2;
+      int k = (int)((p[2] - o[2]) / dh);
+      if (k < 0) k = 0;
+      if (k > Z - 2) k = Z - 2;
+      const long index = (long)k * S + (long)j * X + i;
+      double Finv[9], F[9];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          double acc = 0;
+          for (int q = 0; q < 8; q++) {
+            int sg = c == 0 ? sgnX(q) : (c == 1 ? sgnY(q) : sgnZ(q));
+            double t = u[3 * (index + off[q]) + r] * (sg * gg);
+            acc = (q == 0) ? t : acc + t;
+          }
+          Finv[3 * r + c] = ((r == c) ? 1.0 : 0.0) - acc;
+        }
+      if (std::fabs(det3(Finv)) > 1e-4) {
+        inv3(Finv, F);
+      } else {
+        for (int q = 0; q < 9; q++) F[q] = (q % 4 == 0) ? 1.0 : 0.0;
+      }
+      double Fd[3];
+      for (int r = 0; r < 3; r++) Fd[r] = (F[3 * r] * diff[0] + F[3 * r + 1] * diff[1]) + F[3 * r + 2] * diff[2];
+      for (int r = 0; r < 3; r++) p[r] -= Fd[r];
+    }
+    const int res[3] = {X, Y, Z};
+    for (int c = 0; c < 3; c++) {
+      p[c] = std::max(o[c], p[c]);
+      p[c] = std::min(o[c] + dh * (res[c] - 1), p[c]);
+      vert[3 * x + c] = p[c];
+    }
+  }
+}
+
+// TRIANGLE_BVH (src/geometry/TRIANGLE_BVH.cpp): a bounding-volume tree over
+// triangle boxes (leaf: the triangle's vertex bounds, inner: the union of
+// its children, :163-177), split at the median of the triangle centres
+// along the widest axis of the node, and the reference's recursive collide
+// (:179-192) with aabb::overlaps (aabb.cpp:121-133).  The split rule is this
+// checker's own (the reference splits at an axis-aligned plane, aap.cpp):
+// collide does not depend on it, which is what the GPU's all-pairs test
+// relies on.
+struct BvhNode {
+  double box[6];
+  int left = -1, right = -1;
+};
+static bool boxOverlap(const double* a, const double* b) {
+  if (a[0] > b[3] || a[1] > b[4] || a[2] > b[5]) return false;
+  if (a[3] < b[0] || a[4] < b[1] || a[5] < b[2]) return false;
+  return true;
+}
+static int bvhBuild(std::vector<BvhNode>& nodes, const std::vector<std::array<double, 6>>& tb,
+                    std::vector<int>& ids, int lo, int hi) {
+  BvhNode n;
+  for (int c = 0; c < 3; c++) {
+    n.box[c] = 1e300;
+    n.box[3 + c] = -1e300;
+  }
+  for (int k = lo; k < hi; k++)
+    for (int c = 0; c < 3; c++) {
+      n.box[c] = std::min(n.box[c], tb[ids[k]][c]);
+      n.box[3 + c] = std::max(n.box[3 + c], tb[ids[k]][3 + c]);
+    }
+  const int me = (int)nodes.size();
+  nodes.push_back(n);
+  if (hi - lo > 1) {
+    int axis = 0;
+    for (int c = 1; c < 3; c++)
+      if (n.box[3 + c] - n.box[c] > n.box[3 + axis] - n.box[axis]) axis = c;
+    const int mid = (lo + hi) / 2;
+    std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](int a, int b) {
+      return tb[a][axis] + tb[a][3 + axis] < tb[b][axis] + tb[b][3 + axis];
+    });
+    const int l = bvhBuild(nodes, tb, ids, lo, mid);
+    const int r = bvhBuild(nodes, tb, ids, mid, hi);
+    nodes[me].left = l;
+    nodes[me].right = r;
+  }
+  return me;
+}
+static std::vector<BvhNode> bvhOf(int nt, const int* tri, const double* v) {
+  std::vector<std::array<double, 6>> tb(nt);
+  for (int t = 0; t < nt; t++)
+    for (int c = 0; c < 3; c++) {
+      tb[t][c] = tb[t][3 + c] = v[3 * tri[3 * t] + c];
+      for (int y = 1; y < 3; y++) {
+        tb[t][c] = std::min(tb[t][c], v[3 * tri[3 * t + y] + c]);
+        tb[t][3 + c] = std::max(tb[t][3 + c], v[3 * tri[3 * t + y] + c]);
+      }
+    }
+  std::vector<int> ids(nt);
+  for (int t = 0; t < nt; t++) ids[t] = t;
+  std::vector<BvhNode> nodes;
+  if (nt > 0) bvhBuild(nodes, tb, ids, 0, nt);
+  return nodes;
+}
+static bool bvhCollide(const std::vector<BvhNode>& A, int a, const std::vector<BvhNode>& B, int b) {
+  const BvhNode& x = A[a];
+  const BvhNode& y = B[b];
+  if (!boxOverlap(x.box, y.box)) return false;
+  if (x.left < 0 && y.left < 0) return true;
+  if (x.left < 0) return bvhCollide(A, a, B, y.left) || bvhCollide(A, a, B, y.right);
+  if (y.left >= 0)
+    return bvhCollide(A, x.left, B, y.left) || bvhCollide(A, x.left, B, y.right) ||
+           bvhCollide(A, x.right, B, y.left) || bvhCollide(A, x.right, B, y.right);
+  return bvhCollide(A, x.left, B, b) || bvhCollide(A, x.right, B, b);
+}
+
+// the broad phase of stepImplicit (COUPLED_INTEGRATOR_3D.cpp:384-411) on n
+// solid meshes (mesh s: nv[s] vertices at v + vo[s] (3 doubles each), nt[s]
+// triangles at tri + to[s]): 0.6 when a root box leaves the wall-safe box
+// [o + 2dh, o + (res - 3)dh] (DEFOR_SOLID_3D.cpp:64-66) or two trees
+// collide, else 1.8
+double orc_mesh_broad_phase(int n, const int* nt, const long* vo, const long* to,
+                            const double* v, const int* tri, int X, int Y, int Z, double dh,
+                            const double o[3]) {
+  std::vector<std::vector<BvhNode>> trees;
+  for (int s = 0; s < n; s++) trees.push_back(bvhOf(nt[s], tri + to[s], v + vo[s]));
+  const int res[3] = {X, Y, Z};
+  for (int s = 0; s < n; s++) {
+    const double* b = trees[s][0].box;
+    for (int k = 0; k < 3; k++) {
+      const double lo = o[k] + 2.0 * dh, hi = o[k] + (res[k] - 3) * dh;
+      if (b[k] < lo || b[k] > hi || b[3 + k] < lo || b[3 + k] > hi) return 0.6;
+    }
+  }
+  for (int a = 0; a + 1 < n; a++)
+    for (int b = a + 1; b < n; b++)
+      if (bvhCollide(trees[a], 0, trees[b], 0)) return 0.6;
+  return 1.8;
+}
+
 double orc_last_stage_seconds(void* h, int stage) {
   Scene& s = *static_cast<Scene*>(h);
   if (stage < 0 || stage >= 16) return 0;

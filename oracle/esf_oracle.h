@@ -142,6 +142,12 @@ void orc_corotation(double lambda, double mu, const double F[9],
                     const double dF[9], double out[36], double* energy,
                     double dP_out[9]);
 /* FIELD_3D::interpolate_value over an arbitrary scalar / AoS field */
+void orc_advect_surface_mesh(int X, int Y, int Z, double dh, const double o[3],
+                             const double* vel, const double* u, double dt, int nv,
+                             const double* rest, double* vert);
+double orc_mesh_broad_phase(int n, const int* nt, const long* vo, const long* to,
+                            const double* v, const int* tri, int X, int Y, int Z, double dh,
+                            const double o[3]);
 double orc_interp_scalar(const double* f, int X, int Y, int Z, double dh,
                          const double o[3], const double p[3]);
 void orc_interp_vec3(const double* f, int X, int Y, int Z, double dh,

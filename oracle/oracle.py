@@ -92,6 +92,13 @@ def lib():
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_sphere_rest_dims.argtypes = [ctypes.c_double, ctypes.c_double, _I, _D]
         L.orc_sphere_rest_fill.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]
+        L.orc_advect_surface_mesh.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, _D, _D, _D, ctypes.c_double,
+                                              ctypes.c_int, _D, _D]
+        L.orc_mesh_broad_phase.restype = ctypes.c_double
+        L.orc_mesh_broad_phase.argtypes = [ctypes.c_int, _I, ctypes.POINTER(ctypes.c_long),
+                                           ctypes.POINTER(ctypes.c_long), _D, _I, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_double, _D]
         _lib = L
     return _lib
 
@@ -280,3 +287,35 @@ class OracleScene:
         y = np.zeros_like(x)
         self.L.orc_solid_apply(self.h, dp(x), dp(y))
         return y
+
+
+def advect_surface_mesh(grid, vel, u, dt, rest, vertices):
+    """DEFOR_SOLID_3D::advectSurfaceMesh on one solid's fields; returns the
+    new vertices.  grid = (X, Y, Z, dh, origin)."""
+    X, Y, Z, dh, o = grid
+    oo = np.array(o, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    r = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1)
+    v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1).copy()
+    lib().orc_advect_surface_mesh(X, Y, Z, dh, dp(oo), dp(vel), dp(u), dt, v.size // 3, dp(r),
+                                  dp(v))
+    return v.reshape(-1, 3)
+
+
+def mesh_broad_phase(grid, meshes):
+    """the stepImplicit broad phase over [(vertices, triangles), ...]: the
+    CFL alpha, 0.6 or 1.8 (BVH collide, TRIANGLE_BVH.cpp:179-192)"""
+    X, Y, Z, dh, o = grid
+    oo = np.array(o, dtype=np.float64)
+    vs = [np.ascontiguousarray(v, dtype=np.float64).reshape(-1) for v, _ in meshes]
+    ts = [np.ascontiguousarray(t, dtype=np.int32).reshape(-1) for _, t in meshes]
+    nt = np.array([t.size // 3 for t in ts], dtype=np.int32)
+    vo = np.cumsum([0] + [v.size for v in vs])[:-1].astype(np.int64)
+    to = np.cumsum([0] + [t.size for t in ts])[:-1].astype(np.int64)
+    v = np.concatenate(vs)
+    t = np.concatenate(ts).astype(np.int32)
+    return lib().orc_mesh_broad_phase(
+        len(meshes), nt.ctypes.data_as(_I), vo.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+        to.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), dp(v), t.ctypes.data_as(_I), X, Y, Z,
+        dh, dp(oo))

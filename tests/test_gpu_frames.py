@@ -35,9 +35,9 @@ def test_write_read_resume(esf, tmp_path):
     g.writeFrame(path, 10)
     g.writeFrame(path, 3)
     names = sorted(os.listdir(tmp_path))
-    assert names == ["frame.0003.intensity", "frame.0010.intensity",
-                     "frame.0010.solid.0.displacement", "frame.0010.solid.0.velocity",
-                     "frame.0010.velocity"]
+    assert names == ["frame.0003.intensity", "frame.0003.solid.0.obj", "frame.0010.intensity",
+                     "frame.0010.solid.0.displacement", "frame.0010.solid.0.obj",
+                     "frame.0010.solid.0.velocity", "frame.0010.velocity"]
     grid, v = A.field_read_gz(path + "frame.0010.velocity", 3)
     assert grid[:3] == (17, 17, 17) and grid[3] == 1.0 / 16
     np.testing.assert_array_equal(bits(v), bits(g.download(A.F_VELOCITY)))
