@@ -19,6 +19,7 @@
 #include "esf_krylov.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace esf {
 
@@ -662,15 +663,26 @@ static void fillNodeArgs(esf_ctx* ctx, SolidNodeArgs& a) {
 
 // the CR-update launches are kernel classes 14 (cells) and 15 (nodes) of
 // esf_kernel_timing
-template <int MODE>
-static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
-                     const SolidPartials& sp, const double* x = nullptr) {
+// lanes per cell of the cell pass: kCellLanes, or ESF_SOLID_LANES (2, 4
+// or 8), for tuning
+static int cellLanes() {
+  static const int lanes = [] {
+    const char* e = getenv("ESF_SOLID_LANES");
+    const int v = e ? atoi(e) : 0;
+    return v == 2 || v == 4 || v == 8 ? v : kCellLanes;
+  }();
+  return lanes;
+}
+
+template <int MODE, int LPC>
+static void cellPassLanes(esf_ctx* ctx, const double* in, const CrScalars* sc,
+                          const SolidPartials& sp, const double* x) {
   const int cls = MODE == CELL_CR ? 14 : (MODE == CELL_CRX ? 16 : -1);
   for (size_t b = 0; b < ctx->solids.size(); b++) {
     DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
-      launchTimed(ctx, cls, kSolidCell<MODE, kCellLanes>,
-                  dim3(cdiv((long)kCellLanes * so.nValid, kCellThreads)), dim3(kCellThreads),
+      launchTimed(ctx, cls, kSolidCell<MODE, LPC>,
+                  dim3(cdiv((long)LPC * so.nValid, kCellThreads)), dim3(kCellThreads),
                   ctx->d, so.nValid,
                   so.capValid,
                   (const int*)so.cornerDof, (const double*)so.cellMat, in,
@@ -679,6 +691,18 @@ static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
                   ctx->inboxX + 24L * ctx->capSolid * b);
     }
   }
+}
+
+template <int MODE>
+static void cellPass(esf_ctx* ctx, const double* in, const CrScalars* sc,
+                     const SolidPartials& sp, const double* x = nullptr) {
+  const int lanes = cellLanes();
+  if (lanes == 8)
+    cellPassLanes<MODE, 8>(ctx, in, sc, sp, x);
+  else if (lanes == 4)
+    cellPassLanes<MODE, 4>(ctx, in, sc, sp, x);
+  else
+    cellPassLanes<MODE, 2>(ctx, in, sc, sp, x);
 }
 
 static int nodeBlocks(const esf_ctx* ctx) {
