@@ -17,7 +17,8 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
                    bool recursive = false);
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
                       double* relres, int* refinements, int* innerIters);
-int spmvBlocks(const Dims& d);
+int spmvBlocks(const esf_ctx* ctx);
+void spmvConfigure(esf_ctx* ctx);
 esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs);
 esf_status benchCopy(esf_ctx* ctx, long bytes, int reps, double* avgMs);
 // esf_solid.hip
@@ -233,7 +234,8 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = dalloc(ctx, &ctx->scanBlocks, scanBlocks);
   // up to 5 partials per block of the widest reducing launch (the pressure
   // CR update over every SpMV tile, or 128-thread blocks over every node)
-  ctx->nPartials = std::max<int>(5 * std::max(spmvBlocks(d), cdiv(N, 128)) + 64, 1 << 16);
+  spmvConfigure(ctx);
+  ctx->nPartials = std::max<int>(5 * std::max(spmvBlocks(ctx), cdiv(N, 128)) + 64, 1 << 16);
   if (!st) st = dalloc(ctx, &ctx->partials, ctx->nPartials);
   if (!st) st = dalloc(ctx, &ctx->sc, 8);
   if (!st) st = dalloc(ctx, &ctx->dmisc, 16);

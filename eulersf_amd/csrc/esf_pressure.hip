@@ -462,49 +462,48 @@ __global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// How many iterations ahead the z-march loads (1 or 2): two sets of loads
-// in flight hide more latency but hold more registers.  ESF_SPMV_DEPTH
-// overrides the choice, for tuning.
-static int spmvDepth(int mode) {
-  static const int forced = [] {
-    const char* e = getenv("ESF_SPMV_DEPTH");
-    const int v = e ? atoi(e) : 0;
-    return v >= 1 && v <= 3 ? v : 0;
-  }();
-  if (forced) return forced;
-  return mode == M_UPDATE ? 1 : 2;
+// The z-march's launch shape, fixed per context when it is created:
+//  - cell layers per block: each block loads two layers of x and forms one
+//    node layer beyond the ones it outputs, so deeper blocks waste less;
+//    16 measured best at 128^3 and 256^3 (MI355X).  Small grids take
+//    shallower blocks so that at least 512 blocks run where the grid has
+//    them.  ESF_SPMV_LAYERS (4..64) overrides the choice;
+//  - how many iterations ahead the loads run (1..3): two sets of loads in
+//    flight hide more latency but hold more registers, so the CR update
+//    (more operands) keeps one.  ESF_SPMV_DEPTH (1..3) overrides both.
+// The overrides are for tuning and tests; they are read at context creation.
+static int envInt(const char* name, int lo, int hi) {
+  const char* e = getenv(name);
+  const int v = e ? atoi(e) : 0;
+  return v >= lo && v <= hi ? v : 0;
 }
 
-// Cell layers per block of the z-march.  Each block loads two layers of x
-// and forms one node layer beyond the ones it outputs, so deeper blocks
-// waste less; 16 measured best at 128^3 and 256^3 (MI355X).  Small grids
-// take shallower blocks so that at least 512 blocks run where the grid has
-// them.  ESF_SPMV_LAYERS (4..64) overrides the choice, for tuning.
-static int spmvLayers(const Dims& d, int tilesXY) {
-  static const int forced = [] {
-    const char* e = getenv("ESF_SPMV_LAYERS");
-    const int v = e ? atoi(e) : 0;
-    return v >= 4 && v <= 64 ? v : 0;
-  }();
-  if (forced) return forced;
+void spmvConfigure(esf_ctx* ctx) {
+  const Dims& d = ctx->d;
+  const long tilesXY = (long)cdiv(d.CX, TX) * cdiv(d.CY, TY);
   int layers = 16;
-  while (layers > 4 && (long)tilesXY * cdiv(d.CZ, layers) < 512) layers /= 2;
-  return layers;
+  while (layers > 4 && tilesXY * cdiv(d.CZ, layers) < 512) layers /= 2;
+  const int forcedLayers = envInt("ESF_SPMV_LAYERS", 4, 64);
+  ctx->spmvLayers = forcedLayers ? forcedLayers : layers;
+  const int forcedDepth = envInt("ESF_SPMV_DEPTH", 1, 3);
+  ctx->spmvDepthUpdate = forcedDepth ? forcedDepth : 1;
+  ctx->spmvDepthOther = forcedDepth ? forcedDepth : 2;
 }
 
-static void tileShape(const Dims& d, int& tx, int& ty, int& tz, int& layers) {
+static void tileShape(const esf_ctx* ctx, int& tx, int& ty, int& tz, int& layers) {
+  const Dims& d = ctx->d;
   tx = cdiv(d.CX, TX);
   ty = cdiv(d.CY, TY);
-  layers = spmvLayers(d, tx * ty);
+  layers = ctx->spmvLayers;
   tz = cdiv(d.CZ, layers);
 }
 
 template <typename T, int MODE>
 esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   int tx, ty, tz, layers;
-  tileShape(ctx->d, tx, ty, tz, layers);
+  tileShape(ctx, tx, ty, tz, layers);
   const int nb = tx * ty * tz;
-  const int depth = spmvDepth(MODE);
+  const int depth = MODE == M_UPDATE ? ctx->spmvDepthUpdate : ctx->spmvDepthOther;
   if (depth == 3)
     launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 3>, dim3(nb), dim3(256),
                 ctx->d, a, tx, ty, tz, layers);
@@ -530,9 +529,9 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   return ESF_OK;
 }
 
-int spmvBlocks(const Dims& d) {
+int spmvBlocks(const esf_ctx* ctx) {
   int tx, ty, tz, layers;
-  tileShape(d, tx, ty, tz, layers);
+  tileShape(ctx, tx, ty, tz, layers);
   return tx * ty * tz;
 }
 

@@ -103,6 +103,29 @@ def test_pressure_operator_uneven_tiles(esf, n):
         np.testing.assert_array_equal(g.getMatVecMult(x), o.pressure_apply(x))
 
 
+@pytest.mark.parametrize("layers,depth", [(4, 1), (8, 2), (16, 2), (16, 3), (16, 1), (24, 2)])
+def test_pressure_launch_shapes(esf, monkeypatch, layers, depth):
+    """every z-march shape the tuning knobs select (cell layers per block,
+    load sets in flight; read when a context is created): S x stays bitwise
+    the reference's order, and the coupled solve, whose CR update and
+    checks run in the same shape, stays within 1e-10 of the oracle.  24
+    cell layers leave a partial block at 16."""
+    monkeypatch.setenv("ESF_SPMV_LAYERS", str(layers))
+    monkeypatch.setenv("ESF_SPMV_DEPTH", str(depth))
+    g, o = make_pair(esf, 24, velocity=(0.3, -1.0, 0.2), **TIGHT)
+    g.CFL(1.8)
+    o.L.orc_cfl(o.h, 1.8)
+    g.forces()
+    o.L.orc_forces(o.h)
+    o.L.orc_assemble(o.h)
+    x = np.random.default_rng(layers + depth).standard_normal(g.cells)
+    np.testing.assert_array_equal(g.getMatVecMult(x), o.pressure_apply(x))
+    gs = g.pcrImplicitSolve()
+    o.L.orc_solve(o.h, O.Stats())
+    assert gs["status"] == 0
+    assert rel_err(g.download(esf._abi.F_VELOCITY), o.field(O.VELOCITY)) < 1e-10
+
+
 def test_coupled_solve_matches_oracle(esf):
     g, o = make_pair(esf, 16, velocity=(0.3, -1.0, 0.2), **TIGHT)
     g.CFL(1.8)
