@@ -12,7 +12,8 @@ namespace esf {
 esf_status pressureRhs(esf_ctx* ctx);
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
 template <typename T>
-esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D, T* x, T* r,
+esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
+                   const T* Dinv, T* x, T* r,
                    T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt, int slot,
                    bool recursive = false);
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
@@ -111,28 +112,10 @@ esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need) {
 }
 
 static void freeAll(esf_ctx* ctx) {
-  void* ptrs[] = {ctx->vel,    ctx->velws,  ctx->vort,    ctx->intensity, ctx->intws,
-                  ctx->ws,     ctx->solidMass, ctx->nodeFrac, ctx->qfracSum, ctx->sidx,
-                  ctx->solidNodes, ctx->scanBlocks, ctx->v0, ctx->minv, ctx->minv32,
-                  ctx->c,      ctx->b,      ctx->D,       ctx->lambda,    ctx->lambda2,
-                  ctx->r,      ctx->r2,     ctx->pdir,    ctx->s,         ctx->Ap,
-                  ctx->D32,    ctx->e32,    ctx->r32,     ctx->r32b,      ctx->p32,
-                  ctx->s32,    ctx->Ap32,   ctx->rhs32,   ctx->sForce,    ctx->sMv,
-                  ctx->sDamped, ctx->sKdiag, ctx->sAdiag, ctx->sRhs,      ctx->sV0,
-                  ctx->sDv,    ctx->sB,     ctx->sr,      ctx->sp,        ctx->ss,
-                  ctx->sAp,    ctx->stmp,   ctx->partials, ctx->sc,
-                  ctx->dmisc,  ctx->nodeMask, ctx->sPartials, ctx->inbox,
-                  ctx->sSd,    ctx->sApd,   ctx->s2,      ctx->Ap2,       ctx->s32b,
-                  ctx->Ap32b,  ctx->sXb,    ctx->inboxX};
-  for (void* p : ptrs)
-    if (p) hipFree(p);
-  for (auto& so : ctx->solids) {
-    void* sp[] = {so.rest, so.u,       so.unew,     so.X,       so.vel,     so.velws,
-                  so.mass, so.detFInv, so.qfrac,    so.valid,   so.cellList, so.cellSlot,
-                  so.cellMat, so.cellOut, so.cellDiag, so.cornerDof};
-    for (void* p : sp)
-      if (p) hipFree(p);
-  }
+  // every device buffer of the context went through dalloc
+  for (auto& kv : ctx->allocs) hipFree(kv.first);
+  ctx->allocs.clear();
+  ctx->deviceBytes = 0;
   if (ctx->hostFlags) hipHostFree(ctx->hostFlags);
   if (ctx->hostRing) hipHostFree(ctx->hostRing);
   if (ctx->hostSc) hipHostFree(ctx->hostSc);
@@ -219,12 +202,13 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
             {&ctx->D, C},        {&ctx->lambda, C},    {&ctx->lambda2, C},
             {&ctx->r, C},        {&ctx->r2, C},        {&ctx->pdir, C},
             {&ctx->s, C},        {&ctx->Ap, C},        {&ctx->s2, C},
-            {&ctx->Ap2, C}};
+            {&ctx->Ap2, C},      {&ctx->Dinv, C}};
   for (auto& e : dd)
     if (!st) st = dalloc(ctx, e.p, e.n);
   if (!st && mixed) {
     float** fs[] = {&ctx->D32, &ctx->e32, &ctx->r32,  &ctx->r32b, &ctx->p32,
-                    &ctx->s32, &ctx->s32b, &ctx->Ap32, &ctx->Ap32b, &ctx->rhs32};
+                    &ctx->s32, &ctx->s32b, &ctx->Ap32, &ctx->Ap32b, &ctx->rhs32,
+                    &ctx->Dinv32};
     for (float** f : fs)
       if (!st) st = dalloc(ctx, f, C);
     if (!st) st = dalloc(ctx, &ctx->minv32, N);
@@ -637,7 +621,7 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
                       &s->rel_residual[1], &s->refinements, &s->inner_iterations);
       s->converged[1] = conv;
     } else {
-      st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
+      st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->Dinv, ctx->lambda, ctx->r, ctx->r2,
                            ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, ctx->p.divfree_eps, maxIt, 1);
     }
     if (st) return st;
@@ -779,7 +763,7 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double e
   int it = 0;
   bool conv = false;
   double rr = 0;
-  esf_status st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r,
+  esf_status st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->Dinv, ctx->lambda, ctx->r,
                                   ctx->r2, ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 3);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(x, ctx->lambda, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -801,7 +785,7 @@ esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int maxIt, int*
   if (ctx->p.precision == ESF_MIXED) {
     st = mixedSolve(ctx, eps, maxIt, &it, &conv, &rr, &outer, &inner);
   } else {
-    st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->lambda, ctx->r, ctx->r2,
+    st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->Dinv, ctx->lambda, ctx->r, ctx->r2,
                          ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 3);
   }
   if (st) return st;
@@ -821,25 +805,28 @@ esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int maxIt, int*
 esf_status esf_corotation_batch(int n, double lambda, double mu, const double* F,
                                 const double* dF, double* out, double* energy, double* dP) {
   if (n <= 0 || !F || !dF || !out || !energy || !dP) return ESF_ERR_INVALID;
-  double *dFm, *ddF, *dout, *den, *ddP;
-  if (hipMalloc(&dFm, 72L * n) != hipSuccess) return ESF_ERR_OOM;
-  hipMalloc(&ddF, 72L * n);
-  hipMalloc(&dout, 288L * n);
-  hipMalloc(&den, 8L * n);
-  hipMalloc(&ddP, 72L * n);
-  hipMemcpy(dFm, F, 72L * n, hipMemcpyHostToDevice);
-  hipMemcpy(ddF, dF, 72L * n, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(kCorotBatch, dim3(cdiv(n, 128)), dim3(128), 0, 0, n, lambda, mu, dFm, ddF,
-                     dout, den, ddP);
-  hipError_t e = hipDeviceSynchronize();
-  hipMemcpy(out, dout, 288L * n, hipMemcpyDeviceToHost);
-  hipMemcpy(energy, den, 8L * n, hipMemcpyDeviceToHost);
-  hipMemcpy(dP, ddP, 72L * n, hipMemcpyDeviceToHost);
-  hipFree(dFm);
-  hipFree(ddF);
-  hipFree(dout);
-  hipFree(den);
-  hipFree(ddP);
+  double *dFm = nullptr, *ddF = nullptr, *dout = nullptr, *den = nullptr, *ddP = nullptr;
+  auto release = [&] {
+    for (double* q : {dFm, ddF, dout, den, ddP})
+      if (q) hipFree(q);
+  };
+  if (hipMalloc(&dFm, 72L * n) != hipSuccess || hipMalloc(&ddF, 72L * n) != hipSuccess ||
+      hipMalloc(&dout, 288L * n) != hipSuccess || hipMalloc(&den, 8L * n) != hipSuccess ||
+      hipMalloc(&ddP, 72L * n) != hipSuccess) {
+    release();
+    return ESF_ERR_OOM;
+  }
+  hipError_t e = hipMemcpy(dFm, F, 72L * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ddF, dF, 72L * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(kCorotBatch, dim3(cdiv(n, 128)), dim3(128), 0, 0, n, lambda, mu, dFm,
+                       ddF, dout, den, ddP);
+    e = hipDeviceSynchronize();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 288L * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(energy, den, 8L * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(dP, ddP, 72L * n, hipMemcpyDeviceToHost);
+  release();
   return e == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 

@@ -114,6 +114,8 @@ struct esf_ctx {
   // solve: cell-based (pressure)
   double *c = nullptr, *b = nullptr, *D = nullptr, *lambda = nullptr,
          *lambda2 = nullptr;
+  double* Dinv = nullptr;  // 1 / D (the CR update multiplies by it)
+  float* Dinv32 = nullptr;
   double *r = nullptr, *r2 = nullptr, *pdir = nullptr, *s = nullptr, *Ap = nullptr;
   double *s2 = nullptr, *Ap2 = nullptr;  // second buffers of s and Ap (CR loop)
   float *D32 = nullptr, *e32 = nullptr, *r32 = nullptr, *r32b = nullptr,

@@ -45,6 +45,7 @@ struct SpmvArgs {
   T* out;            // M_APPLY output
   const T* b;        // right-hand side
   const T* D;        // Jacobi diagonal
+  const T* Dinv;     // its reciprocals (M_UPDATE)
   T* x;              // solution (M_UPDATE, M_IR_RESID)
   T* r;              // residual
   T* p;              // direction
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
       if (MODE == M_UPDATE) {
         P.ap[h] = a.Ap[c];
         P.s[h] = a.s[c];
-        P.dd[h] = a.D[c];
+        P.dd[h] = a.Dinv[c];
       } else if (MODE == M_IR_RESID) {
         P.e[h] = a.e32 ? a.e32[c] : 0.0f;
       }
@@ -190,7 +191,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
           // r_new from r_old, Ap_old and s_old, on the halo as well (the
           // owner writes it with the cell's other updates)
           const T apn = P.ap[h] * beta + P.s[h];
-          const T qq = apn / P.dd[h];
+          const T qq = apn * P.dd[h];
           v = P.in[h] - alpha * qq;
         } else if (MODE == M_IR_RESID) {
           // x_old (+ e); the owner writes x_new to the other buffer
@@ -208,7 +209,8 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
   auto fetchOut = [&](Pre& P, int cz) {
     const long c = (long)min(max(cz, 0), zEnd - 1) * d.cslab + cOff;
     if (MODE == M_SETUP_R || MODE == M_CHECK || MODE == M_IR_RESID) P.eB = a.b[c];
-    if (MODE == M_SETUP_R || MODE == M_SETUP_S || MODE == M_UPDATE) P.eD = a.D[c];
+    if (MODE == M_SETUP_R || MODE == M_SETUP_S) P.eD = a.D[c];
+    if (MODE == M_UPDATE) P.eD = a.Dinv[c];
     if (MODE == M_UPDATE) {
       P.eAp = a.Ap[c];
       P.eS = a.s[c];
@@ -347,14 +349,18 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
             a.p[c] = pn;
             a.x[c] = P.eX + alpha * pn;
             a.sOut[c] = acc;
-            const T sq = acc / P.eD;
+            // P.eD holds 1 / D here
+            const T sq = acc * P.eD;
             part0 += (double)ownIn * (double)acc;
             part1 += (double)acc * (double)sq;
             part2 += (double)apn * (double)sq;
-            part3 += (double)apn * (double)(apn / P.eD);
-            // |b - S x| as the recursion carries it: r is D^-1 (b - S x)
-            const double rb = (double)P.eD * (double)ownIn;
-            part4 += rb * rb;
+            part3 += (double)apn * (double)(apn * P.eD);
+            // |b - S x| as the recursion carries it (r is D^-1 (b - S x));
+            // only the FP32 inner solve tests it
+            if (a.recursive) {
+              const double rb = (double)ownIn / (double)P.eD;
+              part4 += rb * rb;
+            }
           } else if (MODE == M_CHECK) {
             const T t = acc - P.eB;
             part0 += (double)t * (double)t;
@@ -426,7 +432,8 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
 __global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
                              const double* __restrict__ vel,
                              const double* __restrict__ minv, double* c,
-                             double* b, double* D, float* D32) {
+                             double* b, double* D, double* Dinv, float* D32,
+                             float* Dinv32) {
   const long r = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (r >= d.cells) return;
   const int cx = r % d.CX, cy = (r / d.CX) % d.CY, cz = r / d.cslab;
@@ -456,7 +463,11 @@ __global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
   c[r] = cc;
   b[r] = tmp - cc;
   D[r] = dd;
-  if (D32) D32[r] = (float)dd;
+  Dinv[r] = 1.0 / dd;
+  if (D32) {
+    D32[r] = (float)dd;
+    Dinv32[r] = (float)(1.0 / dd);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -539,7 +550,8 @@ esf_status pressureRhs(esf_ctx* ctx) {
   const Dims& d = ctx->d;
   hipLaunchKernelGGL(kPressureRhs, dim3(cdiv(d.cells, 256)), dim3(256), 0, ctx->stream, d,
                      ctx->v0, ctx->vel, ctx->minv, ctx->c, ctx->b, ctx->D,
-                     ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr);
+                     ctx->Dinv, ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr,
+                     ctx->Dinv32);
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
@@ -639,6 +651,9 @@ static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
 }
 
 // CONJUGATE_RESIDUAL::solvePCR (matrix-free) on S, FP64 or FP32 vectors.
+// The update multiplies by Dinv = 1 / D where the reference divides by D
+// (JACOBI_PRECONDITIONER.h:35-37): the iterates differ from the
+// reference's in rounding, as alpha's evaluation already makes them.
 // x is the start vector.  Scalar block `which`.  *iters receives the
 // reference's iteration count; *relres the last checked |Sx-b|/|b|.
 // recursive = false: the reference's test, the true residual every 5
@@ -648,8 +663,8 @@ static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
 // recursion carries, and no check SpMV runs.
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
-                   T* x, T* r, T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt,
-                   int slot, bool recursive) {
+                   const T* Dinv, T* x, T* r, T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2,
+                   double eps, int maxIt, int slot, bool recursive) {
   esf_status st = initScalars(ctx, which, eps);
   if (st) return st;
   const Dims& d = ctx->d;
@@ -657,6 +672,7 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   a.minv = minv;
   a.b = b;
   a.D = D;
+  a.Dinv = Dinv;
   a.x = x;
   a.r = r;
   a.p = p;
@@ -711,9 +727,10 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
 }
 
 template esf_status crSolve<double>(esf_ctx*, int, const double*, const double*,
-                                    const double*, double*, double*, double*, double*,
-                                    double*, double*, double*, double*, double, int, int, bool);
-template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*,
+                                    const double*, const double*, double*, double*, double*,
+                                    double*, double*, double*, double*, double*, double, int,
+                                    int, bool);
+template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*, const float*,
                                    const float*, float*, float*, float*, float*, float*,
                                    float*, float*, float*, double, int, int, bool);
 
@@ -757,7 +774,8 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     double need = h.resnorm > 0 ? 0.3 * eps * h.bnorm / h.resnorm : 1.0;
     double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
     hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
-    if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->e32, ctx->r32,
+    if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->Dinv32, ctx->e32,
+                             ctx->r32,
                              ctx->r32b, ctx->p32, ctx->s32, ctx->s32b, ctx->Ap32, ctx->Ap32b,
                              innerEps, maxIt - inner, 4, true)))
       return st;
