@@ -158,6 +158,27 @@ def test_steps_match_oracle(esf):
         assert rel_err(i_g, i_o) < 1e-10, k
 
 
+def test_mixed_form_steps_within_1e6_of_fp64_oracle(esf):
+    """the mixed form (FP32 CR inside FP64 refinement) solving to 1e-8
+    against the FP64 oracle solving to 1e-13: after three steps every
+    field within 1e-6 relative (the north-star bound for the mixed form)"""
+    kw = dict(velocity=(0.3, -1.0, 0.2))
+    g = esf.CoupledIntegrator3D.from_cfg_text(ball_cfg(24, eps=1e-8, max_iterations=20000, **kw),
+                                              esf.ESF_MIXED)
+    o = O.OracleScene.from_cfg_text(ball_cfg(24, **TIGHT, **kw))
+    for k in range(3):
+        gs = g.stepImplicit(1.8)
+        rc, st = o.step(1.8)
+        assert gs["status"] == 0 and rc == 0
+        assert abs(gs["dt"] - st.dt) <= 1e-6 * st.dt
+    for gf, of, solid in ((esf._abi.F_VELOCITY, O.VELOCITY, -1),
+                          (esf._abi.F_INTENSITY, O.INTENSITY, -1), (esf._abi.S_U, O.S_U, 0)):
+        err = rel_err(g.download(gf, solid), o.field(of, solid))
+        assert err < 1e-6, (gf, err)
+    g.close()
+    o.close()
+
+
 def three_sphere_cfg(n=24):
     """three spheres of different stiffness, two of them one cell apart so
     their extended boxes (and possibly nodes) are shared"""
