@@ -770,8 +770,13 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
   const int maxOuter = 1000;
   while (!h.done && outer < maxOuter && inner < maxIt) {
     // inner tolerance: ask for the reduction the outer test still needs,
-    // bounded by what FP32 can deliver
-    double need = h.resnorm > 0 ? 0.3 * eps * h.bnorm / h.resnorm : 1.0;
+    // with a margin, bounded by what FP32 can deliver.  The residual a
+    // converged step leaves sits in the slowly converging modes, and the
+    // inner CR's iterations grow with the reduction asked of them, so the
+    // margin is shallow: on the 128^3 ball, margins 0.5 / 0.3 / 0.1 / 0.03
+    // measured 245 / 287 / 803 / 1760 FP32 iterations over 40 steps
+    // (418 / 414 / 353 / 276 steps/s).
+    double need = h.resnorm > 0 ? 0.5 * eps * h.bnorm / h.resnorm : 1.0;
     double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
     hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->Dinv32, ctx->e32,
