@@ -66,8 +66,10 @@ struct SpmvArgs {
 constexpr int TX = 32, TY = 8;
 constexpr int HALO = (TX + 2) * (TY + 2);  // cells of one layer incl. halo
 constexpr int NODES = (TX + 1) * (TY + 1); // nodes of one layer
-constexpr int HPT = (HALO + 255) / 256;    // halo cells per thread
+constexpr int NRING = HALO - TX * TY;      // halo cells around the tile's own
+constexpr int HPT = 2;                     // halo cells per thread: its own, one of the ring
 constexpr int NEDGE = NODES - TX * TY;     // nodes on the tile's low x and low y edges
+static_assert(TX * TY == 256 && NRING <= 256, "one 256-thread block per tile");
 
 // One block owns a TX x TY tile of cells and `layers` cell layers of it,
 // marching up in z.  Iteration k of the march forms node layer k and cell
@@ -119,17 +121,29 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
   // in-layer offset of this thread's cell (clamped into the grid)
   const int cOff = min(cx, d.CX - 1) + min(cy, d.CY - 1) * d.CX;
 
-  // halo slots of this thread: in-layer offset (clamped) and validity
-  int hOff[HPT];
+  // halo slots of this thread: slot 0 is its own cell, slot 1 (threads
+  // below NRING) a cell of the ring around the tile -- the low row, the
+  // high row, then the low and the high column between them.  Per slot:
+  // in-layer offset (clamped), validity and LDS position.  Owning its cell's
+  // halo values lets the CR update keep them in registers until the cell's
+  // output layer instead of reading them again.
+  int hOff[HPT], hIdx[HPT];
   bool hVal[HPT];
+  {
+    const int r = threadIdx.x < NRING ? threadIdx.x : 0;
+    const int q = r - 2 * (TX + 2);
+    const int rx = r < TX + 2 ? r : (r < 2 * (TX + 2) ? r - (TX + 2) : (q < TY ? 0 : TX + 1));
+    const int ry = r < TX + 2 ? 0 : (r < 2 * (TX + 2) ? TY + 1 : 1 + q % TY);
+    const int hx[HPT] = {lx + 1, rx}, hy[HPT] = {ly + 1, ry};
 #pragma unroll
-  for (int h = 0; h < HPT; h++) {
-    const int t = threadIdx.x + 256 * h;
-    const int gx = x0 - 1 + t % (TX + 2), gy = y0 - 1 + t / (TX + 2);
-    hVal[h] = t < HALO && gx >= 0 && gx < d.CX && gy >= 0 && gy < d.CY;
-    // a slot past the halo re-reads the thread's first slot (no extra lines)
-    hOff[h] = t < HALO ? min(max(gx, 0), d.CX - 1) + min(max(gy, 0), d.CY - 1) * d.CX
-                       : hOff[0];
+    for (int h = 0; h < HPT; h++) {
+      const int gx = x0 - 1 + hx[h], gy = y0 - 1 + hy[h];
+      hIdx[h] = hy[h] * (TX + 2) + hx[h];
+      hVal[h] = gx >= 0 && gx < d.CX && gy >= 0 && gy < d.CY;
+      hOff[h] = min(max(gx, 0), d.CX - 1) + min(max(gy, 0), d.CY - 1) * d.CX;
+    }
+    // a thread without a ring cell re-reads its own (no extra lines)
+    if (threadIdx.x >= NRING) hOff[1] = hOff[0];
   }
   // this thread's node: tile node (lx + 1, ly + 1), the upper corner of its
   // cell; its edge node (threads below NEDGE): (0, t) for t <= TY, else
@@ -162,7 +176,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     T mvMain, mvEdge;                 // minv of the main and the edge node
     T in[HPT], ap[HPT], s[HPT], dd[HPT];  // raw values of the halo cells
     float e[HPT];
-    T eB, eD, eAp, eS, eIn, eX, eP;   // operands of the output cell
+    T eB, eD, eX, eP;                 // operands of the output cell
   };
   auto fetchX = [&](Pre& P, int cz) {
     const long zb = (long)min(max(cz, 0), min(zEnd, d.CZ - 1)) * d.cslab;
@@ -184,8 +198,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     const bool zok = cz >= 0 && cz < d.CZ;
 #pragma unroll
     for (int h = 0; h < HPT; h++) {
-      const int t = threadIdx.x + 256 * h;
-      if (h == 0 || t < HALO) {
+      if (h == 0 || threadIdx.x < NRING) {
         T v = P.in[h];
         if (MODE == M_UPDATE) {
           // r_new from r_old, Ap_old and s_old, on the halo as well (the
@@ -197,9 +210,26 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
           // x_old (+ e); the owner writes x_new to the other buffer
           if (a.e32) v = v + (T)P.e[h];
         }
-        L[t] = zok && hVal[h] ? v : (T)0;
+        L[hIdx[h]] = zok && hVal[h] ? v : (T)0;
       }
     }
+  };
+  // M_UPDATE: the own cell's Ap_new = beta Ap + s, 1/D and r_old, taken from
+  // its halo slot as each layer enters LDS and kept for three layers, until
+  // the cell stage of that layer (the slot after every store, so the third
+  // set is always the output layer's)
+  T oAp0 = 0, oAp1 = 0, oAp2 = 0, oD0 = 0, oD1 = 0, oD2 = 0, oR0 = 0, oR1 = 0, oR2 = 0;
+  auto keepOwn = [&](const Pre& P) {
+    if (MODE != M_UPDATE) return;
+    oAp2 = oAp1;
+    oAp1 = oAp0;
+    oAp0 = P.ap[0] * beta + P.s[0];
+    oD2 = oD1;
+    oD1 = oD0;
+    oD0 = P.dd[0];
+    oR2 = oR1;
+    oR1 = oR0;
+    oR0 = P.in[0];
   };
   auto fetchMinv = [&](Pre& P, int k) {
     const long kb = (long)max(min(max(k, 1), min(zEnd, d.Z - 2)), 0) * d.slab;
@@ -210,11 +240,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
     const long c = (long)min(max(cz, 0), zEnd - 1) * d.cslab + cOff;
     if (MODE == M_SETUP_R || MODE == M_CHECK || MODE == M_IR_RESID) P.eB = a.b[c];
     if (MODE == M_SETUP_R || MODE == M_SETUP_S) P.eD = a.D[c];
-    if (MODE == M_UPDATE) P.eD = a.Dinv[c];
     if (MODE == M_UPDATE) {
-      P.eAp = a.Ap[c];
-      P.eS = a.s[c];
-      P.eIn = a.in[c];
       P.eX = a.x[c];
       P.eP = a.p[c];
     }
@@ -323,7 +349,9 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
       const int cz = k - 2;
       if (cz >= z0 && cz < zEnd) {
         const T acc = cornerSums(accLo, wu, 1);
-        const T ownIn = own2;
+        // the cell's own input; the CR update forms r_new again from the kept
+        // operands, by the expression that stored it in LDS
+        const T ownIn = MODE == M_UPDATE ? oR2 - alpha * (oAp2 * oD2) : own2;
         const long c = (long)cz * d.cslab + cOff;
         if (cellXY) {
           if (MODE == M_APPLY) {
@@ -342,23 +370,23 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
           } else if (MODE == M_UPDATE) {
             // the cell's deferred direction update (Ap, p, x) and its
             // r_new, written to the other buffers, and s = S r_new
-            const T apn = P.eAp * beta + P.eS;
-            const T pn = P.eP * beta + P.eIn;
+            // (Ap_new, 1/D and r_old kept from the halo slot)
+            const T apn = oAp2;
+            const T pn = P.eP * beta + oR2;
             a.r[c] = ownIn;
             a.ApOut[c] = apn;
             a.p[c] = pn;
             a.x[c] = P.eX + alpha * pn;
             a.sOut[c] = acc;
-            // P.eD holds 1 / D here
-            const T sq = acc * P.eD;
+            const T sq = acc * oD2;
             part0 += (double)ownIn * (double)acc;
             part1 += (double)acc * (double)sq;
             part2 += (double)apn * (double)sq;
-            part3 += (double)apn * (double)(apn * P.eD);
+            part3 += (double)apn * (double)(apn * oD2);
             // |b - S x| as the recursion carries it (r is D^-1 (b - S x));
             // only the FP32 inner solve tests it
             if (a.recursive) {
-              const double rb = (double)ownIn / (double)P.eD;
+              const double rb = (double)ownIn / (double)oD2;
               part4 += rb * rb;
             }
           } else if (MODE == M_CHECK) {
@@ -376,6 +404,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
       accLo = cornerSums((T)0, wu, 0);
     }
     if (k + 1 <= zEnd) store(P, xs + bNx, k + 1);
+    keepOwn(P);
     issue(P, k + DEPTH);
     // rotate the LDS buffers
     const int b = bLo;
@@ -391,8 +420,10 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
   Pre P[DEPTH];
   fetchX(P[0], z0 - 1);
   store(P[0], xs + bLo, z0 - 1);
+  keepOwn(P[0]);
   fetchX(P[0], z0);
   store(P[0], xs + bHi, z0);
+  keepOwn(P[0]);
 #pragma unroll
   for (int j = 0; j < DEPTH; j++) issue(P[j], z0 + j);
   __syncthreads();
