@@ -36,6 +36,8 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid);
 esf_status updateBoxes(esf_ctx* ctx, int solid);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
+esf_status forkBodyForces(esf_ctx* ctx);
+esf_status joinBodyForces(esf_ctx* ctx);
 esf_status solidRhs(esf_ctx* ctx);
 esf_status scatterSolidV0(esf_ctx* ctx);
 esf_status correctVelocity(esf_ctx* ctx);
@@ -188,7 +190,14 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
     delete ctx;
     return ESF_ERR_DEVICE;
   }
+  if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ESF_ERR_DEVICE;
+  }
   for (int i = 0; i < 24; i++) hipEventCreate(&ctx->ev[i]);
+  hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming);
+  hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming);
   esf_status st = ESF_OK;
   const long N = d.nodes, C = d.cells;
   const bool mixed = p->precision == ESF_MIXED;
@@ -221,6 +230,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   spmvConfigure(ctx);
   ctx->nPartials = std::max<int>(5 * std::max(spmvBlocks(ctx), cdiv(N, 128)) + 64, 1 << 16);
   if (!st) st = dalloc(ctx, &ctx->partials, ctx->nPartials);
+  if (!st) st = dalloc(ctx, &ctx->energyPart, cdiv(N, 256));
   if (!st) st = dalloc(ctx, &ctx->sc, 8);
   if (!st) st = dalloc(ctx, &ctx->dmisc, 16);
   if (!st) st = dalloc(ctx, &ctx->meshOut, 6 * kMaxSolids + 32);
@@ -235,8 +245,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) fillInt(ctx, N, ctx->sidx, -1);
   if (!st && hipStreamSynchronize(ctx->stream) != hipSuccess) st = ESF_ERR_DEVICE;
   if (st) {
-    freeAll(ctx);
-    delete ctx;
+    esf_ctx_destroy(ctx);
     return st;
   }
   *out = ctx;
@@ -245,13 +254,17 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
 
 void esf_ctx_destroy(esf_ctx* ctx) {
   if (!ctx) return;
+  hipStreamSynchronize(ctx->side);
   hipStreamSynchronize(ctx->stream);
   freeAll(ctx);
   for (int i = 0; i < 24; i++) hipEventDestroy(ctx->ev[i]);
+  hipEventDestroy(ctx->evFork);
+  hipEventDestroy(ctx->evJoin);
   for (auto& t : ctx->timed) {
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
   }
+  hipStreamDestroy(ctx->side);
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -607,6 +620,8 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
     if ((st = solidRhs(ctx))) return st;
     if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, ctx->p.implicit_eps, maxIt, 0))) return st;
     if ((st = scatterSolidV0(ctx))) return st;
+    // the fluid's body forces, if they ran beside the solve (esf_step)
+    if ((st = joinBodyForces(ctx))) return st;
   }
   {
     StageTimer t(ctx, 3);
@@ -703,11 +718,26 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   double dt;
   esf_status st, solveSt;
   if ((st = esf_cfl(ctx, alpha < 0 ? 1.8 : alpha, &dt))) return st;
-  if ((st = esf_forces(ctx, nullptr))) return st;
+  // esf_forces, with the fluid's vorticity and body forces forked onto the
+  // side stream: they run beside the first solid solve, which needs only
+  // the solid nodes' part
+  for (auto& so : ctx->solids)
+    if (so.listDirty) {
+      ctx->err = "solid masses changed: call esf_gather_solid_masses first";
+      return ESF_ERR_STATE;
+    }
+  {
+    StageTimer t(ctx, 1);
+    if ((st = stressStage(ctx, nullptr))) return st;
+    if ((st = forkBodyForces(ctx))) return st;
+  }
   // the solve's reports are read at the end of the step: no stall between
   // the solve and the stages after it
   esf_solve_stats s{};
-  if ((st = coupledSolveEnqueue(ctx, &s))) return st;
+  if ((st = coupledSolveEnqueue(ctx, &s))) {
+    joinBodyForces(ctx);
+    return st;
+  }
   if ((st = esf_solid_copy_velocity(ctx))) return st;
   if ((st = esf_solid_finalize(ctx))) return st;
   // DEFOR_SOLID_3D::advectSurfaceMesh (COUPLED_INTEGRATOR_3D.cpp:863-866)

@@ -552,7 +552,9 @@ __global__ void kVorticity(Dims d, const double* __restrict__ v, const double* _
 // body forces and mass vectors  COUPLED_INTEGRATOR_3D.cpp:505-606, with the
 // fluid part of pcrImplicitSolve folded in: v0 = (dt f + m_f v) / m
 // (:669-670, :686-688).  Solid nodes add to the solid force / momentum /
-// damped mass (after the stress gather).
+// damped mass (after the stress gather): here unless skipSolid is set, in
+// which case kSolidBodyForces does it (the first solid solve needs only
+// that part, so the rest can run beside it).
 struct BodyArgs {
   const double* vel;
   const double* nf;
@@ -568,7 +570,29 @@ struct BodyArgs {
   double* sDamped;
   double fluidDensity, gravity, buoyancy, dampCoeff, vortEps, dt;
   double* energyPart;
+  int skipSolid;
 };
+
+// a solid node's share of the body-force pass (the solidIdx >= 0 branch of
+// kBodyForces, the same expressions in the same order)
+__device__ inline void solidBodyForces(const Dims& d, const BodyArgs& a, long index,
+                                       int solidIdx) {
+  const double dh = d.dh;
+  const double fluidMass = fmax(1.0 - a.nf[index], 0.0) * a.fluidDensity * dh * dh * dh;
+  const double mass = fluidMass + a.solidMass[index];
+  const double* v = &a.vel[3 * index];
+  const long k = 3L * solidIdx;
+  const double dm = a.solidMass[index] * a.dampCoeff + fluidMass;
+  a.sDamped[k] = dm;
+  a.sDamped[k + 1] = dm;
+  a.sDamped[k + 2] = dm;
+  for (int c = 0; c < 3; c++) a.sMv[k + c] += fluidMass * v[c];
+  const double mg = mass * a.gravity;
+  a.sForce[k] += mg * 0;
+  a.sForce[k + 1] += mg * -1;
+  a.sForce[k + 2] += mg * 0;
+  a.sForce[k + 1] += a.buoyancy * fluidMass * a.intensity[index];
+}
 
 __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
   __shared__ double red[16];
@@ -624,18 +648,8 @@ __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
         a.v0[3 * index] = (f0 * a.dt + mv0) * mi;
         a.v0[3 * index + 1] = (f1 * a.dt + mv1) * mi;
         a.v0[3 * index + 2] = (f2 * a.dt + mv2) * mi;
-      } else {
-        const long k = 3L * solidIdx;
-        const double dm = a.solidMass[index] * a.dampCoeff + fluidMass;
-        a.sDamped[k] = dm;
-        a.sDamped[k + 1] = dm;
-        a.sDamped[k + 2] = dm;
-        for (int c = 0; c < 3; c++) a.sMv[k + c] += fluidMass * v[c];
-        const double mg = mass * a.gravity;
-        a.sForce[k] += mg * 0;
-        a.sForce[k + 1] += mg * -1;
-        a.sForce[k + 2] += mg * 0;
-        a.sForce[k + 1] += a.buoyancy * fluidMass * a.intensity[index];
+      } else if (!a.skipSolid) {
+        solidBodyForces(d, a, index, solidIdx);
       }
       a.minv[index] = 1.0 / mass;
       en += 0.5 * fluidMass * (v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]));
@@ -643,6 +657,17 @@ __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
   }
   const double t = blockSum(en, red);
   if (threadIdx.x == 0) a.energyPart[xcdBlock()] = t;
+}
+
+// the solid nodes' part of kBodyForces, one thread per solid DOF node
+__global__ void kSolidBodyForces(Dims d, BodyArgs a, int nSolid, const int* solidNodes) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nSolid) return;
+  const long index = solidNodes[s];
+  const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
+  if (!isInterior(d, x, y, z)) return;
+  const int solidIdx = a.sidx[index];
+  if (solidIdx >= 0) solidBodyForces(d, a, index, solidIdx);
 }
 
 // solid right-hand side dt F + M v  (COUPLED_INTEGRATOR_3D.cpp:664-665)
@@ -963,10 +988,7 @@ __global__ void kRecoverBC(Dims d, RecoverArgs a, double* vel, double* intensity
   hipLaunchKernelGGL(kernel, dim3(cdiv(ctx->d.nodes, 256)), dim3(256), 0, ctx->stream, \
                      __VA_ARGS__)
 
-esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy) {
-  const Dims& d = ctx->d;
-  LAUNCH_NODES(kVorticity, d, ctx->vel, ctx->nodeFrac, ctx->vort, ctx->ws,
-               ctx->vortEps > 0 ? 1 : 0);
+static BodyArgs bodyArgs(esf_ctx* ctx) {
   BodyArgs a{};
   a.vel = ctx->vel;
   a.nf = ctx->nodeFrac;
@@ -986,19 +1008,58 @@ esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy) {
   a.dampCoeff = 1 + ctx->dt * ctx->p.rayleigh_alpha;
   a.vortEps = ctx->vortEps;
   a.dt = ctx->dt;
-  a.energyPart = ctx->partials;
+  a.energyPart = ctx->energyPart;
+  a.skipSolid = 0;
+  return a;
+}
+
+esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy) {
+  const Dims& d = ctx->d;
+  LAUNCH_NODES(kVorticity, d, ctx->vel, ctx->nodeFrac, ctx->vort, ctx->ws,
+               ctx->vortEps > 0 ? 1 : 0);
+  const BodyArgs a = bodyArgs(ctx);
   const int nb = cdiv(d.nodes, 256);
   LAUNCH_NODES(kBodyForces, d, a);
   if (energy) {
     std::vector<double> h(nb);
-    ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, nb * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx->stream));
+    ESF_HIP(hipMemcpyAsync(h.data(), ctx->energyPart, nb * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
     ESF_HIP(hipStreamSynchronize(ctx->stream));
     double e = 0;
     for (double v : h) e += v;
     *energy += e;
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+// The same pass split for the step: the solid nodes' part on the context's
+// stream (the first solid solve reads it), vorticity and the fluid part on
+// the side stream, which the stream joins before the pressure stage
+// (joinBodyForces).  The side kernels write vort, ws, v0 at fluid nodes,
+// minv and the energy sums; the first solid solve touches none of them.
+esf_status forkBodyForces(esf_ctx* ctx) {
+  const Dims& d = ctx->d;
+  BodyArgs a = bodyArgs(ctx);
+  if (ctx->nSolid)
+    hipLaunchKernelGGL(kSolidBodyForces, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
+                       d, a, ctx->nSolid, (const int*)ctx->solidNodes);
+  ESF_HIP(hipEventRecord(ctx->evFork, ctx->stream));
+  ESF_HIP(hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
+  const dim3 grid(cdiv(d.nodes, 256));
+  hipLaunchKernelGGL(kVorticity, grid, dim3(256), 0, ctx->side, d, (const double*)ctx->vel,
+                     (const double*)ctx->nodeFrac, ctx->vort, ctx->ws, ctx->vortEps > 0 ? 1 : 0);
+  a.skipSolid = 1;
+  hipLaunchKernelGGL(kBodyForces, grid, dim3(256), 0, ctx->side, d, a);
+  ESF_HIP(hipEventRecord(ctx->evJoin, ctx->side));
+  ctx->joinPending = true;
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status joinBodyForces(esf_ctx* ctx) {
+  if (!ctx->joinPending) return ESF_OK;
+  ctx->joinPending = false;
+  ESF_HIP(hipStreamWaitEvent(ctx->stream, ctx->evJoin, 0));
+  return ESF_OK;
 }
 
 esf_status solidRhs(esf_ctx* ctx) {

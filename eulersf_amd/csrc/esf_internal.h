@@ -91,6 +91,11 @@ struct esf_ctx {
   esf_params p;
   double vortEps = 0, defaultDt = 0, dt = 0;
   hipStream_t stream = nullptr;
+  // the step runs the fluid's vorticity and body forces on `side` while the
+  // first solid solve runs on `stream` (evFork / evJoin order the two)
+  hipStream_t side = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
+  bool joinPending = false;
   int device = 0;
   std::string err;
   // grid fields
@@ -132,6 +137,7 @@ struct esf_ctx {
   double* inboxX = nullptr;     // rows of K x for that check, laid out as inbox
   // reductions
   double* partials = nullptr;  // per-block partial sums
+  double* energyPart = nullptr;  // per-block energy sums of the body-force pass
   double* sPartials = nullptr; // solid CR: 4 sets of 4 x node-pass blocks
   esf::CrScalars* sc = nullptr;   // device scalars (several solves)
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
