@@ -37,6 +37,8 @@ esf_status updateBoxes(esf_ctx* ctx, int solid);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
 esf_status forkBodyForces(esf_ctx* ctx);
+esf_status cflEnqueue(esf_ctx* ctx, double alpha);
+esf_status cflFinish(esf_ctx* ctx, double* dtOut);
 esf_status joinBodyForces(esf_ctx* ctx);
 esf_status solidRhs(esf_ctx* ctx);
 esf_status scatterSolidV0(esf_ctx* ctx);
@@ -119,6 +121,7 @@ static void freeAll(esf_ctx* ctx) {
   ctx->allocs.clear();
   ctx->deviceBytes = 0;
   if (ctx->hostFlags) hipHostFree(ctx->hostFlags);
+  if (ctx->hostCfl) hipHostFree(ctx->hostCfl);
   if (ctx->hostRing) hipHostFree(ctx->hostRing);
   if (ctx->hostSc) hipHostFree(ctx->hostSc);
 }
@@ -198,6 +201,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   for (int i = 0; i < 24; i++) hipEventCreate(&ctx->ev[i]);
   hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming);
   hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming);
+  hipEventCreateWithFlags(&ctx->evCfl, hipEventDisableTiming);
   esf_status st = ESF_OK;
   const long N = d.nodes, C = d.cells;
   const bool mixed = p->precision == ESF_MIXED;
@@ -239,6 +243,8 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
     st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostFlags, 64 * sizeof(int)) != hipSuccess)
     st = ESF_ERR_OOM;
+  if (!st && hipHostMalloc((void**)&ctx->hostCfl, kCflBlocks * sizeof(double)) != hipSuccess)
+    st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostSc, esf_ctx::kReportSlots * sizeof(CrScalars)) !=
                  hipSuccess)
     st = ESF_ERR_OOM;
@@ -260,6 +266,7 @@ void esf_ctx_destroy(esf_ctx* ctx) {
   for (int i = 0; i < 24; i++) hipEventDestroy(ctx->ev[i]);
   hipEventDestroy(ctx->evFork);
   hipEventDestroy(ctx->evJoin);
+  hipEventDestroy(ctx->evCfl);
   for (auto& t : ctx->timed) {
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
@@ -717,18 +724,20 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if (!ctx) return ESF_ERR_INVALID;
   double dt;
   esf_status st, solveSt;
-  if ((st = esf_cfl(ctx, alpha < 0 ? 1.8 : alpha, &dt))) return st;
-  // esf_forces, with the fluid's vorticity and body forces forked onto the
-  // side stream: they run beside the first solid solve, which needs only
-  // the solid nodes' part
   for (auto& so : ctx->solids)
     if (so.listDirty) {
       ctx->err = "solid masses changed: call esf_gather_solid_masses first";
       return ESF_ERR_STATE;
     }
+  // esf_cfl and esf_forces, reordered: the stress stage does not use dt,
+  // so it is queued behind the CFL kernel before the host waits for dt;
+  // the fluid's vorticity and body forces go to the side stream, beside
+  // the first solid solve, which needs only the solid nodes' part
   {
     StageTimer t(ctx, 1);
+    if ((st = cflEnqueue(ctx, alpha < 0 ? 1.8 : alpha))) return st;
     if ((st = stressStage(ctx, nullptr))) return st;
+    if ((st = cflFinish(ctx, &dt))) return st;
     if ((st = forkBodyForces(ctx))) return st;
   }
   // the solve's reports are read at the end of the step: no stall between

@@ -498,22 +498,38 @@ __global__ void kCfl(int nSolid, const int* solidNodes, const double* vel, doubl
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut) {
+// the CFL kernel and the copy of its block minima to pinned memory, queued;
+// cflFinish waits for that copy only, so work queued in between (the stress
+// stage) runs while the host waits
+static int cflBlocks(const esf_ctx* ctx) { return std::min(cdiv(ctx->nSolid, 256), kCflBlocks); }
+
+esf_status cflEnqueue(esf_ctx* ctx, double alpha) {
+  if (ctx->nSolid == 0) return ESF_OK;
+  const int nb = cflBlocks(ctx);
+  hipLaunchKernelGGL(kCfl, dim3(nb), dim3(256), 0, ctx->stream, ctx->nSolid, ctx->solidNodes,
+                     ctx->vel, ctx->d.dhInv, alpha, ctx->partials);
+  ESF_HIP(hipMemcpyAsync(ctx->hostCfl, ctx->partials, nb * sizeof(double),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipEventRecord(ctx->evCfl, ctx->stream));
+  return ESF_OK;
+}
+
+esf_status cflFinish(esf_ctx* ctx, double* dtOut) {
   double dt = 1e9;
   if (ctx->nSolid > 0) {
-    const int nb = std::min(cdiv(ctx->nSolid, 256), 256);
-    hipLaunchKernelGGL(kCfl, dim3(nb), dim3(256), 0, ctx->stream, ctx->nSolid, ctx->solidNodes,
-                       ctx->vel, ctx->d.dhInv, alpha, ctx->partials);
-    std::vector<double> h(nb);
-    ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, nb * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx->stream));
-    ESF_HIP(hipStreamSynchronize(ctx->stream));
-    for (double v : h) dt = std::min(dt, v);
+    ESF_HIP(hipEventSynchronize(ctx->evCfl));
+    const int nb = cflBlocks(ctx);
+    for (int i = 0; i < nb; i++) dt = std::min(dt, ctx->hostCfl[i]);
   }
   dt = std::min(dt, ctx->defaultDt);
   ctx->dt = dt;
   *dtOut = dt;
   return ESF_OK;
+}
+
+esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut) {
+  esf_status st = cflEnqueue(ctx, alpha);
+  return st ? st : cflFinish(ctx, dtOut);
 }
 
 // ---------------------------------------------------------------------------

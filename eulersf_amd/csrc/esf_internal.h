@@ -30,6 +30,7 @@ struct Dims {
 // solids per context (scene files of the reference use one; the isolated
 // 256^3 system uses eight)
 constexpr int kMaxSolids = 8;
+constexpr int kCflBlocks = 256;  // most blocks of the CFL minimum (grid-stride)
 
 // Per-solid box, in node coordinates (DEFOR_SOLID_3D.h:138-140)
 struct Box {
@@ -143,6 +144,8 @@ struct esf_ctx {
   double* dmisc = nullptr;        // device misc scalars (dt, cfl min, energy)
   double* meshOut = nullptr;      // mesh broad phase: root boxes, then pair flags
   int* hostFlags = nullptr;       // pinned: counts read back by the host
+  double* hostCfl = nullptr;      // pinned: the CFL kernel's block minima
+  hipEvent_t evCfl = nullptr;     // their copy has landed
   esf::CrScalars* hostRing = nullptr;  // pinned ring of polled scalar blocks
   int crQueued = 0;  // iterations the last pressure CR solve queued
   // pressure SpMV launch shape (fixed at creation): cell layers per block,
