@@ -274,3 +274,35 @@ def test_solid_check_returns_checked_iterate(esf, eps, max_it):
     assert rel_err(g.download(esf._abi.F_SOLID_V0), o.field(O.SOLID_V0)) < 1e-12
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_step_overlap_bitwise_equals_staged_sequence(esf, mixed):
+    """esf_step runs the fluid's vorticity and body forces on a second stream
+    beside the first solid solve and queues the stress stage ahead of the dt
+    read-back; the stage entry points called one after another on one stream
+    must give bitwise the same fields (no race, no reordering of
+    arithmetic), step after step, in both forms"""
+    text = ball_cfg(24, velocity=(0.3, -1.0, 0.2), eps=1e-10, max_iterations=20000)
+    prec = esf.ESF_MIXED if mixed else esf.ESF_PLAIN_FP64
+    a = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
+    b = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
+    fields = [(esf._abi.F_VELOCITY, -1), (esf._abi.F_INTENSITY, -1), (esf._abi.S_U, 0),
+              (esf._abi.S_VELOCITY, 0)]
+    for k in range(3):
+        sa = a.stepImplicit(1.8)
+        dt = b.CFL(1.8)
+        b.forces()
+        sb = b.pcrImplicitSolve()
+        b.solidCopyVelocity()
+        b.solidFinalize()
+        b.gatherSolidMasses()
+        b.advectFluidVelocityField()
+        b.advectIntensity()
+        b.recoverAndBoundary()
+        assert sa["dt"] == dt, k
+        for f, s in fields:
+            va, vb = a.download(f, s), b.download(f, s)
+            assert np.array_equal(va.view(np.int64), vb.view(np.int64)), (k, f)
+    a.close()
+    b.close()
