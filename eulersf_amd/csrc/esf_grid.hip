@@ -1055,6 +1055,14 @@ esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy) {
 // minv and the energy sums; the first solid solve touches none of them.
 esf_status forkBodyForces(esf_ctx* ctx) {
   const Dims& d = ctx->d;
+  // the side stream is created on first use, not with the context: HIP hands
+  // the process's few hardware queues to streams in creation order, so
+  // creating it beside every context's own stream would put all of those
+  // on every other queue when many contexts are created together
+  if (!ctx->side && hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+    ctx->side = nullptr;
+    return vorticityAndBodyForces(ctx, nullptr);
+  }
   BodyArgs a = bodyArgs(ctx);
   if (ctx->nSolid)
     hipLaunchKernelGGL(kSolidBodyForces, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
