@@ -4,6 +4,7 @@ its Python mirror read cfgs exactly as the reference's SIMPLE_PARSER does."""
 import ctypes
 import json
 import os
+import tempfile
 import re
 import subprocess
 
@@ -47,13 +48,13 @@ def test_library_is_gfx950_code():
 
 def read_with_host(text):
     lib = _abi.load()
-    path = os.path.join(ROOT, "_work", "host_cfg_test.cfg")
-    os.makedirs(os.path.dirname(path), exist_ok=True)
-    with open(path, "w") as f:
-        f.write(text)
-    g, p, info = _abi.Grid(), _abi.Params(), _abi.SceneInfo()
-    st = lib.esf_cfg_read(path.encode(), ctypes.byref(g), ctypes.byref(p), ctypes.byref(info))
-    os.unlink(path)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "host_cfg_test.cfg")
+        with open(path, "w") as f:
+            f.write(text)
+        g, p, info = _abi.Grid(), _abi.Params(), _abi.SceneInfo()
+        st = lib.esf_cfg_read(path.encode(), ctypes.byref(g), ctypes.byref(p),
+                              ctypes.byref(info))
     return st, g, p, info
 
 

@@ -1,7 +1,7 @@
 """GPU (libesf.so) against the CPU oracle on the same scenes: per stage and
 per step.  Integer/byte work and reordered-but-identical arithmetic must be
-bit-exact; Krylov outputs must agree to 1e-10 relative (north-star parity
-bar), both sides solving to 1e-13."""
+bit-exact; Krylov outputs must agree to 1e-10 relative (the parity bar of
+the FP64 form), both sides solving to 1e-13."""
 import numpy as np
 import pytest
 
@@ -161,7 +161,7 @@ def test_steps_match_oracle(esf):
 def test_mixed_form_steps_within_1e6_of_fp64_oracle(esf):
     """the mixed form (FP32 CR inside FP64 refinement) solving to 1e-8
     against the FP64 oracle solving to 1e-13: after three steps every
-    field within 1e-6 relative (the north-star bound for the mixed form)"""
+    field within 1e-6 relative (the bound set for the mixed form)"""
     kw = dict(velocity=(0.3, -1.0, 0.2))
     g = esf.CoupledIntegrator3D.from_cfg_text(ball_cfg(24, eps=1e-8, max_iterations=20000, **kw),
                                               esf.ESF_MIXED)
