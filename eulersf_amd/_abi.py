@@ -162,6 +162,8 @@ _PROTOS = {
     "esf_solid_get_mesh": (_S, [_P, ctypes.c_int, _D, _D, _I]),
     "esf_advect_surface_mesh": (_S, [_P]),
     "esf_broad_phase": (_S, [_P, _D]),
+    "esf_compute_sdf": (_S, [_P, _D, _D, _I, _I]),
+    "esf_extend_vector_field": (_S, [_P, _I, _D, _D, _D]),
     "esf_obj_read": (_S, [ctypes.c_char_p, _I, _I, _D, ctypes.c_int, _I, ctypes.c_int]),
     "esf_obj_write": (_S, [ctypes.c_char_p, ctypes.c_int, _D, ctypes.c_int, _I]),
     "esf_read_frame": (_S, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),

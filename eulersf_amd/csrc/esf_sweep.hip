@@ -1,0 +1,387 @@
+// esf_sweep.hip -- the solid's fast sweeping on the GPU, bitwise the
+// reference's ordered Gauss-Seidel sweeps:
+//   computeSDF (DEFOR_SOLID_3D.cpp:216-267): phi = -0.5 on nodes with mass,
+//     a large distance elsewhere, the box of the massive nodes widened by the
+//     tight border and the extrapolation cells, two rounds of sweepPhi;
+//   sweepPhi / solveDistance (:369-454): eight ordered sweeps of the grid
+//     Eikonal update over the box;
+//   extendVectorField (:566-574): four rounds of sweepVectorField (:521-564),
+//     eight sweepOnePass (:457-519) and the box faces copied inwards.
+//
+// A sweep visits the box in loop order and each node reads the three
+// neighbours visited before it along i, j and k (or, for the faces, only
+// what the previous loop nest wrote).  A node's new value therefore depends
+// only on those three, so the nodes of one plane i' + j' + k' = p (primed:
+// positions along the sweep's directions) are independent and the planes
+// run in order: one workgroup walks the planes with a barrier between
+// them, and every node gets the reference's value by the reference's
+// arithmetic.  sweepOnePass also carries three flags across the whole pass
+// (solvableX/Y/Z: once cleared they stay cleared).  Whether a node clears
+// one depends on phi and the mass only, never on the field being swept, so
+// a parallel prepass finds, per flag, the first node in loop order that
+// clears it (an integer atomic minimum); a node's flag is then "its loop
+// position comes before that one".
+//
+// These are host-side stages of the reference's finalize that the step does
+// not run; the entry points serve a host integrator that keeps them.
+#include <climits>
+
+#include "esf_internal.h"
+
+namespace esf {
+
+namespace {
+
+constexpr int kWaveThreads = 1024;
+
+struct SweepBox {
+  int X, Y, Z;
+  int x0, y0, z0, x1, y1, z1;
+};
+
+__device__ __host__ inline long nodeAt(const SweepBox& b, int i, int j, int k) {
+  return i + (long)b.X * (j + (long)b.Y * k);
+}
+
+// solveDistance (:369-395); for finite inputs the candidate at i = 0 is
+// always valid, so the reference's loop never steps below 0
+__device__ inline double solveDistance(double p, double q, double r, double v) {
+  double d[3];
+  d[0] = fmin(p, fmin(q, r));
+  d[2] = fmax(p, fmax(q, r));
+  d[1] = p + q + r - d[0] - d[2];
+  double b[3], c[3];
+  b[0] = d[0];
+  c[0] = d[0] * d[0] - 1.0;
+#pragma unroll
+  for (int x = 1; x < 3; x++) {
+    b[x] = d[x] + b[x - 1];
+    c[x] = d[x] * d[x] + c[x - 1];
+  }
+  int i = 2;
+  double discrim = b[2] * b[2] - 3.0 * c[2];
+  double nd = (b[2] + sqrt(discrim)) / 3.0;
+  while ((discrim < 0.0 || nd < d[i]) && i > 0) {
+    i--;
+    discrim = b[i] * b[i] - (double)(i + 1) * c[i];
+    if (discrim > 0.0) nd = (b[i] + sqrt(discrim)) / (double)(i + 1);
+  }
+  return nd < v ? nd : v;
+}
+
+// one ordered sweep: n* nodes along each axis from (i0, j0, k0) in steps
+// (di, dj, dk), loop order k outer, i inner
+struct Sweep {
+  int i0, j0, k0, di, dj, dk, ni, nj, nk;
+};
+
+// phi -> -0.5 / large, and the extent of the massive nodes
+__global__ void kSdfInit(long n, int X, int Y, const double* __restrict__ mass, double large,
+                         double* phi, int* ext) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n;
+       c += (long)gridDim.x * blockDim.x) {
+    const bool m = mass[c] > 0;
+    phi[c] = m ? -0.5 : large;
+    if (m) {
+      const int x = (int)(c % X), y = (int)((c / X) % Y), z = (int)(c / ((long)X * Y));
+      atomicMin(&ext[0], x);
+      atomicMin(&ext[1], y);
+      atomicMin(&ext[2], z);
+      atomicMax(&ext[3], x);
+      atomicMax(&ext[4], y);
+      atomicMax(&ext[5], z);
+    }
+  }
+}
+
+// sweepPhi's sweep s, one workgroup, plane by plane
+__global__ void __launch_bounds__(kWaveThreads) kPhiWave(SweepBox b, Sweep s,
+                                                        const double* __restrict__ mass,
+                                                        double* phi) {
+  const int planes = s.ni + s.nj + s.nk - 2;
+  const int jk = s.nj * s.nk;
+  for (int p = 0; p < planes; p++) {
+    for (int t = threadIdx.x; t < jk; t += blockDim.x) {
+      const int jp = t % s.nj, kp = t / s.nj, ip = p - jp - kp;
+      if (ip < 0 || ip >= s.ni) continue;
+      const int i = s.i0 + s.di * ip, j = s.j0 + s.dj * jp, k = s.k0 + s.dk * kp;
+      const long c = nodeAt(b, i, j, k);
+      if (mass[c] == 0)
+        phi[c] = solveDistance(phi[nodeAt(b, i - s.di, j, k)], phi[nodeAt(b, i, j - s.dj, k)],
+                               phi[nodeAt(b, i, j, k - s.dk)], phi[c]);
+    }
+    __syncthreads();
+  }
+}
+
+// sweepOnePass's per-node decisions, from phi and the mass only:
+// returns false where the node is not updated; clear* say whether the node
+// clears that flag (it does so before any later axis may skip it)
+struct PassNode {
+  double dp, dq, dr;
+  bool clearX, clearY, clearZ;
+};
+__device__ inline bool passNode(const SweepBox& b, const Sweep& s, const double* mass,
+                                const double* phi, int i, int j, int k, PassNode& o) {
+  o.clearX = o.clearY = o.clearZ = false;
+  const long c = nodeAt(b, i, j, k);
+  if (mass[c] != 0) return false;
+  const double pc = phi[c];
+  o.dp = pc - phi[nodeAt(b, i - s.di, j, k)];
+  const double dpn = pc - phi[nodeAt(b, i + s.di, j, k)];
+  o.dq = pc - phi[nodeAt(b, i, j - s.dj, k)];
+  const double dqn = pc - phi[nodeAt(b, i, j + s.dj, k)];
+  o.dr = pc - phi[nodeAt(b, i, j, k - s.dk)];
+  const double drn = pc - phi[nodeAt(b, i, j, k + s.dk)];
+  if (o.dp < 0) {
+    if (dpn >= 0) return false;
+    o.clearX = true;
+  }
+  if (o.dq < 0) {
+    if (dqn >= 0) return false;
+    o.clearY = true;
+  }
+  if (o.dr < 0) {
+    if (drn >= 0) return false;
+    o.clearZ = true;
+  }
+  return true;
+}
+
+// the first node, in loop order, that clears each flag
+__global__ void kPassFlags(SweepBox b, Sweep s, const double* __restrict__ mass,
+                           const double* __restrict__ phi, int* first) {
+  const long n = (long)s.ni * s.nj * s.nk;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int ip = (int)(t % s.ni), jp = (int)((t / s.ni) % s.nj), kp = (int)(t / ((long)s.ni * s.nj));
+    PassNode o;
+    passNode(b, s, mass, phi, s.i0 + s.di * ip, s.j0 + s.dj * jp, s.k0 + s.dk * kp, o);
+    if (o.clearX) atomicMin(&first[0], (int)t);
+    if (o.clearY) atomicMin(&first[1], (int)t);
+    if (o.clearZ) atomicMin(&first[2], (int)t);
+  }
+}
+
+// the pass itself, one workgroup, plane by plane
+__global__ void __launch_bounds__(kWaveThreads) kPassWave(SweepBox b, Sweep s,
+                                                         const double* __restrict__ mass,
+                                                         const double* __restrict__ phi,
+                                                         const int* __restrict__ first,
+                                                         double* field) {
+  const int planes = s.ni + s.nj + s.nk - 2;
+  const int jk = s.nj * s.nk;
+  const int fx = first[0], fy = first[1], fz = first[2];
+  for (int p = 0; p < planes; p++) {
+    for (int t = threadIdx.x; t < jk; t += blockDim.x) {
+      const int jp = t % s.nj, kp = t / s.nj, ip = p - jp - kp;
+      if (ip < 0 || ip >= s.ni) continue;
+      const int i = s.i0 + s.di * ip, j = s.j0 + s.dj * jp, k = s.k0 + s.dk * kp;
+      PassNode o;
+      if (!passNode(b, s, mass, phi, i, j, k, o)) continue;
+      // a flag is cleared from its first clearing node on, that node included
+      const int ord = (kp * s.nj + jp) * s.ni + ip;
+      const double dp = ord >= fx ? 0.0 : o.dp;
+      const double dq = ord >= fy ? 0.0 : o.dq;
+      const double dr = ord >= fz ? 0.0 : o.dr;
+      double alpha, beta;
+      if (dp + dq + dr < 1e-3) {
+        alpha = 0.333;
+        beta = 0.333;
+      } else {
+        alpha = dp / (dp + dq + dr);
+        beta = dq / (dp + dq + dr);
+      }
+      const double gamma = 1 - alpha - beta;
+      const long c = 3 * nodeAt(b, i, j, k), a = 3 * nodeAt(b, i - s.di, j, k),
+                 bb = 3 * nodeAt(b, i, j - s.dj, k), e = 3 * nodeAt(b, i, j, k - s.dk);
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+        field[c + q] = alpha * field[a + q] + beta * field[bb + q] + gamma * field[e + q];
+    }
+    __syncthreads();
+  }
+}
+
+// sweepVectorField's face copies, one axis per launch (z, then y, then x):
+// each thread copies both faces of its line in the reference's order
+__global__ void kFaceCopy(SweepBox b, int axis, const double* __restrict__ mass,
+                          double* field) {
+  const int lo[3] = {b.x0, b.y0, b.z0}, hi[3] = {b.x1, b.y1, b.z1};
+  const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;  // the two other axes
+  const long nu = hi[u] - lo[u], nw = hi[w] - lo[w];
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= nu * nw) return;
+  int pos[3];
+  pos[u] = lo[u] + (int)(t / nw);
+  pos[w] = lo[w] + (int)(t % nw);
+  auto copy = [&](int to, int from) {
+    pos[axis] = to;
+    const long c = nodeAt(b, pos[0], pos[1], pos[2]);
+    pos[axis] = from;
+    const long f = nodeAt(b, pos[0], pos[1], pos[2]);
+    if (mass[c] == 0)
+      for (int q = 0; q < 3; q++) field[3 * c + q] = field[3 * f + q];
+  };
+  copy(lo[axis], lo[axis] + 1);
+  copy(hi[axis] - 1, hi[axis] - 2);
+}
+
+__global__ void kFillFirst(int* first) {
+  if (threadIdx.x < 3) first[threadIdx.x] = INT_MAX;
+}
+
+// a sweep from a to b exclusive (the reference's sweepOnePass bounds)
+Sweep passRange(int i0, int i1, int j0, int j1, int k0, int k1) {
+  Sweep s;
+  s.i0 = i0;
+  s.j0 = j0;
+  s.k0 = k0;
+  s.di = i0 < i1 ? 1 : -1;
+  s.dj = j0 < j1 ? 1 : -1;
+  s.dk = k0 < k1 ? 1 : -1;
+  s.ni = std::max((i1 - i0) * s.di, 0);
+  s.nj = std::max((j1 - j0) * s.dj, 0);
+  s.nk = std::max((k1 - k0) * s.dk, 0);
+  return s;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+};
+
+esf_status checkBox(esf_ctx* ctx, const int box[6]) {
+  const Dims& d = ctx->d;
+  const int res[3] = {d.X, d.Y, d.Z};
+  for (int a = 0; a < 3; a++)
+    if (box[a] < 0 || box[3 + a] > res[a] || box[3 + a] - box[a] < 3) {
+      ctx->err = "sweep box must lie in the grid and span at least 3 nodes per axis";
+      return ESF_ERR_INVALID;
+    }
+  return ESF_OK;
+}
+
+esf_status sweepPhiDevice(esf_ctx* ctx, const SweepBox& b, const double* mass, double* phi) {
+  for (int s = 0; s < 8; s++) {
+    const int dx = (s & 2) ? -1 : 1, dy = (s & 1) ? -1 : 1, dz = (s & 4) ? -1 : 1;
+    // ascending from min + 1 to max - 1, descending from max - 2 to min
+    Sweep w;
+    w.di = dx;
+    w.dj = dy;
+    w.dk = dz;
+    w.i0 = dx > 0 ? b.x0 + 1 : b.x1 - 2;
+    w.j0 = dy > 0 ? b.y0 + 1 : b.y1 - 2;
+    w.k0 = dz > 0 ? b.z0 + 1 : b.z1 - 2;
+    w.ni = b.x1 - b.x0 - 1;
+    w.nj = b.y1 - b.y0 - 1;
+    w.nk = b.z1 - b.z0 - 1;
+    hipLaunchKernelGGL(kPhiWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, w, mass, phi);
+  }
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+}  // namespace
+
+}  // namespace esf
+
+using namespace esf;
+
+esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int box[6],
+                           int tight[6]) {
+  if (!ctx || !mass || !phi || !box || !tight) return ESF_ERR_INVALID;
+  const Dims& d = ctx->d;
+  const long n = d.nodes;
+  DevBuf dm, dphi, dext;
+  if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
+      hipMalloc(&dext.p, 6 * sizeof(int)) != hipSuccess) {
+    ctx->err = "compute_sdf: device allocation failed";
+    return ESF_ERR_OOM;
+  }
+  const int init[6] = {d.X, d.Y, d.Z, 0, 0, 0};
+  ESF_HIP(hipMemcpyAsync(dm.p, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(dext.p, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(kSdfInit, dim3(std::min<long>(cdiv(n, 256), 4096)), dim3(256), 0,
+                     ctx->stream, n, d.X, d.Y, (const double*)dm.p, (double)(d.X + d.Y + d.Z + 2),
+                     (double*)dphi.p, (int*)dext.p);
+  int ext[6];
+  ESF_HIP(hipMemcpyAsync(ext, dext.p, sizeof(ext), hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  const int res[3] = {d.X, d.Y, d.Z};
+  const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
+  for (int a = 0; a < 3; a++) {
+    tight[a] = std::max(ext[a] - tb, 0);
+    tight[3 + a] = std::min(ext[3 + a] + tb, res[a]);
+    box[a] = std::max(ext[a] - ex, 0);
+    box[3 + a] = std::min(ext[3 + a] + ex, res[a]);
+  }
+  // no node with mass: the box is empty and the reference's loops run empty
+  const bool sweep = box[3] > box[0] && box[4] > box[1] && box[5] > box[2];
+  if (sweep) {
+    const SweepBox b{d.X, d.Y, d.Z, box[0], box[1], box[2], box[3], box[4], box[5]};
+    for (int r = 0; r < 2; r++) {
+      esf_status st = sweepPhiDevice(ctx, b, (const double*)dm.p, (double*)dphi.p);
+      if (st) return st;
+    }
+  }
+  ESF_HIP(hipMemcpyAsync(phi, dphi.p, 8 * n, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double* mass,
+                                   const double* phi, double* field) {
+  if (!ctx || !box || !mass || !phi || !field) return ESF_ERR_INVALID;
+  esf_status st = checkBox(ctx, box);
+  if (st) return st;
+  const Dims& d = ctx->d;
+  const long n = d.nodes;
+  DevBuf dm, dphi, dfield, dfirst;
+  if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
+      hipMalloc(&dfield.p, 24 * n) != hipSuccess ||
+      hipMalloc(&dfirst.p, 3 * sizeof(int)) != hipSuccess) {
+    ctx->err = "extend_vector_field: device allocation failed";
+    return ESF_ERR_OOM;
+  }
+  ESF_HIP(hipMemcpyAsync(dm.p, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(dphi.p, phi, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(dfield.p, field, 24 * n, hipMemcpyHostToDevice, ctx->stream));
+  const SweepBox b{d.X, d.Y, d.Z, box[0], box[1], box[2], box[3], box[4], box[5]};
+  const double* m = (const double*)dm.p;
+  const double* ph = (const double*)dphi.p;
+  double* f = (double*)dfield.p;
+  int* first = (int*)dfirst.p;
+  const int x0 = b.x0, y0 = b.y0, z0 = b.z0, x1 = b.x1, y1 = b.y1, z1 = b.z1;
+  // sweepVectorField's eight passes (:524-545)
+  const Sweep passes[8] = {
+      passRange(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0 + 1, z1 - 1),
+      passRange(x0 + 1, x1 - 1, y1 - 2, y0, z0 + 1, z1 - 1),
+      passRange(x1 - 2, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1),
+      passRange(x1 - 2, x0, y1 - 2, y0, z0 + 1, z1 - 1),
+      passRange(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1 - 2, z0),
+      passRange(x0 + 1, x1 - 1, y1 - 2, y0, z1 - 2, z0),
+      passRange(x1 - 2, x0, y0 + 1, y1 - 1, z1 - 2, z0),
+      passRange(x1 - 2, x0, y1 - 2, y0, z1 - 2, z0)};
+  for (int round = 0; round < 4; round++) {
+    for (const Sweep& s : passes) {
+      const long cells = (long)s.ni * s.nj * s.nk;
+      if (cells == 0) continue;
+      hipLaunchKernelGGL(kFillFirst, dim3(1), dim3(64), 0, ctx->stream, first);
+      hipLaunchKernelGGL(kPassFlags, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
+                         ctx->stream, b, s, m, ph, first);
+      hipLaunchKernelGGL(kPassWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, s, m, ph,
+                         (const int*)first, f);
+    }
+    // the faces: z (:547-552), y (:554-558), x (:560-564)
+    const long lines[3] = {(long)(y1 - y0) * (z1 - z0), (long)(x1 - x0) * (z1 - z0),
+                           (long)(x1 - x0) * (y1 - y0)};
+    for (int axis : {2, 1, 0})
+      hipLaunchKernelGGL(kFaceCopy, dim3(cdiv(lines[axis], 256)), dim3(256), 0, ctx->stream, b,
+                         axis, m, f);
+  }
+  ESF_HIP(hipMemcpyAsync(field, f, 24 * n, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}

@@ -238,6 +238,30 @@ class CoupledIntegrator3D:
         check(self.ctx, self.lib.esf_broad_phase(self.ctx, ctypes.byref(a)), "broadPhase")
         return a.value
 
+    def computeSDF(self, mass):
+        """DEFOR_SOLID_3D::computeSDF on a node mass field (Z, Y, X) of this
+        grid: (phi, box, tight), boxes as (xmin, ymin, zmin, xmax, ymax, zmax)"""
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        assert m.size == self.nodes, "mass must be a node field of the grid"
+        phi = np.empty_like(m)
+        box = np.zeros(6, dtype=np.int32)
+        tight = np.zeros(6, dtype=np.int32)
+        check(self.ctx, self.lib.esf_compute_sdf(self.ctx, dptr(m), dptr(phi), box.ctypes.data_as(_abi._I),
+                                                 tight.ctypes.data_as(_abi._I)), "computeSDF")
+        return phi, box, tight
+
+    def extendVectorField(self, box, mass, phi, field):
+        """DEFOR_SOLID_3D::extendVectorField of a (Z, Y, X, 3) node field where
+        mass == 0, over box; returns the extended field"""
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        ph = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.array(field, dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        assert m.size == self.nodes and ph.size == m.size and out.size == 3 * m.size
+        check(self.ctx, self.lib.esf_extend_vector_field(self.ctx, b.ctypes.data_as(_abi._I), dptr(m), dptr(ph),
+                                                         dptr(out)), "extendVectorField")
+        return out
+
     def writeFrame(self, data_path, frame, debug=False):
         """COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011)"""
         check(self.ctx, self.lib.esf_write_frame(self.ctx, data_path.encode(), frame,

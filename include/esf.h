@@ -217,6 +217,24 @@ esf_status esf_advect_surface_mesh(esf_ctx* ctx);
  * boxes (on the GPU; a BVH collision is exactly a pair of overlapping
  * triangle boxes); otherwise each solid's mass bounding box stands in. */
 esf_status esf_broad_phase(esf_ctx* ctx, double* alpha);
+/* DEFOR_SOLID_3D::computeSDF (DEFOR_SOLID_3D.cpp:216-267) on a node mass
+ * field of the context's grid (host arrays of X*Y*Z doubles, x fastest):
+ * phi = -0.5 where mass > 0, X + Y + Z + 2 elsewhere, then two rounds of
+ * sweepPhi (:397-454, solveDistance :369-395) over the box of the massive
+ * nodes widened by "extrapolation cells".  box and tight receive
+ * {xmin, ymin, zmin, xmax, ymax, zmax} as _xmin.._zmax and _tightMin /
+ * _tightMax ("tight border").  Bitwise the reference's sweeps, run on the
+ * GPU plane by plane. */
+esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int box[6],
+                           int tight[6]);
+/* DEFOR_SOLID_3D::extendVectorField (DEFOR_SOLID_3D.cpp:566-574): four
+ * rounds of sweepVectorField (:521-564; eight sweepOnePass :457-519, then
+ * the box faces) on a node field of 3 doubles per node, in place, where
+ * mass == 0; phi and box as computeSDF leaves them.  The box must lie in
+ * the grid and span at least 3 nodes per axis.  Bitwise the reference's
+ * sweeps, run on the GPU plane by plane. */
+esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double* mass,
+                                   const double* phi, double* field);
 /* recoverVelocityField + Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:894-916) */
 esf_status esf_recover_and_bc(esf_ctx* ctx);
 /* One substep of the drop-in: CFL(alpha) -> forces -> coupled solve ->

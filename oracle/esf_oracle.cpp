@@ -1954,3 +1954,186 @@ void orc_sphere_rest_fill(double radius, double mdh, double density, double* out
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// fast sweeping (TEST INFRASTRUCTURE): computeSDF, sweepPhi, solveDistance,
+// sweepOnePass, sweepVectorField, extendVectorField, restated in the
+// reference's loop order (DEFOR_SOLID_3D.cpp:216-267, 369-574)
+// ---------------------------------------------------------------------------
+// solveDistance (:369-395): the grid Eikonal update from the three upwind
+// neighbours, sorted; lowers v only
+double orc_solve_distance(double p, double q, double r, double v) {
+  double d[3];
+  d[0] = std::min(p, std::min(q, r));
+  d[2] = std::max(p, std::max(q, r));
+  d[1] = p + q + r - d[0] - d[2];
+  double b[3], c[3];
+  b[0] = d[0];
+  c[0] = d[0] * d[0] - 1.0;
+  for (int x = 1; x < 3; x++) {
+    b[x] = d[x] + b[x - 1];
+    c[x] = d[x] * d[x] + c[x - 1];
+  }
+  int i = 2;
+  double discrim = b[i] * b[i] - (double)(i + 1) * c[i];
+  double nd = (b[i] + std::sqrt(discrim)) / (double)(i + 1);
+  // the reference steps i down while the candidate is invalid; for finite
+  // inputs i = 0 always yields a valid one (discrim there is d0^2 - c0 = 1
+  // up to rounding), so i never reaches -1
+  while ((discrim < 0.0 || nd < d[i]) && i > 0) {
+    i--;
+    discrim = b[i] * b[i] - (double)(i + 1) * c[i];
+    if (discrim > 0.0) nd = (b[i] + std::sqrt(discrim)) / (double)(i + 1);
+  }
+  return nd < v ? nd : v;
+}
+
+namespace {
+struct SweepGrid {
+  int X, Y, Z;
+  long at(int i, int j, int k) const { return i + (long)X * (j + (long)Y * k); }
+};
+}  // namespace
+
+// sweepPhi (:397-454): eight ordered sweeps over the box, each updating the
+// cells without mass from the neighbours already visited
+void orc_sweep_phi(int X, int Y, int Z, const int box[6], const double* mass, double* phi) {
+  const SweepGrid g{X, Y, Z};
+  const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+  for (int s = 0; s < 8; s++) {
+    const int dx = (s & 2) ? -1 : 1, dy = (s & 1) ? -1 : 1, dz = (s & 4) ? -1 : 1;
+    // the reference's bounds: ascending from min + 1 to max - 1, descending
+    // from max - 2 down to min (inclusive)
+    const int ia = dx > 0 ? x0 + 1 : x1 - 2, ib = dx > 0 ? x1 : x0 - 1;
+    const int ja = dy > 0 ? y0 + 1 : y1 - 2, jb = dy > 0 ? y1 : y0 - 1;
+    const int ka = dz > 0 ? z0 + 1 : z1 - 2, kb = dz > 0 ? z1 : z0 - 1;
+    for (int k = ka; dz > 0 ? k < kb : k > kb; k += dz)
+      for (int j = ja; dy > 0 ? j < jb : j > jb; j += dy)
+        for (int i = ia; dx > 0 ? i < ib : i > ib; i += dx) {
+          const long c = g.at(i, j, k);
+          if (mass[c] == 0)
+            phi[c] = orc_solve_distance(phi[g.at(i - dx, j, k)], phi[g.at(i, j - dy, k)],
+                                        phi[g.at(i, j, k - dz)], phi[c]);
+        }
+  }
+}
+
+// computeSDF (:216-267): -0.5 on cells with mass, a large distance elsewhere,
+// the box of the massive cells widened by the tight border and by the
+// extrapolation cells, then two rounds of sweepPhi
+void orc_compute_sdf(int X, int Y, int Z, const double* mass, int tight_border, int extrap,
+                     double* phi, int box[6], int tight[6]) {
+  const double large = (double)(X + Y + Z + 2);
+  int mn[3] = {X, Y, Z}, mx[3] = {0, 0, 0};
+  long index = 0;
+  for (int z = 0; z < Z; z++)
+    for (int y = 0; y < Y; y++)
+      for (int x = 0; x < X; x++, index++) {
+        phi[index] = large;
+        if (mass[index] > 0) {
+          phi[index] = -0.5;
+          const int p[3] = {x, y, z};
+          for (int a = 0; a < 3; a++) {
+            mn[a] = std::min(mn[a], p[a]);
+            mx[a] = std::max(mx[a], p[a]);
+          }
+        }
+      }
+  const int res[3] = {X, Y, Z};
+  for (int a = 0; a < 3; a++) {
+    tight[a] = std::max(mn[a] - tight_border, 0);
+    tight[3 + a] = std::min(mx[a] + tight_border, res[a]);
+    box[a] = std::max(mn[a] - extrap, 0);
+    box[3 + a] = std::min(mx[a] + extrap, res[a]);
+  }
+  for (int r = 0; r < 2; r++) orc_sweep_phi(X, Y, Z, box, mass, phi);
+}
+
+// sweepOnePass (:457-519) over field (3 per node): i0 -> i1 exclusive with
+// the direction given by their order; solvableX/Y/Z start true and, once
+// cleared, stay cleared for the rest of the pass
+static void sweepOnePass(const SweepGrid& g, const double* mass, const double* phi,
+                         double* field, int i0, int i1, int j0, int j1, int k0, int k1) {
+  const int di = i0 < i1 ? 1 : -1, dj = j0 < j1 ? 1 : -1, dk = k0 < k1 ? 1 : -1;
+  bool sx = true, sy = true, sz = true;
+  for (int k = k0; k != k1; k += dk)
+    for (int j = j0; j != j1; j += dj)
+      for (int i = i0; i != i1; i += di) {
+        const long c = g.at(i, j, k);
+        if (mass[c] != 0) continue;
+        double dp = phi[c] - phi[g.at(i - di, j, k)];
+        const double dpn = phi[c] - phi[g.at(i + di, j, k)];
+        if (dp < 0) {
+          if (dpn >= 0) continue;
+          sx = false;
+        }
+        double dq = phi[c] - phi[g.at(i, j - dj, k)];
+        const double dqn = phi[c] - phi[g.at(i, j + dj, k)];
+        if (dq < 0) {
+          if (dqn >= 0) continue;
+          sy = false;
+        }
+        double dr = phi[c] - phi[g.at(i, j, k - dk)];
+        const double drn = phi[c] - phi[g.at(i, j, k + dk)];
+        if (dr < 0) {
+          if (drn >= 0) continue;
+          sz = false;
+        }
+        if (!sx) dp = 0;
+        if (!sy) dq = 0;
+        if (!sz) dr = 0;
+        double alpha, beta;
+        if (dp + dq + dr < 1e-3) {
+          alpha = 0.333;
+          beta = 0.333;
+        } else {
+          alpha = dp / (dp + dq + dr);
+          beta = dq / (dp + dq + dr);
+        }
+        const long a = 3 * g.at(i - di, j, k), b = 3 * g.at(i, j - dj, k),
+                   e = 3 * g.at(i, j, k - dk);
+        for (int q = 0; q < 3; q++)
+          field[3 * c + q] = alpha * field[a + q] + beta * field[b + q] +
+                             (1 - alpha - beta) * field[e + q];
+      }
+}
+
+// sweepVectorField (:521-564): eight passes, then the box faces copied from
+// the layer inside (z faces, then y, then x) where the node has no mass
+static void sweepVectorField(const SweepGrid& g, const int box[6], const double* mass,
+                             const double* phi, double* field) {
+  const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+  sweepOnePass(g, mass, phi, field, x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0 + 1, z1 - 1);
+  sweepOnePass(g, mass, phi, field, x0 + 1, x1 - 1, y1 - 2, y0, z0 + 1, z1 - 1);
+  sweepOnePass(g, mass, phi, field, x1 - 2, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1);
+  sweepOnePass(g, mass, phi, field, x1 - 2, x0, y1 - 2, y0, z0 + 1, z1 - 1);
+  sweepOnePass(g, mass, phi, field, x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1 - 2, z0);
+  sweepOnePass(g, mass, phi, field, x0 + 1, x1 - 1, y1 - 2, y0, z1 - 2, z0);
+  sweepOnePass(g, mass, phi, field, x1 - 2, x0, y0 + 1, y1 - 1, z1 - 2, z0);
+  sweepOnePass(g, mass, phi, field, x1 - 2, x0, y1 - 2, y0, z1 - 2, z0);
+  auto copy = [&](long to, long from) {
+    if (mass[to] == 0)
+      for (int q = 0; q < 3; q++) field[3 * to + q] = field[3 * from + q];
+  };
+  for (int i = x0; i < x1; i++)
+    for (int j = y0; j < y1; j++) {
+      copy(g.at(i, j, z0), g.at(i, j, z0 + 1));
+      copy(g.at(i, j, z1 - 1), g.at(i, j, z1 - 2));
+    }
+  for (int i = x0; i < x1; i++)
+    for (int k = z0; k < z1; k++) {
+      copy(g.at(i, y0, k), g.at(i, y0 + 1, k));
+      copy(g.at(i, y1 - 1, k), g.at(i, y1 - 2, k));
+    }
+  for (int j = y0; j < y1; j++)
+    for (int k = z0; k < z1; k++) {
+      copy(g.at(x0, j, k), g.at(x0 + 1, j, k));
+      copy(g.at(x1 - 1, j, k), g.at(x1 - 2, j, k));
+    }
+}
+
+void orc_extend_vector_field(int X, int Y, int Z, const int box[6], const double* mass,
+                             const double* phi, double* field) {
+  const SweepGrid g{X, Y, Z};
+  for (int r = 0; r < 4; r++) sweepVectorField(g, box, mass, phi, field);
+}

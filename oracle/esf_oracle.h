@@ -159,6 +159,20 @@ int orc_num_threads(void);
  * |p| - radius: dims and origin first, then the data (X*Y*Z doubles) */
 void orc_sphere_rest_dims(double radius, double mass_dh, int dims[3], double origin[3]);
 void orc_sphere_rest_fill(double radius, double mass_dh, double density, double* out);
+/* ---- fast sweeping (TEST INFRASTRUCTURE) ----------------------------------
+ * computeSDF (DEFOR_SOLID_3D.cpp:216-267) with sweepPhi / solveDistance
+ * (:369-454), and extendVectorField (:566-574) with sweepVectorField
+ * (:521-564) and sweepOnePass (:457-519), restated on plain node arrays of
+ * X*Y*Z (x fastest).  box = {xmin, ymin, zmin, xmax, ymax, zmax} as the
+ * reference's _xmin.._zmax (the max ends exclusive in its loops), tight the
+ * same for _tightMin/_tightMax. */
+void orc_compute_sdf(int X, int Y, int Z, const double* mass, int tight_border, int extrap,
+                     double* phi, int box[6], int tight[6]);
+void orc_sweep_phi(int X, int Y, int Z, const int box[6], const double* mass, double* phi);
+void orc_extend_vector_field(int X, int Y, int Z, const int box[6], const double* mass,
+                             const double* phi, double* field);
+double orc_solve_distance(double p, double q, double r, double v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,12 @@ def lib():
         L.orc_mesh_broad_phase.argtypes = [ctypes.c_int, _I, ctypes.POINTER(ctypes.c_long),
                                            ctypes.POINTER(ctypes.c_long), _D, _I, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_double, _D]
+        _i3 = [ctypes.c_int] * 3
+        L.orc_solve_distance.restype = ctypes.c_double
+        L.orc_solve_distance.argtypes = [ctypes.c_double] * 4
+        L.orc_sweep_phi.argtypes = _i3 + [_I, _D, _D]
+        L.orc_compute_sdf.argtypes = _i3 + [_D, ctypes.c_int, ctypes.c_int, _D, _I, _I]
+        L.orc_extend_vector_field.argtypes = _i3 + [_I, _D, _D, _D]
         _lib = L
     return _lib
 
@@ -319,3 +325,42 @@ def mesh_broad_phase(grid, meshes):
         len(meshes), nt.ctypes.data_as(_I), vo.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
         to.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), dp(v), t.ctypes.data_as(_I), X, Y, Z,
         dh, dp(oo))
+
+
+def solve_distance(p, q, r, v):
+    """solveDistance (DEFOR_SOLID_3D.cpp:369-395): the grid Eikonal update"""
+    return lib().orc_solve_distance(p, q, r, v)
+
+
+def compute_sdf(mass, tight_border=3, extrap=20):
+    """computeSDF (DEFOR_SOLID_3D.cpp:216-267) on a (Z, Y, X) node mass
+    array: (phi, box, tight), boxes as (xmin, ymin, zmin, xmax, ymax, zmax)"""
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    Z, Y, X = m.shape
+    phi = np.empty_like(m)
+    box = np.zeros(6, dtype=np.int32)
+    tight = np.zeros(6, dtype=np.int32)
+    lib().orc_compute_sdf(X, Y, Z, dp(m), tight_border, extrap, dp(phi),
+                          box.ctypes.data_as(_I), tight.ctypes.data_as(_I))
+    return phi, box, tight
+
+
+def sweep_phi(mass, phi, box):
+    """one sweepPhi (DEFOR_SOLID_3D.cpp:397-454) over the box, in place on a copy"""
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    Z, Y, X = m.shape
+    out = np.array(phi, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(box, dtype=np.int32)
+    lib().orc_sweep_phi(X, Y, Z, b.ctypes.data_as(_I), dp(m), dp(out))
+    return out
+
+
+def extend_vector_field(mass, phi, box, field):
+    """extendVectorField (DEFOR_SOLID_3D.cpp:566-574) of a (Z, Y, X, 3) field"""
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    Z, Y, X = m.shape
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.array(field, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(box, dtype=np.int32)
+    lib().orc_extend_vector_field(X, Y, Z, b.ctypes.data_as(_I), dp(m), dp(ph), dp(out))
+    return out

@@ -1,0 +1,104 @@
+"""The oracle's fast sweeping (computeSDF / sweepPhi / solveDistance and
+extendVectorField, DEFOR_SOLID_3D.cpp:216-267, 369-574) pinned by what the
+mathematics fixes: the grid Eikonal update's closed forms in one, two and
+three dimensions, distances along the axes from a single massive node, the
+box arithmetic, and for the extension: massive nodes untouched, constants
+kept, and a field that varies only across a massive plane carried along the
+axis that phi grows on."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 2.25])
+def test_solve_distance_closed_forms(a):
+    big = 1e6
+    # one upwind neighbour: a + 1; two equal: a + 1/sqrt(2); three equal: a + 1/sqrt(3)
+    assert O.solve_distance(a, big, big, 1e9) == pytest.approx(a + 1, abs=1e-14)
+    assert O.solve_distance(a, a, big, 1e9) == pytest.approx(a + 1 / math.sqrt(2), abs=1e-14)
+    assert O.solve_distance(a, a, a, 1e9) == pytest.approx(a + 1 / math.sqrt(3), abs=1e-14)
+    # two neighbours a full step apart: the 1D value wins (the other is not upwind)
+    assert O.solve_distance(a, a + 1, big, 1e9) == pytest.approx(a + 1, abs=1e-14)
+
+
+def test_solve_distance_is_symmetric_and_only_lowers():
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        p, q, r = rng.uniform(0, 5, 3)
+        v = O.solve_distance(p, q, r, 1e9)
+        # symmetric up to the rounding of p + q + r (the middle value is formed
+        # from that sum, in argument order)
+        assert O.solve_distance(q, r, p, 1e9) == pytest.approx(v, abs=1e-14)
+        assert O.solve_distance(r, p, q, 1e9) == pytest.approx(v, abs=1e-14)
+        assert O.solve_distance(p, q, r, v / 2) == v / 2
+        # the Eikonal solution satisfies sum over upwind (v - d)^2 = 1
+        ds = [d for d in (p, q, r) if d < v]
+        assert sum((v - d) ** 2 for d in ds) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_compute_sdf_single_node():
+    n, c, tb, ex = 20, (9, 8, 10), 3, 5
+    m = np.zeros((n, n, n))
+    m[c[2], c[1], c[0]] = 1.0
+    phi, box, tight = O.compute_sdf(m, tb, ex)
+    assert list(box) == [c[0] - ex, c[1] - ex, c[2] - ex, c[0] + ex, c[1] + ex, c[2] + ex]
+    assert list(tight) == [c[0] - tb, c[1] - tb, c[2] - tb, c[0] + tb, c[1] + tb, c[2] + tb]
+    x, y, z = c
+    assert phi[z, y, x] == -0.5
+    for k in range(1, ex):  # inside the box along each axis: k - 0.5
+        assert phi[z, y, x + k] == pytest.approx(k - 0.5, abs=1e-14)
+        assert phi[z, y - k, x] == pytest.approx(k - 0.5, abs=1e-14)
+        assert phi[z + k, y, x] == pytest.approx(k - 0.5, abs=1e-14)
+    # the diagonal neighbours by the 2D and 3D updates
+    d2 = 0.5 + 1 / math.sqrt(2)
+    assert phi[z, y + 1, x + 1] == pytest.approx(d2, abs=1e-14)
+    assert phi[z + 1, y + 1, x + 1] == pytest.approx(d2 + 1 / math.sqrt(3), abs=1e-14)
+    # outside the swept box the large initial distance stays
+    assert phi[z, y, x + ex] == 3 * n + 2
+
+
+def test_compute_sdf_box_clamps_to_the_grid():
+    n = 16
+    m = np.zeros((n, n, n))
+    m[1, 14, 2] = 1.0
+    _, box, tight = O.compute_sdf(m, 3, 20)
+    assert list(box) == [0, 0, 0, n, n, n]
+    assert list(tight) == [0, 11, 0, 5, n, 4]
+
+
+def _slab_scene(n=14, plane=3):
+    m = np.zeros((n, n, n))
+    m[:, :, : plane + 1] = 1.0
+    phi, box, _ = O.compute_sdf(m, 3, 20)
+    return m, phi, box
+
+
+def test_extend_keeps_massive_nodes_and_constants():
+    m, phi, box = _slab_scene()
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=m.shape + (3,))
+    out = O.extend_vector_field(m, phi, box, f)
+    assert np.array_equal(out[m > 0], f[m > 0])
+    c = np.empty_like(f)
+    c[...] = (0.3, -1.25, 2.0)
+    oc = O.extend_vector_field(m, phi, box, c)
+    assert np.abs(oc - c).max() <= 4e-16 * 2.0
+
+
+def test_extend_carries_the_plane_along_x():
+    """phi grows along x only, so every pass takes alpha = 1 (the x
+    neighbour): the field on the last massive plane is carried out along x"""
+    n, plane = 14, 3
+    m, phi, box = _slab_scene(n, plane)
+    yy, zz = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    f = np.zeros((n, n, n, 3))
+    f[:, :, plane, 0] = yy + 10 * zz
+    f[:, :, plane, 1] = -zz
+    f[:, :, plane, 2] = 0.5 * yy
+    out = O.extend_vector_field(m, phi, box, f)
+    inner = slice(1, n - 1)
+    for x in range(plane + 1, n - 1):
+        assert np.array_equal(out[inner, inner, x], f[inner, inner, plane]), x
