@@ -99,10 +99,14 @@ __global__ void __launch_bounds__(kWaveThreads) kPhiWave(SweepBox b, Sweep s,
                                                         const double* __restrict__ mass,
                                                         double* phi) {
   const int planes = s.ni + s.nj + s.nk - 2;
-  const int jk = s.nj * s.nk;
   for (int p = 0; p < planes; p++) {
+    // the (j', k') rectangle that can hold a node of plane p
+    const int jLo = max(0, p - (s.ni - 1) - (s.nk - 1)), jHi = min(s.nj - 1, p);
+    const int kLo = max(0, p - (s.ni - 1) - (s.nj - 1)), kHi = min(s.nk - 1, p);
+    const int wj = jHi - jLo + 1, wk = kHi - kLo + 1;
+    const int jk = wj > 0 && wk > 0 ? wj * wk : 0;
     for (int t = threadIdx.x; t < jk; t += blockDim.x) {
-      const int jp = t % s.nj, kp = t / s.nj, ip = p - jp - kp;
+      const int jp = jLo + t % wj, kp = kLo + t / wj, ip = p - jp - kp;
       if (ip < 0 || ip >= s.ni) continue;
       const int i = s.i0 + s.di * ip, j = s.j0 + s.dj * jp, k = s.k0 + s.dk * kp;
       const long c = nodeAt(b, i, j, k);
@@ -170,11 +174,14 @@ __global__ void __launch_bounds__(kWaveThreads) kPassWave(SweepBox b, Sweep s,
                                                          const int* __restrict__ first,
                                                          double* field) {
   const int planes = s.ni + s.nj + s.nk - 2;
-  const int jk = s.nj * s.nk;
   const int fx = first[0], fy = first[1], fz = first[2];
   for (int p = 0; p < planes; p++) {
+    const int jLo = max(0, p - (s.ni - 1) - (s.nk - 1)), jHi = min(s.nj - 1, p);
+    const int kLo = max(0, p - (s.ni - 1) - (s.nj - 1)), kHi = min(s.nk - 1, p);
+    const int wj = jHi - jLo + 1, wk = kHi - kLo + 1;
+    const int jk = wj > 0 && wk > 0 ? wj * wk : 0;
     for (int t = threadIdx.x; t < jk; t += blockDim.x) {
-      const int jp = t % s.nj, kp = t / s.nj, ip = p - jp - kp;
+      const int jp = jLo + t % wj, kp = kLo + t / wj, ip = p - jp - kp;
       if (ip < 0 || ip >= s.ni) continue;
       const int i = s.i0 + s.di * ip, j = s.j0 + s.dj * jp, k = s.k0 + s.dk * kp;
       PassNode o;
