@@ -110,9 +110,11 @@ __global__ void __launch_bounds__(kWaveThreads) kPhiWave(SweepBox b, Sweep s,
       if (ip < 0 || ip >= s.ni) continue;
       const int i = s.i0 + s.di * ip, j = s.j0 + s.dj * jp, k = s.k0 + s.dk * kp;
       const long c = nodeAt(b, i, j, k);
-      if (mass[c] == 0)
-        phi[c] = solveDistance(phi[nodeAt(b, i - s.di, j, k)], phi[nodeAt(b, i, j - s.dj, k)],
-                               phi[nodeAt(b, i, j, k - s.dk)], phi[c]);
+      // the mass and the four phi values in one round of loads
+      const double mc = mass[c], pa = phi[nodeAt(b, i - s.di, j, k)],
+                   pb = phi[nodeAt(b, i, j - s.dj, k)], pe = phi[nodeAt(b, i, j, k - s.dk)],
+                   pc = phi[c];
+      if (mc == 0) phi[c] = solveDistance(pa, pb, pe, pc);
     }
     __syncthreads();
   }
@@ -167,14 +169,40 @@ __global__ void kPassFlags(SweepBox b, Sweep s, const double* __restrict__ mass,
   }
 }
 
+// every node's coefficients for the pass, from the flags' first clearing
+// nodes: alpha and beta where the node is updated, NaN in alpha where it
+// is not (indexed by loop position), so the wave below reads them together
+// with the neighbours' field values in one round of loads
+__global__ void kPassCoef(SweepBox b, Sweep s, const double* __restrict__ mass,
+                          const double* __restrict__ phi, const int* __restrict__ first,
+                          double2* coef) {
+  const long n = (long)s.ni * s.nj * s.nk;
+  const int fx = first[0], fy = first[1], fz = first[2];
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int ip = (int)(t % s.ni), jp = (int)((t / s.ni) % s.nj), kp = (int)(t / ((long)s.ni * s.nj));
+    PassNode o;
+    double2 ab = make_double2(NAN, 0.0);
+    if (passNode(b, s, mass, phi, s.i0 + s.di * ip, s.j0 + s.dj * jp, s.k0 + s.dk * kp, o)) {
+      // a flag is cleared from its first clearing node on, that node included
+      const double dp = t >= fx ? 0.0 : o.dp;
+      const double dq = t >= fy ? 0.0 : o.dq;
+      const double dr = t >= fz ? 0.0 : o.dr;
+      if (dp + dq + dr < 1e-3) {
+        ab = make_double2(0.333, 0.333);
+      } else {
+        ab = make_double2(dp / (dp + dq + dr), dq / (dp + dq + dr));
+      }
+    }
+    coef[t] = ab;
+  }
+}
+
 // the pass itself, one workgroup, plane by plane
 __global__ void __launch_bounds__(kWaveThreads) kPassWave(SweepBox b, Sweep s,
-                                                         const double* __restrict__ mass,
-                                                         const double* __restrict__ phi,
-                                                         const int* __restrict__ first,
+                                                         const double2* __restrict__ coef,
                                                          double* field) {
   const int planes = s.ni + s.nj + s.nk - 2;
-  const int fx = first[0], fy = first[1], fz = first[2];
   for (int p = 0; p < planes; p++) {
     const int jLo = max(0, p - (s.ni - 1) - (s.nk - 1)), jHi = min(s.nj - 1, p);
     const int kLo = max(0, p - (s.ni - 1) - (s.nj - 1)), kHi = min(s.nk - 1, p);
@@ -184,27 +212,20 @@ __global__ void __launch_bounds__(kWaveThreads) kPassWave(SweepBox b, Sweep s,
       const int jp = jLo + t % wj, kp = kLo + t / wj, ip = p - jp - kp;
       if (ip < 0 || ip >= s.ni) continue;
       const int i = s.i0 + s.di * ip, j = s.j0 + s.dj * jp, k = s.k0 + s.dk * kp;
-      PassNode o;
-      if (!passNode(b, s, mass, phi, i, j, k, o)) continue;
-      // a flag is cleared from its first clearing node on, that node included
-      const int ord = (kp * s.nj + jp) * s.ni + ip;
-      const double dp = ord >= fx ? 0.0 : o.dp;
-      const double dq = ord >= fy ? 0.0 : o.dq;
-      const double dr = ord >= fz ? 0.0 : o.dr;
-      double alpha, beta;
-      if (dp + dq + dr < 1e-3) {
-        alpha = 0.333;
-        beta = 0.333;
-      } else {
-        alpha = dp / (dp + dq + dr);
-        beta = dq / (dp + dq + dr);
-      }
-      const double gamma = 1 - alpha - beta;
+      const double2 ab = coef[(kp * s.nj + jp) * (long)s.ni + ip];
       const long c = 3 * nodeAt(b, i, j, k), a = 3 * nodeAt(b, i - s.di, j, k),
                  bb = 3 * nodeAt(b, i, j - s.dj, k), e = 3 * nodeAt(b, i, j, k - s.dk);
+      double fa[3], fb[3], fe[3];
 #pragma unroll
-      for (int q = 0; q < 3; q++)
-        field[c + q] = alpha * field[a + q] + beta * field[bb + q] + gamma * field[e + q];
+      for (int q = 0; q < 3; q++) {
+        fa[q] = field[a + q];
+        fb[q] = field[bb + q];
+        fe[q] = field[e + q];
+      }
+      if (isnan(ab.x)) continue;  // not updated in this pass
+      const double alpha = ab.x, beta = ab.y, gamma = 1 - alpha - beta;
+#pragma unroll
+      for (int q = 0; q < 3; q++) field[c + q] = alpha * fa[q] + beta * fb[q] + gamma * fe[q];
     }
     __syncthreads();
   }
@@ -345,10 +366,12 @@ esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double*
   if (st) return st;
   const Dims& d = ctx->d;
   const long n = d.nodes;
-  DevBuf dm, dphi, dfield, dfirst;
+  DevBuf dm, dphi, dfield, dfirst, dcoef;
+  const long boxNodes = (long)(box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]);
   if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
       hipMalloc(&dfield.p, 24 * n) != hipSuccess ||
-      hipMalloc(&dfirst.p, 3 * sizeof(int)) != hipSuccess) {
+      hipMalloc(&dfirst.p, 3 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&dcoef.p, 16 * boxNodes) != hipSuccess) {
     ctx->err = "extend_vector_field: device allocation failed";
     return ESF_ERR_OOM;
   }
@@ -378,8 +401,10 @@ esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double*
       hipLaunchKernelGGL(kFillFirst, dim3(1), dim3(64), 0, ctx->stream, first);
       hipLaunchKernelGGL(kPassFlags, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
                          ctx->stream, b, s, m, ph, first);
-      hipLaunchKernelGGL(kPassWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, s, m, ph,
-                         (const int*)first, f);
+      hipLaunchKernelGGL(kPassCoef, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
+                         ctx->stream, b, s, m, ph, (const int*)first, (double2*)dcoef.p);
+      hipLaunchKernelGGL(kPassWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, s,
+                         (const double2*)dcoef.p, f);
     }
     // the faces: z (:547-552), y (:554-558), x (:560-564)
     const long lines[3] = {(long)(y1 - y0) * (z1 - z0), (long)(x1 - x0) * (z1 - z0),
