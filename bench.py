@@ -169,6 +169,46 @@ def cpu_baseline(text, g_fp64, esf, seconds, max_steps):
                       f"{el:.1f} s wall, OpenMP threads={cores}"}
 
 
+def sweeps(esf, cfg, n, reps):
+    """The fast-sweeping entry points on the n^3 ball's mass (the headline
+    scene's sphere, r = 0.15 at (0.5, 0.65, 0.5)): esf_compute_sdf and
+    esf_extend_vector_field of a random field, wall time per call with the
+    host copies, best of `reps`, beside the oracle on one host thread
+    (checker only), and whether the two agree bitwise"""
+    N = n + 1
+    g = esf.CoupledIntegrator3D.from_cfg_text(cfg(n))
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    m = (np.sqrt((x / n - 0.5) ** 2 + (y / n - 0.65) ** 2 + (z / n - 0.5) ** 2) < 0.15).astype(
+        np.float64)
+    f = np.random.default_rng(1).normal(size=(N, N, N, 3))
+    t_sdf, t_ext = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        phi, box, tight = g.computeSDF(m)
+        t_sdf.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        out = g.extendVectorField(box, m, phi, f)
+        t_ext.append(time.perf_counter() - t)
+    g.close()
+    from oracle import oracle as O
+    t = time.perf_counter()
+    phi_o, box_o, _ = O.compute_sdf(m, 5, 20)
+    c_sdf = time.perf_counter() - t
+    t = time.perf_counter()
+    out_o = O.extend_vector_field(m, phi_o, box_o, f)
+    c_ext = time.perf_counter() - t
+    print(json.dumps({
+        "workload": f"fast sweeping on the {n}^3 ball's mass, box {[int(v) for v in box]}",
+        "gpu_ms": {"compute_sdf": round(1e3 * min(t_sdf), 2),
+                   "extend_vector_field": round(1e3 * min(t_ext), 2)},
+        "cpu_oracle_ms_1_thread": {"compute_sdf": round(1e3 * c_sdf, 2),
+                                   "extend_vector_field": round(1e3 * c_ext, 2)},
+        "host_copies_included": True,
+        "bitwise": {"phi": bool(np.array_equal(phi.view(np.int64), phi_o.view(np.int64))),
+                    "field": bool(np.array_equal(out.view(np.int64), out_o.view(np.int64)))},
+    }), flush=True)
+
+
 def spmv_only(esf, cfg, n, reps):
     g = esf.CoupledIntegrator3D.from_cfg_text(cfg(n), esf.ESF_MIXED)
     g.forces()
@@ -375,6 +415,9 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="arrival window of the short served-stream block of the default "
                          "run (0 to skip)")
+    ap.add_argument("--sweep", type=int, default=0, metavar="N",
+                    help="only time the fast-sweeping entry points on the N^3 ball; prints one "
+                         "JSON line")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="N",
                     help="only time the pressure SpMV (FP64, FP32) on the N^3 scene and a "
                          "streaming copy of the same byte count; prints one JSON line")
@@ -394,6 +437,9 @@ def main():
     from eulersf_amd.cfg import unit_cube_ball_cfg
     if args.spmv_only:
         spmv_only(esf, unit_cube_ball_cfg, args.spmv_only, args.spmv_reps)
+        return
+    if args.sweep:
+        sweeps(esf, unit_cube_ball_cfg, args.sweep, 3)
         return
     if args.system:
         isolated_system(esf, args.system, args.seed, args.eps, args.system_reps)
