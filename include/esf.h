@@ -238,10 +238,15 @@ esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double*
 /* recoverVelocityField + Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:894-916) */
 esf_status esf_recover_and_bc(esf_ctx* ctx);
 /* One substep of the drop-in: CFL(alpha) -> forces -> coupled solve ->
- * solid velocity copy -> solid finalize -> gather solid masses -> advect
- * velocity -> advect intensity -> recover + Dirichlet.  The stages of
- * COUPLED_INTEGRATOR_3D::finalizeSolution that stay on the host (FLIP
- * particles, mass resampling, SDF, surface mesh; :833-879) are not run. */
+ * solid velocity copy -> solid finalize -> surface meshes (solids that have
+ * one) -> gather solid masses -> advect velocity -> advect intensity ->
+ * recover + Dirichlet.  The stages of COUPLED_INTEGRATOR_3D::finalizeSolution
+ * that stay with the host integrator (FLIP particles, mass resampling, SDF
+ * and velocity extension -- the last two callable as esf_compute_sdf and
+ * esf_extend_vector_field; :833-879) are not run.  A step that runs alone
+ * in the process computes the fluid's vorticity and body forces on a second
+ * stream of the context, beside the first solid solve; results are the same
+ * either way. */
 esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
 
 /* --- operators (unit level) ---------------------------------------------*/
