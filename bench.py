@@ -143,30 +143,57 @@ def spmv_roofline(g, n_cells, n_int, fp32, reps):
             "algorithmic_bytes": bytes_, "avg_launch_ms": round(ms, 5)}
 
 
-def cpu_baseline(text, g_fp64, esf, seconds, max_steps):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(text, label, g_fp64, esf, alpha, warmup, timed, seconds):
+    """The CPU oracle on the same scene: its state set from the GPU FP64
+    context after the timed steps (grid fields, warm start, the solid's
+    fields, mass and boxes), `warmup` untimed steps, then up to `timed`
+    steps timed one by one (fewer if `seconds` run out); the median step."""
+    import ctypes
     from oracle import oracle as O
     o = O.OracleScene.from_cfg_text(text)
     A = esf._abi
-    # teacher forcing: start from the GPU FP64 state after its timed steps
     o.set_field(O.VELOCITY, g_fp64.download(A.F_VELOCITY))
     o.set_field(O.INTENSITY, g_fp64.download(A.F_INTENSITY))
     o.set_field(O.LAMBDA, g_fp64.download(A.F_LAMBDA))
+    b = (ctypes.c_int * 12)(*g_fp64.boxes(0))
+    o.L.orc_set_boxes(o.h, 0, b)
     for f_o, f_g in ((O.S_U, A.S_U), (O.S_X, A.S_X), (O.S_VELOCITY, A.S_VELOCITY),
-                     (O.S_VEL_WORKSPACE, A.S_VEL_WORKSPACE)):
+                     (O.S_VEL_WORKSPACE, A.S_VEL_WORKSPACE), (O.S_MASS, A.S_MASS),
+                     (O.S_QFRAC, A.S_QFRAC), (O.S_VALID, A.S_VALID), (O.S_PHI, A.S_PHI)):
         o.set_field(f_o, g_fp64.download(f_g, 0), 0)
-    n = 0
+    o.L.orc_gather(o.h)
+    for _ in range(warmup):
+        o.step(alpha)
+    times = []
     t0 = time.perf_counter()
-    while n < max_steps:
-        o.step(-1.0)
-        n += 1
+    while len(times) < timed:
+        s0 = time.perf_counter()
+        o.step(alpha)
+        times.append(time.perf_counter() - s0)
         if time.perf_counter() - t0 > seconds:
             break
-    el = time.perf_counter() - t0
     cores = O.lib().orc_num_threads()
     o.close()
-    return {"value": round(n / el, 4), "unit": "coupled steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} oracle step(s) of the same 128^3 scene from the GPU's warm state, "
-                      f"{el:.1f} s wall, OpenMP threads={cores}"}
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 4), "unit": "coupled steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"median of {len(times)} oracle steps (after {warmup} warm-up steps) of the "
+                      f"same {label} scene from the GPU FP64 context's state; step times "
+                      f"{[round(t, 3) for t in times]} s",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
+                                                   "OMP_PLACES")}}
 
 
 def sweeps(esf, cfg, n, reps):
@@ -333,6 +360,7 @@ def served(esf, seconds, rate, workers, seed, one_at_a_time=True):
     out = {"workload": f"Poisson arrivals at {rate}/s for {seconds} s (seed {seed}): N ~ U[32,160], "
                        "K ~ U[10,100] steps, 1-3 spheres, mixed precision at 1e-8, "
                        "200 GB HBM budget", "requests": len(reqs)}
+    out["hw_queues"] = os.environ.get("GPU_MAX_HW_QUEUES", "runtime default (4)")
     srv = Server(200e9, workers, esf.ESF_MIXED)
     recs, span, info = srv.run(reqs)
     out["concurrent"] = {"slots": workers, **summarize(recs, span), **info}
@@ -391,8 +419,10 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--roofline-n", type=int, default=256,
                     help="grid of the HBM-bound SpMV measurement (0 to skip)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="bound on the CPU baseline's timed steps")
+    ap.add_argument("--cpu-warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-run", action="store_true",
                     help="a few steps of both forms and a calibrating copy, for the "
@@ -428,10 +458,10 @@ def main():
               "the coupled step has no multi-GPU path in this build", file=sys.stderr)
         sys.exit(2)
 
-    # concurrent scenes each own a HIP stream; give the runtime as many
-    # hardware queues as there are concurrent scenes (at most 16)
-    if args.batch or args.serve or args.served_seconds:
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(args.workers, 16))))
+    # the CPU baseline's OpenMP threads, bound one per core (read when the
+    # oracle library loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import eulersf_amd as esf
     esf.load()
     from eulersf_amd.cfg import unit_cube_ball_cfg
@@ -503,13 +533,14 @@ def main():
 
     serve_block = None
     if args.served_seconds > 0:
-        serve_block = served(esf, args.served_seconds, args.serve_rate, args.workers, args.seed,
-                             one_at_a_time=False)
+        serve_block = served(esf, args.served_seconds, args.serve_rate, args.workers, args.seed)
 
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] CPU oracle baseline (<= {args.cpu_steps} steps / {args.cpu_seconds} s)")
-        cpu = cpu_baseline(text, g64, esf, args.cpu_seconds, args.cpu_steps)
+        log(f"[bench] CPU oracle baseline ({args.cpu_warmup} warm-up + median of "
+            f"{args.cpu_steps} steps)")
+        cpu = cpu_baseline(text, f"{n}^3", g64, esf, -1.0, args.cpu_warmup, args.cpu_steps,
+                           args.cpu_seconds)
         log(f"[bench] CPU: {cpu['value']} steps/s on {cpu['cores']} threads")
 
     def summary(lat, iters):

@@ -47,6 +47,7 @@ S_MASS = 103
 S_VALID = 104
 S_QFRAC = 105
 S_VEL_WORKSPACE = 106
+S_PHI = 108
 
 KERNEL_CLASSES = ["spmv64_apply", "spmv64_setup_r", "spmv64_setup_s", "spmv64_update",
                   "spmv64_check", "spmv64_ir_resid", "spmv32_apply", "spmv32_setup_r",
@@ -55,7 +56,7 @@ KERNEL_CLASSES = ["spmv64_apply", "spmv64_setup_r", "spmv64_setup_s", "spmv64_up
                   "solid_cell_crx", "solid_node_crx"]
 
 STAGES = ["cfl", "forces", "solid_solves", "pressure_solve", "solid_finalize", "gather",
-          "advect_velocity", "advect_intensity", "recover_bc"]
+          "advect_velocity", "advect_intensity", "recover_bc", "solid_mass_sdf"]
 
 
 class Grid(ctypes.Structure):
@@ -123,6 +124,8 @@ _PROTOS = {
     "esf_download": (_S, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_size_t]),
     "esf_compute_solid_mass": (_S, [_P, ctypes.c_int]),
     "esf_update_boxes": (_S, [_P, ctypes.c_int]),
+    "esf_compute_solid_mass_sdf": (_S, [_P, ctypes.c_int]),
+    "esf_solid_sdf": (_S, [_P, ctypes.c_int]),
     "esf_gather_solid_masses": (_S, [_P]),
     "esf_recover_velocity": (_S, [_P]),
     "esf_cfl": (_S, [_P, ctypes.c_double, _D]),

@@ -110,7 +110,8 @@ def mass_sample_dh(n):
 
 
 def scene_cfg(n, spheres, fill=0.3, eps=1e-10, max_iterations=20000, fluid_density=1000.0,
-              dt=1.0 / 240.0, vorticity_eps=1.5, gravity=9.8, frames=250):
+              dt=1.0 / 240.0, vorticity_eps=1.5, gravity=9.8, frames=250, buoyancy=0.0,
+              rayleigh_alpha=0.0, rayleigh_beta=0.0):
     """cfg text of a unit-cube scene on an n^3-cell grid ((n+1)^3 nodes,
     dh = 1/n): the smoke carpet (fluid pool) fills the bottom `fill` of the
     domain; `spheres` is a list of dicts with keys center, radius, E, nu,
@@ -138,11 +139,12 @@ def scene_cfg(n, spheres, fill=0.3, eps=1e-10, max_iterations=20000, fluid_densi
         ]
     lines += [
         "fluid density = %r" % fluid_density, "fluid viscocity = 0",
-        "vorticity eps = %r" % vorticity_eps, "buoyancy = 0",
+        "vorticity eps = %r" % vorticity_eps, "buoyancy = %r" % float(buoyancy),
         "smoke carpet thickness = %r" % float(fill),
         "dt = %r" % dt, "gravity = %r" % gravity,
         "extrapolation cells = 20", "tight border = 5",
-        "rayleigh alpha = 0", "rayleigh beta = 0",
+        "rayleigh alpha = %r" % float(rayleigh_alpha),
+        "rayleigh beta = %r" % float(rayleigh_beta),
         "cr max iterations = %d" % max_iterations,
         "implicit solver eps = %r" % eps, "div free solver eps = %r" % eps,
         "mass sample dh = %r" % sdh,
@@ -155,14 +157,16 @@ def scene_cfg(n, spheres, fill=0.3, eps=1e-10, max_iterations=20000, fluid_densi
 def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
                        center=(0.5, 0.65, 0.5), E=1e4, nu=0.3, density=1000.0,
                        fluid_density=1000.0, carpet=0.3, dt=1.0 / 240.0,
-                       vorticity_eps=1.5, gravity=9.8, frames=250, velocity=(0, 0, 0)):
+                       vorticity_eps=1.5, gravity=9.8, frames=250, velocity=(0, 0, 0),
+                       buoyancy=0.0, rayleigh_alpha=0.0, rayleigh_beta=0.0):
     """cfg text of the unit-cube ball scene on an n^3-cell grid: (n+1)^3 nodes,
     dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool."""
     sphere = {"center": center, "radius": radius, "E": E, "nu": nu, "density": density,
               "velocity": velocity}
     return scene_cfg(n, [sphere], fill=carpet, eps=eps, max_iterations=max_iterations,
                      fluid_density=fluid_density, dt=dt, vorticity_eps=vorticity_eps,
-                     gravity=gravity, frames=frames)
+                     gravity=gravity, frames=frames, buoyancy=buoyancy,
+                     rayleigh_alpha=rayleigh_alpha, rayleigh_beta=rayleigh_beta)
 
 
 def random_spheres(rng, k, radius_range, log10_E_range, center_range=None, margin=0.02,

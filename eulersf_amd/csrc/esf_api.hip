@@ -33,8 +33,9 @@ __global__ void kCorotBatch(int n, double lambda, double mu, const double* F, co
                             double* out, double* energy, double* dP);
 // esf_grid.hip
 esf_status gatherSolidMasses(esf_ctx* ctx);
-esf_status computeSolidMass(esf_ctx* ctx, int solid);
+esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf);
 esf_status updateBoxes(esf_ctx* ctx, int solid);
+esf_status solidPhi(esf_ctx* ctx, int solid);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
 esf_status forkBodyForces(esf_ctx* ctx);
@@ -335,6 +336,8 @@ esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu, double density,
     if (!st) st = dalloc(ctx, p, 3 * N);
   if (!st) st = dalloc(ctx, &so.mass, N);
   if (!st) st = dalloc(ctx, &so.detFInv, N);
+  if (!st) st = dalloc(ctx, &so.phi, N);
+  if (!st) st = dalloc(ctx, &so.phi2, N);
   if (!st) st = dalloc(ctx, &so.qfrac, 8 * N);
   if (!st) st = dalloc(ctx, &so.valid, d.cells);
   if (!st) st = dalloc(ctx, &so.cellSlot, d.cells);
@@ -515,6 +518,7 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
       if (forWrite) ctx->massGen++;
       break;
     case ESF_S_VEL_WORKSPACE: *ptr = so.velws; *bytes = 24 * N; break;
+    case ESF_S_PHI: *ptr = so.phi; *bytes = 8 * N; break;
     default:
       ctx->err = "bad field id";
       return ESF_ERR_INVALID;
@@ -558,7 +562,19 @@ esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid) {
     ctx->err = "mass sample dh must be positive and at most dh";
     return ESF_ERR_INVALID;
   }
-  esf_status st = computeSolidMass(ctx, solid);
+  esf_status st = computeSolidMass(ctx, solid, false);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_compute_solid_mass_sdf(esf_ctx* ctx, int solid) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  if (!(ctx->p.mass_sample_dh > 0) || (int)(ctx->d.dh / ctx->p.mass_sample_dh) < 1) {
+    ctx->err = "mass sample dh must be positive and at most dh";
+    return ESF_ERR_INVALID;
+  }
+  esf_status st = computeSolidMass(ctx, solid, true);
   if (st) return st;
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   return ESF_OK;
@@ -567,6 +583,15 @@ esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid) {
 esf_status esf_update_boxes(esf_ctx* ctx, int solid) {
   if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
   return updateBoxes(ctx, solid);
+}
+
+esf_status esf_solid_sdf(esf_ctx* ctx, int solid) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  esf_status st = updateBoxes(ctx, solid);
+  if (!st) st = solidPhi(ctx, solid);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
 }
 
 esf_status esf_gather_solid_masses(esf_ctx* ctx) {
@@ -743,6 +768,9 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
       ctx->err = "solid masses changed: call esf_gather_solid_masses first";
       return ESF_ERR_STATE;
     }
+  // alpha < 0: the broad phase of stepImplicit decides it
+  // (COUPLED_INTEGRATOR_3D.cpp:384-411), as in esf_frame
+  if (alpha < 0 && (st = esf_broad_phase(ctx, &alpha))) return st;
   // esf_cfl and esf_forces, reordered: the stress stage does not use dt,
   // so it is queued behind the CFL kernel before the host waits for dt;
   // a step running alone sends the fluid's vorticity and body forces to the
@@ -750,7 +778,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   // nodes' part (same results either way)
   {
     StageTimer t(ctx, 1);
-    if ((st = cflEnqueue(ctx, alpha < 0 ? 1.8 : alpha))) return st;
+    if ((st = cflEnqueue(ctx, alpha))) return st;
     if ((st = stressStage(ctx, nullptr))) return st;
     if ((st = cflFinish(ctx, &dt))) return st;
     st = inFlight.alone ? forkBodyForces(ctx) : vorticityAndBodyForces(ctx, nullptr);
@@ -765,8 +793,18 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   }
   if ((st = esf_solid_copy_velocity(ctx))) return st;
   if ((st = esf_solid_finalize(ctx))) return st;
-  // DEFOR_SOLID_3D::advectSurfaceMesh (COUPLED_INTEGRATOR_3D.cpp:863-866)
-  if ((st = advectMeshes(ctx))) return st;
+  {
+    // computeSolidMass(true) of every solid, the surface meshes, then
+    // computeSDF's boxes and phi (COUPLED_INTEGRATOR_3D.cpp:857-879)
+    StageTimer t(ctx, 9);
+    for (int b = 0; b < (int)ctx->solids.size(); b++)
+      if ((st = computeSolidMass(ctx, b, true))) return st;
+    if ((st = advectMeshes(ctx))) return st;
+    for (int b = 0; b < (int)ctx->solids.size(); b++) {
+      if ((st = updateBoxes(ctx, b))) return st;
+      if ((st = solidPhi(ctx, b))) return st;
+    }
+  }
   if ((st = esf_gather_solid_masses(ctx))) return st;
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;

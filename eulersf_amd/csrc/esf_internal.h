@@ -63,6 +63,9 @@ struct DevSolid {
   double *u = nullptr, *unew = nullptr, *X = nullptr;
   double *vel = nullptr, *velws = nullptr;  // 3*nodes
   double *mass = nullptr, *detFInv = nullptr;
+  // _phi: the solid's distance field in cells (computeSDF); phi2 is the
+  // second buffer of its relaxation
+  double *phi = nullptr, *phi2 = nullptr;
   double* qfrac = nullptr;                  // 8*nodes
   int8_t* valid = nullptr;                  // cells
   Box box;
@@ -339,6 +342,34 @@ __device__ inline void finishSums(int nb, const double* partials, double* lds,
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// solveDistance (DEFOR_SOLID_3D.cpp:369-395): the grid Eikonal update from
+// the three upwind neighbours p, q, r (x, y, z), sorted; lowers v only.  For
+// finite inputs the candidate at i = 0 is always valid, so the reference's
+// loop never steps below 0.
+__device__ inline double solveDistance(double p, double q, double r, double v) {
+  double d[3];
+  d[0] = fmin(p, fmin(q, r));
+  d[2] = fmax(p, fmax(q, r));
+  d[1] = p + q + r - d[0] - d[2];
+  double b[3], c[3];
+  b[0] = d[0];
+  c[0] = d[0] * d[0] - 1.0;
+#pragma unroll
+  for (int x = 1; x < 3; x++) {
+    b[x] = d[x] + b[x - 1];
+    c[x] = d[x] * d[x] + c[x - 1];
+  }
+  int i = 2;
+  double discrim = b[2] * b[2] - 3.0 * c[2];
+  double nd = (b[2] + sqrt(discrim)) / 3.0;
+  while ((discrim < 0.0 || nd < d[i]) && i > 0) {
+    i--;
+    discrim = b[i] * b[i] - (double)(i + 1) * c[i];
+    if (discrim > 0.0) nd = (b[i] + sqrt(discrim)) / (double)(i + 1);
+  }
+  return nd < v ? nd : v;
+}
 
 // what a finished Krylov solve reports, from its scalar block
 inline void solveReport(const CrScalars& h, int maxIt, int* iters, bool* converged,

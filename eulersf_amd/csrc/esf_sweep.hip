@@ -43,32 +43,6 @@ __device__ __host__ inline long nodeAt(const SweepBox& b, int i, int j, int k) {
   return i + (long)b.X * (j + (long)b.Y * k);
 }
 
-// solveDistance (:369-395); for finite inputs the candidate at i = 0 is
-// always valid, so the reference's loop never steps below 0
-__device__ inline double solveDistance(double p, double q, double r, double v) {
-  double d[3];
-  d[0] = fmin(p, fmin(q, r));
-  d[2] = fmax(p, fmax(q, r));
-  d[1] = p + q + r - d[0] - d[2];
-  double b[3], c[3];
-  b[0] = d[0];
-  c[0] = d[0] * d[0] - 1.0;
-#pragma unroll
-  for (int x = 1; x < 3; x++) {
-    b[x] = d[x] + b[x - 1];
-    c[x] = d[x] * d[x] + c[x - 1];
-  }
-  int i = 2;
-  double discrim = b[2] * b[2] - 3.0 * c[2];
-  double nd = (b[2] + sqrt(discrim)) / 3.0;
-  while ((discrim < 0.0 || nd < d[i]) && i > 0) {
-    i--;
-    discrim = b[i] * b[i] - (double)(i + 1) * c[i];
-    if (discrim > 0.0) nd = (b[i] + sqrt(discrim)) / (double)(i + 1);
-  }
-  return nd < v ? nd : v;
-}
-
 // one ordered sweep: n* nodes along each axis from (i0, j0, k0) in steps
 // (di, dj, dk), loop order k outer, i inner
 struct Sweep {

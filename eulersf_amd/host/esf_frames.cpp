@@ -228,6 +228,8 @@ extern "C" esf_status esf_write_frame(esf_ctx* ctx, const char* dataPath, int fr
     if (!st) st = writeGzField(sb + ".displacement", g, 3, buf.data());
     if (!st) st = esf_download(ctx, ESF_S_VELOCITY, (int)i, buf.data(), 24 * N);
     if (!st) st = writeGzField(sb + ".velocity", g, 3, buf.data());
+    if (!st) st = esf_download(ctx, ESF_S_PHI, (int)i, buf.data(), 8 * N);
+    if (!st) st = writeGzField(sb + ".phi", g, 1, buf.data());
   }
   // the surface mesh every frame (DEFOR_SOLID_3D.cpp:1368)
   for (size_t i = 0; !st && i < ctx->solids.size(); i++) {
@@ -277,6 +279,8 @@ extern "C" esf_status esf_read_frame(esf_ctx* ctx, const char* dataPath, int fra
     if (full) {
       st = readField(sb + ".displacement", 3, u.data());
       if (!st) st = esf_upload(ctx, ESF_S_U, (int)i, u.data(), 24 * N);
+      if (!st) st = readField(sb + ".phi", 1, buf.data());
+      if (!st) st = esf_upload(ctx, ESF_S_PHI, (int)i, buf.data(), 8 * N);
       if (!st) st = readField(sb + ".velocity", 3, buf.data());
       if (!st) st = esf_upload(ctx, ESF_S_VELOCITY, (int)i, buf.data(), 24 * N);
     } else {
@@ -310,8 +314,30 @@ extern "C" esf_status esf_read_frame(esf_ctx* ctx, const char* dataPath, int fra
         ESF_HIP(hipMemcpyAsync(so.meshVert, mv.data(), mv.size() * sizeof(double),
                                hipMemcpyHostToDevice, ctx->stream));
         ESF_HIP(hipStreamSynchronize(ctx->stream));
+        // the boxes from the mesh's bounding box, its corners truncated to
+        // node indices (:1318-1355)
+        double lo[3] = {1e9, 1e9, 1e9}, hi[3] = {-1e9, -1e9, -1e9};
+        for (size_t v = 0; v < mv.size(); v += 3)
+          for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], mv[v + c]);
+            hi[c] = std::max(hi[c], mv[v + c]);
+          }
+        const double o[3] = {g.origin[0], g.origin[1], g.origin[2]};
+        const int res[3] = {g.xres, g.yres, g.zres};
+        const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
+        int boxes[12];
+        for (int c = 0; c < 3; c++) {
+          const int mn = (int)((lo[c] - o[c]) / g.dh), mx = (int)((hi[c] - o[c]) / g.dh);
+          boxes[c] = std::max(mn - tb, 0);
+          boxes[3 + c] = std::min(mx + tb, res[c]);
+          boxes[6 + c] = std::max(mn - ex, 0);
+          boxes[9 + c] = std::min(mx + ex, res[c]);
+        }
+        st = esf_solid_set_boxes(ctx, (int)i, boxes);
       }
     }
+    // computeSolidMass(true) on the read state (:1357)
+    if (!st) st = esf_compute_solid_mass_sdf(ctx, (int)i);
   }
   // grid (COUPLED_INTEGRATOR_3D.cpp:983-990)
   if (!st && full) {

@@ -402,7 +402,7 @@ extern "C" esf_status esf_scene_load_cfg(const char* path, int precision, esf_ct
   // COUPLED_INTEGRATOR_3D::init  COUPLED_INTEGRATOR_3D.cpp:104-135 with
   // DEFOR_SOLID_3D::init (:269-331): full-grid boxes, mass, boxes from mass
   for (int i = 0; i < nSolids; i++) {
-    if ((st = esf_compute_solid_mass(ctx, i)) || (st = esf_update_boxes(ctx, i))) {
+    if ((st = esf_compute_solid_mass(ctx, i)) || (st = esf_solid_sdf(ctx, i))) {
       esf_ctx_destroy(ctx);
       return st;
     }

@@ -99,7 +99,7 @@ class CoupledIntegrator3D:
             _abi.S_U: (3 * n, np.float64), _abi.S_X: (3 * n, np.float64),
             _abi.S_VELOCITY: (3 * n, np.float64), _abi.S_MASS: (n, np.float64),
             _abi.S_VALID: (c, np.int8), _abi.S_QFRAC: (8 * n, np.float64),
-            _abi.S_VEL_WORKSPACE: (3 * n, np.float64),
+            _abi.S_VEL_WORKSPACE: (3 * n, np.float64), _abi.S_PHI: (n, np.float64),
         }
         return table[field]
 
@@ -135,6 +135,27 @@ class CoupledIntegrator3D:
         b = (ctypes.c_int * 12)(*boxes)
         check(self.ctx, self.lib.esf_solid_set_boxes(self.ctx, solid, b))
 
+    def _node_array(self, a, comps, name, copy=False):
+        """a C-ordered float64 copy (or view) of a node field: (Z, Y, X) for
+        comps == 1, (Z, Y, X, 3) for comps == 3, or its flat form; any other
+        shape, or a size that is not the grid's, is refused"""
+        X, Y, Z = self.grid["xres"], self.grid["yres"], self.grid["zres"]
+        a = np.asarray(a)
+        shape = (Z, Y, X) if comps == 1 else (Z, Y, X, 3)
+        if a.shape not in (shape, (comps * self.nodes,)):
+            raise ValueError(f"{name}: expected shape {shape} or ({comps * self.nodes},), "
+                             f"got {a.shape}")
+        return np.array(a, dtype=np.float64, order="C", copy=True) if copy else \
+            np.ascontiguousarray(a, dtype=np.float64)
+
+    @staticmethod
+    def _vector(a, n, name, copy=False):
+        a = np.asarray(a)
+        if a.shape != (n,):
+            raise ValueError(f"{name}: expected shape ({n},), got {a.shape}")
+        return np.array(a, dtype=np.float64, order="C", copy=True) if copy else \
+            np.ascontiguousarray(a, dtype=np.float64)
+
     # -- stages (COUPLED_INTEGRATOR_3D / DEFOR_SOLID_3D members) -------------
     def setMaxDt(self, dt):
         check(self.ctx, self.lib.esf_set_max_dt(self.ctx, dt))
@@ -155,8 +176,16 @@ class CoupledIntegrator3D:
     def computeSolidMass(self, solid=0):
         check(self.ctx, self.lib.esf_compute_solid_mass(self.ctx, solid), "computeSolidMass")
 
+    def computeSolidMassSdf(self, solid=0):
+        """DEFOR_SOLID_3D::computeSolidMass(true): cells with phi > 2 skipped"""
+        check(self.ctx, self.lib.esf_compute_solid_mass_sdf(self.ctx, solid), "computeSolidMass(true)")
+
     def updateBoxes(self, solid=0):
         check(self.ctx, self.lib.esf_update_boxes(self.ctx, solid), "updateBoxes")
+
+    def solidSDF(self, solid=0):
+        """DEFOR_SOLID_3D::computeSDF on the device: boxes and phi"""
+        check(self.ctx, self.lib.esf_solid_sdf(self.ctx, solid), "solidSDF")
 
     def gatherSolidMasses(self):
         check(self.ctx, self.lib.esf_gather_solid_masses(self.ctx), "gatherSolidMasses")
@@ -241,8 +270,7 @@ class CoupledIntegrator3D:
     def computeSDF(self, mass):
         """DEFOR_SOLID_3D::computeSDF on a node mass field (Z, Y, X) of this
         grid: (phi, box, tight), boxes as (xmin, ymin, zmin, xmax, ymax, zmax)"""
-        m = np.ascontiguousarray(mass, dtype=np.float64)
-        assert m.size == self.nodes, "mass must be a node field of the grid"
+        m = self._node_array(mass, 1, "mass")
         phi = np.empty_like(m)
         box = np.zeros(6, dtype=np.int32)
         tight = np.zeros(6, dtype=np.int32)
@@ -253,11 +281,12 @@ class CoupledIntegrator3D:
     def extendVectorField(self, box, mass, phi, field):
         """DEFOR_SOLID_3D::extendVectorField of a (Z, Y, X, 3) node field where
         mass == 0, over box; returns the extended field"""
-        m = np.ascontiguousarray(mass, dtype=np.float64)
-        ph = np.ascontiguousarray(phi, dtype=np.float64)
-        out = np.array(field, dtype=np.float64, copy=True)
+        m = self._node_array(mass, 1, "mass")
+        ph = self._node_array(phi, 1, "phi")
+        out = self._node_array(field, 3, "field", copy=True)
         b = np.ascontiguousarray(box, dtype=np.int32)
-        assert m.size == self.nodes and ph.size == m.size and out.size == 3 * m.size
+        if b.shape != (6,):
+            raise ValueError(f"box: expected 6 ints, got shape {b.shape}")
         check(self.ctx, self.lib.esf_extend_vector_field(self.ctx, b.ctypes.data_as(_abi._I), dptr(m), dptr(ph),
                                                          dptr(out)), "extendVectorField")
         return out
@@ -273,7 +302,7 @@ class CoupledIntegrator3D:
                                                 1 if debug else 0), "readFrame")
 
     def getMatVecMult(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = self._vector(x, self.cells, "x")
         y = np.empty_like(x)
         check(self.ctx, self.lib.esf_pressure_apply(self.ctx, dptr(x), dptr(y)), "pressure apply")
         return y
@@ -284,14 +313,14 @@ class CoupledIntegrator3D:
         return d
 
     def solidApply(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = self._vector(x, 3 * self.num_solid_nodes(), "x")
         y = np.empty_like(x)
         check(self.ctx, self.lib.esf_solid_apply(self.ctx, dptr(x), dptr(y)))
         return y
 
     def pressureSolve(self, b, x0, eps, max_it):
-        b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        b = self._vector(b, self.cells, "b")
+        x = self._vector(x0, self.cells, "x0", copy=True)
         it = ctypes.c_int()
         rr = ctypes.c_double()
         st = check(self.ctx, self.lib.esf_pressure_solve(self.ctx, dptr(b), dptr(x), eps, max_it,

@@ -122,7 +122,8 @@ enum {
   ESF_S_MASS = 103,        /* double[nodes]    _mass         */
   ESF_S_VALID = 104,       /* int8[cells]      _valid        */
   ESF_S_QFRAC = 105,       /* double[8*nodes]  _quarterVolumeFractions */
-  ESF_S_VEL_WORKSPACE = 106/* double[3*nodes]  _vecWorkspace */
+  ESF_S_VEL_WORKSPACE = 106,/* double[3*nodes] _vecWorkspace */
+  ESF_S_PHI = 108          /* double[nodes]    _phi (see esf_solid_sdf) */
 };
 
 /* --- context --------------------------------------------------------- */
@@ -180,8 +181,23 @@ esf_status esf_download(esf_ctx* ctx, int field, int solid, void* dst,
 /* --- stages -------------------------------------------------------------*/
 /* DEFOR_SOLID_3D::computeSolidMass(false) (DEFOR_SOLID_3D.cpp:576-669) */
 esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid);
+/* DEFOR_SOLID_3D::computeSolidMass(true) (DEFOR_SOLID_3D.cpp:576-669): as
+ * esf_compute_solid_mass, but a tight-box cell whose min corner has
+ * _phi > 2 (the solid's phi from the last esf_solid_sdf) takes no samples
+ * and stays invalid (:620). */
+esf_status esf_compute_solid_mass_sdf(esf_ctx* ctx, int solid);
 /* box part of DEFOR_SOLID_3D::computeSDF (DEFOR_SOLID_3D.cpp:216-266) */
 esf_status esf_update_boxes(esf_ctx* ctx, int solid);
+/* DEFOR_SOLID_3D::computeSDF (DEFOR_SOLID_3D.cpp:216-267) on the solid's
+ * own mass, on the device: the tight and extended boxes, then _phi = -0.5
+ * on massive nodes and the grid Eikonal distance (in cells) around them.
+ * The reference sweeps the whole extended box; here phi is relaxed to the
+ * same upwind fixed point ring by ring from the massive nodes, which
+ * settles it where computeSolidMass(true) reads it (phi <= 2 against the
+ * reference's ordered sweeps: the same set, values within 1e-4); further
+ * out it is an upper bound.  For the swept field itself use
+ * esf_compute_sdf. */
+esf_status esf_solid_sdf(esf_ctx* ctx, int solid);
 /* COUPLED_INTEGRATOR_3D::gatherSolidMasses (COUPLED_INTEGRATOR_3D.cpp:327-375) */
 esf_status esf_gather_solid_masses(esf_ctx* ctx);
 /* DEFOR_SOLID_3D::recoverVelocityField for every solid (:1235-1241) */
@@ -238,15 +254,17 @@ esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double*
 /* recoverVelocityField + Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:894-916) */
 esf_status esf_recover_and_bc(esf_ctx* ctx);
 /* One substep of the drop-in: CFL(alpha) -> forces -> coupled solve ->
- * solid velocity copy -> solid finalize -> surface meshes (solids that have
- * one) -> gather solid masses -> advect velocity -> advect intensity ->
- * recover + Dirichlet.  The stages of COUPLED_INTEGRATOR_3D::finalizeSolution
- * that stay with the host integrator (FLIP particles, mass resampling, SDF
- * and velocity extension -- the last two callable as esf_compute_sdf and
- * esf_extend_vector_field; :833-879) are not run.  A step that runs alone
- * in the process computes the fluid's vorticity and body forces on a second
- * stream of the context, beside the first solid solve; results are the same
- * either way. */
+ * solid velocity copy -> solid finalize -> computeSolidMass(true) of every
+ * solid -> surface meshes (solids that have one) -> computeSDF boxes and
+ * phi (esf_solid_sdf) -> gather solid masses -> advect velocity -> advect
+ * intensity -> recover + Dirichlet (COUPLED_INTEGRATOR_3D.cpp:377-917).
+ * alpha < 0 takes the CFL alpha from the broad phase (esf_broad_phase), as
+ * stepImplicit does.  The FLIP particle round trip and the velocity
+ * extension of COUPLED_INTEGRATOR_3D::finalizeSolution (:833-854) are not
+ * run: the solid velocity is the grid velocity copied into its box.  A step
+ * that runs alone in the process computes the fluid's vorticity and body
+ * forces on a second stream of the context, beside the first solid solve;
+ * results are the same either way. */
 esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
 
 /* --- operators (unit level) ---------------------------------------------*/
@@ -367,19 +385,20 @@ esf_status esf_obj_write(const char* path, int nv, const double* vertices,
 /* COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011) with
  * DEFOR_SOLID_3D::writeFrame (DEFOR_SOLID_3D.cpp:1361-1371): under
  * "<data_path>frame.NNNN" (at least four digits, IO.h:27-37) the intensity
- * every frame; the grid velocity and, per solid i, ".solid.i.displacement"
- * and ".solid.i.velocity" when debug or frame % 10 == 0, and every frame
- * ".solid.i.obj" for a solid with a surface mesh.  The solid SDF (".phi")
- * and the ".pbrt" render file are not kept by this library and are not
- * written. */
+ * every frame; the grid velocity and, per solid i, ".solid.i.displacement",
+ * ".solid.i.velocity" and ".solid.i.phi" when debug or frame % 10 == 0, and
+ * every frame ".solid.i.obj" for a solid with a surface mesh.  The ".pbrt"
+ * render file is not written. */
 esf_status esf_write_frame(esf_ctx* ctx, const char* data_path, int frame,
                            int debug);
 /* COUPLED_INTEGRATOR_3D::readFrame (COUPLED_INTEGRATOR_3D.cpp:976-997) with
  * DEFOR_SOLID_3D::readFrame (DEFOR_SOLID_3D.cpp:1299-1317): the same files
  * back (a grid that differs from the context's, or a mesh of other
  * topology, is ESF_ERR_INVALID), each solid's velocity workspace set from
- * its velocity and X = x - u, mesh vertices from ".solid.i.obj", then
- * gatherSolidMasses and recoverVelocityField. */
+ * its velocity and X = x - u, mesh vertices from ".solid.i.obj", the boxes
+ * from the mesh's bounding box (:1318-1355; a solid without a mesh keeps
+ * its boxes) and computeSolidMass(true) (:1357), then gatherSolidMasses and
+ * recoverVelocityField. */
 esf_status esf_read_frame(esf_ctx* ctx, const char* data_path, int frame,
                           int debug);
 

@@ -411,6 +411,7 @@ struct Solid {
   vector<double> rest;
   vector<double> X, u, vel, velws;  // 3*nodes
   vector<double> mass, detFInv;     // nodes
+  vector<double> phi;               // nodes (_phi, computeSDF)
   vector<int8_t> valid;             // cells
   vector<double> qfrac;             // 8*nodes
   int tMin[3], tMax[3], bMin[3], bMax[3];
@@ -509,8 +510,8 @@ double computeF(const Scene& s, const Solid& so, long index, double* F) {
   return 1.0 / Jinv;
 }
 
-// DEFOR_SOLID_3D::computeSolidMass(usesdf=false)  DEFOR_SOLID_3D.cpp:576-669
-void computeSolidMass(Scene& s, Solid& so) {
+// DEFOR_SOLID_3D::computeSolidMass(usesdf)  DEFOR_SOLID_3D.cpp:576-669
+void computeSolidMass(Scene& s, Solid& so, bool usesdf) {
   std::fill(so.mass.begin(), so.mass.end(), 0.0);
   std::fill(so.valid.begin(), so.valid.end(), 0);
   std::fill(so.detFInv.begin(), so.detFInv.end(), 0.0);
@@ -537,6 +538,7 @@ void computeSolidMass(Scene& s, Solid& so) {
         long index = z * s.slab + (long)y * s.X + x;
         double node[3] = {x * s.dh + s.o[0], y * s.dh + s.o[1],
                           z * s.dh + s.o[2]};
+        if (usesdf && so.phi[index] > 2) continue;  // :620
         double* q = &so.qfrac[8 * index];
         for (int pp = 0; pp < msr; pp++)
           for (int qq = 0; qq < msr; qq++)
@@ -606,6 +608,20 @@ void updateBoxes(Scene& s, Solid& so) {
   so.bMax[0] = std::min(xmax + ex, s.X);
   so.bMax[1] = std::min(ymax + ex, s.Y);
   so.bMax[2] = std::min(zmax + ex, s.Z);
+}
+
+extern "C" void orc_sweep_phi(int X, int Y, int Z, const int box[6], const double* mass,
+                              double* phi);
+
+// DEFOR_SOLID_3D::computeSDF  DEFOR_SOLID_3D.cpp:216-267: the boxes, then
+// phi = large, -0.5 on massive nodes, two rounds of sweepPhi over the
+// extended box
+void computeSDF(Scene& s, Solid& so) {
+  updateBoxes(s, so);
+  const double large = (double)(s.X + s.Y + s.Z + 2);
+  for (long i = 0; i < s.total; i++) so.phi[i] = so.mass[i] > 0 ? -0.5 : large;
+  const int box[6] = {so.bMin[0], so.bMin[1], so.bMin[2], so.bMax[0], so.bMax[1], so.bMax[2]};
+  for (int r = 0; r < 2; r++) orc_sweep_phi(s.X, s.Y, s.Z, box, so.mass.data(), so.phi.data());
 }
 
 // COUPLED_INTEGRATOR_3D::gatherSolidMasses  COUPLED_INTEGRATOR_3D.cpp:327-375
@@ -1475,6 +1491,7 @@ int orc_add_solid(void* h, double lambda, double mu, double density, int rX,
   so.velws.assign(3 * s.total, 0.0);
   so.mass.assign(s.total, 0.0);
   so.detFInv.assign(s.total, 0.0);
+  so.phi.assign(s.total, 0.0);
   so.valid.assign(s.nCells, 0);
   so.qfrac.assign(8 * s.total, 0.0);
   so.dt = s.p.dt;
@@ -1523,8 +1540,8 @@ void orc_init(void* h, double carpet_thickness) {
     so.bMax[0] = so.tMax[0] = s.X;
     so.bMax[1] = so.tMax[1] = s.Y;
     so.bMax[2] = so.tMax[2] = s.Z;
-    computeSolidMass(s, so);
-    updateBoxes(s, so);
+    computeSolidMass(s, so, false);
+    computeSDF(s, so);
   }
   gather(s);
   for (Solid& so : s.solids) recoverVelocity(s, so);
@@ -1587,6 +1604,7 @@ void* orc_field(void* h, int id, int solid, size_t* n) {
     case ORC_S_QFRAC: return ret(so.qfrac);
     case ORC_S_VEL_WORKSPACE: return ret(so.velws);
     case ORC_S_DETFINV: return ret(so.detFInv);
+    case ORC_S_PHI: return ret(so.phi);
     default: break;
   }
   *n = 0;
@@ -1645,7 +1663,15 @@ void orc_solid_copy_velocity(void* h) { solidCopyVelocity(*static_cast<Scene*>(h
 void orc_solid_finalize(void* h) { solidFinalize(*static_cast<Scene*>(h)); }
 void orc_compute_solid_mass(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);
-  computeSolidMass(s, s.solids[solid]);
+  computeSolidMass(s, s.solids[solid], false);
+}
+void orc_compute_solid_mass_sdf(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  computeSolidMass(s, s.solids[solid], true);
+}
+void orc_solid_sdf(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  computeSDF(s, s.solids[solid]);
 }
 void orc_update_boxes(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);
@@ -1667,6 +1693,10 @@ int orc_step(void* h, double alpha, orc_stats* st) {
   solidCopyVelocity(s);
   solidFinalize(s);
   double t1 = nowSec();
+  // finalizeSolution :857-879: computeSolidMass(true) of every solid, then
+  // computeSDF of every solid (surface meshes are not kept here)
+  for (Solid& so : s.solids) computeSolidMass(s, so, true);
+  for (Solid& so : s.solids) computeSDF(s, so);
   gather(s);
   s.stageSec[10] += nowSec() - t1;
   advectVelocity(s);

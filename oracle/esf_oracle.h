@@ -71,7 +71,8 @@ enum {
   ORC_S_VALID = 104,      /* int8 cells (X-1)(Y-1)(Z-1) */
   ORC_S_QFRAC = 105,      /* 8*nodes */
   ORC_S_VEL_WORKSPACE = 106, /* 3*nodes */
-  ORC_S_DETFINV = 107     /* nodes */
+  ORC_S_DETFINV = 107,    /* nodes */
+  ORC_S_PHI = 108         /* nodes */
 };
 
 typedef struct {
@@ -111,6 +112,8 @@ void orc_solid_copy_velocity(void* h);           /* DEFOR :1182-1197 */
 void orc_solid_finalize(void* h);                /* DEFOR :1198-1233 */
 void orc_compute_solid_mass(void* h, int solid); /* DEFOR :576-669 (usesdf=false) */
 void orc_update_boxes(void* h, int solid);       /* DEFOR :216-266 (box part) */
+void orc_compute_solid_mass_sdf(void* h, int solid); /* DEFOR :576-669 (usesdf=true) */
+void orc_solid_sdf(void* h, int solid);          /* DEFOR :216-267 (boxes + phi) */
 void orc_gather(void* h);                        /* :327-375 */
 void orc_advect_velocity(void* h);               /* :797-826 */
 void orc_advect_intensity(void* h);              /* :766-796 */

@@ -20,7 +20,7 @@ VELOCITY, INTENSITY, SOLID_MASS, NODE_FRACTION, LAMBDA, VEL_WORKSPACE, SOLID_IND
 V0, V1, MINV, PRESSURE_RHS, DIV_RHS, JT_LAMBDA = 7, 8, 9, 10, 11, 12
 SOLID_FORCE, SOLID_MV, DAMPED_MASS, UNIT_FORCE, MV, SOLID_V0, SOLID_DV = 13, 14, 15, 16, 17, 18, 19
 SYS_DIAG, PRESSURE_DIAG, VORTICITY, WORKSPACE = 20, 21, 22, 23
-S_U, S_X, S_VELOCITY, S_MASS, S_VALID, S_QFRAC, S_VEL_WORKSPACE, S_DETFINV = range(100, 108)
+S_U, S_X, S_VELOCITY, S_MASS, S_VALID, S_QFRAC, S_VEL_WORKSPACE, S_DETFINV, S_PHI = range(100, 109)
 
 _INT_FIELDS = {SOLID_INDEX: np.int32, S_VALID: np.int8}
 
@@ -74,6 +74,8 @@ def lib():
             getattr(L, n).argtypes = [ctypes.c_void_p]
         L.orc_compute_solid_mass.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_update_boxes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_compute_solid_mass_sdf.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_solid_sdf.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Stats)]
         L.orc_pressure_apply.argtypes = [ctypes.c_void_p, _D, _D]

@@ -28,3 +28,37 @@ def solution_order(v_nodes, X, Y, Z):
     interior nodes (3*(X-2)(Y-2)(Z-2))."""
     v = np.asarray(v_nodes).reshape(Z, Y, X, 3)[1:-1, 1:-1, 1:-1]
     return v.reshape(-1)
+
+
+def teacher_force(g, o, n_solids=1):
+    """Copy the oracle scene's state into the GPU context: grid velocity,
+    intensity and the pressure warm start; per solid its boxes, u, X,
+    velocity, velocity workspace, mass, quarter fractions, valid flags and
+    phi; then gatherSolidMasses.  The next step of both then starts from the
+    same state (teacher forcing)."""
+    from eulersf_amd import _abi as A
+    from oracle import oracle as O
+    g.upload(A.F_VELOCITY, o.field(O.VELOCITY))
+    g.upload(A.F_INTENSITY, o.field(O.INTENSITY))
+    g.upload(A.F_LAMBDA, o.field(O.LAMBDA))
+    for b in range(n_solids):
+        g.set_boxes(b, o.boxes(b))
+        for gf, of in ((A.S_U, O.S_U), (A.S_X, O.S_X), (A.S_VELOCITY, O.S_VELOCITY),
+                       (A.S_VEL_WORKSPACE, O.S_VEL_WORKSPACE), (A.S_MASS, O.S_MASS),
+                       (A.S_QFRAC, O.S_QFRAC), (A.S_VALID, O.S_VALID), (A.S_PHI, O.S_PHI)):
+            g.upload(gf, o.field(of, b), b)
+    g.gatherSolidMasses()
+
+
+def tight_box_phi_above_2(phi, boxes, X, Y, Z):
+    """the set computeSolidMass(true) skips: phi > 2 at the min corners of
+    the tight box's cells"""
+    t = boxes[:6]
+    p = np.asarray(phi).reshape(Z, Y, X)
+    return p[t[2]:t[5] - 1, t[1]:t[4] - 1, t[0]:t[3] - 1] > 2
+
+
+def jt_lambda(o, lam):
+    """-J^T lambda over the interior DOFs in the reference's solution order
+    (the pressure compared modulo the null space of S)"""
+    return o.apply_negJT(lam)

@@ -1,0 +1,263 @@
+"""The HIP path against the oracle at the sizes of the measured scenes, and
+the stages and options the small parity scenes leave out.
+
+- the 64^3 unit-cube ball (plain FP64, every field per step), teacher-forced
+  for 10 steps: before each step the GPU context receives the oracle's
+  state, so every step is compared from the same start;
+- the 128^3 ball, teacher-forced for 3 steps, in both forms: FP64 within
+  1e-10 of the oracle, the mixed form within 1e-6;
+- the per-step solid mass (computeSolidMass(true)) and SDF stages from the
+  same inputs;
+- the corotation golden vectors (every branch of COROTATION.cpp, written
+  from the reference's own code) through the GPU's corotation;
+- buoyancy and Rayleigh damping, the frame's substep loop and the broad
+  phase's CFL alpha.
+Both sides solve to 1e-13 (1e-12 at 128^3, where the FP64 CR cannot reach
+1e-13) so that Krylov convergence does not dominate the difference."""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests._scenes import jt_lambda, rel_err, teacher_force, tight_box_phi_above_2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def esf():
+    import eulersf_amd
+    eulersf_amd.load()
+    return eulersf_amd
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare_step(esf, g, o, k, tol, n_solids=1, mass_tol=None):
+    """fields after a step: velocity, -J^T lambda, intensity, each solid's u
+    (relative L2 within tol); the solid's valid cells, mass support, boxes,
+    DOF numbering and the phi > 2 set exactly; the mass within mass_tol"""
+    A = esf._abi
+    vg, vo = g.download(A.F_VELOCITY), o.field(O.VELOCITY)
+    # the velocity relative to its norm, or to that of g dt per component
+    # where the scene is at rest (hydrostatic: the velocity is rounding noise)
+    floor = 9.8 * g.dt() * np.sqrt(vo.size)
+    errs = {
+        "velocity": np.linalg.norm(vg - vo) / max(np.linalg.norm(vo), floor),
+        "JT_lambda": rel_err(jt_lambda(o, g.download(A.F_LAMBDA)), o.field(O.JT_LAMBDA)),
+        "intensity": rel_err(g.download(A.F_INTENSITY), o.field(O.INTENSITY)),
+    }
+    X, Y, Z = g.grid["xres"], g.grid["yres"], g.grid["zres"]
+    for b in range(n_solids):
+        errs[f"u{b}"] = rel_err(g.download(A.S_U, b), o.field(O.S_U, b))
+        assert g.boxes(b) == o.boxes(b), (k, b, g.boxes(b), o.boxes(b))
+        np.testing.assert_array_equal(g.download(A.S_VALID, b), o.field(O.S_VALID, b))
+        mg, mo = g.download(A.S_MASS, b), o.field(O.S_MASS, b)
+        np.testing.assert_array_equal(mg > 0, mo > 0)
+        errs[f"mass{b}"] = rel_err(mg, mo)
+        np.testing.assert_array_equal(
+            tight_box_phi_above_2(g.download(A.S_PHI, b), g.boxes(b), X, Y, Z),
+            tight_box_phi_above_2(o.field(O.S_PHI, b), o.boxes(b), X, Y, Z))
+    np.testing.assert_array_equal(g.download(A.F_SOLID_INDEX), o.field(O.SOLID_INDEX))
+    _say(f"step {k}: " + ", ".join(f"{n} {v:.2e}" for n, v in errs.items()))
+    for n, v in errs.items():
+        lim = mass_tol if (n.startswith("mass") and mass_tol is not None) else tol
+        assert v < lim, (k, n, v, lim)
+    return errs
+
+
+def test_ball64_teacher_forced_steps(esf):
+    """the 64^3 unit-cube ball as configured (sphere r = 0.15 at (0.5, 0.65,
+    0.5) at rest, E = 1e4, nu = 0.3, densities 1000, 30 % fluid pool,
+    dt = 1/240), FP64, ten steps, each from the oracle's state: every field
+    within 1e-10 (the equal densities hold the scene hydrostatic: the
+    velocity is rounding noise against g dt, the pressure carries the
+    weight)"""
+    text = esf.unit_cube_ball_cfg(64, eps=1e-13, max_iterations=20000)
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    for k in range(10):
+        if k:
+            teacher_force(g, o)
+        gs = g.stepImplicit(1.8)
+        rc, st = o.step(1.8)
+        assert gs["status"] == 0 and rc == 0, (k, gs, rc)
+        assert gs["dt"] == st.dt, k
+        compare_step(esf, g, o, k, 1e-10)
+    g.close()
+    o.close()
+
+
+def test_ball32_thrown_teacher_forced_steps(esf):
+    """the ball thrown at (0.3, -1, 0.2) m/s on 32^3 cells: it moves, deforms
+    and changes its mass cells every step.  Five steps, each from the
+    oracle's state.  The pressure CR needs ~1400 iterations per step at
+    1e-11 here and stagnates near 1e-12 (the Q1-P0 operator's near-null
+    hourglass modes), so both sides stop at 1e-11; they take the same
+    iteration counts, and every field agrees within 1e-10, every integer
+    field exactly"""
+    text = esf.unit_cube_ball_cfg(32, eps=1e-11, max_iterations=20000,
+                                  velocity=(0.3, -1.0, 0.2))
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    for k in range(5):
+        if k:
+            teacher_force(g, o)
+        gs = g.stepImplicit(1.8)
+        rc, st = o.step(1.8)
+        _say(f"iterations GPU {gs['iterations']}, oracle {list(st.iterations)}")
+        assert gs["status"] == 0 and rc == 0, (k, gs, rc)
+        assert gs["dt"] == st.dt, k
+        assert abs(gs["iterations"][1] - st.iterations[1]) <= 5, k
+        compare_step(esf, g, o, k, 1e-10)
+    g.close()
+    o.close()
+
+
+def test_ball128_teacher_forced_plain_and_mixed(esf):
+    """the 128^3 ball as configured, three steps from the oracle's state: the FP64 form
+    solving to 1e-12 within 1e-10 of the oracle, the mixed form (FP32 CR in
+    FP64 refinement) solving to 1e-8 within 1e-6 of it"""
+    text64 = esf.unit_cube_ball_cfg(128, eps=1e-12, max_iterations=20000)
+    text_mixed = esf.unit_cube_ball_cfg(128, eps=1e-8, max_iterations=20000)
+    t0 = time.time()
+    o = O.OracleScene.from_cfg_text(text64)
+    g = esf.CoupledIntegrator3D.from_cfg_text(text64)
+    m = esf.CoupledIntegrator3D.from_cfg_text(text_mixed, esf.ESF_MIXED)
+    _say(f"128^3 scenes ready in {time.time() - t0:.1f} s")
+    for k in range(3):
+        if k:
+            teacher_force(g, o)
+            teacher_force(m, o)
+        gs = g.stepImplicit(1.8)
+        ms = m.stepImplicit(1.8)
+        t0 = time.time()
+        rc, st = o.step(1.8)
+        _say(f"oracle step {k}: {time.time() - t0:.1f} s, iterations {list(st.iterations)}; "
+             f"GPU FP64 {gs['iterations']}, mixed {ms['iterations']} "
+             f"({ms['refinements']} refinements, {ms['inner_iterations']} FP32)")
+        assert gs["status"] == 0 and ms["status"] == 0 and rc == 0, (k, gs, ms, rc)
+        compare_step(esf, g, o, k, 1e-10)
+        compare_step(esf, m, o, k, 1e-6)
+    g.close()
+    m.close()
+    o.close()
+
+
+def test_solid_mass_and_sdf_stage_from_the_same_state(esf):
+    """computeSolidMass(true) and computeSDF on identical inputs (the
+    oracle's deformed solid after two steps): mass, valid flags, quarter
+    fractions bitwise; boxes equal; the phi > 2 set the next mass pass reads
+    equal, and phi below 3 within 1e-4 of the ordered sweeps"""
+    A = esf._abi
+    text = esf.unit_cube_ball_cfg(32, eps=1e-10, velocity=(0.4, -1.2, 0.3))
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    for _ in range(2):
+        o.step(1.8)
+    teacher_force(g, o)
+    o.L.orc_solid_finalize(o.h)  # advance u once more so the solid moves off its last mass
+    for f, of in ((A.S_U, O.S_U), (A.S_X, O.S_X)):
+        g.upload(f, o.field(of, 0), 0)
+    g.computeSolidMassSdf(0)
+    o.L.orc_compute_solid_mass_sdf(o.h, 0)
+    for f, of in ((A.S_MASS, O.S_MASS), (A.S_QFRAC, O.S_QFRAC), (A.S_VALID, O.S_VALID)):
+        np.testing.assert_array_equal(g.download(f, 0), o.field(of, 0))
+    g.solidSDF(0)
+    o.L.orc_solid_sdf(o.h, 0)
+    assert g.boxes(0) == o.boxes(0)
+    X = Y = Z = 33
+    pg, po = g.download(A.S_PHI, 0), o.field(O.S_PHI, 0)
+    np.testing.assert_array_equal(tight_box_phi_above_2(pg, g.boxes(0), X, Y, Z),
+                                  tight_box_phi_above_2(po, o.boxes(0), X, Y, Z))
+    near = po <= 3.0
+    assert np.abs(pg[near] - po[near]).max() < 1e-4
+    assert np.all(pg >= po - 1e-12)  # an upper bound further out
+    g.close()
+    o.close()
+
+
+def test_corotation_golden_vectors_on_gpu(esf):
+    """the reference's own COROTATION outputs (tests/golden/corotation.npz:
+    near identity, general, inverted, one, two and three singular values
+    below 1e-4, zero, reflection) against the GPU's corotation, at the
+    tolerances the oracle meets"""
+    from tests.test_oracle_golden import GOLD, _nan_equal_close, corot_tolerances
+    gd = np.load(os.path.join(GOLD, "corotation.npz"))
+    lam, mu = float(gd["lam"]), float(gd["mu"])
+    F = gd["F"].reshape(-1, 3, 3)
+    dF = gd["dF"].reshape(-1, 3, 3)
+    R, S, L, P, e, dP = esf.corotation_batch(F, dF, lam, mu)
+    for i in range(len(F)):
+        kind = str(gd["kind"][i])
+        ref = gd["out"][i].reshape(4, 3, 3)
+        tol = corot_tolerances(kind, lam, mu, gd["F"][i], gd["dF"][i])
+        _nan_equal_close(R[i], ref[0], tol["R"])
+        _nan_equal_close(S[i], ref[1], tol["S"])
+        _nan_equal_close(L[i], ref[2], tol["L"])
+        _nan_equal_close(P[i], ref[3], tol["P"])
+        _nan_equal_close([e[i]], [gd["energy"][i]], tol["e"])
+        _nan_equal_close(dP[i].ravel(), gd["dP"][i], tol["dP"])
+
+
+def test_buoyancy_and_rayleigh_damping_steps(esf):
+    """buoyancy on the smoke intensity and Rayleigh damping (alpha on the
+    solid mass, beta on the stiffness), which the shipped scenes set to 0:
+    three steps against the oracle"""
+    text = esf.unit_cube_ball_cfg(24, eps=1e-13, velocity=(0.2, -0.8, 0.1), buoyancy=0.7,
+                                  rayleigh_alpha=3.0, rayleigh_beta=0.002)
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    for k in range(3):
+        gs = g.stepImplicit(1.8)
+        rc, st = o.step(1.8)
+        assert gs["status"] == 0 and rc == 0
+        assert gs["dt"] == st.dt
+        compare_step(esf, g, o, k, 1e-10)
+    g.close()
+    o.close()
+
+
+def test_frame_substeps_match_oracle(esf):
+    """APPLICATION::step (rubberBall/main.cpp:180-198) through esf_frame: a
+    frame covered by three CFL substeps, the last one capped by the frame's
+    remaining time, against the same loop over the oracle's steps"""
+    text = esf.unit_cube_ball_cfg(24, eps=1e-13, velocity=(0.3, -4.0, 0.2))
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    frame_dt = 0.05
+    n = g.frame(frame_dt, 1.8)
+    sim, k = 0.0, 0
+    while sim < frame_dt:
+        o.L.orc_set_max_dt(o.h, frame_dt - sim)
+        rc, st = o.step(1.8)
+        sim += st.dt
+        k += 1
+    assert n == k >= 3, (n, k)
+    # after free-running substeps the CFL dt may differ in its last bit
+    assert abs(g.dt() - st.dt) <= 1e-12 * st.dt
+    compare_step(esf, g, o, k, 1e-10)
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("center_y,alpha", [(0.15 + 1.5 / 32, 0.6), (0.55, 1.8)])
+def test_broad_phase_alpha_from_negative_alpha(esf, center_y, alpha):
+    """esf_step(alpha < 0) takes the CFL alpha from the broad phase: a
+    sphere within two cells of the floor gives 0.6, one clear of every wall
+    1.8; the step's dt equals the oracle's CFL with that alpha"""
+    text = esf.unit_cube_ball_cfg(32, eps=1e-10, center=(0.5, center_y, 0.5),
+                                  velocity=(0.0, -20.0, 0.0))
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    assert g.broadPhase() == alpha
+    gs = g.stepImplicit(-1.0)
+    assert gs["dt"] == o.L.orc_cfl(o.h, alpha)
+    g.close()
+    o.close()

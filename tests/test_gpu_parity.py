@@ -288,7 +288,7 @@ def test_step_overlap_bitwise_equals_staged_sequence(esf, mixed):
     a = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
     b = esf.CoupledIntegrator3D.from_cfg_text(text, prec)
     fields = [(esf._abi.F_VELOCITY, -1), (esf._abi.F_INTENSITY, -1), (esf._abi.S_U, 0),
-              (esf._abi.S_VELOCITY, 0)]
+              (esf._abi.S_VELOCITY, 0), (esf._abi.S_MASS, 0), (esf._abi.S_PHI, 0)]
     for k in range(3):
         sa = a.stepImplicit(1.8)
         dt = b.CFL(1.8)
@@ -296,6 +296,9 @@ def test_step_overlap_bitwise_equals_staged_sequence(esf, mixed):
         sb = b.pcrImplicitSolve()
         b.solidCopyVelocity()
         b.solidFinalize()
+        b.computeSolidMassSdf(0)
+        b.advectSurfaceMesh()
+        b.solidSDF(0)
         b.gatherSolidMasses()
         b.advectFluidVelocityField()
         b.advectIntensity()
