@@ -166,6 +166,7 @@ _PROTOS = {
     "esf_advect_surface_mesh": (_S, [_P]),
     "esf_broad_phase": (_S, [_P, _D]),
     "esf_compute_sdf": (_S, [_P, _D, _D, _I, _I]),
+    "esf_compute_sdf_noalloc": (_S, [_P, _D, _D, _I, _I]),
     "esf_extend_vector_field": (_S, [_P, _I, _D, _D, _D]),
     "esf_obj_read": (_S, [ctypes.c_char_p, _I, _I, _D, ctypes.c_int, _I, ctypes.c_int]),
     "esf_obj_write": (_S, [ctypes.c_char_p, ctypes.c_int, _D, ctypes.c_int, _I]),

@@ -25,6 +25,7 @@
 // These are host-side stages of the reference's finalize that the step does
 // not run; the entry points serve a host integrator that keeps them.
 #include <climits>
+#include <mutex>
 
 #include "esf_internal.h"
 
@@ -291,25 +292,40 @@ esf_status sweepPhiDevice(esf_ctx* ctx, const SweepBox& b, const double* mass, d
 
 using namespace esf;
 
-esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int box[6],
-                           int tight[6]) {
-  if (!ctx || !mass || !phi || !box || !tight) return ESF_ERR_INVALID;
+// The allocation-free computeSDF's device memory (esf_compute_sdf_noalloc):
+// one array per GPU, resident while the process lives.  [0, 32) doubles hold
+// the extent counters, then the mass and phi of up to kSdfScratchNodes nodes
+// (the (128 + 1)^3 nodes of a 128^3-cell grid), 34.3 MB in all.  Host code
+// never passes `scratch` itself to a kernel or a copy (on the host the name
+// is only a stand-in): it uses the device address hipGetSymbolAddress
+// returns, looked up once per device.
+constexpr long kSdfScratchNodes = 129L * 129 * 129;
+constexpr long kSdfScratchDoubles = 32 + 2 * kSdfScratchNodes;
+static __device__ __align__(256) double scratch[kSdfScratchDoubles];
+
+namespace esf {
+namespace {
+
+// the extent counters start at (X, Y, Z, 0, 0, 0): written by a kernel,
+// so the call works from memory the host cannot address
+__global__ void kExtInit(int* ext, int X, int Y, int Z) {
+  if (threadIdx.x < 6) ext[threadIdx.x] = threadIdx.x == 0 ? X : threadIdx.x == 1 ? Y
+                                          : threadIdx.x == 2 ? Z : 0;
+}
+
+// computeSDF on the device buffers dm (the mass, n doubles), dphi (n
+// doubles) and dext (6 ints), all the caller's: boxes to the host, phi back
+esf_status computeSdfOn(esf_ctx* ctx, const double* mass, double* phi, int box[6], int tight[6],
+                        double* dm, double* dphi, int* dext) {
   const Dims& d = ctx->d;
   const long n = d.nodes;
-  DevBuf dm, dphi, dext;
-  if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
-      hipMalloc(&dext.p, 6 * sizeof(int)) != hipSuccess) {
-    ctx->err = "compute_sdf: device allocation failed";
-    return ESF_ERR_OOM;
-  }
-  const int init[6] = {d.X, d.Y, d.Z, 0, 0, 0};
-  ESF_HIP(hipMemcpyAsync(dm.p, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipMemcpyAsync(dext.p, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(dm, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(kExtInit, dim3(1), dim3(64), 0, ctx->stream, dext, d.X, d.Y, d.Z);
   hipLaunchKernelGGL(kSdfInit, dim3(std::min<long>(cdiv(n, 256), 4096)), dim3(256), 0,
-                     ctx->stream, n, d.X, d.Y, (const double*)dm.p, (double)(d.X + d.Y + d.Z + 2),
-                     (double*)dphi.p, (int*)dext.p);
+                     ctx->stream, n, d.X, d.Y, (const double*)dm, (double)(d.X + d.Y + d.Z + 2),
+                     dphi, dext);
   int ext[6];
-  ESF_HIP(hipMemcpyAsync(ext, dext.p, sizeof(ext), hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(ext, dext, sizeof(ext), hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   const int res[3] = {d.X, d.Y, d.Z};
   const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
@@ -324,13 +340,80 @@ esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int bo
   if (sweep) {
     const SweepBox b{d.X, d.Y, d.Z, box[0], box[1], box[2], box[3], box[4], box[5]};
     for (int r = 0; r < 2; r++) {
-      esf_status st = sweepPhiDevice(ctx, b, (const double*)dm.p, (double*)dphi.p);
+      esf_status st = sweepPhiDevice(ctx, b, dm, dphi);
       if (st) return st;
     }
   }
-  ESF_HIP(hipMemcpyAsync(phi, dphi.p, 8 * n, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(phi, dphi, 8 * n, hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   return ESF_OK;
+}
+
+constexpr int kScratchDevices = 64;
+std::mutex gScratchLock;
+double* gScratch[kScratchDevices] = {};
+
+// the scratch array's address on device `dev`, looked up with that device
+// current on the first call there
+esf_status scratchOn(esf_ctx* ctx, int dev, double** out) {
+  if (dev < 0 || dev >= kScratchDevices) {
+    ctx->err = "device ordinal out of range for the scratch table";
+    return ESF_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(gScratchLock);
+  if (!gScratch[dev]) {
+    int cur = 0;
+    ESF_HIP(hipGetDevice(&cur));
+    if (cur != dev) ESF_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    const hipError_t back = cur != dev ? hipSetDevice(cur) : hipSuccess;
+    if (back != hipSuccess) {
+      ctx->err = std::string("hipSetDevice back to the caller's device: ") +
+                 hipGetErrorString(back);
+      return ESF_ERR_DEVICE;
+    }
+    if (e != hipSuccess || !p) {
+      ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return ESF_ERR_DEVICE;
+    }
+    gScratch[dev] = (double*)p;
+  }
+  *out = gScratch[dev];
+  return ESF_OK;
+}
+
+}  // namespace
+}  // namespace esf
+
+esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int box[6],
+                           int tight[6]) {
+  if (!ctx || !mass || !phi || !box || !tight) return ESF_ERR_INVALID;
+  const long n = ctx->d.nodes;
+  DevBuf dm, dphi, dext;
+  if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
+      hipMalloc(&dext.p, 6 * sizeof(int)) != hipSuccess) {
+    ctx->err = "compute_sdf: device allocation failed";
+    return ESF_ERR_OOM;
+  }
+  return computeSdfOn(ctx, mass, phi, box, tight, (double*)dm.p, (double*)dphi.p,
+                      (int*)dext.p);
+}
+
+esf_status esf_compute_sdf_noalloc(esf_ctx* ctx, const double* mass, double* phi, int box[6],
+                                   int tight[6]) {
+  if (!ctx || !mass || !phi || !box || !tight) return ESF_ERR_INVALID;
+  if (ctx->d.nodes > kSdfScratchNodes) {
+    ctx->err = "compute_sdf_noalloc: the grid has " + std::to_string(ctx->d.nodes) +
+               " nodes, more than the " + std::to_string(kSdfScratchNodes) +
+               " its scratch holds";
+    return ESF_ERR_INVALID;
+  }
+  double* base = nullptr;
+  esf_status st = scratchOn(ctx, ctx->device, &base);
+  if (st) return st;
+  return computeSdfOn(ctx, mass, phi, box, tight, base + 32, base + 32 + kSdfScratchNodes,
+                      (int*)base);
 }
 
 esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double* mass,

@@ -278,6 +278,18 @@ class CoupledIntegrator3D:
                                                  tight.ctypes.data_as(_abi._I)), "computeSDF")
         return phi, box, tight
 
+    def computeSDFNoAlloc(self, mass):
+        """computeSDF as computeSDF() does, through esf_compute_sdf_noalloc
+        (no device allocation: the library's static scratch)"""
+        m = self._node_array(mass, 1, "mass")
+        phi = np.empty_like(m)
+        box = np.zeros(6, dtype=np.int32)
+        tight = np.zeros(6, dtype=np.int32)
+        check(self.ctx, self.lib.esf_compute_sdf_noalloc(
+            self.ctx, dptr(m), dptr(phi), box.ctypes.data_as(_abi._I),
+            tight.ctypes.data_as(_abi._I)), "computeSDFNoAlloc")
+        return phi, box, tight
+
     def extendVectorField(self, box, mass, phi, field):
         """DEFOR_SOLID_3D::extendVectorField of a (Z, Y, X, 3) node field where
         mass == 0, over box; returns the extended field"""

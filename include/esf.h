@@ -243,6 +243,18 @@ esf_status esf_broad_phase(esf_ctx* ctx, double* alpha);
  * GPU plane by plane. */
 esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int box[6],
                            int tight[6]);
+/* esf_compute_sdf without allocating: no reference counterpart.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same results
+ * as esf_compute_sdf, bitwise; its device staging (mass, phi) and extent
+ * counters live in a static device array of the library, one per GPU, held
+ * while the process lives: 34.3 MB, enough for grids of at most 129^3 =
+ * 2,146,689 nodes (larger grids: ESF_ERR_INVALID).  Every call on a GPU
+ * uses that GPU's one array, so calls that overlap on one GPU (from
+ * several threads or contexts) race; calls in sequence are safe, each
+ * starts its counters afresh. */
+esf_status esf_compute_sdf_noalloc(esf_ctx* ctx, const double* mass, double* phi,
+                                   int box[6], int tight[6]);
 /* DEFOR_SOLID_3D::extendVectorField (DEFOR_SOLID_3D.cpp:566-574): four
  * rounds of sweepVectorField (:521-564; eight sweepOnePass :457-519, then
  * the box faces) on a node field of 3 doubles per node, in place, where
