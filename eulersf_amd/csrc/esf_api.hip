@@ -28,7 +28,7 @@ esf_status stressStage(esf_ctx* ctx, double* energy);
 esf_status solidAdiag(esf_ctx* ctx);
 esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y);
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps, int maxIt,
-                      int slot);
+                      int slot, const double* guess);
 __global__ void kCorotBatch(int n, double lambda, double mu, const double* F, const double* dF,
                             double* out, double* energy, double* dP);
 // esf_grid.hip
@@ -222,6 +222,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
     for (float** f : fs)
       if (!st) st = dalloc(ctx, f, C);
     if (!st) st = dalloc(ctx, &ctx->minv32, N);
+    if (!st) st = dalloc(ctx, &ctx->dvPrev, 3 * N);
   }
   if (!st) st = dalloc(ctx, &ctx->sidx, N);
   const long scanBlocks = std::max<long>(N, C) / 1024 + 2;
@@ -508,14 +509,25 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
       *bytes = 8 * N;
       if (forWrite) {
         so.listDirty = true;
+        so.dirtyAll = true;
         ctx->massGen++;
       }
       break;
-    case ESF_S_VALID: *ptr = so.valid; *bytes = C; if (forWrite) so.listDirty = true; break;
+    case ESF_S_VALID:
+      *ptr = so.valid;
+      *bytes = C;
+      if (forWrite) {
+        so.listDirty = true;
+        so.dirtyAll = true;
+      }
+      break;
     case ESF_S_QFRAC:
       *ptr = so.qfrac;
       *bytes = 64 * N;
-      if (forWrite) ctx->massGen++;
+      if (forWrite) {
+        so.dirtyAll = true;
+        ctx->massGen++;
+      }
       break;
     case ESF_S_VEL_WORKSPACE: *ptr = so.velws; *bytes = 24 * N; break;
     case ESF_S_PHI: *ptr = so.phi; *bytes = 8 * N; break;
@@ -646,7 +658,14 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
     StageTimer t(ctx, 2);
     if ((st = solidAdiag(ctx))) return st;
     if ((st = solidRhs(ctx))) return st;
-    if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, ctx->p.implicit_eps, maxIt, 0))) return st;
+    // the mixed form starts the solid solves from the previous step's
+    // solutions at the same nodes (v0, and the correction kept in dvPrev);
+    // the FP64 form starts from 0 as the reference does
+    // (CONJUGATE_RESIDUAL.inl:197-198)
+    const bool warm = ctx->p.precision == ESF_MIXED;
+    if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, ctx->p.implicit_eps, maxIt, 0,
+                         warm ? ctx->v0 : nullptr)))
+      return st;
     if ((st = scatterSolidV0(ctx))) return st;
     // the fluid's body forces, if they ran beside the solve (esf_step)
     if ((st = joinBodyForces(ctx))) return st;
@@ -672,7 +691,9 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
   }
   {
     StageTimer t(ctx, 2);
-    if ((st = solidSolve(ctx, 2, ctx->sB, ctx->sDv, ctx->p.implicit_eps, maxIt, 2))) return st;
+    if ((st = solidSolve(ctx, 2, ctx->sB, ctx->sDv, ctx->p.implicit_eps, maxIt, 2,
+                         ctx->p.precision == ESF_MIXED ? ctx->dvPrev : nullptr)))
+      return st;
     if ((st = solidVelocity(ctx))) return st;
   }
   return ESF_OK;

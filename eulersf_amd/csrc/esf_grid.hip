@@ -115,6 +115,40 @@ esf_status compact(esf_ctx* ctx, long n, Flag flag, Emit emit, int* total) {
   return ESF_OK;
 }
 
+// a box of nodes or cells, [lo, hi) per axis, walked x fastest
+struct BoxRange {
+  int lo[3], hi[3];
+  __host__ __device__ long count() const {
+    return (long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  }
+  __device__ void at(long t, int& x, int& y, int& z) const {
+    const int nx = hi[0] - lo[0], ny = hi[1] - lo[1];
+    x = lo[0] + (int)(t % nx);
+    y = lo[1] + (int)((t / nx) % ny);
+    z = lo[2] + (int)(t / ((long)nx * ny));
+  }
+  __device__ long node(const Dims& d, long t) const {
+    int x, y, z;
+    at(t, x, y, z);
+    return x + (long)y * d.X + (long)z * d.slab;
+  }
+};
+
+// the box spanning a and b, clipped to [0, lim); an empty box where both are
+inline BoxRange spanBoxes(const int a[6], const int b[6], const int lim[3]) {
+  BoxRange r;
+  const bool ea = a[0] >= a[3] || a[1] >= a[4] || a[2] >= a[5];
+  const bool eb = b[0] >= b[3] || b[1] >= b[4] || b[2] >= b[5];
+  for (int k = 0; k < 3; k++) {
+    int lo = ea ? b[k] : eb ? a[k] : std::min(a[k], b[k]);
+    int hi = ea ? b[3 + k] : eb ? a[3 + k] : std::max(a[3 + k], b[3 + k]);
+    r.lo[k] = std::max(lo, 0);
+    r.hi[k] = std::min(hi, lim[k]);
+    if (r.hi[k] < r.lo[k]) r.hi[k] = r.lo[k];
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // gatherSolidMasses  COUPLED_INTEGRATOR_3D.cpp:327-375
 // ---------------------------------------------------------------------------
@@ -124,9 +158,12 @@ struct SolidPtrs {
   const double* qfrac[kMaxSolids];
 };
 
-__global__ void kGatherMass(Dims d, SolidPtrs sp, double* solidMass, double* nodeFrac) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= d.nodes) return;
+// over the nodes of box g ([lo, hi) per axis); outside it both fields are 0
+__global__ void kGatherMass(Dims d, SolidPtrs sp, BoxRange g, double* solidMass,
+                            double* nodeFrac) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= g.count()) return;
+  const long n = g.node(d, t);
   double m = 0;
   for (int b = 0; b < sp.n; b++) m += sp.mass[b][n];
   solidMass[n] = m;
@@ -236,8 +273,29 @@ esf_status gatherSolidMasses(esf_ctx* ctx) {
     sp.mass[b] = ctx->solids[b].mass;
     sp.qfrac[b] = ctx->solids[b].qfrac;
   }
-  hipLaunchKernelGGL(kGatherMass, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, sp,
-                     ctx->solidMass, ctx->nodeFrac);
+  // the gathered fields are nonzero only at the nodes of the solids' last
+  // mass passes (their dirty cells and one node beyond); recompute those and
+  // the region the last gather left nonzero
+  const int lim[3] = {d.X, d.Y, d.Z};
+  const int full[6] = {0, 0, 0, d.X, d.Y, d.Z};
+  int now[6] = {0, 0, 0, 0, 0, 0};
+  for (auto& so : ctx->solids) {
+    int nb[6];
+    for (int k = 0; k < 3; k++) {
+      nb[k] = so.dirty[k];
+      nb[3 + k] = so.dirty[3 + k] + 1;
+    }
+    const BoxRange u = spanBoxes(now, so.dirtyAll ? full : nb, lim);
+    for (int k = 0; k < 3; k++) {
+      now[k] = u.lo[k];
+      now[3 + k] = u.hi[k];
+    }
+  }
+  const BoxRange gb = spanBoxes(ctx->gatherBox, now, lim);
+  for (int k = 0; k < 6; k++) ctx->gatherBox[k] = now[k];
+  if (gb.count() > 0)
+    hipLaunchKernelGGL(kGatherMass, dim3(cdiv(gb.count(), 256)), dim3(256), 0, ctx->stream, d,
+                       sp, gb, ctx->solidMass, ctx->nodeFrac);
   // count solid DOF nodes, grow, then number them
   SolidFlag fl{d, ctx->solidMass};
   const long n = d.nodes;
@@ -328,34 +386,67 @@ __device__ inline bool inTight(const Box& b, int x, int y, int z) {
          z >= b.tMin[2] && z < b.tMax[2] - 1;
 }
 
-// one thread per tight-box cell: |det F^-1| and the subsampled quarter
-// volume fractions (512 samples with dh / mass sample dh = 8)
-__global__ void kMassSamples(Dims d, MassArgs a) {
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (c >= d.cells) return;
-  const int x = c % d.CX, y = (c / d.CX) % d.CY, z = c / d.cslab;
-  if (!inTight(a.box, x, y, z)) return;
+// clear q, _valid and |det F^-1| over the cells of box u (the reference
+// clears the whole fields, :578-581; outside u they are already 0)
+__global__ void kMassClear(Dims d, BoxRange u, double* qfrac, int8_t* valid, double* detFInv) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= u.count()) return;
+  int x, y, z;
+  u.at(t, x, y, z);
   const long index = x + (long)y * d.X + (long)z * d.slab;
-  a.detFInv[index] = fabs(1.0 / detF(d, a.u, index));
+  for (int i = 0; i < 8; i++) qfrac[8 * index + i] = 0.0;
+  detFInv[index] = 0.0;
+  valid[x + (long)y * d.CX + (long)z * d.cslab] = 0;
+}
+
+// one wave per tight-box cell: |det F^-1| (lane 0), and the cell's
+// msr^3 subsamples spread over the lanes (sample s = (p msr + q) msr + r in
+// the reference's loop order, lane l taking s = l, l + 64, ...); per octant
+// the count of samples that carry mass comes from wave ballots.  The
+// reference adds `fraction` to the octant's q once per such sample, so q is
+// that many additions of one constant, which depends on the count only:
+// lane 0 performs them, then the sum, the 0.5 threshold and the flags.
+__global__ void __launch_bounds__(256) kMassSamples(Dims d, MassArgs a, BoxRange t) {
+  const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= t.count()) return;  // uniform per wave
+  int x, y, z;
+  t.at(wave, x, y, z);
+  const long index = x + (long)y * d.X + (long)z * d.slab;
+  const long c = x + (long)y * d.CX + (long)z * d.cslab;
+  if (lane == 0) a.detFInv[index] = fabs(1.0 / detF(d, a.u, index));
   // usesdf (DEFOR_SOLID_3D.cpp:620): no samples, q and _valid stay cleared
   if (a.phi && a.phi[index] > 2) return;
   const FieldGeom g = gridGeom(d);
   const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
-  double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int p = 0; p < a.msr; p++)
-    for (int qq = 0; qq < a.msr; qq++)
-      for (int r = 0; r < a.msr; r++) {
-        const double pos[3] = {node[0] + (p + 0.5) * a.sdh, node[1] + (qq + 0.5) * a.sdh,
-                               node[2] + (r + 0.5) * a.sdh};
-        double rPos[3];
-        interpV(a.X, g, pos, rPos);
-        const double dd = interpS(a.rest, a.rg, rPos) * a.subV;
-        if (dd < 1e-6) continue;
-        int o = p < a.half ? 0 : 1;
-        o += qq < a.half ? 0 : 2;
-        o += r < a.half ? 0 : 4;
-        q[o] += a.fraction;
-      }
+  const int m2 = a.msr * a.msr, m3 = m2 * a.msr;
+  int count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int base = 0; base < m3; base += 64) {
+    const int s = base + lane;
+    bool hit = false;
+    int o = 0;
+    if (s < m3) {
+      const int p = s / m2, qq = (s / a.msr) % a.msr, r = s % a.msr;
+      const double pos[3] = {node[0] + (p + 0.5) * a.sdh, node[1] + (qq + 0.5) * a.sdh,
+                             node[2] + (r + 0.5) * a.sdh};
+      double rPos[3];
+      interpV(a.X, g, pos, rPos);
+      const double dd = interpS(a.rest, a.rg, rPos) * a.subV;
+      hit = !(dd < 1e-6);
+      o = p < a.half ? 0 : 1;
+      o += qq < a.half ? 0 : 2;
+      o += r < a.half ? 0 : 4;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) count[k] += __popcll(__ballot(hit && o == k));
+  }
+  if (lane != 0) return;
+  double q[8];
+  for (int k = 0; k < 8; k++) {
+    double acc = 0;
+    for (int j = 0; j < count[k]; j++) acc += a.fraction;
+    q[k] = acc;
+  }
   double sum = 0;
   for (int i = 0; i < 8; i++) sum += q[i];
   int8_t v = 0;
@@ -370,11 +461,14 @@ __global__ void kMassSamples(Dims d, MassArgs a) {
   a.valid[c] = v;
 }
 
-// mass per node: corners i = 0..7 in the reference's outer-loop order
-__global__ void kMassGather(Dims d, MassArgs a) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= d.nodes) return;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+// mass per node of box u: corners i = 0..7 in the reference's outer-loop
+// order
+__global__ void kMassGather(Dims d, MassArgs a, BoxRange u) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= u.count()) return;
+  int x, y, z;
+  u.at(t, x, y, z);
+  const long n = x + (long)y * d.X + (long)z * d.slab;
   double m = 0;
   for (int i = 0; i < 8; i++) {
     const int cx = x - (i & 1), cy = y - ((i >> 1) & 1), cz = z - (i >> 2);
@@ -391,9 +485,18 @@ __global__ void kMassGather(Dims d, MassArgs a) {
 esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   const Dims& d = ctx->d;
   DevSolid& so = ctx->solids[solid];
-  hipLaunchKernelGGL(kZero, dim3(1024), dim3(256), 0, ctx->stream, d.nodes, so.detFInv);
-  hipLaunchKernelGGL(kZero, dim3(1024), dim3(256), 0, ctx->stream, 8 * d.nodes, so.qfrac);
-  hipLaunchKernelGGL(kZero8, dim3(1024), dim3(256), 0, ctx->stream, d.cells, so.valid);
+  // T: the tight box's cells; U: T and the cells the last pass (or an
+  // upload) may have left nonzero
+  const int climit[3] = {d.CX, d.CY, d.CZ};
+  const int fullCells[6] = {0, 0, 0, d.CX, d.CY, d.CZ};
+  const int tc[6] = {so.box.tMin[0], so.box.tMin[1], so.box.tMin[2], so.box.tMax[0] - 1,
+                     so.box.tMax[1] - 1, so.box.tMax[2] - 1};
+  const int none[6] = {0, 0, 0, 0, 0, 0};
+  const BoxRange T = spanBoxes(tc, none, climit);
+  const BoxRange U = spanBoxes(so.dirtyAll ? fullCells : so.dirty, tc, climit);
+  if (U.count() > 0)
+    hipLaunchKernelGGL(kMassClear, dim3(cdiv(U.count(), 256)), dim3(256), 0, ctx->stream, d, U,
+                       so.qfrac, so.valid, so.detFInv);
   MassArgs a{};
   a.box = so.box;
   a.u = so.u;
@@ -411,8 +514,23 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   a.valid = so.valid;
   a.mass = so.mass;
   a.phi = usesdf ? so.phi : nullptr;
-  hipLaunchKernelGGL(kMassSamples, dim3(cdiv(d.cells, 128)), dim3(128), 0, ctx->stream, d, a);
-  hipLaunchKernelGGL(kMassGather, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, a);
+  if (T.count() > 0)
+    hipLaunchKernelGGL(kMassSamples, dim3(cdiv(64 * T.count(), 256)), dim3(256), 0, ctx->stream,
+                       d, a, T);
+  // the masses of U's nodes (a cell's corners reach one node past it); the
+  // other nodes keep 0 (or a mass that was uploaded: then U is the grid)
+  const int nlimit[3] = {d.X, d.Y, d.Z};
+  const int un[6] = {U.lo[0], U.lo[1], U.lo[2], U.hi[0] + 1, U.hi[1] + 1, U.hi[2] + 1};
+  const BoxRange UN = spanBoxes(so.dirtyAll ? fullCells : un, none, nlimit);
+  const BoxRange M = so.dirtyAll ? BoxRange{{0, 0, 0}, {d.X, d.Y, d.Z}} : UN;
+  if (M.count() > 0)
+    hipLaunchKernelGGL(kMassGather, dim3(cdiv(M.count(), 256)), dim3(256), 0, ctx->stream, d, a,
+                       M);
+  for (int k = 0; k < 3; k++) {
+    so.dirty[k] = T.lo[k];
+    so.dirty[3 + k] = T.hi[k];
+  }
+  so.dirtyAll = false;
   so.listDirty = true;
   ctx->massGen++;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
@@ -789,7 +907,7 @@ __global__ void kScatterSolid(int nSolid, const int* solidNodes, const double* s
 // solid solve (:724-729)
 __global__ void kCorrect(Dims d, const double* __restrict__ lam, const double* __restrict__ v0,
                          const double* __restrict__ minv, const int* __restrict__ sidx,
-                         double* vel, double* sB) {
+                         double* vel, double* sB, double* dvPrev) {
   const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int i = n % d.X, j = (n / d.X) % d.Y, k = n / d.slab;
@@ -813,16 +931,23 @@ __global__ void kCorrect(Dims d, const double* __restrict__ lam, const double* _
     vel[3 * n] = v0[3 * n] + a0 * mi;
     vel[3 * n + 1] = v0[3 * n + 1] + a1 * mi;
     vel[3 * n + 2] = v0[3 * n + 2] + a2 * mi;
+    if (dvPrev) {
+      dvPrev[3 * n] = a0 * mi;
+      dvPrev[3 * n + 1] = a1 * mi;
+      dvPrev[3 * n + 2] = a2 * mi;
+    }
   }
 }
 
 // v1 at solid nodes = v0 + delta_s  (:748-753)
 __global__ void kSolidVelocity(int nSolid, const int* solidNodes, const double* v0,
-                               const double* dv, double* vel) {
+                               const double* dv, double* vel, double* dvPrev) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
   for (int c = 0; c < 3; c++) vel[3 * n + c] = v0[3 * n + c] + dv[3 * (long)s + c];
+  if (dvPrev)
+    for (int c = 0; c < 3; c++) dvPrev[3 * n + c] = dv[3 * (long)s + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1185,14 +1310,15 @@ esf_status scatterSolidV0(esf_ctx* ctx) {
 }
 
 esf_status correctVelocity(esf_ctx* ctx) {
-  LAUNCH_NODES(kCorrect, ctx->d, ctx->lambda, ctx->v0, ctx->minv, ctx->sidx, ctx->vel, ctx->sB);
+  LAUNCH_NODES(kCorrect, ctx->d, ctx->lambda, ctx->v0, ctx->minv, ctx->sidx, ctx->vel, ctx->sB,
+               ctx->dvPrev);
   return ESF_OK;
 }
 
 esf_status solidVelocity(esf_ctx* ctx) {
   if (ctx->nSolid)
     hipLaunchKernelGGL(kSolidVelocity, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
-                       ctx->nSolid, ctx->solidNodes, ctx->v0, ctx->sDv, ctx->vel);
+                       ctx->nSolid, ctx->solidNodes, ctx->v0, ctx->sDv, ctx->vel, ctx->dvPrev);
   return ESF_OK;
 }
 

@@ -69,6 +69,11 @@ struct DevSolid {
   double* qfrac = nullptr;                  // 8*nodes
   int8_t* valid = nullptr;                  // cells
   Box box;
+  // cells where qfrac, valid and detFInv may be nonzero (the tight box of
+  // the last mass pass; [lo, hi) per axis), or anywhere after an upload of
+  // the mass fields: the next mass pass clears only that region
+  int dirty[6] = {0, 0, 0, 0, 0, 0};
+  bool dirtyAll = false;
   // compact list of cells with _valid == 1 inside the tight box
   int nValid = 0, capValid = 0;
   int* cellList = nullptr;   // node index of each valid cell's min corner
@@ -113,12 +118,19 @@ struct esf_ctx {
   // write of them); the gathered fields above are rebuilt only when it moved
   long massGen = 1, gatheredGen = 0;
   int gatheredSolids = -1;
+  // nodes where the gathered mass and fraction may be nonzero ([lo, hi) per
+  // axis; the whole grid until the first gather)
+  int gatherBox[6] = {0, 0, 0, 1 << 30, 1 << 30, 1 << 30};
   uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
   double* inbox = nullptr;      // solid b at 24 capSolid b: 24 doubles per node
   int inboxSolids = 0;          // solids the inbox was sized for
   int* scanBlocks = nullptr;  // scan scratch
   // solve: node-based
   double *v0 = nullptr, *minv = nullptr;  // 3*nodes, nodes
+  // mixed form: the velocity correction of the last solve at every interior
+  // node (solid DOFs and fluid nodes), the next solid correction solve's
+  // warm start
+  double* dvPrev = nullptr;
   float* minv32 = nullptr;
   // solve: cell-based (pressure)
   double *c = nullptr, *b = nullptr, *D = nullptr, *lambda = nullptr,

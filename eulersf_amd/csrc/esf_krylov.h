@@ -19,6 +19,8 @@ enum FinishOp {
   F_CHECK = 3,    // |Ax - b| = sqrt(t0); done when below eps |b|
   F_IR = 4,       // outer residual sqrt(t0); |b| = sqrt(t1) on the first call
   F_CR = 6,       // delta, beta from t0; alpha from t1 + 2 beta t2 + beta^2 t3
+  F_WARM = 7,     // |b| = sqrt(t0), |b - A x0| = sqrt(t1); done at once when that
+                  // is already below eps |b| (a warm start that needs no iteration)
 };
 
 // hostOut (pinned host memory, may be null): the scalar block after the
@@ -39,6 +41,13 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
   }
   if constexpr (OP == F_BNORM) {
     sc->bnorm = sqrt(t[0]);
+  } else if constexpr (OP == F_WARM) {
+    sc->bnorm = sqrt(t[0]);
+    sc->resnorm = sqrt(t[1]);
+    if (sc->resnorm < sc->eps * sc->bnorm) {
+      sc->done = 1;
+      sc->iter = 0;
+    }
   } else if constexpr (OP == F_SETUP_S) {
     sc->delta = t[0];
     sc->beta = 0.0;

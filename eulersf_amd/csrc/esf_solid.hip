@@ -768,8 +768,36 @@ __global__ void kSolidSetup(long n, const double* b, const double* diag, double*
   blockPartials<1>(mine, partials, red);
 }
 
+// a warm start: x at each solid DOF from a node field (3 doubles per node)
+__global__ void kWarmGather(int nSolid, const int* solidNodes, const double* guess, double* x) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nSolid) return;
+  const long n = solidNodes[s];
+  for (int c = 0; c < 3; c++) x[3L * s + c] = guess[3 * n + c];
+}
+
+// the CR set-up from a warm start x0, given y = A x0: r = D^-1 (b - y), no
+// direction yet; partial sums |b|^2 and |b - A x0|^2
+__global__ void kSolidSetupWarm(long n, const double* b, const double* diag, const double* y,
+                                double* r, double* p, double* Ap, double* apd,
+                                double* partials) {
+  __shared__ double red[2 * 4];
+  double part[2] = {0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double res = b[i] - y[i];
+    r[i] = res / diag[i];
+    p[i] = 0.0;
+    Ap[i] = 0.0;
+    apd[i] = 0.0;
+    part[0] += b[i] * b[i];
+    part[1] += res * res;
+  }
+  blockPartials<2>(part, partials, red);
+}
+
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps,
-                      int maxIt, int slot) {
+                      int maxIt, int slot, const double* guess) {
   if (ctx->solids.size() > (size_t)kMaxSolids) {
     ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
@@ -791,10 +819,26 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   CrScalars* sc = &ctx->sc[which];
   initScalarsAsync(ctx, sc, eps);
   const int eb = std::max(1, std::min(cdiv(n, 1024), 256));
-  hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
-                     (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->sApd,
-                     ctx->partials);
-  launchFinish<F_BNORM, 1>(ctx, eb, sc);
+  // a check is waited for once two iterations are queued behind it
+  DonePoller poll(ctx, which, false, 2);
+  if (guess) {
+    // warm start x0 from the node field: r = D^-1 (b - A x0), and done at
+    // once if x0 already meets the tolerance
+    hipLaunchKernelGGL(kWarmGather, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
+                       ctx->nSolid, (const int*)ctx->solidNodes, guess, x);
+    esf_status st = solidApplyDevice(ctx, x, ctx->stmp);
+    if (st) return st;
+    hipLaunchKernelGGL(kSolidSetupWarm, dim3(eb), dim3(256), 0, ctx->stream, n, b,
+                       (const double*)ctx->sAdiag, (const double*)ctx->stmp, ctx->sr, ctx->sp,
+                       ctx->sAp, ctx->sApd, ctx->partials);
+    launchFinish<F_WARM, 2>(ctx, eb, sc, 0, 0, nullptr, poll.slot());
+    poll.mark(-1);
+  } else {
+    hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
+                       (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->sApd,
+                       ctx->partials);
+    launchFinish<F_BNORM, 1>(ctx, eb, sc);
+  }
   // partial sets of 5 x nb: node pass i writes set i mod 3 (the setup pass
   // set 2; row 4 is the check that rides along), a lone check its own set
   const int nb = nodeBlocks(ctx);
@@ -817,8 +861,6 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   nodePass<S_SETUP>(ctx, a);
   a.inboxX = ctx->inboxX;
   a.xb = ctx->sXb;
-  // a check is waited for once two iterations are queued behind it
-  DonePoller poll(ctx, which, false, 2);
   // The true residual of the x that iteration i leaves (every 5 iterations,
   // CONJUGATE_RESIDUAL.inl:246-255) is checked by the passes of iteration
   // i + 1: its cell pass forms K_e x beside K_e r_new, its node pass the
