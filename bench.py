@@ -416,6 +416,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--velocity", type=str, default="0,0,0",
+                    help="the ball's initial velocity (vx,vy,vz); the workload is at rest")
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--roofline-n", type=int, default=256,
                     help="grid of the HBM-bound SpMV measurement (0 to skip)")
@@ -483,7 +485,8 @@ def main():
               flush=True)
         return
     n = args.n
-    text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000)
+    vel = tuple(float(v) for v in args.velocity.split(","))
+    text = unit_cube_ball_cfg(n, eps=args.eps, max_iterations=20000, velocity=vel)
     n_cells, n_int = n ** 3, (n - 1) ** 3
     if args.traffic_run:
         traffic_run(esf, text)
@@ -569,7 +572,7 @@ def main():
         "data": "synthetic scene (analytic sphere), no external data",
         "config": {"workload": f"unit-cube rubber ball, {n}^3 cells ({n + 1}^3 nodes), "
                                "sphere r=0.15 E=1e4 nu=0.3, rho 1000:1000, dt=1/240, "
-                               f"solves to {args.eps:g}",
+                               f"initial velocity {vel}, solves to {args.eps:g}",
                    "grid_cells": n_cells, "solid_nodes": n_sn, "solid_cells": n_sc,
                    "parallelism": "single GPU"},
         "fp64": {"steps_per_s": round(args.steps / wall64, 3),

@@ -116,10 +116,13 @@ class Server:
     steps overlap on the GPU with everyone else's."""
 
     def __init__(self, budget_bytes, max_active, precision=_abi.ESF_MIXED,
-                 bytes_per_node=1200.0):
+                 bytes_per_node=1200.0, on_done=None):
         self.budget = budget_bytes
         self.max_active = max_active
         self.precision = precision
+        # on_done(request, context): called by the request's worker after its
+        # K steps, before the context is released (e.g. to read results)
+        self.on_done = on_done
         # admission estimate per grid node, raised to what contexts really hold
         self.bytes_per_node = bytes_per_node
         self.lock = threading.Lock()
@@ -152,6 +155,8 @@ class Server:
                     self.bytes_per_node = max(self.bytes_per_node,
                                               rec["bytes"] / float((req["n"] + 1) ** 3))
                 step_scene(g, req["k"], rec["lat"])
+                if self.on_done is not None:
+                    self.on_done(req, g)
             except Exception as e:  # kept in the record, reported by the caller
                 rec["error"] = repr(e)
             finally:
