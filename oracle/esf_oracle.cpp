@@ -1665,6 +1665,10 @@ void orc_compute_solid_mass(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);
   computeSolidMass(s, s.solids[solid], false);
 }
+double orc_compute_F(void* h, int solid, long index, double* F) {
+  Scene& s = *static_cast<Scene*>(h);
+  return computeF(s, s.solids[solid], index, F);
+}
 void orc_compute_solid_mass_sdf(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);
   computeSolidMass(s, s.solids[solid], true);

@@ -113,6 +113,8 @@ void orc_solid_finalize(void* h);                /* DEFOR :1198-1233 */
 void orc_compute_solid_mass(void* h, int solid); /* DEFOR :576-669 (usesdf=false) */
 void orc_update_boxes(void* h, int solid);       /* DEFOR :216-266 (box part) */
 void orc_compute_solid_mass_sdf(void* h, int solid); /* DEFOR :576-669 (usesdf=true) */
+/* DEFOR :671-682: F of the cell with min-corner node `index` (row-major), returns det F */
+double orc_compute_F(void* h, int solid, long index, double* F);
 void orc_solid_sdf(void* h, int solid);          /* DEFOR :216-267 (boxes + phi) */
 void orc_gather(void* h);                        /* :327-375 */
 void orc_advect_velocity(void* h);               /* :797-826 */

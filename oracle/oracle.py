@@ -76,6 +76,8 @@ def lib():
         L.orc_update_boxes.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_compute_solid_mass_sdf.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_solid_sdf.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_compute_F.restype = ctypes.c_double
+        L.orc_compute_F.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, _D]
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Stats)]
         L.orc_pressure_apply.argtypes = [ctypes.c_void_p, _D, _D]
