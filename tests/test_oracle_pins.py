@@ -579,3 +579,114 @@ def test_displacement_face_extrapolation_on_the_extended_box():
     Xm = s.field(O.S_X, 0).reshape(X, X, X, 3)
     np.testing.assert_allclose(Xm, pos.reshape(X, X, X, 3) - got, atol=1e-15)
     s.close()
+
+
+# ------------------------------------------------ momentum, RHS, projection
+
+def test_solid_momentum_sums_solid_and_fluid_parts():
+    """M v at solid DOFs: m_s v_s from the solid's own velocity field
+    (computeMV, DEFOR_SOLID_3D.cpp:873-889) plus m_f v from the grid
+    velocity (COUPLED_INTEGRATOR_3D.cpp:592-593)"""
+    n = 8
+    dh = 1.0 / n
+    s = block_scene(n, (2 * dh, 2 * dh, 2 * dh), (5 * dh, 5 * dh, 5 * dh))
+    X = n + 1
+    rng = np.random.default_rng(14)
+    vs = rng.standard_normal((X ** 3, 3))
+    vg = rng.standard_normal((X ** 3, 3))
+    s.set_field(O.S_VELOCITY, vs.reshape(-1), 0)
+    s.set_field(O.VELOCITY, vg.reshape(-1))
+    s.L.orc_set_dt(s.h, 1.0 / 240)
+    s.L.orc_forces(s.h)
+    sidx = s.field(O.SOLID_INDEX)
+    nodes = np.flatnonzero(sidx >= 0)[np.argsort(sidx[sidx >= 0])]
+    ms = s.field(O.S_MASS, 0)[nodes]
+    mf = np.maximum(1 - s.field(O.NODE_FRACTION)[nodes], 0) * 1000.0 * dh ** 3
+    want = ms[:, None] * vs[nodes] + mf[:, None] * vg[nodes]
+    np.testing.assert_allclose(s.field(O.SOLID_MV).reshape(-1, 3), want, rtol=1e-14, atol=1e-18)
+    s.close()
+
+
+def test_divergence_rhs_completes_the_cell_divergence():
+    """c carries the boundary nodes' share of the Q1 cell divergence, so for a
+    linear velocity v = A x + b on every node, J v_interior - c = 4 dh tr(A)
+    in every cell, the boundary cells included (COUPLED_INTEGRATOR_3D.cpp:
+    286-325 with :230-266)"""
+    n = 7
+    dh = 1.0 / n
+    s = fluid_scene(n)
+    X = n + 1
+    rng = np.random.default_rng(15)
+    A = rng.standard_normal((3, 3))
+    b = rng.standard_normal(3)
+    v = node_pos(n) @ A.T + b
+    s.set_field(O.VELOCITY, v.reshape(-1))
+    s.L.orc_set_dt(s.h, 1.0 / 240)
+    s.L.orc_forces(s.h)
+    s.L.orc_assemble(s.h)
+    inner = v.reshape(X, X, X, 3)[1:-1, 1:-1, 1:-1].reshape(-1)
+    got = s.apply_J(inner) - s.field(O.DIV_RHS)
+    np.testing.assert_allclose(got, 4 * dh * np.trace(A), rtol=1e-12, atol=1e-13)
+    s.close()
+
+
+def test_fluid_projection_leaves_the_velocity_divergence_free():
+    """without solids the Schur solve is a pressure projection: v1 = v0 -
+    M^-1 J^T lambda with J M^-1 J^T lambda = J v0 - c, so J v1 = c up to the
+    solve's tolerance (the discrete divergence, boundary share included,
+    vanishes)"""
+    n = 10
+    s = fluid_scene(n, gravity=9.8, divfree_eps=1e-12)
+    X = n + 1
+    rng = np.random.default_rng(16)
+    v = np.zeros((X, X, X, 3))
+    v[1:-1, 1:-1, 1:-1] = rng.standard_normal((X - 2, X - 2, X - 2, 3))
+    s.set_field(O.VELOCITY, v.reshape(-1))
+    s.L.orc_set_dt(s.h, 1.0 / 240)
+    s.L.orc_forces(s.h)
+    s.L.orc_assemble(s.h)
+    st = O.Stats()
+    s.L.orc_solve(s.h, st)
+    assert st.converged[1] == 1
+    v1 = s.field(O.VELOCITY).reshape(X, X, X, 3)[1:-1, 1:-1, 1:-1].reshape(-1)
+    v0 = s.field(O.V0)
+    r = s.apply_J(v1) - s.field(O.DIV_RHS)
+    assert np.linalg.norm(r) < 1e-10 * np.linalg.norm(s.apply_J(v0) - s.field(O.DIV_RHS))
+    s.close()
+
+
+def test_recover_and_dirichlet_faces():
+    """the solid velocity replaces the grid velocity where the solid has mass
+    inside its extended box (recoverVelocityField, DEFOR_SOLID_3D.cpp:
+    1235-1241), then every face node's velocity is zero and the intensity on
+    the y = 0 face is zero (COUPLED_INTEGRATOR_3D.cpp:900-916)"""
+    n = 8
+    dh = 1.0 / n
+    s = block_scene(n, (2 * dh, 2 * dh, 2 * dh), (5 * dh, 5 * dh, 5 * dh))
+    X = n + 1
+    rng = np.random.default_rng(17)
+    vg = rng.standard_normal((X ** 3, 3))
+    vs = rng.standard_normal((X ** 3, 3))
+    inten = rng.uniform(0.1, 1, X ** 3)
+    s.set_field(O.VELOCITY, vg.reshape(-1))
+    s.set_field(O.S_VELOCITY, vs.reshape(-1), 0)
+    s.set_field(O.INTENSITY, inten)
+    s.L.orc_recover_and_bc(s.h)
+    got = s.field(O.VELOCITY).reshape(X, X, X, 3)
+    m = s.field(O.S_MASS, 0).reshape(X, X, X)
+    bx = s.boxes(0)[6:]
+    want = vg.reshape(X, X, X, 3).copy()
+    for z in range(bx[2], bx[5] - 1):
+        for y in range(bx[1], bx[4] - 1):
+            for x in range(bx[0], bx[3] - 1):
+                if m[z, y, x] > 0:
+                    want[z, y, x] = vs.reshape(X, X, X, 3)[z, y, x]
+    for sl in ((0,), (X - 1,)):
+        want[sl] = 0
+        want[:, sl[0]] = 0
+        want[:, :, sl[0]] = 0
+    np.testing.assert_array_equal(got, want)
+    iw = inten.reshape(X, X, X).copy()
+    iw[:, 0, :] = 0
+    np.testing.assert_array_equal(s.field(O.INTENSITY).reshape(X, X, X), iw)
+    s.close()
