@@ -56,7 +56,7 @@ KERNEL_CLASSES = ["spmv64_apply", "spmv64_setup_r", "spmv64_setup_s", "spmv64_up
                   "solid_cell_crx", "solid_node_crx"]
 
 STAGES = ["cfl", "forces", "solid_solves", "pressure_solve", "solid_finalize", "gather",
-          "advect_velocity", "advect_intensity", "recover_bc", "solid_mass_sdf"]
+          "advect_velocity", "advect_intensity", "recover_bc", "solid_mass_sdf", "flip"]
 
 
 class Grid(ctypes.Structure):
@@ -72,7 +72,10 @@ class Params(ctypes.Structure):
                 ("divfree_eps", ctypes.c_double), ("cr_max_iterations", ctypes.c_int),
                 ("mass_sample_dh", ctypes.c_double), ("tight_border", ctypes.c_int),
                 ("extrapolation_cells", ctypes.c_int), ("precision", ctypes.c_int),
-                ("inner_eps", ctypes.c_double)]
+                ("inner_eps", ctypes.c_double), ("particles_per_cell", ctypes.c_int),
+                ("flip_alpha", ctypes.c_double), ("flip_cutoff", ctypes.c_int),
+                ("wall_thickness", ctypes.c_double), ("spring_stiffness", ctypes.c_double),
+                ("friction", ctypes.c_double), ("flip", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -165,6 +168,13 @@ _PROTOS = {
     "esf_solid_get_mesh": (_S, [_P, ctypes.c_int, _D, _D, _I]),
     "esf_advect_surface_mesh": (_S, [_P]),
     "esf_broad_phase": (_S, [_P, _D]),
+    "esf_init_particles": (_S, [_P, ctypes.c_int]),
+    "esf_grid_to_particle": (_S, [_P, ctypes.c_int]),
+    "esf_particle_to_grid": (_S, [_P, ctypes.c_int]),
+    "esf_extend_solid_velocity": (_S, [_P, ctypes.c_int]),
+    "esf_num_particles": (_S, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]),
+    "esf_get_particles": (_S, [_P, ctypes.c_int, _D, _D]),
+    "esf_set_particles": (_S, [_P, ctypes.c_int, ctypes.c_long, _D, _D]),
     "esf_compute_sdf": (_S, [_P, _D, _D, _I, _I]),
     "esf_compute_sdf_noalloc": (_S, [_P, _D, _D, _I, _I]),
     "esf_extend_vector_field": (_S, [_P, _I, _D, _D, _D]),

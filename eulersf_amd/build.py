@@ -34,6 +34,7 @@ SOURCES = [
     os.path.join(CSRC, "esf_grid.hip"),
     os.path.join(CSRC, "esf_mesh.hip"),
     os.path.join(CSRC, "esf_sweep.hip"),
+    os.path.join(CSRC, "esf_flip.hip"),
     os.path.join(CSRC, "esf_api.hip"),
     os.path.join(HOST, "esf_scene.cpp"),
     os.path.join(HOST, "esf_frames.cpp"),

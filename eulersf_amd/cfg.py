@@ -90,6 +90,15 @@ def params_from_cfg(cfg, dh):
         "mass_sample_dh": cfg.get_float("mass sample dh", dh / 10),
         "tight_border": cfg.get_int("tight border", 3),
         "extrapolation_cells": cfg.get_int("extrapolation cells", 20),
+        # FLIP particles (DEFOR_SOLID_3D.cpp:336-337, 901, 948-952, 1101); the
+        # round trip runs in the step only when "flip particles" is set
+        "particles_per_cell": cfg.get_int("number of particles per material cell", 8),
+        "flip_alpha": cfg.get_float("flip alpha", 0.95),
+        "flip_cutoff": cfg.get_int("flip cutoff", 3),
+        "wall_thickness": cfg.get_float("wall thickness", dh * 1.5),
+        "spring_stiffness": cfg.get_float("contact spring stiffness", 10000.0),
+        "friction": cfg.get_float("friction", 0.5),
+        "flip": cfg.get_int("flip particles", 0),
     }
 
 
@@ -111,7 +120,7 @@ def mass_sample_dh(n):
 
 def scene_cfg(n, spheres, fill=0.3, eps=1e-10, max_iterations=20000, fluid_density=1000.0,
               dt=1.0 / 240.0, vorticity_eps=1.5, gravity=9.8, frames=250, buoyancy=0.0,
-              rayleigh_alpha=0.0, rayleigh_beta=0.0):
+              rayleigh_alpha=0.0, rayleigh_beta=0.0, extra=None):
     """cfg text of a unit-cube scene on an n^3-cell grid ((n+1)^3 nodes,
     dh = 1/n): the smoke carpet (fluid pool) fills the bottom `fill` of the
     domain; `spheres` is a list of dicts with keys center, radius, E, nu,
@@ -151,6 +160,7 @@ def scene_cfg(n, spheres, fill=0.3, eps=1e-10, max_iterations=20000, fluid_densi
         "start frame = 0", "end frame = %d" % frames,
         "save data = 0",
     ]
+    lines += ["%s = %s" % (k, v) for k, v in (extra or {}).items()]
     return "\n".join(lines) + "\n"
 
 
@@ -158,15 +168,16 @@ def unit_cube_ball_cfg(n, eps=1e-10, max_iterations=20000, radius=0.15,
                        center=(0.5, 0.65, 0.5), E=1e4, nu=0.3, density=1000.0,
                        fluid_density=1000.0, carpet=0.3, dt=1.0 / 240.0,
                        vorticity_eps=1.5, gravity=9.8, frames=250, velocity=(0, 0, 0),
-                       buoyancy=0.0, rayleigh_alpha=0.0, rayleigh_beta=0.0):
+                       buoyancy=0.0, rayleigh_alpha=0.0, rayleigh_beta=0.0, extra=None):
     """cfg text of the unit-cube ball scene on an n^3-cell grid: (n+1)^3 nodes,
-    dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool."""
+    dh = 1/n, one corotational sphere, the smoke carpet as the fluid pool;
+    `extra`: further cfg keys and values"""
     sphere = {"center": center, "radius": radius, "E": E, "nu": nu, "density": density,
               "velocity": velocity}
     return scene_cfg(n, [sphere], fill=carpet, eps=eps, max_iterations=max_iterations,
                      fluid_density=fluid_density, dt=dt, vorticity_eps=vorticity_eps,
                      gravity=gravity, frames=frames, buoyancy=buoyancy,
-                     rayleigh_alpha=rayleigh_alpha, rayleigh_beta=rayleigh_beta)
+                     rayleigh_alpha=rayleigh_alpha, rayleigh_beta=rayleigh_beta, extra=extra)
 
 
 def random_spheres(rng, k, radius_range, log10_E_range, center_range=None, margin=0.02,

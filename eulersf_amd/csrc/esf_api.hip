@@ -79,6 +79,10 @@ static esf_status dalloc(esf_ctx* ctx, T** p, size_t count) {
   return e == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
+esf_status devAllocBytes(esf_ctx* ctx, void** p, size_t bytes) {
+  return dalloc(ctx, (char**)p, bytes);
+}
+
 esf_status growSolidDofs(esf_ctx* ctx, int need) {
   if (need <= ctx->capSolid && ctx->sForce && ctx->inboxSolids >= (int)ctx->solids.size())
     return ESF_OK;
@@ -597,6 +601,68 @@ esf_status esf_update_boxes(esf_ctx* ctx, int solid) {
   return updateBoxes(ctx, solid);
 }
 
+static esf_status flipCall(esf_ctx* ctx, int solid, esf_status (*fn)(esf_ctx*, int)) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  esf_status st = fn(ctx, solid);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+
+esf_status esf_init_particles(esf_ctx* ctx, int solid) {
+  if (ctx && solid >= 0 && solid < (int)ctx->solids.size() && ctx->solids[solid].listDirty) {
+    ctx->err = "solid masses changed: call esf_gather_solid_masses first";
+    return ESF_ERR_STATE;
+  }
+  return flipCall(ctx, solid, initParticles);
+}
+esf_status esf_grid_to_particle(esf_ctx* ctx, int solid) {
+  return flipCall(ctx, solid, gridToParticle);
+}
+esf_status esf_particle_to_grid(esf_ctx* ctx, int solid) {
+  return flipCall(ctx, solid, particleToGrid);
+}
+esf_status esf_extend_solid_velocity(esf_ctx* ctx, int solid) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  DevSolid& so = ctx->solids[solid];
+  const int box[6] = {so.box.bMin[0], so.box.bMin[1], so.box.bMin[2],
+                      so.box.bMax[0], so.box.bMax[1], so.box.bMax[2]};
+  esf_status st = extendDevice(ctx, box, so.mass, so.phi, so.vel);
+  if (st) return st;
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+esf_status esf_num_particles(esf_ctx* ctx, int solid, long* n) {
+  if (!ctx || !n || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  *n = ctx->solids[solid].nPart;
+  return ESF_OK;
+}
+esf_status esf_get_particles(esf_ctx* ctx, int solid, double* pos, double* vel) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  const DevSolid& so = ctx->solids[solid];
+  if (pos && so.nPart)
+    ESF_HIP(hipMemcpyAsync(pos, so.pPos, 24 * so.nPart, hipMemcpyDeviceToHost, ctx->stream));
+  if (vel && so.nPart)
+    ESF_HIP(hipMemcpyAsync(vel, so.pVel, 24 * so.nPart, hipMemcpyDeviceToHost, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  return ESF_OK;
+}
+esf_status esf_set_particles(esf_ctx* ctx, int solid, long n, const double* pos,
+                             const double* vel) {
+  if (!ctx || n < 0 || (n && (!pos || !vel)) || solid < 0 || solid >= (int)ctx->solids.size())
+    return ESF_ERR_INVALID;
+  esf_status st = growParticlesFor(ctx, solid, n);
+  if (st) return st;
+  DevSolid& so = ctx->solids[solid];
+  if (n) {
+    ESF_HIP(hipMemcpyAsync(so.pPos, pos, 24 * n, hipMemcpyHostToDevice, ctx->stream));
+    ESF_HIP(hipMemcpyAsync(so.pVel, vel, 24 * n, hipMemcpyHostToDevice, ctx->stream));
+  }
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  so.nPart = n;
+  return ESF_OK;
+}
+
 esf_status esf_solid_sdf(esf_ctx* ctx, int solid) {
   if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
   esf_status st = updateBoxes(ctx, solid);
@@ -789,6 +855,11 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
       ctx->err = "solid masses changed: call esf_gather_solid_masses first";
       return ESF_ERR_STATE;
     }
+  if (ctx->p.flip && ctx->solids.size() > 1) {
+    ctx->err = "the FLIP round trip takes scenes of one solid (contact between solids reads "
+               "rest-space SDFs this library does not keep)";
+    return ESF_ERR_INVALID;
+  }
   // alpha < 0: the broad phase of stepImplicit decides it
   // (COUPLED_INTEGRATOR_3D.cpp:384-411), as in esf_frame
   if (alpha < 0 && (st = esf_broad_phase(ctx, &alpha))) return st;
@@ -813,6 +884,22 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     return st;
   }
   if ((st = esf_solid_copy_velocity(ctx))) return st;
+  if (ctx->p.flip) {
+    // the FLIP round trip: G2P, P2G, then the solid velocity extended over
+    // its box by phi (COUPLED_INTEGRATOR_3D.cpp:833-854)
+    StageTimer t(ctx, 10);
+    for (int b = 0; b < (int)ctx->solids.size(); b++)
+      if ((st = gridToParticle(ctx, b))) return st;
+    for (int b = 0; b < (int)ctx->solids.size(); b++)
+      if ((st = particleToGrid(ctx, b))) return st;
+    for (int b = 0; b < (int)ctx->solids.size(); b++) {
+      const Box& bx = ctx->solids[b].box;
+      const int box[6] = {bx.bMin[0], bx.bMin[1], bx.bMin[2], bx.bMax[0], bx.bMax[1], bx.bMax[2]};
+      if ((st = extendDevice(ctx, box, ctx->solids[b].mass, ctx->solids[b].phi,
+                             ctx->solids[b].vel)))
+        return st;
+    }
+  }
   if ((st = esf_solid_finalize(ctx))) return st;
   {
     // computeSolidMass(true) of every solid, the surface meshes, then
@@ -827,6 +914,10 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     }
   }
   if ((st = esf_gather_solid_masses(ctx))) return st;
+  // initMaterialParticles on the new valid cells (:870-877)
+  if (ctx->p.flip)
+    for (int b = 0; b < (int)ctx->solids.size(); b++)
+      if ((st = initParticles(ctx, b))) return st;
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;

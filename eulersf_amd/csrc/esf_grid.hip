@@ -662,6 +662,12 @@ esf_status solidPhi(esf_ctx* ctx, int solid) {
   const double large = (double)(d.X + d.Y + d.Z + 2);
   hipLaunchKernelGGL(kPhiInit, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, bx,
                      (const double*)so.mass, large, so.phi, so.phi2);
+  if (ctx->p.flip) {
+    // the FLIP round trip extends the velocity over the whole box by phi:
+    // the reference's two rounds of ordered sweeps, bitwise
+    const int box[6] = {bx.bMin[0], bx.bMin[1], bx.bMin[2], bx.bMax[0], bx.bMax[1], bx.bMax[2]};
+    return sweepPhiRounds(ctx, box, so.mass, so.phi, 2);
+  }
   const long nb = (long)std::max(bx.bMax[0] - bx.bMin[0], 0) *
                   std::max(bx.bMax[1] - bx.bMin[1], 0) * std::max(bx.bMax[2] - bx.bMin[2], 0);
   if (nb > 0) {

@@ -91,6 +91,11 @@ struct DevSolid {
   int* meshTri = nullptr;
   double* meshBox = nullptr;
   std::vector<int> meshTriHost;
+  // FLIP material particles (DEFOR_SOLID_3D::_materialParticles): position
+  // and velocity, 3 doubles each; their bin (cell) and the bins' contents
+  long nPart = 0, capPart = 0;
+  double *pPos = nullptr, *pVel = nullptr;
+  int *pBin = nullptr, *binItems = nullptr;
 };
 
 }  // namespace esf
@@ -131,6 +136,18 @@ struct esf_ctx {
   // node (solid DOFs and fluid nodes), the next solid correction solve's
   // warm start
   double* dvPrev = nullptr;
+  // fast sweeping and particles: staging of the host-array entry points,
+  // the sweep passes' flags and coefficients, the particle bins (counts,
+  // starts, fill cursors: one set shared by the solids, used in turn), the
+  // weighted sums of particleToGrid and the MT19937 draws of
+  // initMaterialParticles (rand() values, the same every step)
+  double *swMass = nullptr, *swPhi = nullptr, *swField = nullptr;
+  int* swFirst = nullptr;
+  double2* swCoef = nullptr;
+  size_t swCoefCap = 0;
+  int *binCount = nullptr, *binStart = nullptr, *binCursor = nullptr, *binBlocks = nullptr;
+  double* randTable = nullptr;
+  long randCap = 0;
   float* minv32 = nullptr;
   // solve: cell-based (pressure)
   double *c = nullptr, *b = nullptr, *D = nullptr, *lambda = nullptr,
@@ -415,6 +432,28 @@ inline void launchTimed(esf_ctx* ctx, int cls, F kernel, dim3 grid, dim3 block, 
   t->cls = cls;
   hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t->a, t->b, 0, args...);
 }
+
+// esf_api.hip: a device buffer of `count` elements held by the context
+// (replacing the one at *p, which is freed); zero-filled
+esf_status devAllocBytes(esf_ctx* ctx, void** p, size_t bytes);
+template <typename T>
+inline esf_status devAlloc(esf_ctx* ctx, T** p, size_t count) {
+  return devAllocBytes(ctx, (void**)p, count * sizeof(T));
+}
+
+// esf_sweep.hip: rounds of sweepPhi over box (xmin, ymin, zmin, xmax,
+// ymax, zmax) on device fields; extendVectorField (four rounds of
+// sweepVectorField) of a device node field over box
+esf_status sweepPhiRounds(esf_ctx* ctx, const int box[6], const double* mass, double* phi,
+                          int rounds);
+esf_status extendDevice(esf_ctx* ctx, const int box[6], const double* mass, const double* phi,
+                        double* field);
+
+// esf_flip.hip: the FLIP round trip of one solid
+esf_status initParticles(esf_ctx* ctx, int solid);
+esf_status gridToParticle(esf_ctx* ctx, int solid);
+esf_status particleToGrid(esf_ctx* ctx, int solid);
+esf_status growParticlesFor(esf_ctx* ctx, int solid, long n);
 
 // esf_mesh.hip: surface meshes (advectSurfaceMesh for every solid with a
 // mesh; the mesh broad phase: root boxes, 6 doubles per solid, and one

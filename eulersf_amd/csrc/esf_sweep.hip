@@ -249,12 +249,9 @@ Sweep passRange(int i0, int i1, int j0, int j1, int k0, int k1) {
   return s;
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) hipFree(p);
-  }
-};
+// the context's sweep integers: three first-clear flags of a pass, or the
+// six extent counters of computeSDF
+constexpr int kSweepInts = 8;
 
 esf_status checkBox(esf_ctx* ctx, const int box[6]) {
   const Dims& d = ctx->d;
@@ -287,6 +284,77 @@ esf_status sweepPhiDevice(esf_ctx* ctx, const SweepBox& b, const double* mass, d
 }
 
 }  // namespace
+
+esf_status sweepPhiRounds(esf_ctx* ctx, const int box[6], const double* mass, double* phi,
+                          int rounds) {
+  const Dims& d = ctx->d;
+  if (!(box[3] > box[0] && box[4] > box[1] && box[5] > box[2])) return ESF_OK;
+  const SweepBox b{d.X, d.Y, d.Z, box[0], box[1], box[2], box[3], box[4], box[5]};
+  for (int r = 0; r < rounds; r++) {
+    esf_status st = sweepPhiDevice(ctx, b, mass, phi);
+    if (st) return st;
+  }
+  return ESF_OK;
+}
+
+// extendVectorField (:566-574) on device fields; the passes' flags and
+// coefficients in the context's scratch (grown with the box)
+esf_status extendDevice(esf_ctx* ctx, const int box[6], const double* m, const double* ph,
+                        double* f) {
+  esf_status st = checkBox(ctx, box);
+  if (st) return st;
+  const Dims& d = ctx->d;
+  const long boxNodes = (long)(box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]);
+  if (!ctx->swFirst && (st = devAlloc(ctx, &ctx->swFirst, kSweepInts))) return st;
+  if ((size_t)boxNodes > ctx->swCoefCap) {
+    if ((st = devAlloc(ctx, &ctx->swCoef, boxNodes))) return st;
+    ctx->swCoefCap = boxNodes;
+  }
+  const SweepBox b{d.X, d.Y, d.Z, box[0], box[1], box[2], box[3], box[4], box[5]};
+  int* first = ctx->swFirst;
+  const int x0 = b.x0, y0 = b.y0, z0 = b.z0, x1 = b.x1, y1 = b.y1, z1 = b.z1;
+  // sweepVectorField's eight passes (:524-545)
+  const Sweep passes[8] = {
+      passRange(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0 + 1, z1 - 1),
+      passRange(x0 + 1, x1 - 1, y1 - 2, y0, z0 + 1, z1 - 1),
+      passRange(x1 - 2, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1),
+      passRange(x1 - 2, x0, y1 - 2, y0, z0 + 1, z1 - 1),
+      passRange(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1 - 2, z0),
+      passRange(x0 + 1, x1 - 1, y1 - 2, y0, z1 - 2, z0),
+      passRange(x1 - 2, x0, y0 + 1, y1 - 1, z1 - 2, z0),
+      passRange(x1 - 2, x0, y1 - 2, y0, z1 - 2, z0)};
+  for (int round = 0; round < 4; round++) {
+    for (const Sweep& sw : passes) {
+      const long cells = (long)sw.ni * sw.nj * sw.nk;
+      if (cells == 0) continue;
+      hipLaunchKernelGGL(kFillFirst, dim3(1), dim3(64), 0, ctx->stream, first);
+      hipLaunchKernelGGL(kPassFlags, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
+                         ctx->stream, b, sw, m, ph, first);
+      hipLaunchKernelGGL(kPassCoef, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
+                         ctx->stream, b, sw, m, ph, (const int*)first, ctx->swCoef);
+      hipLaunchKernelGGL(kPassWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, sw,
+                         (const double2*)ctx->swCoef, f);
+    }
+    // the faces: z (:547-552), y (:554-558), x (:560-564)
+    const long lines[3] = {(long)(y1 - y0) * (z1 - z0), (long)(x1 - x0) * (z1 - z0),
+                           (long)(x1 - x0) * (y1 - y0)};
+    for (int axis : {2, 1, 0})
+      hipLaunchKernelGGL(kFaceCopy, dim3(cdiv(lines[axis], 256)), dim3(256), 0, ctx->stream, b,
+                         axis, m, f);
+  }
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+// the host-array entry points' device staging, allocated once per context
+esf_status sweepStaging(esf_ctx* ctx) {
+  const long n = ctx->d.nodes;
+  esf_status st = ESF_OK;
+  if (!ctx->swMass && (st = devAlloc(ctx, &ctx->swMass, n))) return st;
+  if (!ctx->swPhi && (st = devAlloc(ctx, &ctx->swPhi, n))) return st;
+  if (!ctx->swField && (st = devAlloc(ctx, &ctx->swField, 3 * n))) return st;
+  if (!ctx->swFirst && (st = devAlloc(ctx, &ctx->swFirst, kSweepInts))) return st;
+  return st;
+}
 
 }  // namespace esf
 
@@ -389,15 +457,10 @@ esf_status scratchOn(esf_ctx* ctx, int dev, double** out) {
 esf_status esf_compute_sdf(esf_ctx* ctx, const double* mass, double* phi, int box[6],
                            int tight[6]) {
   if (!ctx || !mass || !phi || !box || !tight) return ESF_ERR_INVALID;
-  const long n = ctx->d.nodes;
-  DevBuf dm, dphi, dext;
-  if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
-      hipMalloc(&dext.p, 6 * sizeof(int)) != hipSuccess) {
-    ctx->err = "compute_sdf: device allocation failed";
-    return ESF_ERR_OOM;
-  }
-  return computeSdfOn(ctx, mass, phi, box, tight, (double*)dm.p, (double*)dphi.p,
-                      (int*)dext.p);
+  // staging held by the context (allocated on the first call)
+  esf_status st = sweepStaging(ctx);
+  if (st) return st;
+  return computeSdfOn(ctx, mass, phi, box, tight, ctx->swMass, ctx->swPhi, ctx->swFirst);
 }
 
 esf_status esf_compute_sdf_noalloc(esf_ctx* ctx, const double* mass, double* phi, int box[6],
@@ -421,56 +484,14 @@ esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double*
   if (!ctx || !box || !mass || !phi || !field) return ESF_ERR_INVALID;
   esf_status st = checkBox(ctx, box);
   if (st) return st;
-  const Dims& d = ctx->d;
-  const long n = d.nodes;
-  DevBuf dm, dphi, dfield, dfirst, dcoef;
-  const long boxNodes = (long)(box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]);
-  if (hipMalloc(&dm.p, 8 * n) != hipSuccess || hipMalloc(&dphi.p, 8 * n) != hipSuccess ||
-      hipMalloc(&dfield.p, 24 * n) != hipSuccess ||
-      hipMalloc(&dfirst.p, 3 * sizeof(int)) != hipSuccess ||
-      hipMalloc(&dcoef.p, 16 * boxNodes) != hipSuccess) {
-    ctx->err = "extend_vector_field: device allocation failed";
-    return ESF_ERR_OOM;
-  }
-  ESF_HIP(hipMemcpyAsync(dm.p, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipMemcpyAsync(dphi.p, phi, 8 * n, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipMemcpyAsync(dfield.p, field, 24 * n, hipMemcpyHostToDevice, ctx->stream));
-  const SweepBox b{d.X, d.Y, d.Z, box[0], box[1], box[2], box[3], box[4], box[5]};
-  const double* m = (const double*)dm.p;
-  const double* ph = (const double*)dphi.p;
-  double* f = (double*)dfield.p;
-  int* first = (int*)dfirst.p;
-  const int x0 = b.x0, y0 = b.y0, z0 = b.z0, x1 = b.x1, y1 = b.y1, z1 = b.z1;
-  // sweepVectorField's eight passes (:524-545)
-  const Sweep passes[8] = {
-      passRange(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0 + 1, z1 - 1),
-      passRange(x0 + 1, x1 - 1, y1 - 2, y0, z0 + 1, z1 - 1),
-      passRange(x1 - 2, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1),
-      passRange(x1 - 2, x0, y1 - 2, y0, z0 + 1, z1 - 1),
-      passRange(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1 - 2, z0),
-      passRange(x0 + 1, x1 - 1, y1 - 2, y0, z1 - 2, z0),
-      passRange(x1 - 2, x0, y0 + 1, y1 - 1, z1 - 2, z0),
-      passRange(x1 - 2, x0, y1 - 2, y0, z1 - 2, z0)};
-  for (int round = 0; round < 4; round++) {
-    for (const Sweep& s : passes) {
-      const long cells = (long)s.ni * s.nj * s.nk;
-      if (cells == 0) continue;
-      hipLaunchKernelGGL(kFillFirst, dim3(1), dim3(64), 0, ctx->stream, first);
-      hipLaunchKernelGGL(kPassFlags, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
-                         ctx->stream, b, s, m, ph, first);
-      hipLaunchKernelGGL(kPassCoef, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
-                         ctx->stream, b, s, m, ph, (const int*)first, (double2*)dcoef.p);
-      hipLaunchKernelGGL(kPassWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, s,
-                         (const double2*)dcoef.p, f);
-    }
-    // the faces: z (:547-552), y (:554-558), x (:560-564)
-    const long lines[3] = {(long)(y1 - y0) * (z1 - z0), (long)(x1 - x0) * (z1 - z0),
-                           (long)(x1 - x0) * (y1 - y0)};
-    for (int axis : {2, 1, 0})
-      hipLaunchKernelGGL(kFaceCopy, dim3(cdiv(lines[axis], 256)), dim3(256), 0, ctx->stream, b,
-                         axis, m, f);
-  }
-  ESF_HIP(hipMemcpyAsync(field, f, 24 * n, hipMemcpyDeviceToHost, ctx->stream));
+  // staging held by the context (allocated on the first call)
+  if ((st = sweepStaging(ctx))) return st;
+  const long n = ctx->d.nodes;
+  ESF_HIP(hipMemcpyAsync(ctx->swMass, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(ctx->swPhi, phi, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipMemcpyAsync(ctx->swField, field, 24 * n, hipMemcpyHostToDevice, ctx->stream));
+  if ((st = extendDevice(ctx, box, ctx->swMass, ctx->swPhi, ctx->swField))) return st;
+  ESF_HIP(hipMemcpyAsync(field, ctx->swField, 24 * n, hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }

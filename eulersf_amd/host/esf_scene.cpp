@@ -103,6 +103,13 @@ void paramsFromCfg(const Cfg& c, double dh, esf_params* p) {
   p->tight_border = c.getInt("tight border", 3);              // :243
   p->extrapolation_cells = c.getInt("extrapolation cells", 20);  // :244
   p->inner_eps = c.getFloat("mixed inner eps", 1e-4);
+  p->particles_per_cell = c.getInt("number of particles per material cell", 8);  // :336
+  p->flip_alpha = c.getFloat("flip alpha", 0.95);                                 // :901
+  p->flip_cutoff = c.getInt("flip cutoff", 3);                                    // :1101
+  p->wall_thickness = c.getFloat("wall thickness", dh * 1.5);                     // :949
+  p->spring_stiffness = c.getFloat("contact spring stiffness", 10000);            // :950
+  p->friction = c.getFloat("friction", 0.5);                                      // :952
+  p->flip = c.getInt("flip particles", 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -410,6 +417,19 @@ extern "C" esf_status esf_scene_load_cfg(const char* path, int precision, esf_ct
   if ((st = esf_gather_solid_masses(ctx)) || (st = esf_recover_velocity(ctx))) {
     esf_ctx_destroy(ctx);
     return st;
+  }
+  // initMaterialParticles of DEFOR_SOLID_3D::init (:327) when the step runs
+  // the FLIP round trip
+  if (p.flip) {
+    if (nSolids > 1) {
+      esf_ctx_destroy(ctx);
+      return ESF_ERR_INVALID;
+    }
+    for (int i = 0; i < nSolids; i++)
+      if ((st = esf_init_particles(ctx, i))) {
+        esf_ctx_destroy(ctx);
+        return st;
+      }
   }
   // smoke carpet  main.cpp:80-90
   const double carpet = cfg.getFloat("smoke carpet thickness", 0.06);

@@ -303,6 +303,36 @@ class CoupledIntegrator3D:
                                                          dptr(out)), "extendVectorField")
         return out
 
+    # -- FLIP particles (DEFOR_SOLID_3D) --------------------------------------
+    def initMaterialParticles(self, solid=0):
+        check(self.ctx, self.lib.esf_init_particles(self.ctx, solid), "initMaterialParticles")
+
+    def gridToParticle(self, solid=0):
+        check(self.ctx, self.lib.esf_grid_to_particle(self.ctx, solid), "gridToParticle")
+
+    def particleToGrid(self, solid=0):
+        check(self.ctx, self.lib.esf_particle_to_grid(self.ctx, solid), "particleToGrid")
+
+    def extendSolidVelocity(self, solid=0):
+        check(self.ctx, self.lib.esf_extend_solid_velocity(self.ctx, solid), "extendVectorField")
+
+    def particles(self, solid=0):
+        """positions and velocities of the solid's particles, (n, 3) each"""
+        n = ctypes.c_long()
+        check(self.ctx, self.lib.esf_num_particles(self.ctx, solid, ctypes.byref(n)))
+        pos = np.empty(3 * n.value)
+        vel = np.empty(3 * n.value)
+        check(self.ctx, self.lib.esf_get_particles(self.ctx, solid, dptr(pos), dptr(vel)))
+        return pos.reshape(-1, 3), vel.reshape(-1, 3)
+
+    def setParticles(self, pos, vel, solid=0):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        if pos.size != vel.size or pos.size % 3:
+            raise ValueError("positions and velocities: 3 doubles per particle each")
+        check(self.ctx, self.lib.esf_set_particles(self.ctx, solid, pos.size // 3, dptr(pos),
+                                                   dptr(vel)), "setParticles")
+
     def writeFrame(self, data_path, frame, debug=False):
         """COUPLED_INTEGRATOR_3D::writeFrame (COUPLED_INTEGRATOR_3D.cpp:999-1011)"""
         check(self.ctx, self.lib.esf_write_frame(self.ctx, data_path.encode(), frame,

@@ -81,6 +81,15 @@ typedef struct {
   int extrapolation_cells;   /* "extrapolation cells" */
   int precision;             /* ESF_PLAIN_FP64 or ESF_MIXED */
   double inner_eps;          /* mixed: FP32 inner solve relative tolerance */
+  /* FLIP material particles (DEFOR_SOLID_3D.cpp:336-337, 901, 948-952,
+   * 1101); esf_step runs the particle round trip only when flip != 0 */
+  int particles_per_cell;    /* "number of particles per material cell" (8) */
+  double flip_alpha;         /* "flip alpha" (0.95) */
+  int flip_cutoff;           /* "flip cutoff" (3) */
+  double wall_thickness;     /* "wall thickness" (1.5 dh) */
+  double spring_stiffness;   /* "contact spring stiffness" (1e4) */
+  double friction;           /* "friction" (0.5) */
+  int flip;                  /* "flip particles" (0; not a reference key) */
 } esf_params;
 
 /* per-step solve report */
@@ -263,6 +272,32 @@ esf_status esf_compute_sdf_noalloc(esf_ctx* ctx, const double* mass, double* phi
  * sweeps, run on the GPU plane by plane. */
 esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double* mass,
                                    const double* phi, double* field);
+/* --- FLIP material particles (one solid per scene) ---------------------*/
+/* DEFOR_SOLID_3D::initMaterialParticles (DEFOR_SOLID_3D.cpp:333-367) on the
+ * solid's valid cells (after esf_gather_solid_masses): "particles per cell"
+ * particles per cell at offsets drawn from MT19937 seeded with 123456 (the
+ * reference reseeds every call; the three draws of an offset taken x, y,
+ * z), velocity interpolated from the solid's velocity field. */
+esf_status esf_init_particles(esf_ctx* ctx, int solid);
+/* DEFOR_SOLID_3D::gridToParticle (:891-1034): the FLIP/PIC blend over the
+ * 4^3 cubic B-spline stencil of the solid's nodes with mass, the walls'
+ * spring/friction impulse, the move by dt.  Contact between solids
+ * (:969-995) reads other solids' rest-space SDFs and is not done here. */
+esf_status esf_grid_to_particle(esf_ctx* ctx, int solid);
+/* DEFOR_SOLID_3D::particleToGrid (:1036-1107): per node the weighted
+ * velocity of the particles whose stencils cover it; where the weight
+ * reaches "flip cutoff" the solid's velocity becomes that average and its
+ * _mass 1, elsewhere _mass is 0.  Sums in a fixed per-node order (binned
+ * particles), not the reference's particle order: equal to rounding. */
+esf_status esf_particle_to_grid(esf_ctx* ctx, int solid);
+/* DEFOR_SOLID_3D::extendVectorField(_velocity) (:566-574) on the solid's own
+ * velocity, _mass and _phi, over its extended box, on the device */
+esf_status esf_extend_solid_velocity(esf_ctx* ctx, int solid);
+esf_status esf_num_particles(esf_ctx* ctx, int solid, long* n);
+/* positions and velocities, 3 doubles per particle; either may be NULL */
+esf_status esf_get_particles(esf_ctx* ctx, int solid, double* pos, double* vel);
+esf_status esf_set_particles(esf_ctx* ctx, int solid, long n, const double* pos,
+                             const double* vel);
 /* recoverVelocityField + Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:894-916) */
 esf_status esf_recover_and_bc(esf_ctx* ctx);
 /* One substep of the drop-in: CFL(alpha) -> forces -> coupled solve ->
@@ -271,9 +306,11 @@ esf_status esf_recover_and_bc(esf_ctx* ctx);
  * phi (esf_solid_sdf) -> gather solid masses -> advect velocity -> advect
  * intensity -> recover + Dirichlet (COUPLED_INTEGRATOR_3D.cpp:377-917).
  * alpha < 0 takes the CFL alpha from the broad phase (esf_broad_phase), as
- * stepImplicit does.  The FLIP particle round trip and the velocity
- * extension of COUPLED_INTEGRATOR_3D::finalizeSolution (:833-854) are not
- * run: the solid velocity is the grid velocity copied into its box.  A step
+ * stepImplicit does.  With params.flip set (scenes of one solid) the step
+ * is the reference's whole substep: after the velocity copy, gridToParticle,
+ * particleToGrid and the velocity extension (:833-854), phi from the
+ * ordered sweeps, and new particles after the mass gather; without it the
+ * solid velocity is the grid velocity copied into its box.  A step
  * that runs alone in the process computes the fluid's vorticity and body
  * forces on a second stream of the context, beside the first solid solve;
  * results are the same either way. */

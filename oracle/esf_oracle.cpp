@@ -22,6 +22,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <random>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -412,6 +413,9 @@ struct Solid {
   vector<double> X, u, vel, velws;  // 3*nodes
   vector<double> mass, detFInv;     // nodes
   vector<double> phi;               // nodes (_phi, computeSDF)
+  // FLIP material particles (PARTICLE, DEFOR_SOLID_3D.h): position,
+  // velocity (3 each) and mass
+  vector<double> ppos, pvel, pmass;
   vector<int8_t> valid;             // cells
   vector<double> qfrac;             // 8*nodes
   int tMin[3], tMax[3], bMin[3], bMax[3];
@@ -622,6 +626,182 @@ void computeSDF(Scene& s, Solid& so) {
   for (long i = 0; i < s.total; i++) so.phi[i] = so.mass[i] > 0 ? -0.5 : large;
   const int box[6] = {so.bMin[0], so.bMin[1], so.bMin[2], so.bMax[0], so.bMax[1], so.bMax[2]};
   for (int r = 0; r < 2; r++) orc_sweep_phi(s.X, s.Y, s.Z, box, so.mass.data(), so.phi.data());
+}
+
+// ---------------------------------------------------------------------------
+// FLIP material particles  DEFOR_SOLID_3D.cpp:333-367, 891-1107
+// ---------------------------------------------------------------------------
+// DEFOR_SOLID_3D::initMaterialParticles  :333-367.  A fresh MERSENNETWISTER
+// seeded with 123456 every call (MERSENNETWISTER.h:259-263, standard
+// MT19937 as std::mt19937); rand(n) = randInt() * (1/4294967295) * n
+// (:146-150); the offset triple VEC3F(rand(dh), rand(dh), rand(dh)) takes
+// its three draws left to right (clang's argument order, the reference's
+// toolchain).  The particle density and volume are function-local statics
+// of the first solid that ran it (:338-340): `density` is passed in.
+void initParticles(Scene& s, Solid& so, double density) {
+  const int ppc = s.p.particles_per_cell;
+  const double volume = s.dh * s.dh * s.dh / ppc;
+  std::mt19937 twister(123456);
+  auto rnd = [&](double n) { return double(twister()) * (1.0 / 4294967295.0) * n; };
+  so.ppos.clear();
+  so.pvel.clear();
+  so.pmass.clear();
+  const Geom g = s.geom();
+  const long cx = s.X - 1, cxy = (long)(s.X - 1) * (s.Y - 1);
+  for (int z = so.tMin[2]; z < so.tMax[2] - 1; z++)
+    for (int y = so.tMin[1]; y < so.tMax[1] - 1; y++)
+      for (int x = so.tMin[0]; x < so.tMax[0] - 1; x++) {
+        if (so.valid[x + y * cx + z * cxy] != 1) continue;
+        const double grid[3] = {s.o[0] + x * s.dh, s.o[1] + y * s.dh, s.o[2] + z * s.dh};
+        for (int i = 0; i < ppc; i++) {
+          const double ox = rnd(s.dh);
+          const double oy = rnd(s.dh);
+          const double oz = rnd(s.dh);
+          const double pos[3] = {grid[0] + ox, grid[1] + oy, grid[2] + oz};
+          double v[3];
+          interpV(so.vel.data(), g, pos, v);
+          for (int c = 0; c < 3; c++) {
+            so.ppos.push_back(pos[c]);
+            so.pvel.push_back(v[c]);
+          }
+          so.pmass.push_back(density * volume);
+        }
+      }
+}
+
+// the cubic B-spline weight of one axis (:917-925, :1068-1076)
+inline double bspline(double d) {
+  if (std::fabs(d) < 1) return 0.5 * std::fabs(std::pow(d, 3)) - d * d + 2.0 / 3;
+  if (std::fabs(d) < 2) return -1.0 / 6 * std::fabs(std::pow(d, 3)) + d * d - 2 * std::fabs(d) + 4.0 / 3;
+  return 0;
+}
+
+// DEFOR_SOLID_3D::gridToParticle  :891-1034, without the contact with other
+// solids (:969-995, which reads their rest-space SDF; the caller runs it
+// for scenes of one solid), then the walls (:996-1031) and the move
+void gridToParticle(Scene& s, Solid& so) {
+  const double alpha = s.p.flip_alpha;
+  const double bnd[6] = {s.o[0], s.o[1], s.o[2], s.o[0] + s.dh * (s.X - 1),
+                         s.o[1] + s.dh * (s.Y - 1), s.o[2] + s.dh * (s.Z - 1)};
+  static const double wallN[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                     {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+  const long np = (long)so.pmass.size();
+  auto V = [&](const vector<double>& f, long n, int c) { return f[3 * n + c]; };
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < np; c++) {
+    double* pp = &so.ppos[3 * c];
+    double* pv = &so.pvel[3 * c];
+    const int i = (int)std::floor((pp[0] - s.o[0]) / s.dh) - 1;
+    const int j = (int)std::floor((pp[1] - s.o[1]) / s.dh) - 1;
+    const int k = (int)std::floor((pp[2] - s.o[2]) / s.dh) - 1;
+    double pic[3] = {0, 0, 0}, flip[3] = {0, 0, 0};
+    for (int r = k; r < k + 4; r++)
+      for (int t = j; t < j + 4; t++)
+        for (int q = i; q < i + 4; q++) {
+          if (q < 0 || t < 0 || r < 0 || q > s.X - 1 || t > s.Y - 1 || r > s.Z - 1) continue;
+          const long n = r * s.slab + (long)t * s.X + q;
+          if (so.mass[n] == 0) continue;
+          const double diff[3] = {s.dhInv * (pp[0] - s.o[0]) - q, s.dhInv * (pp[1] - s.o[1]) - t,
+                                  s.dhInv * (pp[2] - s.o[2]) - r};
+          const double w = (bspline(diff[0]) * bspline(diff[1])) * bspline(diff[2]);
+          for (int d = 0; d < 3; d++) {
+            pic[d] += V(so.vel, n, d) * w;
+            flip[d] += (V(so.vel, n, d) - V(so.velws, n, d)) * w;
+          }
+        }
+    for (int d = 0; d < 3; d++) {
+      flip[d] += pv[d];
+      pv[d] = (1 - alpha) * pic[d] + alpha * flip[d];
+    }
+  }
+  const double thick = s.p.wall_thickness, ks = s.p.spring_stiffness, fr = s.p.friction;
+  const double dt = so.dt;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < np; c++) {
+    double* pp = &so.ppos[3 * c];
+    double* pv = &so.pvel[3 * c];
+    int i = (int)std::floor((pp[0] - s.o[0]) / s.dh);
+    int j = (int)std::floor((pp[1] - s.o[1]) / s.dh);
+    int k = (int)std::floor((pp[2] - s.o[2]) / s.dh);
+    i = std::min(std::max(i, 0), s.X - 2);
+    j = std::min(std::max(j, 0), s.Y - 2);
+    k = std::min(std::max(k, 0), s.Z - 2);
+    for (int b = 0; b < 6; b++) {
+      const double dist = std::fabs(pp[b % 3] - bnd[b]);
+      if (!(dist < thick)) continue;
+      const double overlap = thick - dist;
+      const int xi = b == 0 ? 0 : (b == 3 ? s.X - 1 : i);
+      const int yi = b == 1 ? 0 : (b == 4 ? s.Y - 1 : j);
+      const int zi = b == 2 ? 0 : (b == 5 ? s.Z - 1 : k);
+      const long n = zi * s.slab + (long)yi * s.X + xi;
+      const double* nb = wallN[b];
+      double vrel[3];
+      for (int d = 0; d < 3; d++) vrel[d] = pv[d] - so.vel[3 * n + d];
+      const double vn = sum3(vrel[0] * nb[0], vrel[1] * nb[1], vrel[2] * nb[2]);
+      if (vn <= 0) {
+        const double impulse = -std::min(dt * ks * overlap, so.pmass[c] * (0.1 * overlap / dt - vn));
+        const double dvn = -impulse / so.pmass[c];
+        double vt[3];
+        for (int d = 0; d < 3; d++) vt[d] = vrel[d] - vn * nb[d];
+        const double vtn = std::sqrt(sum3(vt[0] * vt[0], vt[1] * vt[1], vt[2] * vt[2]));
+        const double sc = std::max(0.1 - fr * dvn / vtn, 0.0);
+        for (int d = 0; d < 3; d++) pv[d] = ((vn + dvn) * nb[d] + sc * vt[d]) + so.vel[3 * n + d];
+        break;
+      }
+    }
+    for (int d = 0; d < 3; d++) pp[d] += pv[d] * dt;
+  }
+}
+
+// DEFOR_SOLID_3D::particleToGrid  :1036-1107: per node the weighted velocity
+// and weight sums over the particles whose 4^3 stencil covers it, in
+// particle order (the single-threaded build); _mass becomes 1 where the
+// weight reaches the cutoff, where the velocity becomes the average, and 0
+// elsewhere
+void particleToGrid(Scene& s, Solid& so) {
+  const long np = (long)so.pmass.size();
+  vector<double> vs(3 * s.total, 0.0), ws(s.total, 0.0);
+  vector<char> touched(s.total, 0);
+  for (long c = 0; c < np; c++) {
+    const double* pp = &so.ppos[3 * c];
+    const double* pv = &so.pvel[3 * c];
+    const int i = (int)std::floor((pp[0] - s.o[0]) / s.dh) - 1;
+    const int j = (int)std::floor((pp[1] - s.o[1]) / s.dh) - 1;
+    const int k = (int)std::floor((pp[2] - s.o[2]) / s.dh) - 1;
+    for (int r = k; r < k + 4; r++)
+      for (int t = j; t < j + 4; t++)
+        for (int q = i; q < i + 4; q++) {
+          if (q < 0 || t < 0 || r < 0 || q > s.X - 1 || t > s.Y - 1 || r > s.Z - 1) continue;
+          const double diff[3] = {s.dhInv * (pp[0] - s.o[0]) - q, s.dhInv * (pp[1] - s.o[1]) - t,
+                                  s.dhInv * (pp[2] - s.o[2]) - r};
+          const double w = (bspline(diff[0]) * bspline(diff[1])) * bspline(diff[2]);
+          const long n = r * s.slab + (long)t * s.X + q;
+          if (!touched[n]) {
+            touched[n] = 1;
+            for (int d = 0; d < 3; d++) vs[3 * n + d] = w * pv[d];
+            ws[n] = w;
+          } else {
+            for (int d = 0; d < 3; d++) vs[3 * n + d] += w * pv[d];
+            ws[n] += w;
+          }
+        }
+  }
+  std::fill(so.mass.begin(), so.mass.end(), 0.0);
+  const double cutoff = s.p.flip_cutoff;
+  for (long n = 0; n < s.total; n++) {
+    if (!touched[n] || ws[n] < cutoff) continue;
+    for (int d = 0; d < 3; d++) so.vel[3 * n + d] = vs[3 * n + d] / ws[n];
+    so.mass[n] = 1;
+  }
+}
+
+extern "C" void orc_extend_vector_field(int X, int Y, int Z, const int box[6], const double* mass,
+                                        const double* phi, double* field);
+
+// DEFOR_SOLID_3D::extendVectorField(_velocity)  :566-574 over the extended box
+void extendSolidVelocity(Scene& s, Solid& so) {
+  const int box[6] = {so.bMin[0], so.bMin[1], so.bMin[2], so.bMax[0], so.bMax[1], so.bMax[2]};
+  orc_extend_vector_field(s.X, s.Y, s.Z, box, so.mass.data(), so.phi.data(), so.vel.data());
 }
 
 // COUPLED_INTEGRATOR_3D::gatherSolidMasses  COUPLED_INTEGRATOR_3D.cpp:327-375
@@ -1542,6 +1722,7 @@ void orc_init(void* h, double carpet_thickness) {
     so.bMax[2] = so.tMax[2] = s.Z;
     computeSolidMass(s, so, false);
     computeSDF(s, so);
+    if (s.p.flip) initParticles(s, so, s.solids[0].qcm / (s.dh * s.dh * s.dh * 0.125));
   }
   gather(s);
   for (Solid& so : s.solids) recoverVelocity(s, so);
@@ -1665,6 +1846,38 @@ void orc_compute_solid_mass(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);
   computeSolidMass(s, s.solids[solid], false);
 }
+void orc_init_particles(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  initParticles(s, s.solids[solid], s.solids[0].qcm / (s.dh * s.dh * s.dh * 0.125));
+}
+void orc_grid_to_particle(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  gridToParticle(s, s.solids[solid]);
+}
+void orc_particle_to_grid(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  particleToGrid(s, s.solids[solid]);
+}
+long orc_num_particles(void* h, int solid) {
+  return (long)static_cast<Scene*>(h)->solids[solid].pmass.size();
+}
+void orc_get_particles(void* h, int solid, double* pos, double* vel) {
+  Solid& so = static_cast<Scene*>(h)->solids[solid];
+  if (pos) std::copy(so.ppos.begin(), so.ppos.end(), pos);
+  if (vel) std::copy(so.pvel.begin(), so.pvel.end(), vel);
+}
+void orc_set_particles(void* h, int solid, long n, const double* pos, const double* vel) {
+  Scene& s = *static_cast<Scene*>(h);
+  Solid& so = s.solids[solid];
+  so.ppos.assign(pos, pos + 3 * n);
+  so.pvel.assign(vel, vel + 3 * n);
+  const double density = s.solids[0].qcm / (s.dh * s.dh * s.dh * 0.125);
+  so.pmass.assign(n, density * (s.dh * s.dh * s.dh / s.p.particles_per_cell));
+}
+void orc_extend_solid_velocity(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  extendSolidVelocity(s, s.solids[solid]);
+}
 double orc_compute_F(void* h, int solid, long index, double* F) {
   Scene& s = *static_cast<Scene*>(h);
   return computeF(s, s.solids[solid], index, F);
@@ -1695,12 +1908,22 @@ int orc_step(void* h, double alpha, orc_stats* st) {
   orc_assemble(h);
   int rc = solve(s, st);
   solidCopyVelocity(s);
+  if (s.p.flip) {
+    // finalizeSolution :833-854: G2P, P2G and the velocity extension
+    for (Solid& so : s.solids) gridToParticle(s, so);
+    for (Solid& so : s.solids) particleToGrid(s, so);
+    for (Solid& so : s.solids) extendSolidVelocity(s, so);
+  }
   solidFinalize(s);
   double t1 = nowSec();
   // finalizeSolution :857-879: computeSolidMass(true) of every solid, then
-  // computeSDF of every solid (surface meshes are not kept here)
+  // computeSDF (and the new particles) of every solid (surface meshes are
+  // not kept here)
   for (Solid& so : s.solids) computeSolidMass(s, so, true);
-  for (Solid& so : s.solids) computeSDF(s, so);
+  for (Solid& so : s.solids) {
+    computeSDF(s, so);
+    if (s.p.flip) initParticles(s, so, s.solids[0].qcm / (s.dh * s.dh * s.dh * 0.125));
+  }
   gather(s);
   s.stageSec[10] += nowSec() - t1;
   advectVelocity(s);

@@ -35,6 +35,15 @@ typedef struct {
   double mass_sample_dh;  /* "mass sample dh"                               */
   int tight_border;       /* "tight border"                                 */
   int extrapolation_cells;/* "extrapolation cells"                          */
+  /* the FLIP particle round trip of finalizeSolution (DEFOR_SOLID_3D.cpp:
+   * 333-367, 891-1107), run by orc_step when flip != 0 */
+  int particles_per_cell; /* "number of particles per material cell" (8)    */
+  double flip_alpha;      /* "flip alpha" (0.95)                            */
+  int flip_cutoff;        /* "flip cutoff" (3)                              */
+  double wall_thickness;  /* "wall thickness" (1.5 dh)                      */
+  double spring_stiffness;/* "contact spring stiffness" (1e4)               */
+  double friction;        /* "friction" (0.5)                               */
+  int flip;
 } orc_params;
 
 /* field ids for orc_field() */
@@ -113,6 +122,17 @@ void orc_solid_finalize(void* h);                /* DEFOR :1198-1233 */
 void orc_compute_solid_mass(void* h, int solid); /* DEFOR :576-669 (usesdf=false) */
 void orc_update_boxes(void* h, int solid);       /* DEFOR :216-266 (box part) */
 void orc_compute_solid_mass_sdf(void* h, int solid); /* DEFOR :576-669 (usesdf=true) */
+/* FLIP particles of a solid (DEFOR_SOLID_3D.cpp:333-367, 891-1107):
+ * initMaterialParticles, gridToParticle, particleToGrid; the count, and
+ * the particles' positions and velocities (3 doubles each) */
+void orc_init_particles(void* h, int solid);
+void orc_grid_to_particle(void* h, int solid);
+void orc_particle_to_grid(void* h, int solid);
+long orc_num_particles(void* h, int solid);
+void orc_get_particles(void* h, int solid, double* pos, double* vel);
+void orc_set_particles(void* h, int solid, long n, const double* pos, const double* vel);
+/* DEFOR :566-574 on the solid's own velocity, mass and phi */
+void orc_extend_solid_velocity(void* h, int solid);
 /* DEFOR :671-682: F of the cell with min-corner node `index` (row-major), returns det F */
 double orc_compute_F(void* h, int solid, long index, double* F);
 void orc_solid_sdf(void* h, int solid);          /* DEFOR :216-267 (boxes + phi) */

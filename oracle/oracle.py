@@ -32,7 +32,10 @@ class Params(ctypes.Structure):
                 ("fluid_density", ctypes.c_double), ("implicit_eps", ctypes.c_double),
                 ("divfree_eps", ctypes.c_double), ("cr_max_iterations", ctypes.c_int),
                 ("mass_sample_dh", ctypes.c_double), ("tight_border", ctypes.c_int),
-                ("extrapolation_cells", ctypes.c_int)]
+                ("extrapolation_cells", ctypes.c_int), ("particles_per_cell", ctypes.c_int),
+                ("flip_alpha", ctypes.c_double), ("flip_cutoff", ctypes.c_int),
+                ("wall_thickness", ctypes.c_double), ("spring_stiffness", ctypes.c_double),
+                ("friction", ctypes.c_double), ("flip", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -76,6 +79,13 @@ def lib():
         L.orc_update_boxes.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_compute_solid_mass_sdf.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_solid_sdf.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        for nm in ("orc_init_particles", "orc_grid_to_particle", "orc_particle_to_grid",
+                   "orc_extend_solid_velocity"):
+            getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_num_particles.restype = ctypes.c_long
+        L.orc_num_particles.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_get_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D]
+        L.orc_set_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, _D, _D]
         L.orc_compute_F.restype = ctypes.c_double
         L.orc_compute_F.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, _D]
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
@@ -199,7 +209,7 @@ class OracleScene:
                 "zres": c.get_int("zRes", 100), "dh": c.get_float("dh", 0.01),
                 "origin": c.get_vec3("origin", (0, 0, 0))}
         params = params_from_cfg(c, grid["dh"])
-        s = cls(grid, params)
+        s = cls(grid, {k: v for k, v in params.items() if k in dict(Params._fields_)})
         for i in range(c.get_int("number of solids", 1)):
             k = str(i)
             density = c.get_float("solid mass density " + k, 1)
@@ -254,6 +264,20 @@ class OracleScene:
         ct = {np.int32: ctypes.c_int32, np.int8: ctypes.c_int8}.get(dt, ctypes.c_double)
         arr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(n.value,))
         arr[:] = np.asarray(values, dtype=dt).reshape(-1)
+
+    def particles(self, solid=0):
+        """the solid's FLIP particles: positions and velocities, (n, 3) each"""
+        n = self.L.orc_num_particles(self.h, solid)
+        pos = np.zeros(3 * n)
+        vel = np.zeros(3 * n)
+        if n:
+            self.L.orc_get_particles(self.h, solid, dp(pos), dp(vel))
+        return pos.reshape(-1, 3), vel.reshape(-1, 3)
+
+    def set_particles(self, pos, vel, solid=0):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        self.L.orc_set_particles(self.h, solid, pos.size // 3, dp(pos), dp(vel))
 
     def boxes(self, solid=0):
         b = (ctypes.c_int * 12)()

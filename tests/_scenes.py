@@ -30,7 +30,7 @@ def solution_order(v_nodes, X, Y, Z):
     return v.reshape(-1)
 
 
-def teacher_force(g, o, n_solids=1):
+def teacher_force(g, o, n_solids=1, particles=False):
     """Copy the oracle scene's state into the GPU context: grid velocity,
     intensity and the pressure warm start; per solid its boxes, u, X,
     velocity, velocity workspace, mass, quarter fractions, valid flags and
@@ -47,6 +47,9 @@ def teacher_force(g, o, n_solids=1):
                        (A.S_VEL_WORKSPACE, O.S_VEL_WORKSPACE), (A.S_MASS, O.S_MASS),
                        (A.S_QFRAC, O.S_QFRAC), (A.S_VALID, O.S_VALID), (A.S_PHI, O.S_PHI)):
             g.upload(gf, o.field(of, b), b)
+        if particles:
+            pos, vel = o.particles(b)
+            g.setParticles(pos, vel, b)
     g.gatherSolidMasses()
 
 

@@ -23,7 +23,8 @@ def params(n, **kw):
     p = dict(dt=1.0 / 240.0, vorticity_eps=0.0, gravity=0.0, buoyancy=0.0, rayleigh_alpha=0.0,
              rayleigh_beta=0.0, fluid_density=1000.0, implicit_eps=1e-12, divfree_eps=1e-12,
              cr_max_iterations=20000, mass_sample_dh=dh / 4, tight_border=3,
-             extrapolation_cells=20)
+             extrapolation_cells=20, particles_per_cell=8, flip_alpha=0.95, flip_cutoff=3,
+             wall_thickness=1.5 * dh, spring_stiffness=1e4, friction=0.5, flip=0)
     p.update(kw)
     return p
 
