@@ -316,7 +316,36 @@ def isolated_system(esf, n, seed, eps, reps):
     Sx = g.getMatVecMult(sols["fp64"])
     out["mixed_vs_fp64"] = {"rel_diff_S_x": float(np.linalg.norm(Sd) / np.linalg.norm(Sx))}
     g.close()
+    out["solid"] = isolated_solid_system(esf, text, len(spheres), rng, eps, reps)
     print(json.dumps(out), flush=True)
+
+
+def isolated_solid_system(esf, text, k, rng, eps, reps):
+    """the solid half of the isolated system on the same scene: each valid
+    cell's deformation gradient a random rotation times diag(N(1, 0.05)),
+    the elastic stiffness assembled from those, A = dt^2 K + M~ (beta = 0)
+    solved for A x = b, b ~ N(0, 1) per solid dof, from x = 0 to `eps` in
+    FP64 by Jacobi-preconditioned CR"""
+    from eulersf_amd.cfg import random_deformations
+    g = esf.CoupledIntegrator3D.from_cfg_text(text, esf.ESF_PLAIN_FP64)
+    cells = 0
+    for b in range(k):
+        nc = g.solidNumCells(b)
+        g.setDeformation(random_deformations(rng, nc), b)
+        cells += nc
+    g.setDt(1.0 / 240)
+    g.forces()
+    ns = g.num_solid_nodes()
+    rhs = rng.standard_normal(3 * ns)
+    times = []
+    for r in range(reps):
+        g.upload(esf._abi.F_SOLID_RHS, rhs)
+        it, rr, ms, st = g.solidSolveResident(eps, 100000)
+        times.append(ms)
+    g.close()
+    return {"solid_nodes": ns, "valid_cells": cells, "iterations": it, "rel_residual": rr,
+            "status": st, "time_to_solution_ms": round(float(np.median(times)), 3),
+            "time_to_solution_ms_all": [round(t, 3) for t in times]}
 
 
 def batched(esf, count, steps, workers, seed):
@@ -371,6 +400,62 @@ def served(esf, seconds, rate, workers, seed, one_at_a_time=True):
         out["throughput_ratio"] = round(out["concurrent"]["steps_per_s"] /
                                         out["one_at_a_time"]["steps_per_s"], 3)
     return out
+
+
+def drift(esf, n, frames, velocity, flip, eps, every):
+    """The free-running drift of the GPU step from the oracle's over
+    `frames` frames of the n^3 ball (APPLICATION::step, frame dt = the cfg
+    dt, CFL alpha 1.8), both with the full finalizeSolution (FLIP particles,
+    velocity extension) when `flip`: per frame the relative difference of
+    the velocity (against max(|v|, g dt)), the displacement, the intensity
+    and the solid mass"""
+    from oracle import oracle as O
+    extra = {"flip particles": 1, "number of particles per material cell": 8} if flip else None
+    text = esf.unit_cube_ball_cfg(n, eps=eps, max_iterations=20000, velocity=velocity, extra=extra)
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    A = esf._abi
+    frame_dt = 1.0 / 240.0
+    rows = []
+    t_gpu = t_cpu = 0.0
+    sub_g = sub_o = 0
+    for f in range(frames):
+        t0 = time.perf_counter()
+        sub_g += g.frame(frame_dt, 1.8)
+        t_gpu += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim = 0.0
+        while sim < frame_dt:
+            o.L.orc_set_max_dt(o.h, frame_dt - sim)
+            rc, st = o.step(1.8)
+            sim += st.dt
+            sub_o += 1
+        t_cpu += time.perf_counter() - t0
+        if f % every == every - 1 or f == frames - 1:
+            vg, vo = g.download(A.F_VELOCITY), o.field(O.VELOCITY)
+            floor = 9.8 * frame_dt * np.sqrt(vo.size)
+            row = {"frame": f + 1,
+                   "velocity": float(np.linalg.norm(vg - vo) / max(np.linalg.norm(vo), floor)),
+                   "displacement": float(np.linalg.norm(g.download(A.S_U, 0) - o.field(O.S_U, 0)) /
+                                         np.linalg.norm(o.field(O.S_U, 0))),
+                   "intensity": float(np.linalg.norm(g.download(A.F_INTENSITY) -
+                                                     o.field(O.INTENSITY)) /
+                                      np.linalg.norm(o.field(O.INTENSITY))),
+                   "solid_mass": float(np.linalg.norm(g.download(A.S_MASS, 0) -
+                                                      o.field(O.S_MASS, 0)) /
+                                       np.linalg.norm(o.field(O.S_MASS, 0)))}
+            rows.append(row)
+            log(f"[bench] drift frame {f + 1}: " + ", ".join(f"{k} {v:.2e}" for k, v in row.items()
+                                                            if k != "frame"))
+    g.close()
+    o.close()
+    print(json.dumps({"workload": f"{n}^3 ball, initial velocity {velocity}, {frames} frames of "
+                                  f"dt 1/240, solves to {eps:g}, "
+                                  f"{'FLIP particles and velocity extension' if flip else 'no FLIP'}",
+                      "substeps": {"gpu": sub_g, "oracle": sub_o},
+                      "seconds": {"gpu": round(t_gpu, 2), "oracle": round(t_cpu, 2)},
+                      "max": {k: max(r[k] for r in rows) for k in rows[0] if k != "frame"},
+                      "frames": rows}), flush=True)
 
 
 def traffic_run(esf, text):
@@ -447,6 +532,11 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="arrival window of the short served-stream block of the default "
                          "run (0 to skip)")
+    ap.add_argument("--drift", type=int, default=0, metavar="FRAMES",
+                    help="free-running GPU vs oracle over FRAMES frames of the --n ball "
+                         "(full finalizeSolution with FLIP unless --no-flip); prints one JSON line")
+    ap.add_argument("--no-flip", action="store_true")
+    ap.add_argument("--drift-every", type=int, default=10)
     ap.add_argument("--sweep", type=int, default=0, metavar="N",
                     help="only time the fast-sweeping entry points on the N^3 ball; prints one "
                          "JSON line")
@@ -472,6 +562,10 @@ def main():
         return
     if args.sweep:
         sweeps(esf, unit_cube_ball_cfg, args.sweep, 3)
+        return
+    if args.drift:
+        vel = tuple(float(v) for v in args.velocity.split(","))
+        drift(esf, args.n, args.drift, vel, not args.no_flip, args.eps, args.drift_every)
         return
     if args.system:
         isolated_system(esf, args.system, args.seed, args.eps, args.system_reps)

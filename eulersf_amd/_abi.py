@@ -35,6 +35,7 @@ F_DIV_RHS = 11
 F_SOLID_FORCE = 13
 F_SOLID_MV = 14
 F_DAMPED_MASS = 15
+F_SOLID_RHS = 16
 F_SOLID_V0 = 18
 F_SOLID_DV = 19
 F_SYS_DIAG = 20
@@ -168,6 +169,9 @@ _PROTOS = {
     "esf_solid_get_mesh": (_S, [_P, ctypes.c_int, _D, _D, _I]),
     "esf_advect_surface_mesh": (_S, [_P]),
     "esf_broad_phase": (_S, [_P, _D]),
+    "esf_solid_num_cells": (_S, [_P, ctypes.c_int, _I]),
+    "esf_solid_set_deformation": (_S, [_P, ctypes.c_int, ctypes.c_long, _D]),
+    "esf_solid_solve_resident": (_S, [_P, ctypes.c_double, ctypes.c_int, _I, _D, _D]),
     "esf_init_particles": (_S, [_P, ctypes.c_int]),
     "esf_grid_to_particle": (_S, [_P, ctypes.c_int]),
     "esf_particle_to_grid": (_S, [_P, ctypes.c_int]),

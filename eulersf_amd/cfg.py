@@ -202,3 +202,18 @@ def random_spheres(rng, k, radius_range, log10_E_range, center_range=None, margi
     if len(out) < k:
         raise ValueError("could not place %d non-overlapping spheres" % k)
     return out
+
+
+def random_deformations(rng, n, stretch_sd=0.05):
+    """n deformation gradients R diag(s): R a uniform random rotation (from a
+    uniform unit quaternion, Shoemake's method), s_i ~ N(1, stretch_sd);
+    returns (n, 3, 3), row-major"""
+    u1, u2, u3 = rng.uniform(0, 1, (3, n))
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], 1)
+    x, y, z, w = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                  2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                  2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1)
+    sv = rng.normal(1.0, stretch_sd, (n, 3))
+    return R.reshape(n, 3, 3) * sv[:, None, :]

@@ -488,6 +488,10 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
     case ESF_F_SYS_DIAG: *ptr = ctx->sAdiag; *bytes = 8 * NS; break;
     case ESF_F_PRESSURE_DIAG: *ptr = ctx->D; *bytes = 8 * C; break;
     case ESF_F_SOLID_V0: *ptr = ctx->sV0; *bytes = 8 * NS; break;
+    case ESF_F_SOLID_RHS:  // writable: the right-hand side of esf_solid_solve_resident
+      *ptr = ctx->sRhs;
+      *bytes = 8 * NS;
+      return ESF_OK;
     case ESF_F_SOLID_DV: *ptr = ctx->sDv; *bytes = 8 * NS; break;
     case ESF_F_WORKSPACE: *ptr = ctx->ws; *bytes = 8 * N; break;
     default: break;
@@ -632,6 +636,57 @@ esf_status esf_extend_solid_velocity(esf_ctx* ctx, int solid) {
   ESF_HIP(hipStreamSynchronize(ctx->stream));
   return ESF_OK;
 }
+esf_status esf_solid_num_cells(esf_ctx* ctx, int solid, int* n) {
+  if (!ctx || !n || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
+  if (ctx->solids[solid].listDirty) {
+    ctx->err = "solid masses changed: call esf_gather_solid_masses first";
+    return ESF_ERR_STATE;
+  }
+  *n = ctx->solids[solid].nValid;
+  return ESF_OK;
+}
+
+esf_status esf_solid_set_deformation(esf_ctx* ctx, int solid, long n, const double* F) {
+  if (!ctx || solid < 0 || solid >= (int)ctx->solids.size() || n < 0 || (n && !F))
+    return ESF_ERR_INVALID;
+  DevSolid& so = ctx->solids[solid];
+  if (n == 0) {
+    so.nFset = -1;
+    return ESF_OK;
+  }
+  if (so.listDirty || n != so.nValid) {
+    ctx->err = "one deformation gradient per valid cell (esf_solid_num_cells) is needed";
+    return ESF_ERR_INVALID;
+  }
+  esf_status st = dalloc(ctx, &so.Fset, 9 * n);
+  if (st) return st;
+  ESF_HIP(hipMemcpyAsync(so.Fset, F, 72 * n, hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  so.nFset = n;
+  return ESF_OK;
+}
+
+esf_status esf_solid_solve_resident(esf_ctx* ctx, double eps, int maxIt, int* iters,
+                                    double* relres, double* deviceMs) {
+  if (!ctx || !(eps > 0) || maxIt < 1) return ESF_ERR_INVALID;
+  esf_status st = solidAdiag(ctx);
+  if (st) return st;
+  ESF_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+  if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, eps, maxIt, 3, nullptr))) return st;
+  ESF_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+  ESF_HIP(hipEventSynchronize(ctx->ev[5]));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
+  int it = 0;
+  bool conv = false;
+  double rr = 0;
+  solveReport(ctx->hostSc[3], maxIt, &it, &conv, &rr);
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  if (deviceMs) *deviceMs = ms;
+  return conv ? ESF_OK : ESF_NOT_CONVERGED;
+}
+
 esf_status esf_num_particles(esf_ctx* ctx, int solid, long* n) {
   if (!ctx || !n || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
   *n = ctx->solids[solid].nPart;

@@ -93,6 +93,11 @@ struct DevSolid {
   std::vector<int> meshTriHost;
   // FLIP material particles (DEFOR_SOLID_3D::_materialParticles): position
   // and velocity, 3 doubles each; their bin (cell) and the bins' contents
+  // a deformation gradient per valid cell set by esf_solid_set_deformation
+  // (9 doubles each, valid-cell order), used by the stress stage while
+  // nFset equals the valid-cell count
+  double* Fset = nullptr;
+  long nFset = -1;
   long nPart = 0, capPart = 0;
   double *pPos = nullptr, *pVel = nullptr;
   int *pBin = nullptr, *binItems = nullptr;

@@ -37,7 +37,8 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, int cap,
                                                const int* __restrict__ sidx, double lambda,
                                                double mu, double* cellMat, double* cellOut,
                                                double* cellDiag, int* cornerDof,
-                                               double* energyPart) {
+                                               double* energyPart,
+                                               const double* __restrict__ Fset) {
   __shared__ double red[16];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double en = 0;
@@ -70,7 +71,13 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, int cap,
       for (int k = 0; k < 9; k++) F[k] = (k % 4 == 0) ? 1.0 : 0.0;
       Jd = 1;
     }
-    const double Jinv = 1.0 / (1.0 / Jd);  // Jinv = 1.0 / computeF(...)  (:760)
+    double Jinv = 1.0 / (1.0 / Jd);  // Jinv = 1.0 / computeF(...)  (:760)
+    if (Fset) {
+      // a deformation gradient given per valid cell (the isolated system's
+      // solids): F as given, det F^-1 = 1 / det F
+      for (int k = 0; k < 9; k++) F[k] = Fset[9L * e + k];
+      Jinv = 1.0 / det(F);
+    }
     double R[9], S[9], L[9], P[9];
     corotDecompose(F, lambda, mu, R, S, L);
     corotP(R, S, lambda, mu, P);
@@ -626,7 +633,8 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
     if (so.nValid > 0)
       hipLaunchKernelGGL(kStress, dim3(nb), dim3(256), 0, ctx->stream, d, so.nValid, so.capValid,
                          so.cellList, so.u, so.mass, ctx->sidx, so.lambda, so.mu, so.cellMat,
-                         so.cellOut, so.cellDiag, so.cornerDof, ctx->partials);
+                         so.cellOut, so.cellDiag, so.cornerDof, ctx->partials,
+                         so.nFset == so.nValid ? (const double*)so.Fset : nullptr);
     if (energy && so.nValid > 0) {
       std::vector<double> h(nb);
       ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, nb * sizeof(double),

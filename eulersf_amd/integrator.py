@@ -96,6 +96,7 @@ class CoupledIntegrator3D:
             _abi.F_DAMPED_MASS: (3 * ns, np.float64), _abi.F_SYS_DIAG: (3 * ns, np.float64),
             _abi.F_PRESSURE_DIAG: (c, np.float64), _abi.F_SOLID_V0: (3 * ns, np.float64),
             _abi.F_SOLID_DV: (3 * ns, np.float64), _abi.F_WORKSPACE: (n, np.float64),
+            _abi.F_SOLID_RHS: (3 * ns, np.float64),
             _abi.S_U: (3 * n, np.float64), _abi.S_X: (3 * n, np.float64),
             _abi.S_VELOCITY: (3 * n, np.float64), _abi.S_MASS: (n, np.float64),
             _abi.S_VALID: (c, np.int8), _abi.S_QFRAC: (8 * n, np.float64),
@@ -302,6 +303,30 @@ class CoupledIntegrator3D:
         check(self.ctx, self.lib.esf_extend_vector_field(self.ctx, b.ctypes.data_as(_abi._I), dptr(m), dptr(ph),
                                                          dptr(out)), "extendVectorField")
         return out
+
+    # -- the isolated system's solids -----------------------------------------
+    def solidNumCells(self, solid=0):
+        n = ctypes.c_int()
+        check(self.ctx, self.lib.esf_solid_num_cells(self.ctx, solid, ctypes.byref(n)))
+        return n.value
+
+    def setDeformation(self, F, solid=0):
+        """one 3x3 F per valid cell (None: back to the displacement)"""
+        if F is None:
+            check(self.ctx, self.lib.esf_solid_set_deformation(self.ctx, solid, 0, None))
+            return
+        f = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+        check(self.ctx, self.lib.esf_solid_set_deformation(self.ctx, solid, f.size // 9, dptr(f)),
+              "setDeformation")
+
+    def solidSolveResident(self, eps, max_it):
+        """A x = b (b in F_SOLID_RHS) from x = 0 after forces(); returns
+        (iterations, relative residual, device ms, status); x in F_SOLID_V0"""
+        it, rr, ms = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        st = check(self.ctx, self.lib.esf_solid_solve_resident(
+            self.ctx, eps, max_it, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(ms)),
+            "solid solve")
+        return it.value, rr.value, ms.value, st
 
     # -- FLIP particles (DEFOR_SOLID_3D) --------------------------------------
     def initMaterialParticles(self, solid=0):

@@ -121,6 +121,7 @@ enum {
   ESF_F_DAMPED_MASS = 15,  /* double[3*nSolid] */
   ESF_F_SYS_DIAG = 20,     /* double[3*nSolid] diag(A) */
   ESF_F_PRESSURE_DIAG = 21,/* double[cells]    diag(J M^-1 J^T) */
+  ESF_F_SOLID_RHS = 16,    /* double[3*nSolid] dt F + M v (writable)   */
   ESF_F_SOLID_V0 = 18,     /* double[3*nSolid] */
   ESF_F_SOLID_DV = 19,     /* double[3*nSolid] */
   ESF_F_WORKSPACE = 23,    /* double[nodes]    |omega| */
@@ -341,6 +342,21 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x,
 esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int max_it,
                                        int* iterations, int* refinements,
                                        double* rel_residual, double* device_ms);
+/* valid (stress-carrying) cells of a solid, after esf_gather_solid_masses */
+esf_status esf_solid_num_cells(esf_ctx* ctx, int solid, int* n);
+/* Measurement entry point, no reference counterpart (the isolated coupled
+ * system): the stress stage takes F from here, one row-major 3x3 per valid
+ * cell in the cells' linear order (n = esf_solid_num_cells), with
+ * det F^-1 = 1 / det F, instead of forming it from the displacement
+ * (DEFOR_SOLID_3D.cpp:758-760).  n = 0 returns to the displacement. */
+esf_status esf_solid_set_deformation(esf_ctx* ctx, int solid, long n, const double* F);
+/* Measurement entry point, no reference counterpart: the first solid solve
+ * of pcrImplicitSolve (COUPLED_INTEGRATOR_3D.cpp:664-675), A x = b with
+ * A = (dt^2 + beta dt) K + diag(M~) after esf_forces, b in ESF_F_SOLID_RHS
+ * (resident), from x = 0 in FP64 (CONJUGATE_RESIDUAL.inl:194-276); x in
+ * ESF_F_SOLID_V0; device_ms between HIP events on the context stream. */
+esf_status esf_solid_solve_resident(esf_ctx* ctx, double eps, int max_it, int* iterations,
+                                    double* rel_residual, double* device_ms);
 /* COROTATION::init + firstPiolaKirchhoff + strainEnergy +
  * firstPKDifferential (COROTATION.cpp:37-107) on n deformation gradients,
  * one GPU thread each.  F, dF: n*9 row-major; out: n*36 (R,S,L,P);

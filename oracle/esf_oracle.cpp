@@ -416,6 +416,9 @@ struct Solid {
   // FLIP material particles (PARTICLE, DEFOR_SOLID_3D.h): position,
   // velocity (3 each) and mass
   vector<double> ppos, pvel, pmass;
+  // a deformation gradient per valid cell (valid-cell order) replacing the
+  // displacement's in the stress stage (the isolated system's solids)
+  vector<double> Fset;
   vector<int8_t> valid;             // cells
   vector<double> qfrac;             // 8*nodes
   int tMin[3], tMax[3], bMin[3], bMax[3];
@@ -893,6 +896,10 @@ double stiffnessAndForce(Scene& s, Solid& so, vector<double>& force,
     long index = cells[e];
     double F[9];
     double Jinv = 1.0 / computeF(s, so, index, F);
+    if ((long)so.Fset.size() == 9 * nc) {
+      for (int k = 0; k < 9; k++) F[k] = so.Fset[9 * e + k];
+      Jinv = 1.0 / det3(F);
+    }
     Corot m;
     m.lambda = so.lambda;
     m.mu = so.mu;
@@ -1845,6 +1852,31 @@ void orc_solid_finalize(void* h) { solidFinalize(*static_cast<Scene*>(h)); }
 void orc_compute_solid_mass(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);
   computeSolidMass(s, s.solids[solid], false);
+}
+long orc_solid_num_cells(void* h, int solid) {
+  Scene& s = *static_cast<Scene*>(h);
+  Solid& so = s.solids[solid];
+  const long cx = s.X - 1, cxy = (long)(s.X - 1) * (s.Y - 1);
+  long n = 0;
+  for (int z = so.tMin[2]; z < so.tMax[2] - 1; z++)
+    for (int y = so.tMin[1]; y < so.tMax[1] - 1; y++)
+      for (int x = so.tMin[0]; x < so.tMax[0] - 1; x++)
+        n += so.valid[x + y * cx + z * cxy] == 1;
+  return n;
+}
+void orc_solid_set_deformation(void* h, int solid, long n, const double* F) {
+  Solid& so = static_cast<Scene*>(h)->solids[solid];
+  so.Fset.assign(F, F + 9 * n);
+}
+// solvePCR on the assembled solid system (CONJUGATE_RESIDUAL.inl:194-276)
+// from x = 0, after orc_forces and orc_assemble
+int orc_solid_cr(void* h, const double* b, double* x, double eps, int max_it, int* iters) {
+  Scene& s = *static_cast<Scene*>(h);
+  vector<double> bb(b, b + s.A.n), xx(s.A.n, 0.0);
+  auto apply = [&](const vector<double>& in, vector<double>& out) { csrMul(s.A, in, out); };
+  bool c = pcr(apply, s.Adiag, bb, xx, eps, max_it, iters, nullptr);
+  std::memcpy(x, xx.data(), sizeof(double) * xx.size());
+  return c ? 0 : 1;
 }
 void orc_init_particles(void* h, int solid) {
   Scene& s = *static_cast<Scene*>(h);

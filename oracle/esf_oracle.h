@@ -131,6 +131,12 @@ void orc_particle_to_grid(void* h, int solid);
 long orc_num_particles(void* h, int solid);
 void orc_get_particles(void* h, int solid, double* pos, double* vel);
 void orc_set_particles(void* h, int solid, long n, const double* pos, const double* vel);
+/* the isolated system: valid cells of a solid, a deformation gradient per
+ * valid cell for the stress stage, the solid CR from x = 0 on the
+ * assembled A */
+long orc_solid_num_cells(void* h, int solid);
+void orc_solid_set_deformation(void* h, int solid, long n, const double* F);
+int orc_solid_cr(void* h, const double* b, double* x, double eps, int max_it, int* iters);
 /* DEFOR :566-574 on the solid's own velocity, mass and phi */
 void orc_extend_solid_velocity(void* h, int solid);
 /* DEFOR :671-682: F of the cell with min-corner node `index` (row-major), returns det F */

@@ -82,6 +82,10 @@ def lib():
         for nm in ("orc_init_particles", "orc_grid_to_particle", "orc_particle_to_grid",
                    "orc_extend_solid_velocity"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_solid_num_cells.restype = ctypes.c_long
+        L.orc_solid_num_cells.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_solid_set_deformation.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, _D]
+        L.orc_solid_cr.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_double, ctypes.c_int, _I]
         L.orc_num_particles.restype = ctypes.c_long
         L.orc_num_particles.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_get_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D]

@@ -261,3 +261,53 @@ def test_broad_phase_alpha_from_negative_alpha(esf, center_y, alpha):
     assert gs["dt"] == o.L.orc_cfl(o.h, alpha)
     g.close()
     o.close()
+
+
+def test_isolated_solid_system_matches_oracle(esf):
+    """the solid half of the isolated coupled system at 24^3 cells: 8 seeded
+    spheres (E log-uniform in [1e3, 1e6]), each valid cell's deformation
+    gradient a random rotation times diag(N(1, 0.05)); the elastic forces,
+    the system diagonal and the solid CR's solution of A x = b from x = 0
+    (b ~ N(0, 1)) against the oracle's assembled solve, both to 1e-13"""
+    import ctypes
+    from eulersf_amd.cfg import random_deformations, random_spheres, scene_cfg
+    A = esf._abi
+    n = 24
+    rng = np.random.default_rng(2026)
+    spheres = random_spheres(rng, 8, (0.05, 0.12), (3.0, 6.0), center_range=(0.15, 0.85))
+    text = scene_cfg(n, spheres, fill=0.6, eps=1e-13, max_iterations=20000)
+    g = esf.CoupledIntegrator3D.from_cfg_text(text)
+    o = O.OracleScene.from_cfg_text(text)
+    g.setDt(1.0 / 240)
+    g.forces()
+    base = g.download(A.F_SOLID_FORCE).copy()
+    for b in range(8):
+        nc = g.solidNumCells(b)
+        assert nc == o.L.orc_solid_num_cells(o.h, b) > 0
+        F = np.ascontiguousarray(random_deformations(rng, nc))
+        g.setDeformation(F, b)
+        o.L.orc_solid_set_deformation(o.h, b, nc, O.dp(F))
+    o.L.orc_set_dt(o.h, 1.0 / 240)
+    g.forces()
+    o.L.orc_forces(o.h)
+    o.L.orc_assemble(o.h)
+    assert rel_err(g.download(A.F_SOLID_FORCE), o.field(O.SOLID_FORCE)) < 1e-12
+    ns = g.num_solid_nodes()
+    rhs = rng.standard_normal(3 * ns)
+    g.upload(A.F_SOLID_RHS, rhs)
+    it, rr, ms, st = g.solidSolveResident(1e-13, 20000)
+    assert st == 0 and rr < 1e-13
+    np.testing.assert_allclose(g.download(A.F_SYS_DIAG), o.field(O.SYS_DIAG), rtol=1e-12)
+    xo = np.zeros(3 * ns)
+    ito = ctypes.c_int()
+    assert o.L.orc_solid_cr(o.h, O.dp(rhs), O.dp(xo), 1e-13, 20000, ctypes.byref(ito)) == 0
+    _say(f"solid system: {ns} nodes, iterations GPU {it}, oracle {ito.value}, {ms:.2f} ms")
+    assert abs(it - ito.value) <= 5
+    assert rel_err(g.download(A.F_SOLID_V0), xo) < 1e-10
+    # the deformation from the displacement again: forces as before any injection
+    for b in range(8):
+        g.setDeformation(None, b)
+    g.forces()
+    np.testing.assert_array_equal(g.download(A.F_SOLID_FORCE), base)
+    g.close()
+    o.close()

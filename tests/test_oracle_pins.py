@@ -246,6 +246,77 @@ def test_element_force_rigid_motions_and_uniform_stretch():
     s.close()
 
 
+def test_injected_deformation_per_cell():
+    """per-cell deformation gradients injected in place of the displacement's:
+    a different F_c = R_c S_c in each valid cell gives the nodal forces
+    -(dh^2 / 4) sum_c sigma_c s_c(n) with sigma_c = det(F_c)^-1 P(F_c) F_c^T,
+    injecting computeF's own values reproduces the displacement's forces
+    to rounding, and the solid CR's answer on the assembled system leaves
+    |A x - b| <= eps |b|"""
+    from eulersf_amd.cfg import random_deformations
+    n = 8
+    dh = 1.0 / n
+    X, C = n + 1, n
+    s = block_scene(n, (2 * dh, 2 * dh, 2 * dh), (5 * dh, 5 * dh, 5 * dh))
+    s.L.orc_set_dt(s.h, 1.0 / 240)
+    valid = s.field(O.S_VALID, 0).reshape(C, C, C) == 1
+    nc = s.L.orc_solid_num_cells(s.h, 0)
+    assert nc == valid.sum() == 27
+    rng = np.random.default_rng(5)
+    Fs = np.ascontiguousarray(random_deformations(rng, nc, 0.08))
+    s.L.orc_solid_set_deformation(s.h, 0, nc, O.dp(Fs))
+    s.L.orc_forces(s.h)
+    want = np.zeros((X, X, X, 3))
+    e = 0
+    for cz in range(C):
+        for cy in range(C):
+            for cx in range(C):
+                if not valid[cz, cy, cx]:
+                    continue
+                F = Fs[e]
+                e += 1
+                U, sv, Vt = np.linalg.svd(F)
+                R, S = U @ Vt, Vt.T @ np.diag(sv) @ Vt
+                E = S - np.eye(3)
+                sigma = R @ (2 * MU * E + LAM * np.trace(E) * np.eye(3)) @ F.T / np.linalg.det(F)
+                for i in range(8):
+                    ox, oy, oz = i & 1, (i >> 1) & 1, i >> 2
+                    sg = np.array([2 * ox - 1, 2 * oy - 1, 2 * oz - 1], float)
+                    want[cz + oz, cy + oy, cx + ox] -= dh ** 2 / 4 * sigma @ sg
+    got = solid_force_by_node(s, n)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * np.abs(want).max())
+    assert np.abs(want).max() > 1.0
+    # the solid CR on the assembled system
+    s.L.orc_assemble(s.h)
+    ndof = 3 * s.L.orc_num_solid_dofs(s.h)
+    b = rng.standard_normal(ndof)
+    x = np.zeros(ndof)
+    it = ctypes.c_int()
+    assert s.L.orc_solid_cr(s.h, O.dp(b), O.dp(x), 1e-12, 20000, ctypes.byref(it)) == 0
+    Ax = np.zeros(ndof)
+    s.L.orc_solid_apply(s.h, O.dp(x), O.dp(Ax))
+    assert 0 < it.value and np.linalg.norm(Ax - b) <= 1e-12 * np.linalg.norm(b)
+    # computeF's values injected: the displacement's forces, bitwise
+    set_deformation(s, n, 0.05 * rng.standard_normal((3, 3)))
+    s.L.orc_solid_set_deformation(s.h, 0, 0, O.dp(Fs))
+    s.L.orc_forces(s.h)
+    base = s.field(O.SOLID_FORCE).copy()
+    Fc = np.zeros((nc, 9))
+    e = 0
+    for cz in range(C):
+        for cy in range(C):
+            for cx in range(C):
+                if valid[cz, cy, cx]:
+                    s.L.orc_compute_F(s.h, 0, cx + cy * X + cz * X * X, O.dp(Fc[e]))
+                    e += 1
+    s.L.orc_solid_set_deformation(s.h, 0, nc, O.dp(Fc))
+    s.L.orc_forces(s.h)
+    # (det F recomputed from the injected F, not as 1 / det F^-1: rounding)
+    np.testing.assert_allclose(s.field(O.SOLID_FORCE), base, rtol=0, atol=1e-13 * np.abs(base).max())
+    assert np.abs(base).max() > 1.0
+    s.close()
+
+
 def test_stiffness_symmetric_and_free_of_rigid_modes():
     """A = dt^2 K + diag(M~) (beta = 0): symmetric (the corotational tangent
     with the doubled lambda trace is still symmetric), A t = M~ t for a
