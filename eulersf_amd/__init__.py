@@ -8,4 +8,4 @@ fails loudly when the library has not been built.
 """
 from ._abi import (ESF_MIXED, ESF_PLAIN_FP64, EsfError, exported_symbols, load)  # noqa: F401
 from .cfg import Cfg, params_from_cfg, unit_cube_ball_cfg  # noqa: F401
-from .integrator import CoupledIntegrator3D, corotation_batch  # noqa: F401
+from .integrator import CoupledIntegrator3D, corotation_batch, step_batch  # noqa: F401

@@ -141,6 +141,8 @@ _PROTOS = {
     "esf_advect_intensity": (_S, [_P]),
     "esf_recover_and_bc": (_S, [_P]),
     "esf_step": (_S, [_P, ctypes.c_double, ctypes.POINTER(SolveStats)]),
+    "esf_step_batch": (_S, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_double,
+                            ctypes.POINTER(SolveStats), _I]),
     "esf_pressure_apply": (_S, [_P, _D, _D]),
     "esf_pressure_diag": (_S, [_P, _D]),
     "esf_solid_apply": (_S, [_P, _D, _D]),

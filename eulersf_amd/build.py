@@ -36,12 +36,30 @@ SOURCES = [
     os.path.join(CSRC, "esf_sweep.hip"),
     os.path.join(CSRC, "esf_flip.hip"),
     os.path.join(CSRC, "esf_api.hip"),
+    os.path.join(CSRC, "esf_batch.hip"),
     os.path.join(HOST, "esf_scene.cpp"),
     os.path.join(HOST, "esf_frames.cpp"),
 ]
 
 
+# host-only sources: the fiber switch is x86-64 assembly, kept out of the
+# offload compiler's device pass
+HOST_ONLY = [os.path.join(HOST, "esf_fiber.cpp")]
+GXX = os.environ.get("CXX", "g++")
+
+
+def _compile_host(src):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    cmd = [GXX, "-O2", "-std=c++17", "-fPIC", "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
 def _compile(src):
+    if src in HOST_ONLY:
+        return _compile_host(src)
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     lang = ["-x", "hip"] if src.endswith(".cpp") else []
     cmd = [HIPCC, *FLAGS, *lang, "-c", src, "-o", obj]
@@ -56,7 +74,7 @@ def build(verbose=False):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, SOURCES + HOST_ONLY))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-lz", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -142,7 +142,7 @@ struct StageTimer {
   ~StageTimer() {
     if (!ctx->stageTiming) return;
     hipEventRecord(ctx->ev[1], ctx->stream);
-    hipEventSynchronize(ctx->ev[1]);
+    eventSync(ctx, ctx->ev[1]);
     float ms = 0;
     hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
     ctx->stageMs[stage] += ms;
@@ -251,7 +251,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
                  hipSuccess)
     st = ESF_ERR_OOM;
   if (!st) fillInt(ctx, N, ctx->sidx, -1);
-  if (!st && hipStreamSynchronize(ctx->stream) != hipSuccess) st = ESF_ERR_DEVICE;
+  if (!st && streamSync(ctx) != hipSuccess) st = ESF_ERR_DEVICE;
   if (st) {
     esf_ctx_destroy(ctx);
     return st;
@@ -263,7 +263,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
 void esf_ctx_destroy(esf_ctx* ctx) {
   if (!ctx) return;
   if (ctx->side) hipStreamSynchronize(ctx->side);
-  hipStreamSynchronize(ctx->stream);
+  streamSync(ctx);
   freeAll(ctx);
   for (int i = 0; i < 24; i++) hipEventDestroy(ctx->ev[i]);
   hipEventDestroy(ctx->evFork);
@@ -355,7 +355,7 @@ esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu, double density,
   ESF_HIP(hipMemcpyAsync(so.rest, rest, sizeof(double) * (long)rx * ry * rz,
                          hipMemcpyHostToDevice, ctx->stream));
   fillInt(ctx, d.cells, so.cellSlot, -1);
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   if (id) *id = (int)ctx->solids.size() - 1;
   return ESF_OK;
 }
@@ -385,7 +385,7 @@ esf_status esf_solid_set_mesh(esf_ctx* ctx, int solid, int nv, const double* res
   ESF_HIP(hipMemcpyAsync(so.meshRest, rest, 24L * nv, hipMemcpyHostToDevice, ctx->stream));
   ESF_HIP(hipMemcpyAsync(so.meshVert, vertices, 24L * nv, hipMemcpyHostToDevice, ctx->stream));
   ESF_HIP(hipMemcpyAsync(so.meshTri, tris, 12L * nt, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   so.meshTriHost.assign(tris, tris + 3L * nt);
   so.meshV = nv;
   so.meshT = nt;
@@ -414,7 +414,7 @@ esf_status esf_solid_get_mesh(esf_ctx* ctx, int solid, double* vertices, double*
   if (rest)
     ESF_HIP(hipMemcpyAsync(rest, so.meshRest, 24L * so.meshV, hipMemcpyDeviceToHost,
                            ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   if (tris) std::copy(so.meshTriHost.begin(), so.meshTriHost.end(), tris);
   return ESF_OK;
 }
@@ -423,7 +423,7 @@ esf_status esf_advect_surface_mesh(esf_ctx* ctx) {
   if (!ctx) return ESF_ERR_INVALID;
   esf_status st = advectMeshes(ctx);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -557,7 +557,7 @@ esf_status esf_upload(esf_ctx* ctx, int field, int solid, const void* src, size_
     return ESF_ERR_INVALID;
   }
   ESF_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -572,7 +572,7 @@ esf_status esf_download(esf_ctx* ctx, int field, int solid, void* dst, size_t by
     return ESF_ERR_INVALID;
   }
   ESF_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -584,7 +584,7 @@ esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid) {
   }
   esf_status st = computeSolidMass(ctx, solid, false);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -596,7 +596,7 @@ esf_status esf_compute_solid_mass_sdf(esf_ctx* ctx, int solid) {
   }
   esf_status st = computeSolidMass(ctx, solid, true);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -609,7 +609,7 @@ static esf_status flipCall(esf_ctx* ctx, int solid, esf_status (*fn)(esf_ctx*, i
   if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
   esf_status st = fn(ctx, solid);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -633,7 +633,7 @@ esf_status esf_extend_solid_velocity(esf_ctx* ctx, int solid) {
                       so.box.bMax[0], so.box.bMax[1], so.box.bMax[2]};
   esf_status st = extendDevice(ctx, box, so.mass, so.phi, so.vel);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 esf_status esf_solid_num_cells(esf_ctx* ctx, int solid, int* n) {
@@ -661,7 +661,7 @@ esf_status esf_solid_set_deformation(esf_ctx* ctx, int solid, long n, const doub
   esf_status st = dalloc(ctx, &so.Fset, 9 * n);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(so.Fset, F, 72 * n, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   so.nFset = n;
   return ESF_OK;
 }
@@ -674,7 +674,7 @@ esf_status esf_solid_solve_resident(esf_ctx* ctx, double eps, int maxIt, int* it
   ESF_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
   if ((st = solidSolve(ctx, 2, ctx->sRhs, ctx->sV0, eps, maxIt, 3, nullptr))) return st;
   ESF_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-  ESF_HIP(hipEventSynchronize(ctx->ev[5]));
+  ESF_HIP(eventSync(ctx, ctx->ev[5]));
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
   int it = 0;
@@ -699,7 +699,7 @@ esf_status esf_get_particles(esf_ctx* ctx, int solid, double* pos, double* vel) 
     ESF_HIP(hipMemcpyAsync(pos, so.pPos, 24 * so.nPart, hipMemcpyDeviceToHost, ctx->stream));
   if (vel && so.nPart)
     ESF_HIP(hipMemcpyAsync(vel, so.pVel, 24 * so.nPart, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 esf_status esf_set_particles(esf_ctx* ctx, int solid, long n, const double* pos,
@@ -713,7 +713,7 @@ esf_status esf_set_particles(esf_ctx* ctx, int solid, long n, const double* pos,
     ESF_HIP(hipMemcpyAsync(so.pPos, pos, 24 * n, hipMemcpyHostToDevice, ctx->stream));
     ESF_HIP(hipMemcpyAsync(so.pVel, vel, 24 * n, hipMemcpyHostToDevice, ctx->stream));
   }
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   so.nPart = n;
   return ESF_OK;
 }
@@ -723,7 +723,7 @@ esf_status esf_solid_sdf(esf_ctx* ctx, int solid) {
   esf_status st = updateBoxes(ctx, solid);
   if (!st) st = solidPhi(ctx, solid);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -737,7 +737,7 @@ esf_status esf_recover_velocity(esf_ctx* ctx) {
   if (!ctx) return ESF_ERR_INVALID;
   esf_status st = recoverAndBC(ctx, 0);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -851,7 +851,7 @@ esf_status esf_coupled_solve(esf_ctx* ctx, esf_solve_stats* stats) {
   esf_solve_stats s{};
   esf_status st = coupledSolveEnqueue(ctx, &s);
   if (st) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   st = coupledSolveReport(ctx, &s);
   if (stats) *stats = s;
   return st;
@@ -928,7 +928,8 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     if ((st = cflEnqueue(ctx, alpha))) return st;
     if ((st = stressStage(ctx, nullptr))) return st;
     if ((st = cflFinish(ctx, &dt))) return st;
-    st = inFlight.alone ? forkBodyForces(ctx) : vorticityAndBodyForces(ctx, nullptr);
+    st = inFlight.alone && !ctx->fiber ? forkBodyForces(ctx)
+                                       : vorticityAndBodyForces(ctx, nullptr);
     if (st) return st;
   }
   // the solve's reports are read at the end of the step: no stall between
@@ -976,7 +977,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   solveSt = coupledSolveReport(ctx, &s);
   if (stats) *stats = s;
   return solveSt;
@@ -989,7 +990,7 @@ esf_status esf_pressure_apply(esf_ctx* ctx, const double* x, double* y) {
   esf_status st = pressureApplyDevice(ctx, ctx->r, ctx->s);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(y, ctx->s, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -998,7 +999,7 @@ esf_status esf_pressure_diag(esf_ctx* ctx, double* dOut) {
   esf_status st = pressureRhs(ctx);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(dOut, ctx->D, 8 * ctx->d.cells, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -1010,7 +1011,7 @@ esf_status esf_solid_apply(esf_ctx* ctx, const double* x, double* y) {
   esf_status st = solidApplyDevice(ctx, ctx->stmp, ctx->sr);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(y, ctx->sr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -1027,7 +1028,7 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double e
                                   ctx->r2, ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 3);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(x, ctx->lambda, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   solveReport(ctx->hostSc[3], maxIt, &it, &conv, &rr);
   if (iters) *iters = it;
   if (relres) *relres = rr;
@@ -1050,7 +1051,7 @@ esf_status esf_pressure_solve_resident(esf_ctx* ctx, double eps, int maxIt, int*
   }
   if (st) return st;
   ESF_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-  ESF_HIP(hipEventSynchronize(ctx->ev[5]));
+  ESF_HIP(eventSync(ctx, ctx->ev[5]));
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
   if (ctx->p.precision != ESF_MIXED) solveReport(ctx->hostSc[3], maxIt, &it, &conv, &rr);
@@ -1106,7 +1107,7 @@ esf_status esf_ctx_device_bytes(const esf_ctx* ctx, size_t* bytes) {
 
 esf_status esf_synchronize(esf_ctx* ctx) {
   if (!ctx) return ESF_ERR_INVALID;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -1134,7 +1135,7 @@ esf_status esf_set_kernel_timing(esf_ctx* ctx, int on) {
 
 esf_status esf_kernel_timing(esf_ctx* ctx, double* total_ms, long* counts, int reset) {
   if (!ctx) return ESF_ERR_INVALID;
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   for (size_t i = 0; i < ctx->timedUsed; i++) {
     float ms = 0;
     const auto& t = ctx->timed[i];

@@ -38,10 +38,13 @@ __device__ inline double bspline(double d) {
   return 0;
 }
 
-__global__ void kInitParticles(Dims d, int nValid, int ppc, const int* __restrict__ cellList,
-                               const double* __restrict__ rnd, const double* __restrict__ vel,
-                               double* pos, double* pv) {
-  const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kInitParticles(const GridT& wg, Dims d, int nValid, int ppc,
+                                               const int* __restrict__ cellList,
+                                               const double* __restrict__ rnd,
+                                               const double* __restrict__ vel, double* pos,
+                                               double* pv) {
+  const long k = wg.block() * (long)blockDim.x + threadIdx.x;
   if (k >= (long)nValid * ppc) return;
   const long node = cellList[k / ppc];
   const int x = node % d.X, y = (node / d.X) % d.Y, z = node / d.slab;
@@ -55,6 +58,7 @@ __global__ void kInitParticles(Dims d, int nValid, int ppc, const int* __restric
     pv[3 * k + c] = v[c];
   }
 }
+ESF_BOUNDS(kInitParticles, 1024);
 
 struct G2PArgs {
   long n;
@@ -66,8 +70,9 @@ struct G2PArgs {
   double alpha, dt, thick, ks, friction, pmass;
 };
 
-__global__ void __launch_bounds__(256) kG2P(Dims d, G2PArgs a) {
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kG2P(const GridT& wg, Dims d, G2PArgs a) {
+  const long c = wg.block() * (long)blockDim.x + threadIdx.x;
   if (c >= a.n) return;
   double pp[3], pv[3];
   for (int q = 0; q < 3; q++) {
@@ -134,6 +139,7 @@ __global__ void __launch_bounds__(256) kG2P(Dims d, G2PArgs a) {
     a.pv[3 * c + q] = pv[q];
   }
 }
+ESF_BOUNDS(kG2P, 256);
 
 // bins: the cells of the grid widened by two on every side, a particle's
 // bin from its floor((pos - o) / dh) clamped there
@@ -153,26 +159,32 @@ __device__ inline int binOf(const Dims& d, const Bins& b, const double* p) {
   return q[0] + b.nx * (q[1] + b.ny * q[2]);
 }
 
-__global__ void kZeroInts(long n, int* a) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+template <class GridT>
+__device__ __forceinline__ void kZeroInts(const GridT& wg, long n, int* a) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     a[i] = 0;
 }
+ESF_BOUNDS(kZeroInts, 1024);
 
-__global__ void kBinCount(Dims d, Bins b, long n, const double* __restrict__ pos, int* pBin,
-                          int* count) {
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kBinCount(const GridT& wg, Dims d, Bins b, long n,
+                                          const double* __restrict__ pos, int* pBin, int* count) {
+  const long c = wg.block() * (long)blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int bin = binOf(d, b, &pos[3 * c]);
   pBin[c] = bin;
   atomicAdd(&count[bin], 1);
 }
+ESF_BOUNDS(kBinCount, 1024);
 
 // exclusive scan of counts[0, n) into starts, in tiles of 1024 (block
 // totals into blocks[], scanned by one block, then added back)
-__global__ void kTileSums(long n, const int* __restrict__ count, int* blocks) {
+template <class GridT>
+__device__ __forceinline__ void kTileSums(const GridT& wg, long n, const int* __restrict__ count,
+                                          int* blocks) {
   __shared__ int red[256];
-  const long base = (long)blockIdx.x * 1024;
+  const long base = (long)wg.block() * 1024;
   int s = 0;
   for (int k = 0; k < 4; k++) {
     const long i = base + threadIdx.x * 4 + k;
@@ -184,10 +196,12 @@ __global__ void kTileSums(long n, const int* __restrict__ count, int* blocks) {
     if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) blocks[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) blocks[wg.block()] = red[0];
 }
+ESF_BOUNDS(kTileSums, 1024);
 
-__global__ void kTileScan(int nb, int* blocks) {
+template <class GridT>
+__device__ __forceinline__ void kTileScan(const GridT& wg, int nb, int* blocks) {
   if (threadIdx.x != 0) return;
   int acc = 0;
   for (int i = 0; i < nb; i++) {
@@ -196,11 +210,13 @@ __global__ void kTileScan(int nb, int* blocks) {
     acc += v;
   }
 }
+ESF_BOUNDS(kTileScan, 1024);
 
-__global__ void kTileEmit(long n, const int* __restrict__ count, const int* __restrict__ blocks,
-                          int* starts, int* cursor) {
+template <class GridT>
+__device__ __forceinline__ void kTileEmit(const GridT& wg, long n, const int* __restrict__ count,
+                                          const int* __restrict__ blocks, int* starts, int* cursor) {
   __shared__ int pre[256];
-  const long base = (long)blockIdx.x * 1024;
+  const long base = (long)wg.block() * 1024;
   int v[4], s = 0;
   for (int k = 0; k < 4; k++) {
     const long i = base + threadIdx.x * 4 + k;
@@ -218,7 +234,7 @@ __global__ void kTileEmit(long n, const int* __restrict__ count, const int* __re
     }
   }
   __syncthreads();
-  int acc = blocks[blockIdx.x] + pre[threadIdx.x];
+  int acc = blocks[wg.block()] + pre[threadIdx.x];
   for (int k = 0; k < 4; k++) {
     const long i = base + threadIdx.x * 4 + k;
     if (i < n) {
@@ -229,18 +245,23 @@ __global__ void kTileEmit(long n, const int* __restrict__ count, const int* __re
   }
   if (base + 1024 >= n && threadIdx.x == 255) starts[n] = acc;
 }
+ESF_BOUNDS(kTileEmit, 1024);
 
-__global__ void kBinFill(long n, const int* __restrict__ pBin, const int* __restrict__ starts,
-                         int* cursor, int* items) {
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kBinFill(const GridT& wg, long n, const int* __restrict__ pBin,
+                                         const int* __restrict__ starts, int* cursor, int* items) {
+  const long c = wg.block() * (long)blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int b = pBin[c];
   items[starts[b] + atomicAdd(&cursor[b], 1)] = (int)c;
 }
+ESF_BOUNDS(kBinFill, 1024);
 
 // each bin's particles in index order (insertion sort; bins hold a few)
-__global__ void kBinSort(long nbins, const int* __restrict__ starts, int* items) {
-  const long b = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kBinSort(const GridT& wg, long nbins, const int* __restrict__ starts,
+                                         int* items) {
+  const long b = wg.block() * (long)blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   const int lo = starts[b], hi = starts[b + 1];
   for (int i = lo + 1; i < hi; i++) {
@@ -253,6 +274,7 @@ __global__ void kBinSort(long nbins, const int* __restrict__ starts, int* items)
     items[j + 1] = v;
   }
 }
+ESF_BOUNDS(kBinSort, 1024);
 
 struct P2GArgs {
   Bins b;
@@ -265,8 +287,9 @@ struct P2GArgs {
   double* mass;  // the solid's _mass: 1 there, 0 elsewhere
 };
 
-__global__ void __launch_bounds__(256) kP2G(Dims d, P2GArgs a) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kP2G(const GridT& wg, Dims d, P2GArgs a) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   const double o[3] = {d.o0, d.o1, d.o2};
@@ -312,6 +335,7 @@ __global__ void __launch_bounds__(256) kP2G(Dims d, P2GArgs a) {
     a.mass[n] = 0;
   }
 }
+ESF_BOUNDS(kP2G, 256);
 
 Bins binsOf(const Dims& d) { return Bins{d.X + 4, d.Y + 4, d.Z + 4}; }
 
@@ -338,7 +362,7 @@ esf_status growRandTable(esf_ctx* ctx, long draws) {
   esf_status st = devAlloc(ctx, &ctx->randTable, cap);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(ctx->randTable, h.data(), 8 * cap, hipMemcpyHostToDevice, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   ctx->randCap = cap;
   return ESF_OK;
 }
@@ -362,9 +386,9 @@ esf_status initParticles(esf_ctx* ctx, int solid) {
   if ((st = growRandTable(ctx, 3 * n))) return st;
   so.nPart = n;
   if (n)
-    hipLaunchKernelGGL(kInitParticles, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, ctx->d,
-                       so.nValid, ppc, (const int*)so.cellList, (const double*)ctx->randTable,
-                       (const double*)so.vel, so.pPos, so.pVel);
+    launch<ESF_K(kInitParticles)>(ctx, dim3(cdiv(n, 256)), dim3(256), ctx->d, so.nValid, ppc,
+                                  (const int*)so.cellList, (const double*)ctx->randTable,
+                                  (const double*)so.vel, so.pPos, so.pVel);
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
@@ -388,7 +412,7 @@ esf_status gridToParticle(esf_ctx* ctx, int solid) {
   // first solid that created particles (DEFOR_SOLID_3D.cpp:338-340)
   const double dh3 = d.dh * d.dh * d.dh;
   a.pmass = (ctx->solids[0].qcm / (dh3 * 0.125)) * (dh3 / ctx->p.particles_per_cell);
-  hipLaunchKernelGGL(kG2P, dim3(cdiv(a.n, 256)), dim3(256), 0, ctx->stream, d, a);
+  launch<ESF_K(kG2P)>(ctx, dim3(cdiv(a.n, 256)), dim3(256), d, a);
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
@@ -406,24 +430,22 @@ esf_status particleToGrid(esf_ctx* ctx, int solid) {
   }
   const long n = so.nPart;
   if (n && (st = growParticles(ctx, so, n))) return st;
-  hipLaunchKernelGGL(kZeroInts, dim3(std::min<long>(cdiv(nb, 256), 4096)), dim3(256), 0,
-                     ctx->stream, nb, ctx->binCount);
+  launch<ESF_K(kZeroInts)>(ctx, dim3(std::min<long>(cdiv(nb, 256), 4096)), dim3(256), nb,
+                           ctx->binCount);
   if (n)
-    hipLaunchKernelGGL(kBinCount, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, d, b, n,
-                       (const double*)so.pPos, so.pBin, ctx->binCount);
+    launch<ESF_K(kBinCount)>(ctx, dim3(cdiv(n, 256)), dim3(256), d, b, n, (const double*)so.pPos,
+                             so.pBin, ctx->binCount);
   const int tiles = cdiv(nb, 1024);
-  hipLaunchKernelGGL(kTileSums, dim3(tiles), dim3(256), 0, ctx->stream, nb,
-                     (const int*)ctx->binCount, ctx->binBlocks);
-  hipLaunchKernelGGL(kTileScan, dim3(1), dim3(64), 0, ctx->stream, tiles, ctx->binBlocks);
-  hipLaunchKernelGGL(kTileEmit, dim3(tiles), dim3(256), 0, ctx->stream, nb,
-                     (const int*)ctx->binCount, (const int*)ctx->binBlocks, ctx->binStart,
-                     ctx->binCursor);
+  launch<ESF_K(kTileSums)>(ctx, dim3(tiles), dim3(256), nb, (const int*)ctx->binCount,
+                           ctx->binBlocks);
+  launch<ESF_K(kTileScan)>(ctx, dim3(1), dim3(64), tiles, ctx->binBlocks);
+  launch<ESF_K(kTileEmit)>(ctx, dim3(tiles), dim3(256), nb, (const int*)ctx->binCount,
+                           (const int*)ctx->binBlocks, ctx->binStart, ctx->binCursor);
   if (n) {
-    hipLaunchKernelGGL(kBinFill, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, n,
-                       (const int*)so.pBin, (const int*)ctx->binStart, ctx->binCursor,
-                       so.binItems);
-    hipLaunchKernelGGL(kBinSort, dim3(cdiv(nb, 256)), dim3(256), 0, ctx->stream, nb,
-                       (const int*)ctx->binStart, so.binItems);
+    launch<ESF_K(kBinFill)>(ctx, dim3(cdiv(n, 256)), dim3(256), n, (const int*)so.pBin,
+                            (const int*)ctx->binStart, ctx->binCursor, so.binItems);
+    launch<ESF_K(kBinSort)>(ctx, dim3(cdiv(nb, 256)), dim3(256), nb, (const int*)ctx->binStart,
+                            so.binItems);
   }
   P2GArgs a{};
   a.b = b;
@@ -434,7 +456,7 @@ esf_status particleToGrid(esf_ctx* ctx, int solid) {
   a.cutoff = ctx->p.flip_cutoff;
   a.vel = so.vel;
   a.mass = so.mass;
-  hipLaunchKernelGGL(kP2G, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, a);
+  launch<ESF_K(kP2G)>(ctx, dim3(cdiv(d.nodes, 256)), dim3(256), d, a);
   // _mass now holds the particles' 0/1 indicator over the whole grid: the
   // next mass pass clears and rebuilds all of it
   so.listDirty = true;

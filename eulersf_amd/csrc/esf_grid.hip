@@ -23,10 +23,10 @@ __device__ __forceinline__ bool isInterior(const Dims& d, int x, int y, int z) {
 // ---------------------------------------------------------------------------
 constexpr int kScanTile = 1024;  // 256 threads x 4
 
-template <class Flag>
-__global__ void kScanCount(long n, Flag flag, int* blockCounts) {
+template <class Flag, class GridT>
+__device__ __forceinline__ void kScanCount(const GridT& wg, long n, Flag flag, int* blockCounts) {
   __shared__ int red[256];
-  const long base = (long)blockIdx.x * kScanTile;
+  const long base = (long)wg.block() * kScanTile;
   int c = 0;
   for (int k = 0; k < 4; k++) {
     const long i = base + threadIdx.x * 4 + k;
@@ -38,11 +38,13 @@ __global__ void kScanCount(long n, Flag flag, int* blockCounts) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) blockCounts[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) blockCounts[wg.block()] = red[0];
 }
+ESF_BOUNDS(kScanCount, 1024);
 
 // exclusive scan of nb block counts by one block; total into blockCounts[nb]
-__global__ void kScanBlocks(int nb, int* blockCounts) {
+template <class GridT>
+__device__ __forceinline__ void kScanBlocks(const GridT& wg, int nb, int* blockCounts) {
   __shared__ int part[1024];
   const int per = (nb + 1023) / 1024;
   const int b0 = threadIdx.x * per;
@@ -69,11 +71,13 @@ __global__ void kScanBlocks(int nb, int* blockCounts) {
       acc += v;
     }
 }
+ESF_BOUNDS(kScanBlocks, 1024);
 
-template <class Flag, class Emit>
-__global__ void kScanEmit(long n, Flag flag, const int* blockCounts, Emit emit) {
+template <class Flag, class Emit, class GridT>
+__device__ __forceinline__ void kScanEmit(const GridT& wg, long n, Flag flag, const int* blockCounts,
+                                          Emit emit) {
   __shared__ int pre[256];
-  const long base = (long)blockIdx.x * kScanTile;
+  const long base = (long)wg.block() * kScanTile;
   int f[4], c = 0;
   for (int k = 0; k < 4; k++) {
     const long i = base + threadIdx.x * 4 + k;
@@ -91,7 +95,7 @@ __global__ void kScanEmit(long n, Flag flag, const int* blockCounts, Emit emit) 
     }
   }
   __syncthreads();
-  int idx = blockCounts[blockIdx.x] + pre[threadIdx.x];
+  int idx = blockCounts[wg.block()] + pre[threadIdx.x];
   for (int k = 0; k < 4; k++) {
     const long i = base + threadIdx.x * 4 + k;
     if (i >= n) break;
@@ -99,18 +103,18 @@ __global__ void kScanEmit(long n, Flag flag, const int* blockCounts, Emit emit) 
     idx += f[k];
   }
 }
+ESF_BOUNDS(kScanEmit, 1024);
 
 template <class Flag, class Emit>
 esf_status compact(esf_ctx* ctx, long n, Flag flag, Emit emit, int* total) {
   const int nb = std::max(cdiv(n, kScanTile), 1);
-  hipLaunchKernelGGL((kScanCount<Flag>), dim3(nb), dim3(256), 0, ctx->stream, n, flag,
-                     ctx->scanBlocks);
-  hipLaunchKernelGGL(kScanBlocks, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->scanBlocks);
-  hipLaunchKernelGGL((kScanEmit<Flag, Emit>), dim3(nb), dim3(256), 0, ctx->stream, n, flag,
-                     (const int*)ctx->scanBlocks, emit);
+  launch<ESF_KT(kScanCount, Flag)>(ctx, dim3(nb), dim3(256), n, flag, ctx->scanBlocks);
+  launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
+  launch<ESF_KT(kScanEmit, Flag, Emit)>(ctx, dim3(nb), dim3(256), n, flag,
+                                        (const int*)ctx->scanBlocks, emit);
   ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
                          hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   *total = ctx->hostFlags[0];
   return ESF_OK;
 }
@@ -159,9 +163,10 @@ struct SolidPtrs {
 };
 
 // over the nodes of box g ([lo, hi) per axis); outside it both fields are 0
-__global__ void kGatherMass(Dims d, SolidPtrs sp, BoxRange g, double* solidMass,
-                            double* nodeFrac) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kGatherMass(const GridT& wg, Dims d, SolidPtrs sp, BoxRange g,
+                                            double* solidMass, double* nodeFrac) {
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= g.count()) return;
   const long n = g.node(d, t);
   double m = 0;
@@ -182,6 +187,7 @@ __global__ void kGatherMass(Dims d, SolidPtrs sp, BoxRange g, double* solidMass,
   }
   nodeFrac[n] = f * 0.125;
 }
+ESF_BOUNDS(kGatherMass, 1024);
 
 struct SolidFlag {
   Dims d;
@@ -235,18 +241,17 @@ esf_status buildValidList(esf_ctx* ctx, DevSolid& so) {
   ValidFlag fl{ctx->d, so.box, so.valid};
   const long n = ctx->d.cells;
   const int nb = std::max(cdiv(n, kScanTile), 1);
-  hipLaunchKernelGGL((kScanCount<ValidFlag>), dim3(nb), dim3(256), 0, ctx->stream, n, fl,
-                     ctx->scanBlocks);
-  hipLaunchKernelGGL(kScanBlocks, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->scanBlocks);
+  launch<ESF_KT(kScanCount, ValidFlag)>(ctx, dim3(nb), dim3(256), n, fl, ctx->scanBlocks);
+  launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
   ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
                          hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   total = ctx->hostFlags[0];
   esf_status st = growSolidCells(ctx, so, total);
   if (st) return st;
   ValidEmit em{ctx->d, so.cellSlot, so.cellList};
-  hipLaunchKernelGGL((kScanEmit<ValidFlag, ValidEmit>), dim3(nb), dim3(256), 0, ctx->stream, n,
-                     fl, (const int*)ctx->scanBlocks, em);
+  launch<ESF_KT(kScanEmit, ValidFlag, ValidEmit)>(ctx, dim3(nb), dim3(256), n, fl,
+                                                  (const int*)ctx->scanBlocks, em);
   so.nValid = total;
   so.listDirty = false;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
@@ -294,24 +299,23 @@ esf_status gatherSolidMasses(esf_ctx* ctx) {
   const BoxRange gb = spanBoxes(ctx->gatherBox, now, lim);
   for (int k = 0; k < 6; k++) ctx->gatherBox[k] = now[k];
   if (gb.count() > 0)
-    hipLaunchKernelGGL(kGatherMass, dim3(cdiv(gb.count(), 256)), dim3(256), 0, ctx->stream, d,
-                       sp, gb, ctx->solidMass, ctx->nodeFrac);
+    launch<ESF_K(kGatherMass)>(ctx, dim3(cdiv(gb.count(), 256)), dim3(256), d, sp, gb,
+                               ctx->solidMass, ctx->nodeFrac);
   // count solid DOF nodes, grow, then number them
   SolidFlag fl{d, ctx->solidMass};
   const long n = d.nodes;
   const int nb = std::max(cdiv(n, kScanTile), 1);
-  hipLaunchKernelGGL((kScanCount<SolidFlag>), dim3(nb), dim3(256), 0, ctx->stream, n, fl,
-                     ctx->scanBlocks);
-  hipLaunchKernelGGL(kScanBlocks, dim3(1), dim3(1024), 0, ctx->stream, nb, ctx->scanBlocks);
+  launch<ESF_KT(kScanCount, SolidFlag)>(ctx, dim3(nb), dim3(256), n, fl, ctx->scanBlocks);
+  launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
   ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
                          hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   const int total = ctx->hostFlags[0];
   esf_status st = growSolidDofs(ctx, total);
   if (st) return st;
   SolidEmit em{ctx->sidx, ctx->solidNodes};
-  hipLaunchKernelGGL((kScanEmit<SolidFlag, SolidEmit>), dim3(nb), dim3(256), 0, ctx->stream, n,
-                     fl, (const int*)ctx->scanBlocks, em);
+  launch<ESF_KT(kScanEmit, SolidFlag, SolidEmit)>(ctx, dim3(nb), dim3(256), n, fl,
+                                                  (const int*)ctx->scanBlocks, em);
   ctx->nSolid = total;
   ctx->gatheredGen = ctx->massGen;
   ctx->gatheredSolids = (int)ctx->solids.size();
@@ -325,21 +329,27 @@ esf_status gatherSolidMasses(esf_ctx* ctx) {
 // ---------------------------------------------------------------------------
 // computeSolidMass(usesdf = false)  DEFOR_SOLID_3D.cpp:576-669
 // ---------------------------------------------------------------------------
-__global__ void kZero(long n, double* a) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+template <class GridT>
+__device__ __forceinline__ void kZero(const GridT& wg, long n, double* a) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     a[i] = 0.0;
 }
-__global__ void kZero8(long n, int8_t* a) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+ESF_BOUNDS(kZero, 1024);
+template <class GridT>
+__device__ __forceinline__ void kZero8(const GridT& wg, long n, int8_t* a) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     a[i] = 0;
 }
-__global__ void kFillInt(long n, int* a, int v) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+ESF_BOUNDS(kZero8, 1024);
+template <class GridT>
+__device__ __forceinline__ void kFillInt(const GridT& wg, long n, int* a, int v) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     a[i] = v;
 }
+ESF_BOUNDS(kFillInt, 1024);
 
 __device__ inline double det3(const double* m) {
   return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[3] * (m[1] * m[8] - m[2] * m[7]) +
@@ -388,8 +398,10 @@ __device__ inline bool inTight(const Box& b, int x, int y, int z) {
 
 // clear q, _valid and |det F^-1| over the cells of box u (the reference
 // clears the whole fields, :578-581; outside u they are already 0)
-__global__ void kMassClear(Dims d, BoxRange u, double* qfrac, int8_t* valid, double* detFInv) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kMassClear(const GridT& wg, Dims d, BoxRange u, double* qfrac,
+                                           int8_t* valid, double* detFInv) {
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= u.count()) return;
   int x, y, z;
   u.at(t, x, y, z);
@@ -398,6 +410,7 @@ __global__ void kMassClear(Dims d, BoxRange u, double* qfrac, int8_t* valid, dou
   detFInv[index] = 0.0;
   valid[x + (long)y * d.CX + (long)z * d.cslab] = 0;
 }
+ESF_BOUNDS(kMassClear, 1024);
 
 // one wave per tight-box cell: |det F^-1| (lane 0), and the cell's
 // msr^3 subsamples spread over the lanes (sample s = (p msr + q) msr + r in
@@ -406,8 +419,9 @@ __global__ void kMassClear(Dims d, BoxRange u, double* qfrac, int8_t* valid, dou
 // reference adds `fraction` to the octant's q once per such sample, so q is
 // that many additions of one constant, which depends on the count only:
 // lane 0 performs them, then the sum, the 0.5 threshold and the flags.
-__global__ void __launch_bounds__(256) kMassSamples(Dims d, MassArgs a, BoxRange t) {
-  const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6;
+template <class GridT>
+__device__ __forceinline__ void kMassSamples(const GridT& wg, Dims d, MassArgs a, BoxRange t) {
+  const long wave = (wg.block() * (long)blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= t.count()) return;  // uniform per wave
   int x, y, z;
@@ -460,11 +474,13 @@ __global__ void __launch_bounds__(256) kMassSamples(Dims d, MassArgs a, BoxRange
   for (int i = 0; i < 8; i++) a.qfrac[8 * index + i] = qv;
   a.valid[c] = v;
 }
+ESF_BOUNDS(kMassSamples, 256);
 
 // mass per node of box u: corners i = 0..7 in the reference's outer-loop
 // order
-__global__ void kMassGather(Dims d, MassArgs a, BoxRange u) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kMassGather(const GridT& wg, Dims d, MassArgs a, BoxRange u) {
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= u.count()) return;
   int x, y, z;
   u.at(t, x, y, z);
@@ -481,6 +497,7 @@ __global__ void kMassGather(Dims d, MassArgs a, BoxRange u) {
   }
   a.mass[n] = m;
 }
+ESF_BOUNDS(kMassGather, 1024);
 
 esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   const Dims& d = ctx->d;
@@ -495,8 +512,8 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   const BoxRange T = spanBoxes(tc, none, climit);
   const BoxRange U = spanBoxes(so.dirtyAll ? fullCells : so.dirty, tc, climit);
   if (U.count() > 0)
-    hipLaunchKernelGGL(kMassClear, dim3(cdiv(U.count(), 256)), dim3(256), 0, ctx->stream, d, U,
-                       so.qfrac, so.valid, so.detFInv);
+    launch<ESF_K(kMassClear)>(ctx, dim3(cdiv(U.count(), 256)), dim3(256), d, U, so.qfrac, so.valid,
+                              so.detFInv);
   MassArgs a{};
   a.box = so.box;
   a.u = so.u;
@@ -515,8 +532,7 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   a.mass = so.mass;
   a.phi = usesdf ? so.phi : nullptr;
   if (T.count() > 0)
-    hipLaunchKernelGGL(kMassSamples, dim3(cdiv(64 * T.count(), 256)), dim3(256), 0, ctx->stream,
-                       d, a, T);
+    launch<ESF_K(kMassSamples)>(ctx, dim3(cdiv(64 * T.count(), 256)), dim3(256), d, a, T);
   // the masses of U's nodes (a cell's corners reach one node past it); the
   // other nodes keep 0 (or a mass that was uploaded: then U is the grid)
   const int nlimit[3] = {d.X, d.Y, d.Z};
@@ -524,8 +540,7 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   const BoxRange UN = spanBoxes(so.dirtyAll ? fullCells : un, none, nlimit);
   const BoxRange M = so.dirtyAll ? BoxRange{{0, 0, 0}, {d.X, d.Y, d.Z}} : UN;
   if (M.count() > 0)
-    hipLaunchKernelGGL(kMassGather, dim3(cdiv(M.count(), 256)), dim3(256), 0, ctx->stream, d, a,
-                       M);
+    launch<ESF_K(kMassGather)>(ctx, dim3(cdiv(M.count(), 256)), dim3(256), d, a, M);
   for (int k = 0; k < 3; k++) {
     so.dirty[k] = T.lo[k];
     so.dirty[3 + k] = T.hi[k];
@@ -538,11 +553,12 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
 
 // bounding box of nodes with mass > 0 (computeSDF :227-241): per-block
 // min/max of x, y, z into partials, finished on the host
-__global__ void kMassBox(Dims d, const double* mass, int* part) {
+template <class GridT>
+__device__ __forceinline__ void kMassBox(const GridT& wg, Dims d, const double* mass, int* part) {
   __shared__ int s[6][256];
   int mn[3] = {1 << 30, 1 << 30, 1 << 30}, mx[3] = {-1, -1, -1};
-  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < d.nodes;
-       n += (long)gridDim.x * blockDim.x) {
+  for (long n = wg.block() * (long)blockDim.x + threadIdx.x; n < d.nodes;
+       n += (long)wg.blocks() * blockDim.x) {
     if (mass[n] > 0) {
       const int c[3] = {(int)(n % d.X), (int)((n / d.X) % d.Y), (int)(n / d.slab)};
       for (int k = 0; k < 3; k++) {
@@ -565,18 +581,18 @@ __global__ void kMassBox(Dims d, const double* mass, int* part) {
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = s[k][0];
+    for (int k = 0; k < 6; k++) part[6 * wg.block() + k] = s[k][0];
 }
+ESF_BOUNDS(kMassBox, 1024);
 
 esf_status updateBoxes(esf_ctx* ctx, int solid) {
   DevSolid& so = ctx->solids[solid];
   const int nb = 256;
-  hipLaunchKernelGGL(kMassBox, dim3(nb), dim3(256), 0, ctx->stream, ctx->d, so.mass,
-                     (int*)ctx->partials);
+  launch<ESF_K(kMassBox)>(ctx, dim3(nb), dim3(256), ctx->d, so.mass, (int*)ctx->partials);
   std::vector<int> h(6 * nb);
   ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, h.size() * sizeof(int), hipMemcpyDeviceToHost,
                          ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   const Dims& d = ctx->d;
   int mn[3] = {d.X, d.Y, d.Z}, mx[3] = {0, 0, 0};
   for (int b = 0; b < nb; b++)
@@ -619,9 +635,11 @@ esf_status updateBoxes(esf_ctx* ctx, int solid) {
 constexpr int kPhiRelaxations = 8;
 
 // phi = -0.5 on nodes with mass inside the box, large everywhere else
-__global__ void kPhiInit(Dims d, Box bx, const double* __restrict__ mass, double large,
-                         double* phi, double* phi2) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kPhiInit(const GridT& wg, Dims d, Box bx,
+                                         const double* __restrict__ mass, double large, double* phi,
+                                         double* phi2) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   const bool in = x >= bx.bMin[0] && x < bx.bMax[0] && y >= bx.bMin[1] && y < bx.bMax[1] &&
@@ -630,14 +648,17 @@ __global__ void kPhiInit(Dims d, Box bx, const double* __restrict__ mass, double
   phi[n] = v;
   phi2[n] = v;
 }
+ESF_BOUNDS(kPhiInit, 1024);
 
 // one relaxation over the box: nodes without mass take the upwind update
 // of their neighbours' previous values
-__global__ void kPhiRelax(Dims d, Box bx, const double* __restrict__ mass,
-                          const double* __restrict__ in, double* out, double large) {
+template <class GridT>
+__device__ __forceinline__ void kPhiRelax(const GridT& wg, Dims d, Box bx,
+                                          const double* __restrict__ mass,
+                                          const double* __restrict__ in, double* out, double large) {
   const int nx = bx.bMax[0] - bx.bMin[0], ny = bx.bMax[1] - bx.bMin[1];
   const long nb = (long)nx * ny * (bx.bMax[2] - bx.bMin[2]);
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= nb) return;
   const int x = bx.bMin[0] + (int)(t % nx), y = bx.bMin[1] + (int)((t / nx) % ny),
             z = bx.bMin[2] + (int)(t / ((long)nx * ny));
@@ -654,14 +675,15 @@ __global__ void kPhiRelax(Dims d, Box bx, const double* __restrict__ mass,
   const double r = fmin(at(x, y, z - 1), at(x, y, z + 1));
   out[n] = solveDistance(p, q, r, in[n]);
 }
+ESF_BOUNDS(kPhiRelax, 1024);
 
 esf_status solidPhi(esf_ctx* ctx, int solid) {
   const Dims& d = ctx->d;
   DevSolid& so = ctx->solids[solid];
   const Box& bx = so.box;
   const double large = (double)(d.X + d.Y + d.Z + 2);
-  hipLaunchKernelGGL(kPhiInit, dim3(cdiv(d.nodes, 256)), dim3(256), 0, ctx->stream, d, bx,
-                     (const double*)so.mass, large, so.phi, so.phi2);
+  launch<ESF_K(kPhiInit)>(ctx, dim3(cdiv(d.nodes, 256)), dim3(256), d, bx, (const double*)so.mass,
+                          large, so.phi, so.phi2);
   if (ctx->p.flip) {
     // the FLIP round trip extends the velocity over the whole box by phi:
     // the reference's two rounds of ordered sweeps, bitwise
@@ -675,8 +697,8 @@ esf_status solidPhi(esf_ctx* ctx, int solid) {
     for (int k = 0; k < kPhiRelaxations; k++) {
       const double* src = (k & 1) ? so.phi2 : so.phi;
       double* dst = (k & 1) ? so.phi : so.phi2;
-      hipLaunchKernelGGL(kPhiRelax, dim3(cdiv(nb, 256)), dim3(256), 0, ctx->stream, d, bx,
-                         (const double*)so.mass, src, dst, large);
+      launch<ESF_K(kPhiRelax)>(ctx, dim3(cdiv(nb, 256)), dim3(256), d, bx, (const double*)so.mass,
+                               src, dst, large);
     }
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
@@ -686,11 +708,12 @@ esf_status solidPhi(esf_ctx* ctx, int solid) {
 // CFL  COUPLED_INTEGRATOR_3D.cpp:88-102 (solid DOF nodes are exactly the
 // interior nodes with solid mass)
 // ---------------------------------------------------------------------------
-__global__ void kCfl(int nSolid, const int* solidNodes, const double* vel, double dhInv,
-                     double alpha, double* part) {
+template <class GridT>
+__device__ __forceinline__ void kCfl(const GridT& wg, int nSolid, const int* solidNodes,
+                                     const double* vel, double dhInv, double alpha, double* part) {
   __shared__ double s[256];
   double dt = 1e9;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nSolid; i += gridDim.x * blockDim.x) {
+  for (int i = wg.block() * blockDim.x + threadIdx.x; i < nSolid; i += wg.blocks() * blockDim.x) {
     const long n = solidNodes[i];
     const double m = fmax(fmax(fabs(vel[3 * n]), fabs(vel[3 * n + 1])), fabs(vel[3 * n + 2]));
     const double vmax = m * dhInv;
@@ -702,8 +725,9 @@ __global__ void kCfl(int nSolid, const int* solidNodes, const double* vel, doubl
     if (threadIdx.x < st) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + st]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) part[wg.block()] = s[0];
 }
+ESF_BOUNDS(kCfl, 1024);
 
 // the CFL kernel and the copy of its block minima to pinned memory, queued;
 // cflFinish waits for that copy only, so work queued in between (the stress
@@ -713,8 +737,8 @@ static int cflBlocks(const esf_ctx* ctx) { return std::min(cdiv(ctx->nSolid, 256
 esf_status cflEnqueue(esf_ctx* ctx, double alpha) {
   if (ctx->nSolid == 0) return ESF_OK;
   const int nb = cflBlocks(ctx);
-  hipLaunchKernelGGL(kCfl, dim3(nb), dim3(256), 0, ctx->stream, ctx->nSolid, ctx->solidNodes,
-                     ctx->vel, ctx->d.dhInv, alpha, ctx->partials);
+  launch<ESF_K(kCfl)>(ctx, dim3(nb), dim3(256), ctx->nSolid, ctx->solidNodes, ctx->vel,
+                      ctx->d.dhInv, alpha, ctx->partials);
   ESF_HIP(hipMemcpyAsync(ctx->hostCfl, ctx->partials, nb * sizeof(double),
                          hipMemcpyDeviceToHost, ctx->stream));
   ESF_HIP(hipEventRecord(ctx->evCfl, ctx->stream));
@@ -724,7 +748,7 @@ esf_status cflEnqueue(esf_ctx* ctx, double alpha) {
 esf_status cflFinish(esf_ctx* ctx, double* dtOut) {
   double dt = 1e9;
   if (ctx->nSolid > 0) {
-    ESF_HIP(hipEventSynchronize(ctx->evCfl));
+    ESF_HIP(eventSync(ctx, ctx->evCfl));
     const int nb = cflBlocks(ctx);
     for (int i = 0; i < nb; i++) dt = std::min(dt, ctx->hostCfl[i]);
   }
@@ -743,9 +767,11 @@ esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut) {
 // vorticity  COUPLED_INTEGRATOR_3D.cpp:450-497; _workspace cleared first
 // (:443) and written with |omega| where the node is not fully solid
 // ---------------------------------------------------------------------------
-__global__ void kVorticity(Dims d, const double* __restrict__ v, const double* __restrict__ nf,
-                           double* vort, double* ws, int enabled) {
-  const long index = xcdBlock() * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kVorticity(const GridT& wg, Dims d, const double* __restrict__ v,
+                                           const double* __restrict__ nf, double* vort, double* ws,
+                                           int enabled) {
+  const long index = wg.block() * (long)blockDim.x + threadIdx.x;
   if (index >= d.nodes) return;
   const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
   double out = 0.0;
@@ -771,6 +797,7 @@ __global__ void kVorticity(Dims d, const double* __restrict__ v, const double* _
   }
   ws[index] = out;
 }
+ESF_BOUNDS(kVorticity, 1024);
 
 // body forces and mass vectors  COUPLED_INTEGRATOR_3D.cpp:505-606, with the
 // fluid part of pcrImplicitSolve folded in: v0 = (dt f + m_f v) / m
@@ -817,9 +844,10 @@ __device__ inline void solidBodyForces(const Dims& d, const BodyArgs& a, long in
   a.sForce[k + 1] += a.buoyancy * fluidMass * a.intensity[index];
 }
 
-__global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
+template <class GridT>
+__device__ __forceinline__ void kBodyForces(const GridT& wg, Dims d, BodyArgs a) {
   __shared__ double red[16];
-  const long index = xcdBlock() * (long)blockDim.x + threadIdx.x;
+  const long index = wg.block() * (long)blockDim.x + threadIdx.x;
   double en = 0;
   if (index < d.nodes) {
     const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
@@ -879,12 +907,15 @@ __global__ void __launch_bounds__(256) kBodyForces(Dims d, BodyArgs a) {
     }
   }
   const double t = blockSum(en, red);
-  if (threadIdx.x == 0) a.energyPart[xcdBlock()] = t;
+  if (threadIdx.x == 0) a.energyPart[wg.block()] = t;
 }
+ESF_BOUNDS(kBodyForces, 256);
 
 // the solid nodes' part of kBodyForces, one thread per solid DOF node
-__global__ void kSolidBodyForces(Dims d, BodyArgs a, int nSolid, const int* solidNodes) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kSolidBodyForces(const GridT& wg, Dims d, BodyArgs a, int nSolid,
+                                                 const int* solidNodes) {
+  const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long index = solidNodes[s];
   const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
@@ -892,29 +923,39 @@ __global__ void kSolidBodyForces(Dims d, BodyArgs a, int nSolid, const int* soli
   const int solidIdx = a.sidx[index];
   if (solidIdx >= 0) solidBodyForces(d, a, index, solidIdx);
 }
+ESF_BOUNDS(kSolidBodyForces, 1024);
 
 // solid right-hand side dt F + M v  (COUPLED_INTEGRATOR_3D.cpp:664-665)
-__global__ void kSolidRhs(long n, const double* f, const double* mv, double dt, double* rhs) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kSolidRhs(const GridT& wg, long n, const double* f, const double* mv,
+                                          double dt, double* rhs) {
+  const long i = wg.block() * (long)blockDim.x + threadIdx.x;
   if (i < n) rhs[i] = f[i] * dt + mv[i];
 }
+ESF_BOUNDS(kSolidRhs, 1024);
 
 // v0 at solid nodes from the first solid solve (:690-691)
-__global__ void kScatterSolid(int nSolid, const int* solidNodes, const double* sv, double* v0) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kScatterSolid(const GridT& wg, int nSolid, const int* solidNodes,
+                                              const double* sv, double* v0) {
+  const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
   for (int c = 0; c < 3; c++) v0[3 * n + c] = sv[3 * (long)s + c];
 }
+ESF_BOUNDS(kScatterSolid, 1024);
 
 // g = -J^T lambda per interior node (cells in ascending order, the column
 // order of Eigen's transposed product, :719); fluid nodes get
 // v1 = v0 + g / m (:745-747, :753-763); solid nodes hand g to the second
 // solid solve (:724-729)
-__global__ void kCorrect(Dims d, const double* __restrict__ lam, const double* __restrict__ v0,
-                         const double* __restrict__ minv, const int* __restrict__ sidx,
-                         double* vel, double* sB, double* dvPrev) {
-  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kCorrect(const GridT& wg, Dims d, const double* __restrict__ lam,
+                                         const double* __restrict__ v0,
+                                         const double* __restrict__ minv,
+                                         const int* __restrict__ sidx, double* vel, double* sB,
+                                         double* dvPrev) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int i = n % d.X, j = (n / d.X) % d.Y, k = n / d.slab;
   if (!isInterior(d, i, j, k)) return;
@@ -944,17 +985,21 @@ __global__ void kCorrect(Dims d, const double* __restrict__ lam, const double* _
     }
   }
 }
+ESF_BOUNDS(kCorrect, 1024);
 
 // v1 at solid nodes = v0 + delta_s  (:748-753)
-__global__ void kSolidVelocity(int nSolid, const int* solidNodes, const double* v0,
-                               const double* dv, double* vel, double* dvPrev) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kSolidVelocity(const GridT& wg, int nSolid, const int* solidNodes,
+                                               const double* v0, const double* dv, double* vel,
+                                               double* dvPrev) {
+  const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
   for (int c = 0; c < 3; c++) vel[3 * n + c] = v0[3 * n + c] + dv[3 * (long)s + c];
   if (dvPrev)
     for (int c = 0; c < 3; c++) dvPrev[3 * n + c] = dv[3 * (long)s + c];
 }
+ESF_BOUNDS(kSolidVelocity, 1024);
 
 // ---------------------------------------------------------------------------
 // advection
@@ -973,8 +1018,7 @@ constexpr int kSLHX = kSLX + 2, kSLHY = kSLY + 2;
 constexpr int kSLLayer = kSLHX * kSLHY;  // halo nodes per layer
 static_assert(kSLX * kSLY == 256, "one node per thread and layer");
 
-__device__ inline void tileOf(int tilesX, int tilesY, int& x0, int& y0, int& z0) {
-  const int b = xcdBlock();
+__device__ inline void tileOf(int b, int tilesX, int tilesY, int& x0, int& y0, int& z0) {
   const int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
   x0 = tx * kSLX;
   y0 = ty * kSLY;
@@ -982,10 +1026,12 @@ __device__ inline void tileOf(int tilesX, int tilesY, int& x0, int& y0, int& z0)
 }
 
 // advectFluidVelocityField  COUPLED_INTEGRATOR_3D.cpp:797-826
-__global__ void __launch_bounds__(256) kAdvectVelocity(Dims d, const double* __restrict__ vel,
-                                                       const double* __restrict__ solidMass,
-                                                       double dt, double* velws) {
-  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kAdvectVelocity(const GridT& wg, Dims d,
+                                                const double* __restrict__ vel,
+                                                const double* __restrict__ solidMass, double dt,
+                                                double* velws) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   if (!isInterior(d, x, y, z) || solidMass[n] > 0) return;
@@ -1004,14 +1050,16 @@ __global__ void __launch_bounds__(256) kAdvectVelocity(Dims d, const double* __r
   }
   interpV(vel, g, np, &velws[3 * n]);
 }
+ESF_BOUNDS(kAdvectVelocity, 256);
 
 // advectField(_intensity)  COUPLED_INTEGRATOR_3D.cpp:766-796 (workspace
 // cleared first, so skipped nodes receive 0)
-__global__ void __launch_bounds__(256) kAdvectScalar(Dims d, const double* __restrict__ vel,
-                                                     const double* __restrict__ solidMass,
-                                                     const double* __restrict__ field, double dt,
-                                                     double* out) {
-  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kAdvectScalar(const GridT& wg, Dims d, const double* __restrict__ vel,
+                                              const double* __restrict__ solidMass,
+                                              const double* __restrict__ field, double dt,
+                                              double* out) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   if (solidMass[n] > 0) {
     out[n] = 0.0;
@@ -1032,19 +1080,20 @@ __global__ void __launch_bounds__(256) kAdvectScalar(Dims d, const double* __res
   }
   out[n] = interpS(field, g, np);
 }
+ESF_BOUNDS(kAdvectScalar, 256);
 
 // DEFOR_SOLID_3D::finalizeSolution()  DEFOR_SOLID_3D.cpp:1198-1233.
 // u += v dt over all nodes and the advection of u (:1108-1123) fuse: w =
 // u + v dt is formed where the interpolation reads it, rounding as the
 // separate pass would.
 // Tiled as described above; the window holds w.
-__global__ void __launch_bounds__(256) kSolidAdvectU(Dims d, const double* __restrict__ u,
-                                                     const double* __restrict__ sv, double dt,
-                                                     double* __restrict__ unew, int tilesX,
-                                                     int tilesY) {
+template <class GridT>
+__device__ __forceinline__ void kSolidAdvectU(const GridT& wg, Dims d, const double* __restrict__ u,
+                                              const double* __restrict__ sv, double dt,
+                                              double* __restrict__ unew, int tilesX, int tilesY) {
   __shared__ double w[3][3 * kSLLayer];
   int x0, y0, z0;
-  tileOf(tilesX, tilesY, x0, y0, z0);
+  tileOf(wg.block(), tilesX, tilesY, x0, y0, z0);
   const int zEnd = min(z0 + kSLZ, d.Z);
   // layer z of w into ring slot (z + 3) mod 3 (zeros outside the grid:
   // never read, the interpolation corners stay inside)
@@ -1102,11 +1151,13 @@ __global__ void __launch_bounds__(256) kSolidAdvectU(Dims d, const double* __res
     __syncthreads();
   }
 }
+ESF_BOUNDS(kSolidAdvectU, 256);
 
 // face extrapolation of u on the extended box (:1205-1220); axis 2 (z
 // faces) first, then y, then x, each face pair per thread in the
 // reference's statement order
-__global__ void kSolidExtrap(Dims d, Box bx, int axis, double* u) {
+template <class GridT>
+__device__ __forceinline__ void kSolidExtrap(const GridT& wg, Dims d, Box bx, int axis, double* u) {
   const int a0 = axis == 2 ? 0 : 0;  // first free axis
   int fa = 0, fb = 1;
   if (axis == 2) {
@@ -1121,7 +1172,7 @@ __global__ void kSolidExtrap(Dims d, Box bx, int axis, double* u) {
   }
   (void)a0;
   const int na = bx.bMax[fa] - bx.bMin[fa], nbb = bx.bMax[fb] - bx.bMin[fb];
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= (long)na * nbb) return;
   int c[3];
   c[fa] = bx.bMin[fa] + (int)(t % na);
@@ -1140,6 +1191,7 @@ __global__ void kSolidExtrap(Dims d, Box bx, int axis, double* u) {
     at(hi - 1)[k] = 2 * a1 - a2;
   }
 }
+ESF_BOUNDS(kSolidExtrap, 1024);
 
 // X = x - u (:1222-1230) and _vecWorkspace = _velocity (:1232), over the
 // 3 N doubles of the AoS fields as flat pairs (16-byte accesses, every
@@ -1154,12 +1206,12 @@ __device__ inline double recoverX(const Dims& d, long i, double u) {
   return pos + (o - u);
 }
 
-__global__ void __launch_bounds__(256) kSolidRecoverX(Dims d, const double* __restrict__ u,
-                                                      const double* __restrict__ sv,
-                                                      double* __restrict__ X,
-                                                      double* __restrict__ svws) {
+template <class GridT>
+__device__ __forceinline__ void kSolidRecoverX(const GridT& wg, Dims d, const double* __restrict__ u,
+                                               const double* __restrict__ sv,
+                                               double* __restrict__ X, double* __restrict__ svws) {
   const long n3 = 3 * d.nodes;
-  const long i = 2 * (blockIdx.x * (long)blockDim.x + threadIdx.x);
+  const long i = 2 * (wg.block() * (long)blockDim.x + threadIdx.x);
   if (i + 1 < n3) {
     const double2 uu = *reinterpret_cast<const double2*>(u + i);
     const double2 vv = *reinterpret_cast<const double2*>(sv + i);
@@ -1173,12 +1225,15 @@ __global__ void __launch_bounds__(256) kSolidRecoverX(Dims d, const double* __re
     svws[i] = sv[i];
   }
 }
+ESF_BOUNDS(kSolidRecoverX, 256);
 
 // DEFOR_SOLID_3D::finalizeSolution(velocity, dt)  (:1182-1197): copy the
 // grid velocity into the solid's extended box (nodes outside keep their
 // old value: VEC3F_FIELD_3D::clear() is a no-op, VEC3F_FIELD_3D.h:177)
-__global__ void kSolidCopy(Dims d, Box bx, const double* vel, double* sv) {
-  const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kSolidCopy(const GridT& wg, Dims d, Box bx, const double* vel,
+                                           double* sv) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   if (x < bx.bMin[0] || x >= bx.bMax[0] || y < bx.bMin[1] || y >= bx.bMax[1] ||
@@ -1186,6 +1241,7 @@ __global__ void kSolidCopy(Dims d, Box bx, const double* vel, double* sv) {
     return;
   for (int c = 0; c < 3; c++) sv[3 * n + c] = vel[3 * n + c];
 }
+ESF_BOUNDS(kSolidCopy, 1024);
 
 // recoverVelocityField for every solid (DEFOR_SOLID_3D.cpp:1235-1241) then
 // the Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:900-916)
@@ -1195,8 +1251,10 @@ struct RecoverArgs {
   const double* mass[kMaxSolids];
   const double* vel[kMaxSolids];
 };
-__global__ void kRecoverBC(Dims d, RecoverArgs a, double* vel, double* intensity, int bc) {
-  const long n = xcdBlock() * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kRecoverBC(const GridT& wg, Dims d, RecoverArgs a, double* vel,
+                                           double* intensity, int bc) {
+  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   for (int b = 0; b < a.n; b++) {
@@ -1210,13 +1268,13 @@ __global__ void kRecoverBC(Dims d, RecoverArgs a, double* vel, double* intensity
     for (int c = 0; c < 3; c++) vel[3 * n + c] = 0;
   if (y == 0) intensity[n] = 0;
 }
+ESF_BOUNDS(kRecoverBC, 1024);
 
 // ---------------------------------------------------------------------------
 // host wrappers
 // ---------------------------------------------------------------------------
-#define LAUNCH_NODES(kernel, ...)                                                            \
-  hipLaunchKernelGGL(kernel, dim3(cdiv(ctx->d.nodes, 256)), dim3(256), 0, ctx->stream, \
-                     __VA_ARGS__)
+#define LAUNCH_NODES(kernel, ...) \
+  launch<ESF_K(kernel)>(ctx, dim3(cdiv(ctx->d.nodes, 256)), dim3(256), __VA_ARGS__)
 
 static BodyArgs bodyArgs(esf_ctx* ctx) {
   BodyArgs a{};
@@ -1254,7 +1312,7 @@ esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy) {
     std::vector<double> h(nb);
     ESF_HIP(hipMemcpyAsync(h.data(), ctx->energyPart, nb * sizeof(double),
                            hipMemcpyDeviceToHost, ctx->stream));
-    ESF_HIP(hipStreamSynchronize(ctx->stream));
+    ESF_HIP(streamSync(ctx));
     double e = 0;
     for (double v : h) e += v;
     *energy += e;
@@ -1279,15 +1337,16 @@ esf_status forkBodyForces(esf_ctx* ctx) {
   }
   BodyArgs a = bodyArgs(ctx);
   if (ctx->nSolid)
-    hipLaunchKernelGGL(kSolidBodyForces, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
-                       d, a, ctx->nSolid, (const int*)ctx->solidNodes);
+    launch<ESF_K(kSolidBodyForces)>(ctx, dim3(cdiv(ctx->nSolid, 256)), dim3(256), d, a, ctx->nSolid,
+                                    (const int*)ctx->solidNodes);
   ESF_HIP(hipEventRecord(ctx->evFork, ctx->stream));
   ESF_HIP(hipStreamWaitEvent(ctx->side, ctx->evFork, 0));
   const dim3 grid(cdiv(d.nodes, 256));
-  hipLaunchKernelGGL(kVorticity, grid, dim3(256), 0, ctx->side, d, (const double*)ctx->vel,
-                     (const double*)ctx->nodeFrac, ctx->vort, ctx->ws, ctx->vortEps > 0 ? 1 : 0);
+  launchOn<ESF_K(kVorticity)>(ctx, ctx->side, grid, dim3(256), d, (const double*)ctx->vel,
+                              (const double*)ctx->nodeFrac, ctx->vort, ctx->ws,
+                              ctx->vortEps > 0 ? 1 : 0);
   a.skipSolid = 1;
-  hipLaunchKernelGGL(kBodyForces, grid, dim3(256), 0, ctx->side, d, a);
+  launchOn<ESF_K(kBodyForces)>(ctx, ctx->side, grid, dim3(256), d, a);
   ESF_HIP(hipEventRecord(ctx->evJoin, ctx->side));
   ctx->joinPending = true;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
@@ -1303,15 +1362,15 @@ esf_status joinBodyForces(esf_ctx* ctx) {
 esf_status solidRhs(esf_ctx* ctx) {
   const long n = 3L * ctx->nSolid;
   if (n)
-    hipLaunchKernelGGL(kSolidRhs, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, n, ctx->sForce,
-                       ctx->sMv, ctx->dt, ctx->sRhs);
+    launch<ESF_K(kSolidRhs)>(ctx, dim3(cdiv(n, 256)), dim3(256), n, ctx->sForce, ctx->sMv, ctx->dt,
+                             ctx->sRhs);
   return ESF_OK;
 }
 
 esf_status scatterSolidV0(esf_ctx* ctx) {
   if (ctx->nSolid)
-    hipLaunchKernelGGL(kScatterSolid, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
-                       ctx->nSolid, ctx->solidNodes, ctx->sV0, ctx->v0);
+    launch<ESF_K(kScatterSolid)>(ctx, dim3(cdiv(ctx->nSolid, 256)), dim3(256), ctx->nSolid,
+                                 ctx->solidNodes, ctx->sV0, ctx->v0);
   return ESF_OK;
 }
 
@@ -1323,8 +1382,8 @@ esf_status correctVelocity(esf_ctx* ctx) {
 
 esf_status solidVelocity(esf_ctx* ctx) {
   if (ctx->nSolid)
-    hipLaunchKernelGGL(kSolidVelocity, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
-                       ctx->nSolid, ctx->solidNodes, ctx->v0, ctx->sDv, ctx->vel, ctx->dvPrev);
+    launch<ESF_K(kSolidVelocity)>(ctx, dim3(cdiv(ctx->nSolid, 256)), dim3(256), ctx->nSolid,
+                                  ctx->solidNodes, ctx->v0, ctx->sDv, ctx->vel, ctx->dvPrev);
   return ESF_OK;
 }
 
@@ -1349,8 +1408,8 @@ esf_status solidCopyVelocity(esf_ctx* ctx) {
 esf_status solidFinalize(esf_ctx* ctx, double dt) {
   for (auto& so : ctx->solids) {
     const int tX = cdiv(ctx->d.X, kSLX), tY = cdiv(ctx->d.Y, kSLY), tZ = cdiv(ctx->d.Z, kSLZ);
-    hipLaunchKernelGGL(kSolidAdvectU, dim3(tX * tY * tZ), dim3(256), 0, ctx->stream, ctx->d,
-                       (const double*)so.u, (const double*)so.vel, dt, so.unew, tX, tY);
+    launch<ESF_K(kSolidAdvectU)>(ctx, dim3(tX * tY * tZ), dim3(256), ctx->d, (const double*)so.u,
+                                 (const double*)so.vel, dt, so.unew, tX, tY);
     std::swap(so.u, so.unew);
     const Box& b = so.box;
     const int axes[3] = {2, 1, 0};
@@ -1358,11 +1417,10 @@ esf_status solidFinalize(esf_ctx* ctx, double dt) {
       int fa = a == 0 ? 1 : 0, fb = a == 2 ? 1 : 2;
       const long cnt = (long)(b.bMax[fa] - b.bMin[fa]) * (b.bMax[fb] - b.bMin[fb]);
       if (cnt > 0 && b.bMax[a] - b.bMin[a] >= 3)
-        hipLaunchKernelGGL(kSolidExtrap, dim3(cdiv(cnt, 256)), dim3(256), 0, ctx->stream, ctx->d,
-                           b, a, so.u);
+        launch<ESF_K(kSolidExtrap)>(ctx, dim3(cdiv(cnt, 256)), dim3(256), ctx->d, b, a, so.u);
     }
-    hipLaunchKernelGGL(kSolidRecoverX, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)), dim3(256), 0,
-                       ctx->stream, ctx->d, so.u, so.vel, so.X, so.velws);
+    launch<ESF_K(kSolidRecoverX)>(ctx, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)), dim3(256),
+                                  ctx->d, so.u, so.vel, so.X, so.velws);
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -1384,11 +1442,11 @@ esf_status recoverAndBC(esf_ctx* ctx, int bc) {
 }
 
 esf_status fillInt(esf_ctx* ctx, long n, int* p, int v) {
-  hipLaunchKernelGGL(kFillInt, dim3(1024), dim3(256), 0, ctx->stream, n, p, v);
+  launch<ESF_K(kFillInt)>(ctx, dim3(1024), dim3(256), n, p, v);
   return ESF_OK;
 }
 esf_status zeroD(esf_ctx* ctx, long n, double* p) {
-  hipLaunchKernelGGL(kZero, dim3(1024), dim3(256), 0, ctx->stream, n, p);
+  launch<ESF_K(kZero)>(ctx, dim3(1024), dim3(256), n, p);
   return ESF_OK;
 }
 

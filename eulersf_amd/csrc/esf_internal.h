@@ -103,6 +103,8 @@ struct DevSolid {
   int *pBin = nullptr, *binItems = nullptr;
 };
 
+struct SceneFiber;  // esf_batch.hip: a scene's step inside esf_step_batch
+
 }  // namespace esf
 
 struct esf_ctx {
@@ -213,6 +215,9 @@ struct esf_ctx {
   size_t timedUsed = 0;
   double kernelMs[ESF_KERNEL_CLASSES] = {0};
   long kernelCount[ESF_KERNEL_CLASSES] = {0};
+  // set while the context's step runs inside esf_step_batch: launches go to
+  // the batch scheduler, waits let the other scenes go on
+  esf::SceneFiber* fiber = nullptr;
 };
 
 namespace esf {
@@ -327,11 +332,6 @@ __device__ inline void blockPartialsAt(const double (&mine)[K], double* partials
     partials[(size_t)threadIdx.x * nb + bid] = r;
   }
 }
-template <int K>
-__device__ inline void blockPartials(const double (&mine)[K], double* partials, double* lds) {
-  blockPartialsAt<K>(mine, partials, lds, gridDim.x * gridDim.y * gridDim.z,
-                     blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-}
 
 // in the finisher (one block, up to 1024 threads): the K totals of the
 // producer's nb blocks, valid in thread 0.  Every load of a round of 8
@@ -417,27 +417,6 @@ inline void solveReport(const CrScalars& h, int maxIt, int* iters, bool* converg
 // 12/13 unused, solid cell pass 14, solid node pass 15, the same two
 // carrying the true-residual check 16 / 17
 //
-// launchTimed: when kernel timing is on and cls >= 0, the launch carries a
-// start/stop event pair in its own dispatch (hipExtLaunchKernelGGL), so the
-// pair brackets the kernel's execution alone -- no separate event packets
-// around it that would add their own latency to the measurement.
-template <typename F, typename... Args>
-inline void launchTimed(esf_ctx* ctx, int cls, F kernel, dim3 grid, dim3 block, Args... args) {
-  if (cls < 0 || !ctx->kernelTiming) {
-    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
-    return;
-  }
-  if (ctx->timedUsed == ctx->timed.size()) {
-    esf_ctx::TimedLaunch t{};
-    hipEventCreate(&t.a);
-    hipEventCreate(&t.b);
-    ctx->timed.push_back(t);
-  }
-  esf_ctx::TimedLaunch* t = &ctx->timed[ctx->timedUsed++];
-  t->cls = cls;
-  hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t->a, t->b, 0, args...);
-}
-
 // esf_api.hip: a device buffer of `count` elements held by the context
 // (replacing the one at *p, which is freed); zero-filled
 esf_status devAllocBytes(esf_ctx* ctx, void** p, size_t bytes);
@@ -467,3 +446,5 @@ esf_status advectMeshes(esf_ctx* ctx);
 esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<int>& pairs);
 
 }  // namespace esf
+
+#include "esf_launch.h"

@@ -25,9 +25,9 @@ enum FinishOp {
 
 // hostOut (pinned host memory, may be null): the scalar block after the
 // update, written by the kernel itself, so a host poll needs no copy
-template <int OP, int K>
-__global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, CrScalars* sc,
-                                                int iter, int first, CrScalars* hostOut) {
+template <int OP, int K, class GridT>
+__device__ __forceinline__ void kFinish(const GridT& wg, int nb, const double* partials,
+                                        CrScalars* sc, int iter, int first, CrScalars* hostOut) {
   // the sums' loads go out together with the done flag's (the partials
   // of a finished solve's early-exit launches are never used)
   const bool done = sc->done != 0;
@@ -92,37 +92,42 @@ __global__ void __launch_bounds__(1024) kFinish(int nb, const double* partials, 
   }
   if (hostOut) *hostOut = *sc;
 }
+ESF_BOUNDS(kFinish, 1024);
 
 // partials: the producer's block partials (default: the context's buffer);
 // hostOut: pinned slot that receives the updated scalar block
 template <int OP, int K>
 inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int first = 0,
                          const double* partials = nullptr, CrScalars* hostOut = nullptr) {
-  hipLaunchKernelGGL((kFinish<OP, K>), dim3(1), dim3(1024), 0, ctx->stream, nb,
-                     partials ? partials : (const double*)ctx->partials, sc, iter, first,
-                     hostOut);
+  launch<ESF_KT(kFinish, OP, K)>(ctx, dim3(1), dim3(1024), nb,
+                                 partials ? partials : (const double*)ctx->partials, sc, iter,
+                                 first, hostOut);
 }
 
 // a solve's scalar block at its start: tolerance eps, nothing done yet
-__global__ void kScalarsInit(CrScalars* sc, double eps) {
+template <class GridT>
+__device__ __forceinline__ void kScalarsInit(const GridT& wg, CrScalars* sc, double eps) {
   if (threadIdx.x != 0) return;
   CrScalars h{};
   h.eps = eps;
   h.iter = -1;
   *sc = h;
 }
+ESF_BOUNDS(kScalarsInit, 1024);
 
 // a scalar block into pinned host memory (the report of a queued solve)
-__global__ void kScalarsCopy(const CrScalars* sc, CrScalars* out) {
+template <class GridT>
+__device__ __forceinline__ void kScalarsCopy(const GridT& wg, const CrScalars* sc, CrScalars* out) {
   if (threadIdx.x == 0) *out = *sc;
 }
+ESF_BOUNDS(kScalarsCopy, 1024);
 
 inline void initScalarsAsync(esf_ctx* ctx, CrScalars* sc, double eps) {
-  hipLaunchKernelGGL(kScalarsInit, dim3(1), dim3(64), 0, ctx->stream, sc, eps);
+  launch<ESF_K(kScalarsInit)>(ctx, dim3(1), dim3(64), sc, eps);
 }
 
 inline void copyScalarsAsync(esf_ctx* ctx, const CrScalars* sc, CrScalars* out) {
-  hipLaunchKernelGGL(kScalarsCopy, dim3(1), dim3(64), 0, ctx->stream, sc, out);
+  launch<ESF_K(kScalarsCopy)>(ctx, dim3(1), dim3(64), sc, out);
 }
 
 }  // namespace
@@ -152,6 +157,7 @@ struct DonePoller {
   int markIt[kRing];
   double best = 0;
   bool stagnated = false;
+  int stopIt = -1;  // stagnated: the last iteration the solve runs
   hipEvent_t ev[kRing];
   explicit DonePoller(esf_ctx* c, int w, bool stag = false, int lagIters = 2, int flatPolls = 4)
       : ctx(c), which(w), stagnation(stag), lag(lagIters), flatMax(flatPolls) {
@@ -166,6 +172,7 @@ struct DonePoller {
       flat = 0;
     } else if (++flat >= flatMax) {
       stagnated = true;
+      stopIt = markIt[ws] + lag;
       return true;
     }
     return false;
@@ -176,7 +183,7 @@ struct DonePoller {
   void mark(int it) {
     const int slot = issued % kRing;
     markIt[slot] = it;
-    hipEventRecord(ev[slot], ctx->stream);
+    (void)hipEventRecord(ev[slot], ctx->stream);
     issued++;
   }
   // after iteration `it` is queued: true when the solve is known to be done
@@ -185,7 +192,7 @@ struct DonePoller {
     while (waited < issued) {
       const int ws = waited % kRing;
       if (markIt[ws] + lag <= it || issued - waited >= kRing) {
-        hipEventSynchronize(ev[ws]);
+        (void)eventSync(ctx, ev[ws]);
       } else if (hipEventQuery(ev[ws]) != hipSuccess) {
         break;
       }

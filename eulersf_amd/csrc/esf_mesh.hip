@@ -55,10 +55,12 @@ __device__ inline void cellF(const Dims& d, const double* u, long index, double*
 // advectSurfaceMesh, one thread per vertex: move with the solid velocity
 // for dt, then up to 10 Newton steps towards x - u(x) = X_rest (the step
 // uses F of the cell holding x), then clamp into the domain
-__global__ void __launch_bounds__(256) kAdvectMesh(Dims d, int nv, const double* __restrict__ rest,
-                                                   double* vert, const double* __restrict__ vel,
-                                                   const double* __restrict__ u, double dt) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kAdvectMesh(const GridT& wg, Dims d, int nv,
+                                            const double* __restrict__ rest, double* vert,
+                                            const double* __restrict__ vel,
+                                            const double* __restrict__ u, double dt) {
+  const int x = wg.block() * blockDim.x + threadIdx.x;
   if (x >= nv) return;
   const FieldGeom g = gridGeom(d);
   double p[3] = {vert[3 * x], vert[3 * x + 1], vert[3 * x + 2]};
@@ -97,11 +99,13 @@ __global__ void __launch_bounds__(256) kAdvectMesh(Dims d, int nv, const double*
     vert[3 * x + c] = p[c];
   }
 }
+ESF_BOUNDS(kAdvectMesh, 256);
 
 // triangle boxes (min xyz, max xyz per triangle), TRIANGLE_BVH_NODE::refit
-__global__ void kTriBoxes(int nt, const int* __restrict__ tri, const double* __restrict__ vert,
-                          double* box) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kTriBoxes(const GridT& wg, int nt, const int* __restrict__ tri,
+                                          const double* __restrict__ vert, double* box) {
+  const int t = wg.block() * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   double mn[3], mx[3];
 #pragma unroll
@@ -120,10 +124,12 @@ __global__ void kTriBoxes(int nt, const int* __restrict__ tri, const double* __r
     box[6 * t + 3 + c] = mx[c];
   }
 }
+ESF_BOUNDS(kTriBoxes, 1024);
 
 // the union of a mesh's triangle boxes (the BVH root), one block
-__global__ void __launch_bounds__(256) kMeshRoot(int nt, const double* __restrict__ box,
-                                                 double* root) {
+template <class GridT>
+__device__ __forceinline__ void kMeshRoot(const GridT& wg, int nt, const double* __restrict__ box,
+                                          double* root) {
   __shared__ double s[6][256];
   double m[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   for (int t = threadIdx.x; t < nt; t += blockDim.x)
@@ -145,12 +151,14 @@ __global__ void __launch_bounds__(256) kMeshRoot(int nt, const double* __restric
   }
   if (threadIdx.x < 6) root[threadIdx.x] = s[threadIdx.x][0];
 }
+ESF_BOUNDS(kMeshRoot, 256);
 
 // does some triangle box of mesh a overlap some triangle box of mesh b?
 // (aabb::overlaps, closed intervals); a hit sets *flag
-__global__ void __launch_bounds__(256) kMeshPairs(int na, const double* __restrict__ boxA, int nb,
-                                                  const double* __restrict__ boxB, int* flag) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kMeshPairs(const GridT& wg, int na, const double* __restrict__ boxA,
+                                           int nb, const double* __restrict__ boxB, int* flag) {
+  const int t = wg.block() * blockDim.x + threadIdx.x;
   if (t >= na) return;
   double a[6];
 #pragma unroll
@@ -163,18 +171,21 @@ __global__ void __launch_bounds__(256) kMeshPairs(int na, const double* __restri
   }
   if (hit) atomicOr(flag, 1);
 }
+ESF_BOUNDS(kMeshPairs, 256);
 
-__global__ void kZeroInts(int n, int* a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kZeroInts(const GridT& wg, int n, int* a) {
+  const int i = wg.block() * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
 }
+ESF_BOUNDS(kZeroInts, 1024);
 
 esf_status advectMeshes(esf_ctx* ctx) {
   for (const DevSolid& so : ctx->solids) {
     if (so.meshV == 0) continue;
-    hipLaunchKernelGGL(kAdvectMesh, dim3(cdiv(so.meshV, 256)), dim3(256), 0, ctx->stream,
-                       ctx->d, so.meshV, (const double*)so.meshRest, so.meshVert,
-                       (const double*)so.vel, (const double*)so.u, ctx->dt);
+    launch<ESF_K(kAdvectMesh)>(ctx, dim3(cdiv(so.meshV, 256)), dim3(256), ctx->d, so.meshV,
+                               (const double*)so.meshRest, so.meshVert, (const double*)so.vel,
+                               (const double*)so.u, ctx->dt);
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -187,22 +198,22 @@ esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<
   double* root = ctx->meshOut;
   int* flags = reinterpret_cast<int*>(ctx->meshOut + 6 * kMaxSolids);
   if (np > 0)
-    hipLaunchKernelGGL(kZeroInts, dim3(1), dim3(64), 0, ctx->stream, np, flags);
+    launch<ESF_K(kZeroInts)>(ctx, dim3(1), dim3(64), np, flags);
   for (int s = 0; s < n; s++) {
     const DevSolid& so = ctx->solids[s];
-    hipLaunchKernelGGL(kTriBoxes, dim3(cdiv(so.meshT, 256)), dim3(256), 0, ctx->stream,
-                       so.meshT, (const int*)so.meshTri, (const double*)so.meshVert, so.meshBox);
-    hipLaunchKernelGGL(kMeshRoot, dim3(1), dim3(256), 0, ctx->stream, so.meshT,
-                       (const double*)so.meshBox, root + 6 * s);
+    launch<ESF_K(kTriBoxes)>(ctx, dim3(cdiv(so.meshT, 256)), dim3(256), so.meshT,
+                             (const int*)so.meshTri, (const double*)so.meshVert, so.meshBox);
+    launch<ESF_K(kMeshRoot)>(ctx, dim3(1), dim3(256), so.meshT, (const double*)so.meshBox,
+                             root + 6 * s);
   }
   int q = 0;
   for (int a = 0; a < n; a++)
     for (int b = a + 1; b < n; b++, q++) {
       const DevSolid& A = ctx->solids[a];
       const DevSolid& B = ctx->solids[b];
-      hipLaunchKernelGGL(kMeshPairs, dim3(cdiv(A.meshT, 256)), dim3(256), 0, ctx->stream,
-                         A.meshT, (const double*)A.meshBox, B.meshT, (const double*)B.meshBox,
-                         flags + q);
+      launch<ESF_K(kMeshPairs)>(ctx, dim3(cdiv(A.meshT, 256)), dim3(256), A.meshT,
+                                (const double*)A.meshBox, B.meshT, (const double*)B.meshBox,
+                                flags + q);
     }
   roots.assign(6 * n, 0.0);
   pairs.assign(np, 0);
@@ -211,7 +222,7 @@ esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<
   if (np > 0)
     ESF_HIP(hipMemcpyAsync(pairs.data(), flags, np * sizeof(int), hipMemcpyDeviceToHost,
                            ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 

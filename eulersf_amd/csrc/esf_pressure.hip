@@ -92,9 +92,9 @@ static_assert(TX * TY == 256 && NRING <= 256, "one 256-thread block per tile");
 // the running sum starts at +0 and adding a zero never changes it.
 // Out-of-grid halo cells are read at a clamped address and replaced by 0,
 // so the loop has no per-cell branches.
-template <typename T, int MODE, int DEPTH>
-__global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, int tilesY,
-                                             int tilesZ, int layers) {
+template <typename T, int MODE, int DEPTH, class GridT>
+__device__ __forceinline__ void kSpmv(const GridT& wg, Dims d, SpmvArgs<T> a, int tilesX, int tilesY,
+                                      int tilesZ, int layers) {
   if (MODE != M_APPLY && a.sc && a.sc->done) return;
   __shared__ T xs[3 * HALO];
   __shared__ T ws[2 * NODES * 3];
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
 
   // XCD-aware tile order: each XCD gets a contiguous run of tiles, so
   // neighbouring tiles' shared halos stay in one L2
-  const int tile = xcdBlock();
+  const int tile = wg.block();
   const int tx = tile % tilesX, ty = (tile / tilesX) % tilesY,
             tz = tile / (tilesX * tilesY);
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * layers;
@@ -450,22 +450,23 @@ __global__ void __launch_bounds__(256) kSpmv(Dims d, SpmvArgs<T> a, int tilesX, 
   if (MODE == M_APPLY) return;
   if (MODE == M_UPDATE) {
     double mine[5] = {part0, part1, part2, part3, part4};
-    blockPartials<5>(mine, a.partials, red);
+    blockPartialsAt<5>(mine, a.partials, red, wg.blocks(), wg.block());
   } else {
     double mine[2] = {part0, part1};
-    blockPartials<2>(mine, a.partials, red);
+    blockPartialsAt<2>(mine, a.partials, red, wg.blocks(), wg.block());
   }
 }
+ESF_BOUNDS(kSpmv, 256);
 
 // b = J v0 - c, c (boundary velocities), D = diag(S); also the FP32 copies
 // used by the mixed solve.  COUPLED_INTEGRATOR_3D.cpp:286-325 (c),
 // :694-709 (D, b).
-__global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
-                             const double* __restrict__ vel,
-                             const double* __restrict__ minv, double* c,
-                             double* b, double* D, double* Dinv, float* D32,
-                             float* Dinv32) {
-  const long r = xcdBlock() * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kPressureRhs(const GridT& wg, Dims d, const double* __restrict__ v0,
+                                             const double* __restrict__ vel,
+                                             const double* __restrict__ minv, double* c, double* b,
+                                             double* D, double* Dinv, float* D32, float* Dinv32) {
+  const long r = wg.block() * (long)blockDim.x + threadIdx.x;
   if (r >= d.cells) return;
   const int cx = r % d.CX, cy = (r / d.CX) % d.CY, cz = r / d.cslab;
   const long base = cx + (long)cy * d.X + (long)cz * d.slab;
@@ -500,6 +501,7 @@ __global__ void kPressureRhs(Dims d, const double* __restrict__ v0,
     Dinv32[r] = (float)(1.0 / dd);
   }
 }
+ESF_BOUNDS(kPressureRhs, 1024);
 
 // ---------------------------------------------------------------------------
 // host side
@@ -547,14 +549,14 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   const int nb = tx * ty * tz;
   const int depth = MODE == M_UPDATE ? ctx->spmvDepthUpdate : ctx->spmvDepthOther;
   if (depth == 3)
-    launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 3>, dim3(nb), dim3(256),
-                ctx->d, a, tx, ty, tz, layers);
+    launchTimed<ESF_KT(kSpmv, T, MODE, 3)>(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), dim3(nb),
+                                           dim3(256), ctx->d, a, tx, ty, tz, layers);
   else if (depth == 2)
-    launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 2>, dim3(nb), dim3(256),
-                ctx->d, a, tx, ty, tz, layers);
+    launchTimed<ESF_KT(kSpmv, T, MODE, 2)>(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), dim3(nb),
+                                           dim3(256), ctx->d, a, tx, ty, tz, layers);
   else
-    launchTimed(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), kSpmv<T, MODE, 1>, dim3(nb), dim3(256),
-                ctx->d, a, tx, ty, tz, layers);
+    launchTimed<ESF_KT(kSpmv, T, MODE, 1)>(ctx, MODE + (sizeof(T) == 4 ? 6 : 0), dim3(nb),
+                                           dim3(256), ctx->d, a, tx, ty, tz, layers);
   // the grid-wide sums of the mode (kSpmv's epilogue partials)
   if constexpr (MODE == M_SETUP_R) launchFinish<F_BNORM, 1>(ctx, nb, a.sc);
   if constexpr (MODE == M_SETUP_S) launchFinish<F_SETUP_S, 2>(ctx, nb, a.sc);
@@ -579,26 +581,29 @@ int spmvBlocks(const esf_ctx* ctx) {
 
 esf_status pressureRhs(esf_ctx* ctx) {
   const Dims& d = ctx->d;
-  hipLaunchKernelGGL(kPressureRhs, dim3(cdiv(d.cells, 256)), dim3(256), 0, ctx->stream, d,
-                     ctx->v0, ctx->vel, ctx->minv, ctx->c, ctx->b, ctx->D,
-                     ctx->Dinv, ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr,
-                     ctx->Dinv32);
+  launch<ESF_K(kPressureRhs)>(ctx, dim3(cdiv(d.cells, 256)), dim3(256), d, ctx->v0, ctx->vel,
+                              ctx->minv, ctx->c, ctx->b, ctx->D, ctx->Dinv,
+                              ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr, ctx->Dinv32);
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
-__global__ void kToFloat(long n, const double* a, float* b) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+template <class GridT>
+__device__ __forceinline__ void kToFloat(const GridT& wg, long n, const double* a, float* b) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     b[i] = (float)a[i];
 }
+ESF_BOUNDS(kToFloat, 1024);
 // streaming copy of n doubles (calibration of the HBM byte counters and
 // the box's copy bandwidth): 16-byte vector loads and stores
-__global__ void __launch_bounds__(256) kCopy(long n2, const double2* __restrict__ a,
-                                             double2* __restrict__ b) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2;
-       i += (long)gridDim.x * blockDim.x)
+template <class GridT>
+__device__ __forceinline__ void kCopy(const GridT& wg, long n2, const double2* __restrict__ a,
+                                      double2* __restrict__ b) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n2;
+       i += (long)wg.blocks() * blockDim.x)
     b[i] = a[i];
 }
+ESF_BOUNDS(kCopy, 256);
 
 esf_status benchCopy(esf_ctx* ctx, long bytes, int reps, double* avgMs) {
   const long n2 = bytes / 16;
@@ -610,12 +615,12 @@ esf_status benchCopy(esf_ctx* ctx, long bytes, int reps, double* avgMs) {
   }
   hipMemsetAsync(a, 0, n2 * 16, ctx->stream);
   const int blocks = 256 * 32;
-  hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, ctx->stream, n2, (const double2*)a, b);
+  launch<ESF_K(kCopy)>(ctx, dim3(blocks), dim3(256), n2, (const double2*)a, b);
   hipEventRecord(ctx->ev[2], ctx->stream);
   for (int i = 0; i < reps; i++)
-    hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, ctx->stream, n2, (const double2*)a, b);
+    launch<ESF_K(kCopy)>(ctx, dim3(blocks), dim3(256), n2, (const double2*)a, b);
   hipEventRecord(ctx->ev[3], ctx->stream);
-  hipError_t e = hipEventSynchronize(ctx->ev[3]);
+  hipError_t e = eventSync(ctx, ctx->ev[3]);
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
   *avgMs = ms / reps;
@@ -634,8 +639,8 @@ esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs) {
   }
   esf_status st = ESF_OK;
   if (fp32)
-    hipLaunchKernelGGL(kToFloat, dim3(1024), dim3(256), 0, ctx->stream, d.nodes,
-                       (const double*)ctx->minv, ctx->minv32);
+    launch<ESF_K(kToFloat)>(ctx, dim3(1024), dim3(256), d.nodes, (const double*)ctx->minv,
+                            ctx->minv32);
   hipEventRecord(ctx->ev[2], ctx->stream);
   for (int i = 0; i < reps && !st; i++) {
     if (fp32) {
@@ -653,7 +658,7 @@ esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs) {
     }
   }
   hipEventRecord(ctx->ev[3], ctx->stream);
-  if (hipEventSynchronize(ctx->ev[3]) != hipSuccess) return ESF_ERR_DEVICE;
+  if (eventSync(ctx, ctx->ev[3]) != hipSuccess) return ESF_ERR_DEVICE;
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
   *avgMs = reps > 0 ? ms / reps : 0;
@@ -676,7 +681,7 @@ static esf_status initScalars(esf_ctx* ctx, int which, double eps) {
 // synchronous read of scalar block `which` (through report slot 3)
 static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
   copyScalarsAsync(ctx, &ctx->sc[which], &ctx->hostSc[3]);
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   *out = ctx->hostSc[3];
   return ESF_OK;
 }
@@ -723,20 +728,25 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   DonePoller poll(ctx, which, sizeof(T) == 4, 2, 4);
   T *sA = s, *sB = s2, *apA = Ap, *apB = Ap2;
   a.recursive = recursive ? 1 : 0;
-  int i = 0;
-  for (; i < maxIt; i++) {
-    a.iter = i;
+  // iteration `it`: the fused update and S r, then the buffers swap
+  auto update = [&](int it, CrScalars* hostOut) {
+    a.iter = it;
     a.in = rA;
     a.r = rB;
     a.s = sA;
     a.sOut = sB;
     a.Ap = apA;
     a.ApOut = apB;
-    a.hostOut = recursive ? poll.slot() : nullptr;
-    if ((st = launchSpmv<T, M_UPDATE>(ctx, a))) return st;
+    a.hostOut = hostOut;
+    const esf_status e = launchSpmv<T, M_UPDATE>(ctx, a);
     std::swap(rA, rB);
     std::swap(sA, sB);
     std::swap(apA, apB);
+    return e;
+  };
+  int i = 0;
+  for (; i < maxIt; i++) {
+    if ((st = update(i, recursive ? poll.slot() : nullptr))) return st;
     if (recursive) {
       poll.mark(i);
     } else if (i > 0 && i % 5 == 0) {
@@ -746,6 +756,12 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
       poll.mark(i);
     }
     if (poll.drain(i)) {
+      // a solve that stagnated runs on to `lag` iterations past the check
+      // that showed it, however early the host saw that check, so the
+      // iterate it leaves does not depend on the timing of the polls (a
+      // converged solve's further launches exit at once)
+      while (poll.stagnated && i < poll.stopIt && i + 1 < maxIt)
+        if ((st = update(++i, nullptr))) return st;
       i++;
       break;
     }
@@ -765,11 +781,13 @@ template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*, co
                                    const float*, float*, float*, float*, float*, float*,
                                    float*, float*, float*, double, int, int, bool);
 
-__global__ void kZeroF(long n, float* a) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+template <class GridT>
+__device__ __forceinline__ void kZeroF(const GridT& wg, long n, float* a) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     a[i] = 0.0f;
 }
+ESF_BOUNDS(kZeroF, 1024);
 
 // Mixed-precision pressure solve: FP32 CR on S (FP32 minv, FP32 vectors,
 // FP64 inner products) inside FP64 iterative refinement.  The outer
@@ -779,8 +797,8 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
   const Dims& d = ctx->d;
   esf_status st = initScalars(ctx, 0, eps);
   if (st) return st;
-  hipLaunchKernelGGL(kToFloat, dim3(1024), dim3(256), 0, ctx->stream, d.nodes,
-                     (const double*)ctx->minv, ctx->minv32);
+  launch<ESF_K(kToFloat)>(ctx, dim3(1024), dim3(256), d.nodes, (const double*)ctx->minv,
+                          ctx->minv32);
   SpmvArgs<double> o{};
   o.minv = ctx->minv;
   o.b = ctx->b;
@@ -809,7 +827,7 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     // (418 / 414 / 353 / 276 steps/s).
     double need = h.resnorm > 0 ? 0.5 * eps * h.bnorm / h.resnorm : 1.0;
     double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
-    hipLaunchKernelGGL(kZeroF, dim3(1024), dim3(256), 0, ctx->stream, d.cells, ctx->e32);
+    launch<ESF_K(kZeroF)>(ctx, dim3(1024), dim3(256), d.cells, ctx->e32);
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->Dinv32, ctx->e32,
                              ctx->r32,
                              ctx->r32b, ctx->p32, ctx->s32, ctx->s32b, ctx->Ap32, ctx->Ap32b,

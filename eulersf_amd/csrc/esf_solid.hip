@@ -30,17 +30,17 @@ __device__ __forceinline__ int sgn(int bit) { return bit ? 1 : -1; }
 // Per-cell arrays are structure-of-arrays with stride `cap` (the slot
 // capacity): component k of slot e lives at [k * cap + e], so a wave reads
 // 64 consecutive slots of one component in one coalesced access.
-__global__ void __launch_bounds__(256) kStress(Dims d, int nValid, int cap,
-                                               const int* __restrict__ cellList,
-                                               const double* __restrict__ u,
-                                               const double* __restrict__ mass,
-                                               const int* __restrict__ sidx, double lambda,
-                                               double mu, double* cellMat, double* cellOut,
-                                               double* cellDiag, int* cornerDof,
-                                               double* energyPart,
-                                               const double* __restrict__ Fset) {
+template <class GridT>
+__device__ __forceinline__ void kStress(const GridT& wg, Dims d, int nValid, int cap,
+                                        const int* __restrict__ cellList,
+                                        const double* __restrict__ u,
+                                        const double* __restrict__ mass,
+                                        const int* __restrict__ sidx, double lambda, double mu,
+                                        double* cellMat, double* cellOut, double* cellDiag,
+                                        int* cornerDof, double* energyPart,
+                                        const double* __restrict__ Fset) {
   __shared__ double red[16];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = wg.block() * blockDim.x + threadIdx.x;
   double en = 0;
   if (e < nValid) {
     const long index = cellList[e];
@@ -134,20 +134,25 @@ __global__ void __launch_bounds__(256) kStress(Dims d, int nValid, int cap,
     m[27L * cap] = Jinv;
   }
   const double tot = blockSum(en, red);
-  if (threadIdx.x == 0) energyPart[blockIdx.x] = tot;
+  if (threadIdx.x == 0) energyPart[wg.block()] = tot;
 }
+ESF_BOUNDS(kStress, 256);
 
 // Per solid DOF node: gather the stress force and the stiffness diagonal
 // from adjacent valid cells (ascending cell order), and computeMV
 // (DEFOR_SOLID_3D.cpp:873-889).  `first` initialises (solid 0), later
 // solids accumulate, matching `force += localForce` per solid (:865).
-__global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict__ solidNodes,
-                             const int* __restrict__ cellSlot, const double* __restrict__ cellOut,
-                             const double* __restrict__ cellDiag,
-                             const double* __restrict__ smass, const double* __restrict__ svel,
-                             Box box, int first, double* sForce, double* sKdiag, double* sMv,
-                             uint8_t* nodeMaskByte) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kSolidGather(const GridT& wg, Dims d, int nSolid, int cap,
+                                             const int* __restrict__ solidNodes,
+                                             const int* __restrict__ cellSlot,
+                                             const double* __restrict__ cellOut,
+                                             const double* __restrict__ cellDiag,
+                                             const double* __restrict__ smass,
+                                             const double* __restrict__ svel, Box box, int first,
+                                             double* sForce, double* sKdiag, double* sMv,
+                                             uint8_t* nodeMaskByte) {
+  const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
@@ -192,6 +197,7 @@ __global__ void kSolidGather(Dims d, int nSolid, int cap, const int* __restrict_
     }
   }
 }
+ESF_BOUNDS(kSolidGather, 1024);
 
 // alpha and beta of solid CR iteration i, from the partial sums the node
 // pass of iteration i - 1 left in `cur` (delta_i, s.D^-1 s, Ap.D^-1 s,
@@ -308,25 +314,23 @@ __device__ inline double sumGroup(double v) {
 
 constexpr int kCellThreads = 256;  // cell-pass block (512 measured slower)
 
-template <int MODE, int LPC>
-__global__ void __launch_bounds__(kCellThreads) kSolidCell(Dims d, int nValid, int cap,
-                                                  const int* __restrict__ cornerDof,
-                                                  const double* __restrict__ cellMat,
-                                                  const double* __restrict__ in,
-                                                  const double* __restrict__ sd,
-                                                  const double* __restrict__ apd,
-                                                  const double* __restrict__ xin,
-                                                  const CrScalars* sc, SolidPartials sp,
-                                                  double lambda, double mu,
-                                                  double* __restrict__ inbox,
-                                                  double* __restrict__ inboxX) {
+template <int MODE, int LPC, class GridT>
+__device__ __forceinline__ void kSolidCell(const GridT& wg, Dims d, int nValid, int cap,
+                                           const int* __restrict__ cornerDof,
+                                           const double* __restrict__ cellMat,
+                                           const double* __restrict__ in,
+                                           const double* __restrict__ sd,
+                                           const double* __restrict__ apd,
+                                           const double* __restrict__ xin, const CrScalars* sc,
+                                           SolidPartials sp, double lambda, double mu,
+                                           double* __restrict__ inbox, double* __restrict__ inboxX) {
 #pragma clang fp contract(on)
   static_assert(LPC == 2 || LPC == 4 || LPC == 8, "lanes per cell");
   constexpr bool kCR = MODE != CELL_IN;   // the CR residual update at the corners
   constexpr bool kX = MODE == CELL_CRX;   // also K_e x
   constexpr int CPL = 8 / LPC;  // corners per lane
   __shared__ double grp[5 * kScalMaxGroups];
-  const int t = xcdBlock() * blockDim.x + threadIdx.x;
+  const int t = wg.block() * blockDim.x + threadIdx.x;
   const int e = t / LPC, l = t % LPC;
   const bool live = e < nValid;
   // first round of loads, all unconditional (indices clamped in bounds,
@@ -429,6 +433,7 @@ __global__ void __launch_bounds__(kCellThreads) kSolidCell(Dims d, int nValid, i
   product(v, inbox);
   if (kX) product(xx, inboxX);
 }
+ESF_BOUNDS(kSolidCell, kCellThreads);
 
 enum SolidMode { S_APPLY = 0, S_SETUP = 1, S_CR = 2, S_CHECK = 3, S_CRX = 4 };
 
@@ -473,8 +478,8 @@ struct SolidNodeArgs {
 // afterwards), so a pass costs one memory round trip.
 constexpr int kNodeThreads = 512;  // node-pass block: half the partials of 256 (1024 measured slower)
 
-template <int MODE>
-__global__ void __launch_bounds__(kNodeThreads) kSolidNode(Dims d, SolidNodeArgs a) {
+template <int MODE, class GridT>
+__device__ __forceinline__ void kSolidNode(const GridT& wg, Dims d, SolidNodeArgs a) {
 #pragma clang fp contract(on)
   constexpr bool kCR = MODE == S_CR || MODE == S_CRX;
   constexpr bool kX = MODE == S_CRX;
@@ -482,7 +487,7 @@ __global__ void __launch_bounds__(kNodeThreads) kSolidNode(Dims d, SolidNodeArgs
   __shared__ double grp[5 * kScalMaxGroups];
   const long nDof = 3L * a.nSolid;
   if (nDof == 0) return;
-  const int blk = xcdBlock();
+  const int blk = wg.block();
   const long k0 = blk * (long)blockDim.x + threadIdx.x;
   const bool live = k0 < nDof;
   const long k = live ? k0 : nDof - 1;
@@ -593,28 +598,34 @@ __global__ void __launch_bounds__(kNodeThreads) kSolidNode(Dims d, SolidNodeArgs
   }
   if (MODE == S_APPLY) return;
   if (kX) {
-    blockPartialsAt<5>(part, a.partials, red, gridDim.x, blk);
+    blockPartialsAt<5>(part, a.partials, red, wg.blocks(), blk);
   } else {
     double p4[4] = {part[0], part[1], part[2], part[3]};
-    blockPartialsAt<4>(p4, a.partials, red, gridDim.x, blk);
+    blockPartialsAt<4>(p4, a.partials, red, wg.blocks(), blk);
   }
 }
+ESF_BOUNDS(kSolidNode, kNodeThreads);
 
 // the end of a solid solve whose convergence a riding check found: x back
 // to the iterate that check accepted
-__global__ void kSolidFixX(long n, const CrScalars* sc, const double* xb, double* x) {
+template <class GridT>
+__device__ __forceinline__ void kSolidFixX(const GridT& wg, long n, const CrScalars* sc,
+                                           const double* xb, double* x) {
   if (!(sc->done && sc->aux != 0.0)) return;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x)
     x[i] = xb[i];
 }
+ESF_BOUNDS(kSolidFixX, 1024);
 
 // diag(A) = scale * diag(K) + M~   (COUPLED_INTEGRATOR_3D.cpp:613-617)
-__global__ void kSolidDiag(long n, const double* kdiag, const double* damped, double scale,
-                           double* adiag) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kSolidDiag(const GridT& wg, long n, const double* kdiag,
+                                           const double* damped, double scale, double* adiag) {
+  const long i = wg.block() * (long)blockDim.x + threadIdx.x;
   if (i < n) adiag[i] = kdiag[i] * scale + damped[i];
 }
+ESF_BOUNDS(kSolidDiag, 1024);
 
 
 // ---------------------------------------------------------------------------
@@ -631,23 +642,22 @@ esf_status stressStage(esf_ctx* ctx, double* energy) {
     DevSolid& so = ctx->solids[b];
     const int nb = cdiv(std::max(so.nValid, 1), 256);
     if (so.nValid > 0)
-      hipLaunchKernelGGL(kStress, dim3(nb), dim3(256), 0, ctx->stream, d, so.nValid, so.capValid,
-                         so.cellList, so.u, so.mass, ctx->sidx, so.lambda, so.mu, so.cellMat,
-                         so.cellOut, so.cellDiag, so.cornerDof, ctx->partials,
-                         so.nFset == so.nValid ? (const double*)so.Fset : nullptr);
+      launch<ESF_K(kStress)>(ctx, dim3(nb), dim3(256), d, so.nValid, so.capValid, so.cellList, so.u,
+                             so.mass, ctx->sidx, so.lambda, so.mu, so.cellMat, so.cellOut,
+                             so.cellDiag, so.cornerDof, ctx->partials,
+                             so.nFset == so.nValid ? (const double*)so.Fset : nullptr);
     if (energy && so.nValid > 0) {
       std::vector<double> h(nb);
       ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, nb * sizeof(double),
                              hipMemcpyDeviceToHost, ctx->stream));
-      ESF_HIP(hipStreamSynchronize(ctx->stream));
+      ESF_HIP(streamSync(ctx));
       for (double v : h) en += v;
     }
     if (ctx->nSolid > 0)
-      hipLaunchKernelGGL(kSolidGather, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0,
-                         ctx->stream, d, ctx->nSolid, so.capValid, ctx->solidNodes, so.cellSlot,
-                         so.cellOut,
-                         so.cellDiag, so.mass, so.vel, so.box, b == 0 ? 1 : 0, ctx->sForce,
-                         ctx->sKdiag, ctx->sMv, (uint8_t*)ctx->nodeMask + b);
+      launch<ESF_K(kSolidGather)>(ctx, dim3(cdiv(ctx->nSolid, 256)), dim3(256), d, ctx->nSolid,
+                                  so.capValid, ctx->solidNodes, so.cellSlot, so.cellOut,
+                                  so.cellDiag, so.mass, so.vel, so.box, b == 0 ? 1 : 0, ctx->sForce,
+                                  ctx->sKdiag, ctx->sMv, (uint8_t*)ctx->nodeMask + b);
   }
   if (energy) *energy = en;
   hipError_t e = hipGetLastError();
@@ -689,14 +699,15 @@ static void cellPassLanes(esf_ctx* ctx, const double* in, const CrScalars* sc,
   for (size_t b = 0; b < ctx->solids.size(); b++) {
     DevSolid& so = ctx->solids[b];
     if (so.nValid > 0) {
-      launchTimed(ctx, cls, kSolidCell<MODE, LPC>,
-                  dim3(cdiv((long)LPC * so.nValid, kCellThreads)), dim3(kCellThreads),
-                  ctx->d, so.nValid,
-                  so.capValid,
-                  (const int*)so.cornerDof, (const double*)so.cellMat, in,
-                  (const double*)ctx->sSd, (const double*)ctx->sApd, x,
-                  sc, sp, so.lambda, so.mu, ctx->inbox + 24L * ctx->capSolid * b,
-                  ctx->inboxX + 24L * ctx->capSolid * b);
+      launchTimed<ESF_KT(kSolidCell, MODE, LPC)>(ctx, cls,
+                                                 dim3(cdiv((long)LPC * so.nValid, kCellThreads)),
+                                                 dim3(kCellThreads), ctx->d, so.nValid, so.capValid,
+                                                 (const int*)so.cornerDof,
+                                                 (const double*)so.cellMat, in,
+                                                 (const double*)ctx->sSd, (const double*)ctx->sApd,
+                                                 x, sc, sp, so.lambda, so.mu,
+                                                 ctx->inbox + 24L * ctx->capSolid * b,
+                                                 ctx->inboxX + 24L * ctx->capSolid * b);
     }
   }
 }
@@ -719,16 +730,16 @@ static int nodeBlocks(const esf_ctx* ctx) {
 
 template <int MODE>
 static void nodePass(esf_ctx* ctx, SolidNodeArgs& a) {
-  launchTimed(ctx, MODE == S_CR ? 15 : (MODE == S_CRX ? 17 : -1), kSolidNode<MODE>,
-              dim3(nodeBlocks(ctx)), dim3(kNodeThreads), ctx->d, a);
+  launchTimed<ESF_KT(kSolidNode, MODE)>(ctx, MODE == S_CR ? 15 : (MODE == S_CRX ? 17 : -1),
+                                        dim3(nodeBlocks(ctx)), dim3(kNodeThreads), ctx->d, a);
 }
 
 esf_status solidAdiag(esf_ctx* ctx) {
   const long n = 3L * ctx->nSolid;
   if (n == 0) return ESF_OK;
   const double scale = ctx->dt * ctx->dt + ctx->p.rayleigh_beta * ctx->dt;
-  hipLaunchKernelGGL(kSolidDiag, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, n, ctx->sKdiag,
-                     ctx->sDamped, scale, ctx->sAdiag);
+  launch<ESF_K(kSolidDiag)>(ctx, dim3(cdiv(n, 256)), dim3(256), n, ctx->sKdiag, ctx->sDamped, scale,
+                            ctx->sAdiag);
   return ESF_OK;
 }
 
@@ -759,12 +770,14 @@ esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
 // beta^2 Ap.D^-1 Ap lets the node kernel's one reduction provide it, so
 // no separate direction pass (and no third grid-wide reduction) is needed.
 // Only the rounding of alpha differs from the reference's evaluation.
-__global__ void kSolidSetup(long n, const double* b, const double* diag, double* x, double* r,
-                            double* p, double* Ap, double* apd, double* partials) {
+template <class GridT>
+__device__ __forceinline__ void kSolidSetup(const GridT& wg, long n, const double* b,
+                                            const double* diag, double* x, double* r, double* p,
+                                            double* Ap, double* apd, double* partials) {
   __shared__ double red[16];
   double part = 0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x) {
     x[i] = 0.0;
     r[i] = b[i] / diag[i];
     p[i] = 0.0;   // with beta = 0 the first direction is p = r, Ap = s
@@ -773,26 +786,32 @@ __global__ void kSolidSetup(long n, const double* b, const double* diag, double*
     part += b[i] * b[i];
   }
   double mine[1] = {part};
-  blockPartials<1>(mine, partials, red);
+  blockPartialsAt<1>(mine, partials, red, wg.blocks(), wg.block());
 }
+ESF_BOUNDS(kSolidSetup, 1024);
 
 // a warm start: x at each solid DOF from a node field (3 doubles per node)
-__global__ void kWarmGather(int nSolid, const int* solidNodes, const double* guess, double* x) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <class GridT>
+__device__ __forceinline__ void kWarmGather(const GridT& wg, int nSolid, const int* solidNodes,
+                                            const double* guess, double* x) {
+  const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
   for (int c = 0; c < 3; c++) x[3L * s + c] = guess[3 * n + c];
 }
+ESF_BOUNDS(kWarmGather, 1024);
 
 // the CR set-up from a warm start x0, given y = A x0: r = D^-1 (b - y), no
 // direction yet; partial sums |b|^2 and |b - A x0|^2
-__global__ void kSolidSetupWarm(long n, const double* b, const double* diag, const double* y,
-                                double* r, double* p, double* Ap, double* apd,
-                                double* partials) {
+template <class GridT>
+__device__ __forceinline__ void kSolidSetupWarm(const GridT& wg, long n, const double* b,
+                                                const double* diag, const double* y, double* r,
+                                                double* p, double* Ap, double* apd,
+                                                double* partials) {
   __shared__ double red[2 * 4];
   double part[2] = {0, 0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)wg.blocks() * blockDim.x) {
     const double res = b[i] - y[i];
     r[i] = res / diag[i];
     p[i] = 0.0;
@@ -801,8 +820,9 @@ __global__ void kSolidSetupWarm(long n, const double* b, const double* diag, con
     part[0] += b[i] * b[i];
     part[1] += res * res;
   }
-  blockPartials<2>(part, partials, red);
+  blockPartialsAt<2>(part, partials, red, wg.blocks(), wg.block());
 }
+ESF_BOUNDS(kSolidSetupWarm, 1024);
 
 esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, double eps,
                       int maxIt, int slot, const double* guess) {
@@ -813,7 +833,7 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   const long n = 3L * ctx->nSolid;
   if (n == 0) {
     // nothing queued for this slot: report "converged after 0 iterations"
-    ESF_HIP(hipStreamSynchronize(ctx->stream));
+    ESF_HIP(streamSync(ctx));
     ctx->hostSc[slot] = CrScalars{};
     ctx->hostSc[slot].done = 1;
     ctx->hostSc[slot].iter = 0;
@@ -832,19 +852,18 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
   if (guess) {
     // warm start x0 from the node field: r = D^-1 (b - A x0), and done at
     // once if x0 already meets the tolerance
-    hipLaunchKernelGGL(kWarmGather, dim3(cdiv(ctx->nSolid, 256)), dim3(256), 0, ctx->stream,
-                       ctx->nSolid, (const int*)ctx->solidNodes, guess, x);
+    launch<ESF_K(kWarmGather)>(ctx, dim3(cdiv(ctx->nSolid, 256)), dim3(256), ctx->nSolid,
+                               (const int*)ctx->solidNodes, guess, x);
     esf_status st = solidApplyDevice(ctx, x, ctx->stmp);
     if (st) return st;
-    hipLaunchKernelGGL(kSolidSetupWarm, dim3(eb), dim3(256), 0, ctx->stream, n, b,
-                       (const double*)ctx->sAdiag, (const double*)ctx->stmp, ctx->sr, ctx->sp,
-                       ctx->sAp, ctx->sApd, ctx->partials);
+    launch<ESF_K(kSolidSetupWarm)>(ctx, dim3(eb), dim3(256), n, b, (const double*)ctx->sAdiag,
+                                   (const double*)ctx->stmp, ctx->sr, ctx->sp, ctx->sAp, ctx->sApd,
+                                   ctx->partials);
     launchFinish<F_WARM, 2>(ctx, eb, sc, 0, 0, nullptr, poll.slot());
     poll.mark(-1);
   } else {
-    hipLaunchKernelGGL(kSolidSetup, dim3(eb), dim3(256), 0, ctx->stream, n, b,
-                       (const double*)ctx->sAdiag, x, ctx->sr, ctx->sp, ctx->sAp, ctx->sApd,
-                       ctx->partials);
+    launch<ESF_K(kSolidSetup)>(ctx, dim3(eb), dim3(256), n, b, (const double*)ctx->sAdiag, x,
+                               ctx->sr, ctx->sp, ctx->sAp, ctx->sApd, ctx->partials);
     launchFinish<F_BNORM, 1>(ctx, eb, sc);
   }
   // partial sets of 5 x nb: node pass i writes set i mod 3 (the setup pass
@@ -899,8 +918,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
     }
     if (poll.drain(i)) break;
   }
-  hipLaunchKernelGGL(kSolidFixX, dim3(std::max(1, std::min(cdiv(n, 256), 1024))), dim3(256), 0,
-                     ctx->stream, n, (const CrScalars*)sc, (const double*)ctx->sXb, x);
+  launch<ESF_K(kSolidFixX)>(ctx, dim3(std::max(1, std::min(cdiv(n, 256), 1024))), dim3(256), n,
+                            (const CrScalars*)sc, (const double*)ctx->sXb, x);
   // the report is written without waiting; the caller reads report slot
   // `slot` after its next synchronisation
   copyScalarsAsync(ctx, sc, &ctx->hostSc[slot]);

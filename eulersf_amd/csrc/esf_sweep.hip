@@ -51,10 +51,12 @@ struct Sweep {
 };
 
 // phi -> -0.5 / large, and the extent of the massive nodes
-__global__ void kSdfInit(long n, int X, int Y, const double* __restrict__ mass, double large,
-                         double* phi, int* ext) {
-  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n;
-       c += (long)gridDim.x * blockDim.x) {
+template <class GridT>
+__device__ __forceinline__ void kSdfInit(const GridT& wg, long n, int X, int Y,
+                                         const double* __restrict__ mass, double large, double* phi,
+                                         int* ext) {
+  for (long c = wg.block() * (long)blockDim.x + threadIdx.x; c < n;
+       c += (long)wg.blocks() * blockDim.x) {
     const bool m = mass[c] > 0;
     phi[c] = m ? -0.5 : large;
     if (m) {
@@ -68,11 +70,12 @@ __global__ void kSdfInit(long n, int X, int Y, const double* __restrict__ mass, 
     }
   }
 }
+ESF_BOUNDS(kSdfInit, 1024);
 
 // sweepPhi's sweep s, one workgroup, plane by plane
-__global__ void __launch_bounds__(kWaveThreads) kPhiWave(SweepBox b, Sweep s,
-                                                        const double* __restrict__ mass,
-                                                        double* phi) {
+template <class GridT>
+__device__ __forceinline__ void kPhiWave(const GridT& wg, SweepBox b, Sweep s,
+                                         const double* __restrict__ mass, double* phi) {
   const int planes = s.ni + s.nj + s.nk - 2;
   for (int p = 0; p < planes; p++) {
     // the (j', k') rectangle that can hold a node of plane p
@@ -94,6 +97,7 @@ __global__ void __launch_bounds__(kWaveThreads) kPhiWave(SweepBox b, Sweep s,
     __syncthreads();
   }
 }
+ESF_BOUNDS(kPhiWave, kWaveThreads);
 
 // sweepOnePass's per-node decisions, from phi and the mass only:
 // returns false where the node is not updated; clear* say whether the node
@@ -130,11 +134,13 @@ __device__ inline bool passNode(const SweepBox& b, const Sweep& s, const double*
 }
 
 // the first node, in loop order, that clears each flag
-__global__ void kPassFlags(SweepBox b, Sweep s, const double* __restrict__ mass,
-                           const double* __restrict__ phi, int* first) {
+template <class GridT>
+__device__ __forceinline__ void kPassFlags(const GridT& wg, SweepBox b, Sweep s,
+                                           const double* __restrict__ mass,
+                                           const double* __restrict__ phi, int* first) {
   const long n = (long)s.ni * s.nj * s.nk;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
+  for (long t = wg.block() * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)wg.blocks() * blockDim.x) {
     const int ip = (int)(t % s.ni), jp = (int)((t / s.ni) % s.nj), kp = (int)(t / ((long)s.ni * s.nj));
     PassNode o;
     passNode(b, s, mass, phi, s.i0 + s.di * ip, s.j0 + s.dj * jp, s.k0 + s.dk * kp, o);
@@ -143,18 +149,21 @@ __global__ void kPassFlags(SweepBox b, Sweep s, const double* __restrict__ mass,
     if (o.clearZ) atomicMin(&first[2], (int)t);
   }
 }
+ESF_BOUNDS(kPassFlags, 1024);
 
 // every node's coefficients for the pass, from the flags' first clearing
 // nodes: alpha and beta where the node is updated, NaN in alpha where it
 // is not (indexed by loop position), so the wave below reads them together
 // with the neighbours' field values in one round of loads
-__global__ void kPassCoef(SweepBox b, Sweep s, const double* __restrict__ mass,
-                          const double* __restrict__ phi, const int* __restrict__ first,
-                          double2* coef) {
+template <class GridT>
+__device__ __forceinline__ void kPassCoef(const GridT& wg, SweepBox b, Sweep s,
+                                          const double* __restrict__ mass,
+                                          const double* __restrict__ phi,
+                                          const int* __restrict__ first, double2* coef) {
   const long n = (long)s.ni * s.nj * s.nk;
   const int fx = first[0], fy = first[1], fz = first[2];
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
+  for (long t = wg.block() * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)wg.blocks() * blockDim.x) {
     const int ip = (int)(t % s.ni), jp = (int)((t / s.ni) % s.nj), kp = (int)(t / ((long)s.ni * s.nj));
     PassNode o;
     double2 ab = make_double2(NAN, 0.0);
@@ -172,11 +181,12 @@ __global__ void kPassCoef(SweepBox b, Sweep s, const double* __restrict__ mass,
     coef[t] = ab;
   }
 }
+ESF_BOUNDS(kPassCoef, 1024);
 
 // the pass itself, one workgroup, plane by plane
-__global__ void __launch_bounds__(kWaveThreads) kPassWave(SweepBox b, Sweep s,
-                                                         const double2* __restrict__ coef,
-                                                         double* field) {
+template <class GridT>
+__device__ __forceinline__ void kPassWave(const GridT& wg, SweepBox b, Sweep s,
+                                          const double2* __restrict__ coef, double* field) {
   const int planes = s.ni + s.nj + s.nk - 2;
   for (int p = 0; p < planes; p++) {
     const int jLo = max(0, p - (s.ni - 1) - (s.nk - 1)), jHi = min(s.nj - 1, p);
@@ -205,15 +215,17 @@ __global__ void __launch_bounds__(kWaveThreads) kPassWave(SweepBox b, Sweep s,
     __syncthreads();
   }
 }
+ESF_BOUNDS(kPassWave, kWaveThreads);
 
 // sweepVectorField's face copies, one axis per launch (z, then y, then x):
 // each thread copies both faces of its line in the reference's order
-__global__ void kFaceCopy(SweepBox b, int axis, const double* __restrict__ mass,
-                          double* field) {
+template <class GridT>
+__device__ __forceinline__ void kFaceCopy(const GridT& wg, SweepBox b, int axis,
+                                          const double* __restrict__ mass, double* field) {
   const int lo[3] = {b.x0, b.y0, b.z0}, hi[3] = {b.x1, b.y1, b.z1};
   const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;  // the two other axes
   const long nu = hi[u] - lo[u], nw = hi[w] - lo[w];
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= nu * nw) return;
   int pos[3];
   pos[u] = lo[u] + (int)(t / nw);
@@ -229,10 +241,13 @@ __global__ void kFaceCopy(SweepBox b, int axis, const double* __restrict__ mass,
   copy(lo[axis], lo[axis] + 1);
   copy(hi[axis] - 1, hi[axis] - 2);
 }
+ESF_BOUNDS(kFaceCopy, 1024);
 
-__global__ void kFillFirst(int* first) {
+template <class GridT>
+__device__ __forceinline__ void kFillFirst(const GridT& wg, int* first) {
   if (threadIdx.x < 3) first[threadIdx.x] = INT_MAX;
 }
+ESF_BOUNDS(kFillFirst, 1024);
 
 // a sweep from a to b exclusive (the reference's sweepOnePass bounds)
 Sweep passRange(int i0, int i1, int j0, int j1, int k0, int k1) {
@@ -278,7 +293,7 @@ esf_status sweepPhiDevice(esf_ctx* ctx, const SweepBox& b, const double* mass, d
     w.ni = b.x1 - b.x0 - 1;
     w.nj = b.y1 - b.y0 - 1;
     w.nk = b.z1 - b.z0 - 1;
-    hipLaunchKernelGGL(kPhiWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, w, mass, phi);
+    launch<ESF_K(kPhiWave)>(ctx, dim3(1), dim3(kWaveThreads), b, w, mass, phi);
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -327,20 +342,19 @@ esf_status extendDevice(esf_ctx* ctx, const int box[6], const double* m, const d
     for (const Sweep& sw : passes) {
       const long cells = (long)sw.ni * sw.nj * sw.nk;
       if (cells == 0) continue;
-      hipLaunchKernelGGL(kFillFirst, dim3(1), dim3(64), 0, ctx->stream, first);
-      hipLaunchKernelGGL(kPassFlags, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
-                         ctx->stream, b, sw, m, ph, first);
-      hipLaunchKernelGGL(kPassCoef, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), 0,
-                         ctx->stream, b, sw, m, ph, (const int*)first, ctx->swCoef);
-      hipLaunchKernelGGL(kPassWave, dim3(1), dim3(kWaveThreads), 0, ctx->stream, b, sw,
-                         (const double2*)ctx->swCoef, f);
+      launch<ESF_K(kFillFirst)>(ctx, dim3(1), dim3(64), first);
+      launch<ESF_K(kPassFlags)>(ctx, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), b, sw,
+                                m, ph, first);
+      launch<ESF_K(kPassCoef)>(ctx, dim3(std::min<long>(cdiv(cells, 256), 2048)), dim3(256), b, sw,
+                               m, ph, (const int*)first, ctx->swCoef);
+      launch<ESF_K(kPassWave)>(ctx, dim3(1), dim3(kWaveThreads), b, sw, (const double2*)ctx->swCoef,
+                               f);
     }
     // the faces: z (:547-552), y (:554-558), x (:560-564)
     const long lines[3] = {(long)(y1 - y0) * (z1 - z0), (long)(x1 - x0) * (z1 - z0),
                            (long)(x1 - x0) * (y1 - y0)};
     for (int axis : {2, 1, 0})
-      hipLaunchKernelGGL(kFaceCopy, dim3(cdiv(lines[axis], 256)), dim3(256), 0, ctx->stream, b,
-                         axis, m, f);
+      launch<ESF_K(kFaceCopy)>(ctx, dim3(cdiv(lines[axis], 256)), dim3(256), b, axis, m, f);
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -376,10 +390,12 @@ namespace {
 
 // the extent counters start at (X, Y, Z, 0, 0, 0): written by a kernel,
 // so the call works from memory the host cannot address
-__global__ void kExtInit(int* ext, int X, int Y, int Z) {
+template <class GridT>
+__device__ __forceinline__ void kExtInit(const GridT& wg, int* ext, int X, int Y, int Z) {
   if (threadIdx.x < 6) ext[threadIdx.x] = threadIdx.x == 0 ? X : threadIdx.x == 1 ? Y
                                           : threadIdx.x == 2 ? Z : 0;
 }
+ESF_BOUNDS(kExtInit, 1024);
 
 // computeSDF on the device buffers dm (the mass, n doubles), dphi (n
 // doubles) and dext (6 ints), all the caller's: boxes to the host, phi back
@@ -388,13 +404,12 @@ esf_status computeSdfOn(esf_ctx* ctx, const double* mass, double* phi, int box[6
   const Dims& d = ctx->d;
   const long n = d.nodes;
   ESF_HIP(hipMemcpyAsync(dm, mass, 8 * n, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(kExtInit, dim3(1), dim3(64), 0, ctx->stream, dext, d.X, d.Y, d.Z);
-  hipLaunchKernelGGL(kSdfInit, dim3(std::min<long>(cdiv(n, 256), 4096)), dim3(256), 0,
-                     ctx->stream, n, d.X, d.Y, (const double*)dm, (double)(d.X + d.Y + d.Z + 2),
-                     dphi, dext);
+  launch<ESF_K(kExtInit)>(ctx, dim3(1), dim3(64), dext, d.X, d.Y, d.Z);
+  launch<ESF_K(kSdfInit)>(ctx, dim3(std::min<long>(cdiv(n, 256), 4096)), dim3(256), n, d.X, d.Y,
+                          (const double*)dm, (double)(d.X + d.Y + d.Z + 2), dphi, dext);
   int ext[6];
   ESF_HIP(hipMemcpyAsync(ext, dext, sizeof(ext), hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   const int res[3] = {d.X, d.Y, d.Z};
   const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
   for (int a = 0; a < 3; a++) {
@@ -413,7 +428,7 @@ esf_status computeSdfOn(esf_ctx* ctx, const double* mass, double* phi, int box[6
     }
   }
   ESF_HIP(hipMemcpyAsync(phi, dphi, 8 * n, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return ESF_OK;
 }
 
@@ -492,6 +507,6 @@ esf_status esf_extend_vector_field(esf_ctx* ctx, const int box[6], const double*
   ESF_HIP(hipMemcpyAsync(ctx->swField, field, 24 * n, hipMemcpyHostToDevice, ctx->stream));
   if ((st = extendDevice(ctx, box, ctx->swMass, ctx->swPhi, ctx->swField))) return st;
   ESF_HIP(hipMemcpyAsync(field, ctx->swField, 24 * n, hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipStreamSynchronize(ctx->stream));
+  ESF_HIP(streamSync(ctx));
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }

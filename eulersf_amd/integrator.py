@@ -452,3 +452,27 @@ def corotation_batch(F, dF, lam, mu):
         raise _abi.EsfError(st, "esf_corotation_batch")
     o = out.reshape(n, 4, 3, 3)
     return o[:, 0], o[:, 1], o[:, 2], o[:, 3], en, dP.reshape(n, 3, 3)
+
+
+def step_batch(scenes, alpha=-1.0):
+    """One substep of every scene in `scenes` (CoupledIntegrator3D objects)
+    through esf_step_batch: each kernel of the step launched once over the
+    scenes that reach it together, each scene's results bitwise those of
+    its own stepImplicit.  Returns the per-scene solve reports (with their
+    "status"); raises EsfError when a scene's step fails."""
+    lib = _abi.load()
+    n = len(scenes)
+    ctxs = (ctypes.c_void_p * n)(*[g.ctx.value for g in scenes])
+    stats = (_abi.SolveStats * n)()
+    status = (ctypes.c_int * n)()
+    st = lib.esf_step_batch(ctxs, n, alpha, stats, status)
+    out = []
+    for i in range(n):
+        d = stats[i].as_dict()
+        d["status"] = status[i]
+        out.append(d)
+    if st < 0:
+        bad = [i for i in range(n) if status[i] < 0]
+        check(scenes[bad[0]].ctx if bad else scenes[0].ctx, st, "esf_step_batch")
+    return out
+

@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _abi
 from .cfg import random_spheres, scene_cfg
-from .integrator import CoupledIntegrator3D
+from .integrator import CoupledIntegrator3D, step_batch
 
 
 def random_scene(rng, n, eps=1e-8):
@@ -70,6 +70,20 @@ class BatchRunner:
         t0 = time.perf_counter()
         for g in self.scenes:
             step_scene(g, steps, lat)
+        return time.perf_counter() - t0, lat
+
+    def run_batched(self, steps):
+        """`steps` substeps of every scene, each substep one esf_step_batch
+        call over all the scenes; the latency of a batch step is that of
+        each of its scenes' steps"""
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            for r in step_batch(self.scenes):
+                if r["status"] < 0:
+                    raise RuntimeError(f"step failed: {r}")
+            lat.append(time.perf_counter() - t1)
         return time.perf_counter() - t0, lat
 
     def run_concurrent(self, steps, workers):

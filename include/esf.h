@@ -316,6 +316,17 @@ esf_status esf_recover_and_bc(esf_ctx* ctx);
  * forces on a second stream of the context, beside the first solid solve;
  * results are the same either way. */
 esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
+/* esf_step of n contexts at once, each kernel of the step launched once
+ * over the scenes that reach it together (the reference steps its one scene
+ * per process; this batches independent scenes on one GPU).  stats[i] and
+ * status[i] (either may be NULL) receive context i's report and
+ * esf_step's return value; each context's results are bitwise those of
+ * esf_step on it alone.  The work runs on ctxs[0]'s stream, ordered after
+ * the work already queued on every context's stream and before any queued
+ * on them later.  Contexts must be distinct and on the current device; one
+ * batch runs at a time per device.  Returns the most severe status. */
+esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha, esf_solve_stats* stats,
+                          esf_status* status);
 
 /* --- operators (unit level) ---------------------------------------------*/
 /* getMatVecMult (COUPLED_INTEGRATOR_3D.cpp:638-660): y = J M^-1 J^T x on
