@@ -35,6 +35,8 @@ __global__ void kCorotBatch(int n, double lambda, double mu, const double* F, co
 esf_status gatherSolidMasses(esf_ctx* ctx);
 esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf);
 esf_status updateBoxes(esf_ctx* ctx, int solid);
+esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid);
+void updateBoxesFinish(esf_ctx* ctx, int solid);
 esf_status solidPhi(esf_ctx* ctx, int solid);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
@@ -128,6 +130,7 @@ static void freeAll(esf_ctx* ctx) {
   ctx->deviceBytes = 0;
   if (ctx->hostFlags) hipHostFree(ctx->hostFlags);
   if (ctx->hostCfl) hipHostFree(ctx->hostCfl);
+  if (ctx->hostMesh) hipHostFree(ctx->hostMesh);
   if (ctx->hostRing) hipHostFree(ctx->hostRing);
   if (ctx->hostSc) hipHostFree(ctx->hostSc);
 }
@@ -246,6 +249,9 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st && hipHostMalloc((void**)&ctx->hostFlags, 64 * sizeof(int)) != hipSuccess)
     st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostCfl, kCflBlocks * sizeof(double)) != hipSuccess)
+    st = ESF_ERR_OOM;
+  if (!st && hipHostMalloc((void**)&ctx->hostMesh, (6 * kMaxSolids + 32) * sizeof(double)) !=
+                 hipSuccess)
     st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostSc, esf_ctx::kReportSlots * sizeof(CrScalars)) !=
                  hipSuccess)
@@ -775,6 +781,7 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
   esf_status st;
   const int maxIt = ctx->p.cr_max_iterations;
   *s = esf_solve_stats{};
+  batchBarrier(ctx);
   {
     StageTimer t(ctx, 2);
     if ((st = solidAdiag(ctx))) return st;
@@ -791,6 +798,7 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
     // the fluid's body forces, if they ran beside the solve (esf_step)
     if ((st = joinBodyForces(ctx))) return st;
   }
+  batchBarrier(ctx);
   {
     StageTimer t(ctx, 3);
     if ((st = pressureRhs(ctx))) return st;
@@ -810,6 +818,7 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
     if (st) return st;
     if ((st = correctVelocity(ctx))) return st;
   }
+  batchBarrier(ctx);
   {
     StageTimer t(ctx, 2);
     if ((st = solidSolve(ctx, 2, ctx->sB, ctx->sDv, ctx->p.implicit_eps, maxIt, 2,
@@ -923,6 +932,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   // a step running alone sends the fluid's vorticity and body forces to the
   // side stream, beside the first solid solve, which needs only the solid
   // nodes' part (same results either way)
+  batchBarrier(ctx);
   {
     StageTimer t(ctx, 1);
     if ((st = cflEnqueue(ctx, alpha))) return st;
@@ -939,6 +949,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     joinBodyForces(ctx);
     return st;
   }
+  batchBarrier(ctx);
   if ((st = esf_solid_copy_velocity(ctx))) return st;
   if (ctx->p.flip) {
     // the FLIP round trip: G2P, P2G, then the solid velocity extended over
@@ -957,23 +968,30 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     }
   }
   if ((st = esf_solid_finalize(ctx))) return st;
+  batchBarrier(ctx);
   {
     // computeSolidMass(true) of every solid, the surface meshes, then
-    // computeSDF's boxes and phi (COUPLED_INTEGRATOR_3D.cpp:857-879)
+    // computeSDF's boxes (all solids' read back at one synchronisation) and
+    // phi (COUPLED_INTEGRATOR_3D.cpp:857-879)
     StageTimer t(ctx, 9);
     for (int b = 0; b < (int)ctx->solids.size(); b++)
       if ((st = computeSolidMass(ctx, b, true))) return st;
     if ((st = advectMeshes(ctx))) return st;
+    for (int b = 0; b < (int)ctx->solids.size(); b++)
+      if ((st = updateBoxesEnqueue(ctx, b))) return st;
+    ESF_HIP(streamSync(ctx));
     for (int b = 0; b < (int)ctx->solids.size(); b++) {
-      if ((st = updateBoxes(ctx, b))) return st;
+      updateBoxesFinish(ctx, b);
       if ((st = solidPhi(ctx, b))) return st;
     }
   }
+  batchBarrier(ctx);
   if ((st = esf_gather_solid_masses(ctx))) return st;
   // initMaterialParticles on the new valid cells (:870-877)
   if (ctx->p.flip)
     for (int b = 0; b < (int)ctx->solids.size(); b++)
       if ((st = initParticles(ctx, b))) return st;
+  batchBarrier(ctx);
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;

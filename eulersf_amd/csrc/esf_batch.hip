@@ -15,6 +15,10 @@
 // kernels compute what they would in its own launches (esf_launch.h), and
 // every host decision of its step reads the same device values, so each
 // scene's results are bitwise those of esf_step on that scene alone.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -26,13 +30,19 @@
 
 namespace esf {
 
+const char kMarkKeyObj = 0;
+
+struct BatchRun;
+
 struct SceneFiber {
   Fiber f;
   esf_ctx* ctx = nullptr;
-  enum State { RUNNABLE, DEPOSITED, WAITING, FINISHED } state = RUNNABLE;
+  BatchRun* run = nullptr;
+  enum State { RUNNABLE, DEPOSITED, WAITING, BARRIER, FINISHED } state = RUNNABLE;
+  long depRound = 0;          // DEPOSITED: the round of the deposit
   hipEvent_t wait = nullptr;  // WAITING: the event it waits for
   hipError_t waitResult = hipSuccess;
-  hipEvent_t own = nullptr;   // recorded by streamSync
+  hipEvent_t own = nullptr;   // orders the batch stream with the context's own
   double alpha = -1;
   esf_solve_stats stats{};
   esf_status status = ESF_OK;
@@ -42,6 +52,7 @@ struct SceneFiber {
 constexpr size_t kStageChunk = 4u << 20;  // bytes per chunk of the staging ring
 constexpr int kStageChunks = 4;
 constexpr size_t kFiberStack = 2u << 20;
+constexpr int kMarkEvents = 256;  // events the scheduler records for the fibers' marks
 
 struct BatchRun {
   hipStream_t stream = nullptr;
@@ -52,8 +63,23 @@ struct BatchRun {
   hipEvent_t chunkDone[kStageChunks] = {};
   bool chunkPending[kStageChunks] = {};
   hipError_t err = hipSuccess;
-  long launches = 0, scenesLaunched = 0;
+  // mark events, recorded round-robin: re-recording one that a fiber still
+  // holds only makes that fiber wait longer, never less; `known` caches the
+  // last query of each (round, result), reset when it is recorded again
+  hipEvent_t marks[kMarkEvents] = {};
+  long markRound[kMarkEvents] = {};
+  hipError_t markKnown[kMarkEvents] = {};
+  int nextMark = 0;
+  std::unordered_map<hipEvent_t, int> markIndex;
+  // host time of the batch (ESF_BATCH_TRACE=1 prints it per call)
+  long rounds = 0, launches = 0, scenesLaunched = 0, waits = 0, markRecords = 0;
+  double tScenes = 0, tSubmit = 0, tWait = 0;
 };
+
+static double nowS() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
 
 // per device: the staging ring and the fibers, kept between calls; one
 // batch at a time per device
@@ -85,8 +111,22 @@ void fiberDeposit(esf_ctx* ctx) {
   esf_fiber_yield(&s->f);
 }
 
+// the result of querying `ev` in this round (one query per mark event per
+// round; a completed mark stays completed until it is recorded again)
+static hipError_t cachedQuery(BatchRun& run, hipEvent_t ev) {
+  auto it = run.markIndex.find(ev);
+  if (it == run.markIndex.end()) return hipEventQuery(ev);
+  const int k = it->second;
+  if (run.markKnown[k] == hipSuccess || run.markRound[k] == run.rounds) return run.markKnown[k];
+  run.markKnown[k] = hipEventQuery(ev);
+  run.markRound[k] = run.rounds;
+  return run.markKnown[k];
+}
+
+hipError_t fiberQuery(esf_ctx* ctx, hipEvent_t ev) { return cachedQuery(*ctx->fiber->run, ev); }
+
 hipError_t fiberWait(esf_ctx* ctx, hipEvent_t ev) {
-  const hipError_t q = hipEventQuery(ev);
+  const hipError_t q = fiberQuery(ctx, ev);
   if (q != hipErrorNotReady) return q;
   SceneFiber* s = ctx->fiber;
   s->wait = ev;
@@ -95,7 +135,22 @@ hipError_t fiberWait(esf_ctx* ctx, hipEvent_t ev) {
   return s->waitResult;
 }
 
-hipEvent_t fiberEvent(esf_ctx* ctx) { return ctx->fiber->own; }
+void fiberBarrier(esf_ctx* ctx) {
+  SceneFiber* s = ctx->fiber;
+  s->state = SceneFiber::BARRIER;
+  esf_fiber_yield(&s->f);
+}
+
+hipEvent_t fiberMark(esf_ctx* ctx) {
+  SceneFiber* s = ctx->fiber;
+  s->op.key = kMarkKey;
+  s->op.threads = 0;
+  s->op.blocks = 0;
+  s->op.mark = nullptr;
+  s->op.name = "mark";
+  fiberDeposit(ctx);
+  return s->op.mark;
+}
 
 // ---------------------------------------------------------------------------
 // staging ring
@@ -144,44 +199,49 @@ static void sceneMain(void* p) {
   s->status = esf_step(s->ctx, s->alpha, &s->stats);
 }
 
+static int traceLevel() {
+  static const int t = getenv("ESF_BATCH_TRACE") ? atoi(getenv("ESF_BATCH_TRACE")) : 0;
+  return t;
+}
+
+// ESF_BATCH_TRACE=2: one line per launch group
+static void traceGroup(const std::vector<SceneFiber*>& group, size_t held, long round) {
+  const char* nm = group[0]->op.name;
+  const char* f = strstr(nm, "FH = ");
+  std::string k = f ? std::string(f + 5, strcspn(f + 5, ",;]")) : std::string(nm);
+  fprintf(stderr, "  round %ld: %zu scenes (%zu held)  %s\n", round, group.size(), held,
+          k.c_str());
+}
+
+// One group per round.  Of the deposits held, the group of equal deposits
+// (same kernel instance and block size, or mark requests) whose oldest
+// deposit is the most recent goes first, the larger of equally recent ones:
+// fibers that ran on while others did extra work (more solids, a wait for
+// their own poll) hold their deposits until the others catch up, so the
+// scenes' launches fall back into step instead of splitting.  A fiber that
+// waits is woken once its event has completed; when no fiber holds a
+// deposit the scheduler blocks on the oldest wait.  The step's barriers
+// (batchBarrier) release once every live fiber has reached one.
 static void schedule(BatchRun& run, const std::vector<SceneFiber*>& fs) {
-  std::vector<SceneFiber*> dep;
-  std::vector<FiberOp*> group;
-  std::vector<char> used;
+  std::vector<SceneFiber*> dep, group;
+  std::vector<FiberOp*> ops;
   for (;;) {
+    double t0 = nowS();
     for (SceneFiber* s : fs)
       if (s->state == SceneFiber::RUNNABLE) {
         esf_fiber_resume(&s->f);
         if (s->f.finished) s->state = SceneFiber::FINISHED;
+        if (s->state == SceneFiber::DEPOSITED) s->depRound = run.rounds;
       }
-    dep.clear();
-    for (SceneFiber* s : fs)
-      if (s->state == SceneFiber::DEPOSITED) dep.push_back(s);
-    if (!dep.empty()) {
-      // one launch per (kernel instance, block size), scenes in order
-      used.assign(dep.size(), 0);
-      for (size_t i = 0; i < dep.size(); i++) {
-        if (used[i]) continue;
-        group.clear();
-        for (size_t j = i; j < dep.size(); j++)
-          if (!used[j] && dep[j]->op.key == dep[i]->op.key &&
-              dep[j]->op.threads == dep[i]->op.threads) {
-            group.push_back(&dep[j]->op);
-            used[j] = 1;
-          }
-        group[0]->submit(run, group.data(), (int)group.size());
-        run.launches++;
-        run.scenesLaunched += (long)group.size();
-      }
-      for (SceneFiber* s : dep) s->state = SceneFiber::RUNNABLE;
-      continue;
-    }
-    // nothing deposited: every live fiber waits for the device
+    double t1 = nowS();
+    run.tScenes += t1 - t0;
+    run.rounds++;
+    // waits whose events have completed
     SceneFiber* oldest = nullptr;
     bool woke = false;
     for (SceneFiber* s : fs) {
       if (s->state != SceneFiber::WAITING) continue;
-      const hipError_t q = hipEventQuery(s->wait);
+      const hipError_t q = cachedQuery(run, s->wait);
       if (q == hipErrorNotReady) {
         if (!oldest) oldest = s;
         continue;
@@ -190,10 +250,73 @@ static void schedule(BatchRun& run, const std::vector<SceneFiber*>& fs) {
       s->state = SceneFiber::RUNNABLE;
       woke = true;
     }
-    if (woke) continue;
-    if (!oldest) return;  // all finished
-    oldest->waitResult = hipEventSynchronize(oldest->wait);
-    oldest->state = SceneFiber::RUNNABLE;
+    if (woke) {
+      // the woken fibers deposit before a group is chosen
+      run.tWait += nowS() - t1;
+      continue;
+    }
+    dep.clear();
+    for (SceneFiber* s : fs)
+      if (s->state == SceneFiber::DEPOSITED) dep.push_back(s);
+    if (dep.empty()) {
+      if (!oldest) {
+        // every live fiber is at a barrier (or none is left)
+        bool any = false;
+        for (SceneFiber* s : fs)
+          if (s->state == SceneFiber::BARRIER) {
+            s->state = SceneFiber::RUNNABLE;
+            any = true;
+          }
+        if (!any) return;
+        continue;
+      }
+      oldest->waitResult = hipEventSynchronize(oldest->wait);
+      oldest->state = SceneFiber::RUNNABLE;
+      if (traceLevel() > 1) fprintf(stderr, "  round %ld: blocking wait\n", run.rounds);
+      run.waits++;
+      run.tWait += nowS() - t1;
+      continue;
+    }
+    // the group whose oldest deposit is the most recent; the larger on a tie
+    size_t best = 0, bestSize = 0;
+    long bestFirst = -1;
+    for (size_t i = 0; i < dep.size(); i++) {
+      size_t k = 0;
+      long first = dep[i]->depRound;
+      for (size_t j = 0; j < dep.size(); j++)
+        if (dep[j]->op.key == dep[i]->op.key && dep[j]->op.threads == dep[i]->op.threads) {
+          k++;
+          first = std::min(first, dep[j]->depRound);
+        }
+      if (first > bestFirst || (first == bestFirst && k > bestSize)) {
+        bestFirst = first;
+        bestSize = k;
+        best = i;
+      }
+    }
+    group.clear();
+    for (SceneFiber* s : dep)
+      if (s->op.key == dep[best]->op.key && s->op.threads == dep[best]->op.threads)
+        group.push_back(s);
+    if (traceLevel() > 1) traceGroup(group, dep.size() - group.size(), run.rounds);
+    if (group[0]->op.key == kMarkKey) {
+      const int k = run.nextMark;
+      run.nextMark = (k + 1) % kMarkEvents;
+      const hipError_t e = hipEventRecord(run.marks[k], run.stream);
+      if (e != hipSuccess) run.err = e;
+      run.markKnown[k] = hipErrorNotReady;
+      run.markRound[k] = -1;
+      for (SceneFiber* s : group) s->op.mark = run.marks[k];
+      run.markRecords++;
+    } else {
+      ops.clear();
+      for (SceneFiber* s : group) ops.push_back(&s->op);
+      ops[0]->submit(run, ops.data(), (int)ops.size());
+      run.launches++;
+      run.scenesLaunched += (long)ops.size();
+    }
+    for (SceneFiber* s : group) s->state = SceneFiber::RUNNABLE;
+    run.tSubmit += nowS() - t1;
   }
 }
 
@@ -211,6 +334,13 @@ static esf_status prepare(BatchPool& pool, int n, std::string& why) {
         why = "batch staging: event";
         return ESF_ERR_DEVICE;
       }
+    for (int k = 0; k < kMarkEvents; k++) {
+      if (hipEventCreateWithFlags(&run.marks[k], hipEventDisableTiming) != hipSuccess) {
+        why = "batch: event";
+        return ESF_ERR_DEVICE;
+      }
+      run.markIndex[run.marks[k]] = k;
+    }
     pool.ready = true;
   }
   while ((int)pool.fibers.size() < n) {
@@ -268,6 +398,13 @@ extern "C" esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha,
   run.chunk = 0;
   for (bool& p : run.chunkPending) p = false;
   run.err = hipSuccess;
+  run.rounds = run.launches = run.scenesLaunched = run.waits = run.markRecords = 0;
+  for (int k = 0; k < kMarkEvents; k++) {
+    run.markKnown[k] = hipSuccess;
+    run.markRound[k] = -1;
+  }
+  run.tScenes = run.tSubmit = run.tWait = 0;
+  const double tStart = nowS();
   // all scenes' work on the batch stream; timing off for the batch
   struct Saved {
     hipStream_t stream;
@@ -281,6 +418,7 @@ extern "C" esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha,
     SceneFiber* s = pool->fibers[i].get();
     esf_fiber_reset(&s->f, sceneMain, s);
     s->ctx = c;
+    s->run = &run;
     s->alpha = alpha;
     s->state = SceneFiber::RUNNABLE;
     s->stats = esf_solve_stats{};
@@ -316,6 +454,14 @@ extern "C" esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha,
     if (si < 0 && (worst >= 0 || si < worst)) worst = si;
     if (si > 0 && worst == ESF_OK) worst = si;
   }
+  if (traceLevel() > 0)
+    fprintf(stderr,
+            "[esf_step_batch] %d scenes: %.3f ms, %ld rounds, %ld launches (%.1f scenes each), "
+            "%ld marks, %ld blocking waits; host ms: scenes %.3f, submit %.3f, waits %.3f\n",
+            n, 1e3 * (nowS() - tStart), run.rounds, run.launches,
+            run.launches ? (double)run.scenesLaunched / run.launches : 0.0, run.markRecords,
+            run.waits,
+            1e3 * run.tScenes, 1e3 * run.tSubmit, 1e3 * run.tWait);
   if (run.err != hipSuccess) {
     ctxs[0]->err = std::string("batched launch: ") + hipGetErrorString(run.err);
     return ESF_ERR_DEVICE;

@@ -112,8 +112,7 @@ esf_status compact(esf_ctx* ctx, long n, Flag flag, Emit emit, int* total) {
   launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
   launch<ESF_KT(kScanEmit, Flag, Emit)>(ctx, dim3(nb), dim3(256), n, flag,
                                         (const int*)ctx->scanBlocks, emit);
-  ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
-                         hipMemcpyDeviceToHost, ctx->stream));
+  toHostAsync(ctx, ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int));
   ESF_HIP(streamSync(ctx));
   *total = ctx->hostFlags[0];
   return ESF_OK;
@@ -169,8 +168,12 @@ __device__ __forceinline__ void kGatherMass(const GridT& wg, Dims d, SolidPtrs s
   const long t = wg.block() * (long)blockDim.x + threadIdx.x;
   if (t >= g.count()) return;
   const long n = g.node(d, t);
+  // (loops over the solids unrolled: constant indices into the argument
+  // arrays, so the arguments stay in registers)
   double m = 0;
-  for (int b = 0; b < sp.n; b++) m += sp.mass[b][n];
+#pragma unroll
+  for (int b = 0; b < kMaxSolids; b++)
+    if (b < sp.n) m += sp.mass[b][n];
   solidMass[n] = m;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   double f = 0;
@@ -182,7 +185,9 @@ __device__ __forceinline__ void kGatherMass(const GridT& wg, Dims d, SolidPtrs s
     if (!isInterior(d, cx, cy, cz)) continue;
     const long c = cx + (long)cy * d.X + (long)cz * d.slab;
     double q = 0;
-    for (int b = 0; b < sp.n; b++) q += sp.qfrac[b][8 * c + i];
+#pragma unroll
+    for (int b = 0; b < kMaxSolids; b++)
+      if (b < sp.n) q += sp.qfrac[b][8 * c + i];
     f += q;
   }
   nodeFrac[n] = f * 0.125;
@@ -243,8 +248,7 @@ esf_status buildValidList(esf_ctx* ctx, DevSolid& so) {
   const int nb = std::max(cdiv(n, kScanTile), 1);
   launch<ESF_KT(kScanCount, ValidFlag)>(ctx, dim3(nb), dim3(256), n, fl, ctx->scanBlocks);
   launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
-  ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
-                         hipMemcpyDeviceToHost, ctx->stream));
+  toHostAsync(ctx, ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int));
   ESF_HIP(streamSync(ctx));
   total = ctx->hostFlags[0];
   esf_status st = growSolidCells(ctx, so, total);
@@ -307,8 +311,7 @@ esf_status gatherSolidMasses(esf_ctx* ctx) {
   const int nb = std::max(cdiv(n, kScanTile), 1);
   launch<ESF_KT(kScanCount, SolidFlag)>(ctx, dim3(nb), dim3(256), n, fl, ctx->scanBlocks);
   launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
-  ESF_HIP(hipMemcpyAsync(ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int),
-                         hipMemcpyDeviceToHost, ctx->stream));
+  toHostAsync(ctx, ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int));
   ESF_HIP(streamSync(ctx));
   const int total = ctx->hostFlags[0];
   esf_status st = growSolidDofs(ctx, total);
@@ -396,60 +399,95 @@ __device__ inline bool inTight(const Box& b, int x, int y, int z) {
          z >= b.tMin[2] && z < b.tMax[2] - 1;
 }
 
-// clear q, _valid and |det F^-1| over the cells of box u (the reference
-// clears the whole fields, :578-581; outside u they are already 0)
+// clear q and _valid over the cells of box u (the reference clears the
+// whole fields, :578-581; outside u they are already 0), and |det F^-1|:
+// computeF's determinant at the cells of the tight box t (:613-619), 0 at
+// the others of u
 template <class GridT>
-__device__ __forceinline__ void kMassClear(const GridT& wg, Dims d, BoxRange u, double* qfrac,
-                                           int8_t* valid, double* detFInv) {
-  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
-  if (t >= u.count()) return;
+__device__ __forceinline__ void kMassClear(const GridT& wg, Dims d, BoxRange u, BoxRange t,
+                                           const double* uField, double* qfrac, int8_t* valid,
+                                           double* detFInv) {
+  const long k = wg.block() * (long)blockDim.x + threadIdx.x;
+  if (k >= u.count()) return;
   int x, y, z;
-  u.at(t, x, y, z);
+  u.at(k, x, y, z);
   const long index = x + (long)y * d.X + (long)z * d.slab;
   for (int i = 0; i < 8; i++) qfrac[8 * index + i] = 0.0;
-  detFInv[index] = 0.0;
+  const bool inT = x >= t.lo[0] && x < t.hi[0] && y >= t.lo[1] && y < t.hi[1] && z >= t.lo[2] &&
+                   z < t.hi[2];
+  detFInv[index] = inT ? fabs(1.0 / detF(d, uField, index)) : 0.0;
   valid[x + (long)y * d.CX + (long)z * d.cslab] = 0;
 }
 ESF_BOUNDS(kMassClear, 1024);
 
-// one wave per tight-box cell: |det F^-1| (lane 0), and the cell's
-// msr^3 subsamples spread over the lanes (sample s = (p msr + q) msr + r in
-// the reference's loop order, lane l taking s = l, l + 64, ...); per octant
-// the count of samples that carry mass comes from wave ballots.  The
-// reference adds `fraction` to the octant's q once per such sample, so q is
-// that many additions of one constant, which depends on the count only:
-// lane 0 performs them, then the sum, the 0.5 threshold and the flags.
+// one wave per tight-box cell: the cell's msr^3 subsamples spread over the
+// lanes (sample s = (p msr + q) msr + r in the reference's loop order, lane
+// l taking s = l, l + 64, ...); per octant the count of samples that carry
+// mass comes from wave ballots.  The reference adds `fraction` to the
+// octant's q once per such sample, so q is that many additions of one
+// constant, which depends on the count only: lane 0 performs them, then the
+// sum, the 0.5 threshold and the flags.  The cell's phi and the first
+// samples' interpolation of X are loaded together, so a cell that phi
+// skips costs one round of loads.
 template <class GridT>
 __device__ __forceinline__ void kMassSamples(const GridT& wg, Dims d, MassArgs a, BoxRange t) {
-  const long wave = (wg.block() * (long)blockDim.x + threadIdx.x) >> 6;
+  const long wave = __builtin_amdgcn_readfirstlane(
+      (int)((wg.block() * (long)blockDim.x + threadIdx.x) >> 6));
   const int lane = threadIdx.x & 63;
   if (wave >= t.count()) return;  // uniform per wave
   int x, y, z;
   t.at(wave, x, y, z);
   const long index = x + (long)y * d.X + (long)z * d.slab;
   const long c = x + (long)y * d.CX + (long)z * d.cslab;
-  if (lane == 0) a.detFInv[index] = fabs(1.0 / detF(d, a.u, index));
-  // usesdf (DEFOR_SOLID_3D.cpp:620): no samples, q and _valid stay cleared
-  if (a.phi && a.phi[index] > 2) return;
   const FieldGeom g = gridGeom(d);
   const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
   const int m2 = a.msr * a.msr, m3 = m2 * a.msr;
+  // every sample lies inside node cell (x, y, z) (its offset from the cell's
+  // low corner is between sdh / 2 and dh - sdh / 2 per axis), so X's
+  // interpolation reads the same 8 corners for all of them: loaded once
+  double cv[3][8];
+  {
+    const long X = d.X, S = d.slab;
+    const long off[8] = {0, 1, X, X + 1, S, S + 1, S + X, S + X + 1};
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) cv[c][k] = a.X[3 * (index + off[k]) + c];
+  }
+  // sample s's position and its rest position X(pos)
+  auto restPos = [&](int s, double (&rPos)[3], int& o) {
+    const int p = s / m2, qq = (s / a.msr) % a.msr, r = s % a.msr;
+    const double pos[3] = {node[0] + (p + 0.5) * a.sdh, node[1] + (qq + 0.5) * a.sdh,
+                           node[2] + (r + 0.5) * a.sdh};
+    interpVCell(cv, g, pos, rPos);
+    o = p < a.half ? 0 : 1;
+    o += qq < a.half ? 0 : 2;
+    o += r < a.half ? 0 : 4;
+  };
+  const int s0 = lane < m3 ? lane : m3 - 1;
+  double rPos0[3];
+  int o0;
+  const double ph = a.phi ? a.phi[index] : 0.0;
+  restPos(s0, rPos0, o0);
+  // both rounds of loads above are issued before the branch on phi
+  asm volatile("" ::: "memory");
+  // usesdf (DEFOR_SOLID_3D.cpp:620): no samples, q and _valid stay cleared
+  if (ph > 2) return;
   int count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int base = 0; base < m3; base += 64) {
     const int s = base + lane;
     bool hit = false;
     int o = 0;
     if (s < m3) {
-      const int p = s / m2, qq = (s / a.msr) % a.msr, r = s % a.msr;
-      const double pos[3] = {node[0] + (p + 0.5) * a.sdh, node[1] + (qq + 0.5) * a.sdh,
-                             node[2] + (r + 0.5) * a.sdh};
       double rPos[3];
-      interpV(a.X, g, pos, rPos);
+      if (base == 0) {
+        for (int k = 0; k < 3; k++) rPos[k] = rPos0[k];
+        o = o0;
+      } else {
+        restPos(s, rPos, o);
+      }
       const double dd = interpS(a.rest, a.rg, rPos) * a.subV;
       hit = !(dd < 1e-6);
-      o = p < a.half ? 0 : 1;
-      o += qq < a.half ? 0 : 2;
-      o += r < a.half ? 0 : 4;
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) count[k] += __popcll(__ballot(hit && o == k));
@@ -512,8 +550,8 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   const BoxRange T = spanBoxes(tc, none, climit);
   const BoxRange U = spanBoxes(so.dirtyAll ? fullCells : so.dirty, tc, climit);
   if (U.count() > 0)
-    launch<ESF_K(kMassClear)>(ctx, dim3(cdiv(U.count(), 256)), dim3(256), d, U, so.qfrac, so.valid,
-                              so.detFInv);
+    launch<ESF_K(kMassClear)>(ctx, dim3(cdiv(U.count(), 256)), dim3(256), d, U, T,
+                              (const double*)so.u, so.qfrac, so.valid, so.detFInv);
   MassArgs a{};
   a.box = so.box;
   a.u = so.u;
@@ -585,22 +623,56 @@ __device__ __forceinline__ void kMassBox(const GridT& wg, Dims d, const double* 
 }
 ESF_BOUNDS(kMassBox, 1024);
 
-esf_status updateBoxes(esf_ctx* ctx, int solid) {
+// the per-block boxes of kMassBox into one (blocks with no massive node,
+// max -1, left out; none at all: min = resolution, max = 0), as the host
+// would, into `out` (6 ints, pinned host memory)
+template <class GridT>
+__device__ __forceinline__ void kMassBoxFinish(const GridT& wg, const int* part, int nb, int X,
+                                               int Y, int Z, int* out) {
+  __shared__ int s[6][256];
+  int mn[3] = {X, Y, Z}, mx[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int k = 0; k < 3; k++) {
+      if (part[6 * b + 3 + k] < 0) continue;
+      mn[k] = min(mn[k], part[6 * b + k]);
+      mx[k] = max(mx[k], part[6 * b + 3 + k]);
+    }
+  for (int k = 0; k < 3; k++) {
+    s[k][threadIdx.x] = mn[k];
+    s[3 + k][threadIdx.x] = mx[k];
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = min(s[k][threadIdx.x], s[k][threadIdx.x + st]);
+        s[3 + k][threadIdx.x] = max(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+}
+ESF_BOUNDS(kMassBoxFinish, 256);
+
+// computeSDF's boxes (DEFOR_SOLID_3D.cpp:227-241): the bounding box of the
+// nodes with mass, widened by the tight border and the extrapolation cells.
+// Enqueue writes the box of solid b into pinned hostFlags[8 + 6 b ...];
+// finish reads it after a synchronisation.
+esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid) {
   DevSolid& so = ctx->solids[solid];
   const int nb = 256;
-  launch<ESF_K(kMassBox)>(ctx, dim3(nb), dim3(256), ctx->d, so.mass, (int*)ctx->partials);
-  std::vector<int> h(6 * nb);
-  ESF_HIP(hipMemcpyAsync(h.data(), ctx->partials, h.size() * sizeof(int), hipMemcpyDeviceToHost,
-                         ctx->stream));
-  ESF_HIP(streamSync(ctx));
+  double* part = ctx->partials + 6L * nb * solid;  // one partials region per solid
+  launch<ESF_K(kMassBox)>(ctx, dim3(nb), dim3(256), ctx->d, so.mass, (int*)part);
+  launch<ESF_K(kMassBoxFinish)>(ctx, dim3(1), dim3(256), (const int*)part, nb, ctx->d.X,
+                                ctx->d.Y, ctx->d.Z, ctx->hostFlags + 8 + 6 * solid);
+  return ESF_OK;
+}
+
+void updateBoxesFinish(esf_ctx* ctx, int solid) {
+  DevSolid& so = ctx->solids[solid];
   const Dims& d = ctx->d;
-  int mn[3] = {d.X, d.Y, d.Z}, mx[3] = {0, 0, 0};
-  for (int b = 0; b < nb; b++)
-    for (int k = 0; k < 3; k++) {
-      if (h[6 * b + 3 + k] < 0) continue;
-      mn[k] = std::min(mn[k], h[6 * b + k]);
-      mx[k] = std::max(mx[k], h[6 * b + 3 + k]);
-    }
+  const int* h = ctx->hostFlags + 8 + 6 * solid;
+  const int mn[3] = {h[0], h[1], h[2]}, mx[3] = {h[3], h[4], h[5]};
   const int res[3] = {d.X, d.Y, d.Z};
   const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
   for (int k = 0; k < 3; k++) {
@@ -610,6 +682,13 @@ esf_status updateBoxes(esf_ctx* ctx, int solid) {
     so.box.bMax[k] = std::min(mx[k] + ex, res[k]);
   }
   so.listDirty = true;
+}
+
+esf_status updateBoxes(esf_ctx* ctx, int solid) {
+  esf_status st = updateBoxesEnqueue(ctx, solid);
+  if (st) return st;
+  ESF_HIP(streamSync(ctx));
+  updateBoxesFinish(ctx, solid);
   return ESF_OK;
 }
 
@@ -739,16 +818,15 @@ esf_status cflEnqueue(esf_ctx* ctx, double alpha) {
   const int nb = cflBlocks(ctx);
   launch<ESF_K(kCfl)>(ctx, dim3(nb), dim3(256), ctx->nSolid, ctx->solidNodes, ctx->vel,
                       ctx->d.dhInv, alpha, ctx->partials);
-  ESF_HIP(hipMemcpyAsync(ctx->hostCfl, ctx->partials, nb * sizeof(double),
-                         hipMemcpyDeviceToHost, ctx->stream));
-  ESF_HIP(hipEventRecord(ctx->evCfl, ctx->stream));
+  toHostAsync(ctx, ctx->hostCfl, ctx->partials, nb * sizeof(double));
+  ctx->cflMark = markEvent(ctx, ctx->evCfl);
   return ESF_OK;
 }
 
 esf_status cflFinish(esf_ctx* ctx, double* dtOut) {
   double dt = 1e9;
   if (ctx->nSolid > 0) {
-    ESF_HIP(eventSync(ctx, ctx->evCfl));
+    ESF_HIP(eventSync(ctx, ctx->cflMark));
     const int nb = cflBlocks(ctx);
     for (int i = 0; i < nb; i++) dt = std::min(dt, ctx->hostCfl[i]);
   }
@@ -1257,7 +1335,9 @@ __device__ __forceinline__ void kRecoverBC(const GridT& wg, Dims d, RecoverArgs 
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
-  for (int b = 0; b < a.n; b++) {
+#pragma unroll
+  for (int b = 0; b < kMaxSolids; b++) {
+    if (b >= a.n) break;
     const Box& bx = a.box[b];
     if (x >= bx.bMin[0] && x < bx.bMax[0] - 1 && y >= bx.bMin[1] && y < bx.bMax[1] - 1 &&
         z >= bx.bMin[2] && z < bx.bMax[2] - 1 && a.mass[b][n] > 0)

@@ -185,6 +185,8 @@ struct esf_ctx {
   int* hostFlags = nullptr;       // pinned: counts read back by the host
   double* hostCfl = nullptr;      // pinned: the CFL kernel's block minima
   hipEvent_t evCfl = nullptr;     // their copy has landed
+  hipEvent_t cflMark = nullptr;   // the event cflFinish waits for (evCfl alone)
+  double* hostMesh = nullptr;     // pinned: root boxes and pair flags of the broad phase
   esf::CrScalars* hostRing = nullptr;  // pinned ring of polled scalar blocks
   int crQueued = 0;  // iterations the last pressure CR solve queued
   // pressure SpMV launch shape (fixed at creation): cell layers per block,

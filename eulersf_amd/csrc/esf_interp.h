@@ -65,6 +65,22 @@ __device__ __forceinline__ void interpV(const double* d, const FieldGeom& g, con
   }
 }
 
+// interpV at a point p inside node cell b (the cell locate(p) finds), with
+// the cell's 24 corner values given: cv[c][k] = d[3 (b + off_k) + c] for the
+// corners k in interpV's order (b, b+1, b+X, b+X+1, then the same + S).
+// The same arithmetic as interpV, without its loads.
+__device__ __forceinline__ void interpVCell(const double (&cv)[3][8], const FieldGeom& g,
+                                            const double* p, double* out) {
+  double fx, fy, fz;
+  (void)locate(g, p, fx, fy, fz);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double z0 = lerp(lerp(cv[c][0], cv[c][1], fx), lerp(cv[c][2], cv[c][3], fx), fy);
+    const double z1 = lerp(lerp(cv[c][4], cv[c][5], fx), lerp(cv[c][6], cv[c][7], fx), fy);
+    out[c] = lerp(z0, z1, fz);
+  }
+}
+
 __device__ __forceinline__ FieldGeom gridGeom(const Dims& d) {
   return FieldGeom{d.X, d.Y, d.Z, d.dh, d.o0, d.o1, d.o2};
 }

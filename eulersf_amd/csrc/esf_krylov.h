@@ -158,10 +158,11 @@ struct DonePoller {
   double best = 0;
   bool stagnated = false;
   int stopIt = -1;  // stagnated: the last iteration the solve runs
-  hipEvent_t ev[kRing];
+  hipEvent_t own[kRing];  // the context's events for the marks
+  hipEvent_t ev[kRing];   // the event marking each check (own[] alone)
   explicit DonePoller(esf_ctx* c, int w, bool stag = false, int lagIters = 2, int flatPolls = 4)
       : ctx(c), which(w), stagnation(stag), lag(lagIters), flatMax(flatPolls) {
-    for (int i = 0; i < kRing; i++) ev[i] = ctx->ev[8 + i];
+    for (int i = 0; i < kRing; i++) own[i] = ev[i] = ctx->ev[8 + i];
   }
   bool seen(int ws) {
     const CrScalars& h = ctx->hostRing[ws];
@@ -183,7 +184,7 @@ struct DonePoller {
   void mark(int it) {
     const int slot = issued % kRing;
     markIt[slot] = it;
-    (void)hipEventRecord(ev[slot], ctx->stream);
+    ev[slot] = markEvent(ctx, own[slot]);
     issued++;
   }
   // after iteration `it` is queued: true when the solve is known to be done
@@ -193,7 +194,7 @@ struct DonePoller {
       const int ws = waited % kRing;
       if (markIt[ws] + lag <= it || issued - waited >= kRing) {
         (void)eventSync(ctx, ev[ws]);
-      } else if (hipEventQuery(ev[ws]) != hipSuccess) {
+      } else if (queryEvent(ctx, ev[ws]) != hipSuccess) {
         break;
       }
       waited++;

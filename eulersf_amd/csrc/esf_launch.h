@@ -15,6 +15,7 @@
 // esf_step_batch, hands the launch to the batch scheduler, which puts the
 // launches that the scenes reach together into one kMany<kFoo> launch.
 #pragma once
+#include <algorithm>
 #include <cstddef>
 #include <cstring>
 #include <utility>
@@ -56,16 +57,25 @@ inline ArgPack<H, N, T...> makePack(const H& h, const N& n, const T&... t) {
   return p;
 }
 
+// the packs a launch over several scenes reads are constant for the whole
+// kernel: read through the constant address space, so a by-value argument
+// copied out of a pack needs no private copy (the compiler reads the pack
+// in place, as it reads a kernel's own arguments)
+#define ESF_CONST __attribute__((address_space(4)))
+
 template <std::size_t I, typename H, typename... T>
-__device__ __forceinline__ const auto& packGet(const ArgPack<H, T...>& p) {
-  if constexpr (I == 0)
-    return p.h;
-  else
+__device__ __forceinline__ auto packGet(const ESF_CONST ArgPack<H, T...>& p) {
+  if constexpr (I == 0) {
+    H v;
+    __builtin_memcpy(&v, (const ESF_CONST void*)&p.h, sizeof(H));
+    return v;
+  } else {
     return packGet<I - 1>(p.t);
+  }
 }
 
 template <auto F, typename G, typename... P, std::size_t... I>
-__device__ __forceinline__ void callPack(const G& g, const ArgPack<P...>& p,
+__device__ __forceinline__ void callPack(const G& g, const ESF_CONST ArgPack<P...>& p,
                                          std::index_sequence<I...>) {
   F(g, packGet<I>(p)...);
 }
@@ -100,7 +110,8 @@ __global__ void __launch_bounds__(LB) kMany(const unsigned char* __restrict__ ta
   }
   s = __builtin_amdgcn_readfirstlane(s);
   const int first = start[s], count = start[n + s];
-  const ArgPack<P...>& pk = *(const ArgPack<P...>*)(tab + packOff + (long)s * stride);
+  const ESF_CONST ArgPack<P...>& pk =
+      *(const ESF_CONST ArgPack<P...>*)(tab + packOff + (long)s * stride);
   callPack<F>(SceneGrid{g - first, count}, pk, std::index_sequence_for<P...>{});
 }
 
@@ -110,21 +121,28 @@ __global__ void __launch_bounds__(LB) kMany(const unsigned char* __restrict__ ta
 constexpr int kMaxPack = 1024;  // bytes of one launch's arguments
 
 struct BatchRun;
-// one launch a scene's fiber hands to the scheduler
+// what a scene's fiber hands to the scheduler: a launch, or (key ==
+// kMarkKey) a request for an event that marks its work queued so far
 struct FiberOp {
   const void* key;  // the kMany instance
   int blocks, threads;
   int packBytes;
   // launch kMany for `n` scenes' ops of the same key and block size
   void (*submit)(BatchRun& run, FiberOp* const* ops, int n);
+  hipEvent_t mark;  // kMarkKey: the event the scheduler recorded
+  const char* name; // the launch's signature (diagnostics)
   alignas(16) unsigned char pack[kMaxPack];
 };
+extern const char kMarkKeyObj;
+#define kMarkKey ((const void*)&::esf::kMarkKeyObj)
 
 // esf_batch.hip: the scheduler's side of a deposit and of a wait
 FiberOp* fiberOp(esf_ctx* ctx);
 void fiberDeposit(esf_ctx* ctx);
 hipError_t fiberWait(esf_ctx* ctx, hipEvent_t ev);
-hipEvent_t fiberEvent(esf_ctx* ctx);
+hipEvent_t fiberMark(esf_ctx* ctx);
+void fiberBarrier(esf_ctx* ctx);
+hipError_t fiberQuery(esf_ctx* ctx, hipEvent_t ev);
 // staging of the launch tables: `bytes` of pinned host memory whose
 // device copy is at *dev once stageCopy has been queued
 unsigned char* stageAlloc(BatchRun& run, size_t bytes, const unsigned char** dev);
@@ -169,6 +187,7 @@ inline void launchImpl(esf_ctx* ctx, hipStream_t stream, int cls, dim3 grid, dim
     op->blocks = (int)grid.x;
     op->threads = (int)block.x;
     op->submit = &submitMany<FS, LB, P...>;
+    op->name = __PRETTY_FUNCTION__;
     fiberDeposit(ctx);
     return;
   }
@@ -213,17 +232,52 @@ inline void launchOn(esf_ctx* ctx, hipStream_t s, dim3 grid, dim3 block, A&&... 
                          std::forward<A>(a)...);
 }
 
-// waits that let a batched step's other scenes go on meanwhile
-inline hipError_t streamSync(esf_ctx* ctx) {
-  if (!ctx->fiber) return hipStreamSynchronize(ctx->stream);
-  hipEvent_t e = fiberEvent(ctx);
-  const hipError_t r = hipEventRecord(e, ctx->stream);
-  if (r != hipSuccess) return r;
-  return fiberWait(ctx, e);
+// Events and waits.  Alone, markEvent records `own` on the context's
+// stream and returns it; in a batched step it returns an event the
+// scheduler records once for all scenes that ask in the same round (later
+// than the scene's own work, never earlier).  The waits and queries let the
+// other scenes of a batch go on meanwhile.
+inline hipEvent_t markEvent(esf_ctx* ctx, hipEvent_t own) {
+  if (ctx->fiber) return fiberMark(ctx);
+  (void)hipEventRecord(own, ctx->stream);
+  return own;
+}
+inline hipError_t queryEvent(esf_ctx* ctx, hipEvent_t ev) {
+  return ctx->fiber ? fiberQuery(ctx, ev) : hipEventQuery(ev);
 }
 inline hipError_t eventSync(esf_ctx* ctx, hipEvent_t ev) {
   if (!ctx->fiber) return hipEventSynchronize(ev);
   return fiberWait(ctx, ev);
+}
+// a step's stage boundary: in a batched step the scene waits here until
+// every scene of the batch has reached it (so the next stage's launches
+// start together); alone it does nothing
+inline void batchBarrier(esf_ctx* ctx) {
+  if (ctx->fiber) fiberBarrier(ctx);
+}
+
+inline hipError_t streamSync(esf_ctx* ctx) {
+  if (!ctx->fiber) return hipStreamSynchronize(ctx->stream);
+  return fiberWait(ctx, fiberMark(ctx));
+}
+
+// `bytes` (a multiple of 4) of device memory into pinned host memory,
+// queued on the context's stream: a kernel, so a batched step puts the
+// scenes' copies into one launch
+template <class GridT>
+__device__ __forceinline__ void kToHost(const GridT& wg, const unsigned* src, unsigned* dst,
+                                        long words) {
+  for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < words;
+       i += (long)wg.blocks() * blockDim.x)
+    dst[i] = src[i];
+}
+ESF_BOUNDS(kToHost, 256);
+
+inline void toHostAsync(esf_ctx* ctx, void* hostPinned, const void* dev, size_t bytes) {
+  const long words = (long)(bytes / 4);
+  const int nb = (int)std::min<long>(std::max<long>((words + 255) / 256, 1), 64);
+  launch<ESF_K(kToHost)>(ctx, dim3(nb), dim3(256), (const unsigned*)dev, (unsigned*)hostPinned,
+                         words);
 }
 
 }  // namespace esf

@@ -154,21 +154,25 @@ __device__ __forceinline__ void kMeshRoot(const GridT& wg, int nt, const double*
 ESF_BOUNDS(kMeshRoot, 256);
 
 // does some triangle box of mesh a overlap some triangle box of mesh b?
-// (aabb::overlaps, closed intervals); a hit sets *flag
+// (aabb::overlaps, closed intervals); a hit sets *flag.  Every triangle box
+// of b lies inside b's root box, so a triangle of a that misses the root
+// misses them all and skips the loop (as a BVH descent prunes at the root).
 template <class GridT>
 __device__ __forceinline__ void kMeshPairs(const GridT& wg, int na, const double* __restrict__ boxA,
-                                           int nb, const double* __restrict__ boxB, int* flag) {
+                                           int nb, const double* __restrict__ boxB,
+                                           const double* __restrict__ rootB, int* flag) {
   const int t = wg.block() * blockDim.x + threadIdx.x;
   if (t >= na) return;
   double a[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) a[c] = boxA[6 * t + c];
+  auto overlaps = [&](const double* b) {
+    return !(a[0] > b[3] || a[1] > b[4] || a[2] > b[5] || a[3] < b[0] || a[4] < b[1] ||
+             a[5] < b[2]);
+  };
+  if (!overlaps(rootB)) return;
   bool hit = false;
-  for (int q = 0; q < nb && !hit; q++) {
-    const double* b = boxB + 6 * q;
-    hit = !(a[0] > b[3] || a[1] > b[4] || a[2] > b[5] || a[3] < b[0] || a[4] < b[1] ||
-            a[5] < b[2]);
-  }
+  for (int q = 0; q < nb && !hit; q++) hit = overlaps(boxB + 6 * q);
   if (hit) atomicOr(flag, 1);
 }
 ESF_BOUNDS(kMeshPairs, 256);
@@ -213,16 +217,14 @@ esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<
       const DevSolid& B = ctx->solids[b];
       launch<ESF_K(kMeshPairs)>(ctx, dim3(cdiv(A.meshT, 256)), dim3(256), A.meshT,
                                 (const double*)A.meshBox, B.meshT, (const double*)B.meshBox,
-                                flags + q);
+                                (const double*)(root + 6 * b), flags + q);
     }
-  roots.assign(6 * n, 0.0);
-  pairs.assign(np, 0);
-  ESF_HIP(hipMemcpyAsync(roots.data(), root, 6 * n * sizeof(double), hipMemcpyDeviceToHost,
-                         ctx->stream));
-  if (np > 0)
-    ESF_HIP(hipMemcpyAsync(pairs.data(), flags, np * sizeof(int), hipMemcpyDeviceToHost,
-                           ctx->stream));
+  // root boxes and flags, laid out as in meshOut, into pinned memory
+  toHostAsync(ctx, ctx->hostMesh, root, (6 * kMaxSolids) * sizeof(double) + np * sizeof(int));
   ESF_HIP(streamSync(ctx));
+  const int* hflags = reinterpret_cast<const int*>(ctx->hostMesh + 6 * kMaxSolids);
+  roots.assign(ctx->hostMesh, ctx->hostMesh + 6 * n);
+  pairs.assign(hflags, hflags + np);
   return ESF_OK;
 }
 
