@@ -387,28 +387,35 @@ def batched(esf, count, steps, workers, seed):
     return out
 
 
-def served(esf, seconds, rate, workers, seed, one_at_a_time=True):
+def served(esf, seconds, rate, workers, seed, one_at_a_time=True, batch_slots=64):
     """The serving stream: Poisson arrivals at `rate`/s for `seconds`, cubic
     scenes with N uniform in [32, 160] asking for K uniform in [10, 100]
-    steps, admitted against a 200 GB HBM budget; run with `workers` slots
-    and, for comparison, one scene at a time."""
-    from eulersf_amd.serve import Server, poisson_requests, summarize
+    steps, admitted against a 200 GB HBM budget.  Served three ways: the
+    batched loop (every in-flight scene advanced by one esf_step_batch per
+    step, up to `batch_slots` in flight), `workers` host threads stepping
+    their scenes concurrently, and one scene at a time."""
+    from eulersf_amd.serve import BatchedServer, Server, poisson_requests, summarize
     rng = np.random.default_rng(seed)
     reqs = poisson_requests(rng, rate, seconds)
-    log(f"[bench] served: {len(reqs)} requests over {seconds} s, {workers} slots")
+    log(f"[bench] served: {len(reqs)} requests over {seconds} s")
     out = {"workload": f"Poisson arrivals at {rate}/s for {seconds} s (seed {seed}): N ~ U[32,160], "
                        "K ~ U[10,100] steps, 1-3 spheres, mixed precision at 1e-8, "
                        "200 GB HBM budget", "requests": len(reqs)}
     out["hw_queues"] = os.environ.get("GPU_MAX_HW_QUEUES", "runtime default (4)")
-    srv = Server(200e9, workers, esf.ESF_MIXED)
-    recs, span, info = srv.run(reqs)
+    recs, span, info = BatchedServer(200e9, batch_slots, esf.ESF_MIXED).run(reqs)
+    out["batched"] = {"slots": batch_slots, **summarize(recs, span), **info}
+    log(f"[bench] served, batched: {out['batched']['steps_per_s']} steps/s")
+    recs, span, info = Server(200e9, workers, esf.ESF_MIXED).run(reqs)
     out["concurrent"] = {"slots": workers, **summarize(recs, span), **info}
+    log(f"[bench] served, {workers} threads: {out['concurrent']['steps_per_s']} steps/s")
     if one_at_a_time:
         log("[bench] served, one scene at a time")
         recs1, span1, info1 = Server(200e9, 1, esf.ESF_MIXED).run(reqs)
         out["one_at_a_time"] = {**summarize(recs1, span1), **info1}
-        out["throughput_ratio"] = round(out["concurrent"]["steps_per_s"] /
+        out["throughput_ratio"] = round(out["batched"]["steps_per_s"] /
                                         out["one_at_a_time"]["steps_per_s"], 3)
+        out["throughput_ratio_threads"] = round(out["concurrent"]["steps_per_s"] /
+                                                out["one_at_a_time"]["steps_per_s"], 3)
     return out
 
 
@@ -539,6 +546,8 @@ def main():
                     help="the serving stream over SECONDS of Poisson arrivals, with the "
                          "concurrent server and one scene at a time; prints one JSON line")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--batched-scenes", type=int, default=64,
+                    help="scenes of the batched-workload block of the default run (0 to skip)")
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="arrival window of the short served-stream block of the default "
                          "run (0 to skip)")
@@ -642,6 +651,10 @@ def main():
     if args.served_seconds > 0:
         serve_block = served(esf, args.served_seconds, args.serve_rate, args.workers, args.seed)
 
+    batch_block = None
+    if args.batched_scenes > 0:
+        batch_block = batched(esf, args.batched_scenes, args.batch_steps, args.workers, args.seed)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline ({args.cpu_warmup} warm-up + median of "
@@ -699,6 +712,7 @@ def main():
         "spmv_apply_at_workload": {"fp64": roof64, "fp32": roof32},
         "roofline_hbm_sized": roof_big,
         "served_stream": serve_block,
+        "batched_scenes": batch_block,
         "cpu_baseline": cpu,
     }
     g64.close()

@@ -81,23 +81,36 @@ static double nowS() {
       .count();
 }
 
-// per device: the staging ring and the fibers, kept between calls; one
-// batch at a time per device
+// the staging ring and the fibers of one batch in flight, kept between
+// calls; each batch in flight (a calling thread) takes a pool of its device
+// that no other batch holds
 struct BatchPool {
-  std::mutex m;
+  int device = 0;
+  bool inUse = false;
   BatchRun run;
   std::vector<std::unique_ptr<SceneFiber>> fibers;
   bool ready = false;
 };
 
 static std::mutex gPoolsMutex;
-static std::unordered_map<int, std::unique_ptr<BatchPool>> gPools;
+static std::vector<std::unique_ptr<BatchPool>> gPools;
 
-static BatchPool* poolFor(int device) {
+static BatchPool* takePool(int device) {
   std::lock_guard<std::mutex> g(gPoolsMutex);
-  auto& p = gPools[device];
-  if (!p) p.reset(new BatchPool);
-  return p.get();
+  for (auto& p : gPools)
+    if (p->device == device && !p->inUse) {
+      p->inUse = true;
+      return p.get();
+    }
+  gPools.emplace_back(new BatchPool);
+  gPools.back()->device = device;
+  gPools.back()->inUse = true;
+  return gPools.back().get();
+}
+
+static void releasePool(BatchPool* p) {
+  std::lock_guard<std::mutex> g(gPoolsMutex);
+  p->inUse = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -384,8 +397,11 @@ extern "C" esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha,
   }
   int device = 0;
   if (hipGetDevice(&device) != hipSuccess) return ESF_ERR_DEVICE;
-  BatchPool* pool = poolFor(device);
-  std::lock_guard<std::mutex> g(pool->m);
+  BatchPool* pool = takePool(device);
+  struct Release {
+    BatchPool* p;
+    ~Release() { releasePool(p); }
+  } release{pool};
   std::string why;
   esf_status st = prepare(*pool, n, why);
   if (st) {

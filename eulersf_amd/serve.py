@@ -213,6 +213,168 @@ class Server:
         return done, makespan, {"peak_active": peak_active, "peak_budget_bytes": peak_bytes}
 
 
+class BatchedServer:
+    """Continuous batching of scene requests, each step one batched launch
+    per kernel over the scenes in flight: every in-flight scene is advanced
+    by one substep per esf_step_batch call.  Scenes are grouped by size into
+    `lanes` (N ranges of equal width over `n_range`); each lane's thread
+    steps its scenes as one batch, so a step of small scenes does not wait
+    for the solves of large ones, and the lanes' batches run side by side on
+    the GPU.  Requests are admitted in arrival order while fewer than
+    `max_active` scenes are in flight (or being built) and their estimated
+    device memory fits the HBM budget; `builders` threads create the
+    admitted scenes' contexts (grids, level set, masses, Krylov state
+    resident in HBM) beside the stepping, and a scene joins its lane's batch
+    at the lane's next step once built.  A scene's step latency is its batch
+    step's; it leaves after its K steps and its memory returns to the
+    budget."""
+
+    def __init__(self, budget_bytes, max_active, precision=_abi.ESF_MIXED,
+                 bytes_per_node=1200.0, builders=16, lanes=8, n_range=(32, 160), on_done=None):
+        self.budget = budget_bytes
+        self.max_active = max_active
+        self.precision = precision
+        self.bytes_per_node = bytes_per_node
+        self.builders = builders
+        self.lanes = lanes
+        self.n_range = n_range
+        self.on_done = on_done
+
+    def estimate(self, n):
+        return int(self.bytes_per_node * (n + 1) ** 3)
+
+    def lane_of(self, n):
+        lo, hi = self.n_range
+        w = (hi - lo + 1) / float(self.lanes)
+        return min(self.lanes - 1, max(0, int((n - lo) / w)))
+
+    def run(self, requests):
+        """requests: list of dicts with keys id, t (arrival, s), n, k, text.
+        Returns per-request records, the makespan and admission counters."""
+        reqs = sorted(requests, key=lambda r: r["t"])
+        t_start = time.perf_counter()
+        cv = threading.Condition()
+        st = {"used": 0, "inflight": 0, "peak_active": 0, "peak_bytes": 0, "stop": False}
+        todo = deque()                         # admitted, waiting for a builder
+        ready = [deque() for _ in range(self.lanes)]   # built, joining a lane
+        done = []
+        batch_sizes = []
+
+        def builder():
+            while True:
+                with cv:
+                    while not todo and not st["stop"]:
+                        cv.wait()
+                    if not todo:
+                        return
+                    req, est = todo.popleft()
+                rec = {"id": req["id"], "n": req["n"], "k": req["k"], "arrival": req["t"],
+                       "lat": [], "error": None, "est": est,
+                       "start": time.perf_counter() - t_start}
+                g = None
+                try:
+                    g = CoupledIntegrator3D.from_cfg_text(req["text"], self.precision)
+                    rec["bytes"] = g.device_bytes()
+                except Exception as e:  # kept in the record, reported by the caller
+                    rec["error"] = repr(e)
+                with cv:
+                    if rec["error"] is None:
+                        self.bytes_per_node = max(self.bytes_per_node,
+                                                  rec["bytes"] / float((req["n"] + 1) ** 3))
+                        ready[self.lane_of(req["n"])].append([rec, g, est, 0])
+                    else:
+                        rec["end"] = time.perf_counter() - t_start
+                        st["used"] -= est
+                        st["inflight"] -= 1
+                        done.append(rec)
+                    cv.notify_all()
+
+        def lane(li):
+            active = []
+            while True:
+                with cv:
+                    while not active and not ready[li] and not st["stop"]:
+                        cv.wait(0.01)
+                    while ready[li]:
+                        active.append(ready[li].popleft())
+                    if not active and st["stop"]:
+                        return
+                if not active:
+                    continue
+                t0 = time.perf_counter()
+                try:
+                    reps = step_batch([a[1] for a in active])
+                    errs = [None if r["status"] >= 0 else f"step failed: {r}" for r in reps]
+                except _abi.EsfError as e:
+                    errs = [repr(e)] * len(active)
+                dt = time.perf_counter() - t0
+                keep, leaving = [], []
+                for a, err in zip(active, errs):
+                    rec, g = a[0], a[1]
+                    rec["lat"].append(dt)
+                    a[3] += 1
+                    if err is not None:
+                        rec["error"] = err
+                    if a[3] >= rec["k"] or err is not None:
+                        if self.on_done is not None and err is None:
+                            self.on_done({"id": rec["id"], "n": rec["n"], "k": rec["k"]}, g)
+                        g.close()
+                        rec["end"] = time.perf_counter() - t_start
+                        leaving.append(a)
+                    else:
+                        keep.append(a)
+                with cv:
+                    batch_sizes.append(len(active))
+                    for a in leaving:
+                        st["used"] -= a[2]
+                        st["inflight"] -= 1
+                        done.append(a[0])
+                    cv.notify_all()
+                active = keep
+
+        threads = [threading.Thread(target=builder, daemon=True) for _ in range(self.builders)]
+        threads += [threading.Thread(target=lane, args=(i,), daemon=True)
+                    for i in range(self.lanes)]
+        for th in threads:
+            th.start()
+        nxt = 0
+        pending = deque()
+        try:
+            while True:
+                now = time.perf_counter() - t_start
+                while nxt < len(reqs) and reqs[nxt]["t"] <= now:
+                    pending.append(reqs[nxt])
+                    nxt += 1
+                with cv:
+                    if len(done) == len(reqs):
+                        break
+                    # admit in arrival order while a slot and the budget allow
+                    while pending and st["inflight"] < self.max_active:
+                        est = self.estimate(pending[0]["n"])
+                        if st["used"] + est > self.budget and st["inflight"]:
+                            break
+                        req = pending.popleft()
+                        st["used"] += est
+                        st["inflight"] += 1
+                        todo.append((req, est))
+                        cv.notify_all()
+                    st["peak_active"] = max(st["peak_active"], st["inflight"])
+                    st["peak_bytes"] = max(st["peak_bytes"], st["used"])
+                    cv.wait(0.0005)
+        finally:
+            with cv:
+                st["stop"] = True
+                cv.notify_all()
+            for th in threads:
+                th.join()
+        makespan = time.perf_counter() - t_start
+        mean_batch = float(np.mean(batch_sizes)) if batch_sizes else 0.0
+        return done, makespan, {"peak_active": st["peak_active"],
+                                "peak_budget_bytes": st["peak_bytes"],
+                                "lanes": self.lanes, "batch_steps": len(batch_sizes),
+                                "mean_scenes_per_batch_step": round(mean_batch, 2)}
+
+
 def poisson_requests(rng, rate, seconds, n_range=(32, 160), k_range=(10, 100), eps=1e-8):
     """Poisson arrivals at `rate` per second over `seconds`: cubic scenes
     with N uniform in n_range, K uniform in k_range steps"""

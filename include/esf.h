@@ -323,8 +323,9 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
  * esf_step's return value; each context's results are bitwise those of
  * esf_step on it alone.  The work runs on ctxs[0]'s stream, ordered after
  * the work already queued on every context's stream and before any queued
- * on them later.  Contexts must be distinct and on the current device; one
- * batch runs at a time per device.  Returns the most severe status. */
+ * on them later.  Contexts must be distinct, on the current device and in
+ * no other batch; several threads may each run a batch of their own.
+ * Returns the most severe status. */
 esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha, esf_solve_stats* stats,
                           esf_status* status);
 

@@ -93,3 +93,38 @@ def test_batch_at_workload_sizes(esf):
     alone = run_alone(esf, texts, esf.ESF_MIXED, 2)
     _say("batch workload: alone done")
     check_batched(esf, texts, esf.ESF_MIXED, 2, alone)
+
+
+def test_batched_server_results_same_as_alone(esf):
+    """a served stream through the batched serving loop (one esf_step_batch
+    per step over the scenes in flight, scenes built beside the stepping):
+    every request completes with its K steps, never more than the slots in
+    flight, and each result is bitwise that of the scene stepped alone"""
+    from eulersf_amd.serve import BatchedServer, poisson_requests
+    rng = np.random.default_rng(9)
+    reqs = poisson_requests(rng, rate=40.0, seconds=0.5, n_range=(12, 24), k_range=(2, 5),
+                            eps=1e-10)
+    assert len(reqs) >= 5
+    got = {}
+
+    def keep(req, g):
+        got[req["id"]] = fields(esf, g, next(r for r in reqs if r["id"] == req["id"])["text"]
+                                .count("solid sphere radius"))
+
+    srv = BatchedServer(budget_bytes=4e9, max_active=6, precision=esf.ESF_MIXED, builders=2,
+                        on_done=keep)
+    recs, span, info = srv.run(reqs)
+    _say(f"batched server: {len(recs)} requests, {info}")
+    assert len(recs) == len(reqs)
+    assert all(r["error"] is None for r in recs), [r["error"] for r in recs]
+    assert all(len(r["lat"]) == r["k"] for r in recs)
+    assert info["peak_active"] <= 6
+    assert info["mean_scenes_per_batch_step"] > 1
+    for r in reqs:
+        g = esf.CoupledIntegrator3D.from_cfg_text(r["text"], esf.ESF_MIXED)
+        for _ in range(r["k"]):
+            g.stepImplicit(-1.0)
+        alone = fields(esf, g, r["text"].count("solid sphere radius"))
+        g.close()
+        for a, b in zip(alone, got[r["id"]]):
+            np.testing.assert_array_equal(a, b)
