@@ -268,7 +268,7 @@ def divergence_field(u, n):
     return b.reshape(-1)
 
 
-def isolated_system(esf, n, seed, eps, reps):
+def isolated_system(esf, n, seed, eps, reps, max_it=100000):
     """The isolated coupled linear system at n^3 cells: k = 8 spheres from
     the seed (centres uniform in [0.15, 0.85]^3, non-overlapping, radii
     uniform in [0.05, 0.12], E log-uniform in [1e3, 1e6]), the domain 60 %
@@ -298,7 +298,7 @@ def isolated_system(esf, n, seed, eps, reps):
         for r in range(reps):
             g.upload(esf._abi.F_PRESSURE_RHS, b)
             g.upload(esf._abi.F_LAMBDA, np.zeros(n ** 3))
-            it, ref, rr, ms, st = g.pressureSolveResident(eps, 100000)
+            it, ref, rr, ms, st = g.pressureSolveResident(eps, max_it)
             times.append(ms)
         sols[name] = g.download(esf._abi.F_LAMBDA)
         spmv = spmv_roofline(g, n ** 3, (n - 1) ** 3, prec == esf.ESF_MIXED, 20)
@@ -316,11 +316,11 @@ def isolated_system(esf, n, seed, eps, reps):
     Sx = g.getMatVecMult(sols["fp64"])
     out["mixed_vs_fp64"] = {"rel_diff_S_x": float(np.linalg.norm(Sd) / np.linalg.norm(Sx))}
     g.close()
-    out["solid"] = isolated_solid_system(esf, text, len(spheres), rng, eps, reps)
+    out["solid"] = isolated_solid_system(esf, text, len(spheres), rng, eps, reps, max_it)
     print(json.dumps(out), flush=True)
 
 
-def isolated_solid_system(esf, text, k, rng, eps, reps):
+def isolated_solid_system(esf, text, k, rng, eps, reps, max_it=100000):
     """the solid half of the isolated system on the same scene: each valid
     cell's deformation gradient a random rotation times diag(N(1, 0.05)),
     the elastic stiffness assembled from those, A = dt^2 K + M~ (beta = 0)
@@ -340,7 +340,7 @@ def isolated_solid_system(esf, text, k, rng, eps, reps):
     times = []
     for r in range(reps):
         g.upload(esf._abi.F_SOLID_RHS, rhs)
-        it, rr, ms, st = g.solidSolveResident(eps, 100000)
+        it, rr, ms, st = g.solidSolveResident(eps, max_it)
         times.append(ms)
     g.close()
     return {"solid_nodes": ns, "valid_cells": cells, "iterations": it, "rel_residual": rr,
@@ -536,6 +536,8 @@ def main():
                          "FP64 and mixed solves to --eps from x = 0; prints one JSON line")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--system-reps", type=int, default=3)
+    ap.add_argument("--system-iters", type=int, default=100000,
+                    help="iteration cap of the isolated system's solves (short profiling runs)")
     ap.add_argument("--batch", type=int, default=0, metavar="S",
                     help="the batched uneven scenes: S scenes stepped one at a time and "
                          "concurrently; prints one JSON line")
@@ -587,7 +589,8 @@ def main():
         drift(esf, args.n, args.drift, vel, not args.no_flip, args.eps, args.drift_every)
         return
     if args.system:
-        isolated_system(esf, args.system, args.seed, args.eps, args.system_reps)
+        isolated_system(esf, args.system, args.seed, args.eps, args.system_reps,
+                        args.system_iters)
         return
     if args.batch:
         print(json.dumps(batched(esf, args.batch, args.batch_steps, args.workers, args.seed)),
