@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "esf_internal.h"
@@ -32,7 +33,8 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
 __global__ void kCorotBatch(int n, double lambda, double mu, const double* F, const double* dF,
                             double* out, double* energy, double* dP);
 // esf_grid.hip
-esf_status gatherSolidMasses(esf_ctx* ctx);
+esf_status gatherSolidMasses(esf_ctx* ctx, bool defer);
+esf_status gatherFinish(esf_ctx* ctx);
 esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf);
 esf_status updateBoxes(esf_ctx* ctx, int solid);
 esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid);
@@ -41,8 +43,9 @@ esf_status solidPhi(esf_ctx* ctx, int solid);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
 esf_status forkBodyForces(esf_ctx* ctx);
-esf_status cflEnqueue(esf_ctx* ctx, double alpha);
-esf_status cflFinish(esf_ctx* ctx, double* dtOut);
+esf_status cflEnqueue(esf_ctx* ctx, bool mark);
+esf_status cflWait(esf_ctx* ctx);
+esf_status cflFinish(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status joinBodyForces(esf_ctx* ctx);
 esf_status solidRhs(esf_ctx* ctx);
 esf_status scatterSolidV0(esf_ctx* ctx);
@@ -85,11 +88,22 @@ esf_status devAllocBytes(esf_ctx* ctx, void** p, size_t bytes) {
   return dalloc(ctx, (char**)p, bytes);
 }
 
+// Lists grow with a quarter of headroom; tight lists (tests) take exactly
+// a known count (need > 0), shrinking too.
+static size_t listCapacity(const esf_ctx* ctx, int need, int cap) {
+  if (ctx->tightLists) return std::max(need > 0 ? need : cap, 1);
+  need = std::max(need, cap);
+  return std::max<size_t>(need + need / 4 + 64, 256);
+}
+static bool listFits(const esf_ctx* ctx, int need, int cap) {
+  return need <= cap && !(ctx->tightLists && need > 0 && need != cap);
+}
+
 esf_status growSolidDofs(esf_ctx* ctx, int need) {
-  if (need <= ctx->capSolid && ctx->sForce && ctx->inboxSolids >= (int)ctx->solids.size())
+  if (listFits(ctx, need, ctx->capSolid) && ctx->sForce &&
+      ctx->inboxSolids >= (int)ctx->solids.size())
     return ESF_OK;
-  need = std::max(need, ctx->capSolid);
-  const size_t cap = std::max<size_t>(need + need / 4 + 64, 256);
+  const size_t cap = listCapacity(ctx, need, ctx->capSolid);
   double** vecs[] = {&ctx->sForce, &ctx->sMv,  &ctx->sDamped, &ctx->sKdiag, &ctx->sAdiag,
                      &ctx->sRhs,   &ctx->sV0,  &ctx->sDv,     &ctx->sB,     &ctx->sr,
                      &ctx->sp,     &ctx->ss,   &ctx->sAp,     &ctx->stmp,   &ctx->sSd,
@@ -111,8 +125,8 @@ esf_status growSolidDofs(esf_ctx* ctx, int need) {
 }
 
 esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need) {
-  if (need <= so.capValid && so.cellList) return ESF_OK;
-  const size_t cap = std::max<size_t>(need + need / 4 + 64, 256);
+  if (listFits(ctx, need, so.capValid) && so.cellList) return ESF_OK;
+  const size_t cap = listCapacity(ctx, need, so.capValid);
   esf_status st;
   if ((st = dalloc(ctx, &so.cellList, cap))) return st;
   if ((st = dalloc(ctx, &so.cellMat, 28 * cap))) return st;
@@ -179,6 +193,11 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ESF_ERR_DEVICE;
   esf_ctx* ctx = new esf_ctx();
   hipGetDevice(&ctx->device);
+  // ESF_TIGHT_LISTS=1 (tests): the solid DOF and valid-cell lists are
+  // sized to exactly their last counts, so a step whose count rises takes
+  // the path that grows a list after its scan and scans again
+  const char* tight = getenv("ESF_TIGHT_LISTS");
+  ctx->tightLists = tight && tight[0] == '1';
   Dims& d = ctx->d;
   d.X = g->xres;
   d.Y = g->yres;
@@ -246,7 +265,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   if (!st) st = growSolidDofs(ctx, 0);
   if (!st && hipHostMalloc((void**)&ctx->hostRing, 8 * sizeof(CrScalars)) != hipSuccess)
     st = ESF_ERR_OOM;
-  if (!st && hipHostMalloc((void**)&ctx->hostFlags, 64 * sizeof(int)) != hipSuccess)
+  if (!st && hipHostMalloc((void**)&ctx->hostFlags, kHostFlagInts * sizeof(int)) != hipSuccess)
     st = ESF_ERR_OOM;
   if (!st && hipHostMalloc((void**)&ctx->hostCfl, kCflBlocks * sizeof(double)) != hipSuccess)
     st = ESF_ERR_OOM;
@@ -736,7 +755,7 @@ esf_status esf_solid_sdf(esf_ctx* ctx, int solid) {
 esf_status esf_gather_solid_masses(esf_ctx* ctx) {
   if (!ctx) return ESF_ERR_INVALID;
   StageTimer t(ctx, 5);
-  return gatherSolidMasses(ctx);
+  return gatherSolidMasses(ctx, false);
 }
 
 esf_status esf_recover_velocity(esf_ctx* ctx) {
@@ -925,19 +944,23 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     return ESF_ERR_INVALID;
   }
   // alpha < 0: the broad phase of stepImplicit decides it
-  // (COUPLED_INTEGRATOR_3D.cpp:384-411), as in esf_frame
-  if (alpha < 0 && (st = esf_broad_phase(ctx, &alpha))) return st;
-  // esf_cfl and esf_forces, reordered: the stress stage does not use dt,
-  // so it is queued behind the CFL kernel before the host waits for dt;
-  // a step running alone sends the fluid's vorticity and body forces to the
-  // side stream, beside the first solid solve, which needs only the solid
-  // nodes' part (same results either way)
+  // (COUPLED_INTEGRATOR_3D.cpp:384-411), as in esf_frame.  Its mesh kernels
+  // and their read-back are queued ahead of the CFL kernel, the stress stage
+  // behind it (it does not use dt), and the host waits once for both
+  // read-backs.  A step running alone sends the fluid's vorticity and body
+  // forces to the side stream, beside the first solid solve, which needs
+  // only the solid nodes' part (same results either way).
   batchBarrier(ctx);
   {
     StageTimer t(ctx, 1);
-    if ((st = cflEnqueue(ctx, alpha))) return st;
+    const bool broad = alpha < 0;
+    bool meshes = false;
+    if (broad && (st = broadPhaseEnqueue(ctx, &meshes))) return st;
+    if ((st = cflEnqueue(ctx, broad && meshes))) return st;
     if ((st = stressStage(ctx, nullptr))) return st;
-    if ((st = cflFinish(ctx, &dt))) return st;
+    if ((st = cflWait(ctx))) return st;
+    if (broad && (st = broadPhaseFinish(ctx, meshes, &alpha))) return st;
+    if ((st = cflFinish(ctx, alpha, &dt))) return st;
     st = inFlight.alone && !ctx->fiber ? forkBodyForces(ctx)
                                        : vorticityAndBodyForces(ctx, nullptr);
     if (st) return st;
@@ -986,7 +1009,13 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     }
   }
   batchBarrier(ctx);
-  if ((st = esf_gather_solid_masses(ctx))) return st;
+  {
+    // the DOF and valid-cell counts are read after the step's last
+    // synchronisation (nothing before it launches by them), except where
+    // the FLIP particles are seeded on the new valid cells
+    StageTimer t(ctx, 5);
+    if ((st = gatherSolidMasses(ctx, !ctx->p.flip))) return st;
+  }
   // initMaterialParticles on the new valid cells (:870-877)
   if (ctx->p.flip)
     for (int b = 0; b < (int)ctx->solids.size(); b++)
@@ -996,6 +1025,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;
   ESF_HIP(streamSync(ctx));
+  if ((st = gatherFinish(ctx))) return st;
   solveSt = coupledSolveReport(ctx, &s);
   if (stats) *stats = s;
   return solveSt;

@@ -23,99 +23,111 @@ __device__ __forceinline__ bool isInterior(const Dims& d, int x, int y, int z) {
 // ---------------------------------------------------------------------------
 constexpr int kScanTile = 1024;  // 256 threads x 4
 
-template <class Flag, class GridT>
-__device__ __forceinline__ void kScanCount(const GridT& wg, long n, Flag flag, int* blockCounts) {
-  __shared__ int red[256];
-  const long base = (long)wg.block() * kScanTile;
-  int c = 0;
-  for (int k = 0; k < 4; k++) {
-    const long i = base + threadIdx.x * 4 + k;
-    if (i < n && flag(i)) c++;
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ inline int waveInclusiveScan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
   }
-  red[threadIdx.x] = c;
+  return v;
+}
+
+// exclusive prefix sum of v over the block (a whole number of waves, at
+// most 16) and, in *total, the block's sum; wsum is 16 ints of LDS
+__device__ inline int blockExclusiveScan(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int inc = waveInclusiveScan(v);
+  if (lane == 63) wsum[w] = inc;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
+  int pre = 0, all = 0;
+  for (int k = 0; k < nw; k++) {
+    const int t = wsum[k];
+    pre += k < w ? t : 0;
+    all += t;
   }
-  if (threadIdx.x == 0) blockCounts[wg.block()] = red[0];
+  *total = all;
+  return pre + inc - v;
+}
+
+// The compactions walk a box of nodes or cells (BoxWalk, defined below):
+// each thread takes 4 consecutive elements of the box, x fastest, decoding
+// the first one's coordinates and stepping to the others.
+template <class Walk, class Flag, class GridT>
+__device__ __forceinline__ void kScanCount(const GridT& wg, Walk w, long n, Flag flag,
+                                           int* blockCounts) {
+  __shared__ int wsum[16];
+  const long t0 = (long)wg.block() * kScanTile + threadIdx.x * 4;
+  int c = 0;
+  if (t0 < n) {
+    int x, y, z;
+    w.start(t0, x, y, z);
+    for (int k = 0; k < 4 && t0 + k < n; k++, w.next(x, y, z))
+      if (flag(x, y, z, w.index(x, y, z))) c++;
+  }
+  int total;
+  blockExclusiveScan(c, wsum, &total);
+  if (threadIdx.x == 0) blockCounts[wg.block()] = total;
 }
 ESF_BOUNDS(kScanCount, 1024);
 
 // exclusive scan of nb block counts by one block; total into blockCounts[nb]
 template <class GridT>
 __device__ __forceinline__ void kScanBlocks(const GridT& wg, int nb, int* blockCounts) {
-  __shared__ int part[1024];
+  __shared__ int wsum[16];
   const int per = (nb + 1023) / 1024;
   const int b0 = threadIdx.x * per;
   int s = 0;
   for (int i = 0; i < per; i++)
     if (b0 + i < nb) s += blockCounts[b0 + i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int t = 0; t < 1024; t++) {
-      const int v = part[t];
-      part[t] = acc;
-      acc += v;
-    }
-    blockCounts[nb] = acc;
-  }
-  __syncthreads();
-  int acc = part[threadIdx.x];
+  int total;
+  int acc = blockExclusiveScan(s, wsum, &total);
   for (int i = 0; i < per; i++)
     if (b0 + i < nb) {
       const int v = blockCounts[b0 + i];
       blockCounts[b0 + i] = acc;
       acc += v;
     }
+  if (threadIdx.x == 0) blockCounts[nb] = total;
 }
 ESF_BOUNDS(kScanBlocks, 1024);
 
-template <class Flag, class Emit, class GridT>
-__device__ __forceinline__ void kScanEmit(const GridT& wg, long n, Flag flag, const int* blockCounts,
-                                          Emit emit) {
-  __shared__ int pre[256];
-  const long base = (long)wg.block() * kScanTile;
-  int f[4], c = 0;
-  for (int k = 0; k < 4; k++) {
-    const long i = base + threadIdx.x * 4 + k;
-    f[k] = (i < n && flag(i)) ? 1 : 0;
-    c += f[k];
-  }
-  pre[threadIdx.x] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int t = 0; t < 256; t++) {
-      const int v = pre[t];
-      pre[t] = acc;
-      acc += v;
+template <class Walk, class Flag, class Emit, class GridT>
+__device__ __forceinline__ void kScanEmit(const GridT& wg, Walk w, long n, Flag flag,
+                                          const int* blockCounts, Emit emit) {
+  __shared__ int wsum[16];
+  const long t0 = (long)wg.block() * kScanTile + threadIdx.x * 4;
+  int f[4] = {0, 0, 0, 0}, c = 0;
+  int x = 0, y = 0, z = 0;
+  if (t0 < n) {
+    w.start(t0, x, y, z);
+    int xs = x, ys = y, zs = z;
+    for (int k = 0; k < 4 && t0 + k < n; k++, w.next(xs, ys, zs)) {
+      f[k] = flag(xs, ys, zs, w.index(xs, ys, zs)) ? 1 : 0;
+      c += f[k];
     }
   }
-  __syncthreads();
-  int idx = blockCounts[wg.block()] + pre[threadIdx.x];
-  for (int k = 0; k < 4; k++) {
-    const long i = base + threadIdx.x * 4 + k;
-    if (i >= n) break;
-    emit(i, f[k] ? idx : -1);
+  int total;
+  int idx = blockCounts[wg.block()] + blockExclusiveScan(c, wsum, &total);
+  if (t0 >= n) return;
+  for (int k = 0; k < 4 && t0 + k < n; k++, w.next(x, y, z)) {
+    emit(x, y, z, w.index(x, y, z), f[k] ? idx : -1);
     idx += f[k];
   }
 }
 ESF_BOUNDS(kScanEmit, 1024);
 
-template <class Flag, class Emit>
-esf_status compact(esf_ctx* ctx, long n, Flag flag, Emit emit, int* total) {
+// count, scan and emit over the n elements of a walk, queued; the count is
+// copied to the pinned int *hostTotal, read after a later synchronisation
+template <class Walk, class Flag, class Emit>
+void compactEnqueue(esf_ctx* ctx, Walk w, long n, Flag flag, Emit emit, int* hostTotal) {
   const int nb = std::max(cdiv(n, kScanTile), 1);
-  launch<ESF_KT(kScanCount, Flag)>(ctx, dim3(nb), dim3(256), n, flag, ctx->scanBlocks);
+  launch<ESF_KT(kScanCount, Walk, Flag)>(ctx, dim3(nb), dim3(256), w, n, flag, ctx->scanBlocks);
   launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
-  launch<ESF_KT(kScanEmit, Flag, Emit)>(ctx, dim3(nb), dim3(256), n, flag,
-                                        (const int*)ctx->scanBlocks, emit);
-  toHostAsync(ctx, ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int));
-  ESF_HIP(streamSync(ctx));
-  *total = ctx->hostFlags[0];
-  return ESF_OK;
+  launch<ESF_KT(kScanEmit, Walk, Flag, Emit)>(ctx, dim3(nb), dim3(256), w, n, flag,
+                                              (const int*)ctx->scanBlocks, emit);
+  toHostAsync(ctx, hostTotal, ctx->scanBlocks + nb, sizeof(int));
 }
 
 // a box of nodes or cells, [lo, hi) per axis, walked x fastest
@@ -151,6 +163,29 @@ inline BoxRange spanBoxes(const int a[6], const int b[6], const int lim[3]) {
   }
   return r;
 }
+
+// the elements of a box, walked x fastest: element t is grid element
+// x + y sy + z sz, so a compaction over the box emits in the whole grid's
+// ascending order (box counts stay below 2^31: the grid's do)
+struct BoxWalk {
+  BoxRange r;
+  long sy, sz;
+  __device__ void start(long t, int& x, int& y, int& z) const {
+    const int nx = r.hi[0] - r.lo[0], ny = r.hi[1] - r.lo[1];
+    const int ti = (int)t, q = ti / nx;
+    x = r.lo[0] + (ti - q * nx);
+    y = r.lo[1] + q % ny;
+    z = r.lo[2] + q / ny;
+  }
+  __device__ void next(int& x, int& y, int& z) const {
+    if (++x < r.hi[0]) return;
+    x = r.lo[0];
+    if (++y < r.hi[1]) return;
+    y = r.lo[1];
+    ++z;
+  }
+  __device__ long index(int x, int y, int z) const { return x + y * sy + z * sz; }
+};
 
 // ---------------------------------------------------------------------------
 // gatherSolidMasses  COUPLED_INTEGRATOR_3D.cpp:327-375
@@ -197,29 +232,27 @@ ESF_BOUNDS(kGatherMass, 1024);
 struct SolidFlag {
   Dims d;
   const double* solidMass;
-  __device__ bool operator()(long n) const {
-    const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  __device__ bool operator()(int x, int y, int z, long n) const {
     return isInterior(d, x, y, z) && solidMass[n] > 0;
   }
 };
+// numbers past `cap` (solidNodes' length) are counted, not stored: the
+// count read later shows the list must grow and the scan run again
 struct SolidEmit {
   int* sidx;
   int* solidNodes;
-  __device__ void operator()(long n, int idx) const {
+  int cap;
+  __device__ void operator()(int, int, int, long n, int idx) const {
     sidx[n] = idx;
-    if (idx >= 0) solidNodes[idx] = (int)n;
+    if (idx >= 0 && idx < cap) solidNodes[idx] = (int)n;
   }
 };
 
-esf_status gatherSolidMasses(esf_ctx* ctx);
-
 // valid-cell list of one solid: cells of the tight box with _valid == 1
 struct ValidFlag {
-  Dims d;
   Box box;
   const int8_t* valid;
-  __device__ bool operator()(long c) const {
-    const int x = c % d.CX, y = (c / d.CX) % d.CY, z = c / d.cslab;
+  __device__ bool operator()(int x, int y, int z, long c) const {
     return x >= box.tMin[0] && x < box.tMax[0] - 1 && y >= box.tMin[1] &&
            y < box.tMax[1] - 1 && z >= box.tMin[2] && z < box.tMax[2] - 1 && valid[c] == 1;
   }
@@ -228,105 +261,162 @@ struct ValidEmit {
   Dims d;
   int* cellSlot;
   int* cellList;
-  __device__ void operator()(long c, int idx) const {
+  int cap;  // as SolidEmit's
+  __device__ void operator()(int x, int y, int z, long c, int idx) const {
     cellSlot[c] = idx;
-    if (idx >= 0) {
-      const int x = c % d.CX, y = (c / d.CX) % d.CY, z = c / d.cslab;
-      cellList[idx] = x + y * d.X + z * (int)d.slab;
-    }
+    if (idx >= 0 && idx < cap) cellList[idx] = x + y * d.X + z * (int)d.slab;
   }
 };
 
 esf_status growSolidCells(esf_ctx* ctx, DevSolid& so, int need);
 esf_status growSolidDofs(esf_ctx* ctx, int need);
 
-esf_status buildValidList(esf_ctx* ctx, DevSolid& so) {
-  int total = 0;
-  // count first, grow, then emit
-  ValidFlag fl{ctx->d, so.box, so.valid};
-  const long n = ctx->d.cells;
-  const int nb = std::max(cdiv(n, kScanTile), 1);
-  launch<ESF_KT(kScanCount, ValidFlag)>(ctx, dim3(nb), dim3(256), n, fl, ctx->scanBlocks);
-  launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
-  toHostAsync(ctx, ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int));
-  ESF_HIP(streamSync(ctx));
-  total = ctx->hostFlags[0];
-  esf_status st = growSolidCells(ctx, so, total);
+static BoxRange rangeOf(const int r[6]) {
+  BoxRange b;
+  for (int k = 0; k < 3; k++) {
+    b.lo[k] = r[k];
+    b.hi[k] = r[3 + k];
+  }
+  return b;
+}
+
+// Compactions are queued and their counts read at a later synchronisation
+// (gatherFinish): a step reads them at its end, so the stages in between
+// need not wait for them.
+static void queueSolidDofs(esf_ctx* ctx) {
+  const Dims& d = ctx->d;
+  const BoxRange r = rangeOf(ctx->dofScan);
+  compactEnqueue(ctx, BoxWalk{r, d.X, d.slab}, r.count(), SolidFlag{d, ctx->solidMass},
+                 SolidEmit{ctx->sidx, ctx->solidNodes, ctx->capSolid},
+                 ctx->hostFlags + kHostDofCount);
+  ctx->dofPending = true;
+}
+
+static void queueValidScan(esf_ctx* ctx, int b) {
+  DevSolid& so = ctx->solids[b];
+  const Dims& d = ctx->d;
+  const BoxRange r = rangeOf(so.slotScan);
+  compactEnqueue(ctx, BoxWalk{r, d.CX, d.cslab}, r.count(), ValidFlag{so.box, so.valid},
+                 ValidEmit{d, so.cellSlot, so.cellList, so.capValid},
+                 ctx->hostFlags + kHostValidCount + b);
+  so.listPending = true;
+}
+
+// the valid-cell list of solid b, queued.  The scan walks the cells of the
+// tight box (ValidFlag's range) and of the last build's, where cellSlot may
+// be >= 0; every other cell's slot is -1 already and stays so.
+static esf_status queueValidList(esf_ctx* ctx, int b) {
+  DevSolid& so = ctx->solids[b];
+  const Dims& d = ctx->d;
+  esf_status st = growSolidCells(ctx, so, 0);
   if (st) return st;
-  ValidEmit em{ctx->d, so.cellSlot, so.cellList};
-  launch<ESF_KT(kScanEmit, ValidFlag, ValidEmit)>(ctx, dim3(nb), dim3(256), n, fl,
-                                                  (const int*)ctx->scanBlocks, em);
-  so.nValid = total;
-  so.listDirty = false;
+  const Box& bx = so.box;
+  const int cur[6] = {bx.tMin[0], bx.tMin[1], bx.tMin[2],
+                      bx.tMax[0] - 1, bx.tMax[1] - 1, bx.tMax[2] - 1};
+  const int lim[3] = {d.CX, d.CY, d.CZ};
+  const BoxRange r = spanBoxes(so.slotSpan, cur, lim);
+  for (int k = 0; k < 3; k++) {
+    so.slotScan[k] = r.lo[k];
+    so.slotScan[3 + k] = r.hi[k];
+  }
+  for (int k = 0; k < 6; k++) so.slotSpan[k] = cur[k];
+  queueValidScan(ctx, b);
+  return ESF_OK;
+}
+
+// after a synchronisation that the queued compactions' counts precede: the
+// counts; a list that outgrew its capacity is grown and scanned again
+esf_status gatherFinish(esf_ctx* ctx) {
+  esf_status st;
+  if (ctx->dofPending) {
+    int total = ctx->hostFlags[kHostDofCount];
+    if (total > ctx->capSolid || (ctx->tightLists && total != ctx->capSolid)) {
+      if ((st = growSolidDofs(ctx, total))) return st;
+      queueSolidDofs(ctx);
+      ESF_HIP(streamSync(ctx));
+      total = ctx->hostFlags[kHostDofCount];
+    }
+    ctx->nSolid = total;
+    ctx->dofPending = false;
+  }
+  for (int b = 0; b < (int)ctx->solids.size(); b++) {
+    DevSolid& so = ctx->solids[b];
+    if (!so.listPending) continue;
+    int total = ctx->hostFlags[kHostValidCount + b];
+    if (total > so.capValid || (ctx->tightLists && total != so.capValid)) {
+      if ((st = growSolidCells(ctx, so, total))) return st;
+      queueValidScan(ctx, b);
+      ESF_HIP(streamSync(ctx));
+      total = ctx->hostFlags[kHostValidCount + b];
+    }
+    so.nValid = total;
+    so.listPending = false;
+    so.listDirty = false;
+  }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
-esf_status gatherSolidMasses(esf_ctx* ctx) {
+// gatherSolidMasses  COUPLED_INTEGRATOR_3D.cpp:327-375.  `defer`: leave the
+// compactions' counts for gatherFinish after the caller's next
+// synchronisation (the DOF count and the valid-cell counts are not read
+// before it); otherwise synchronise and finish here.
+esf_status gatherSolidMasses(esf_ctx* ctx, bool defer) {
   const Dims& d = ctx->d;
   if (ctx->solids.size() > (size_t)kMaxSolids) {
     ctx->err = "too many solids per context";
     return ESF_ERR_INVALID;
   }
+  esf_status st;
   // the gathered fields depend on the solids' masses and quarter fractions
   // only: when neither moved since the last gather (the usual step: they
   // are computed once per scene), the fields and the DOF numbering stand
-  if (ctx->gatheredGen == ctx->massGen && ctx->gatheredSolids == (int)ctx->solids.size()) {
-    esf_status st = ESF_OK;
-    for (auto& so : ctx->solids)
-      if (so.listDirty && (st = buildValidList(ctx, so))) return st;
-    return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
-  }
-  SolidPtrs sp{};
-  sp.n = (int)ctx->solids.size();
-  for (int b = 0; b < sp.n; b++) {
-    sp.mass[b] = ctx->solids[b].mass;
-    sp.qfrac[b] = ctx->solids[b].qfrac;
-  }
-  // the gathered fields are nonzero only at the nodes of the solids' last
-  // mass passes (their dirty cells and one node beyond); recompute those and
-  // the region the last gather left nonzero
-  const int lim[3] = {d.X, d.Y, d.Z};
-  const int full[6] = {0, 0, 0, d.X, d.Y, d.Z};
-  int now[6] = {0, 0, 0, 0, 0, 0};
-  for (auto& so : ctx->solids) {
-    int nb[6];
-    for (int k = 0; k < 3; k++) {
-      nb[k] = so.dirty[k];
-      nb[3 + k] = so.dirty[3 + k] + 1;
+  if (ctx->gatheredGen != ctx->massGen || ctx->gatheredSolids != (int)ctx->solids.size()) {
+    SolidPtrs sp{};
+    sp.n = (int)ctx->solids.size();
+    for (int b = 0; b < sp.n; b++) {
+      sp.mass[b] = ctx->solids[b].mass;
+      sp.qfrac[b] = ctx->solids[b].qfrac;
     }
-    const BoxRange u = spanBoxes(now, so.dirtyAll ? full : nb, lim);
-    for (int k = 0; k < 3; k++) {
-      now[k] = u.lo[k];
-      now[3 + k] = u.hi[k];
+    // the gathered fields are nonzero only at the nodes of the solids' last
+    // mass passes (their dirty cells and one node beyond); recompute those
+    // and the region the last gather left nonzero
+    const int lim[3] = {d.X, d.Y, d.Z};
+    const int full[6] = {0, 0, 0, d.X, d.Y, d.Z};
+    int now[6] = {0, 0, 0, 0, 0, 0};
+    for (auto& so : ctx->solids) {
+      int nb[6];
+      for (int k = 0; k < 3; k++) {
+        nb[k] = so.dirty[k];
+        nb[3 + k] = so.dirty[3 + k] + 1;
+      }
+      const BoxRange u = spanBoxes(now, so.dirtyAll ? full : nb, lim);
+      for (int k = 0; k < 3; k++) {
+        now[k] = u.lo[k];
+        now[3 + k] = u.hi[k];
+      }
     }
+    const BoxRange gb = spanBoxes(ctx->gatherBox, now, lim);
+    for (int k = 0; k < 6; k++) ctx->gatherBox[k] = now[k];
+    if (gb.count() > 0)
+      launch<ESF_K(kGatherMass)>(ctx, dim3(cdiv(gb.count(), 256)), dim3(256), d, sp, gb,
+                                 ctx->solidMass, ctx->nodeFrac);
+    // number the solid DOF nodes.  Outside gb the last gather left no solid
+    // node (sidx = -1) and nothing changed, so the scan walks gb only
+    if ((st = growSolidDofs(ctx, 0))) return st;
+    for (int k = 0; k < 3; k++) {
+      ctx->dofScan[k] = gb.lo[k];
+      ctx->dofScan[3 + k] = gb.hi[k];
+    }
+    queueSolidDofs(ctx);
+    ctx->gatheredGen = ctx->massGen;
+    ctx->gatheredSolids = (int)ctx->solids.size();
   }
-  const BoxRange gb = spanBoxes(ctx->gatherBox, now, lim);
-  for (int k = 0; k < 6; k++) ctx->gatherBox[k] = now[k];
-  if (gb.count() > 0)
-    launch<ESF_K(kGatherMass)>(ctx, dim3(cdiv(gb.count(), 256)), dim3(256), d, sp, gb,
-                               ctx->solidMass, ctx->nodeFrac);
-  // count solid DOF nodes, grow, then number them
-  SolidFlag fl{d, ctx->solidMass};
-  const long n = d.nodes;
-  const int nb = std::max(cdiv(n, kScanTile), 1);
-  launch<ESF_KT(kScanCount, SolidFlag)>(ctx, dim3(nb), dim3(256), n, fl, ctx->scanBlocks);
-  launch<ESF_K(kScanBlocks)>(ctx, dim3(1), dim3(1024), nb, ctx->scanBlocks);
-  toHostAsync(ctx, ctx->hostFlags, ctx->scanBlocks + nb, sizeof(int));
+  for (int b = 0; b < (int)ctx->solids.size(); b++)
+    if (ctx->solids[b].listDirty && (st = queueValidList(ctx, b)))
+      return st;
+  if (defer) return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
   ESF_HIP(streamSync(ctx));
-  const int total = ctx->hostFlags[0];
-  esf_status st = growSolidDofs(ctx, total);
-  if (st) return st;
-  SolidEmit em{ctx->sidx, ctx->solidNodes};
-  launch<ESF_KT(kScanEmit, SolidFlag, SolidEmit)>(ctx, dim3(nb), dim3(256), n, fl,
-                                                  (const int*)ctx->scanBlocks, em);
-  ctx->nSolid = total;
-  ctx->gatheredGen = ctx->massGen;
-  ctx->gatheredSolids = (int)ctx->solids.size();
-  for (auto& so : ctx->solids)
-    if (so.listDirty) {
-      if ((st = buildValidList(ctx, so))) return st;
-    }
-  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+  return gatherFinish(ctx);
 }
 
 // ---------------------------------------------------------------------------
@@ -656,7 +746,7 @@ ESF_BOUNDS(kMassBoxFinish, 256);
 
 // computeSDF's boxes (DEFOR_SOLID_3D.cpp:227-241): the bounding box of the
 // nodes with mass, widened by the tight border and the extrapolation cells.
-// Enqueue writes the box of solid b into pinned hostFlags[8 + 6 b ...];
+// Enqueue writes the box of solid b into pinned hostFlags[kHostBoxes + 6 b ...];
 // finish reads it after a synchronisation.
 esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid) {
   DevSolid& so = ctx->solids[solid];
@@ -664,14 +754,14 @@ esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid) {
   double* part = ctx->partials + 6L * nb * solid;  // one partials region per solid
   launch<ESF_K(kMassBox)>(ctx, dim3(nb), dim3(256), ctx->d, so.mass, (int*)part);
   launch<ESF_K(kMassBoxFinish)>(ctx, dim3(1), dim3(256), (const int*)part, nb, ctx->d.X,
-                                ctx->d.Y, ctx->d.Z, ctx->hostFlags + 8 + 6 * solid);
+                                ctx->d.Y, ctx->d.Z, ctx->hostFlags + kHostBoxes + 6 * solid);
   return ESF_OK;
 }
 
 void updateBoxesFinish(esf_ctx* ctx, int solid) {
   DevSolid& so = ctx->solids[solid];
   const Dims& d = ctx->d;
-  const int* h = ctx->hostFlags + 8 + 6 * solid;
+  const int* h = ctx->hostFlags + kHostBoxes + 6 * solid;
   const int mn[3] = {h[0], h[1], h[2]}, mx[3] = {h[3], h[4], h[5]};
   const int res[3] = {d.X, d.Y, d.Z};
   const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
@@ -785,50 +875,64 @@ esf_status solidPhi(esf_ctx* ctx, int solid) {
 
 // ---------------------------------------------------------------------------
 // CFL  COUPLED_INTEGRATOR_3D.cpp:88-102 (solid DOF nodes are exactly the
-// interior nodes with solid mass)
+// interior nodes with solid mass).  The reference takes dt = min over the
+// nodes of alpha / vmax, starting from 1e9.  Division by a positive vmax
+// and its rounding are monotone, so that minimum is alpha / (max vmax),
+// rounded once: the kernel reduces vmax, and the host divides once alpha
+// is known (the broad phase that picks alpha runs beside it).  Both skip
+// NaN as the reference's std::min does.
 // ---------------------------------------------------------------------------
 template <class GridT>
 __device__ __forceinline__ void kCfl(const GridT& wg, int nSolid, const int* solidNodes,
-                                     const double* vel, double dhInv, double alpha, double* part) {
+                                     const double* vel, double dhInv, double* part) {
   __shared__ double s[256];
-  double dt = 1e9;
+  double vm = 0;
   for (int i = wg.block() * blockDim.x + threadIdx.x; i < nSolid; i += wg.blocks() * blockDim.x) {
     const long n = solidNodes[i];
     const double m = fmax(fmax(fabs(vel[3 * n]), fabs(vel[3 * n + 1])), fabs(vel[3 * n + 2]));
-    const double vmax = m * dhInv;
-    dt = fmin(dt, alpha / vmax);
+    vm = fmax(vm, m * dhInv);
   }
-  s[threadIdx.x] = dt;
+  s[threadIdx.x] = vm;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + st]);
+    if (threadIdx.x < st) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + st]);
     __syncthreads();
   }
   if (threadIdx.x == 0) part[wg.block()] = s[0];
 }
 ESF_BOUNDS(kCfl, 1024);
 
-// the CFL kernel and the copy of its block minima to pinned memory, queued;
-// cflFinish waits for that copy only, so work queued in between (the stress
-// stage) runs while the host waits
+// the CFL kernel and the copy of its block maxima to pinned memory, queued;
+// cflWait waits for that copy only, so work queued in between (the stress
+// stage) runs while the host waits.  `mark` also marks the stream when
+// there is no solid node (a caller that queued other copies before it).
 static int cflBlocks(const esf_ctx* ctx) { return std::min(cdiv(ctx->nSolid, 256), kCflBlocks); }
 
-esf_status cflEnqueue(esf_ctx* ctx, double alpha) {
-  if (ctx->nSolid == 0) return ESF_OK;
-  const int nb = cflBlocks(ctx);
-  launch<ESF_K(kCfl)>(ctx, dim3(nb), dim3(256), ctx->nSolid, ctx->solidNodes, ctx->vel,
-                      ctx->d.dhInv, alpha, ctx->partials);
-  toHostAsync(ctx, ctx->hostCfl, ctx->partials, nb * sizeof(double));
-  ctx->cflMark = markEvent(ctx, ctx->evCfl);
+esf_status cflEnqueue(esf_ctx* ctx, bool mark) {
+  ctx->cflMark = nullptr;
+  if (ctx->nSolid > 0) {
+    const int nb = cflBlocks(ctx);
+    launch<ESF_K(kCfl)>(ctx, dim3(nb), dim3(256), ctx->nSolid, ctx->solidNodes, ctx->vel,
+                        ctx->d.dhInv, ctx->partials);
+    toHostAsync(ctx, ctx->hostCfl, ctx->partials, nb * sizeof(double));
+  }
+  if (ctx->nSolid > 0 || mark) ctx->cflMark = markEvent(ctx, ctx->evCfl);
   return ESF_OK;
 }
 
-esf_status cflFinish(esf_ctx* ctx, double* dtOut) {
+esf_status cflWait(esf_ctx* ctx) {
+  if (ctx->cflMark) ESF_HIP(eventSync(ctx, ctx->cflMark));
+  return ESF_OK;
+}
+
+// after cflWait
+esf_status cflFinish(esf_ctx* ctx, double alpha, double* dtOut) {
   double dt = 1e9;
   if (ctx->nSolid > 0) {
-    ESF_HIP(eventSync(ctx, ctx->cflMark));
     const int nb = cflBlocks(ctx);
-    for (int i = 0; i < nb; i++) dt = std::min(dt, ctx->hostCfl[i]);
+    double vm = 0;
+    for (int i = 0; i < nb; i++) vm = std::fmax(vm, ctx->hostCfl[i]);
+    dt = std::min(dt, alpha / vm);
   }
   dt = std::min(dt, ctx->defaultDt);
   ctx->dt = dt;
@@ -837,8 +941,9 @@ esf_status cflFinish(esf_ctx* ctx, double* dtOut) {
 }
 
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut) {
-  esf_status st = cflEnqueue(ctx, alpha);
-  return st ? st : cflFinish(ctx, dtOut);
+  esf_status st = cflEnqueue(ctx, false);
+  if (!st) st = cflWait(ctx);
+  return st ? st : cflFinish(ctx, alpha, dtOut);
 }
 
 // ---------------------------------------------------------------------------

@@ -31,6 +31,12 @@ struct Dims {
 // 256^3 system uses eight)
 constexpr int kMaxSolids = 8;
 constexpr int kCflBlocks = 256;  // most blocks of the CFL minimum (grid-stride)
+// pinned ints of a context (hostFlags): the solid DOF count, the solids'
+// boxes (6 each from kHostBoxes) and their valid-cell counts
+constexpr int kHostDofCount = 0;
+constexpr int kHostBoxes = 8;
+constexpr int kHostValidCount = kHostBoxes + 6 * kMaxSolids;
+constexpr int kHostFlagInts = kHostValidCount + kMaxSolids;
 
 // Per-solid box, in node coordinates (DEFOR_SOLID_3D.h:138-140)
 struct Box {
@@ -78,6 +84,11 @@ struct DevSolid {
   int nValid = 0, capValid = 0;
   int* cellList = nullptr;   // node index of each valid cell's min corner
   int* cellSlot = nullptr;   // cells -> slot or -1
+  // cells where cellSlot may be >= 0 (the tight box's cell range at the
+  // last list build; [lo, hi) per axis, the whole grid until the first)
+  int slotSpan[6] = {0, 0, 0, 1 << 30, 1 << 30, 1 << 30};
+  int slotScan[6] = {0, 0, 0, 0, 0, 0};  // the cells the queued list scan walks
+  bool listPending = false;  // list scan queued, its count not read yet
   double* cellMat = nullptr; // per slot: R(9) L(9) F(9) Jinv(1)
   double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
@@ -133,6 +144,9 @@ struct esf_ctx {
   // nodes where the gathered mass and fraction may be nonzero ([lo, hi) per
   // axis; the whole grid until the first gather)
   int gatherBox[6] = {0, 0, 0, 1 << 30, 1 << 30, 1 << 30};
+  int dofScan[6] = {0, 0, 0, 0, 0, 0};  // the nodes the queued DOF numbering walks
+  bool dofPending = false;  // DOF numbering queued, its count not read yet
+  bool tightLists = false;  // lists grow to exactly their counts (ESF_TIGHT_LISTS)
   uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
   double* inbox = nullptr;      // solid b at 24 capSolid b: 24 doubles per node
   int inboxSolids = 0;          // solids the inbox was sized for
@@ -445,7 +459,13 @@ esf_status growParticlesFor(esf_ctx* ctx, int solid, long n);
 // mesh; the mesh broad phase: root boxes, 6 doubles per solid, and one
 // collide flag per solid pair i < j)
 esf_status advectMeshes(esf_ctx* ctx);
-esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<int>& pairs);
+esf_status meshBroadPhaseEnqueue(esf_ctx* ctx);
+void meshBroadPhaseRead(esf_ctx* ctx, std::vector<double>& roots, std::vector<int>& pairs);
+// the broad phase of stepImplicit in two halves (esf_scene.cpp): queue
+// (mesh kernels and their read-back, when every solid has a mesh), then,
+// after the stream has passed the read-back, the CFL alpha
+esf_status broadPhaseEnqueue(esf_ctx* ctx, bool* meshes);
+esf_status broadPhaseFinish(esf_ctx* ctx, bool meshes, double* alpha);
 
 }  // namespace esf
 

@@ -195,8 +195,10 @@ esf_status advectMeshes(esf_ctx* ctx) {
 }
 
 // the mesh broad phase: root boxes of every solid's mesh (6 doubles each)
-// and, for every pair i < j in order, whether their meshes collide
-esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<int>& pairs) {
+// and, for every pair i < j in order, whether their meshes collide.  The
+// kernels and the copy of their results to pinned memory are queued here;
+// meshBroadPhaseRead takes the results once the copy has landed.
+esf_status meshBroadPhaseEnqueue(esf_ctx* ctx) {
   const int n = (int)ctx->solids.size();
   const int np = n * (n - 1) / 2;
   double* root = ctx->meshOut;
@@ -221,11 +223,15 @@ esf_status meshBroadPhase(esf_ctx* ctx, std::vector<double>& roots, std::vector<
     }
   // root boxes and flags, laid out as in meshOut, into pinned memory
   toHostAsync(ctx, ctx->hostMesh, root, (6 * kMaxSolids) * sizeof(double) + np * sizeof(int));
-  ESF_HIP(streamSync(ctx));
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+void meshBroadPhaseRead(esf_ctx* ctx, std::vector<double>& roots, std::vector<int>& pairs) {
+  const int n = (int)ctx->solids.size();
+  const int np = n * (n - 1) / 2;
   const int* hflags = reinterpret_cast<const int*>(ctx->hostMesh + 6 * kMaxSolids);
   roots.assign(ctx->hostMesh, ctx->hostMesh + 6 * n);
   pairs.assign(hflags, hflags + np);
-  return ESF_OK;
 }
 
 }  // namespace esf

@@ -486,18 +486,22 @@ static double boxBroadPhase(esf_ctx* ctx) {
   return 1.8;
 }
 
-extern "C" esf_status esf_broad_phase(esf_ctx* ctx, double* alpha) {
-  if (!ctx || !alpha) return ESF_ERR_INVALID;
-  bool meshes = !ctx->solids.empty();
-  for (const DevSolid& so : ctx->solids) meshes = meshes && so.meshV > 0;
+namespace esf {
+esf_status broadPhaseEnqueue(esf_ctx* ctx, bool* meshes) {
+  bool m = !ctx->solids.empty();
+  for (const DevSolid& so : ctx->solids) m = m && so.meshV > 0;
+  *meshes = m;
+  return m ? meshBroadPhaseEnqueue(ctx) : ESF_OK;
+}
+
+esf_status broadPhaseFinish(esf_ctx* ctx, bool meshes, double* alpha) {
   if (!meshes) {
     *alpha = boxBroadPhase(ctx);
     return ESF_OK;
   }
   std::vector<double> roots;
   std::vector<int> pairs;
-  esf_status st = meshBroadPhase(ctx, roots, pairs);
-  if (st) return st;
+  meshBroadPhaseRead(ctx, roots, pairs);
   const double o[3] = {ctx->d.o0, ctx->d.o1, ctx->d.o2};
   const int res[3] = {ctx->d.X, ctx->d.Y, ctx->d.Z};
   bool contact = false;
@@ -513,6 +517,16 @@ extern "C" esf_status esf_broad_phase(esf_ctx* ctx, double* alpha) {
   for (int f : pairs) contact = contact || f != 0;
   *alpha = contact ? 0.6 : 1.8;
   return ESF_OK;
+}
+}  // namespace esf
+
+extern "C" esf_status esf_broad_phase(esf_ctx* ctx, double* alpha) {
+  if (!ctx || !alpha) return ESF_ERR_INVALID;
+  bool meshes = false;
+  esf_status st = broadPhaseEnqueue(ctx, &meshes);
+  if (st) return st;
+  if (meshes) ESF_HIP(streamSync(ctx));
+  return broadPhaseFinish(ctx, meshes, alpha);
 }
 
 extern "C" esf_status esf_frame(esf_ctx* ctx, double frameDt, double alpha, int* substeps) {

@@ -65,3 +65,43 @@ def jt_lambda(o, lam):
     """-J^T lambda over the interior DOFs in the reference's solution order
     (the pressure compared modulo the null space of S)"""
     return o.apply_negJT(lam)
+
+
+def _solve_distance(p, q, r, v):
+    """solveDistance (DEFOR_SOLID_3D.cpp:369-395) elementwise, in the same
+    operation order as the device's, so results are bitwise comparable"""
+    with np.errstate(invalid="ignore"):
+        d0 = np.fmin(p, np.fmin(q, r))
+        d2 = np.fmax(p, np.fmax(q, r))
+        d1 = p + q + r - d0 - d2
+        b0, c0 = d0, d0 * d0 - 1.0
+        b1, c1 = d1 + b0, d1 * d1 + c0
+        b2, c2 = d2 + b1, d2 * d2 + c1
+        disc = b2 * b2 - 3.0 * c2
+        nd = (b2 + np.sqrt(disc)) / 3.0
+        go = (disc < 0.0) | (nd < d2)
+        disc = b1 * b1 - 2.0 * c1
+        nd = np.where(go & (disc > 0.0), (b1 + np.sqrt(disc)) / 2.0, nd)
+        go = go & ((disc < 0.0) | (nd < d1))
+        disc = b0 * b0 - 1.0 * c0
+        nd = np.where(go & (disc > 0.0), (b0 + np.sqrt(disc)) / 1.0, nd)
+    return np.where(nd < v, nd, v)
+
+
+def jacobi_phi(mass, ext, X, Y, Z, relaxations):
+    """the box's phi after `relaxations` whole-box Jacobi relaxations of the
+    upwind update from phi = -0.5 on nodes with mass, `large` elsewhere;
+    neighbours outside the extended box `ext` ([lo, hi) per axis, x first)
+    read as `large`, nodes with mass keep their value.  Returns the box's
+    values, (z, y, x) ordered."""
+    large = float(X + Y + Z + 2)
+    m = np.asarray(mass).reshape(Z, Y, X)[ext[2]:ext[5], ext[1]:ext[4], ext[0]:ext[3]]
+    cur = np.where(m > 0, -0.5, large)
+    free = m == 0
+    for _ in range(relaxations):
+        pad = np.pad(cur, 1, constant_values=large)
+        p = np.fmin(pad[1:-1, 1:-1, :-2], pad[1:-1, 1:-1, 2:])
+        q = np.fmin(pad[1:-1, :-2, 1:-1], pad[1:-1, 2:, 1:-1])
+        r = np.fmin(pad[:-2, 1:-1, 1:-1], pad[2:, 1:-1, 1:-1])
+        cur = np.where(free, _solve_distance(p, q, r, cur), cur)
+    return cur

@@ -128,3 +128,33 @@ def test_batched_server_results_same_as_alone(esf):
         g.close()
         for a, b in zip(alone, got[r["id"]]):
             np.testing.assert_array_equal(a, b)
+
+
+def test_lists_grown_after_their_scan_same_bits(esf, monkeypatch):
+    """A step reads the solid DOF and valid-cell counts only at its end: a
+    list whose scan ran past its capacity is grown there and scanned again.
+    ESF_TIGHT_LISTS=1 sizes the lists to exactly their last counts, so every
+    step whose count rises takes that path.  A thrown ball and a random
+    scene over 9 steps, alone and batched: the same bits and reports as with
+    the usual headroom, and the counts do rise."""
+    from eulersf_amd.cfg import unit_cube_ball_cfg
+    from eulersf_amd.serve import random_scene
+    rng = np.random.default_rng(17)
+    thrown = unit_cube_ball_cfg(20, eps=1e-10, velocity=(0.5, -1.4, 0.3))
+    texts = [thrown, random_scene(rng, 22, eps=1e-10)]
+    steps = 9
+    usual = run_alone(esf, texts, esf.ESF_MIXED, steps)
+    monkeypatch.setenv("ESF_TIGHT_LISTS", "1")
+    tight = run_alone(esf, texts, esf.ESF_MIXED, steps)
+    rises = 0
+    for (fu, ru), (ft, rt) in zip(usual, tight):
+        for a, b in zip(fu, ft):
+            np.testing.assert_array_equal(a, b)
+        assert ru == rt
+        for key in ("n_solid_nodes", "n_solid_cells"):
+            seq = [r[key] for r in rt]
+            rises += sum(b > a for a, b in zip(seq, seq[1:]))
+    _say(f"tight lists: same bits alone, {rises} count rises")
+    assert rises > 0
+    check_batched(esf, texts, esf.ESF_MIXED, steps, usual)
+    _say("tight lists: same bits batched")

@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests._scenes import jt_lambda, rel_err, teacher_force, tight_box_phi_above_2
+from tests._scenes import jacobi_phi, jt_lambda, rel_err, teacher_force, tight_box_phi_above_2
 
 pytestmark = pytest.mark.gpu
 
@@ -179,6 +179,12 @@ def test_solid_mass_and_sdf_stage_from_the_same_state(esf):
     near = po <= 3.0
     assert np.abs(pg[near] - po[near]).max() < 1e-4
     assert np.all(pg >= po - 1e-12)  # an upper bound further out
+    # the device relaxes several times per launch in LDS tiles: bitwise the
+    # 8 whole-box Jacobi relaxations it stands for
+    ext = g.boxes(0)[6:]
+    want = jacobi_phi(g.download(A.S_MASS, 0), ext, X, Y, Z, 8)
+    got = pg.reshape(Z, Y, X)[ext[2]:ext[5], ext[1]:ext[4], ext[0]:ext[3]]
+    np.testing.assert_array_equal(got.view(np.int64), want.view(np.int64))
     g.close()
     o.close()
 
