@@ -12,6 +12,8 @@
 namespace esf {
 // esf_pressure.hip
 esf_status pressureRhs(esf_ctx* ctx);
+esf_status pressureRhsDeferDiag(esf_ctx* ctx);
+esf_status pressureDiagFlush(esf_ctx* ctx);
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
@@ -511,7 +513,15 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
     case ESF_F_SOLID_MV: *ptr = ctx->sMv; *bytes = 8 * NS; break;
     case ESF_F_DAMPED_MASS: *ptr = ctx->sDamped; *bytes = 8 * NS; break;
     case ESF_F_SYS_DIAG: *ptr = ctx->sAdiag; *bytes = 8 * NS; break;
-    case ESF_F_PRESSURE_DIAG: *ptr = ctx->D; *bytes = 8 * C; break;
+    case ESF_F_PRESSURE_DIAG:
+      if (ctx->diagPending) {
+        esf_status st = pressureDiagFlush(ctx);
+        if (st) return st;
+        ESF_HIP(streamSync(ctx));
+      }
+      *ptr = ctx->D;
+      *bytes = 8 * C;
+      break;
     case ESF_F_SOLID_V0: *ptr = ctx->sV0; *bytes = 8 * NS; break;
     case ESF_F_SOLID_RHS:  // writable: the right-hand side of esf_solid_solve_resident
       *ptr = ctx->sRhs;
@@ -820,7 +830,9 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
   batchBarrier(ctx);
   {
     StageTimer t(ctx, 3);
-    if ((st = pressureRhs(ctx))) return st;
+    // the mixed form forms the diagonal only where its inner solve runs
+    if ((st = ctx->p.precision == ESF_MIXED ? pressureRhsDeferDiag(ctx) : pressureRhs(ctx)))
+      return st;
     if (!ctx->lambdaInit) {
       if ((st = zeroD(ctx, ctx->d.cells, ctx->lambda))) return st;
       ctx->lambdaInit = true;
@@ -1072,7 +1084,9 @@ esf_status esf_pressure_solve(esf_ctx* ctx, const double* b, double* x, double e
   int it = 0;
   bool conv = false;
   double rr = 0;
-  esf_status st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->Dinv, ctx->lambda, ctx->r,
+  esf_status st = pressureDiagFlush(ctx);
+  if (st) return st;
+  st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->Dinv, ctx->lambda, ctx->r,
                                   ctx->r2, ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 3);
   if (st) return st;
   ESF_HIP(hipMemcpyAsync(x, ctx->lambda, bytes, hipMemcpyDeviceToHost, ctx->stream));

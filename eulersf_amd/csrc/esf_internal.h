@@ -147,6 +147,7 @@ struct esf_ctx {
   int dofScan[6] = {0, 0, 0, 0, 0, 0};  // the nodes the queued DOF numbering walks
   bool dofPending = false;  // DOF numbering queued, its count not read yet
   bool tightLists = false;  // lists grow to exactly their counts (ESF_TIGHT_LISTS)
+  bool diagPending = false; // the pressure diagonal of the current masses not formed yet
   uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
   double* inbox = nullptr;      // solid b at 24 capSolid b: 24 doubles per node
   int inboxSolids = 0;          // solids the inbox was sized for

@@ -460,15 +460,16 @@ ESF_BOUNDS(kSpmv, 256);
 
 // b = J v0 - c, c (boundary velocities), D = diag(S); also the FP32 copies
 // used by the mixed solve.  COUPLED_INTEGRATOR_3D.cpp:286-325 (c),
-// :694-709 (D, b).
+// :694-709 (D, b).  D == nullptr: b and c only (the mixed step forms the
+// diagonal with kPressureDiag when its inner solve first needs it).
 template <class GridT>
 __device__ __forceinline__ void kPressureRhs(const GridT& wg, Dims d, const double* __restrict__ v0,
                                              const double* __restrict__ vel,
                                              const double* __restrict__ minv, double* c, double* b,
                                              double* D, double* Dinv, float* D32, float* Dinv32) {
-  const long r = wg.block() * (long)blockDim.x + threadIdx.x;
+  const int r = wg.block() * blockDim.x + threadIdx.x;
   if (r >= d.cells) return;
-  const int cx = r % d.CX, cy = (r / d.CX) % d.CY, cz = r / d.cslab;
+  const int q = r / d.CX, cx = r - q * d.CX, cy = q % d.CY, cz = q / d.CY;
   const long base = cx + (long)cy * d.X + (long)cz * d.slab;
   double tmp = 0, cc = 0, dd = 0;
   for (int k = 0; k < 8; k++) {
@@ -482,10 +483,12 @@ __device__ __forceinline__ void kPressureRhs(const GridT& wg, Dims d, const doub
       tmp += sx * v0[3 * n];
       tmp += sy * v0[3 * n + 1];
       tmp += sz * v0[3 * n + 2];
-      const double mi = minv[n];
-      dd += mi * 1.0;
-      dd += mi * 1.0;
-      dd += mi * 1.0;
+      if (D) {
+        const double mi = minv[n];
+        dd += mi * 1.0;
+        dd += mi * 1.0;
+        dd += mi * 1.0;
+      }
     } else {
       cc -= sx * vel[3 * n];
       cc -= sy * vel[3 * n + 1];
@@ -494,14 +497,46 @@ __device__ __forceinline__ void kPressureRhs(const GridT& wg, Dims d, const doub
   }
   c[r] = cc;
   b[r] = tmp - cc;
-  D[r] = dd;
-  Dinv[r] = 1.0 / dd;
-  if (D32) {
-    D32[r] = (float)dd;
-    Dinv32[r] = (float)(1.0 / dd);
+  if (D) {
+    D[r] = dd;
+    Dinv[r] = 1.0 / dd;
+    if (D32) {
+      D32[r] = (float)dd;
+      Dinv32[r] = (float)(1.0 / dd);
+    }
   }
 }
 ESF_BOUNDS(kPressureRhs, 1024);
+
+// D = diag(S), its reciprocals and their FP32 copies, and minv in FP32, for
+// the mixed solve's inner CR: the sums of kPressureRhs in the same order
+template <class GridT>
+__device__ __forceinline__ void kPressureDiag(const GridT& wg, Dims d,
+                                              const double* __restrict__ minv, double* D,
+                                              double* Dinv, float* D32, float* Dinv32,
+                                              float* minv32) {
+  const int t = wg.block() * blockDim.x + threadIdx.x;
+  if (t < d.nodes) minv32[t] = (float)minv[t];
+  if (t >= d.cells) return;
+  const int q = t / d.CX, cx = t - q * d.CX, cy = q % d.CY, cz = q / d.CY;
+  const long base = cx + (long)cy * d.X + (long)cz * d.slab;
+  double dd = 0;
+  for (int k = 0; k < 8; k++) {
+    const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+    const int ni = cx + ox, nj = cy + oy, nk = cz + oz;
+    if (ni >= 1 && ni <= d.X - 2 && nj >= 1 && nj <= d.Y - 2 && nk >= 1 && nk <= d.Z - 2) {
+      const double mi = minv[base + ox + (long)oy * d.X + (long)oz * d.slab];
+      dd += mi * 1.0;
+      dd += mi * 1.0;
+      dd += mi * 1.0;
+    }
+  }
+  D[t] = dd;
+  Dinv[t] = 1.0 / dd;
+  D32[t] = (float)dd;
+  Dinv32[t] = (float)(1.0 / dd);
+}
+ESF_BOUNDS(kPressureDiag, 1024);
 
 // ---------------------------------------------------------------------------
 // host side
@@ -581,9 +616,33 @@ int spmvBlocks(const esf_ctx* ctx) {
 
 esf_status pressureRhs(esf_ctx* ctx) {
   const Dims& d = ctx->d;
+  const bool mixed = ctx->p.precision == ESF_MIXED;
   launch<ESF_K(kPressureRhs)>(ctx, dim3(cdiv(d.cells, 256)), dim3(256), d, ctx->v0, ctx->vel,
                               ctx->minv, ctx->c, ctx->b, ctx->D, ctx->Dinv,
-                              ctx->p.precision == ESF_MIXED ? ctx->D32 : nullptr, ctx->Dinv32);
+                              mixed ? ctx->D32 : nullptr, ctx->Dinv32);
+  ctx->diagPending = false;
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+// the mixed step's right-hand side: b and c only.  The diagonal and minv in
+// FP32 follow (pressureDiagFlush) only where an inner solve runs, or where
+// a caller reads the diagonal.
+esf_status pressureRhsDeferDiag(esf_ctx* ctx) {
+  const Dims& d = ctx->d;
+  launch<ESF_K(kPressureRhs)>(ctx, dim3(cdiv(d.cells, 256)), dim3(256), d, ctx->v0, ctx->vel,
+                              ctx->minv, ctx->c, ctx->b, (double*)nullptr, (double*)nullptr,
+                              (float*)nullptr, (float*)nullptr);
+  ctx->diagPending = true;
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status pressureDiagFlush(esf_ctx* ctx) {
+  if (!ctx->diagPending) return ESF_OK;
+  const Dims& d = ctx->d;
+  launch<ESF_K(kPressureDiag)>(ctx, dim3(cdiv(std::max(d.nodes, d.cells), 256)), dim3(256), d,
+                               (const double*)ctx->minv, ctx->D, ctx->Dinv, ctx->D32,
+                               ctx->Dinv32, ctx->minv32);
+  ctx->diagPending = false;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
@@ -797,8 +856,6 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
   const Dims& d = ctx->d;
   esf_status st = initScalars(ctx, 0, eps);
   if (st) return st;
-  launch<ESF_K(kToFloat)>(ctx, dim3(1024), dim3(256), d.nodes, (const double*)ctx->minv,
-                          ctx->minv32);
   SpmvArgs<double> o{};
   o.minv = ctx->minv;
   o.b = ctx->b;
@@ -827,6 +884,16 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     // (418 / 414 / 353 / 276 steps/s).
     double need = h.resnorm > 0 ? 0.5 * eps * h.bnorm / h.resnorm : 1.0;
     double innerEps = std::min(0.5, std::max(need, ctx->p.inner_eps));
+    if (outer == 0) {
+      // the inner solve's operands: minv in FP32, and the diagonal where
+      // the step left it for here
+      if (ctx->diagPending) {
+        if ((st = pressureDiagFlush(ctx))) return st;
+      } else {
+        launch<ESF_K(kToFloat)>(ctx, dim3(1024), dim3(256), d.nodes, (const double*)ctx->minv,
+                                ctx->minv32);
+      }
+    }
     launch<ESF_K(kZeroF)>(ctx, dim3(1024), dim3(256), d.cells, ctx->e32);
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->Dinv32, ctx->e32,
                              ctx->r32,

@@ -179,8 +179,8 @@ def test_solid_mass_and_sdf_stage_from_the_same_state(esf):
     near = po <= 3.0
     assert np.abs(pg[near] - po[near]).max() < 1e-4
     assert np.all(pg >= po - 1e-12)  # an upper bound further out
-    # the device relaxes several times per launch in LDS tiles: bitwise the
-    # 8 whole-box Jacobi relaxations it stands for
+    # the device's phi: bitwise 8 whole-box Jacobi relaxations of the
+    # upwind update (the relaxation the host test pins against the sweeps)
     ext = g.boxes(0)[6:]
     want = jacobi_phi(g.download(A.S_MASS, 0), ext, X, Y, Z, 8)
     got = pg.reshape(Z, Y, X)[ext[2]:ext[5], ext[1]:ext[4], ext[0]:ext[3]]
