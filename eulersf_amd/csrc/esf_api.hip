@@ -40,8 +40,9 @@ esf_status gatherFinish(esf_ctx* ctx);
 esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf);
 esf_status updateBoxes(esf_ctx* ctx, int solid);
 esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid);
-void updateBoxesFinish(esf_ctx* ctx, int solid);
-esf_status solidPhi(esf_ctx* ctx, int solid);
+void updateBoxesFinish(esf_ctx* ctx, int solid, bool markList);
+esf_status uploadBox(esf_ctx* ctx, int solid);
+esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
 esf_status forkBodyForces(esf_ctx* ctx);
@@ -253,6 +254,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
     if (!st) st = dalloc(ctx, &ctx->dvPrev, 3 * N);
   }
   if (!st) st = dalloc(ctx, &ctx->sidx, N);
+  if (!st) st = dalloc(ctx, &ctx->dBoxes, kMaxSolids);
   const long scanBlocks = std::max<long>(N, C) / 1024 + 2;
   if (!st) st = dalloc(ctx, &ctx->scanBlocks, scanBlocks);
   // up to 5 partials per block of the widest reducing launch (the pressure
@@ -383,6 +385,7 @@ esf_status esf_solid_add(esf_ctx* ctx, double lambda, double mu, double density,
                          hipMemcpyHostToDevice, ctx->stream));
   fillInt(ctx, d.cells, so.cellSlot, -1);
   ESF_HIP(streamSync(ctx));
+  if ((st = uploadBox(ctx, (int)ctx->solids.size() - 1))) return st;
   if (id) *id = (int)ctx->solids.size() - 1;
   return ESF_OK;
 }
@@ -471,7 +474,7 @@ esf_status esf_solid_set_boxes(esf_ctx* ctx, int solid, const int b[12]) {
     bx.bMax[k] = b[9 + k];
   }
   ctx->solids[solid].listDirty = true;
-  return ESF_OK;
+  return uploadBox(ctx, solid);
 }
 
 esf_status esf_solid_get_boxes(esf_ctx* ctx, int solid, int b[12]) {
@@ -756,7 +759,7 @@ esf_status esf_set_particles(esf_ctx* ctx, int solid, long n, const double* pos,
 esf_status esf_solid_sdf(esf_ctx* ctx, int solid) {
   if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
   esf_status st = updateBoxes(ctx, solid);
-  if (!st) st = solidPhi(ctx, solid);
+  if (!st) st = solidPhi(ctx, solid, ctx->solids[solid].box);
   if (st) return st;
   ESF_HIP(streamSync(ctx));
   return ESF_OK;
@@ -940,11 +943,27 @@ struct StepCount {
 };
 }  // namespace
 
+// a step that ended early (an error) may leave box reads and list counts
+// queued: settle them before anything reads the host copies
+static esf_status settlePending(esf_ctx* ctx) {
+  bool any = ctx->dofPending;
+  for (auto& so : ctx->solids) any = any || so.boxPending || so.listPending;
+  if (!any) return ESF_OK;
+  ESF_HIP(streamSync(ctx));
+  for (int b = 0; b < (int)ctx->solids.size(); b++)
+    if (ctx->solids[b].boxPending) {
+      updateBoxesFinish(ctx, b, false);
+      ctx->solids[b].boxPending = false;
+    }
+  return gatherFinish(ctx);
+}
+
 esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if (!ctx) return ESF_ERR_INVALID;
   const StepCount inFlight;
   double dt;
   esf_status st, solveSt;
+  if ((st = settlePending(ctx))) return st;
   for (auto& so : ctx->solids)
     if (so.listDirty) {
       ctx->err = "solid masses changed: call esf_gather_solid_masses first";
@@ -1014,10 +1033,23 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
     if ((st = advectMeshes(ctx))) return st;
     for (int b = 0; b < (int)ctx->solids.size(); b++)
       if ((st = updateBoxesEnqueue(ctx, b))) return st;
-    ESF_HIP(streamSync(ctx));
-    for (int b = 0; b < (int)ctx->solids.size(); b++) {
-      updateBoxesFinish(ctx, b);
-      if ((st = solidPhi(ctx, b))) return st;
+    if (ctx->p.flip) {
+      // the FLIP stages of this step and the next read the boxes on the host
+      ESF_HIP(streamSync(ctx));
+      for (int b = 0; b < (int)ctx->solids.size(); b++) {
+        updateBoxesFinish(ctx, b, true);
+        if ((st = solidPhi(ctx, b, ctx->solids[b].box))) return st;
+      }
+    } else {
+      // the stages up to the step's end read the boxes from device memory
+      // and launch over boxes that contain them; the host copy is read at
+      // the end
+      for (int b = 0; b < (int)ctx->solids.size(); b++) {
+        DevSolid& so = ctx->solids[b];
+        so.boxPending = true;
+        so.listDirty = true;
+        if ((st = solidPhi(ctx, b, so.boxBound))) return st;
+      }
     }
   }
   batchBarrier(ctx);
@@ -1036,8 +1068,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;
-  ESF_HIP(streamSync(ctx));
-  if ((st = gatherFinish(ctx))) return st;
+  if ((st = settlePending(ctx))) return st;
   solveSt = coupledSolveReport(ctx, &s);
   if (stats) *stats = s;
   return solveSt;

@@ -250,11 +250,12 @@ struct SolidEmit {
 
 // valid-cell list of one solid: cells of the tight box with _valid == 1
 struct ValidFlag {
-  Box box;
+  const Box* box;  // the solid's boxes in device memory
   const int8_t* valid;
   __device__ bool operator()(int x, int y, int z, long c) const {
-    return x >= box.tMin[0] && x < box.tMax[0] - 1 && y >= box.tMin[1] &&
-           y < box.tMax[1] - 1 && z >= box.tMin[2] && z < box.tMax[2] - 1 && valid[c] == 1;
+    const Box b = *box;
+    return x >= b.tMin[0] && x < b.tMax[0] - 1 && y >= b.tMin[1] && y < b.tMax[1] - 1 &&
+           z >= b.tMin[2] && z < b.tMax[2] - 1 && valid[c] == 1;
   }
 };
 struct ValidEmit {
@@ -296,21 +297,22 @@ static void queueValidScan(esf_ctx* ctx, int b) {
   DevSolid& so = ctx->solids[b];
   const Dims& d = ctx->d;
   const BoxRange r = rangeOf(so.slotScan);
-  compactEnqueue(ctx, BoxWalk{r, d.CX, d.cslab}, r.count(), ValidFlag{so.box, so.valid},
+  compactEnqueue(ctx, BoxWalk{r, d.CX, d.cslab}, r.count(), ValidFlag{ctx->dBoxes + b, so.valid},
                  ValidEmit{d, so.cellSlot, so.cellList, so.capValid},
                  ctx->hostFlags + kHostValidCount + b);
   so.listPending = true;
 }
 
 // the valid-cell list of solid b, queued.  The scan walks the cells of the
-// tight box (ValidFlag's range) and of the last build's, where cellSlot may
-// be >= 0; every other cell's slot is -1 already and stays so.
+// tight box (ValidFlag's range; a box that contains it while the solid's
+// boxes are still on their way to the host) and of the last build's, where
+// cellSlot may be >= 0; every other cell's slot is -1 already and stays so.
 static esf_status queueValidList(esf_ctx* ctx, int b) {
   DevSolid& so = ctx->solids[b];
   const Dims& d = ctx->d;
   esf_status st = growSolidCells(ctx, so, 0);
   if (st) return st;
-  const Box& bx = so.box;
+  const Box& bx = so.boxPending ? so.boxBound : so.box;
   const int cur[6] = {bx.tMin[0], bx.tMin[1], bx.tMin[2],
                       bx.tMax[0] - 1, bx.tMax[1] - 1, bx.tMax[2] - 1};
   const int lim[3] = {d.CX, d.CY, d.CZ};
@@ -679,21 +681,23 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
-// bounding box of nodes with mass > 0 (computeSDF :227-241): per-block
-// min/max of x, y, z into partials, finished on the host
+// bounding box of nodes with mass > 0 (computeSDF :227-241) over the node
+// range r (where the mass can be nonzero): per-block min/max of x, y, z
 template <class GridT>
-__device__ __forceinline__ void kMassBox(const GridT& wg, Dims d, const double* mass, int* part) {
+__device__ __forceinline__ void kMassBox(const GridT& wg, Dims d, BoxRange r, const double* mass,
+                                         int* part) {
   __shared__ int s[6][256];
   int mn[3] = {1 << 30, 1 << 30, 1 << 30}, mx[3] = {-1, -1, -1};
-  for (long n = wg.block() * (long)blockDim.x + threadIdx.x; n < d.nodes;
-       n += (long)wg.blocks() * blockDim.x) {
-    if (mass[n] > 0) {
-      const int c[3] = {(int)(n % d.X), (int)((n / d.X) % d.Y), (int)(n / d.slab)};
+  const int nx = r.hi[0] - r.lo[0], ny = r.hi[1] - r.lo[1];
+  const int cnt = (int)r.count();
+  for (int t = wg.block() * blockDim.x + threadIdx.x; t < cnt; t += wg.blocks() * blockDim.x) {
+    const int q = t / nx;
+    const int c[3] = {r.lo[0] + (t - q * nx), r.lo[1] + q % ny, r.lo[2] + q / ny};
+    if (mass[c[0] + (long)c[1] * d.X + (long)c[2] * d.slab] > 0)
       for (int k = 0; k < 3; k++) {
         mn[k] = min(mn[k], c[k]);
         mx[k] = max(mx[k], c[k]);
       }
-    }
   }
   for (int k = 0; k < 3; k++) {
     s[k][threadIdx.x] = mn[k];
@@ -713,12 +717,29 @@ __device__ __forceinline__ void kMassBox(const GridT& wg, Dims d, const double* 
 }
 ESF_BOUNDS(kMassBox, 1024);
 
+// the boxes of computeSDF from the node bounds mn, mx: the tight box widened
+// by the tight border, the extended box by the extrapolation cells, both
+// clipped to the grid
+__host__ __device__ inline Box boxesFrom(const int mn[3], const int mx[3], const int res[3],
+                                         int tb, int ex) {
+  Box b;
+  for (int k = 0; k < 3; k++) {
+    b.tMin[k] = mn[k] - tb > 0 ? mn[k] - tb : 0;
+    b.tMax[k] = mx[k] + tb < res[k] ? mx[k] + tb : res[k];
+    b.bMin[k] = mn[k] - ex > 0 ? mn[k] - ex : 0;
+    b.bMax[k] = mx[k] + ex < res[k] ? mx[k] + ex : res[k];
+  }
+  return b;
+}
+
 // the per-block boxes of kMassBox into one (blocks with no massive node,
 // max -1, left out; none at all: min = resolution, max = 0), as the host
-// would, into `out` (6 ints, pinned host memory)
+// would: the node bounds into `out` (6 ints, pinned host memory) and the
+// solid's boxes into `box` (device memory, read by the later stages of the
+// step without a round trip to the host)
 template <class GridT>
 __device__ __forceinline__ void kMassBoxFinish(const GridT& wg, const int* part, int nb, int X,
-                                               int Y, int Z, int* out) {
+                                               int Y, int Z, int tb, int ex, Box* box, int* out) {
   __shared__ int s[6][256];
   int mn[3] = {X, Y, Z}, mx[3] = {0, 0, 0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
@@ -741,44 +762,70 @@ __device__ __forceinline__ void kMassBoxFinish(const GridT& wg, const int* part,
     __syncthreads();
   }
   if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x == 0) {
+    const int bmn[3] = {s[0][0], s[1][0], s[2][0]}, bmx[3] = {s[3][0], s[4][0], s[5][0]};
+    const int res[3] = {X, Y, Z};
+    *box = boxesFrom(bmn, bmx, res, tb, ex);
+  }
 }
 ESF_BOUNDS(kMassBoxFinish, 256);
 
 // computeSDF's boxes (DEFOR_SOLID_3D.cpp:227-241): the bounding box of the
 // nodes with mass, widened by the tight border and the extrapolation cells.
-// Enqueue writes the box of solid b into pinned hostFlags[kHostBoxes + 6 b ...];
-// finish reads it after a synchronisation.
+// Enqueue scans the nodes where the mass can be nonzero (the corners of the
+// last mass pass's cells, or the whole grid after an upload or in FLIP
+// scenes, whose particles set masses), writes the solid's boxes to device
+// memory and the node bounds into pinned hostFlags[kHostBoxes + 6 b ...],
+// and keeps in so.boxBound boxes that contain whatever it finds; finish
+// reads the bounds after a synchronisation.
 esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid) {
   DevSolid& so = ctx->solids[solid];
+  const Dims& d = ctx->d;
   const int nb = 256;
+  const int res[3] = {d.X, d.Y, d.Z};
+  const int full[6] = {0, 0, 0, d.X, d.Y, d.Z};
+  const int none[6] = {0, 0, 0, 0, 0, 0};
+  const int corners[6] = {so.dirty[0], so.dirty[1], so.dirty[2],
+                          so.dirty[3] + 1, so.dirty[4] + 1, so.dirty[5] + 1};
+  const BoxRange r = spanBoxes(so.dirtyAll || ctx->p.flip ? full : corners, none, res);
   double* part = ctx->partials + 6L * nb * solid;  // one partials region per solid
-  launch<ESF_K(kMassBox)>(ctx, dim3(nb), dim3(256), ctx->d, so.mass, (int*)part);
-  launch<ESF_K(kMassBoxFinish)>(ctx, dim3(1), dim3(256), (const int*)part, nb, ctx->d.X,
-                                ctx->d.Y, ctx->d.Z, ctx->hostFlags + kHostBoxes + 6 * solid);
-  return ESF_OK;
+  launch<ESF_K(kMassBox)>(ctx, dim3(nb), dim3(256), d, r, so.mass, (int*)part);
+  launch<ESF_K(kMassBoxFinish)>(ctx, dim3(1), dim3(256), (const int*)part, nb, d.X, d.Y, d.Z,
+                                ctx->p.tight_border, ctx->p.extrapolation_cells,
+                                ctx->dBoxes + solid, ctx->hostFlags + kHostBoxes + 6 * solid);
+  const int mx[3] = {r.hi[0] - 1, r.hi[1] - 1, r.hi[2] - 1};
+  so.boxBound = r.count() > 0 ? boxesFrom(r.lo, mx, res, ctx->p.tight_border,
+                                          ctx->p.extrapolation_cells)
+                              : Box{};
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 
-void updateBoxesFinish(esf_ctx* ctx, int solid) {
+// markList: the valid-cell list follows the new tight box (a caller that
+// built it from the device copy already passes false)
+void updateBoxesFinish(esf_ctx* ctx, int solid, bool markList) {
   DevSolid& so = ctx->solids[solid];
   const Dims& d = ctx->d;
   const int* h = ctx->hostFlags + kHostBoxes + 6 * solid;
   const int mn[3] = {h[0], h[1], h[2]}, mx[3] = {h[3], h[4], h[5]};
   const int res[3] = {d.X, d.Y, d.Z};
-  const int tb = ctx->p.tight_border, ex = ctx->p.extrapolation_cells;
-  for (int k = 0; k < 3; k++) {
-    so.box.tMin[k] = std::max(mn[k] - tb, 0);
-    so.box.tMax[k] = std::min(mx[k] + tb, res[k]);
-    so.box.bMin[k] = std::max(mn[k] - ex, 0);
-    so.box.bMax[k] = std::min(mx[k] + ex, res[k]);
-  }
-  so.listDirty = true;
+  so.box = boxesFrom(mn, mx, res, ctx->p.tight_border, ctx->p.extrapolation_cells);
+  if (markList) so.listDirty = true;
+}
+
+// the host's copy of a solid's boxes into device memory (after a change
+// that did not come from kMassBoxFinish)
+esf_status uploadBox(esf_ctx* ctx, int solid) {
+  ESF_HIP(hipMemcpyAsync(ctx->dBoxes + solid, &ctx->solids[solid].box, sizeof(Box),
+                         hipMemcpyHostToDevice, ctx->stream));
+  ESF_HIP(streamSync(ctx));
+  return ESF_OK;
 }
 
 esf_status updateBoxes(esf_ctx* ctx, int solid) {
   esf_status st = updateBoxesEnqueue(ctx, solid);
   if (st) return st;
   ESF_HIP(streamSync(ctx));
-  updateBoxesFinish(ctx, solid);
+  updateBoxesFinish(ctx, solid, true);
   return ESF_OK;
 }
 
@@ -803,13 +850,15 @@ esf_status updateBoxes(esf_ctx* ctx, int solid) {
 // ---------------------------------------------------------------------------
 constexpr int kPhiRelaxations = 8;
 
-// phi = -0.5 on nodes with mass inside the box, large everywhere else
+// phi = -0.5 on nodes with mass inside the box, large everywhere else (the
+// box read from device memory)
 template <class GridT>
-__device__ __forceinline__ void kPhiInit(const GridT& wg, Dims d, Box bx,
+__device__ __forceinline__ void kPhiInit(const GridT& wg, Dims d, const Box* box,
                                          const double* __restrict__ mass, double large, double* phi,
                                          double* phi2) {
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
+  const Box bx = *box;
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   const bool in = x >= bx.bMin[0] && x < bx.bMax[0] && y >= bx.bMin[1] && y < bx.bMax[1] &&
                   z >= bx.bMin[2] && z < bx.bMax[2];
@@ -820,54 +869,65 @@ __device__ __forceinline__ void kPhiInit(const GridT& wg, Dims d, Box bx,
 ESF_BOUNDS(kPhiInit, 1024);
 
 // one relaxation over the box: nodes without mass take the upwind update
-// of their neighbours' previous values
+// of their neighbours' previous values.  The launch covers the extended
+// box of `r` (a box that contains the solid's); the box itself comes from
+// device memory.
 template <class GridT>
-__device__ __forceinline__ void kPhiRelax(const GridT& wg, Dims d, Box bx,
+__device__ __forceinline__ void kPhiRelax(const GridT& wg, Dims d, const Box* box, BoxRange r,
                                           const double* __restrict__ mass,
                                           const double* __restrict__ in, double* out, double large) {
-  const int nx = bx.bMax[0] - bx.bMin[0], ny = bx.bMax[1] - bx.bMin[1];
-  const long nb = (long)nx * ny * (bx.bMax[2] - bx.bMin[2]);
-  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
-  if (t >= nb) return;
-  const int x = bx.bMin[0] + (int)(t % nx), y = bx.bMin[1] + (int)((t / nx) % ny),
-            z = bx.bMin[2] + (int)(t / ((long)nx * ny));
+  const int nx = r.hi[0] - r.lo[0], ny = r.hi[1] - r.lo[1];
+  const int t = wg.block() * blockDim.x + threadIdx.x;
+  if (t >= (int)r.count()) return;
+  const int q = t / nx;
+  const int x = r.lo[0] + (t - q * nx), y = r.lo[1] + q % ny, z = r.lo[2] + q / ny;
+  const Box bx = *box;
+  auto inBox = [&](int i, int j, int k) {
+    return i >= bx.bMin[0] && i < bx.bMax[0] && j >= bx.bMin[1] && j < bx.bMax[1] &&
+           k >= bx.bMin[2] && k < bx.bMax[2];
+  };
+  if (!inBox(x, y, z)) return;
   const long n = x + (long)y * d.X + (long)z * d.slab;
   if (mass[n] != 0) return;
   auto at = [&](int i, int j, int k) {
-    return (i < bx.bMin[0] || i >= bx.bMax[0] || j < bx.bMin[1] || j >= bx.bMax[1] ||
-            k < bx.bMin[2] || k >= bx.bMax[2])
-               ? large
-               : in[i + (long)j * d.X + (long)k * d.slab];
+    return inBox(i, j, k) ? in[i + (long)j * d.X + (long)k * d.slab] : large;
   };
   const double p = fmin(at(x - 1, y, z), at(x + 1, y, z));
-  const double q = fmin(at(x, y - 1, z), at(x, y + 1, z));
-  const double r = fmin(at(x, y, z - 1), at(x, y, z + 1));
-  out[n] = solveDistance(p, q, r, in[n]);
+  const double q2 = fmin(at(x, y - 1, z), at(x, y + 1, z));
+  const double r2 = fmin(at(x, y, z - 1), at(x, y, z + 1));
+  out[n] = solveDistance(p, q2, r2, in[n]);
 }
 ESF_BOUNDS(kPhiRelax, 1024);
 
-esf_status solidPhi(esf_ctx* ctx, int solid) {
+// `launchBox`: the solid's boxes or boxes that contain them (the kernels
+// read the solid's own from device memory)
+esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox) {
   const Dims& d = ctx->d;
   DevSolid& so = ctx->solids[solid];
-  const Box& bx = so.box;
   const double large = (double)(d.X + d.Y + d.Z + 2);
-  launch<ESF_K(kPhiInit)>(ctx, dim3(cdiv(d.nodes, 256)), dim3(256), d, bx, (const double*)so.mass,
-                          large, so.phi, so.phi2);
+  const Box* dbox = ctx->dBoxes + solid;
+  launch<ESF_K(kPhiInit)>(ctx, dim3(cdiv(d.nodes, 256)), dim3(256), d, dbox,
+                          (const double*)so.mass, large, so.phi, so.phi2);
   if (ctx->p.flip) {
     // the FLIP round trip extends the velocity over the whole box by phi:
     // the reference's two rounds of ordered sweeps, bitwise
+    const Box& bx = so.box;
     const int box[6] = {bx.bMin[0], bx.bMin[1], bx.bMin[2], bx.bMax[0], bx.bMax[1], bx.bMax[2]};
     return sweepPhiRounds(ctx, box, so.mass, so.phi, 2);
   }
-  const long nb = (long)std::max(bx.bMax[0] - bx.bMin[0], 0) *
-                  std::max(bx.bMax[1] - bx.bMin[1], 0) * std::max(bx.bMax[2] - bx.bMin[2], 0);
+  BoxRange r;
+  for (int k = 0; k < 3; k++) {
+    r.lo[k] = launchBox.bMin[k];
+    r.hi[k] = std::max(launchBox.bMax[k], launchBox.bMin[k]);
+  }
+  const long nb = r.count();
   if (nb > 0) {
     static_assert(kPhiRelaxations % 2 == 0, "the relaxation ends in phi");
     for (int k = 0; k < kPhiRelaxations; k++) {
       const double* src = (k & 1) ? so.phi2 : so.phi;
       double* dst = (k & 1) ? so.phi : so.phi2;
-      launch<ESF_K(kPhiRelax)>(ctx, dim3(cdiv(nb, 256)), dim3(256), d, bx, (const double*)so.mass,
-                               src, dst, large);
+      launch<ESF_K(kPhiRelax)>(ctx, dim3(cdiv(nb, 256)), dim3(256), d, dbox, r,
+                               (const double*)so.mass, src, dst, large);
     }
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
@@ -1430,7 +1490,7 @@ ESF_BOUNDS(kSolidCopy, 1024);
 // the Dirichlet faces (COUPLED_INTEGRATOR_3D.cpp:900-916)
 struct RecoverArgs {
   int n;
-  Box box[kMaxSolids];
+  const Box* box;  // the solids' boxes in device memory
   const double* mass[kMaxSolids];
   const double* vel[kMaxSolids];
 };
@@ -1443,7 +1503,7 @@ __device__ __forceinline__ void kRecoverBC(const GridT& wg, Dims d, RecoverArgs 
 #pragma unroll
   for (int b = 0; b < kMaxSolids; b++) {
     if (b >= a.n) break;
-    const Box& bx = a.box[b];
+    const Box bx = a.box[b];
     if (x >= bx.bMin[0] && x < bx.bMax[0] - 1 && y >= bx.bMin[1] && y < bx.bMax[1] - 1 &&
         z >= bx.bMin[2] && z < bx.bMax[2] - 1 && a.mass[b][n] > 0)
       for (int c = 0; c < 3; c++) vel[3 * n + c] = a.vel[b][3 * n + c];
@@ -1617,8 +1677,8 @@ esf_status recoverAndBC(esf_ctx* ctx, int bc) {
   }
   RecoverArgs a{};
   a.n = (int)ctx->solids.size();
+  a.box = ctx->dBoxes;
   for (int b = 0; b < a.n; b++) {
-    a.box[b] = ctx->solids[b].box;
     a.mass[b] = ctx->solids[b].mass;
     a.vel[b] = ctx->solids[b].vel;
   }

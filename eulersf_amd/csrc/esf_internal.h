@@ -89,6 +89,11 @@ struct DevSolid {
   int slotSpan[6] = {0, 0, 0, 1 << 30, 1 << 30, 1 << 30};
   int slotScan[6] = {0, 0, 0, 0, 0, 0};  // the cells the queued list scan walks
   bool listPending = false;  // list scan queued, its count not read yet
+  // boxes that contain the ones the last box scan finds (its scan range
+  // widened as computeSDF widens), and whether the host copy `box` still
+  // waits for that scan's result (the device copy is current)
+  Box boxBound{};
+  bool boxPending = false;
   double* cellMat = nullptr; // per slot: R(9) L(9) F(9) Jinv(1)
   double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
@@ -148,6 +153,7 @@ struct esf_ctx {
   bool dofPending = false;  // DOF numbering queued, its count not read yet
   bool tightLists = false;  // lists grow to exactly their counts (ESF_TIGHT_LISTS)
   bool diagPending = false; // the pressure diagonal of the current masses not formed yet
+  esf::Box* dBoxes = nullptr;  // device copies of the solids' boxes (kMaxSolids)
   uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
   double* inbox = nullptr;      // solid b at 24 capSolid b: 24 doubles per node
   int inboxSolids = 0;          // solids the inbox was sized for
