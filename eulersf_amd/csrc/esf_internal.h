@@ -212,7 +212,7 @@ struct esf_ctx {
   int crQueued = 0;  // iterations the last pressure CR solve queued
   // pressure SpMV launch shape (fixed at creation): cell layers per block,
   // load sets in flight for the CR update and for the other modes
-  int spmvLayers = 16, spmvDepthUpdate = 1, spmvDepthOther = 2;
+  int spmvLayers = 16, spmvLayersWide = 8, spmvDepthUpdate = 1, spmvDepthOther = 2;
   // device allocations held by the context (admission control of served scenes)
   std::unordered_map<void*, size_t> allocs;
   size_t deviceBytes = 0;

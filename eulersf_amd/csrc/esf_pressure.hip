@@ -543,10 +543,15 @@ ESF_BOUNDS(kPressureDiag, 1024);
 // ---------------------------------------------------------------------------
 // The z-march's launch shape, fixed per context when it is created:
 //  - cell layers per block: each block loads two layers of x and forms one
-//    node layer beyond the ones it outputs, so deeper blocks waste less;
-//    16 measured best at 128^3 and 256^3 (MI355X).  Small grids take
-//    shallower blocks so that at least 512 blocks run where the grid has
-//    them.  ESF_SPMV_LAYERS (4..64) overrides the choice;
+//    node layer beyond the ones it outputs, so deeper blocks waste less,
+//    but fewer blocks hide less latency.  The FP64 CR update (the most
+//    registers) takes 16 layers, shallower only so that at least 512
+//    blocks run; the other launches (the FP32 kernels, the FP64 residual of
+//    the refinement, the applies and checks) take at least 1024 blocks.
+//    On MI355X at 128^3 that is 8 layers: FP32 update 20.5 vs 22.8 us and
+//    refinement residual 23.3 vs 27.9 us against 16, while the FP64 update
+//    measured 30.5 at 16 against 32.5 at 8.  ESF_SPMV_LAYERS (4..64)
+//    overrides both choices;
 //  - how many iterations ahead the loads run (1..3): two sets of loads in
 //    flight hide more latency but hold more registers, so the CR update
 //    (more operands) keeps one.  ESF_SPMV_DEPTH (1..3) overrides both.
@@ -560,27 +565,32 @@ static int envInt(const char* name, int lo, int hi) {
 void spmvConfigure(esf_ctx* ctx) {
   const Dims& d = ctx->d;
   const long tilesXY = (long)cdiv(d.CX, TX) * cdiv(d.CY, TY);
-  int layers = 16;
-  while (layers > 4 && tilesXY * cdiv(d.CZ, layers) < 512) layers /= 2;
+  auto layersFor = [&](long minBlocks) {
+    int layers = 16;
+    while (layers > 4 && tilesXY * cdiv(d.CZ, layers) < minBlocks) layers /= 2;
+    return layers;
+  };
   const int forcedLayers = envInt("ESF_SPMV_LAYERS", 4, 64);
-  ctx->spmvLayers = forcedLayers ? forcedLayers : layers;
+  ctx->spmvLayers = forcedLayers ? forcedLayers : layersFor(512);
+  ctx->spmvLayersWide = forcedLayers ? forcedLayers : layersFor(1024);
   const int forcedDepth = envInt("ESF_SPMV_DEPTH", 1, 3);
   ctx->spmvDepthUpdate = forcedDepth ? forcedDepth : 1;
   ctx->spmvDepthOther = forcedDepth ? forcedDepth : 2;
 }
 
-static void tileShape(const esf_ctx* ctx, int& tx, int& ty, int& tz, int& layers) {
+// `deep`: the FP64 CR update's shape
+static void tileShape(const esf_ctx* ctx, bool deep, int& tx, int& ty, int& tz, int& layers) {
   const Dims& d = ctx->d;
   tx = cdiv(d.CX, TX);
   ty = cdiv(d.CY, TY);
-  layers = ctx->spmvLayers;
+  layers = deep ? ctx->spmvLayers : ctx->spmvLayersWide;
   tz = cdiv(d.CZ, layers);
 }
 
 template <typename T, int MODE>
 esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   int tx, ty, tz, layers;
-  tileShape(ctx, tx, ty, tz, layers);
+  tileShape(ctx, sizeof(T) == 8 && MODE == M_UPDATE, tx, ty, tz, layers);
   const int nb = tx * ty * tz;
   const int depth = MODE == M_UPDATE ? ctx->spmvDepthUpdate : ctx->spmvDepthOther;
   if (depth == 3)
@@ -608,10 +618,15 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
   return ESF_OK;
 }
 
+// the most blocks a launch takes (the partials buffer's size)
 int spmvBlocks(const esf_ctx* ctx) {
-  int tx, ty, tz, layers;
-  tileShape(ctx, tx, ty, tz, layers);
-  return tx * ty * tz;
+  int nb = 0;
+  for (bool deep : {true, false}) {
+    int tx, ty, tz, layers;
+    tileShape(ctx, deep, tx, ty, tz, layers);
+    nb = std::max(nb, tx * ty * tz);
+  }
+  return nb;
 }
 
 esf_status pressureRhs(esf_ctx* ctx) {
