@@ -194,7 +194,19 @@ struct SolidPtrs {
   int n;
   const double* mass[kMaxSolids];
   const double* qfrac[kMaxSolids];
+  // per solid, the cells where its quarter fractions may be nonzero ([lo,
+  // hi) per axis; its masses reach one node beyond): elsewhere both are 0,
+  // and adding 0 leaves a sum unchanged, so the gather skips the load
+  int cells[kMaxSolids][6];
 };
+
+__device__ inline bool inCells(const int (&r)[6], int x, int y, int z) {
+  return x >= r[0] && x < r[3] && y >= r[1] && y < r[4] && z >= r[2] && z < r[5];
+}
+// a node is a corner of a cell of the range when each coordinate is in [lo, hi]
+__device__ inline bool atCellCorner(const int (&r)[6], int x, int y, int z) {
+  return x >= r[0] && x <= r[3] && y >= r[1] && y <= r[4] && z >= r[2] && z <= r[5];
+}
 
 // over the nodes of box g ([lo, hi) per axis); outside it both fields are 0
 template <class GridT>
@@ -205,12 +217,13 @@ __device__ __forceinline__ void kGatherMass(const GridT& wg, Dims d, SolidPtrs s
   const long n = g.node(d, t);
   // (loops over the solids unrolled: constant indices into the argument
   // arrays, so the arguments stay in registers)
+  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   double m = 0;
 #pragma unroll
   for (int b = 0; b < kMaxSolids; b++)
-    if (b < sp.n) m += sp.mass[b][n];
+    if (b < sp.n && atCellCorner(sp.cells[b], x, y, z))
+      m += sp.mass[b][n];
   solidMass[n] = m;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
   double f = 0;
   // corner i of the cell whose min corner is n - off[i], for cells with
   // min corner in [1, X-2]^3 (the reference's loop range, :350-356)
@@ -222,7 +235,7 @@ __device__ __forceinline__ void kGatherMass(const GridT& wg, Dims d, SolidPtrs s
     double q = 0;
 #pragma unroll
     for (int b = 0; b < kMaxSolids; b++)
-      if (b < sp.n) q += sp.qfrac[b][8 * c + i];
+      if (b < sp.n && inCells(sp.cells[b], cx, cy, cz)) q += sp.qfrac[b][8 * c + i];
     f += q;
   }
   nodeFrac[n] = f * 0.125;
@@ -376,8 +389,15 @@ esf_status gatherSolidMasses(esf_ctx* ctx, bool defer) {
     SolidPtrs sp{};
     sp.n = (int)ctx->solids.size();
     for (int b = 0; b < sp.n; b++) {
-      sp.mass[b] = ctx->solids[b].mass;
-      sp.qfrac[b] = ctx->solids[b].qfrac;
+      const DevSolid& so = ctx->solids[b];
+      sp.mass[b] = so.mass;
+      sp.qfrac[b] = so.qfrac;
+      // (FLIP scenes' particles set masses of their own: no skipping)
+      const bool all = so.dirtyAll || ctx->p.flip;
+      for (int k = 0; k < 3; k++) {
+        sp.cells[b][k] = all ? 0 : so.dirty[k];
+        sp.cells[b][3 + k] = all ? (1 << 30) : so.dirty[3 + k];
+      }
     }
     // the gathered fields are nonzero only at the nodes of the solids' last
     // mass passes (their dirty cells and one node beyond); recompute those
