@@ -576,7 +576,15 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
       }
       break;
     case ESF_S_VEL_WORKSPACE: *ptr = so.velws; *bytes = 24 * N; break;
-    case ESF_S_PHI: *ptr = so.phi; *bytes = 8 * N; break;
+    case ESF_S_PHI:
+      *ptr = so.phi;
+      *bytes = 8 * N;
+      if (forWrite)
+        for (int k = 0; k < 3; k++) {
+          so.phiSpan[k] = 0;
+          so.phiSpan[3 + k] = 1 << 30;
+        }
+      break;
     default:
       ctx->err = "bad field id";
       return ESF_ERR_INVALID;

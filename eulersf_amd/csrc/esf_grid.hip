@@ -871,15 +871,18 @@ esf_status updateBoxes(esf_ctx* ctx, int solid) {
 constexpr int kPhiRelaxations = 8;
 
 // phi = -0.5 on nodes with mass inside the box, large everywhere else (the
-// box read from device memory)
+// box read from device memory), over the nodes of r: outside r phi is
+// large already
 template <class GridT>
-__device__ __forceinline__ void kPhiInit(const GridT& wg, Dims d, const Box* box,
+__device__ __forceinline__ void kPhiInit(const GridT& wg, Dims d, const Box* box, BoxRange r,
                                          const double* __restrict__ mass, double large, double* phi,
                                          double* phi2) {
-  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
-  if (n >= d.nodes) return;
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
+  if (t >= r.count()) return;
   const Box bx = *box;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  r.at(t, x, y, z);
+  const long n = x + (long)y * d.X + (long)z * d.slab;
   const bool in = x >= bx.bMin[0] && x < bx.bMax[0] && y >= bx.bMin[1] && y < bx.bMax[1] &&
                   z >= bx.bMin[2] && z < bx.bMax[2];
   const double v = (in && mass[n] > 0) ? -0.5 : large;
@@ -926,8 +929,18 @@ esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox) {
   DevSolid& so = ctx->solids[solid];
   const double large = (double)(d.X + d.Y + d.Z + 2);
   const Box* dbox = ctx->dBoxes + solid;
-  launch<ESF_K(kPhiInit)>(ctx, dim3(cdiv(d.nodes, 256)), dim3(256), d, dbox,
-                          (const double*)so.mass, large, so.phi, so.phi2);
+  // phi differs from `large` only inside the extended box: (re)initialise
+  // that box and the one the last initialisation left
+  {
+    const int lim[3] = {d.X, d.Y, d.Z};
+    const int ext[6] = {launchBox.bMin[0], launchBox.bMin[1], launchBox.bMin[2],
+                        launchBox.bMax[0], launchBox.bMax[1], launchBox.bMax[2]};
+    const BoxRange r = spanBoxes(so.phiSpan, ext, lim);
+    if (r.count() > 0)
+      launch<ESF_K(kPhiInit)>(ctx, dim3(cdiv(r.count(), 256)), dim3(256), d, dbox, r,
+                              (const double*)so.mass, large, so.phi, so.phi2);
+    for (int k = 0; k < 6; k++) so.phiSpan[k] = ext[k];
+  }
   if (ctx->p.flip) {
     // the FLIP round trip extends the velocity over the whole box by phi:
     // the reference's two rounds of ordered sweeps, bitwise
@@ -1494,14 +1507,11 @@ ESF_BOUNDS(kSolidRecoverX, 256);
 // grid velocity into the solid's extended box (nodes outside keep their
 // old value: VEC3F_FIELD_3D::clear() is a no-op, VEC3F_FIELD_3D.h:177)
 template <class GridT>
-__device__ __forceinline__ void kSolidCopy(const GridT& wg, Dims d, Box bx, const double* vel,
+__device__ __forceinline__ void kSolidCopy(const GridT& wg, Dims d, BoxRange r, const double* vel,
                                            double* sv) {
-  const long n = wg.block() * (long)blockDim.x + threadIdx.x;
-  if (n >= d.nodes) return;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
-  if (x < bx.bMin[0] || x >= bx.bMax[0] || y < bx.bMin[1] || y >= bx.bMax[1] ||
-      z < bx.bMin[2] || z >= bx.bMax[2])
-    return;
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
+  if (t >= r.count()) return;
+  const long n = r.node(d, t);
   for (int c = 0; c < 3; c++) sv[3 * n + c] = vel[3 * n + c];
 }
 ESF_BOUNDS(kSolidCopy, 1024);
@@ -1666,7 +1676,18 @@ esf_status advectIntensity(esf_ctx* ctx) {
 }
 
 esf_status solidCopyVelocity(esf_ctx* ctx) {
-  for (auto& so : ctx->solids) LAUNCH_NODES(kSolidCopy, ctx->d, so.box, ctx->vel, so.vel);
+  // over the extended box's nodes (the solid's velocity elsewhere stays as
+  // it was, as the reference's box loop leaves it)
+  for (auto& so : ctx->solids) {
+    BoxRange r;
+    for (int k = 0; k < 3; k++) {
+      r.lo[k] = so.box.bMin[k];
+      r.hi[k] = std::max(so.box.bMax[k], so.box.bMin[k]);
+    }
+    if (r.count() > 0)
+      launch<ESF_K(kSolidCopy)>(ctx, dim3(cdiv(r.count(), 256)), dim3(256), ctx->d, r,
+                                (const double*)ctx->vel, so.vel);
+  }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 

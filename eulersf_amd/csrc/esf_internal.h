@@ -94,6 +94,9 @@ struct DevSolid {
   // waits for that scan's result (the device copy is current)
   Box boxBound{};
   bool boxPending = false;
+  // nodes where phi (and its relaxation buffer) may differ from `large`:
+  // the extended box of the last phi pass, the whole grid after an upload
+  int phiSpan[6] = {0, 0, 0, 1 << 30, 1 << 30, 1 << 30};
   double* cellMat = nullptr; // per slot: R(9) L(9) F(9) Jinv(1)
   double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
