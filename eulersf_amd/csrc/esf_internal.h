@@ -60,6 +60,14 @@ struct CrScalars {
   int pad[2];
 };
 
+// a solve's scalar block at its start: tolerance eps, nothing done yet
+__device__ inline void initScalarBlock(CrScalars* sc, double eps) {
+  CrScalars h{};
+  h.eps = eps;
+  h.iter = -1;
+  *sc = h;
+}
+
 // per-solid device state: DEFOR_SOLID_3D members kept on the GPU
 struct DevSolid {
   double lambda = 0, mu = 0, qcm = 0;  // qcm = _quarterCellSolidMass

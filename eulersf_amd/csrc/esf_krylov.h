@@ -108,10 +108,7 @@ inline void launchFinish(esf_ctx* ctx, int nb, CrScalars* sc, int iter = 0, int 
 template <class GridT>
 __device__ __forceinline__ void kScalarsInit(const GridT& wg, CrScalars* sc, double eps) {
   if (threadIdx.x != 0) return;
-  CrScalars h{};
-  h.eps = eps;
-  h.iter = -1;
-  *sc = h;
+  initScalarBlock(sc, eps);
 }
 ESF_BOUNDS(kScalarsInit, 1024);
 

@@ -610,7 +610,10 @@ ESF_BOUNDS(kSolidNode, kNodeThreads);
 // to the iterate that check accepted
 template <class GridT>
 __device__ __forceinline__ void kSolidFixX(const GridT& wg, long n, const CrScalars* sc,
-                                           const double* xb, double* x) {
+                                           const double* xb, double* x, CrScalars* report) {
+  // the solve's report into its pinned slot (read after a later
+  // synchronisation)
+  if (wg.block() == 0 && threadIdx.x == 0) *report = *sc;
   if (!(sc->done && sc->aux != 0.0)) return;
   for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)wg.blocks() * blockDim.x)
@@ -773,8 +776,10 @@ esf_status solidApplyDevice(esf_ctx* ctx, const double* x, double* y) {
 template <class GridT>
 __device__ __forceinline__ void kSolidSetup(const GridT& wg, long n, const double* b,
                                             const double* diag, double* x, double* r, double* p,
-                                            double* Ap, double* apd, double* partials) {
+                                            double* Ap, double* apd, double* partials,
+                                            CrScalars* sc, double eps) {
   __shared__ double red[16];
+  if (wg.block() == 0 && threadIdx.x == 0) initScalarBlock(sc, eps);
   double part = 0;
   for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)wg.blocks() * blockDim.x) {
@@ -793,8 +798,10 @@ ESF_BOUNDS(kSolidSetup, 1024);
 // a warm start: x at each solid DOF from a node field (3 doubles per node)
 template <class GridT>
 __device__ __forceinline__ void kWarmGather(const GridT& wg, int nSolid, const int* solidNodes,
-                                            const double* guess, double* x) {
+                                            const double* guess, double* x, CrScalars* sc,
+                                            double eps) {
   const int s = wg.block() * blockDim.x + threadIdx.x;
+  if (s == 0) initScalarBlock(sc, eps);
   if (s >= nSolid) return;
   const long n = solidNodes[s];
   for (int c = 0; c < 3; c++) x[3L * s + c] = guess[3 * n + c];
@@ -845,15 +852,15 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
     return ESF_ERR_INVALID;
   }
   CrScalars* sc = &ctx->sc[which];
-  initScalarsAsync(ctx, sc, eps);
   const int eb = std::max(1, std::min(cdiv(n, 1024), 256));
   // a check is waited for once two iterations are queued behind it
   DonePoller poll(ctx, which, false, 2);
   if (guess) {
     // warm start x0 from the node field: r = D^-1 (b - A x0), and done at
     // once if x0 already meets the tolerance
+    // (the scalar block is initialised by the first thread)
     launch<ESF_K(kWarmGather)>(ctx, dim3(cdiv(ctx->nSolid, 256)), dim3(256), ctx->nSolid,
-                               (const int*)ctx->solidNodes, guess, x);
+                               (const int*)ctx->solidNodes, guess, x, sc, eps);
     esf_status st = solidApplyDevice(ctx, x, ctx->stmp);
     if (st) return st;
     launch<ESF_K(kSolidSetupWarm)>(ctx, dim3(eb), dim3(256), n, b, (const double*)ctx->sAdiag,
@@ -863,7 +870,7 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
     poll.mark(-1);
   } else {
     launch<ESF_K(kSolidSetup)>(ctx, dim3(eb), dim3(256), n, b, (const double*)ctx->sAdiag, x,
-                               ctx->sr, ctx->sp, ctx->sAp, ctx->sApd, ctx->partials);
+                               ctx->sr, ctx->sp, ctx->sAp, ctx->sApd, ctx->partials, sc, eps);
     launchFinish<F_BNORM, 1>(ctx, eb, sc);
   }
   // partial sets of 5 x nb: node pass i writes set i mod 3 (the setup pass
@@ -918,11 +925,11 @@ esf_status solidSolve(esf_ctx* ctx, int which, const double* b, double* x, doubl
     }
     if (poll.drain(i)) break;
   }
+  // the accepted x, and the report written without waiting (the caller
+  // reads report slot `slot` after its next synchronisation)
   launch<ESF_K(kSolidFixX)>(ctx, dim3(std::max(1, std::min(cdiv(n, 256), 1024))), dim3(256), n,
-                            (const CrScalars*)sc, (const double*)ctx->sXb, x);
-  // the report is written without waiting; the caller reads report slot
-  // `slot` after its next synchronisation
-  copyScalarsAsync(ctx, sc, &ctx->hostSc[slot]);
+                            (const CrScalars*)sc, (const double*)ctx->sXb, x,
+                            &ctx->hostSc[slot]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("solid solve: ") + hipGetErrorString(e);
