@@ -11,17 +11,18 @@
 
 namespace esf {
 // esf_pressure.hip
-esf_status pressureRhs(esf_ctx* ctx);
-esf_status pressureRhsDeferDiag(esf_ctx* ctx);
+esf_status pressureRhs(esf_ctx* ctx, CrScalars* sc = nullptr, double eps = 0);
+esf_status pressureRhsDeferDiag(esf_ctx* ctx, CrScalars* sc, double eps);
 esf_status pressureDiagFlush(esf_ctx* ctx);
 esf_status pressureApplyDevice(esf_ctx* ctx, const double* x, double* y);
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
                    const T* Dinv, T* x, T* r,
                    T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2, double eps, int maxIt, int slot,
-                   bool recursive = false);
+                   bool recursive = false, bool scalarsReady = false);
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
-                      double* relres, int* refinements, int* innerIters);
+                      double* relres, int* refinements, int* innerIters,
+                      bool scalarsReady = false);
 int spmvBlocks(const esf_ctx* ctx);
 void spmvConfigure(esf_ctx* ctx);
 esf_status benchPressureApply(esf_ctx* ctx, int fp32, int reps, double* avgMs);
@@ -841,8 +842,12 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
   batchBarrier(ctx);
   {
     StageTimer t(ctx, 3);
-    // the mixed form forms the diagonal only where its inner solve runs
-    if ((st = ctx->p.precision == ESF_MIXED ? pressureRhsDeferDiag(ctx) : pressureRhs(ctx)))
+    // the mixed form forms the diagonal only where its inner solve runs;
+    // both forms' right-hand side kernels initialise the solve's scalars
+    CrScalars* sc0 = &ctx->sc[0];
+    const double eps = ctx->p.divfree_eps;
+    if ((st = ctx->p.precision == ESF_MIXED ? pressureRhsDeferDiag(ctx, sc0, eps)
+                                            : pressureRhs(ctx, sc0, eps)))
       return st;
     if (!ctx->lambdaInit) {
       if ((st = zeroD(ctx, ctx->d.cells, ctx->lambda))) return st;
@@ -850,12 +855,12 @@ static esf_status coupledSolveEnqueue(esf_ctx* ctx, esf_solve_stats* s) {
     }
     if (ctx->p.precision == ESF_MIXED) {
       bool conv = false;
-      st = mixedSolve(ctx, ctx->p.divfree_eps, maxIt, &s->iterations[1], &conv,
-                      &s->rel_residual[1], &s->refinements, &s->inner_iterations);
+      st = mixedSolve(ctx, eps, maxIt, &s->iterations[1], &conv, &s->rel_residual[1],
+                      &s->refinements, &s->inner_iterations, true);
       s->converged[1] = conv;
     } else {
       st = crSolve<double>(ctx, 0, ctx->minv, ctx->b, ctx->D, ctx->Dinv, ctx->lambda, ctx->r, ctx->r2,
-                           ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, ctx->p.divfree_eps, maxIt, 1);
+                           ctx->pdir, ctx->s, ctx->s2, ctx->Ap, ctx->Ap2, eps, maxIt, 1, false, true);
     }
     if (st) return st;
     if ((st = correctVelocity(ctx))) return st;

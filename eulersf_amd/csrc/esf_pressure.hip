@@ -466,8 +466,10 @@ template <class GridT>
 __device__ __forceinline__ void kPressureRhs(const GridT& wg, Dims d, const double* __restrict__ v0,
                                              const double* __restrict__ vel,
                                              const double* __restrict__ minv, double* c, double* b,
-                                             double* D, double* Dinv, float* D32, float* Dinv32) {
+                                             double* D, double* Dinv, float* D32, float* Dinv32,
+                                             CrScalars* sc, double eps) {
   const int r = wg.block() * blockDim.x + threadIdx.x;
+  if (sc && r == 0) initScalarBlock(sc, eps);
   if (r >= d.cells) return;
   const int q = r / d.CX, cx = r - q * d.CX, cy = q % d.CY, cz = q / d.CY;
   const long base = cx + (long)cy * d.X + (long)cz * d.slab;
@@ -609,7 +611,8 @@ esf_status launchSpmv(esf_ctx* ctx, const SpmvArgs<T>& a) {
     launchFinish<F_CR, 5>(ctx, nb, a.sc, a.iter, a.recursive, nullptr, a.hostOut);
   if constexpr (MODE == M_CHECK)
     launchFinish<F_CHECK, 1>(ctx, nb, a.sc, a.iter, 0, nullptr, a.hostOut);
-  if constexpr (MODE == M_IR_RESID) launchFinish<F_IR, 2>(ctx, nb, a.sc, 0, a.first);
+  if constexpr (MODE == M_IR_RESID)
+    launchFinish<F_IR, 2>(ctx, nb, a.sc, 0, a.first, nullptr, a.hostOut);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("kSpmv launch: ") + hipGetErrorString(e);
@@ -629,12 +632,14 @@ int spmvBlocks(const esf_ctx* ctx) {
   return nb;
 }
 
-esf_status pressureRhs(esf_ctx* ctx) {
+// sc (may be null): a scalar block initialised with tolerance eps for the
+// solve that follows
+esf_status pressureRhs(esf_ctx* ctx, CrScalars* sc, double eps) {
   const Dims& d = ctx->d;
   const bool mixed = ctx->p.precision == ESF_MIXED;
   launch<ESF_K(kPressureRhs)>(ctx, dim3(cdiv(d.cells, 256)), dim3(256), d, ctx->v0, ctx->vel,
                               ctx->minv, ctx->c, ctx->b, ctx->D, ctx->Dinv,
-                              mixed ? ctx->D32 : nullptr, ctx->Dinv32);
+                              mixed ? ctx->D32 : nullptr, ctx->Dinv32, sc, eps);
   ctx->diagPending = false;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -642,11 +647,13 @@ esf_status pressureRhs(esf_ctx* ctx) {
 // the mixed step's right-hand side: b and c only.  The diagonal and minv in
 // FP32 follow (pressureDiagFlush) only where an inner solve runs, or where
 // a caller reads the diagonal.
-esf_status pressureRhsDeferDiag(esf_ctx* ctx) {
+// It also initialises scalar block `sc` with tolerance eps for the solve
+// that follows (null: none).
+esf_status pressureRhsDeferDiag(esf_ctx* ctx, CrScalars* sc, double eps) {
   const Dims& d = ctx->d;
   launch<ESF_K(kPressureRhs)>(ctx, dim3(cdiv(d.cells, 256)), dim3(256), d, ctx->v0, ctx->vel,
                               ctx->minv, ctx->c, ctx->b, (double*)nullptr, (double*)nullptr,
-                              (float*)nullptr, (float*)nullptr);
+                              (float*)nullptr, (float*)nullptr, sc, eps);
   ctx->diagPending = true;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -752,9 +759,9 @@ static esf_status initScalars(esf_ctx* ctx, int which, double eps) {
   return ESF_OK;
 }
 
-// synchronous read of scalar block `which` (through report slot 3)
-static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
-  copyScalarsAsync(ctx, &ctx->sc[which], &ctx->hostSc[3]);
+// the scalar block a finisher queued into report slot 3, after a
+// synchronisation
+static esf_status waitSlot3(esf_ctx* ctx, CrScalars* out) {
   ESF_HIP(streamSync(ctx));
   *out = ctx->hostSc[3];
   return ESF_OK;
@@ -774,8 +781,9 @@ static esf_status readScalars(esf_ctx* ctx, int which, CrScalars* out) {
 template <typename T>
 esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* D,
                    const T* Dinv, T* x, T* r, T* r2, T* p, T* s, T* s2, T* Ap, T* Ap2,
-                   double eps, int maxIt, int slot, bool recursive) {
-  esf_status st = initScalars(ctx, which, eps);
+                   double eps, int maxIt, int slot, bool recursive, bool scalarsReady) {
+  // scalarsReady: the kernel before the solve initialised scalar block `which`
+  esf_status st = scalarsReady ? ESF_OK : initScalars(ctx, which, eps);
   if (st) return st;
   const Dims& d = ctx->d;
   SpmvArgs<T> a{};
@@ -850,13 +858,16 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
 template esf_status crSolve<double>(esf_ctx*, int, const double*, const double*,
                                     const double*, const double*, double*, double*, double*,
                                     double*, double*, double*, double*, double*, double, int,
-                                    int, bool);
+                                    int, bool, bool);
 template esf_status crSolve<float>(esf_ctx*, int, const float*, const float*, const float*,
                                    const float*, float*, float*, float*, float*, float*,
-                                   float*, float*, float*, double, int, int, bool);
+                                   float*, float*, float*, double, int, int, bool, bool);
 
+// the inner solve's correction zeroed, and its scalar block initialised
 template <class GridT>
-__device__ __forceinline__ void kZeroF(const GridT& wg, long n, float* a) {
+__device__ __forceinline__ void kZeroF(const GridT& wg, long n, float* a, CrScalars* sc,
+                                       double eps) {
+  if (wg.block() == 0 && threadIdx.x == 0) initScalarBlock(sc, eps);
   for (long i = wg.block() * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)wg.blocks() * blockDim.x)
     a[i] = 0.0f;
@@ -867,9 +878,10 @@ ESF_BOUNDS(kZeroF, 1024);
 // FP64 inner products) inside FP64 iterative refinement.  The outer
 // residual b - S x, the correction x += e and |r| stay in FP64.
 esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* converged,
-                      double* relres, int* refinements, int* innerIters) {
+                      double* relres, int* refinements, int* innerIters, bool scalarsReady) {
   const Dims& d = ctx->d;
-  esf_status st = initScalars(ctx, 0, eps);
+  // scalarsReady: the right-hand side's kernel initialised scalar block 0
+  esf_status st = scalarsReady ? ESF_OK : initScalars(ctx, 0, eps);
   if (st) return st;
   SpmvArgs<double> o{};
   o.minv = ctx->minv;
@@ -881,9 +893,10 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
   o.sc = &ctx->sc[0];
   o.partials = ctx->partials;
   o.first = 1;
+  o.hostOut = &ctx->hostSc[3];
   if ((st = launchSpmv<double, M_IR_RESID>(ctx, o))) return st;
   CrScalars h;
-  if ((st = readScalars(ctx, 0, &h))) return st;
+  if ((st = waitSlot3(ctx, &h))) return st;
   int outer = 0, inner = 0;
   // refinement rounds: each gains roughly the FP32 solve's attainable
   // reduction (about eps32 * cond on the smooth modes), so an ill-conditioned
@@ -911,11 +924,11 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
                                 ctx->minv32);
       }
     }
-    launch<ESF_K(kZeroF)>(ctx, dim3(1024), dim3(256), d.cells, ctx->e32);
+    launch<ESF_K(kZeroF)>(ctx, dim3(1024), dim3(256), d.cells, ctx->e32, &ctx->sc[1], innerEps);
     if ((st = crSolve<float>(ctx, 1, ctx->minv32, ctx->rhs32, ctx->D32, ctx->Dinv32, ctx->e32,
                              ctx->r32,
                              ctx->r32b, ctx->p32, ctx->s32, ctx->s32b, ctx->Ap32, ctx->Ap32b,
-                             innerEps, maxIt - inner, 4, true)))
+                             innerEps, maxIt - inner, 4, true, true)))
       return st;
     outer++;
     o.e32 = ctx->e32;
@@ -924,7 +937,7 @@ esf_status mixedSolve(esf_ctx* ctx, double eps, int maxIt, int* iters, bool* con
     o.x = ctx->lambda2;
     if ((st = launchSpmv<double, M_IR_RESID>(ctx, o))) return st;
     std::swap(ctx->lambda, ctx->lambda2);
-    if ((st = readScalars(ctx, 0, &h))) return st;  // also completes report slot 4
+    if ((st = waitSlot3(ctx, &h))) return st;  // also completes report slot 4
     int it = 0;
     bool conv = false;
     double rr = 0;
