@@ -390,6 +390,12 @@ extern "C" esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha,
         return ESF_ERR_INVALID;
       }
   }
+  // a batch of one is esf_step itself (nothing to merge its launches with)
+  if (n == 1) {
+    const esf_status r = esf_step(ctxs[0], alpha, stats);
+    if (status) status[0] = r;
+    return r;
+  }
   // the argument records of one launch must fit one staging chunk
   if ((size_t)n * (kMaxPack + 8) + 256 > kStageChunk) {
     ctxs[0]->err = "too many scenes in one batch";

@@ -325,7 +325,8 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
  * the work already queued on every context's stream and before any queued
  * on them later.  Contexts must be distinct, on the current device and in
  * no other batch; several threads may each run a batch of their own.
- * Returns the most severe status. */
+ * A batch of one runs esf_step on its context.  Returns the most severe
+ * status. */
 esf_status esf_step_batch(esf_ctx** ctxs, int n, double alpha, esf_solve_stats* stats,
                           esf_status* status);
 
