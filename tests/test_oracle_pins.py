@@ -766,3 +766,32 @@ def test_recover_and_dirichlet_faces():
     iw[:, 0, :] = 0
     np.testing.assert_array_equal(s.field(O.INTENSITY).reshape(X, X, X), iw)
     s.close()
+
+
+def test_cfl_minimum_taken_as_alpha_over_the_largest_speed():
+    """CFL (COUPLED_INTEGRATOR_3D.cpp:88-102) takes dt = min(1e9, min over
+    solid nodes of alpha / vmax); the device reduces the largest vmax and the
+    host divides once.  Division and its rounding are monotone, so the two
+    agree bit for bit: checked on random speeds over many magnitudes, ties,
+    zero speeds (alpha / 0 = inf, so dt = 1e9) and NaN speeds, which both
+    forms skip (std::min keeps dt; fmax keeps the maximum)."""
+    rng = np.random.default_rng(5)
+    for trial in range(300):
+        n = int(rng.integers(1, 400))
+        v = np.exp(rng.uniform(-30, 30, n)) * rng.choice([1.0, 1.0 + 2 ** -52, 3.0], n)
+        if trial % 7 == 0:
+            v[rng.integers(0, n)] = 0.0
+        if trial % 11 == 0:
+            v[rng.integers(0, n)] = np.nan
+        if trial % 13 == 0:
+            v[:] = 0.0
+        alpha = [0.6, 1.8, float(rng.uniform(0.1, 3))][trial % 3]
+        dt_ref = 1e9
+        for x in v:
+            q = alpha / x if x != 0 else np.inf
+            dt_ref = q if q < dt_ref else dt_ref  # std::min(dt, q), NaN never smaller
+        vm = 0.0
+        for x in v:
+            vm = np.fmax(vm, x)
+        dt_max = min(1e9, alpha / vm if vm != 0 else np.inf)
+        assert dt_max == dt_ref, (trial, dt_max, dt_ref)
