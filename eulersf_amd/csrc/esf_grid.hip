@@ -1482,23 +1482,88 @@ __device__ inline double recoverX(const Dims& d, long i, double u) {
   return pos + (o - u);
 }
 
+// The face extrapolation of the extended box (:1205-1220: z faces, then
+// y, then x, each face value 2 a1 - a2 of the two nodes inward as the
+// passes before left them), evaluated per node from the field before it.
+// For a box at least kFaceFuseMin nodes wide on every axis the inward
+// nodes a face reads are faces of no axis (lo + 1, lo + 2, hi - 2, hi - 3),
+// so the chain reads only nodes the extrapolation leaves alone, and a
+// kernel may write each face node in place while others read.
+struct FaceBox {
+  int on;  // 0: no extrapolation here (done by kSolidExtrap)
+  int lo[3], hi[3];
+};
+constexpr int kFaceFuseMin = 5;
+
+__device__ inline bool onFace(const FaceBox& b, int x, int y, int z) {
+  const bool in = x >= b.lo[0] && x < b.hi[0] && y >= b.lo[1] && y < b.hi[1] && z >= b.lo[2] &&
+                  z < b.hi[2];
+  return b.on && in &&
+         (x == b.lo[0] || x == b.hi[0] - 1 || y == b.lo[1] || y == b.hi[1] - 1 ||
+          z == b.lo[2] || z == b.hi[2] - 1);
+}
+
+__device__ inline double extrapolated(const Dims& d, const double* u, const FaceBox& b, int x,
+                                      int y, int z, int c) {
+  auto at = [&](int i, int j, int k) { return u[3 * (i + (long)j * d.X + (long)k * d.slab) + c]; };
+  auto uz = [&](int i, int j, int k) {
+    if (k == b.lo[2]) return 2 * at(i, j, k + 1) - at(i, j, k + 2);
+    if (k == b.hi[2] - 1) return 2 * at(i, j, k - 1) - at(i, j, k - 2);
+    return at(i, j, k);
+  };
+  auto uy = [&](int i, int j, int k) {
+    if (j == b.lo[1]) return 2 * uz(i, j + 1, k) - uz(i, j + 2, k);
+    if (j == b.hi[1] - 1) return 2 * uz(i, j - 1, k) - uz(i, j - 2, k);
+    return uz(i, j, k);
+  };
+  if (x == b.lo[0]) return 2 * uy(x + 1, y, z) - uy(x + 2, y, z);
+  if (x == b.hi[0] - 1) return 2 * uy(x - 1, y, z) - uy(x - 2, y, z);
+  return uy(x, y, z);
+}
+
+// X = x - u (:1222-1230) and _vecWorkspace = _velocity (:1232), with the
+// face extrapolation of f folded in when f.on (face nodes' u written here)
 template <class GridT>
-__device__ __forceinline__ void kSolidRecoverX(const GridT& wg, Dims d, const double* __restrict__ u,
+__device__ __forceinline__ void kSolidRecoverX(const GridT& wg, Dims d, double* u,
                                                const double* __restrict__ sv,
-                                               double* __restrict__ X, double* __restrict__ svws) {
+                                               double* __restrict__ X, double* __restrict__ svws,
+                                               FaceBox f) {
   const long n3 = 3 * d.nodes;
   const long i = 2 * (wg.block() * (long)blockDim.x + threadIdx.x);
-  if (i + 1 < n3) {
-    const double2 uu = *reinterpret_cast<const double2*>(u + i);
-    const double2 vv = *reinterpret_cast<const double2*>(sv + i);
-    double2 xx;
-    xx.x = recoverX(d, i, uu.x);
-    xx.y = recoverX(d, i + 1, uu.y);
-    *reinterpret_cast<double2*>(X + i) = xx;
-    *reinterpret_cast<double2*>(svws + i) = vv;
-  } else if (i < n3) {
-    X[i] = recoverX(d, i, u[i]);
-    svws[i] = sv[i];
+  if (i >= n3) return;
+  // the pair's elements (the last element may be alone)
+  const bool pair = i + 1 < n3;
+  double uu[2], vv[2];
+  if (pair) {
+    const double2 a = *reinterpret_cast<const double2*>(u + i);
+    const double2 b = *reinterpret_cast<const double2*>(sv + i);
+    uu[0] = a.x;
+    uu[1] = a.y;
+    vv[0] = b.x;
+    vv[1] = b.y;
+  } else {
+    uu[0] = u[i];
+    vv[0] = sv[i];
+  }
+  const int m = pair ? 2 : 1;
+  double xx[2];
+  for (int e = 0; e < m; e++) {
+    const long k = i + e;
+    const long n = k / 3;
+    const int c = (int)(k - 3 * n);
+    const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+    if (onFace(f, x, y, z)) {
+      uu[e] = extrapolated(d, u, f, x, y, z, c);
+      u[k] = uu[e];
+    }
+    xx[e] = recoverX(d, k, uu[e]);
+  }
+  if (pair) {
+    *reinterpret_cast<double2*>(X + i) = make_double2(xx[0], xx[1]);
+    *reinterpret_cast<double2*>(svws + i) = make_double2(vv[0], vv[1]);
+  } else {
+    X[i] = xx[0];
+    svws[i] = vv[0];
   }
 }
 ESF_BOUNDS(kSolidRecoverX, 256);
@@ -1698,15 +1763,27 @@ esf_status solidFinalize(esf_ctx* ctx, double dt) {
                                  (const double*)so.vel, dt, so.unew, tX, tY);
     std::swap(so.u, so.unew);
     const Box& b = so.box;
-    const int axes[3] = {2, 1, 0};
-    for (int a : axes) {
-      int fa = a == 0 ? 1 : 0, fb = a == 2 ? 1 : 2;
-      const long cnt = (long)(b.bMax[fa] - b.bMin[fa]) * (b.bMax[fb] - b.bMin[fb]);
-      if (cnt > 0 && b.bMax[a] - b.bMin[a] >= 3)
-        launch<ESF_K(kSolidExtrap)>(ctx, dim3(cdiv(cnt, 256)), dim3(256), ctx->d, b, a, so.u);
+    // the face extrapolation rides with X = x - u where the box is wide
+    // enough (every axis at least kFaceFuseMin nodes), else it runs first,
+    // axis by axis
+    FaceBox f{};
+    f.on = 1;
+    for (int k = 0; k < 3; k++) {
+      f.lo[k] = b.bMin[k];
+      f.hi[k] = b.bMax[k];
+      f.on = f.on && b.bMax[k] - b.bMin[k] >= kFaceFuseMin;
+    }
+    if (!f.on) {
+      const int axes[3] = {2, 1, 0};
+      for (int a : axes) {
+        int fa = a == 0 ? 1 : 0, fb = a == 2 ? 1 : 2;
+        const long cnt = (long)(b.bMax[fa] - b.bMin[fa]) * (b.bMax[fb] - b.bMin[fb]);
+        if (cnt > 0 && b.bMax[a] - b.bMin[a] >= 3)
+          launch<ESF_K(kSolidExtrap)>(ctx, dim3(cdiv(cnt, 256)), dim3(256), ctx->d, b, a, so.u);
+      }
     }
     launch<ESF_K(kSolidRecoverX)>(ctx, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)), dim3(256),
-                                  ctx->d, so.u, so.vel, so.X, so.velws);
+                                  ctx->d, so.u, (const double*)so.vel, so.X, so.velws, f);
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
