@@ -548,9 +548,29 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
   }
   DevSolid& so = ctx->solids[solid];
   switch (field) {
-    case ESF_S_U: *ptr = so.u; *bytes = 24 * N; break;
-    case ESF_S_X: *ptr = so.X; *bytes = 24 * N; break;
-    case ESF_S_VELOCITY: *ptr = so.vel; *bytes = 24 * N; break;
+    case ESF_S_U:
+      // _X keeps x - u of the u being replaced
+      if (forWrite && so.xLazy) {
+        esf_status st = materialiseX(ctx, so);
+        if (st) return st;
+      }
+      *ptr = so.u;
+      *bytes = 24 * N;
+      break;
+    case ESF_S_X:
+      if (so.xLazy && !forWrite) {
+        esf_status st = materialiseX(ctx, so);
+        if (st) return st;
+      }
+      so.xLazy = false;  // an upload replaces the whole field
+      *ptr = so.X;
+      *bytes = 24 * N;
+      break;
+    case ESF_S_VELOCITY:
+      *ptr = so.vel;
+      *bytes = 24 * N;
+      if (forWrite) so.wsState = 2;
+      break;
     case ESF_S_MASS:
       *ptr = so.mass;
       *bytes = 8 * N;
@@ -576,7 +596,11 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
         ctx->massGen++;
       }
       break;
-    case ESF_S_VEL_WORKSPACE: *ptr = so.velws; *bytes = 24 * N; break;
+    case ESF_S_VEL_WORKSPACE:
+      *ptr = so.velws;
+      *bytes = 24 * N;
+      if (forWrite) so.wsState = 2;
+      break;
     case ESF_S_PHI:
       *ptr = so.phi;
       *bytes = 8 * N;
@@ -678,6 +702,7 @@ esf_status esf_extend_solid_velocity(esf_ctx* ctx, int solid) {
   DevSolid& so = ctx->solids[solid];
   const int box[6] = {so.box.bMin[0], so.box.bMin[1], so.box.bMin[2],
                       so.box.bMax[0], so.box.bMax[1], so.box.bMax[2]};
+  so.wsState = 2;
   esf_status st = extendDevice(ctx, box, so.mass, so.phi, so.vel);
   if (st) return st;
   ESF_HIP(streamSync(ctx));

@@ -461,6 +461,7 @@ esf_status particleToGrid(esf_ctx* ctx, int solid) {
   // next mass pass clears and rebuilds all of it
   so.listDirty = true;
   so.dirtyAll = true;
+  so.wsState = 2;  // the solid's velocity changed beyond the copied box
   ctx->massGen++;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }

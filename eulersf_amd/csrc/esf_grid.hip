@@ -138,6 +138,13 @@ struct BoxRange {
   }
   __device__ void at(long t, int& x, int& y, int& z) const {
     const int nx = hi[0] - lo[0], ny = hi[1] - lo[1];
+    if (t <= 0x7fffffffL) {  // 32-bit division where the index fits
+      const int ti = (int)t, q = ti / nx;
+      x = lo[0] + (ti - q * nx);
+      y = lo[1] + q % ny;
+      z = lo[2] + q / ny;
+      return;
+    }
     x = lo[0] + (int)(t % nx);
     y = lo[1] + (int)((t / nx) % ny);
     z = lo[2] + (int)(t / ((long)nx * ny));
@@ -661,6 +668,16 @@ esf_status computeSolidMass(esf_ctx* ctx, int solid, bool usesdf) {
   const int none[6] = {0, 0, 0, 0, 0, 0};
   const BoxRange T = spanBoxes(tc, none, climit);
   const BoxRange U = spanBoxes(so.dirtyAll ? fullCells : so.dirty, tc, climit);
+  // the samples read _X at the corners of T's cells
+  if (so.xLazy && T.count() > 0) {
+    bool inside = true;
+    for (int k = 0; k < 3; k++)
+      inside = inside && so.xBox[k] <= T.lo[k] && T.hi[k] + 1 <= so.xBox[3 + k];
+    if (!inside) {
+      esf_status st = materialiseX(ctx, so);
+      if (st) return st;
+    }
+  }
   if (U.count() > 0)
     launch<ESF_K(kMassClear)>(ctx, dim3(cdiv(U.count(), 256)), dim3(256), d, U, T,
                               (const double*)so.u, so.qfrac, so.valid, so.detFInv);
@@ -1473,13 +1490,16 @@ ESF_BOUNDS(kSolidExtrap, 1024);
 // 3 N doubles of the AoS fields as flat pairs (16-byte accesses, every
 // lane of a load on consecutive bytes); element i is component i mod 3 of
 // node i / 3
+__device__ inline double recoverXAt(const Dims& d, int x, int y, int z, int c, double u) {
+  const double pos = (c == 0 ? x : (c == 1 ? y : z)) * d.dh;
+  const double o = c == 0 ? d.o0 : (c == 1 ? d.o1 : d.o2);
+  return pos + (o - u);
+}
 __device__ inline double recoverX(const Dims& d, long i, double u) {
   const long n = i / 3;
   const int c = (int)(i - 3 * n);
   const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
-  const double pos = (c == 0 ? x : (c == 1 ? y : z)) * d.dh;
-  const double o = c == 0 ? d.o0 : (c == 1 ? d.o1 : d.o2);
-  return pos + (o - u);
+  return recoverXAt(d, x, y, z, c, u);
 }
 
 // The face extrapolation of the extended box (:1205-1220: z faces, then
@@ -1533,17 +1553,19 @@ __device__ __forceinline__ void kSolidRecoverX(const GridT& wg, Dims d, double* 
   if (i >= n3) return;
   // the pair's elements (the last element may be alone)
   const bool pair = i + 1 < n3;
-  double uu[2], vv[2];
+  double uu[2], vv[2] = {0, 0};
   if (pair) {
     const double2 a = *reinterpret_cast<const double2*>(u + i);
-    const double2 b = *reinterpret_cast<const double2*>(sv + i);
     uu[0] = a.x;
     uu[1] = a.y;
-    vv[0] = b.x;
-    vv[1] = b.y;
+    if (svws) {
+      const double2 b = *reinterpret_cast<const double2*>(sv + i);
+      vv[0] = b.x;
+      vv[1] = b.y;
+    }
   } else {
     uu[0] = u[i];
-    vv[0] = sv[i];
+    if (svws) vv[0] = sv[i];
   }
   const int m = pair ? 2 : 1;
   double xx[2];
@@ -1560,13 +1582,45 @@ __device__ __forceinline__ void kSolidRecoverX(const GridT& wg, Dims d, double* 
   }
   if (pair) {
     *reinterpret_cast<double2*>(X + i) = make_double2(xx[0], xx[1]);
-    *reinterpret_cast<double2*>(svws + i) = make_double2(vv[0], vv[1]);
+    if (svws) *reinterpret_cast<double2*>(svws + i) = make_double2(vv[0], vv[1]);
   } else {
     X[i] = xx[0];
-    svws[i] = vv[0];
+    if (svws) svws[i] = vv[0];
   }
 }
 ESF_BOUNDS(kSolidRecoverX, 256);
+
+// The step's form of the end of finalizeSolution that stays inside the
+// extended box (node range r): the face extrapolation of u at the face
+// nodes of f (f.on: every box dimension at least kFaceFuseMin, so each face
+// value reads only nodes off every face, which no thread writes), _X = x - u
+// inside r (the mass pass reads no other node), and, when svws is given,
+// _vecWorkspace = _velocity inside r, the only nodes where the two can
+// differ.  A thread per node and component (a face value reads only its
+// own component).
+template <class GridT>
+__device__ __forceinline__ void kSolidBoxFinish(const GridT& wg, Dims d, BoxRange r, FaceBox f,
+                                                double* u, const double* __restrict__ sv,
+                                                double* __restrict__ svws,
+                                                double* __restrict__ X) {
+  const long t = wg.block() * (long)blockDim.x + threadIdx.x;
+  if (t >= 3 * r.count()) return;
+  const long m = t / 3;
+  const int c = (int)(t - 3 * m);
+  int x, y, z;
+  r.at(m, x, y, z);
+  const long k = 3 * (x + (long)y * d.X + (long)z * d.slab) + c;
+  if (svws) svws[k] = sv[k];
+  double v;
+  if (f.on && onFace(f, x, y, z)) {
+    v = extrapolated(d, u, f, x, y, z, c);
+    u[k] = v;
+  } else {
+    v = u[k];
+  }
+  X[k] = recoverXAt(d, x, y, z, c, v);
+}
+ESF_BOUNDS(kSolidBoxFinish, 256);
 
 // DEFOR_SOLID_3D::finalizeSolution(velocity, dt)  (:1182-1197): copy the
 // grid velocity into the solid's extended box (nodes outside keep their
@@ -1749,9 +1803,18 @@ esf_status solidCopyVelocity(esf_ctx* ctx) {
       r.lo[k] = so.box.bMin[k];
       r.hi[k] = std::max(so.box.bMax[k], so.box.bMin[k]);
     }
-    if (r.count() > 0)
+    if (r.count() > 0) {
       launch<ESF_K(kSolidCopy)>(ctx, dim3(cdiv(r.count(), 256)), dim3(256), ctx->d, r,
                                 (const double*)ctx->vel, so.vel);
+      // velws now differs from vel at most inside r
+      const int rb[6] = {r.lo[0], r.lo[1], r.lo[2], r.hi[0], r.hi[1], r.hi[2]};
+      if (so.wsState == 0) {
+        so.wsState = 1;
+        std::copy(rb, rb + 6, so.wsBox);
+      } else if (so.wsState == 1 && !std::equal(rb, rb + 6, so.wsBox)) {
+        so.wsState = 2;
+      }
+    }
   }
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
@@ -1782,9 +1845,53 @@ esf_status solidFinalize(esf_ctx* ctx, double dt) {
           launch<ESF_K(kSolidExtrap)>(ctx, dim3(cdiv(cnt, 256)), dim3(256), ctx->d, b, a, so.u);
       }
     }
-    launch<ESF_K(kSolidRecoverX)>(ctx, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)), dim3(256),
-                                  ctx->d, so.u, (const double*)so.vel, so.X, so.velws, f);
+    // Outside FLIP scenes, with a wide box and velws known to differ from
+    // vel at most inside the box the velocity copy wrote, the work stays
+    // inside the boxes: the faces, _X and velws there (_X elsewhere waits
+    // for materialiseX).  Otherwise one pass over every node: the faces
+    // (f.on), _X and velws.
+    if (f.on && !ctx->p.flip && so.wsState != 2) {
+      BoxRange fr;
+      for (int k = 0; k < 3; k++) {
+        fr.lo[k] = f.lo[k];
+        fr.hi[k] = f.hi[k];
+      }
+      const int fb[6] = {f.lo[0], f.lo[1], f.lo[2], f.hi[0], f.hi[1], f.hi[2]};
+      const bool wsHere = so.wsState == 1 && std::equal(fb, fb + 6, so.wsBox);
+      launch<ESF_K(kSolidBoxFinish)>(ctx, dim3(cdiv(3 * fr.count(), 256)), dim3(256), ctx->d,
+                                     fr, f, so.u, wsHere ? (const double*)so.vel : nullptr,
+                                     wsHere ? so.velws : nullptr, so.X);
+      if (so.wsState == 1 && !wsHere) {
+        BoxRange wr;
+        for (int k = 0; k < 3; k++) {
+          wr.lo[k] = so.wsBox[k];
+          wr.hi[k] = so.wsBox[3 + k];
+        }
+        if (wr.count() > 0)
+          launch<ESF_K(kSolidCopy)>(ctx, dim3(cdiv(wr.count(), 256)), dim3(256), ctx->d, wr,
+                                    (const double*)so.vel, so.velws);
+      }
+      so.xLazy = true;
+      std::copy(fb, fb + 6, so.xBox);
+    } else {
+      launch<ESF_K(kSolidRecoverX)>(ctx, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)),
+                                    dim3(256), ctx->d, so.u, (const double*)so.vel, so.X,
+                                    so.velws, f);
+      so.xLazy = false;
+    }
+    so.wsState = 0;
   }
+  return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+// _X = x - u at every node, into the buffer X (after a step left it to the
+// mass pass)
+esf_status materialiseX(esf_ctx* ctx, DevSolid& so) {
+  const FaceBox off{};
+  launch<ESF_K(kSolidRecoverX)>(ctx, dim3(cdiv((3 * ctx->d.nodes + 1) / 2, 256)), dim3(256),
+                                ctx->d, so.u, (const double*)nullptr, so.X, (double*)nullptr,
+                                off);
+  so.xLazy = false;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
 }
 

@@ -76,6 +76,17 @@ struct DevSolid {
   double* rest = nullptr;              // rest mass FIELD_3Df
   double *u = nullptr, *unew = nullptr, *X = nullptr;
   double *vel = nullptr, *velws = nullptr;  // 3*nodes
+  // xLazy: the buffer X holds _X = x - u only inside the node range xBox
+  // ([lo, hi) per axis, where the step wrote it); materialiseX writes the
+  // rest before X is handed to a caller, u is overwritten, or a mass pass
+  // reads outside xBox
+  bool xLazy = false;
+  int xBox[6] = {0, 0, 0, 0, 0, 0};
+  // velws against vel: 0 equal at every node, 1 equal outside the node
+  // range wsBox ([lo, hi) per axis: the copy that changed vel since), 2 not
+  // known (an upload or a FLIP transfer changed either)
+  int wsState = 0;
+  int wsBox[6] = {0, 0, 0, 0, 0, 0};
   double *mass = nullptr, *detFInv = nullptr;
   // _phi: the solid's distance field in cells (computeSDF); phi2 is the
   // second buffer of its relaxation
@@ -466,6 +477,9 @@ esf_status sweepPhiRounds(esf_ctx* ctx, const int box[6], const double* mass, do
                           int rounds);
 esf_status extendDevice(esf_ctx* ctx, const int box[6], const double* mass, const double* phi,
                         double* field);
+
+// esf_grid.hip: _X = x - u at every node into the solid's X buffer
+esf_status materialiseX(esf_ctx* ctx, DevSolid& so);
 
 // esf_flip.hip: the FLIP round trip of one solid
 esf_status initParticles(esf_ctx* ctx, int solid);

@@ -158,6 +158,79 @@ def test_steps_match_oracle(esf):
         assert rel_err(i_g, i_o) < 1e-10, k
 
 
+def test_material_positions_and_velocity_workspace_follow_the_reference(esf):
+    """free-running steps write _X and _vecWorkspace only inside the
+    extended box (the rest of _X when a caller reads it): downloaded after
+    each step both equal the oracle's, and an upload of u leaves _X at x - u
+    of the u it replaced, as in the reference"""
+    A = esf._abi
+    g, o = make_pair(esf, 16, velocity=(0.3, -1.0, 0.2), **TIGHT)
+    fields = ((A.S_X, O.S_X), (A.S_VEL_WORKSPACE, O.S_VEL_WORKSPACE),
+              (A.S_VELOCITY, O.S_VELOCITY), (A.S_MASS, O.S_MASS), (A.S_U, O.S_U))
+
+    def same(k):
+        for gf, of in fields:
+            assert rel_err(g.download(gf, 0), o.field(of, 0)) < 1e-10, (k, gf)
+
+    for k in range(2):
+        g.stepImplicit(1.8)
+        o.step(1.8)
+        same(k)
+    x_before = g.download(A.S_X, 0)
+    u2 = o.field(O.S_U, 0) * 1.5
+    g.upload(A.S_U, u2, 0)
+    o.set_field(O.S_U, u2, 0)
+    np.testing.assert_array_equal(g.download(A.S_X, 0), x_before)
+    same("upload")
+    for k in range(2, 4):
+        g.stepImplicit(1.8)
+        o.step(1.8)
+        same(k)
+    g.close()
+    o.close()
+
+
+def test_material_positions_outside_the_box_written_before_a_wider_mass_pass(esf):
+    """A step writes _X = x - u inside the extended box only.  A block of
+    nodes outside it is displaced so that x - u lands inside the ball's rest
+    shape; a mass pass over a tight box widened by the caller to that block
+    then finds mass there, equal bit for bit to a context whose _X was
+    downloaded (written out at every node) first."""
+    A = esf._abi
+    n = 80
+    N = n + 1
+    dh = 1.0 / n
+    ctxs = [esf.CoupledIntegrator3D.from_cfg_text(ball_cfg(n, velocity=(0.3, -1.0, 0.2)))
+            for _ in range(2)]
+    ext = ctxs[0].boxes(0)[6:]
+    assert ext[0] >= 6, ext  # the block x < 6 lies outside the extended box
+    pos = np.stack(np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")[::-1],
+                   axis=-1) * dh
+    u = ctxs[0].download(A.S_U, 0).reshape(N, N, N, 3).copy()
+    X0 = ctxs[0].download(A.S_X, 0).reshape(N, N, N, 3)
+    # the block takes the material positions of the nodes 37 to its right,
+    # around the ball's middle
+    z, y, x = np.meshgrid(np.arange(36, 45), np.arange(46, 55), np.arange(1, 6), indexing="ij")
+    u[z, y, x] = pos[z, y, x] - X0[z, y, x + 37]
+    for g in ctxs:
+        g.upload(A.S_U, u.reshape(-1), 0)
+        st = g.stepImplicit(1.8)
+        assert st["status"] == 0
+    ctxs[1].download(A.S_X, 0)
+    wide = [1, 30, 30, 60, 60, 60, 0, 0, 0, N, N, N]
+    for g in ctxs:
+        g.set_boxes(0, wide)
+        g.computeSolidMass(0)
+    m0, m1 = (g.download(A.S_MASS, 0).reshape(N, N, N) for g in ctxs)
+    np.testing.assert_array_equal(m0, m1)
+    assert m0[36:45, 46:55, 1:6].sum() > 0
+    uu = ctxs[0].download(A.S_U, 0).reshape(N, N, N, 3)
+    np.testing.assert_array_equal(ctxs[0].download(A.S_X, 0).reshape(N, N, N, 3),
+                                  pos + (0.0 - uu))
+    for g in ctxs:
+        g.close()
+
+
 def test_mixed_form_steps_within_1e6_of_fp64_oracle(esf):
     """the mixed form (FP32 CR inside FP64 refinement) solving to 1e-8
     against the FP64 oracle solving to 1e-13: after three steps every
