@@ -47,7 +47,8 @@ __device__ __forceinline__ void kInitParticles(const GridT& wg, Dims d, int nVal
   const long k = wg.block() * (long)blockDim.x + threadIdx.x;
   if (k >= (long)nValid * ppc) return;
   const long node = cellList[k / ppc];
-  const int x = node % d.X, y = (node / d.X) % d.Y, z = node / d.slab;
+  int x, y, z;
+  nodeXYZ(d, node, x, y, z);
   const double grid[3] = {d.o0 + x * d.dh, d.o1 + y * d.dh, d.o2 + z * d.dh};
   double p[3];
   for (int c = 0; c < 3; c++) p[c] = grid[c] + rnd[3 * k + c] * d.dh;
@@ -291,7 +292,8 @@ template <class GridT>
 __device__ __forceinline__ void kP2G(const GridT& wg, Dims d, P2GArgs a) {
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
   const double o[3] = {d.o0, d.o1, d.o2};
   double vs[3] = {0, 0, 0}, ws = 0;
   bool touched = false;

@@ -224,7 +224,8 @@ __device__ __forceinline__ void kGatherMass(const GridT& wg, Dims d, SolidPtrs s
   const long n = g.node(d, t);
   // (loops over the solids unrolled: constant indices into the argument
   // arrays, so the arguments stay in registers)
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
   double m = 0;
 #pragma unroll
   for (int b = 0; b < kMaxSolids; b++)
@@ -1066,7 +1067,8 @@ __device__ __forceinline__ void kVorticity(const GridT& wg, Dims d, const double
                                            int enabled) {
   const long index = wg.block() * (long)blockDim.x + threadIdx.x;
   if (index >= d.nodes) return;
-  const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
+  int x, y, z;
+  nodeXYZ(d, index, x, y, z);
   double out = 0.0;
   if (enabled && isInterior(d, x, y, z) && !(nf[index] >= 1)) {
     const long X = d.X, S = d.slab;
@@ -1143,7 +1145,8 @@ __device__ __forceinline__ void kBodyForces(const GridT& wg, Dims d, BodyArgs a)
   const long index = wg.block() * (long)blockDim.x + threadIdx.x;
   double en = 0;
   if (index < d.nodes) {
-    const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
+    int x, y, z;
+    nodeXYZ(d, index, x, y, z);
     if (isInterior(d, x, y, z)) {
       const double* nf = a.nf;
       const double dh = d.dh;
@@ -1211,7 +1214,8 @@ __device__ __forceinline__ void kSolidBodyForces(const GridT& wg, Dims d, BodyAr
   const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long index = solidNodes[s];
-  const int x = index % d.X, y = (index / d.X) % d.Y, z = index / d.slab;
+  int x, y, z;
+  nodeXYZ(d, index, x, y, z);
   if (!isInterior(d, x, y, z)) return;
   const int solidIdx = a.sidx[index];
   if (solidIdx >= 0) solidBodyForces(d, a, index, solidIdx);
@@ -1250,7 +1254,8 @@ __device__ __forceinline__ void kCorrect(const GridT& wg, Dims d, const double* 
                                          double* dvPrev) {
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
-  const int i = n % d.X, j = (n / d.X) % d.Y, k = n / d.slab;
+  int i, j, k;
+  nodeXYZ(d, n, i, j, k);
   if (!isInterior(d, i, j, k)) return;
   double a0 = 0, a1 = 0, a2 = 0;
   for (int dz = 0; dz < 2; dz++)
@@ -1306,7 +1311,7 @@ ESF_BOUNDS(kSolidVelocity, 1024);
 // kernels stay one thread per node: tiled the same way they measured
 // slower, 137 vs 86 us and 66 vs 32 us at 128^3, their departure points
 // leaving the window more often.)
-constexpr int kSLX = 32, kSLY = 8, kSLZ = 4;
+constexpr int kSLX = 32, kSLY = 8, kSLZ = 8;
 constexpr int kSLHX = kSLX + 2, kSLHY = kSLY + 2;
 constexpr int kSLLayer = kSLHX * kSLHY;  // halo nodes per layer
 static_assert(kSLX * kSLY == 256, "one node per thread and layer");
@@ -1326,7 +1331,8 @@ __device__ __forceinline__ void kAdvectVelocity(const GridT& wg, Dims d,
                                                 double* velws) {
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
   if (!isInterior(d, x, y, z) || solidMass[n] > 0) return;
   const FieldGeom g = gridGeom(d);
   const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
@@ -1358,7 +1364,8 @@ __device__ __forceinline__ void kAdvectScalar(const GridT& wg, Dims d, const dou
     out[n] = 0.0;
     return;
   }
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
   const FieldGeom g = gridGeom(d);
   const double node[3] = {x * d.dh + d.o0, y * d.dh + d.o1, z * d.dh + d.o2};
   const double v[3] = {vel[3 * n], vel[3 * n + 1], vel[3 * n + 2]};
@@ -1418,8 +1425,8 @@ __device__ __forceinline__ void kSolidAdvectU(const GridT& wg, Dims d, const dou
       double np[3];
       for (int c = 0; c < 3; c++) np[c] = node[c] - sv[3 * n + c] * dt;
       double fx, fy, fz;
-      const long b = locate(g, np, fx, fy, fz);
-      const int bi = (int)(b % X), bj = (int)((b / X) % d.Y), bk = (int)(b / S);
+      int bi, bj, bk;
+      const long b = locate(g, np, fx, fy, fz, bi, bj, bk);
       const int hx = bi - (x0 - 1), hy = bj - (y0 - 1);
       const bool inWin = hx >= 0 && hx + 1 < kSLHX && hy >= 0 && hy + 1 < kSLHY &&
                          bk >= z - 1 && bk <= z;
@@ -1496,9 +1503,10 @@ __device__ inline double recoverXAt(const Dims& d, int x, int y, int z, int c, d
   return pos + (o - u);
 }
 __device__ inline double recoverX(const Dims& d, long i, double u) {
-  const long n = i / 3;
-  const int c = (int)(i - 3 * n);
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  const int n = (int)i / 3;
+  const int c = (int)i - 3 * n;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
   return recoverXAt(d, x, y, z, c, u);
 }
 
@@ -1571,14 +1579,15 @@ __device__ __forceinline__ void kSolidRecoverX(const GridT& wg, Dims d, double* 
   double xx[2];
   for (int e = 0; e < m; e++) {
     const long k = i + e;
-    const long n = k / 3;
-    const int c = (int)(k - 3 * n);
-    const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+    const int n = (int)k / 3;
+    const int c = (int)k - 3 * n;
+    int x, y, z;
+    nodeXYZ(d, n, x, y, z);
     if (onFace(f, x, y, z)) {
       uu[e] = extrapolated(d, u, f, x, y, z, c);
       u[k] = uu[e];
     }
-    xx[e] = recoverX(d, k, uu[e]);
+    xx[e] = recoverXAt(d, x, y, z, c, uu[e]);
   }
   if (pair) {
     *reinterpret_cast<double2*>(X + i) = make_double2(xx[0], xx[1]);
@@ -1648,7 +1657,8 @@ __device__ __forceinline__ void kRecoverBC(const GridT& wg, Dims d, RecoverArgs 
                                            double* intensity, int bc) {
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
 #pragma unroll
   for (int b = 0; b < kMaxSolids; b++) {
     if (b >= a.n) break;

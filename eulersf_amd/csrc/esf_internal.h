@@ -60,6 +60,15 @@ struct CrScalars {
   int pad[2];
 };
 
+// node index -> (x, y, z).  A context holds at most 2^28 nodes (checked when
+// it is created), so 32-bit division suffices.
+__device__ inline void nodeXYZ(const Dims& d, long n, int& x, int& y, int& z) {
+  const int ni = (int)n, q = ni / d.X;
+  x = ni - q * d.X;
+  y = q % d.Y;
+  z = q / d.Y;
+}
+
 // a solve's scalar block at its start: tolerance eps, nothing done yet
 __device__ inline void initScalarBlock(CrScalars* sc, double eps) {
   CrScalars h{};

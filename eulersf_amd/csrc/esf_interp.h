@@ -30,13 +30,18 @@ __device__ __forceinline__ void bary(double x, double dh, int hi, int& i, double
 }
 __device__ __forceinline__ double lerp(double a, double b, double f) { return (1 - f) * a + f * b; }
 
+// the cell (i, j, k) holding p and p's offsets in it; returns its node index
 __device__ __forceinline__ long locate(const FieldGeom& g, const double* p, double& fx, double& fy,
-                                       double& fz) {
-  int i, j, k;
+                                       double& fz, int& i, int& j, int& k) {
   bary(p[0] - g.o0, g.dh, g.X - 2, i, fx);
   bary(p[1] - g.o1, g.dh, g.Y - 2, j, fy);
   bary(p[2] - g.o2, g.dh, g.Z - 2, k, fz);
   return i + (long)j * g.X + (long)k * g.X * g.Y;
+}
+__device__ __forceinline__ long locate(const FieldGeom& g, const double* p, double& fx, double& fy,
+                                       double& fz) {
+  int i, j, k;
+  return locate(g, p, fx, fy, fz, i, j, k);
 }
 
 __device__ __forceinline__ double interpS(const double* d, const FieldGeom& g, const double* p) {

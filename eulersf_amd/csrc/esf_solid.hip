@@ -155,7 +155,8 @@ __device__ __forceinline__ void kSolidGather(const GridT& wg, Dims d, int nSolid
   const int s = wg.block() * blockDim.x + threadIdx.x;
   if (s >= nSolid) return;
   const long n = solidNodes[s];
-  const int x = n % d.X, y = (n / d.X) % d.Y, z = n / d.slab;
+  int x, y, z;
+  nodeXYZ(d, n, x, y, z);
   double f[3] = {0, 0, 0}, kd[3] = {0, 0, 0};
   const bool rowKeep = smass[n] != 0;
   int nk = 0;
