@@ -808,6 +808,12 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   // the FP32 inner solve of the mixed form stops when its residual
   // stagnates; a check is waited for once two iterations are queued behind it
   DonePoller poll(ctx, which, sizeof(T) == 4, 2, 4);
+  // the recursive solve's finishers test convergence on the device at every
+  // iteration (later launches exit at once); the host reads every third of
+  // them, since each read leaves the stream idle for some microseconds
+  // (measured at 128^3: mixed steps/s 893 reading every one, 914 every
+  // third, 909 every fourth, 902 every sixth)
+  constexpr int kPollEvery = 3;
   T *sA = s, *sB = s2, *apA = Ap, *apB = Ap2;
   a.recursive = recursive ? 1 : 0;
   // iteration `it`: the fused update and S r, then the buffers swap
@@ -828,9 +834,10 @@ esf_status crSolve(esf_ctx* ctx, int which, const T* minv, const T* b, const T* 
   };
   int i = 0;
   for (; i < maxIt; i++) {
-    if ((st = update(i, recursive ? poll.slot() : nullptr))) return st;
+    const bool read = recursive && i % kPollEvery == kPollEvery - 1;
+    if ((st = update(i, read ? poll.slot() : nullptr))) return st;
     if (recursive) {
-      poll.mark(i);
+      if (read) poll.mark(i);
     } else if (i > 0 && i % 5 == 0) {
       a.in = x;
       a.hostOut = poll.slot();
