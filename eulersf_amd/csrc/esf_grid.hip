@@ -1251,7 +1251,7 @@ __device__ __forceinline__ void kCorrect(const GridT& wg, Dims d, const double* 
                                          const double* __restrict__ v0,
                                          const double* __restrict__ minv,
                                          const int* __restrict__ sidx, double* vel, double* sB,
-                                         double* dvPrev) {
+                                         double* dvPrev, const Box* boxes, int nBoxes) {
   const long n = wg.block() * (long)blockDim.x + threadIdx.x;
   if (n >= d.nodes) return;
   int i, j, k;
@@ -1276,7 +1276,17 @@ __device__ __forceinline__ void kCorrect(const GridT& wg, Dims d, const double* 
     vel[3 * n] = v0[3 * n] + a0 * mi;
     vel[3 * n + 1] = v0[3 * n + 1] + a1 * mi;
     vel[3 * n + 2] = v0[3 * n + 2] + a2 * mi;
-    if (dvPrev) {
+    // dvPrev is read (the next step's second solid solve starts from it) at
+    // the next step's solid nodes, which the mass pass of this step finds
+    // inside this step's boxes: written there only (everywhere: nBoxes < 0)
+    bool keep = dvPrev && nBoxes < 0;
+    for (int b = 0; dvPrev && b < nBoxes && !keep; b++) {
+      const Box bx = boxes[b];
+      keep = i >= min(bx.tMin[0], bx.bMin[0]) && i < max(bx.tMax[0], bx.bMax[0]) &&
+             j >= min(bx.tMin[1], bx.bMin[1]) && j < max(bx.tMax[1], bx.bMax[1]) &&
+             k >= min(bx.tMin[2], bx.bMin[2]) && k < max(bx.tMax[2], bx.bMax[2]);
+    }
+    if (keep) {
       dvPrev[3 * n] = a0 * mi;
       dvPrev[3 * n + 1] = a1 * mi;
       dvPrev[3 * n + 2] = a2 * mi;
@@ -1779,8 +1789,10 @@ esf_status scatterSolidV0(esf_ctx* ctx) {
 }
 
 esf_status correctVelocity(esf_ctx* ctx) {
+  // FLIP scenes' particles set masses anywhere: dvPrev kept at every node
   LAUNCH_NODES(kCorrect, ctx->d, ctx->lambda, ctx->v0, ctx->minv, ctx->sidx, ctx->vel, ctx->sB,
-               ctx->dvPrev);
+               ctx->dvPrev, (const Box*)ctx->dBoxes,
+               ctx->p.flip ? -1 : (int)ctx->solids.size());
   return ESF_OK;
 }
 
