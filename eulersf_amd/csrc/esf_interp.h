@@ -12,10 +12,21 @@ namespace esf {
 struct FieldGeom {
   int X, Y, Z;
   double dh, o0, o1, o2;
+  // 1 / dh, and whether dh is a power of two: then x * inv is x / dh to the
+  // bit (the reciprocal is exact, and both round the same real quotient),
+  // one multiplication in place of a division
+  double inv;
+  int pow2;
 };
+__host__ __device__ inline FieldGeom fieldGeom(int X, int Y, int Z, double dh, double o0,
+                                               double o1, double o2) {
+  int e = 0;
+  const int pow2 = frexp(dh, &e) == 0.5 ? 1 : 0;
+  return FieldGeom{X, Y, Z, dh, o0, o1, o2, 1.0 / dh, pow2};
+}
 
-__device__ __forceinline__ void bary(double x, double dh, int hi, int& i, double& f) {
-  x /= dh;
+__device__ __forceinline__ void bary(double x, const FieldGeom& g, int hi, int& i, double& f) {
+  x = g.pow2 ? x * g.inv : x / g.dh;
   const double s = floor(x);
   i = (int)s;
   if (i < 0) {
@@ -33,9 +44,9 @@ __device__ __forceinline__ double lerp(double a, double b, double f) { return (1
 // the cell (i, j, k) holding p and p's offsets in it; returns its node index
 __device__ __forceinline__ long locate(const FieldGeom& g, const double* p, double& fx, double& fy,
                                        double& fz, int& i, int& j, int& k) {
-  bary(p[0] - g.o0, g.dh, g.X - 2, i, fx);
-  bary(p[1] - g.o1, g.dh, g.Y - 2, j, fy);
-  bary(p[2] - g.o2, g.dh, g.Z - 2, k, fz);
+  bary(p[0] - g.o0, g, g.X - 2, i, fx);
+  bary(p[1] - g.o1, g, g.Y - 2, j, fy);
+  bary(p[2] - g.o2, g, g.Z - 2, k, fz);
   return i + (long)j * g.X + (long)k * g.X * g.Y;
 }
 __device__ __forceinline__ long locate(const FieldGeom& g, const double* p, double& fx, double& fy,
@@ -87,7 +98,7 @@ __device__ __forceinline__ void interpVCell(const double (&cv)[3][8], const Fiel
 }
 
 __device__ __forceinline__ FieldGeom gridGeom(const Dims& d) {
-  return FieldGeom{d.X, d.Y, d.Z, d.dh, d.o0, d.o1, d.o2};
+  return fieldGeom(d.X, d.Y, d.Z, d.dh, d.o0, d.o1, d.o2);
 }
 
 }  // namespace esf
