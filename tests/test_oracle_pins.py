@@ -795,3 +795,24 @@ def test_cfl_minimum_taken_as_alpha_over_the_largest_speed():
             vm = np.fmax(vm, x)
         dt_max = min(1e9, alpha / vm if vm != 0 else np.inf)
         assert dt_max == dt_ref, (trial, dt_max, dt_ref)
+
+
+def test_scaling_by_a_power_of_two_reciprocal_equals_the_division():
+    """The trilinear lookups (FIELD_3D.h:368-382 divide the offset by dh)
+    multiply by 1 / dh where dh is a power of two.  The reciprocal is then
+    exact and both operations round the same real quotient, so they agree
+    bit for bit, including subnormal, huge, signed-zero and non-finite
+    offsets; for a dh that is not a power of two (1/80, 1/24) they do not,
+    which is why the other grids keep the division."""
+    rng = np.random.default_rng(9)
+    x = np.concatenate([
+        rng.uniform(-2, 2, 20000),
+        np.exp(rng.uniform(-700, 700, 20000)) * rng.choice([-1.0, 1.0], 20000),
+        np.array([0.0, -0.0, 5e-324, -5e-324, 1.7e308, np.inf, -np.inf, np.nan]),
+    ])
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        for k in (1, 4, 7, 10, 16, 30):
+            dh = 2.0 ** -k
+            np.testing.assert_array_equal((x / dh).view(np.int64), (x * (1.0 / dh)).view(np.int64))
+        for dh in (1.0 / 80, 1.0 / 24):
+            assert np.any((x / dh) != (x * (1.0 / dh)))
