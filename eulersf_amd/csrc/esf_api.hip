@@ -43,7 +43,7 @@ esf_status updateBoxes(esf_ctx* ctx, int solid);
 esf_status updateBoxesEnqueue(esf_ctx* ctx, int solid);
 void updateBoxesFinish(esf_ctx* ctx, int solid, bool markList);
 esf_status uploadBox(esf_ctx* ctx, int solid);
-esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox);
+esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox, bool tightOnly);
 esf_status cfl(esf_ctx* ctx, double alpha, double* dtOut);
 esf_status vorticityAndBodyForces(esf_ctx* ctx, double* energy);
 esf_status forkBodyForces(esf_ctx* ctx);
@@ -202,6 +202,10 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   // the path that grows a list after its scan and scans again
   const char* tight = getenv("ESF_TIGHT_LISTS");
   ctx->tightLists = tight && tight[0] == '1';
+  // ESF_PHI_AS_LEFT=1 (tests): a download of phi returns what the step left
+  // (relaxed only near the tight box) instead of the whole-box relaxation
+  const char* asLeft = getenv("ESF_PHI_AS_LEFT");
+  ctx->phiAsLeft = asLeft && asLeft[0] == '1';
   Dims& d = ctx->d;
   d.X = g->xres;
   d.Y = g->yres;
@@ -468,6 +472,8 @@ esf_status esf_solid_set_boxes(esf_ctx* ctx, int solid, const int b[12]) {
       return ESF_ERR_INVALID;
     }
   }
+  esf_status st = phiWhole(ctx, solid);  // relaxed on the boxes it was computed for
+  if (st) return st;
   for (int k = 0; k < 3; k++) {
     bx.tMin[k] = b[k];
     bx.tMax[k] = b[3 + k];
@@ -575,6 +581,8 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
       *ptr = so.mass;
       *bytes = 8 * N;
       if (forWrite) {
+        esf_status st = phiWhole(ctx, solid);  // from the mass being replaced
+        if (st) return st;
         so.listDirty = true;
         so.dirtyAll = true;
         ctx->massGen++;
@@ -602,6 +610,16 @@ static esf_status fieldPtr(esf_ctx* ctx, int field, int solid, void** ptr, size_
       if (forWrite) so.wsState = 2;
       break;
     case ESF_S_PHI:
+      if (!forWrite && ctx->phiAsLeft) {
+        *ptr = so.phi;
+        *bytes = 8 * N;
+        return ESF_OK;
+      }
+      if (!forWrite) {
+        esf_status st = phiWhole(ctx, solid);
+        if (st) return st;
+      }
+      so.phiPartial = false;  // an upload replaces the whole field
       *ptr = so.phi;
       *bytes = 8 * N;
       if (forWrite)
@@ -653,7 +671,8 @@ esf_status esf_compute_solid_mass(esf_ctx* ctx, int solid) {
     ctx->err = "mass sample dh must be positive and at most dh";
     return ESF_ERR_INVALID;
   }
-  esf_status st = computeSolidMass(ctx, solid, false);
+  esf_status st = phiWhole(ctx, solid);  // from the mass it was computed for
+  if (!st) st = computeSolidMass(ctx, solid, false);
   if (st) return st;
   ESF_HIP(streamSync(ctx));
   return ESF_OK;
@@ -665,7 +684,8 @@ esf_status esf_compute_solid_mass_sdf(esf_ctx* ctx, int solid) {
     ctx->err = "mass sample dh must be positive and at most dh";
     return ESF_ERR_INVALID;
   }
-  esf_status st = computeSolidMass(ctx, solid, true);
+  esf_status st = phiWhole(ctx, solid);
+  if (!st) st = computeSolidMass(ctx, solid, true);
   if (st) return st;
   ESF_HIP(streamSync(ctx));
   return ESF_OK;
@@ -673,7 +693,8 @@ esf_status esf_compute_solid_mass_sdf(esf_ctx* ctx, int solid) {
 
 esf_status esf_update_boxes(esf_ctx* ctx, int solid) {
   if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
-  return updateBoxes(ctx, solid);
+  esf_status st = phiWhole(ctx, solid);
+  return st ? st : updateBoxes(ctx, solid);
 }
 
 static esf_status flipCall(esf_ctx* ctx, int solid, esf_status (*fn)(esf_ctx*, int)) {
@@ -703,7 +724,8 @@ esf_status esf_extend_solid_velocity(esf_ctx* ctx, int solid) {
   const int box[6] = {so.box.bMin[0], so.box.bMin[1], so.box.bMin[2],
                       so.box.bMax[0], so.box.bMax[1], so.box.bMax[2]};
   so.wsState = 2;
-  esf_status st = extendDevice(ctx, box, so.mass, so.phi, so.vel);
+  esf_status st = phiWhole(ctx, solid);
+  if (!st) st = extendDevice(ctx, box, so.mass, so.phi, so.vel);
   if (st) return st;
   ESF_HIP(streamSync(ctx));
   return ESF_OK;
@@ -793,7 +815,7 @@ esf_status esf_set_particles(esf_ctx* ctx, int solid, long n, const double* pos,
 esf_status esf_solid_sdf(esf_ctx* ctx, int solid) {
   if (!ctx || solid < 0 || solid >= (int)ctx->solids.size()) return ESF_ERR_INVALID;
   esf_status st = updateBoxes(ctx, solid);
-  if (!st) st = solidPhi(ctx, solid, ctx->solids[solid].box);
+  if (!st) st = solidPhi(ctx, solid, ctx->solids[solid].box, false);
   if (st) return st;
   ESF_HIP(streamSync(ctx));
   return ESF_OK;
@@ -1076,7 +1098,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
       ESF_HIP(streamSync(ctx));
       for (int b = 0; b < (int)ctx->solids.size(); b++) {
         updateBoxesFinish(ctx, b, true);
-        if ((st = solidPhi(ctx, b, ctx->solids[b].box))) return st;
+        if ((st = solidPhi(ctx, b, ctx->solids[b].box, false))) return st;
       }
     } else {
       // the stages up to the step's end read the boxes from device memory
@@ -1086,7 +1108,7 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
         DevSolid& so = ctx->solids[b];
         so.boxPending = true;
         so.listDirty = true;
-        if ((st = solidPhi(ctx, b, so.boxBound))) return st;
+        if ((st = solidPhi(ctx, b, so.boxBound, true))) return st;
       }
     }
   }

@@ -916,7 +916,8 @@ ESF_BOUNDS(kPhiInit, 1024);
 template <class GridT>
 __device__ __forceinline__ void kPhiRelax(const GridT& wg, Dims d, const Box* box, BoxRange r,
                                           const double* __restrict__ mass,
-                                          const double* __restrict__ in, double* out, double large) {
+                                          const double* __restrict__ in, double* out, double large,
+                                          int margin) {
   const int nx = r.hi[0] - r.lo[0], ny = r.hi[1] - r.lo[1];
   const int t = wg.block() * blockDim.x + threadIdx.x;
   if (t >= (int)r.count()) return;
@@ -928,6 +929,12 @@ __device__ __forceinline__ void kPhiRelax(const GridT& wg, Dims d, const Box* bo
            k >= bx.bMin[2] && k < bx.bMax[2];
   };
   if (!inBox(x, y, z)) return;
+  // margin >= 0: only the nodes within `margin` of the tight box (a node's
+  // value after the remaining relaxations reads no node further out)
+  if (margin >= 0 && (x < bx.tMin[0] - margin || x >= bx.tMax[0] + margin ||
+                      y < bx.tMin[1] - margin || y >= bx.tMax[1] + margin ||
+                      z < bx.tMin[2] - margin || z >= bx.tMax[2] + margin))
+    return;
   const long n = x + (long)y * d.X + (long)z * d.slab;
   if (mass[n] != 0) return;
   auto at = [&](int i, int j, int k) {
@@ -942,7 +949,11 @@ ESF_BOUNDS(kPhiRelax, 1024);
 
 // `launchBox`: the solid's boxes or boxes that contain them (the kernels
 // read the solid's own from device memory)
-esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox) {
+// tightOnly (the step's pass, outside FLIP scenes): relaxation k runs only
+// within kPhiRelaxations - 1 - k nodes of the tight box, so phi is the
+// whole-box relaxation's to the bit inside the tight box, where the next
+// mass pass reads it, and phiPartial is set
+esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox, bool tightOnly) {
   const Dims& d = ctx->d;
   DevSolid& so = ctx->solids[solid];
   const double large = (double)(d.X + d.Y + d.Z + 2);
@@ -962,26 +973,38 @@ esf_status solidPhi(esf_ctx* ctx, int solid, const Box& launchBox) {
   if (ctx->p.flip) {
     // the FLIP round trip extends the velocity over the whole box by phi:
     // the reference's two rounds of ordered sweeps, bitwise
+    so.phiPartial = false;
     const Box& bx = so.box;
     const int box[6] = {bx.bMin[0], bx.bMin[1], bx.bMin[2], bx.bMax[0], bx.bMax[1], bx.bMax[2]};
     return sweepPhiRounds(ctx, box, so.mass, so.phi, 2);
   }
-  BoxRange r;
-  for (int k = 0; k < 3; k++) {
-    r.lo[k] = launchBox.bMin[k];
-    r.hi[k] = std::max(launchBox.bMax[k], launchBox.bMin[k]);
-  }
-  const long nb = r.count();
-  if (nb > 0) {
-    static_assert(kPhiRelaxations % 2 == 0, "the relaxation ends in phi");
-    for (int k = 0; k < kPhiRelaxations; k++) {
-      const double* src = (k & 1) ? so.phi2 : so.phi;
-      double* dst = (k & 1) ? so.phi : so.phi2;
-      launch<ESF_K(kPhiRelax)>(ctx, dim3(cdiv(nb, 256)), dim3(256), d, dbox, r,
-                               (const double*)so.mass, src, dst, large);
+  static_assert(kPhiRelaxations % 2 == 0, "the relaxation ends in phi");
+  for (int k = 0; k < kPhiRelaxations; k++) {
+    const int margin = tightOnly ? kPhiRelaxations - 1 - k : -1;
+    BoxRange r;
+    for (int a = 0; a < 3; a++) {
+      r.lo[a] = launchBox.bMin[a];
+      r.hi[a] = std::max(launchBox.bMax[a], launchBox.bMin[a]);
+      if (margin >= 0) {
+        r.lo[a] = std::max(r.lo[a], launchBox.tMin[a] - margin);
+        r.hi[a] = std::max(r.lo[a], std::min(r.hi[a], launchBox.tMax[a] + margin));
+      }
     }
+    const long nb = r.count();
+    if (nb <= 0) continue;
+    const double* src = (k & 1) ? so.phi2 : so.phi;
+    double* dst = (k & 1) ? so.phi : so.phi2;
+    launch<ESF_K(kPhiRelax)>(ctx, dim3(cdiv(nb, 256)), dim3(256), d, dbox, r,
+                             (const double*)so.mass, src, dst, large, margin);
   }
+  so.phiPartial = tightOnly;
   return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
+}
+
+esf_status phiWhole(esf_ctx* ctx, int solid) {
+  DevSolid& so = ctx->solids[solid];
+  if (!so.phiPartial) return ESF_OK;
+  return solidPhi(ctx, solid, so.box, false);
 }
 
 // ---------------------------------------------------------------------------

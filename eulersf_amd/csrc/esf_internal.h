@@ -125,6 +125,10 @@ struct DevSolid {
   // nodes where phi (and its relaxation buffer) may differ from `large`:
   // the extended box of the last phi pass, the whole grid after an upload
   int phiSpan[6] = {0, 0, 0, 1 << 30, 1 << 30, 1 << 30};
+  // phi holds its relaxed value only where the next mass pass reads it (the
+  // tight box; the step's own pass); phiWhole relaxes the whole box before
+  // phi is handed out or read elsewhere, or the mass or the boxes change
+  bool phiPartial = false;
   double* cellMat = nullptr; // per slot: R(9) L(9) F(9) Jinv(1)
   double* cellOut = nullptr; // per slot: 24 values (force, K x, diag K)
   double* cellDiag = nullptr;// per slot: 24 diagonal stiffness entries
@@ -183,6 +187,7 @@ struct esf_ctx {
   int dofScan[6] = {0, 0, 0, 0, 0, 0};  // the nodes the queued DOF numbering walks
   bool dofPending = false;  // DOF numbering queued, its count not read yet
   bool tightLists = false;  // lists grow to exactly their counts (ESF_TIGHT_LISTS)
+  bool phiAsLeft = false;   // downloads of phi as a step left it (ESF_PHI_AS_LEFT)
   bool diagPending = false; // the pressure diagonal of the current masses not formed yet
   esf::Box* dBoxes = nullptr;  // device copies of the solids' boxes (kMaxSolids)
   uint64_t* nodeMask = nullptr; // per solid node, byte b: solid b's cells with K rows there
@@ -487,8 +492,10 @@ esf_status sweepPhiRounds(esf_ctx* ctx, const int box[6], const double* mass, do
 esf_status extendDevice(esf_ctx* ctx, const int box[6], const double* mass, const double* phi,
                         double* field);
 
-// esf_grid.hip: _X = x - u at every node into the solid's X buffer
+// esf_grid.hip: _X = x - u at every node into the solid's X buffer; phi
+// relaxed over the whole extended box where a step left it partial
 esf_status materialiseX(esf_ctx* ctx, DevSolid& so);
+esf_status phiWhole(esf_ctx* ctx, int solid);
 
 // esf_flip.hip: the FLIP round trip of one solid
 esf_status initParticles(esf_ctx* ctx, int solid);

@@ -189,6 +189,38 @@ def test_solid_mass_and_sdf_stage_from_the_same_state(esf):
     o.close()
 
 
+def test_step_relaxes_phi_near_the_tight_box_only(esf, monkeypatch):
+    """A step relaxes phi only within the relaxations still to come of the
+    tight box, where the next mass pass reads it.  As the step left it, phi
+    equals the whole-box relaxation bit for bit inside the tight box and
+    differs further out; downloaded as usual it is the whole-box relaxation
+    across the extended box."""
+    A = esf._abi
+    n = 80
+    N = n + 1
+    text = esf.unit_cube_ball_cfg(n, velocity=(0.3, -1.0, 0.2))
+    monkeypatch.setenv("ESF_PHI_AS_LEFT", "1")
+    g1 = esf.CoupledIntegrator3D.from_cfg_text(text)
+    monkeypatch.delenv("ESF_PHI_AS_LEFT")
+    g2 = esf.CoupledIntegrator3D.from_cfg_text(text)
+    for g in (g1, g2):
+        assert g.stepImplicit(1.8)["status"] == 0
+    b = g1.boxes(0)
+    assert b == g2.boxes(0)
+    t, ext = b[:6], b[6:]
+    assert all(t[k] - ext[k] > 8 for k in range(3))  # room outside the tight box
+    want = jacobi_phi(g1.download(A.S_MASS, 0), ext, N, N, N, 8)
+    region = (slice(ext[2], ext[5]), slice(ext[1], ext[4]), slice(ext[0], ext[3]))
+    left = g1.download(A.S_PHI, 0).reshape(N, N, N)[region]
+    whole = g2.download(A.S_PHI, 0).reshape(N, N, N)[region]
+    np.testing.assert_array_equal(whole.view(np.int64), want.view(np.int64))
+    tight = tuple(slice(t[k] - ext[k], t[3 + k] - ext[k]) for k in (2, 1, 0))
+    np.testing.assert_array_equal(left[tight].view(np.int64), want[tight].view(np.int64))
+    assert np.any(left != want)
+    for g in (g1, g2):
+        g.close()
+
+
 def test_corotation_golden_vectors_on_gpu(esf):
     """the reference's own COROTATION outputs (tests/golden/corotation.npz:
     near identity, general, inverted, one, two and three singular values

@@ -142,3 +142,50 @@ def test_whole_box_jacobi_settles_the_set_the_mass_pass_reads():
         assert np.all(got >= box_po - 1e-12)
     finally:
         o.close()
+
+
+def test_relaxation_near_the_tight_box_equals_the_whole_box_there():
+    """A step runs relaxation k (of R) only at the nodes within R - 1 - k of
+    the tight box, on two buffers as the device does (a node it skips keeps
+    the buffer's older value).  Inside the tight box the result is the R
+    whole-box relaxations' bit for bit, since a node's value after the
+    remaining relaxations reads no node further out; one node less of margin
+    already changes it."""
+    from tests._scenes import _solve_distance, jacobi_phi
+    rng = np.random.default_rng(4)
+    X = Y = Z = 40
+    mass = np.zeros((Z, Y, X))
+    zz, yy, xx = np.mgrid[0:Z, 0:Y, 0:X]
+    mass[(xx - 19.3) ** 2 + (yy - 21.1) ** 2 + (zz - 18.7) ** 2 < 30] = 1.0
+    mass[rng.random(mass.shape) < 0.002] = 1.0
+    ext = [2, 3, 1, 38, 39, 37]
+    tight = [12, 13, 11, 27, 29, 26]
+    R = 8
+    want = jacobi_phi(mass.reshape(-1), ext, X, Y, Z, R)
+    large = float(X + Y + Z + 2)
+    m = mass[ext[2]:ext[5], ext[1]:ext[4], ext[0]:ext[3]]
+    free = m == 0
+    lo = [tight[k] - ext[k] for k in range(3)]
+    hi = [tight[3 + k] - ext[k] for k in range(3)]
+    z, y, x = np.mgrid[0:m.shape[0], 0:m.shape[1], 0:m.shape[2]]
+
+    def run(extra):
+        buf = [np.where(m > 0, -0.5, large), np.where(m > 0, -0.5, large)]
+        for k in range(R):
+            mg = R - 1 - k - extra
+            src, dst = buf[k & 1], buf[1 - (k & 1)]
+            pad = np.pad(src, 1, constant_values=large)
+            p = np.fmin(pad[1:-1, 1:-1, :-2], pad[1:-1, 1:-1, 2:])
+            q = np.fmin(pad[1:-1, :-2, 1:-1], pad[1:-1, 2:, 1:-1])
+            r = np.fmin(pad[:-2, 1:-1, 1:-1], pad[2:, 1:-1, 1:-1])
+            near = ((x >= lo[0] - mg) & (x < hi[0] + mg) & (y >= lo[1] - mg) & (y < hi[1] + mg) &
+                    (z >= lo[2] - mg) & (z < hi[2] + mg))
+            upd = free & near
+            dst[upd] = _solve_distance(p, q, r, src)[upd]
+        return buf[R & 1]
+
+    t = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+    got = run(0)
+    np.testing.assert_array_equal(got[t].view(np.int64), want[t].view(np.int64))
+    assert np.any(got != want)
+    assert np.any(run(1)[t] != want[t])
