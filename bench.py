@@ -49,10 +49,13 @@ def log(*a):
 
 
 def run_form(esf, text, precision, warmup, steps, profiled_steps):
-    """warm-up, then `steps` timed steps (wall clock, per-step latency), then
-    `profiled_steps` more steps with HIP events around every launch of the
-    timed kernel classes (the events add gaps between launches, so they are
-    kept out of the throughput measurement)"""
+    """warm-up, then `steps` timed steps back to back (wall clock between two
+    synchronisations; per-step latency is each call's time: a step returns
+    once what the host reads of it has landed, its last launches finishing
+    behind the next step's first), then `profiled_steps` more steps with HIP
+    events around every launch of the timed kernel classes (the events add
+    gaps between launches, so they are kept out of the throughput
+    measurement)"""
     g = esf.CoupledIntegrator3D.from_cfg_text(text, precision)
     g.u0 = g.download(esf._abi.S_U, 0)
     # about 0.3 s of back-to-back pressure applies (results unused) first, so
@@ -68,7 +71,6 @@ def run_form(esf, text, precision, warmup, steps, profiled_steps):
     for _ in range(steps):
         s0 = time.perf_counter()
         r = g.stepImplicit(-1.0)
-        g.synchronize()
         lat.append(time.perf_counter() - s0)
         iters.append(r)
     g.synchronize()

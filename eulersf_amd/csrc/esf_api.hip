@@ -233,6 +233,7 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   hipEventCreateWithFlags(&ctx->evFork, hipEventDisableTiming);
   hipEventCreateWithFlags(&ctx->evJoin, hipEventDisableTiming);
   hipEventCreateWithFlags(&ctx->evCfl, hipEventDisableTiming);
+  hipEventCreateWithFlags(&ctx->evSettle, hipEventDisableTiming);
   esf_status st = ESF_OK;
   const long N = d.nodes, C = d.cells;
   const bool mixed = p->precision == ESF_MIXED;
@@ -303,6 +304,7 @@ void esf_ctx_destroy(esf_ctx* ctx) {
   hipEventDestroy(ctx->evFork);
   hipEventDestroy(ctx->evJoin);
   hipEventDestroy(ctx->evCfl);
+  hipEventDestroy(ctx->evSettle);
   for (auto& t : ctx->timed) {
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
@@ -1005,11 +1007,13 @@ struct StepCount {
 
 // a step that ended early (an error) may leave box reads and list counts
 // queued: settle them before anything reads the host copies
-static esf_status settlePending(esf_ctx* ctx) {
+// `ready`: an event after which everything pending has landed (null: the
+// whole stream)
+static esf_status settlePending(esf_ctx* ctx, hipEvent_t ready = nullptr) {
   bool any = ctx->dofPending;
   for (auto& so : ctx->solids) any = any || so.boxPending || so.listPending;
   if (!any) return ESF_OK;
-  ESF_HIP(streamSync(ctx));
+  ESF_HIP(ready ? eventSync(ctx, ready) : streamSync(ctx));
   for (int b = 0; b < (int)ctx->solids.size(); b++)
     if (ctx->solids[b].boxPending) {
       updateBoxesFinish(ctx, b, false);
@@ -1124,11 +1128,17 @@ esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats) {
   if (ctx->p.flip)
     for (int b = 0; b < (int)ctx->solids.size(); b++)
       if ((st = initParticles(ctx, b))) return st;
+  // A step alone returns once what the host reads at its end has landed:
+  // the advection and recovery behind it run on while the caller comes back
+  // for the next step (queued behind them on the same stream; every read
+  // through this interface synchronises first).  Batched scenes wait for
+  // their whole step.
+  const hipEvent_t ready = ctx->fiber ? nullptr : markEvent(ctx, ctx->evSettle);
   batchBarrier(ctx);
   if ((st = esf_advect_velocity(ctx))) return st;
   if ((st = esf_advect_intensity(ctx))) return st;
   if ((st = esf_recover_and_bc(ctx))) return st;
-  if ((st = settlePending(ctx))) return st;
+  if ((st = settlePending(ctx, ready))) return st;
   solveSt = coupledSolveReport(ctx, &s);
   if (stats) *stats = s;
   return solveSt;

@@ -243,6 +243,9 @@ struct esf_ctx {
   double* hostCfl = nullptr;      // pinned: the CFL kernel's block minima
   hipEvent_t evCfl = nullptr;     // their copy has landed
   hipEvent_t cflMark = nullptr;   // the event cflFinish waits for (evCfl alone)
+  // after a step's gather stage: what the host reads at the step's end is
+  // ready there (the advection and recovery behind it may still run)
+  hipEvent_t evSettle = nullptr;
   double* hostMesh = nullptr;     // pinned: root boxes and pair flags of the broad phase
   esf::CrScalars* hostRing = nullptr;  // pinned ring of polled scalar blocks
   int crQueued = 0;  // iterations the last pressure CR solve queued
