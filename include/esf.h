@@ -314,7 +314,11 @@ esf_status esf_recover_and_bc(esf_ctx* ctx);
  * solid velocity is the grid velocity copied into its box.  A step
  * that runs alone in the process computes the fluid's vorticity and body
  * forces on a second stream of the context, beside the first solid solve;
- * results are the same either way. */
+ * results are the same either way.  It returns once the report, the boxes
+ * and the solid counts have landed; the advection and recovery may still
+ * run on the context's stream, ahead of anything queued on the context
+ * later (reads through this interface wait for them; esf_synchronize waits
+ * for all). */
 esf_status esf_step(esf_ctx* ctx, double alpha, esf_solve_stats* stats);
 /* esf_step of n contexts at once, each kernel of the step launched once
  * over the scenes that reach it together (the reference steps its one scene
