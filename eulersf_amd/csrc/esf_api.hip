@@ -261,7 +261,8 @@ esf_status esf_ctx_create(esf_ctx** out, const esf_grid* g, const esf_params* p)
   }
   if (!st) st = dalloc(ctx, &ctx->sidx, N);
   if (!st) st = dalloc(ctx, &ctx->dBoxes, kMaxSolids);
-  const long scanBlocks = std::max<long>(N, C) / 1024 + 2;
+  // block counts of two compactions at once (nodes and cells) and totals
+  const long scanBlocks = N / 1024 + C / 1024 + 4;
   if (!st) st = dalloc(ctx, &ctx->scanBlocks, scanBlocks);
   // up to 5 partials per block of the widest reducing launch (the pressure
   // CR update over every SpMV tile, or 128-thread blocks over every node)

@@ -73,9 +73,7 @@ __device__ __forceinline__ void kScanCount(const GridT& wg, Walk w, long n, Flag
 ESF_BOUNDS(kScanCount, 1024);
 
 // exclusive scan of nb block counts by one block; total into blockCounts[nb]
-template <class GridT>
-__device__ __forceinline__ void kScanBlocks(const GridT& wg, int nb, int* blockCounts) {
-  __shared__ int wsum[16];
+__device__ inline int scanBlockCounts(int nb, int* blockCounts, int* wsum) {
   const int per = (nb + 1023) / 1024;
   const int b0 = threadIdx.x * per;
   int s = 0;
@@ -90,8 +88,15 @@ __device__ __forceinline__ void kScanBlocks(const GridT& wg, int nb, int* blockC
       acc += v;
     }
   if (threadIdx.x == 0) blockCounts[nb] = total;
+  return total;
+}
+template <class GridT>
+__device__ __forceinline__ void kScanBlocks(const GridT& wg, int nb, int* blockCounts) {
+  __shared__ int wsum[16];
+  (void)scanBlockCounts(nb, blockCounts, wsum);
 }
 ESF_BOUNDS(kScanBlocks, 1024);
+
 
 template <class Walk, class Flag, class Emit, class GridT>
 __device__ __forceinline__ void kScanEmit(const GridT& wg, Walk w, long n, Flag flag,
@@ -117,6 +122,100 @@ __device__ __forceinline__ void kScanEmit(const GridT& wg, Walk w, long n, Flag 
   }
 }
 ESF_BOUNDS(kScanEmit, 1024);
+
+// Two compactions in one set of launches (a step's DOF numbering and a
+// solid's valid-cell list): blocks [0, nbA) take walk A, the next nbB walk
+// B; A's block counts sit at [0, nbA) with its total behind them, B's at
+// [nbA + 1, nbA + 1 + nbB) with its total behind, and the scan block writes
+// both totals to pinned host memory.
+template <class WA, class FA, class WB, class FB, class GridT>
+__device__ __forceinline__ void kScanCount2(const GridT& wg, WA wa, long na, FA fa, int nbA,
+                                            WB wb, long nb, FB fb, int* blockCounts) {
+  __shared__ int wsum[16];
+  const bool isA = (int)wg.block() < nbA;  // uniform per block
+  const int lb = isA ? wg.block() : wg.block() - nbA;
+  const long t0 = (long)lb * kScanTile + threadIdx.x * 4;
+  int c = 0;
+  int x, y, z;
+  if (isA) {
+    if (t0 < na) {
+      wa.start(t0, x, y, z);
+      for (int k = 0; k < 4 && t0 + k < na; k++, wa.next(x, y, z))
+        if (fa(x, y, z, wa.index(x, y, z))) c++;
+    }
+  } else if (t0 < nb) {
+    wb.start(t0, x, y, z);
+    for (int k = 0; k < 4 && t0 + k < nb; k++, wb.next(x, y, z))
+      if (fb(x, y, z, wb.index(x, y, z))) c++;
+  }
+  int total;
+  blockExclusiveScan(c, wsum, &total);
+  if (threadIdx.x == 0) blockCounts[isA ? lb : nbA + 1 + lb] = total;
+}
+ESF_BOUNDS(kScanCount2, 1024);
+
+template <class GridT>
+__device__ __forceinline__ void kScanBlocks2(const GridT& wg, int nbA, int nbB, int* blockCounts,
+                                             int* hostA, int* hostB) {
+  __shared__ int wsum[16];
+  const int ta = scanBlockCounts(nbA, blockCounts, wsum);
+  __syncthreads();
+  const int tb = scanBlockCounts(nbB, blockCounts + nbA + 1, wsum);
+  if (threadIdx.x == 0) {
+    *hostA = ta;
+    *hostB = tb;
+  }
+}
+ESF_BOUNDS(kScanBlocks2, 1024);
+
+// one block's emit of one walk (kScanEmit's body)
+template <class Walk, class Flag, class Emit>
+__device__ inline void emitTile(int lb, Walk w, long n, Flag flag, int base, Emit emit,
+                                int* wsum) {
+  const long t0 = (long)lb * kScanTile + threadIdx.x * 4;
+  int f[4] = {0, 0, 0, 0}, c = 0;
+  int x = 0, y = 0, z = 0;
+  if (t0 < n) {
+    w.start(t0, x, y, z);
+    int xs = x, ys = y, zs = z;
+    for (int k = 0; k < 4 && t0 + k < n; k++, w.next(xs, ys, zs)) {
+      f[k] = flag(xs, ys, zs, w.index(xs, ys, zs)) ? 1 : 0;
+      c += f[k];
+    }
+  }
+  int total;
+  int idx = base + blockExclusiveScan(c, wsum, &total);
+  if (t0 >= n) return;
+  for (int k = 0; k < 4 && t0 + k < n; k++, w.next(x, y, z)) {
+    emit(x, y, z, w.index(x, y, z), f[k] ? idx : -1);
+    idx += f[k];
+  }
+}
+
+template <class WA, class FA, class EA, class WB, class FB, class EB, class GridT>
+__device__ __forceinline__ void kScanEmit2(const GridT& wg, WA wa, long na, FA fa, EA ea, int nbA,
+                                           WB wb, long nb, FB fb, EB eb,
+                                           const int* blockCounts) {
+  __shared__ int wsum[16];
+  const int blk = wg.block();
+  if (blk < nbA)
+    emitTile(blk, wa, na, fa, blockCounts[blk], ea, wsum);
+  else
+    emitTile(blk - nbA, wb, nb, fb, blockCounts[nbA + 1 + blk - nbA], eb, wsum);
+}
+ESF_BOUNDS(kScanEmit2, 1024);
+
+template <class WA, class FA, class EA, class WB, class FB, class EB>
+void compactEnqueue2(esf_ctx* ctx, WA wa, long na, FA fa, EA ea, int* hostA, WB wb, long nb,
+                     FB fb, EB eb, int* hostB) {
+  const int nbA = std::max(cdiv(na, kScanTile), 1), nbB = std::max(cdiv(nb, kScanTile), 1);
+  launch<ESF_KT(kScanCount2, WA, FA, WB, FB)>(ctx, dim3(nbA + nbB), dim3(256), wa, na, fa, nbA,
+                                              wb, nb, fb, ctx->scanBlocks);
+  launch<ESF_K(kScanBlocks2)>(ctx, dim3(1), dim3(1024), nbA, nbB, ctx->scanBlocks, hostA, hostB);
+  launch<ESF_KT(kScanEmit2, WA, FA, EA, WB, FB, EB)>(ctx, dim3(nbA + nbB), dim3(256), wa, na,
+                                                     fa, ea, nbA, wb, nb, fb, eb,
+                                                     (const int*)ctx->scanBlocks);
+}
 
 // count, scan and emit over the n elements of a walk, queued; the count is
 // copied to the pinned int *hostTotal, read after a later synchronisation
@@ -324,11 +423,28 @@ static void queueValidScan(esf_ctx* ctx, int b) {
   so.listPending = true;
 }
 
-// the valid-cell list of solid b, queued.  The scan walks the cells of the
-// tight box (ValidFlag's range; a box that contains it while the solid's
-// boxes are still on their way to the host) and of the last build's, where
-// cellSlot may be >= 0; every other cell's slot is -1 already and stays so.
-static esf_status queueValidList(esf_ctx* ctx, int b) {
+// the DOF numbering and solid 0's valid-cell list, queued in one set of
+// launches (their scan ranges set up already)
+static void queueDofsAndValid(esf_ctx* ctx) {
+  DevSolid& so = ctx->solids[0];
+  const Dims& d = ctx->d;
+  const BoxRange ra = rangeOf(ctx->dofScan), rb = rangeOf(so.slotScan);
+  compactEnqueue2(ctx, BoxWalk{ra, d.X, d.slab}, ra.count(), SolidFlag{d, ctx->solidMass},
+                  SolidEmit{ctx->sidx, ctx->solidNodes, ctx->capSolid},
+                  ctx->hostFlags + kHostDofCount, BoxWalk{rb, d.CX, d.cslab}, rb.count(),
+                  ValidFlag{ctx->dBoxes, so.valid},
+                  ValidEmit{d, so.cellSlot, so.cellList, so.capValid},
+                  ctx->hostFlags + kHostValidCount);
+  ctx->dofPending = true;
+  so.listPending = true;
+}
+
+// the valid-cell list of solid b: its scan range, then (queue) the scan.
+// The scan walks the cells of the tight box (ValidFlag's range; a box that
+// contains it while the solid's boxes are still on their way to the host)
+// and of the last build's, where cellSlot may be >= 0; every other cell's
+// slot is -1 already and stays so.
+static esf_status queueValidList(esf_ctx* ctx, int b, bool queue = true) {
   DevSolid& so = ctx->solids[b];
   const Dims& d = ctx->d;
   esf_status st = growSolidCells(ctx, so, 0);
@@ -343,7 +459,7 @@ static esf_status queueValidList(esf_ctx* ctx, int b) {
     so.slotScan[3 + k] = r.hi[k];
   }
   for (int k = 0; k < 6; k++) so.slotSpan[k] = cur[k];
-  queueValidScan(ctx, b);
+  if (queue) queueValidScan(ctx, b);
   return ESF_OK;
 }
 
@@ -390,6 +506,7 @@ esf_status gatherSolidMasses(esf_ctx* ctx, bool defer) {
     return ESF_ERR_INVALID;
   }
   esf_status st;
+  bool dofs = false;  // DOF numbering to queue
   // the gathered fields depend on the solids' masses and quarter fractions
   // only: when neither moved since the last gather (the usual step: they
   // are computed once per scene), the fields and the DOF numbering stand
@@ -437,11 +554,21 @@ esf_status gatherSolidMasses(esf_ctx* ctx, bool defer) {
       ctx->dofScan[k] = gb.lo[k];
       ctx->dofScan[3 + k] = gb.hi[k];
     }
-    queueSolidDofs(ctx);
+    dofs = true;
     ctx->gatheredGen = ctx->massGen;
     ctx->gatheredSolids = (int)ctx->solids.size();
   }
-  for (int b = 0; b < (int)ctx->solids.size(); b++)
+  // the DOF numbering and the first solid's valid cells (the usual step
+  // needs both) share one set of launches
+  int b0 = 0;
+  if (dofs && !ctx->solids.empty() && ctx->solids[0].listDirty) {
+    if ((st = queueValidList(ctx, 0, false))) return st;
+    queueDofsAndValid(ctx);
+    b0 = 1;
+  } else if (dofs) {
+    queueSolidDofs(ctx);
+  }
+  for (int b = b0; b < (int)ctx->solids.size(); b++)
     if (ctx->solids[b].listDirty && (st = queueValidList(ctx, b)))
       return st;
   if (defer) return hipGetLastError() == hipSuccess ? ESF_OK : ESF_ERR_DEVICE;
